@@ -1,0 +1,142 @@
+/*
+ * leswht.h - C ABI of the MI355X (gfx950) LESwHT time step.
+ *
+ * The reference solver has no library boundary: solver.f90 calls external
+ * Fortran subroutines with implicit interfaces on module-global state
+ * (solver.f90:26-196; state in mod_common.f90, mod_flowarray.f90,
+ * mod_poiss.f90). This header replaces that in-process operator API with a
+ * handle that owns the state in HBM:
+ *
+ *   reference (solver.f90)                         here
+ *   readsettings/readbcs/readgeom/alloinit/         lw_create (arrays) or
+ *     allo_array/ibmpreread/nutzeroread             lw_create_from_dir (files)
+ *     (misc_init.f90:19-440)
+ *   prefld (misc_init.f90:610-786)                  lw_set_fields / lw_create_from_dir
+ *   dttimeinit/rk3coefinit/lhsinit/poisinit/        inside lw_create
+ *     sgsfilterinit (solver.f90:44-60)
+ *   one pass of the time loop solver.f90:85-181     lw_step
+ *     (stepinit, sgscalc(_t), rhsnlhs, divgs,
+ *      poisson, ucalc, pcalc, lagforce, rhsnlhs_t,
+ *      convergence_check, masscheck)
+ *   writefield (misc_hist.f90:212-252)              lw_get_fields / lw_write_field
+ *
+ * Conventions: every function returns 0 on success and a negative code on
+ * failure (the message is available from lw_last_error); nothing throws across
+ * the ABI. Arrays are passed as plain host pointers in the reference's
+ * Fortran layouts: padded fields (0:n1, 0:n2, 0:n3) with i fastest, 1-D
+ * metrics at their Fortran index (x[1..n1], c2cx[0..n1], ipv[1..n1m]),
+ * interior masks (n1m, n2m, n3m) with i fastest. Integers are int64 like the
+ * reference's -fdefault-integer-8 build. A handle is used from one host
+ * thread at a time; separate handles may run concurrently on their own
+ * streams.
+ */
+#ifndef LESWHT_H
+#define LESWHT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LW_OK 0
+#define LW_ERR_ARG -1      /* invalid argument or unsupported configuration */
+#define LW_ERR_DEVICE -2   /* no usable gfx950 device or HIP runtime error */
+#define LW_ERR_IO -3       /* file contract violated */
+#define LW_ERR_NUMERIC -4  /* non-finite values detected */
+
+/* settings.input + boundary.input + ibmpre_prdic.bin
+ * (misc_init.f90:19-158; mod_common.f90:2-22) */
+typedef struct lw_settings {
+  double re, pr, gr, tflu_i, tsol_i, eps_ptr, udrv_i, dt_size, cflfac, resid1,
+      wwsor, csgsts, csgshf, tend, ptb_tst, avg_tst;
+  int64_t grdir, ireset, iread, iavg, ipzero, nend, nprint, npriavg, npin,
+      idtopt, nlev, nbli, ioldv, mgitr, imgsor, iles, insmdl, itemdl, idvmon,
+      filter, ibmon, masson, imovingon, ihtrans, idecomp;
+  int64_t bc_ybtm, bc_ytop, bc_zbtm, bc_ztop, bc_t_ybtm, bc_t_ytop, bc_t_zbtm,
+      bc_t_ztop, ich, iconjg, xprdic, yprdic, zprdic, iintp;
+  double val_t_ybtm, val_t_ytop, val_t_zbtm, val_t_ztop;
+} lw_settings;
+
+/* grid.bin as readgeom holds it (misc_init.f90:204-251) */
+typedef struct lw_grid {
+  int64_t n1, n2, n3, n1m, n2m, n3m;
+  double xl, yl, zl;
+  const double *x, *y, *z;
+  const int64_t *ipv, *jpv, *kpv, *imv, *jmv, *kmv;
+  const int64_t *fixil, *fixjl, *fixkl, *fixiu, *fixju, *fixku;
+  const double *c2cx, *c2cy, *c2cz, *c2cxi, *c2cyi, *c2czi;
+  const double *f2fx, *f2fy, *f2fz, *f2fxi, *f2fyi, *f2fzi;
+  const double *xmp, *ymp, *zmp;
+} lw_grid;
+
+/* ibmpre tables as ibmpreread/nutzeroread hold them (misc_init.f90:312-409);
+ * component c = 0..3 is u, v, w, T. intpindx is 3 per point, geomfac 27 per
+ * point ordered [x-offset][y-offset][z-offset]. inz/jnz/knz are the values
+ * on disk (0-based, used by the solver as 1-based indices). */
+typedef struct lw_tables {
+  int64_t nintp[4], nbody[4];
+  const int64_t *ifc[4], *jfc[4], *kfc[4], *intpindx[4];
+  const double *geomfac[4];
+  int64_t nzero;
+  const int64_t *inz, *jnz, *knz;
+  const int64_t *nwall_dvm;
+} lw_tables;
+
+/* per-step results (stdout / fhist.dat quantities of the reference) */
+typedef struct lw_step_stats {
+  double dt, time, cflmax, dvmax, qmmax, csgsts, csgshf;
+  int64_t ntime;
+  int64_t mg_cycles_max;   /* largest V-cycle count of any wavenumber, step */
+  int64_t mg_cycles_sum;   /* sum over wavenumbers and substeps */
+  double ms_sgs, ms_momentum, ms_poisson, ms_energy; /* device time, ms */
+} lw_step_stats;
+
+typedef struct lw_handle lw_handle;
+
+/* body level set used inside the step (convbc, nutzero, wallbc_t):
+ * kind 0 = circular cylinder sqrt(x^2+y^2)-p0 (funcbody_cylinder.f90:13) */
+typedef struct lw_body {
+  int64_t kind;
+  double p[4];
+} lw_body;
+
+int lw_create(const lw_settings *s, const lw_grid *g, const lw_tables *t,
+              const lw_body *body, lw_handle **out);
+/* readsettings .. prefld from a reference case directory; solver_dir is the
+ * directory holding settings.input (the reference runs with it as cwd) */
+int lw_create_from_dir(const char *solver_dir, const lw_body *body, lw_handle **out);
+int lw_destroy(lw_handle *h);
+
+int lw_set_fields(lw_handle *h, const double *u, const double *v, const double *w,
+                  const double *p, const double *t, int64_t ihist, int64_t m,
+                  double time, double dt);
+int lw_get_fields(lw_handle *h, double *u, double *v, double *w, double *p, double *t);
+/* nsteps passes of solver.f90:85-181; stats of the last step */
+int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats);
+/* writefield: fld file with the reference's record layout */
+int lw_write_field(lw_handle *h, const char *path);
+/* the fld writer without a handle: n = (n1, n2, n3), padded host arrays */
+int lw_write_fld(const char *path, const int64_t n[3], double re, double pr, double gr,
+                 int64_t ihist, int64_t m, double time, double dt, const int64_t prdic[3],
+                 const int64_t bcs[6], const int64_t ich[2], const double *u, const double *v,
+                 const double *w, const double *p, const double *t);
+
+/* sizes: n[0..2] = n1, n2, n3 */
+int lw_get_dims(const lw_handle *h, int64_t n[3]);
+/* named internal array for phase checks ("nusgs", "nusgs1_1".."nusgs1_3",
+ * "alsgs", "alsgs1_1".."alsgs1_3", "divgsum", "phi", "qmass");
+ * n = capacity in doubles. Returns the element count or a negative code. */
+int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n);
+/* phase stepping for kernel-level parity checks, in solver.f90 order:
+ * 0 stepinit, 10+m substepinit(m), 1 sgs, 2 rhsnlhs, 3 divgs, 4 poisson,
+ * 5 ucalc+pcalc+lagforce, 6 energy, 7 end-of-step diagnostics */
+int lw_phase(lw_handle *h, int phase);
+/* device scratch footprint in bytes */
+int64_t lw_device_bytes(const lw_handle *h);
+
+const char *lw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""Builds the gfx950 library ``leswht_amd/_lib/libleswht.so`` with hipcc.
+
+FP64 parity with the reference needs IEEE evaluation in the written order,
+so every translation unit is compiled with -ffp-contract=off (no FMA
+contraction) and without fast-math.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(OUT_DIR, "libleswht.so")
+ROOT = os.path.dirname(HERE)
+
+HIP_SOURCES = ["lw_solver.hip", "lw_kern_flow.hip", "lw_kern_sgs.hip", "lw_kern_pois.hip"]
+CXX_SOURCES = ["lw_io.cpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("LESWHT_ARCH", "gfx950")
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+          "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+        raise RuntimeError("compile failed: " + " ".join(cmd[-3:]))
+    return res
+
+
+def build(verbose: bool = False, jobs: int = 8) -> str:
+    os.makedirs(os.path.join(OUT_DIR, "obj"), exist_ok=True)
+    objs = []
+    procs = []
+    for src in HIP_SOURCES + CXX_SOURCES:
+        obj = os.path.join(OUT_DIR, "obj", src + ".o")
+        objs.append(obj)
+        cmd = [HIPCC] + COMMON
+        if src.endswith(".hip"):
+            cmd += DEVICE + ["-x", "hip"]
+        cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+    failed = []
+    for cmd, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            failed.append((cmd, out + err))
+        elif verbose and err.strip():
+            sys.stderr.write(err)
+    if failed:
+        for cmd, msg in failed:
+            sys.stderr.write(" ".join(cmd) + "\n" + msg + "\n")
+        raise RuntimeError("hipcc failed on %d file(s)" % len(failed))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

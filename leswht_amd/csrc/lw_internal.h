@@ -1,0 +1,80 @@
+// Internal definitions shared by the gfx950 kernels and the host driver of
+// the LESwHT time step. Fields live in HBM in the reference's padded layout
+// (0:n1, 0:n2, 0:n3) with i fastest, so every stencil reads coalesced along
+// i and the C ABI copies arrays without reshaping.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/leswht.h"
+
+namespace lw {
+
+// 1-D metrics and index maps on the device (Fortran index = array index)
+struct Geo {
+  int n1, n2, n3, n1m, n2m, n3m;
+  long NX, NY, NZ;  // n1+1, n2+1, n3+1
+  double xl, yl, zl;
+  const double *x, *y, *z;
+  const double *c2cx, *c2cy, *c2cz, *c2cxi, *c2cyi, *c2czi;
+  const double *f2fx, *f2fy, *f2fz, *f2fxi, *f2fyi, *f2fzi;
+  const double *xmp, *ymp, *zmp;
+  const int *ipv, *jpv, *kpv, *imv, *jmv, *kmv;
+  const int *fixil, *fixjl, *fixkl, *fixiu, *fixju, *fixku;
+  // ADI coefficients (slv_mmtm.f90:180-249)
+  const double *aiu, *ciu, *aivw, *civw, *ajv, *cjv, *ajuw, *cjuw, *akw, *ckw, *akuv, *ckuv;
+  // SGS test-filter weights cf?1(n,-1:1) stored [3*i + d + 1]
+  const double *cfx1, *cfy1, *cfz1;
+  int ib, jb, kb;  // i_bgpx, j_bgpy, k_bgpz
+};
+
+__host__ __device__ inline long P(const Geo &g, int i, int j, int k) {
+  return (long)i + g.NX * ((long)j + g.NY * (long)k);
+}
+
+// per-substep scalars (substepinit, slv_auxi.f90:105-123)
+struct Sub {
+  double dt, gam, ro, alpha, dtconst, dtconsti, acoef, acoefi, re, pr;
+  int msub;
+};
+
+// bits of the per-cell mask array
+enum : uint8_t {
+  M_BODY_U = 1,   // u point is an IB forcing point
+  M_BODY_V = 2,
+  M_BODY_W = 4,
+  M_BODY_T = 8,
+  M_SOLID_C = 16,  // cell centre has funcbody < 1e-10 (nutzero/alpzero)
+  M_NUTZERO = 32,  // listed in ibmpre_nutzero.bin (shifted by the D7 rule)
+};
+
+struct IBComp {
+  int nintp = 0, nbody = 0;
+  int *ifc = nullptr, *jfc = nullptr, *kfc = nullptr;  // nbody
+  int *indx = nullptr;                                   // 3*nintp
+  double *geomfac = nullptr;                             // 27*nintp
+  double *targ = nullptr;                                // nintp
+  double *fcv = nullptr, *fcvavg = nullptr, *dudtr = nullptr;
+};
+
+struct MG {
+  int nlev, n1md, n3mh, nbli, mgitr;
+  long LD;       // leading dimension of MG planes: n1md + 1
+  long PLANE;    // (n1md + 1) * (n2 + 1)
+  long PCLD;     // n1 + 1
+  long PCPLANE;  // (n1 + 1) * (n2 + 1)
+  int iimg[21][4];
+  // coefficients
+  double *aw, *ae, *as, *an, *coi1, *coi2, *cor1, *cor2;
+  int *ipm, *imm;
+  double *ac, *gam, *bet;  // [kv][j][i] with LD, PLANE (real)
+  double2 *gg, *resd;      // [kv][j][i]
+  double2 *pc, *rhs;       // [kv][j][i] with PCLD, PCPLANE
+  unsigned long long *sumres;  // n3mh, max-norm bit patterns
+  double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
+};
+
+}  // namespace lw

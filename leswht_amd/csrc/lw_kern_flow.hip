@@ -1,0 +1,1050 @@
+// gfx950 kernels of the momentum, continuity, projection and energy phases
+// of the LESwHT time step. Arithmetic follows the reference expression by
+// expression (built with -ffp-contract=off) so the HIP results agree with the
+// FP64 reference to rounding; the reference file:line is on each kernel.
+#include "lw_kernels.h"
+
+namespace lw {
+
+#define BLK3 dim3(64, 4, 1)
+static inline dim3 grid3(int ni, int nj, int nk) {
+  return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)(nk > 0 ? nk : 1));
+}
+
+// ---------------------------------------------------------------------------
+// reductions: wave64 shuffles, then one LDS pass; fixed shapes -> deterministic
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <int NT>
+__device__ inline double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (w == 0) {
+    r = lane < NT / 64 ? sh[lane] : 0.0;
+    r = wave_sum(r);
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+__device__ inline void atomic_max_nonneg(unsigned long long *a, double v) {
+  atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// momentum right-hand side, slv_mmtm.f90:315-567, with the LES cross terms of
+// rhssgs (sgs.f90:1136-1200) computed in place of the rhs1 round trip
+// ---------------------------------------------------------------------------
+__global__ void k_rhs_u(Geo g, Sub sb, Fld f, int les) {
+  const int i = g.ib + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
+  const long c = P(g, i, j, k);
+  const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
+  const double fju = g.fixju[j], fjl = g.fixjl[j], fku = g.fixku[k], fkl = g.fixkl[k];
+  const double un = 0.5 * (g.f2fy[j + 1] * u[c] + g.f2fy[j] * u[c + dy]) * g.c2cyi[j + 1] * (1. - fju) +
+                    u[P(g, i, g.n2, k)] * fju;
+  const double us = 0.5 * (g.f2fy[j] * u[c - dy] + g.f2fy[j - 1] * u[c]) * g.c2cyi[j] * (1. - fjl) +
+                    u[P(g, i, 0, k)] * fjl;
+  const double uc = 0.5 * (g.f2fz[k + 1] * u[c] + g.f2fz[k] * u[c + dz]) * g.c2czi[k + 1] * (1. - fku) +
+                    u[P(g, i, j, g.n3)] * fku;
+  const double uf = 0.5 * (g.f2fz[k] * u[c - dz] + g.f2fz[k - 1] * u[c]) * g.c2czi[k] * (1. - fkl) +
+                    u[P(g, i, j, 0)] * fkl;
+  const double vnx = 0.5 * (g.f2fx[i - 1] * v[c + dy] + g.f2fx[i] * v[c - dx + dy]) * g.c2cxi[i];
+  const double vsx = 0.5 * (g.f2fx[i - 1] * v[c] + g.f2fx[i] * v[c - dx]) * g.c2cxi[i];
+  const double wcx = 0.5 * (g.f2fx[i - 1] * w[c + dz] + g.f2fx[i] * w[c - dx + dz]) * g.c2cxi[i];
+  const double wfx = 0.5 * (g.f2fx[i - 1] * w[c] + g.f2fx[i] * w[c - dx]) * g.c2cxi[i];
+  const double anxe = 0.5 * (u[c + dx] + u[c]);
+  const double anxw = 0.5 * (u[c] + u[c - dx]);
+  const double anxx = (anxe * anxe - anxw * anxw) * g.c2cxi[i];
+  const double anx1 = (un * vnx - us * vsx) * g.f2fyi[j];
+  const double anx2 = (uc * wcx - uf * wfx) * g.f2fzi[k];
+  double rk3x = -anxx - anx1 - anx2;
+  const double alx1 = (u[c + dx] - u[c]) * g.f2fxi[i];
+  const double alx2 = (u[c] - u[c - dx]) * g.f2fxi[i - 1];
+  const double alx3 = (u[c + dy] - u[c]) * g.c2cyi[j + 1];
+  const double alx4 = (u[c] - u[c - dy]) * g.c2cyi[j];
+  const double alx5 = (u[c + dz] - u[c]) * g.c2czi[k + 1];
+  const double alx6 = (u[c] - u[c - dz]) * g.c2czi[k];
+  const double alxx = (alx1 - alx2) * g.c2cxi[i];
+  const double alxy = (alx3 - alx4) * g.f2fyi[j];
+  const double alxz = (alx5 - alx6) * g.f2fzi[k];
+  double alx = 1. / sb.re * (alxx + alxy + alxz);
+  if (les) {
+    const double *nu = f.nu, *n2 = f.nu1[1], *n3 = f.nu1[2];
+    const double talxx = (nu[c] * alx1 - nu[c - dx] * alx2) * g.c2cxi[i];
+    const double talxy = (n3[c + dy] * alx3 - n3[c] * alx4) * g.f2fyi[j];
+    const double talxz = (n2[c + dz] * alx5 - n2[c] * alx6) * g.f2fzi[k];
+    alx = alx + 1.0 * (2. * talxx + talxy + talxz);
+    const double rhsu2 = g.f2fyi[j] * g.c2cxi[i] *
+                         (n3[c + dy] * (v[c + dy] - v[c - dx + dy]) - n3[c] * (v[c] - v[c - dx]));
+    const double rhsu3 = g.f2fzi[k] * g.c2cxi[i] *
+                         (n2[c + dz] * (w[c + dz] - w[c - dx + dz]) - n2[c] * (w[c] - w[c - dx]));
+    rk3x = rk3x + 1.0 * (rhsu2 + rhsu3);
+  }
+  const double ro = f.rk3o[0][c];
+  f.rhs[0][c] = sb.dt * (sb.gam * rk3x + sb.ro * ro + 2. * sb.alpha * alx -
+                         2. * sb.alpha * (p[c] - p[c - dx]) * g.c2cxi[i] - 2. * sb.alpha * 0.0 * 0.0);
+  f.rk3oo[0][c] = ro;
+  f.rk3o[0][c] = rk3x;
+}
+
+__global__ void k_rhs_v(Geo g, Sub sb, Fld f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = g.jb + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
+  const long c = P(g, i, j, k);
+  const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
+  const double fiu = g.fixiu[i], fil = g.fixil[i], fku = g.fixku[k], fkl = g.fixkl[k];
+  const double ve = (0.5 * (g.f2fx[i + 1] * v[c] + g.f2fx[i] * v[c + dx]) * g.c2cxi[i + 1]) * (1. - fiu) +
+                    v[P(g, g.n1, j, k)] * fiu;
+  const double vw = (0.5 * (g.f2fx[i] * v[c - dx] + g.f2fx[i - 1] * v[c]) * g.c2cxi[i]) * (1. - fil) +
+                    v[P(g, 0, j, k)] * fil;
+  const double vc = (0.5 * (g.f2fz[k + 1] * v[c] + g.f2fz[k] * v[c + dz]) * g.c2czi[k + 1]) * (1. - fku) +
+                    v[P(g, i, j, g.n3)] * fku;
+  const double vf = (0.5 * (g.f2fz[k] * v[c - dz] + g.f2fz[k - 1] * v[c]) * g.c2czi[k]) * (1. - fkl) +
+                    v[P(g, i, j, 0)] * fkl;
+  const double uey = 0.5 * (g.f2fy[j - 1] * u[c + dx] + g.f2fy[j] * u[c + dx - dy]) * g.c2cyi[j];
+  const double uwy = 0.5 * (g.f2fy[j - 1] * u[c] + g.f2fy[j] * u[c - dy]) * g.c2cyi[j];
+  const double wcy = 0.5 * (g.f2fy[j - 1] * w[c + dz] + g.f2fy[j] * w[c - dy + dz]) * g.c2cyi[j];
+  const double wfy = 0.5 * (g.f2fy[j - 1] * w[c] + g.f2fy[j] * w[c - dy]) * g.c2cyi[j];
+  const double anyn = 0.5 * (v[c + dy] + v[c]);
+  const double anys = 0.5 * (v[c] + v[c - dy]);
+  const double anyy = g.c2cyi[j] * (anyn * anyn - anys * anys);
+  const double any1 = (uey * ve - uwy * vw) * g.f2fxi[i];
+  const double any2 = (vc * wcy - vf * wfy) * g.f2fzi[k];
+  double rk3y = -anyy - any1 - any2;
+  const double aly1 = (v[c + dx] - v[c]) * g.c2cxi[i + 1];
+  const double aly2 = (v[c] - v[c - dx]) * g.c2cxi[i];
+  const double aly3 = (v[c + dy] - v[c]) * g.f2fyi[j];
+  const double aly4 = (v[c] - v[c - dy]) * g.f2fyi[j - 1];
+  const double aly5 = (v[c + dz] - v[c]) * g.c2czi[k + 1];
+  const double aly6 = (v[c] - v[c - dz]) * g.c2czi[k];
+  const double alyx = (aly1 - aly2) * g.f2fxi[i];
+  const double alyy = (aly3 - aly4) * g.c2cyi[j];
+  const double alyz = (aly5 - aly6) * g.f2fzi[k];
+  double aly = 1. / sb.re * (alyx + alyy + alyz);
+  if (les) {
+    const double *nu = f.nu, *n1 = f.nu1[0], *n3 = f.nu1[2];
+    const double talyx = (n3[c + dx] * aly1 - n3[c] * aly2) * g.f2fxi[i];
+    const double talyy = (nu[c] * aly3 - nu[c - dy] * aly4) * g.c2cyi[j];
+    const double talyz = (n1[c + dz] * aly5 - n1[c] * aly6) * g.f2fzi[k];
+    aly = aly + 1.0 * (talyx + 2. * talyy + talyz);
+    const double rhsv1 = g.f2fxi[i] * g.c2cyi[j] *
+                         (n3[c + dx] * (u[c + dx] - u[c + dx - dy]) - n3[c] * (u[c] - u[c - dy]));
+    const double rhsv3 = g.f2fzi[k] * g.c2cyi[j] *
+                         (n1[c + dz] * (w[c + dz] - w[c - dy + dz]) - n1[c] * (w[c] - w[c - dy]));
+    rk3y = rk3y + 1.0 * (rhsv1 + rhsv3);
+  }
+  const double ro = f.rk3o[1][c];
+  f.rhs[1][c] = sb.dt * (sb.gam * rk3y + sb.ro * ro + 2. * sb.alpha * aly -
+                         2. * sb.alpha * (p[c] - p[c - dy]) * g.c2cyi[j]);
+  f.rk3oo[1][c] = ro;
+  f.rk3o[1][c] = rk3y;
+}
+
+__global__ void k_rhs_w(Geo g, Sub sb, Fld f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = g.kb + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
+  const long c = P(g, i, j, k);
+  const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
+  const double fiu = g.fixiu[i], fil = g.fixil[i], fju = g.fixju[j], fjl = g.fixjl[j];
+  const double we = 0.5 * (g.f2fx[i + 1] * w[c] + g.f2fx[i] * w[c + dx]) * g.c2cxi[i + 1] * (1. - fiu) +
+                    fiu * w[P(g, g.n1, j, k)];
+  const double ww = 0.5 * (g.f2fx[i] * w[c - dx] + g.f2fx[i - 1] * w[c]) * g.c2cxi[i] * (1. - fil) +
+                    fil * w[P(g, 0, j, k)];
+  const double wn = (0.5 * (g.f2fy[j + 1] * w[c] + g.f2fy[j] * w[c + dy]) * g.c2cyi[j + 1]) * (1. - fju) +
+                    fju * w[P(g, i, g.n2, k)];
+  const double ws = (0.5 * (g.f2fy[j] * w[c - dy] + g.f2fy[j - 1] * w[c]) * g.c2cyi[j]) * (1. - fjl) +
+                    fjl * w[P(g, i, 0, k)];
+  const double uez = 0.5 * (g.f2fz[k - 1] * u[c + dx] + g.f2fz[k] * u[c + dx - dz]) * g.c2czi[k];
+  const double uwz = 0.5 * (g.f2fz[k - 1] * u[c] + g.f2fz[k] * u[c - dz]) * g.c2czi[k];
+  const double vnz = 0.5 * (g.f2fz[k - 1] * v[c + dy] + g.f2fz[k] * v[c + dy - dz]) * g.c2czi[k];
+  const double vsz = 0.5 * (g.f2fz[k - 1] * v[c] + g.f2fz[k] * v[c - dz]) * g.c2czi[k];
+  const double anzc = 0.5 * (w[c + dz] + w[c]);
+  const double anzf = 0.5 * (w[c] + w[c - dz]);
+  const double anzz = (anzc * anzc - anzf * anzf) * g.c2czi[k];
+  const double anz1 = (uez * we - uwz * ww) * g.f2fxi[i];
+  const double anz2 = (vnz * wn - vsz * ws) * g.f2fyi[j];
+  double rk3z = -anzz - anz1 - anz2;
+  const double alz1 = (w[c + dx] - w[c]) * g.c2cxi[i + 1];
+  const double alz2 = (w[c] - w[c - dx]) * g.c2cxi[i];
+  const double alz3 = (w[c + dy] - w[c]) * g.c2cyi[j + 1];
+  const double alz4 = (w[c] - w[c - dy]) * g.c2cyi[j];
+  const double alz5 = (w[c + dz] - w[c]) * g.f2fzi[k];
+  const double alz6 = (w[c] - w[c - dz]) * g.f2fzi[k - 1];
+  const double alzx = (alz1 - alz2) * g.f2fxi[i];
+  const double alzy = (alz3 - alz4) * g.f2fyi[j];
+  const double alzz = (alz5 - alz6) * g.c2czi[k];
+  double alz = 1. / sb.re * (alzx + alzy + alzz);
+  if (les) {
+    const double *nu = f.nu, *n1 = f.nu1[0], *n2 = f.nu1[1];
+    const double talzx = (n2[c + dx] * alz1 - n2[c] * alz2) * g.f2fxi[i];
+    const double talzy = (n1[c + dy] * alz3 - n1[c] * alz4) * g.f2fyi[j];
+    const double talzz = (nu[c] * alz5 - nu[c - dz] * alz6) * g.c2czi[k];
+    alz = alz + 1.0 * (talzx + talzy + 2. * talzz);
+    const double rhsw1 = g.f2fxi[i] * g.c2czi[k] *
+                         (n2[c + dx] * (u[c + dx] - u[c + dx - dz]) - n2[c] * (u[c] - u[c - dz]));
+    const double rhsw2 = g.f2fyi[j] * g.c2czi[k] *
+                         (n1[c + dy] * (v[c + dy] - v[c + dy - dz]) - n1[c] * (v[c] - v[c - dz]));
+    rk3z = rk3z + 1.0 * (rhsw1 + rhsw2);
+  }
+  const double ro = f.rk3o[2][c];
+  f.rhs[2][c] = sb.dt * (sb.gam * rk3z + sb.ro * ro + 2. * sb.alpha * alz -
+                         2. * sb.alpha * (p[c] - p[c - dz]) * g.c2czi[k]);
+  f.rk3oo[2][c] = ro;
+  f.rk3o[2][c] = rk3z;
+}
+
+void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
+  hipLaunchKernelGGL(k_rhs_u, grid3(g.n1m - g.ib + 1, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
+  hipLaunchKernelGGL(k_rhs_v, grid3(g.n1m, g.n2m - g.jb + 1, g.n3m), BLK3, 0, s, g, sb, f, les);
+  if (g.n3m != 1)
+    hipLaunchKernelGGL(k_rhs_w, grid3(g.n1m, g.n2m, g.n3m - g.kb + 1), BLK3, 0, s, g, sb, f, les);
+}
+
+// ---------------------------------------------------------------------------
+// immersed-boundary forcing, slv_mmtm.f90:569-819 and slv_engy.f90:201-306:
+// phase 1 gathers the 27-point targets from the unmodified rhs1 (the
+// reference's rhstmp copy), phase 2 scatters; inner points force to zero.
+// ---------------------------------------------------------------------------
+__device__ inline void reindex_dev(int n, int nm, int &ip, int &ipp) {
+  if (ipp == n) ipp = 1;
+  else if (ipp == n + 1) { ip = 1; ipp = 2; }
+  else if (ipp == 0) ipp = nm;
+  else if (ipp == -1) { ip = nm; ipp = nm - 1; }
+}
+
+__global__ void k_ibm_gather(Geo g, IBComp ib, const double *fld, const double *rhs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ib.nintp) return;
+  const int ii = ib.ifc[n], jj = ib.jfc[n], kk = ib.kfc[n];
+  const int *ix = ib.indx + 3 * n;
+  int ip = ii + ix[0], jp = jj + ix[1], kp = kk + ix[2];
+  int ipp = ii + ix[0] * 2, jpp = jj + ix[1] * 2, kpp = kk + ix[2] * 2;
+  if (ipp >= g.n1 || ipp <= 0) reindex_dev(g.n1, g.n1m, ip, ipp);
+  if (kpp >= g.n3 || kpp <= 0) reindex_dev(g.n3, g.n3m, kp, kpp);
+  const int is[3] = {ii, ip, ipp}, js[3] = {jj, jp, jpp}, ks[3] = {kk, kp, kpp};
+  const double *gf = ib.geomfac + 27 * (long)n;
+  double targ = gf[0] * 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (a == 0 && b == 0 && d == 0) continue;
+        const long q = P(g, is[a], js[b], ks[d]);
+        targ = targ + gf[9 * a + 3 * b + d] * (fld[q] + rhs[q]);
+      }
+  ib.targ[n] = targ;
+}
+
+__global__ void k_ibm_scatter(Geo g, IBComp ib, const double *fld, double *rhs, double dti, int has_dudtr) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ib.nbody) return;
+  const long q = P(g, ib.ifc[n], ib.jfc[n], ib.kfc[n]);
+  const double targ = n < ib.nintp ? ib.targ[n] : 0.0;
+  const double fv = fld[q];
+  const double fc = (targ - fv - rhs[q]) * dti;
+  ib.fcv[n] = fc;
+  if (has_dudtr) ib.dudtr[n] = (targ - fv) * dti;
+  rhs[q] = targ - fv;
+  ib.fcvavg[n] = ib.fcvavg[n] + fc;
+}
+
+void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp) {
+  const double *fld = comp == 0 ? f.u : comp == 1 ? f.v : comp == 2 ? f.w : f.t;
+  const double dti = 1. / sb.dt;
+  IBComp &c = ib[comp];
+  if (c.nintp > 0)
+    hipLaunchKernelGGL(k_ibm_gather, dim3((c.nintp + 255) / 256), dim3(256), 0, s, g, c, fld, f.rhs[comp]);
+  if (c.nbody > 0)
+    hipLaunchKernelGGL(k_ibm_scatter, dim3((c.nbody + 255) / 256), dim3(256), 0, s, g, c, fld,
+                       f.rhs[comp], dti, comp < 3 ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// convective outlet, slv_mmtm.f90:873-988 (convbc) + 991-1041 (rhsincorpbc):
+// one workgroup runs the three dependent reductions and the plane updates
+// ---------------------------------------------------------------------------
+constexpr int kBC = 1024;
+
+__device__ inline double plane_val(const double *a, const Geo &g, int j, int k) { return a[j + g.NY * k]; }
+
+__global__ __launch_bounds__(kBC) void k_convbc_mom(Geo g, Sub sb, Fld f, int les) {
+  __shared__ double sh[kBC / 64];
+  __shared__ double s_bc[4];
+  const int tid = threadIdx.x;
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
+  const double *u = f.u, *v = f.v, *w = f.w;
+  // qin: the reference sums each loop serially; here each loop is a fixed
+  // deterministic tree (rounding-level difference only)
+  double a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    a += u[P(g, 1, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  double qin = block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n1m * n3m; q += kBC) {
+    const int i = 1 + q % n1m, k = 1 + q / n1m;
+    a += (v[P(g, i, 1, k)] - v[P(g, i, n2, k)]) * g.f2fx[i] * g.f2fz[k];
+  }
+  qin = qin + block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n1m * n2m; q += kBC) {
+    const int i = 1 + q % n1m, j = 1 + q / n1m;
+    a += (w[P(g, i, j, 1)] - w[P(g, i, j, n3)]) * g.f2fx[i] * g.f2fy[j];
+  }
+  qin = qin + block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    a += u[P(g, n1, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  double ubar = block_sum<kBC>(a, sh);
+  if (tid == 0) {
+    s_bc[0] = qin;
+    s_bc[1] = ubar / (g.yl * g.zl);
+  }
+  __syncthreads();
+  qin = s_bc[0];
+  ubar = s_bc[1];
+  const double ucoef = ubar * sb.dtconst * g.f2fxi[n1m];
+  const double vwcoef = ubar * sb.dtconst * g.c2cxi[n1];
+  a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long pb = j + g.NY * k;
+    double uo, vo, wo;
+    if (f.mask[P(g, n1m, j, k)] & 64) {  // outlet cell solid: zero gradient
+      uo = u[P(g, n1m, j, k)];
+      vo = v[P(g, n1m, j, k)];
+      wo = w[P(g, n1m, j, k)];
+    } else {
+      uo = u[P(g, n1, j, k)] - ucoef * (u[P(g, n1, j, k)] - u[P(g, n1m, j, k)]);
+      vo = v[P(g, n1, j, k)] - vwcoef * (v[P(g, n1, j, k)] - v[P(g, n1m, j, k)]);
+      wo = w[P(g, n1, j, k)] - vwcoef * (w[P(g, n1, j, k)] - w[P(g, n1m, j, k)]);
+    }
+    f.uout[pb] = uo;
+    f.vout[pb] = vo;
+    f.wout[pb] = wo;
+    a += uo * g.f2fy[j] * g.f2fz[k];
+  }
+  const double qout = block_sum<kBC>(a, sh);
+  if (tid == 0) s_bc[2] = qin / qout;
+  __syncthreads();
+  const double qratio = s_bc[2];
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long pb = j + g.NY * k;
+    if (!(f.mask[P(g, n1m, j, k)] & 64)) {
+      f.uout[pb] = f.uout[pb] * qratio;
+      f.vout[pb] = f.vout[pb] * qratio;
+      f.wout[pb] = f.wout[pb] * qratio;
+    }
+    f.duout[pb] = f.uout[pb] - u[P(g, n1, j, k)];
+    f.dvout[pb] = f.vout[pb] - v[P(g, n1, j, k)];
+    f.dwout[pb] = f.wout[pb] - w[P(g, n1, j, k)];
+  }
+  __syncthreads();
+  for (int j = tid; j <= n2; j += kBC) {  // z periodic ghosts of the deltas
+    f.duout[j + g.NY * n3] = f.duout[j + g.NY * 1];
+    f.duout[j] = f.duout[j + g.NY * n3m];
+    f.dvout[j + g.NY * n3] = f.dvout[j + g.NY * 1];
+    f.dvout[j] = f.dvout[j + g.NY * n3m];
+    f.dwout[j + g.NY * n3] = f.dwout[j + g.NY * 1];
+    f.dwout[j] = f.dwout[j + g.NY * n3m];
+  }
+  __syncthreads();
+  // rhsincorpbc (ICH = 0)
+  const double cre = les ? sb.re : 0., cre2 = les ? 2. * sb.re : 0.;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long pb = j + g.NY * k;
+    const long c = P(g, n1m, j, k);
+    f.rhs[0][c] = f.rhs[0][c] - sb.acoef * g.ciu[n1m] * (1. + cre2 * f.nu[c]) * f.duout[pb];
+    if (j >= 2)
+      f.rhs[1][c] = f.rhs[1][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[2][P(g, n1, j, k)]) * f.dvout[pb];
+    if (k >= g.kb)
+      f.rhs[2][c] = f.rhs[2][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[1][P(g, n1, j, k)]) * f.dwout[pb];
+  }
+}
+
+void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &, int les) {
+  hipLaunchKernelGGL(k_convbc_mom, dim3(1), dim3(kBC), 0, s, g, sb, f, les);
+}
+
+// ---------------------------------------------------------------------------
+// ADI line solves, slv_mmtm.f90:1044-1641 and slv_engy.f90:460-609.
+// One thread per line; lines are mapped so neighbouring threads touch
+// neighbouring addresses (i for the z and y sweeps). C = 0,1,2 momentum
+// component, 3 temperature.
+// ---------------------------------------------------------------------------
+struct Coef {
+  double a, b, c, g;
+};
+
+template <int C>
+__device__ inline void coef_z(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
+                              int j, int k, Coef &o) {
+  const long q = P(g, i, j, k), dz = g.NX * g.NY;
+  double ak, ck, bd;
+  if (C == 0) {
+    ak = g.akuv[k] * (1. + cre * f.nu1[1][q]);
+    ck = g.ckuv[k] * (1. + cre * f.nu1[1][q + dz]);
+    bd = sb.acoefi;
+  } else if (C == 1) {
+    ak = g.akuv[k] * (1. + cre * f.nu1[0][q]);
+    ck = g.ckuv[k] * (1. + cre * f.nu1[0][q + dz]);
+    bd = sb.acoefi;
+  } else if (C == 2) {
+    ak = g.akw[k] * (1. + cre2 * f.nu[q - dz]);
+    ck = g.ckw[k] * (1. + cre2 * f.nu[q]);
+    bd = sb.acoefi;
+  } else {
+    ak = g.akuv[k] * (1. * 1. + cre * 1.0 * f.al1[1][q]);
+    ck = g.ckuv[k] * (1. * 1. + cre * 1.0 * f.al1[1][q + dz]);
+    bd = sb.acoefi * sb.pr;
+  }
+  o.a = ak;
+  o.c = ck;
+  o.b = bd - ak - ck;
+  o.g = bd * f.rhs[C][q];
+}
+
+template <int C>
+__device__ inline void coef_y(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
+                              int j, int k, Coef &o) {
+  const long q = P(g, i, j, k), dy = g.NX;
+  double aj, cj, bd;
+  if (C == 0) {
+    aj = g.ajuw[j] * (1. + cre * f.nu1[2][q]);
+    cj = g.cjuw[j] * (1. + cre * f.nu1[2][q + dy]);
+    bd = sb.acoefi;
+  } else if (C == 1) {
+    aj = g.ajv[j] * (1. + cre2 * f.nu[q - dy]);
+    cj = g.cjv[j] * (1. + cre2 * f.nu[q]);
+    bd = sb.acoefi;
+  } else if (C == 2) {
+    aj = g.ajuw[j] * (1. + cre * f.nu1[0][q]);
+    cj = g.cjuw[j] * (1. + cre * f.nu1[0][q + dy]);
+    bd = sb.acoefi;
+  } else {
+    aj = g.ajuw[j] * (1. * 1. + cre * 1.0 * f.al1[2][q]);
+    cj = g.cjuw[j] * (1. * 1. + cre * 1.0 * f.al1[2][q + dy]);
+    bd = sb.acoefi * sb.pr;
+  }
+  o.a = aj;
+  o.c = cj;
+  o.b = bd - aj - cj;
+  o.g = bd * f.rhs[C][q];
+}
+
+template <int C>
+__device__ inline void coef_x(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
+                              int j, int k, Coef &o) {
+  const long q = P(g, i, j, k);
+  double ai, ci, bd;
+  if (C == 0) {
+    ai = g.aiu[i] * (1. + cre2 * f.nu[q - 1]);
+    ci = g.ciu[i] * (1. + cre2 * f.nu[q]);
+    bd = sb.acoefi;
+  } else if (C == 1) {
+    ai = g.aivw[i] * (1. + cre * f.nu1[2][q]);
+    ci = g.civw[i] * (1. + cre * f.nu1[2][q + 1]);
+    bd = sb.acoefi;
+  } else if (C == 2) {
+    ai = g.aivw[i] * (1. + cre * f.nu1[1][q]);
+    ci = g.civw[i] * (1. + cre * f.nu1[1][q + 1]);
+    bd = sb.acoefi;
+  } else {
+    ai = g.aivw[i] * (1. * 1. + cre * 1.0 * f.al1[0][q]);
+    ci = g.civw[i] * (1. * 1. + cre * 1.0 * f.al1[0][q + 1]);
+    bd = sb.acoefi * sb.pr;
+  }
+  o.a = ai;
+  o.c = ci;
+  o.b = bd - ai - ci;
+  o.g = bd * f.rhs[C][q];
+}
+
+// periodic z line (trdiag3p, slv_mmtm.f90:1572-1641); result into rhs
+template <int C>
+__global__ void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo) {
+  const int i = ilo + blockIdx.x * 64 + threadIdx.x;
+  const int j = jlo + blockIdx.y * 4 + threadIdx.y;
+  if (i > g.n1m || j > g.n2m) return;
+  const int n = g.n3m;
+  double *r = f.rhs[C];
+  double *sq = f.s1, *ss = f.s2;
+  Coef cf;
+  coef_z<C>(g, f, sb, cre, cre2, i, j, n, cf);
+  const double fn = cf.g;
+  coef_z<C>(g, f, sb, cre, cre2, i, j, 1, cf);
+  const double binv = 1. / cf.b;
+  double qp = -cf.c * binv, sp = -cf.a * binv, fp = cf.g * binv;
+  long q1 = P(g, i, j, 1);
+  sq[q1] = qp;
+  ss[q1] = sp;
+  r[q1] = fp;
+  double an = 0, bn = 0, cn = 0;
+  for (int k = 2; k <= n; ++k) {
+    coef_z<C>(g, f, sb, cre, cre2, i, j, k, cf);
+    const double pn = 1. / (cf.b + cf.a * qp);
+    qp = -cf.c * pn;
+    sp = -cf.a * sp * pn;
+    fp = (cf.g - cf.a * fp) * pn;
+    const long q = P(g, i, j, k);
+    sq[q] = qp;
+    ss[q] = sp;
+    r[q] = fp;
+    an = cf.a;
+    bn = cf.b;
+    cn = cf.c;
+  }
+  // backward pass: s(n) = 1, qe(n) = 0; qe overwrites f in place
+  double snext = 1.0, qenext = 0.0;
+  for (int k = n - 1; k >= 1; --k) {
+    const long q = P(g, i, j, k);
+    const double qk = sq[q];
+    const double sk = ss[q] + qk * snext;
+    const double qek = r[q] + qk * qenext;
+    ss[q] = sk;
+    r[q] = qek;
+    snext = sk;
+    qenext = qek;
+  }
+  const double qe1 = r[q1], s1 = ss[q1];
+  const long qn1 = P(g, i, j, n - 1);
+  const double qenm = r[qn1], snm = ss[qn1];
+  const double fnn = (fn - cn * qe1 - an * qenm) / (cn * s1 + an * snm + bn);
+  r[P(g, i, j, n)] = fnn;
+  for (int k = n - 1; k >= 1; --k) {
+    const long q = P(g, i, j, k);
+    r[q] = fnn * ss[q] + r[q];
+  }
+}
+
+// non-periodic y line (trdiag2, slv_mmtm.f90:1415-1455); result into rhs
+template <int C>
+__global__ void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo, int klo) {
+  const int i = ilo + blockIdx.x * 64 + threadIdx.x;
+  const int k = klo + blockIdx.y * 4 + threadIdx.y;
+  if (i > g.n1m || k > g.n3m) return;
+  double *r = f.rhs[C];
+  double *gm = f.s1;
+  Coef cf;
+  coef_y<C>(g, f, sb, cre, cre2, i, jlo, k, cf);
+  double bet = 1. / cf.b;
+  double up = cf.g * bet;
+  double cprev = cf.c;
+  r[P(g, i, jlo, k)] = up;
+  for (int j = jlo + 1; j <= g.n2m; ++j) {
+    coef_y<C>(g, f, sb, cre, cre2, i, j, k, cf);
+    const double gam = cprev * bet;
+    bet = 1. / (cf.b - cf.a * gam);
+    up = (cf.g - cf.a * up) * bet;
+    const long q = P(g, i, j, k);
+    gm[q] = gam;
+    r[q] = up;
+    cprev = cf.c;
+  }
+  double unext = up;
+  for (int j = g.n2m - 1; j >= jlo; --j) {
+    const long q = P(g, i, j, k);
+    const double uu = r[q] - gm[q + g.NX] * unext;
+    r[q] = uu;
+    unext = uu;
+  }
+}
+
+// non-periodic x line (trdiag1, slv_mmtm.f90:1365-1412); field += solution
+template <int C>
+__global__ void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo, int klo) {
+  const int j = jlo + blockIdx.x * 64 + threadIdx.x;
+  const int k = klo + blockIdx.y * 4 + threadIdx.y;
+  if (j > g.n2m || k > g.n3m) return;
+  double *fld = C == 0 ? f.u : C == 1 ? f.v : C == 2 ? f.w : f.t;
+  double *r = f.rhs[C];
+  double *gm = f.s1;
+  Coef cf;
+  coef_x<C>(g, f, sb, cre, cre2, ilo, j, k, cf);
+  double bet = 1. / cf.b;
+  double up = cf.g * bet;
+  double cprev = cf.c;
+  r[P(g, ilo, j, k)] = up;
+  for (int i = ilo + 1; i <= g.n1m; ++i) {
+    coef_x<C>(g, f, sb, cre, cre2, i, j, k, cf);
+    const double gam = cprev * bet;
+    bet = 1. / (cf.b - cf.a * gam);
+    up = (cf.g - cf.a * up) * bet;
+    const long q = P(g, i, j, k);
+    gm[q] = gam;
+    r[q] = up;
+    cprev = cf.c;
+  }
+  double unext = up;
+  long q = P(g, g.n1m, j, k);
+  fld[q] = unext + fld[q];
+  for (int i = g.n1m - 1; i >= ilo; --i) {
+    q = P(g, i, j, k);
+    const double uu = r[q] - gm[q + 1] * unext;
+    fld[q] = uu + fld[q];
+    unext = uu;
+  }
+}
+
+template <int C>
+static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+  const int ilo = C == 0 ? g.ib : 1;
+  const int jlo = C == 1 ? g.jb : 1;
+  const int klo = C == 2 ? g.kb : 1;
+  if (g.n3m != 1)
+    hipLaunchKernelGGL(k_adi_z<C>, grid3(g.n1m - ilo + 1, g.n2m - jlo + 1, 1), BLK3, 0, s, g, sb, f, cre,
+                       cre2, ilo, jlo);
+  hipLaunchKernelGGL(k_adi_y<C>, grid3(g.n1m - ilo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre, cre2,
+                     ilo, jlo, klo);
+  hipLaunchKernelGGL(k_adi_x<C>, grid3(g.n2m - jlo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre, cre2,
+                     ilo, jlo, klo);
+}
+
+void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les) {
+  const double cre = (double)les * sb.re, cre2 = 2. * (double)les * sb.re;
+  if (comp == 0) adi3<0>(s, g, sb, f, cre, cre2);
+  else if (comp == 1) adi3<1>(s, g, sb, f, cre, cre2);
+  else adi3<2>(s, g, sb, f, cre, cre2);
+}
+
+void launch_adi_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
+  const double cre = (double)les * sb.re * sb.pr;
+  adi3<3>(s, g, sb, f, cre, 0.0);
+}
+
+// retrv_uvw + prdic_adj_uvw(0) for periodic z (slv_mmtm.f90:1710-1857)
+__global__ void k_retrv_mom(Geo g, Fld f) {
+  const int j = blockIdx.x * 64 + threadIdx.x;  // 0..n2
+  const int k = blockIdx.y;                     // 0..n3
+  if (j > g.n2) return;
+  if (k >= 1 && k <= g.n3m && j >= 1 && j <= g.n2m) {
+    const long pb = j + g.NY * k;
+    f.u[P(g, g.n1, j, k)] = f.uout[pb];
+    if (j >= 2) f.v[P(g, g.n1, j, k)] = f.vout[pb];
+    if (k >= g.kb) f.w[P(g, g.n1, j, k)] = f.wout[pb];
+  }
+}
+
+__global__ void k_zghost_uvw(Geo g, Fld f) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i > g.n1) return;
+  const long lo = P(g, i, j, 0), hi = P(g, i, j, g.n3), a = P(g, i, j, g.n3m), b = P(g, i, j, 1);
+  if (i >= 1) {
+    f.u[lo] = f.u[a];
+    f.u[hi] = f.u[b];
+  }
+  if (j >= 1) {
+    f.v[lo] = f.v[a];
+    f.v[hi] = f.v[b];
+  }
+  f.w[lo] = f.w[a];
+  f.w[hi] = f.w[b];
+}
+
+void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f) {
+  hipLaunchKernelGGL(k_retrv_mom, dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);
+  hipLaunchKernelGGL(k_zghost_uvw, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+}
+
+// ---------------------------------------------------------------------------
+// divgs, slv_cont.f90:2-208. The IB mass source is applied per cell in the
+// reference's loop order (u own, u from i+1, v own, v from j+1, w own, w from
+// k+1), which reproduces its sequential accumulation without atomics.
+// ---------------------------------------------------------------------------
+__global__ void k_divgs(Geo g, Sub sb, Fld f, int masson) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const long c = P(g, i, j, k);
+  const long dy = g.NX, dz = g.NX * g.NY;
+  const double d1 = (f.u[c + 1] - f.u[c]) * g.f2fxi[i];
+  const double d2 = (f.v[c + dy] - f.v[c]) * g.f2fyi[j];
+  const double d3 = (f.w[c + dz] - f.w[c]) * g.f2fzi[k];
+  double d = (d1 + d2 + d3);
+  if (masson) {
+    double qm = 0.0;
+    const uint8_t m0 = f.mask[c];
+    // u: own point, then point i+1 whose imv is i (non-periodic x: i+1 <= n1m)
+    if (m0 & M_BODY_U) {
+      const double val = f.u[c] * g.f2fxi[i];
+      qm = qm - val;
+      d = d + val;
+    }
+    if (i + 1 <= g.n1m && g.imv[i + 1] == i && (f.mask[c + 1] & M_BODY_U)) {
+      const double val = f.u[c + 1] * g.f2fxi[i];
+      qm = qm + val;
+      d = d - val;
+    }
+    if (m0 & M_BODY_V) {
+      const double val = f.v[c] * g.f2fyi[j];
+      qm = qm - val;
+      d = d + val;
+    }
+    if (j + 1 <= g.n2m && g.jmv[j + 1] == j && (f.mask[c + dy] & M_BODY_V)) {
+      const double val = f.v[c + dy] * g.f2fyi[j];
+      qm = qm + val;
+      d = d - val;
+    }
+    if (m0 & M_BODY_W) {
+      const double val = f.w[c] * g.f2fzi[k];
+      qm = qm - val;
+      d = d + val;
+    }
+    {
+      const int kp = g.kpv[k];  // the point whose kmv is k
+      const long cp = P(g, i, j, kp);
+      if (g.kmv[kp] == k && (f.mask[cp] & M_BODY_W)) {
+        const double val = f.w[cp] * g.f2fzi[k];
+        qm = qm + val;
+        d = d - val;
+      }
+    }
+    f.qmass[c] = qm;
+  }
+  f.div[c] = d * sb.dtconsti;
+}
+
+void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson) {
+  hipLaunchKernelGGL(k_divgs, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, masson);
+}
+
+// ---------------------------------------------------------------------------
+// ucalc (slv_auxi.f90:233-416) and pcalc (419-493) fused; the outlet-plane
+// mean of phi is reduced first by one workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBC) void k_phiref(Geo g, Fld f) {
+  __shared__ double sh[kBC / 64];
+  double a = 0.0;
+  for (int q = threadIdx.x; q < g.n2m * g.n3m; q += kBC) {
+    const int j = 1 + q % g.n2m, k = 1 + q / g.n2m;
+    a += f.phi[P(g, g.n1m, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  const double s = block_sum<kBC>(a, sh);
+  if (threadIdx.x == 0) f.red[0] = s / (g.yl * g.zl);
+}
+
+__global__ void k_ucalc_pcalc(Geo g, Sub sb, Fld f) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const long c = P(g, i, j, k);
+  const double *phi = f.phi;
+  const double dc = sb.dtconst;
+  if (i >= g.ib) {
+    const int im = g.imv[i];
+    f.u[c] = f.u[c] - dc * (phi[c] - phi[P(g, im, j, k)]) * g.c2cxi[i];
+  }
+  if (j >= 2) f.v[c] = f.v[c] - dc * (phi[c] - phi[c - g.NX]) * g.c2cyi[j];
+  if (k >= g.kb) {
+    const int km = g.kmv[k];
+    f.w[c] = f.w[c] - dc * (phi[c] - phi[P(g, i, j, km)]) * g.c2czi[k];
+  }
+  const double pn = f.p[c] + phi[c] - 0.5 * sb.dtconst * f.div[c] * 1. / sb.re;
+  f.p[c] = pn - f.red[0];
+}
+
+__global__ void k_zghost_p(Geo g, Fld f) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y;
+  if (i > g.n1m) return;
+  f.p[P(g, i, j, 0)] = f.p[P(g, i, j, g.n3m)];
+  f.p[P(g, i, j, g.n3)] = f.p[P(g, i, j, 1)];
+}
+
+void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f) {
+  hipLaunchKernelGGL(k_phiref, dim3(1), dim3(kBC), 0, s, g, f);
+  hipLaunchKernelGGL(k_ucalc_pcalc, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f);
+  hipLaunchKernelGGL(k_zghost_uvw, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+  hipLaunchKernelGGL(k_zghost_p, dim3((g.n1m + 63) / 64, g.n2m), dim3(64), 0, s, g, f);
+}
+
+// ---------------------------------------------------------------------------
+// energy right-hand side, slv_engy.f90:29-198 (ICONJG = 0, IDECOMP = 0)
+// ---------------------------------------------------------------------------
+__global__ void k_rhs_t(Geo g, Sub sb, Fld f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const double *u = f.u, *v = f.v, *w = f.w, *t = f.t;
+  const int kp = g.kpv[k], km = g.kmv[k], jp = g.jpv[j], jm = g.jmv[j], ip = g.ipv[i], im = g.imv[i];
+  const double fiu = g.fixiu[i], fil = g.fixil[i], fju = g.fixju[j], fjl = g.fixjl[j], fku = g.fixku[k],
+               fkl = g.fixkl[k];
+  const long c = P(g, i, j, k);
+  const long cip = P(g, ip, j, k), cim = P(g, im, j, k), cjp = P(g, i, jp, k), cjm = P(g, i, jm, k),
+             ckp = P(g, i, j, kp), ckm = P(g, i, j, km);
+  const double te = 0.5 * (g.f2fx[i] * t[cip] + g.f2fx[ip] * t[c]) * g.c2cxi[ip] * (1. - fiu) + t[cip] * fiu;
+  const double tw = 0.5 * (g.f2fx[im] * t[c] + g.f2fx[i] * t[cim]) * g.c2cxi[i] * (1. - fil) + t[cim] * fil;
+  const double tn = 0.5 * (g.f2fy[j] * t[cjp] + g.f2fy[jp] * t[c]) * g.c2cyi[jp] * (1. - fju) + t[cjp] * fju;
+  const double ts = 0.5 * (g.f2fy[jm] * t[c] + g.f2fy[j] * t[cjm]) * g.c2cyi[j] * (1. - fjl) + t[cjm] * fjl;
+  const double tc = 0.5 * (g.f2fz[k] * t[ckp] + g.f2fz[kp] * t[c]) * g.c2czi[kp] * (1. - fku) + t[ckp] * fku;
+  const double tf = 0.5 * (g.f2fz[km] * t[c] + g.f2fz[k] * t[ckm]) * g.c2czi[k] * (1. - fkl) + t[ckm] * fkl;
+  const double ant1 = (te * u[cip] - tw * u[c]) * g.f2fxi[i];
+  const double ant2 = (tn * v[cjp] - ts * v[c]) * g.f2fyi[j];
+  const double ant3 = (tc * w[ckp] - tf * w[c]) * g.f2fzi[k];
+  const double omega = 1.0;
+  double rk3t = -omega * (ant1 + ant2 + ant3) + 0.0;
+  const double alt1 = (t[cip] - t[c]) * g.c2cxi[ip];
+  const double alt2 = (t[c] - t[cim]) * g.c2cxi[i];
+  const double alt3 = (t[cjp] - t[c]) * g.c2cyi[jp];
+  const double alt4 = (t[c] - t[cjm]) * g.c2cyi[j];
+  const double alt5 = (t[ckp] - t[c]) * g.c2czi[kp];
+  const double alt6 = (t[c] - t[ckm]) * g.c2czi[k];
+  const double altx = (alt1 - alt2) * g.f2fxi[i];
+  const double alty = (alt3 - alt4) * g.f2fyi[j];
+  const double altz = (alt5 - alt6) * g.f2fzi[k];
+  double alt = 1. / (sb.re * sb.pr) * (altx + alty + altz);
+  if (les) {
+    const double *a1 = f.al1[0], *a2 = f.al1[1], *a3 = f.al1[2];
+    const double taltx = g.f2fxi[i] * (a1[cip] * g.c2cxi[ip] * (t[cip] - t[c]) -
+                                       a1[c] * g.c2cxi[i] * (t[c] - t[cim]));
+    const double talty = g.f2fyi[j] * (a2[cjp] * g.c2cyi[jp] * (t[cjp] - t[c]) -
+                                       a2[c] * g.c2cyi[j] * (t[c] - t[cjm]));
+    const double taltz = g.f2fzi[k] * (a3[ckp] * g.c2czi[kp] * (t[ckp] - t[c]) -
+                                       a3[c] * g.c2czi[k] * (t[c] - t[ckm]));
+    alt = alt + 1.0 * omega * (taltx + talty + taltz);
+    // rhssgs_t leaves zero in rhs1(:,:,:,4) (sgs.f90:1776-1803)
+    rk3t = rk3t + 1.0 * omega * (0.0 + 0.0 + 0.0);
+  }
+  const double ro = f.rk3o[3][c];
+  f.rhs[3][c] = sb.dt * (sb.gam * rk3t + sb.ro * ro + 2. * sb.alpha * alt);
+  f.rk3oo[3][c] = ro;
+  f.rk3o[3][c] = rk3t;
+}
+
+void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
+  hipLaunchKernelGGL(k_rhs_t, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
+}
+
+// convbc_t (slv_engy.f90:309-413) + rhsincorpbc_t (416-457), ICONJG = 0
+__global__ __launch_bounds__(kBC) void k_convbc_t(Geo g, Sub sb, Fld f, int les) {
+  __shared__ double sh[kBC / 64];
+  __shared__ double s_bc[4];
+  const int tid = threadIdx.x;
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
+  const double *u = f.u, *v = f.v, *w = f.w, *t = f.t;
+  double a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    a += u[P(g, 1, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  double qin = block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n1m * n3m; q += kBC) {
+    const int i = 1 + q % n1m, k = 1 + q / n1m;
+    a += (v[P(g, i, 1, k)] - v[P(g, i, n2, k)]) * g.f2fx[i] * g.f2fz[k];
+  }
+  qin = qin + block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n1m * n2m; q += kBC) {
+    const int i = 1 + q % n1m, j = 1 + q / n1m;
+    a += (w[P(g, i, j, 1)] - w[P(g, i, j, n3)]) * g.f2fx[i] * g.f2fy[j];
+  }
+  qin = qin + block_sum<kBC>(a, sh);
+  a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    a += u[P(g, n1, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  double ubar = block_sum<kBC>(a, sh);
+  if (tid == 0) {
+    s_bc[0] = qin;
+    s_bc[1] = ubar / (g.yl * g.zl);
+  }
+  __syncthreads();
+  qin = s_bc[0];
+  ubar = s_bc[1];
+  const double ucoef = ubar * sb.dtconst * g.f2fxi[n1m];
+  const double vwcoef = ubar * sb.dtconst * g.c2cxi[n1];
+  a = 0.0;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long pb = j + g.NY * k;
+    const double uo = u[P(g, n1, j, k)] - ucoef * (u[P(g, n1, j, k)] - u[P(g, n1m, j, k)]);
+    f.uout[pb] = uo;
+    if (f.mask[P(g, n1m, j, k)] & 64)
+      f.tout[pb] = t[P(g, n1m, j, k)];
+    else
+      f.tout[pb] = t[P(g, n1, j, k)] - vwcoef * (t[P(g, n1, j, k)] - t[P(g, n1m, j, k)]);
+    a += uo * g.f2fy[j] * g.f2fz[k];
+  }
+  const double qout = block_sum<kBC>(a, sh);
+  if (tid == 0) s_bc[2] = fabs(qout) > 1.0e-14 ? qin / qout : 1.0;
+  __syncthreads();
+  const double qratio = s_bc[2];
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long pb = j + g.NY * k;
+    if (!(f.mask[P(g, n1m, j, k)] & 64)) f.tout[pb] = f.tout[pb] * qratio;
+    f.dtout[pb] = f.tout[pb] - t[P(g, n1, j, k)];
+  }
+  __syncthreads();
+  for (int j = tid; j <= n2; j += kBC) {
+    f.dtout[j + g.NY * n3] = f.dtout[j + g.NY * 1];
+    f.dtout[j] = f.dtout[j + g.NY * n3m];
+  }
+  __syncthreads();
+  const double cre2 = les ? 2. * sb.re * sb.pr : 0.;
+  for (int q = tid; q < n2m * n3m; q += kBC) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    const long c = P(g, n1m, j, k);
+    f.rhs[3][c] = f.rhs[3][c] - sb.acoef * 1. * 1. * g.ciu[n1m] * (1. + cre2 * 1.0 * f.al[c]) *
+                                    f.dtout[j + g.NY * k];
+  }
+}
+
+void launch_convbc_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &, int les,
+                     const lw_settings &) {
+  hipLaunchKernelGGL(k_convbc_t, dim3(1), dim3(kBC), 0, s, g, sb, f, les);
+}
+
+// retrv_t, prdic_adj_t (periodic z) and wallbc_t (slv_engy.f90:612-812)
+__global__ void k_retrv_t(Geo g, Fld f) {
+  const int j = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int k = 1 + blockIdx.y;
+  if (j > g.n2m) return;
+  f.t[P(g, g.n1, j, k)] = f.tout[j + g.NY * k];
+}
+
+__global__ void k_zghost_t(Geo g, Fld f) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i > g.n1) return;
+  f.t[P(g, i, j, 0)] = f.t[P(g, i, j, g.n3m)];
+  f.t[P(g, i, j, g.n3)] = f.t[P(g, i, j, 1)];
+}
+
+__global__ void k_inlet_t(Geo g, Fld f, double tsol, double tflu) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (j > g.n2) return;
+  const long c = P(g, 0, j, k);
+  f.t[c] = (f.mask[c] & 128) ? tsol : tflu;
+}
+
+__global__ void k_ywall_t(Geo g, Fld f, int bb, int bt, double vb, double vt) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i > g.n1) return;
+  const long c0 = P(g, i, 0, k), c1 = P(g, i, 1, k), cn = P(g, i, g.n2, k), cnm = P(g, i, g.n2m, k);
+  if (bb == 0) f.t[c0] = vb;
+  else if (bb == 1) f.t[c0] = f.t[c1] - vb / g.c2cyi[1];
+  if (bt == 0) f.t[cn] = vt;
+  else if (bt == 1) f.t[cn] = f.t[cnm] + vt / g.c2cyi[g.n2];
+}
+
+void launch_energy_bcs(hipStream_t s, const Geo &g, const Fld &f, const Body &, const lw_settings &st) {
+  hipLaunchKernelGGL(k_retrv_t, dim3((g.n2m + 63) / 64, g.n3m), dim3(64), 0, s, g, f);
+  hipLaunchKernelGGL(k_zghost_t, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+  hipLaunchKernelGGL(k_inlet_t, dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f, st.tsol_i, st.tflu_i);
+  hipLaunchKernelGGL(k_ywall_t, dim3((g.n1 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f, (int)st.bc_t_ybtm,
+                     (int)st.bc_t_ytop, st.val_t_ybtm, st.val_t_ytop);
+}
+
+// ---------------------------------------------------------------------------
+// CFL (slv_auxi.f90:53-102), convergence_check / masscheck
+// (misc_init.f90:1047-1104), lagforce (slv_auxi.f90:519-570)
+// ---------------------------------------------------------------------------
+__global__ void k_cfl(Geo g, Fld f, double dt) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double c0 = 0.0;
+  if (i <= g.n1m && j <= g.n2m) {
+    const long c = P(g, i, j, k);
+    const double c1 = fabs(f.u[c] + f.u[c + 1]) * 0.5 * g.f2fxi[i];
+    const double c2 = fabs(f.v[c] + f.v[c + g.NX]) * 0.5 * g.f2fyi[j];
+    const double c3 = fabs(f.w[c] + f.w[c + g.NX * g.NY]) * 0.5 * g.f2fzi[k];
+    c0 = (c1 + c2 + c3) * dt;
+  }
+  for (int o = 32; o > 0; o >>= 1) c0 = fmax(c0, __shfl_down(c0, o, 64));
+  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) atomic_max_nonneg(&f.redmax[0], c0);
+}
+
+void launch_cfl(hipStream_t s, const Geo &g, const Fld &f, double dt) {
+  hipLaunchKernelGGL(k_cfl, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, dt);
+}
+
+__global__ void k_diag(Geo g, Fld f, int masson) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double dv = 0.0, qm = 0.0;
+  if (i <= g.n1m && j <= g.n2m) {
+    const long c = P(g, i, j, k);
+    const double d1 = (f.u[c + 1] - f.u[c]) * g.f2fxi[i];
+    const double d2 = (f.v[c + g.NX] - f.v[c]) * g.f2fyi[j];
+    const double d3 = (f.w[c + g.NX * g.NY] - f.w[c]) * g.f2fzi[k];
+    const double dd = masson ? d1 + d2 + d3 - f.qmass[c] : d1 + d2 + d3;
+    dv = fabs(dd);
+    qm = masson ? fabs(f.qmass[c]) : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dv = fmax(dv, __shfl_down(dv, o, 64));
+    qm = fmax(qm, __shfl_down(qm, o, 64));
+  }
+  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
+    atomic_max_nonneg(&f.redmax[1], dv);
+    atomic_max_nonneg(&f.redmax[2], qm);
+  }
+}
+
+void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson) {
+  hipLaunchKernelGGL(k_diag, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, masson);
+}
+
+__global__ __launch_bounds__(256) void k_lagforce(Geo g, Sub sb, Fld f, IBComp u, IBComp v, IBComp w) {
+  __shared__ double sh[4];
+  const IBComp *cs[3] = {&u, &v, &w};
+  double acc[3] = {0, 0, 0};
+  for (int c = 0; c < 3; ++c) {
+    const IBComp &ib = *cs[c];
+    for (int n = threadIdx.x; n < ib.nbody; n += 256) {
+      const int ii = ib.ifc[n], jj = ib.jfc[n], kk = ib.kfc[n];
+      const long q = P(g, ii, jj, kk);
+      const double rr = ib.dudtr[n] - (sb.gam * f.rk3o[c][q] + sb.ro * f.rk3oo[c][q]);
+      double vol;
+      if (c == 0) vol = g.c2cx[ii] * g.f2fy[jj] * g.f2fz[kk];
+      else if (c == 1) vol = g.f2fx[ii] * g.c2cy[jj] * g.f2fz[kk];
+      else vol = g.f2fx[ii] * g.f2fy[jj] * g.c2cz[kk];
+      acc[c] += vol * rr;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double s = block_sum<256>(acc[c], sh);
+    if (threadIdx.x == 0) f.red[8 + c] = (sb.msub == 1 ? 0.0 : f.red[8 + c]) + s;
+  }
+}
+
+void launch_lagforce(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib) {
+  hipLaunchKernelGGL(k_lagforce, dim3(1), dim3(256), 0, s, g, sb, f, ib[0], ib[1], ib[2]);
+}
+
+}  // namespace lw

@@ -1,0 +1,542 @@
+// gfx950 kernels of the subgrid-scale models (sgs.f90): dynamic Vreman with a
+// global Germano coefficient (sgs_dvmg, 405-638), constant Vreman (sgs_cvm,
+// 335-402), dynamic and constant eddy diffusivity (sgs_edm 1297-1478,
+// sgs_cedm 1250-1294), the Simpson box test filter (test_filter 641-769),
+// zeroing in the body (nutzero/alpzero) and edge interpolation
+// (nutinterpol 941-1133, alpinterpol 1591-1773).
+#include "lw_kernels.h"
+
+namespace lw {
+
+#define BLK3 dim3(64, 4, 1)
+static inline dim3 grid3(int ni, int nj, int nk) {
+  return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)(nk > 0 ? nk : 1));
+}
+
+__device__ inline long I3(const Geo &g, int i, int j, int k) {
+  return (long)(i - 1) + (long)g.n1m * ((long)(j - 1) + (long)g.n2m * (long)(k - 1));
+}
+
+// velgrad (sgs.f90:772-875): a = du_j/dx_i in (11,21,31,12,22,32,13,23,33), sr = S_ij
+__device__ void velgrad(const Geo &g, const Fld &f, int i, int j, int k, double a[9], double sr[6]) {
+  const double *u = f.u, *v = f.v, *w = f.w;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  const int kp = g.kpv[k], km = g.kmv[k], jp = g.jpv[j], jm = g.jmv[j], ip = g.ipv[i], im = g.imv[i];
+  const double fju = g.fixju[j], fjl = g.fixjl[j], fku = g.fixku[k], fkl = g.fixkl[k], fiu = g.fixiu[i],
+               fil = g.fixil[i];
+#define U(a_, b_, c_) u[P(g, a_, b_, c_)]
+#define V(a_, b_, c_) v[P(g, a_, b_, c_)]
+#define W(a_, b_, c_) w[P(g, a_, b_, c_)]
+  const double vg11 = g.f2fxi[i] * (U(ip, j, k) - U(i, j, k));
+  double up = g.c2cyi[jp] * 0.25 * (g.f2fy[jp] * (U(i, j, k) + U(ip, j, k)) + g.f2fy[j] * (U(i, jp, k) + U(ip, jp, k))) *
+                  (1. - fju) +
+              0.5 * (U(i, n2, k) + U(ip, n2, k)) * fju;
+  double um = g.c2cyi[j] * 0.25 * (g.f2fy[j] * (U(i, jm, k) + U(ip, jm, k)) + g.f2fy[jm] * (U(i, j, k) + U(ip, j, k))) *
+                  (1. - fjl) +
+              0.5 * (U(i, 0, k) + U(ip, 0, k)) * fjl;
+  const double vg12 = g.f2fyi[j] * (up - um);
+  up = g.c2czi[kp] * 0.25 * (g.f2fz[kp] * (U(i, j, k) + U(ip, j, k)) + g.f2fz[k] * (U(i, j, kp) + U(ip, j, kp))) *
+           (1. - fku) +
+       0.5 * (U(i, j, n3) + U(ip, j, n3)) * fku;
+  um = g.c2czi[k] * 0.25 * (g.f2fz[k] * (U(i, j, km) + U(ip, j, km)) + g.f2fz[km] * (U(i, j, k) + U(ip, j, k))) *
+           (1. - fkl) +
+       0.5 * (U(i, j, 0) + U(ip, j, 0)) * fkl;
+  const double vg13 = g.f2fzi[k] * (up - um);
+  double vp = g.c2cxi[ip] * 0.25 * (g.f2fx[ip] * (V(i, j, k) + V(i, jp, k)) + g.f2fx[i] * (V(ip, j, k) + V(ip, jp, k))) *
+                  (1. - fiu) +
+              0.5 * (V(n1, j, k) + V(n1, jp, k)) * fiu;
+  double vm = g.c2cxi[i] * 0.25 * (g.f2fx[i] * (V(im, j, k) + V(im, jp, k)) + g.f2fx[im] * (V(i, j, k) + V(i, jp, k))) *
+                  (1. - fil) +
+              0.5 * (V(0, j, k) + V(0, jp, k)) * fil;
+  const double vg21 = g.f2fxi[i] * (vp - vm);
+  const double vg22 = g.f2fyi[j] * (V(i, jp, k) - V(i, j, k));
+  vp = g.c2czi[kp] * 0.25 * (g.f2fz[kp] * (V(i, j, k) + V(i, jp, k)) + g.f2fz[k] * (V(i, j, kp) + V(i, jp, kp))) *
+           (1. - fku) +
+       0.5 * (V(i, j, n3) + V(i, jp, n3)) * fku;
+  vm = g.c2czi[k] * 0.25 * (g.f2fz[k] * (V(i, j, km) + V(i, jp, km)) + g.f2fz[km] * (V(i, j, k) + V(i, jp, k))) *
+           (1. - fkl) +
+       0.5 * (V(i, j, 0) + V(i, jp, 0)) * fkl;
+  const double vg23 = g.f2fzi[k] * (vp - vm);
+  double wp = g.c2cxi[ip] * 0.25 * (g.f2fx[ip] * (W(i, j, k) + W(i, j, kp)) + g.f2fx[i] * (W(ip, j, k) + W(ip, j, kp))) *
+                  (1. - fiu) +
+              0.5 * (W(n1, j, k) + W(n1, j, kp)) * fiu;
+  double wm = g.c2cxi[i] * 0.25 * (g.f2fx[i] * (W(im, j, k) + W(im, j, kp)) + g.f2fx[im] * (W(i, j, k) + W(i, j, kp))) *
+                  (1. - fil) +
+              0.5 * (W(0, j, k) + W(0, j, kp)) * fil;
+  const double vg31 = g.f2fxi[i] * (wp - wm);
+  wp = g.c2cyi[jp] * 0.25 * (g.f2fy[jp] * (W(i, j, k) + W(i, j, kp)) + g.f2fy[j] * (W(i, jp, k) + W(i, jp, kp))) *
+           (1. - fju) +
+       0.5 * (W(i, n2, k) + W(i, n2, kp)) * fju;
+  wm = g.c2cyi[j] * 0.25 * (g.f2fy[j] * (W(i, jm, k) + W(i, jm, kp)) + g.f2fy[jm] * (W(i, j, k) + W(i, j, kp))) *
+           (1. - fjl) +
+       0.5 * (W(i, 0, k) + W(i, 0, kp)) * fjl;
+  const double vg32 = g.f2fyi[j] * (wp - wm);
+  const double vg33 = g.f2fzi[k] * (W(i, j, kp) - W(i, j, k));
+#undef U
+#undef V
+#undef W
+  a[0] = vg11; a[1] = vg21; a[2] = vg31;
+  a[3] = vg12; a[4] = vg22; a[5] = vg32;
+  a[6] = vg13; a[7] = vg23; a[8] = vg33;
+  sr[0] = vg11;
+  sr[1] = 0.5 * (vg12 + vg21);
+  sr[2] = 0.5 * (vg13 + vg31);
+  sr[3] = vg22;
+  sr[4] = 0.5 * (vg23 + vg32);
+  sr[5] = vg33;
+}
+
+// Vreman sqrt(B_beta / alpha:alpha) (sgs.f90:442-470)
+__device__ inline double vreman(const double alp[9], double sx, double sy, double sz) {
+  double b[6];
+  b[0] = sx * alp[0] * alp[0] + sy * alp[3] * alp[3] + sz * alp[6] * alp[6];
+  b[1] = sx * alp[0] * alp[1] + sy * alp[3] * alp[4] + sz * alp[6] * alp[7];
+  b[2] = sx * alp[0] * alp[2] + sy * alp[3] * alp[5] + sz * alp[6] * alp[8];
+  b[3] = sx * alp[1] * alp[1] + sy * alp[4] * alp[4] + sz * alp[7] * alp[7];
+  b[4] = sx * alp[1] * alp[2] + sy * alp[4] * alp[5] + sz * alp[7] * alp[8];
+  b[5] = sx * alp[2] * alp[2] + sy * alp[5] * alp[5] + sz * alp[8] * alp[8];
+  const double aaa = alp[0] * alp[0] + alp[1] * alp[1] + alp[2] * alp[2] + alp[3] * alp[3] +
+                     alp[4] * alp[4] + alp[5] * alp[5] + alp[6] * alp[6] + alp[7] * alp[7] + alp[8] * alp[8];
+  double bbb = b[0] * b[3] + b[3] * b[5] + b[5] * b[0] - b[1] * b[1] - b[2] * b[2] - b[4] * b[4];
+  bbb = fmax(bbb, 1.0e-16);
+  if (aaa == 0.) return 0.0;
+  return sqrt(bbb / aaa);
+}
+
+__device__ inline double fwidth(double fm, double f0, double fp, int fixu, int fixl) {
+  return 0.5 * fm * (1. - (double)fixu) + f0 + 0.5 * fp * (1. - (double)fixl);
+}
+
+// temgrad (sgs.f90:1480-1525)
+__device__ void temgrad(const Geo &g, const Fld &f, int i, int j, int k, double a[3]) {
+  const double *t = f.t;
+  const int kp = g.kpv[k], km = g.kmv[k], jp = g.jpv[j], jm = g.jmv[j], ip = g.ipv[i], im = g.imv[i];
+  const double fiu = g.fixiu[i], fil = g.fixil[i], fju = g.fixju[j], fjl = g.fixjl[j], fku = g.fixku[k],
+               fkl = g.fixkl[k];
+  const long c = P(g, i, j, k);
+  double tp = 0.5 * (g.f2fx[i] * t[P(g, ip, j, k)] + g.f2fx[ip] * t[c]) * g.c2cxi[ip] * (1. - fiu) +
+              t[P(g, ip, j, k)] * fiu;
+  double tm = 0.5 * (g.f2fx[im] * t[c] + g.f2fx[i] * t[P(g, im, j, k)]) * g.c2cxi[i] * (1. - fil) +
+              t[P(g, im, j, k)] * fil;
+  a[0] = g.f2fxi[i] * (tp - tm);
+  tp = 0.5 * (g.f2fy[j] * t[P(g, i, jp, k)] + g.f2fy[jp] * t[c]) * g.c2cyi[jp] * (1. - fju) + t[P(g, i, jp, k)] * fju;
+  tm = 0.5 * (g.f2fy[jm] * t[c] + g.f2fy[j] * t[P(g, i, jm, k)]) * g.c2cyi[j] * (1. - fjl) + t[P(g, i, jm, k)] * fjl;
+  a[1] = g.f2fyi[j] * (tp - tm);
+  tp = 0.5 * (g.f2fz[k] * t[P(g, i, j, kp)] + g.f2fz[kp] * t[c]) * g.c2czi[kp] * (1. - fku) + t[P(g, i, j, kp)] * fku;
+  tm = 0.5 * (g.f2fz[km] * t[c] + g.f2fz[k] * t[P(g, i, j, km)]) * g.c2czi[k] * (1. - fkl) + t[P(g, i, j, km)] * fkl;
+  a[2] = g.f2fzi[k] * (tp - tm);
+}
+
+// ---------------------------------------------------------------------------
+// Vreman, dynamic global coefficient (sgs_dvmg)
+// buffers: a[0..8] alpha_ij, a[9..14] M_ij, a[15..17] u_i, a[18..23] L_ij
+// ---------------------------------------------------------------------------
+__global__ void k_dvmg1(Geo g, Fld f, SgsBuf b) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  double alp[9], str[6];
+  velgrad(g, f, i, j, k, alp, str);
+  const double r = vreman(alp, g.f2fx[i] * g.f2fx[i], g.f2fy[j] * g.f2fy[j], g.f2fz[k] * g.f2fz[k]);
+  f.nu[P(g, i, j, k)] = r;
+  const long q = I3(g, i, j, k);
+#pragma unroll
+  for (int l = 0; l < 9; ++l) b.a[l][q] = alp[l];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) b.a[9 + l][q] = r * str[l];
+}
+
+__global__ void k_dvmg2(Geo g, Fld f, SgsBuf b) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const long q = I3(g, i, j, k);
+  double alp[9], str[6];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) alp[l] = b.a[l][q];
+  str[0] = alp[0];
+  str[1] = 0.5 * (alp[3] + alp[1]);
+  str[2] = 0.5 * (alp[6] + alp[2]);
+  str[3] = alp[4];
+  str[4] = 0.5 * (alp[7] + alp[5]);
+  str[5] = alp[8];
+  const double dx = fwidth(g.f2fx[i - 1], g.f2fx[i], g.f2fx[i + 1], g.fixiu[i], g.fixil[i]);
+  const double dy = fwidth(g.f2fy[j - 1], g.f2fy[j], g.f2fy[j + 1], g.fixju[j], g.fixjl[j]);
+  const double dz = fwidth(g.f2fz[k - 1], g.f2fz[k], g.f2fz[k + 1], g.fixku[k], g.fixkl[k]);
+  const double ami = vreman(alp, dx * dx, dy * dy, dz * dz);
+#pragma unroll
+  for (int l = 0; l < 6; ++l) b.a[9 + l][q] = ami * str[l] - b.a[9 + l][q];
+  const long c = P(g, i, j, k);
+  const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
+  const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
+  const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
+  b.a[15][q] = u1;
+  b.a[16][q] = u2;
+  b.a[17][q] = u3;
+  b.a[18][q] = u1 * u1;
+  b.a[19][q] = u1 * u2;
+  b.a[20][q] = u1 * u3;
+  b.a[21][q] = u2 * u2;
+  b.a[22][q] = u2 * u3;
+  b.a[23][q] = u3 * u3;
+}
+
+// per-cell Germano products, summed per block into partial[2*block + 0/1]
+template <int NT>
+__device__ inline double bsum(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (w == 0) {
+    r = lane < NT / 64 ? sh[lane] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_down(r, o, 64);
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dvmg3(Geo g, Fld f, SgsBuf b, const int64_t *nwall) {
+  __shared__ double sh[kRedThreads / 64];
+  const long n = (long)g.n1m * g.n2m * g.n3m;
+  double lm = 0.0, mm = 0.0;
+  for (long q = blockIdx.x * (long)kRedThreads + threadIdx.x; q < n; q += (long)gridDim.x * kRedThreads) {
+    const int i = 1 + (int)(q % g.n1m);
+    const int j = 1 + (int)((q / g.n1m) % g.n2m);
+    const int k = 1 + (int)(q / ((long)g.n1m * g.n2m));
+    const double volume = g.f2fx[i] * g.f2fy[j] * g.f2fz[k];
+    const double U1 = b.a[15][q], U2 = b.a[16][q], U3 = b.a[17][q];
+    const double l1 = b.a[18][q] - U1 * U1, l2 = b.a[19][q] - U1 * U2, l3 = b.a[20][q] - U1 * U3,
+                 l4 = b.a[21][q] - U2 * U2, l5 = b.a[22][q] - U2 * U3, l6 = b.a[23][q] - U3 * U3;
+    const double m1 = b.a[9][q], m2 = b.a[10][q], m3 = b.a[11][q], m4 = b.a[12][q], m5 = b.a[13][q],
+                 m6 = b.a[14][q];
+    const double nw = (double)nwall[q];
+    lm += (2. * l2 * m2 + l1 * m1 + 2. * l3 * m3 + l4 * m4 + 2. * l5 * m5 + l6 * m6) * nw * volume;
+    mm += (2. * (m2 * m2) + m1 * m1 + 2. * (m3 * m3) + m4 * m4 + 2. * (m5 * m5) + m6 * m6) * nw * volume;
+  }
+  const double s1 = bsum<kRedThreads>(lm, sh);
+  const double s2 = bsum<kRedThreads>(mm, sh);
+  if (threadIdx.x == 0) {
+    f.partial[2 * blockIdx.x] = s1;
+    f.partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// final sum of the block partials and the clipped coefficient into red[slot]
+__global__ __launch_bounds__(kRedThreads) void k_coef(Fld f, int nblk, double scale, int slot) {
+  __shared__ double sh[kRedThreads / 64];
+  double a = 0.0, c = 0.0;
+  for (int q = threadIdx.x; q < nblk; q += kRedThreads) {
+    a += f.partial[2 * q];
+    c += f.partial[2 * q + 1];
+  }
+  const double lm = bsum<kRedThreads>(a, sh);
+  const double mm = bsum<kRedThreads>(c, sh);
+  if (threadIdx.x == 0) f.red[slot] = fmax(0., scale * lm / mm);
+}
+
+// 3-point Simpson box test filter in x (i = 2..n1m-1, non-periodic) and z
+// (periodic), in that order (test_filter, sgs.f90:641-769). in -> out.
+__device__ inline double xfilt(const Geo &g, const double *a, int i, int j, int k, int fx) {
+  const long q = I3(g, i, j, k);
+  if (!fx || i < 2 || i > g.n1m - 1) return a[q];
+  return g.cfx1[3 * i] * a[q - 1] + g.cfx1[3 * i + 1] * a[q] + g.cfx1[3 * i + 2] * a[q + 1];
+}
+
+__global__ void k_filter(Geo g, const double *in, double *out, int fx, int fz) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  double r;
+  if (fz) {
+    const int km = k == 1 ? g.n3m : k - 1;
+    const int kp = k == g.n3m ? 1 : k + 1;
+    r = g.cfz1[3 * k] * xfilt(g, in, i, j, km, fx) + g.cfz1[3 * k + 1] * xfilt(g, in, i, j, k, fx) +
+        g.cfz1[3 * k + 2] * xfilt(g, in, i, j, kp, fx);
+  } else {
+    r = xfilt(g, in, i, j, k, fx);
+  }
+  out[I3(g, i, j, k)] = r;
+}
+
+static void filter_many(hipStream_t s, const Geo &g, const SgsBuf &b, int lo, int hi, int filter) {
+  const int fx = (filter == 1 || filter == 3) ? 1 : 0;
+  const int fz = (filter == 2 || filter == 3) ? 1 : 0;
+  const size_t bytes = sizeof(double) * (size_t)g.n1m * g.n2m * g.n3m;
+  for (int l = lo; l <= hi; ++l) {
+    hipLaunchKernelGGL(k_filter, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, b.a[l], b.tmp, fx, fz);
+    hipMemcpyAsync(b.a[l], b.tmp, bytes, hipMemcpyDeviceToDevice, s);
+  }
+}
+
+// red slots: 4 = csgsts, 5 = csgshf
+void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter) {
+  hipLaunchKernelGGL(k_dvmg1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  filter_many(s, g, b, 9, 14, filter);
+  filter_many(s, g, b, 0, 8, filter);
+  hipLaunchKernelGGL(k_dvmg2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  filter_many(s, g, b, 15, 23, filter);
+}
+
+void launch_sgs_dvmg_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
+  hipLaunchKernelGGL(k_dvmg3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
+  hipLaunchKernelGGL(k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -0.5, 4);
+}
+
+__global__ void k_cvm(Geo g, Fld f, double csgsts) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  double a[9], sr[6];
+  velgrad(g, f, i, j, k, a, sr);
+  const double r = vreman(a, g.f2fx[i] * g.f2fx[i], g.f2fy[j] * g.f2fy[j], g.f2fz[k] * g.f2fz[k]);
+  f.nu[P(g, i, j, k)] = r == 0.0 ? 0.0 : csgsts * r;
+}
+
+void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts) {
+  hipLaunchKernelGGL(k_cvm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgsts);
+}
+
+// ---------------------------------------------------------------------------
+// eddy diffusivity, dynamic global coefficient (sgs_edm)
+// buffers: a[0..5] S_ij, a[6..8] dT/dx_k, a[9..11] M_k, a[15..17] u_i,
+// a[18..20] L_k, a[0..2] reused for T after the first filter pass
+// ---------------------------------------------------------------------------
+__global__ void k_edm1(Geo g, Fld f, SgsBuf b) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  double d[9], s[6], alp[3];
+  velgrad(g, f, i, j, k, d, s);
+  temgrad(g, f, i, j, k, alp);
+  const double del = pow(g.f2fx[i] * g.f2fy[j] * g.f2fz[k], 1. / 3.);
+  double sss = s[0] * s[0] + 2 * (s[1] * s[1]) + 2 * (s[2] * s[2]) + s[3] * s[3] + 2 * (s[4] * s[4]) + s[5] * s[5];
+  sss = fmax(sss, 1.0e-16);
+  const double al = (del * del) * sqrt(2. * sss);
+  f.al[P(g, i, j, k)] = al;
+  const long q = I3(g, i, j, k);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) b.a[9 + l][q] = al * alp[l];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) b.a[l][q] = s[l];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) b.a[6 + l][q] = alp[l];
+}
+
+__global__ void k_edm2(Geo g, Fld f, SgsBuf b) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const long q = I3(g, i, j, k);
+  const double d1 = fwidth(g.f2fx[i - 1], g.f2fx[i], g.f2fx[i + 1], g.fixiu[i], g.fixil[i]);
+  const double d2 = fwidth(g.f2fy[j - 1], g.f2fy[j], g.f2fy[j + 1], g.fixju[j], g.fixjl[j]);
+  const double d3 = fwidth(g.f2fz[k - 1], g.f2fz[k], g.f2fz[k + 1], g.fixku[k], g.fixkl[k]);
+  const double del = pow(d1 * d2 * d3, 1. / 3.);
+  const double a1 = b.a[0][q], a2 = b.a[1][q], a3 = b.a[2][q], a4 = b.a[3][q], a5 = b.a[4][q], a6 = b.a[5][q];
+  double sss = a1 * a1 + 2 * (a2 * a2) + 2 * (a3 * a3) + a4 * a4 + 2 * (a5 * a5) + a6 * a6;
+  sss = fmax(sss, 1.0e-16);
+  const double ami = (del * del) * sqrt(2. * sss);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) b.a[9 + l][q] = ami * b.a[6 + l][q] - b.a[9 + l][q];
+  const long c = P(g, i, j, k);
+  const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
+  const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
+  const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
+  const double tc = f.t[c];
+  b.a[15][q] = u1;
+  b.a[16][q] = u2;
+  b.a[17][q] = u3;
+  b.a[0][q] = tc;
+  b.a[1][q] = tc;
+  b.a[2][q] = tc;
+  b.a[18][q] = u1 * tc;
+  b.a[19][q] = u2 * tc;
+  b.a[20][q] = u3 * tc;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_edm3(Geo g, Fld f, SgsBuf b, const int64_t *nwall) {
+  __shared__ double sh[kRedThreads / 64];
+  const long n = (long)g.n1m * g.n2m * g.n3m;
+  double lm = 0.0, mm = 0.0;
+  for (long q = blockIdx.x * (long)kRedThreads + threadIdx.x; q < n; q += (long)gridDim.x * kRedThreads) {
+    const int i = 1 + (int)(q % g.n1m);
+    const int j = 1 + (int)((q / g.n1m) % g.n2m);
+    const int k = 1 + (int)(q / ((long)g.n1m * g.n2m));
+    const double volume = g.f2fx[i] * g.f2fy[j] * g.f2fz[k];
+    const double l1 = b.a[18][q] - b.a[15][q] * b.a[0][q];
+    const double l2 = b.a[19][q] - b.a[16][q] * b.a[1][q];
+    const double l3 = b.a[20][q] - b.a[17][q] * b.a[2][q];
+    const double m1 = b.a[9][q], m2 = b.a[10][q], m3 = b.a[11][q];
+    const double nw = (double)nwall[q];
+    lm += (l1 * m1 + l2 * m2 + l3 * m3) * nw * volume;
+    mm += (m1 * m1 + m2 * m2 + m3 * m3) * nw * volume;
+  }
+  const double s1 = bsum<kRedThreads>(lm, sh);
+  const double s2 = bsum<kRedThreads>(mm, sh);
+  if (threadIdx.x == 0) {
+    f.partial[2 * blockIdx.x] = s1;
+    f.partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter) {
+  hipLaunchKernelGGL(k_edm1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  filter_many(s, g, b, 9, 11, filter);
+  filter_many(s, g, b, 0, 8, filter);
+  hipLaunchKernelGGL(k_edm2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  filter_many(s, g, b, 15, 17, filter);
+  filter_many(s, g, b, 0, 2, filter);
+  filter_many(s, g, b, 18, 20, filter);
+}
+
+void launch_sgs_edm_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
+  hipLaunchKernelGGL(k_edm3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
+  hipLaunchKernelGGL(k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -1.0, 5);
+}
+
+__global__ void k_cedm(Geo g, Fld f, double csgshf) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  double d[9], s[6];
+  velgrad(g, f, i, j, k, d, s);
+  const double del = pow(g.f2fx[i] * g.f2fy[j] * g.f2fz[k], 1. / 3.);
+  double sss = s[0] * s[0] + 2. * (s[1] * s[1]) + 2. * (s[2] * s[2]) + s[3] * s[3] + 2. * (s[4] * s[4]) + s[5] * s[5];
+  sss = fmax(sss, 1.0e-16);
+  f.al[P(g, i, j, k)] = csgshf * (del * del) * sqrt(2. * sss);
+}
+
+void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf) {
+  hipLaunchKernelGGL(k_cedm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgshf);
+}
+
+// ---------------------------------------------------------------------------
+// scale by the dynamic coefficient (slot >= 0) and zero in the body and at
+// the listed points (nutzero/alpzero); then z-periodic ghosts
+// ---------------------------------------------------------------------------
+__global__ void k_scale_zero(Geo g, Fld f, double *nu, int slot) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const long c = P(g, i, j, k);
+  double v = nu[c];
+  if (slot >= 0) v = f.red[slot] * v;
+  if (f.mask[c] & (M_SOLID_C | M_NUTZERO)) v = 0.;
+  nu[c] = v;
+}
+
+// listed points that fall on ghost indices (the 0-based values read as
+// 1-based can be 0) are zeroed here; interior ones carry M_NUTZERO
+__global__ void k_zero_list(Geo g, double *nu, const int *lst, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  nu[P(g, lst[3 * q], lst[3 * q + 1], lst[3 * q + 2])] = 0.;
+}
+
+__global__ void k_zghost_scalar(Geo g, double *a) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i > g.n1) return;
+  a[P(g, i, j, 0)] = a[P(g, i, j, g.n3m)];
+  a[P(g, i, j, g.n3)] = a[P(g, i, j, 1)];
+}
+
+// edge interpolation: the final value of each nu1(i,j,k,c) after the
+// reference's sequence of loops (interior formula, x-boundary, y-boundary,
+// z-periodic copy), last writer wins. alp = 1 selects alpinterpol's ranges.
+struct EdgeCtx {
+  int ib, jb, kb;
+  int ybwall, ytwall;
+};
+
+__device__ inline double edge_main(const Geo &g, const double *nu, int i, int j, int k, int c) {
+  const double *fx = g.f2fx, *fy = g.f2fy, *fz = g.f2fz;
+#define N(a_, b_, c_) nu[P(g, a_, b_, c_)]
+  if (c == 0)
+    return g.c2cyi[j] * g.c2czi[k] * 0.25 *
+           (fy[j - 1] * (fz[k - 1] * N(i, j, k) + fz[k] * N(i, j, k - 1)) +
+            fy[j] * (fz[k - 1] * N(i, j - 1, k) + fz[k] * N(i, j - 1, k - 1)));
+  if (c == 1)
+    return g.c2czi[k] * g.c2cxi[i] * 0.25 *
+           (fz[k - 1] * (fx[i - 1] * N(i, j, k) + fx[i] * N(i - 1, j, k)) +
+            fz[k] * (fx[i - 1] * N(i, j, k - 1) + fx[i] * N(i - 1, j, k - 1)));
+  return g.c2cxi[i] * g.c2cyi[j] * 0.25 *
+         (fx[i - 1] * (fy[j - 1] * N(i, j, k) + fy[j] * N(i, j - 1, k)) +
+          fx[i] * (fy[j - 1] * N(i - 1, j, k) + fy[j] * N(i - 1, j - 1, k)));
+}
+
+__device__ double edge_234(const Geo &g, const double *nu, const EdgeCtx &e, int i, int j, int k, int c) {
+  const int n1 = g.n1, n2 = g.n2, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  double v = 0.0;
+  // main loops
+  bool inmain;
+  if (c == 0) inmain = k >= e.kb && k <= n3m && j >= e.jb && j <= n2m && i >= 1 && i <= n1m;
+  else if (c == 1) inmain = k >= e.kb && k <= n3m && j >= 1 && j <= n2m && i >= e.ib && i <= n1m;
+  else inmain = k >= 1 && k <= n3m && j >= e.jb && j <= n2m && i >= e.ib && i <= n1m;
+  if (inmain) v = edge_main(g, nu, i, j, k, c);
+  const double *fx = g.f2fx, *fy = g.f2fy, *fz = g.f2fz;
+  // x boundaries (inflow/outflow, never walls)
+  if ((i == 1 || i == n1) && c >= 1 && j >= e.jb && j <= n2m && k >= e.kb && k <= n3m) {
+    const int jm = g.jmv[j], km = g.kmv[k];
+    const int is = i == 1 ? 2 : n1m;
+    if (c == 1) v = g.c2czi[k] * 0.5 * (fz[k] * N(is, j, km) + fz[km] * N(is, j, k));
+    else v = g.c2cyi[j] * 0.5 * (fy[j] * N(is, jm, k) + fy[jm] * N(is, j, k));
+  }
+  // y boundaries
+  if ((j == 1 || j == n2) && k >= e.kb && k <= n3m && i >= e.ib && i <= n1m) {
+    const bool wall = j == 1 ? e.ybwall : e.ytwall;
+    if (wall) v = 0.;
+    else if (c >= 1) {
+      const int im = g.imv[i], km = g.kmv[k];
+      const int js = j == 1 ? 2 : n2m;
+      if (c == 1) v = g.c2czi[k] * 0.5 * (fz[k] * N(i, js, km) + fz[km] * N(i, js, k));
+      else v = g.c2cxi[i] * 0.5 * (fx[i] * N(im, js, k) + fx[im] * N(i, js, k));
+    }
+  }
+#undef N
+  return v;
+}
+
+__global__ void k_edges(Geo g, const double *nu, double *o1, double *o2, double *o3, EdgeCtx e) {
+  const int i = blockIdx.x * 64 + threadIdx.x;  // 0..n1
+  const int j = blockIdx.y * 4 + threadIdx.y;   // 0..n2
+  const int k = blockIdx.z;                     // 0..n3
+  if (i > g.n1 || j > g.n2) return;
+  double *out[3] = {o1, o2, o3};
+  const bool zcopy = k == g.n3 && i >= e.ib && i <= g.n1m && j >= e.jb && j <= g.n2m;
+  for (int c = 0; c < 3; ++c) {
+    double v;
+    if (zcopy) v = edge_234(g, nu, e, i, j, 1, c);
+    else if (k == 0 || k == g.n3) v = 0.0;  // never written by the reference loops
+    else v = edge_234(g, nu, e, i, j, k, c);
+    out[c][P(g, i, j, k)] = v;
+  }
+}
+
+void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1, int alp,
+                       int scale_slot, const lw_settings &st) {
+  hipLaunchKernelGGL(k_scale_zero, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, nu, scale_slot);
+  hipLaunchKernelGGL(k_zghost_scalar, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, nu);
+  EdgeCtx e;
+  e.ib = alp ? 1 : g.ib;
+  e.jb = alp ? 1 : g.jb;
+  e.kb = alp ? 1 : g.kb;
+  e.ybwall = st.bc_ybtm == 0;
+  e.ytwall = st.bc_ytop == 0;
+  hipLaunchKernelGGL(k_edges, grid3(g.n1 + 1, g.n2 + 1, g.n3 + 1), BLK3, 0, s, g, nu, nu1[0], nu1[1], nu1[2], e);
+}
+
+void launch_zero_list(hipStream_t s, const Geo &g, double *nu, const int *lst, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_list, dim3((n + 255) / 256), dim3(256), 0, s, g, nu, lst, n);
+}
+
+}  // namespace lw

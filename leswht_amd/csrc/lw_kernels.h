@@ -1,0 +1,75 @@
+// Kernel launchers of the LESwHT time step (gfx950). Each launcher enqueues
+// on the given stream; all arrays are device pointers.
+#pragma once
+#include "lw_internal.h"
+
+namespace lw {
+
+struct Fld {
+  double *u, *v, *w, *p, *t;
+  double *rhs[4];
+  double *rk3o[4], *rk3oo[4];
+  double *nu, *nu1[3], *al, *al1[3];
+  double *qmass, *div, *phi;
+  double *s1, *s2;  // ADI scratch (gam, or q and s of the periodic solve)
+  uint8_t *mask;
+  // outlet planes (0:n2, 0:n3), index j + NY*k
+  double *uout, *vout, *wout, *tout, *duout, *dvout, *dwout, *dtout;
+  // reduction scratch: red[0..63] doubles, partials[...] per block
+  double *red, *partial;
+  unsigned long long *redmax;  // [0] cfl, [1] dvmax, [2] qmmax
+};
+
+struct Body {
+  int kind;
+  double p0, p1, p2, p3;
+};
+
+// block-partial sum buffer size (blocks of the two-level reductions)
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+// ---- momentum / continuity / projection / energy (lw_kern_flow.hip) ----
+void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp);
+void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b, int les);
+void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les);
+void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f);
+void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson);
+void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f);
+void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+void launch_convbc_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b,
+                     int les, const lw_settings &st);
+void launch_adi_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+void launch_energy_bcs(hipStream_t s, const Geo &g, const Fld &f, const Body &b, const lw_settings &st);
+void launch_cfl(hipStream_t s, const Geo &g, const Fld &f, double dt);
+void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson);
+void launch_lagforce(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib);
+
+// ---- subgrid scale (lw_kern_sgs.hip) ----
+struct SgsBuf {
+  double *a[24];  // interior arrays (n1m*n2m*n3m each)
+  double *tmp;
+};
+void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter);
+void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts);
+void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter);
+void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf);
+void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1,
+                       int alp, int scale_slot, const lw_settings &st);
+
+// ---- Poisson (lw_kern_pois.hip) ----
+void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
+void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
+void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
+// toplevel (poiss_z_ft.f90:302-380); id 0 or 1
+void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv,
+                        unsigned long long m0, unsigned long long m1);
+void launch_mg_relax(hipStream_t s, const Geo &g, const MG &mg, int ilev, double oldv,
+                     int iiter, unsigned long long m0, unsigned long long m1);
+void launch_mg_godown(hipStream_t s, const Geo &g, const MG &mg, int ilev,
+                      unsigned long long m0, unsigned long long m1);
+void launch_mg_goup(hipStream_t s, const Geo &g, const MG &mg, int ilev,
+                    unsigned long long m0, unsigned long long m1);
+
+}  // namespace lw

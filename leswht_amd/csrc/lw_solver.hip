@@ -1,0 +1,828 @@
+// Host driver and C ABI of the LESwHT time step on MI355X (gfx950).
+//
+// One handle owns one case's state in HBM and enqueues the step on its own
+// HIP stream in the reference's order (solver.f90:85-181). Set-up work that
+// the reference does once (lhsinit, poisinit/coefmg, sgsfilterinit, the IB
+// masks derived from funcbody) is computed here on the host in FP64 with the
+// reference's expressions and uploaded.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lw_kernels.h"
+
+namespace lw {
+void launch_sgs_dvmg_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall);
+void launch_sgs_edm_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall);
+}  // namespace lw
+
+using namespace lw;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      g_err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x;       \
+      throw LW_ERR_DEVICE;                                                          \
+    }                                                                               \
+  } while (0)
+
+struct lw_handle {
+  hipStream_t s = nullptr;
+  lw_settings st;
+  lw_body body;
+  int les = 0, heat = 0, ncomp = 3;
+  Geo g;
+  Fld f;
+  MG mg;
+  IBComp ib[4];
+  SgsBuf sb;
+  int64_t *nwall = nullptr;
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  // host copies of metrics (Fortran indices)
+  std::vector<double> xmp, ymp, zmp, x;
+  // scalars
+  int64_t ntime = 0, ihist = 0, m = 0;
+  double dt = 0, time = 0, gamma[4] = {0}, ro[4] = {0}, cflmax = 0, dvmax = 0, qmmax = 0;
+  double csgsts = 0, csgshf = 0;
+  int64_t cyc_max = 0, cyc_sum = 0;
+  Sub sub;
+  hipEvent_t ev[8];
+  double ms[4] = {0, 0, 0, 0};
+  unsigned long long *h_sumres = nullptr;  // pinned
+  double *h_red = nullptr;                 // pinned
+
+  template <class T>
+  T *dalloc(size_t n) {
+    void *p = nullptr;
+    size_t b = sizeof(T) * (n ? n : 1);
+    HIPCHK(hipMalloc(&p, b));
+    HIPCHK(hipMemsetAsync(p, 0, b, s));
+    allocs.push_back(p);
+    bytes += (int64_t)b;
+    return (T *)p;
+  }
+  template <class T>
+  T *upload(const std::vector<T> &v) {
+    T *p = dalloc<T>(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+    return p;
+  }
+};
+
+static double funcbody(const lw_body &b, double x, double y, double z) {
+  (void)z;
+  switch (b.kind) {
+    default:
+      // geometry/funcbody_example/funcbody_cylinder.f90:13
+      return sqrt(x * x + y * y) - b.p[0];
+  }
+}
+
+static std::vector<double> vd(const double *src, int64_t lo, int64_t hi) {
+  std::vector<double> v((size_t)(hi + 2), 0.0);
+  for (int64_t q = lo; q <= hi; ++q) v[q] = src[q];
+  return v;
+}
+static std::vector<int> vi(const int64_t *src, int64_t lo, int64_t hi) {
+  std::vector<int> v((size_t)(hi + 2), 0);
+  for (int64_t q = lo; q <= hi; ++q) v[q] = (int)src[q];
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// set-up: lhsinit, poisinit/pmat/mwavenumber/coefmg, sgsfilterinit
+// ---------------------------------------------------------------------------
+static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
+  const lw_settings &st = h->st;
+  Geo &g = h->g;
+  g.n1 = (int)gr->n1; g.n2 = (int)gr->n2; g.n3 = (int)gr->n3;
+  g.n1m = (int)gr->n1m; g.n2m = (int)gr->n2m; g.n3m = (int)gr->n3m;
+  g.NX = g.n1 + 1; g.NY = g.n2 + 1; g.NZ = g.n3 + 1;
+  g.xl = gr->xl; g.yl = gr->yl; g.zl = gr->zl;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  auto c2cx = vd(gr->c2cx, 0, n1), c2cy = vd(gr->c2cy, 0, n2), c2cz = vd(gr->c2cz, 0, n3);
+  auto c2cxi = vd(gr->c2cxi, 0, n1), c2cyi = vd(gr->c2cyi, 0, n2), c2czi = vd(gr->c2czi, 0, n3);
+  auto f2fx = vd(gr->f2fx, 0, n1), f2fy = vd(gr->f2fy, 0, n2), f2fz = vd(gr->f2fz, 0, n3);
+  auto f2fxi = vd(gr->f2fxi, 0, n1), f2fyi = vd(gr->f2fyi, 0, n2), f2fzi = vd(gr->f2fzi, 0, n3);
+  h->xmp = vd(gr->xmp, 0, n1); h->ymp = vd(gr->ymp, 0, n2); h->zmp = vd(gr->zmp, 0, n3);
+  h->x = vd(gr->x, 1, n1);
+  auto y = vd(gr->y, 1, n2), z = vd(gr->z, 1, n3);
+  auto ipv = vi(gr->ipv, 1, n1m), jpv = vi(gr->jpv, 1, n2m), kpv = vi(gr->kpv, 1, n3m);
+  auto imv = vi(gr->imv, 1, n1m), jmv = vi(gr->jmv, 1, n2m), kmv = vi(gr->kmv, 1, n3m);
+  auto fixil = vi(gr->fixil, 1, n1m), fixjl = vi(gr->fixjl, 1, n2m), fixkl = vi(gr->fixkl, 1, n3m);
+  auto fixiu = vi(gr->fixiu, 1, n1m), fixju = vi(gr->fixju, 1, n2m), fixku = vi(gr->fixku, 1, n3m);
+  g.x = h->upload(h->x); g.y = h->upload(y); g.z = h->upload(z);
+  g.c2cx = h->upload(c2cx); g.c2cy = h->upload(c2cy); g.c2cz = h->upload(c2cz);
+  g.c2cxi = h->upload(c2cxi); g.c2cyi = h->upload(c2cyi); g.c2czi = h->upload(c2czi);
+  g.f2fx = h->upload(f2fx); g.f2fy = h->upload(f2fy); g.f2fz = h->upload(f2fz);
+  g.f2fxi = h->upload(f2fxi); g.f2fyi = h->upload(f2fyi); g.f2fzi = h->upload(f2fzi);
+  g.xmp = h->upload(h->xmp); g.ymp = h->upload(h->ymp); g.zmp = h->upload(h->zmp);
+  g.ipv = h->upload(ipv); g.jpv = h->upload(jpv); g.kpv = h->upload(kpv);
+  g.imv = h->upload(imv); g.jmv = h->upload(jmv); g.kmv = h->upload(kmv);
+  g.fixil = h->upload(fixil); g.fixjl = h->upload(fixjl); g.fixkl = h->upload(fixkl);
+  g.fixiu = h->upload(fixiu); g.fixju = h->upload(fixju); g.fixku = h->upload(fixku);
+  // lhsinit (slv_mmtm.f90:180-249)
+  g.ib = st.xprdic == 1 ? 1 : 2;
+  g.jb = st.yprdic == 1 ? 1 : 2;
+  g.kb = st.zprdic == 1 ? 1 : 2;
+  std::vector<double> aiu(n1 + 2), ciu(n1 + 2), aivw(n1 + 2), civw(n1 + 2), ajv(n2 + 2), cjv(n2 + 2),
+      ajuw(n2 + 2), cjuw(n2 + 2), akw(n3 + 2), ckw(n3 + 2), akuv(n3 + 2), ckuv(n3 + 2);
+  for (int ic = g.ib; ic <= n1m; ++ic) {
+    const int im = imv[ic];
+    aiu[ic] = -c2cxi[ic] * f2fxi[im];
+    ciu[ic] = -c2cxi[ic] * f2fxi[ic];
+  }
+  for (int ic = 1; ic <= n1m; ++ic) {
+    const int ip = ipv[ic];
+    aivw[ic] = -c2cxi[ic] * f2fxi[ic];
+    civw[ic] = -c2cxi[ip] * f2fxi[ic];
+  }
+  for (int jc = g.jb; jc <= n2m; ++jc) {
+    const int jm = jmv[jc];
+    ajv[jc] = -c2cyi[jc] * f2fyi[jm];
+    cjv[jc] = -c2cyi[jc] * f2fyi[jc];
+  }
+  for (int jc = 1; jc <= n2m; ++jc) {
+    const int jp = jpv[jc];
+    ajuw[jc] = -c2cyi[jc] * f2fyi[jc];
+    cjuw[jc] = -c2cyi[jp] * f2fyi[jc];
+  }
+  for (int kc = g.kb; kc <= n3m; ++kc) {
+    const int km = kmv[kc];
+    akw[kc] = -c2czi[kc] * f2fzi[km];
+    ckw[kc] = -c2czi[kc] * f2fzi[kc];
+  }
+  for (int kc = 1; kc <= n3m; ++kc) {
+    const int kp = kpv[kc];
+    akuv[kc] = -c2czi[kc] * f2fzi[kc];
+    ckuv[kc] = -c2czi[kp] * f2fzi[kc];
+  }
+  g.aiu = h->upload(aiu); g.ciu = h->upload(ciu); g.aivw = h->upload(aivw); g.civw = h->upload(civw);
+  g.ajv = h->upload(ajv); g.cjv = h->upload(cjv); g.ajuw = h->upload(ajuw); g.cjuw = h->upload(cjuw);
+  g.akw = h->upload(akw); g.ckw = h->upload(ckw); g.akuv = h->upload(akuv); g.ckuv = h->upload(ckuv);
+  // sgsfilterinit (sgs.f90:2-285), cf?1 only (the filter uses no cf?2)
+  auto filt = [](const std::vector<double> &c2c, int nm) {
+    std::vector<double> cf(3 * (nm + 2), 0.0);
+    if (nm != 1)
+      for (int i = 1; i <= nm; ++i) {
+        const double h1 = c2c[i], h2 = c2c[i + 1];
+        cf[3 * i + 0] = (2. * h1 - h2) / 6. / h1;
+        cf[3 * i + 1] = ((h1 + h2) * (h1 + h2)) / 6. / h1 / h2;
+        cf[3 * i + 2] = (2. * h2 - h1) / 6. / h2;
+      }
+    return cf;
+  };
+  g.cfx1 = h->upload(filt(c2cx, n1m));
+  g.cfy1 = h->upload(filt(c2cy, n2m));
+  g.cfz1 = h->upload(filt(c2cz, n3m));
+
+  // ---- fields ----
+  const size_t np = (size_t)g.NX * g.NY * g.NZ, ni = (size_t)n1m * n2m * n3m, nb = (size_t)g.NY * g.NZ;
+  Fld &f = h->f;
+  f.u = h->dalloc<double>(np); f.v = h->dalloc<double>(np); f.w = h->dalloc<double>(np);
+  f.p = h->dalloc<double>(np); f.t = h->dalloc<double>(np);
+  for (int c = 0; c < 4; ++c) {
+    f.rhs[c] = (c < 3 || h->heat) ? h->dalloc<double>(np) : nullptr;
+    f.rk3o[c] = (c < 3 || h->heat) ? h->dalloc<double>(np) : nullptr;
+    f.rk3oo[c] = (c < 3 || h->heat) ? h->dalloc<double>(np) : nullptr;
+  }
+  f.nu = h->dalloc<double>(np);
+  f.al = h->dalloc<double>(np);
+  for (int c = 0; c < 3; ++c) {
+    f.nu1[c] = h->dalloc<double>(np);
+    f.al1[c] = h->dalloc<double>(np);
+  }
+  f.qmass = h->dalloc<double>(np);
+  f.div = h->dalloc<double>(np);
+  f.phi = h->dalloc<double>(np);
+  f.s1 = h->dalloc<double>(np);
+  f.s2 = h->dalloc<double>(np);
+  f.uout = h->dalloc<double>(nb); f.vout = h->dalloc<double>(nb); f.wout = h->dalloc<double>(nb);
+  f.tout = h->dalloc<double>(nb); f.duout = h->dalloc<double>(nb); f.dvout = h->dalloc<double>(nb);
+  f.dwout = h->dalloc<double>(nb); f.dtout = h->dalloc<double>(nb);
+  f.red = h->dalloc<double>(64);
+  f.partial = h->dalloc<double>(2 * kRedBlocks);
+  f.redmax = h->dalloc<unsigned long long>(8);
+  if (h->les) {
+    for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
+    h->sb.tmp = h->dalloc<double>(ni);
+    std::vector<int64_t> nw(tb->nwall_dvm, tb->nwall_dvm + ni);
+    h->nwall = h->upload(nw);
+  }
+
+  // ---- IB tables and masks ----
+  std::vector<uint8_t> mask(np, 0);
+  auto PIh = [&](int i, int j, int k) { return (size_t)i + (size_t)g.NX * ((size_t)j + (size_t)g.NY * k); };
+  const uint8_t bodybit[4] = {M_BODY_U, M_BODY_V, M_BODY_W, M_BODY_T};
+  for (int c = 0; c < h->ncomp; ++c) {
+    IBComp &ib = h->ib[c];
+    ib.nintp = (int)tb->nintp[c];
+    ib.nbody = (int)tb->nbody[c];
+    std::vector<int> ifc(ib.nbody), jfc(ib.nbody), kfc(ib.nbody), ix(3 * (size_t)ib.nintp);
+    for (int n = 0; n < ib.nbody; ++n) {
+      ifc[n] = (int)tb->ifc[c][n];
+      jfc[n] = (int)tb->jfc[c][n];
+      kfc[n] = (int)tb->kfc[c][n];
+      if (ifc[n] < 0 || ifc[n] > n1 || jfc[n] < 0 || jfc[n] > n2 || kfc[n] < 0 || kfc[n] > n3) {
+        g_err = "IB forcing point outside the grid";
+        throw LW_ERR_ARG;
+      }
+      mask[PIh(ifc[n], jfc[n], kfc[n])] |= bodybit[c];
+    }
+    for (size_t q = 0; q < ix.size(); ++q) ix[q] = (int)tb->intpindx[c][q];
+    ib.ifc = h->upload(ifc); ib.jfc = h->upload(jfc); ib.kfc = h->upload(kfc); ib.indx = h->upload(ix);
+    std::vector<double> gf(tb->geomfac[c], tb->geomfac[c] + 27 * (size_t)ib.nintp);
+    ib.geomfac = h->upload(gf);
+    ib.targ = h->dalloc<double>(ib.nintp);
+    ib.fcv = h->dalloc<double>(ib.nbody);
+    ib.fcvavg = h->dalloc<double>(ib.nbody);
+    ib.dudtr = c < 3 ? h->dalloc<double>(ib.nbody) : nullptr;
+  }
+  for (int k = 1; k <= n3m; ++k)
+    for (int j = 1; j <= n2m; ++j)
+      for (int i = 1; i <= n1m; ++i)
+        if (!(funcbody(h->body, h->xmp[i], h->ymp[j], h->zmp[k]) >= 1.e-10)) mask[PIh(i, j, k)] |= M_SOLID_C;
+  if (h->les)
+    for (int64_t l = 0; l < tb->nzero; ++l) {
+      const int64_t i = tb->inz[l], j = tb->jnz[l], k = tb->knz[l];
+      if (i >= 1 && i <= n1m && j >= 1 && j <= n2m && k >= 1 && k <= n3m) mask[PIh(i, j, k)] |= M_NUTZERO;
+    }
+  for (int k = 1; k <= n3m; ++k)
+    for (int j = 1; j <= n2m; ++j)
+      if (funcbody(h->body, h->xmp[n1m], h->ymp[j], h->zmp[k]) <= 1.e-10) mask[PIh(n1m, j, k)] |= 64;
+  for (int k = 0; k <= n3; ++k)
+    for (int j = 0; j <= n2; ++j)
+      if (funcbody(h->body, h->xmp[0], h->ymp[j], h->zmp[k]) <= 1.e-10) mask[PIh(0, j, k)] |= 128;
+  f.mask = h->upload(mask);
+
+  // ---- Poisson (poisinit, poiss_z_ft.f90:18-113 and coefmg 643-798) ----
+  MG &mg = h->mg;
+  const int nlev = (int)st.nlev;
+  mg.nlev = nlev;
+  mg.n1md = 2 * n1m;
+  mg.n3mh = n3m / 2 + 1;
+  mg.nbli = (int)st.nbli;
+  mg.mgitr = (int)st.mgitr;
+  mg.LD = mg.n1md + 1;
+  mg.PLANE = mg.LD * (n2 + 1);
+  mg.PCLD = n1 + 1;
+  mg.PCPLANE = mg.PCLD * (n2 + 1);
+  const int n1md = mg.n1md, n3mh = mg.n3mh;
+  std::vector<double> aw(n1md + 2), ae(n1md + 2), as(n2 + 2), an(n2 + 2), coi1(n1md + 2), coi2(n1md + 2),
+      cor1(n1md + 2), cor2(n1md + 2), ak3(n3mh + 2);
+  std::vector<int> ipm(n1md + 2), imm(n1md + 2);
+  std::vector<double> ac((size_t)n3mh * mg.PLANE, 0.0), gam(ac.size(), 0.0), bet(ac.size(), 0.0);
+  auto AC = [&](int i, int j, int k) -> double & { return ac[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
+  auto GAMh = [&](int i, int j, int k) -> double & { return gam[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
+  auto BETh = [&](int i, int j, int k) -> double & { return bet[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
+  for (int i = 1; i <= n1m; ++i) {
+    const int ip = ipv[i];
+    aw[i] = (1.0 - fixil[i]) * c2cxi[i] * f2fxi[i];
+    ae[i] = (1.0 - fixiu[i]) * c2cxi[ip] * f2fxi[i];
+  }
+  for (int j = 1; j <= n2m; ++j) {
+    const int jp = jpv[j];
+    as[j] = (1.0 - fixjl[j]) * c2cyi[j] * f2fyi[j];
+    an[j] = (1.0 - fixju[j]) * c2cyi[jp] * f2fyi[j];
+  }
+  for (int j = 1; j <= n2m; ++j)
+    for (int i = 1; i <= n1m; ++i) AC(i, j, 1) = -1.0 * (aw[i] + ae[i] + as[j] + an[j]);
+  if (n3m > 1) {
+    const double pi = acos(-1.0);
+    for (int k = 1; k <= n3mh; ++k)
+      ak3[k] = 2.0 * (1.0 - cos(2.0 * (double)(k - 1) * pi / (double)n3m)) * f2fzi[1] * f2fzi[1];
+  }
+  for (int k = 2; k <= n3mh; ++k)
+    for (int j = 1; j <= n2m; ++j)
+      for (int i = 1; i <= n1m; ++i) AC(i, j, k) = AC(i, j, 1) - ak3[k];
+  // coefmg
+  {
+    const int minrow = n1m / (1 << nlev);
+    std::vector<double> xmpm(n1md + 2, 0.0), iwz(n1md + 2, 0.0), iez(n1md + 2, 0.0);
+    auto &iimg = mg.iimg;
+    iimg[nlev][1] = 1;
+    iimg[nlev][2] = n1m;
+    iimg[nlev][3] = n1m;
+    for (int i = 1; i <= n1m; ++i) {
+      xmpm[i] = h->xmp[i];
+      iwz[i] = (double)(1 - 1 / i);
+      iez[i] = (double)(1 - i / n1m);
+    }
+    const std::vector<double> &X = h->x;
+    for (int ilev = nlev - 1; ilev >= 0; --ilev) {
+      iimg[ilev][1] = iimg[ilev + 1][1] + iimg[ilev + 1][3];
+      iimg[ilev][3] = minrow * (1 << ilev);
+      iimg[ilev][2] = iimg[ilev][1] + iimg[ilev][3] - 1;
+      const int ibg = iimg[ilev][1], iend = iimg[ilev][2];
+      const int isp = 1 << (nlev - ilev);
+      int ic = 0;
+      for (int i = ibg; i <= iend; ++i) {
+        ic += 1;
+        xmpm[i] = 0.5 * (X[ic * isp + 1] + X[(ic - 1) * isp + 1]);
+        iwz[i] = (double)(1 - ibg / i);
+        iez[i] = (double)(1 - i / iend);
+      }
+      ic = 0;
+      for (int i = ibg; i <= iend; ++i) {
+        ic += 1;
+        const double dxc = X[ic * isp + 1] - X[(ic - 1) * isp + 1];
+        aw[i] = iwz[i] / ((xmpm[i] - xmpm[i - 1]) * dxc);
+        ae[i] = iez[i] / ((xmpm[i + 1] - xmpm[i]) * dxc);
+      }
+      for (int j = 1; j <= n2m; ++j)
+        for (int i = ibg; i <= iend; ++i) AC(i, j, 1) = -1.0 * (aw[i] + ae[i] + as[j] + an[j]);
+      for (int k = 2; k <= n3mh; ++k)
+        for (int j = 1; j <= n2m; ++j)
+          for (int i = ibg; i <= iend; ++i) AC(i, j, k) = AC(i, j, 1) - ak3[k];
+    }
+    for (int ilev = nlev; ilev >= 1; --ilev) {
+      int ibgh = iimg[ilev][1];
+      for (int i = iimg[ilev - 1][1]; i <= iimg[ilev - 1][2]; ++i) {
+        cor1[i] = (xmpm[ibgh + 1] - xmpm[i]) / (xmpm[ibgh + 1] - xmpm[ibgh]);
+        cor2[i] = 1.0 - cor1[i];
+        ibgh += 2;
+      }
+    }
+    for (int ilev = 0; ilev <= nlev - 1; ++ilev) {
+      int ibgh = iimg[ilev + 1][1] + 1;
+      for (int i = iimg[ilev][1]; i <= iimg[ilev][2] - 1; ++i) {
+        coi1[i] = (xmpm[i + 1] - xmpm[ibgh]) / (xmpm[i + 1] - xmpm[i]);
+        coi2[i] = 1.0 - coi1[i];
+        ibgh += 2;
+      }
+      coi1[iimg[ilev][2]] = 1.0;
+    }
+    for (int ilev = nlev; ilev >= 0; --ilev)
+      for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) {
+        ipm[i] = i + 1;
+        imm[i] = i - 1;
+      }
+    for (int ilev = 0; ilev <= nlev; ++ilev)
+      for (int k = 1; k <= n3mh; ++k) {
+        for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) BETh(i, 1, k) = 1.0 / AC(i, 1, k);
+        for (int j = 2; j <= n2m; ++j)
+          for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) {
+            GAMh(i, j, k) = an[j - 1] * BETh(i, j - 1, k);
+            BETh(i, j, k) = 1.0 / (AC(i, j, k) - as[j] * GAMh(i, j, k));
+          }
+      }
+  }
+  mg.aw = h->upload(aw); mg.ae = h->upload(ae); mg.as = h->upload(as); mg.an = h->upload(an);
+  mg.coi1 = h->upload(coi1); mg.coi2 = h->upload(coi2); mg.cor1 = h->upload(cor1); mg.cor2 = h->upload(cor2);
+  mg.ipm = h->upload(ipm); mg.imm = h->upload(imm);
+  mg.ac = h->upload(ac); mg.gam = h->upload(gam); mg.bet = h->upload(bet);
+  mg.gg = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+  mg.resd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+  mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
+  mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
+  mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);
+  std::vector<double> twr(n3m), twi(n3m);
+  const double pi2 = 8.0 * atan(1.0);
+  for (int q = 0; q < n3m; ++q) {
+    const double ang = -pi2 * (double)q / (double)n3m;
+    twr[q] = cos(ang);
+    twi[q] = sin(ang);
+  }
+  mg.twr = h->upload(twr);
+  mg.twi = h->upload(twi);
+}
+
+static void check_scope(const lw_settings *s, const lw_grid *g) {
+  auto bad = [](const char *m) {
+    g_err = m;
+    throw LW_ERR_ARG;
+  };
+  if (s->xprdic || s->yprdic) bad("x- or y-periodic grids are not supported (poiss_z_ft path, inflow/outflow in x)");
+  if (s->zprdic != 1) bad("the z direction must be periodic (poiss_z_ft)");
+  if (s->ich != 0) bad("ICH != 0 (channel driving) is not supported");
+  if (s->iconjg) bad("ICONJG = 1 (conjugate heat transfer) is not supported");
+  if (s->imovingon) bad("IMOVING = 1 is not supported");
+  if (s->idecomp) bad("IDECOMP = 1 is not supported");
+  if (s->gr != 0.0) bad("GR != 0 (buoyancy) is not supported");
+  if (s->iles == 1 && s->insmdl != 1) bad("only the Vreman model (INSMDL = 1) exists in the reference");
+  if (s->iles == 1 && s->ihtrans == 1 && s->itemdl != 0) bad("ITEMDL must be 0");
+  if (s->iles == 1 && (s->filter < 1 || s->filter > 3)) bad("FILTER must be 1, 2 or 3");
+  if (s->nlev < 1 || s->nlev > 19) bad("NLEV must be in 1..19");
+  if (g->n1m % (1 << s->nlev) != 0) bad("n1m must be divisible by 2**NLEV");
+  if (g->n1m % 2 != 0) bad("n1m must be even");
+  if (s->imgsor < g->n3m / 2 + 1) bad("IMGSOR < n3m/2+1 (line-SOR wavenumbers) is not supported");
+  if (g->n3m / 2 + 1 > 128) bad("n3m must be at most 254");
+  if (g->n3m < 2) bad("n3m must be at least 2");
+  if (s->nbli < 1) bad("NBLI must be >= 1");
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *lw_last_error(void) { return g_err.c_str(); }
+
+int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body, lw_handle **out) {
+  if (!s || !gr || !t || !out) return fail(LW_ERR_ARG, "null argument");
+  *out = nullptr;
+  lw_handle *h = nullptr;
+  try {
+    check_scope(s, gr);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(LW_ERR_DEVICE, "no HIP device");
+    h = new lw_handle();
+    h->st = *s;
+    if (body) h->body = *body;
+    else {
+      h->body.kind = 0;
+      h->body.p[0] = 0.5;
+    }
+    h->les = s->iles == 1;
+    h->heat = s->ihtrans == 1;
+    h->ncomp = h->heat ? 4 : 3;
+    h->csgsts = s->csgsts;
+    h->csgshf = s->csgshf;
+    HIPCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
+    HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
+    setup(h, gr, t);
+    // dttimeinit / rk3coefinit (misc_init.f90:1001-1044)
+    h->gamma[1] = 8. / 15.; h->gamma[2] = 5. / 12.; h->gamma[3] = 3. / 4.;
+    h->ro[1] = 0.; h->ro[2] = -17. / 60.; h->ro[3] = -5. / 12.;
+    h->dt = s->dt_size;
+    HIPCHK(hipStreamSynchronize(h->s));
+    *out = h;
+    return LW_OK;
+  } catch (int code) {
+    if (h) lw_destroy(h);
+    return code;
+  }
+}
+
+int lw_destroy(lw_handle *h) {
+  if (!h) return LW_OK;
+  if (h->s) hipStreamSynchronize(h->s);
+  for (void *p : h->allocs) hipFree(p);
+  for (int e = 0; e < 8; ++e)
+    if (h->ev[e]) hipEventDestroy(h->ev[e]);
+  if (h->h_sumres) hipHostFree(h->h_sumres);
+  if (h->h_red) hipHostFree(h->h_red);
+  if (h->s) hipStreamDestroy(h->s);
+  delete h;
+  return LW_OK;
+}
+
+int lw_get_dims(const lw_handle *h, int64_t n[3]) {
+  if (!h || !n) return fail(LW_ERR_ARG, "null argument");
+  n[0] = h->g.n1; n[1] = h->g.n2; n[2] = h->g.n3;
+  return LW_OK;
+}
+
+int64_t lw_device_bytes(const lw_handle *h) { return h ? h->bytes : 0; }
+
+int lw_set_fields(lw_handle *h, const double *u, const double *v, const double *w, const double *p, const double *t,
+                  int64_t ihist, int64_t m, double time, double dt) {
+  if (!h || !u || !v || !w || !p) return fail(LW_ERR_ARG, "null argument");
+  if (h->heat && !t) return fail(LW_ERR_ARG, "temperature field required (IHTRANS = 1)");
+  try {
+    const size_t b = sizeof(double) * (size_t)h->g.NX * h->g.NY * h->g.NZ;
+    HIPCHK(hipMemcpyAsync(h->f.u, u, b, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->f.v, v, b, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->f.w, w, b, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->f.p, p, b, hipMemcpyHostToDevice, h->s));
+    if (t) HIPCHK(hipMemcpyAsync(h->f.t, t, b, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    h->ihist = ihist;
+    h->m = m;
+    h->time = time;
+    h->dt = dt;
+    // dttimeinit (misc_init.f90:1001-1028)
+    if (h->st.idtopt == 0) h->dt = h->st.dt_size;
+    if (h->st.idtopt != 0 && h->st.iread == 0) h->dt = h->st.dt_size;
+    if (h->st.ireset == 1) {
+      h->ihist = 0;
+      h->time = 0.;
+    }
+    h->ntime = 0;
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+int lw_get_fields(lw_handle *h, double *u, double *v, double *w, double *p, double *t) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  try {
+    const size_t b = sizeof(double) * (size_t)h->g.NX * h->g.NY * h->g.NZ;
+    if (u) HIPCHK(hipMemcpyAsync(u, h->f.u, b, hipMemcpyDeviceToHost, h->s));
+    if (v) HIPCHK(hipMemcpyAsync(v, h->f.v, b, hipMemcpyDeviceToHost, h->s));
+    if (w) HIPCHK(hipMemcpyAsync(w, h->f.w, b, hipMemcpyDeviceToHost, h->s));
+    if (p) HIPCHK(hipMemcpyAsync(p, h->f.p, b, hipMemcpyDeviceToHost, h->s));
+    if (t && h->heat) HIPCHK(hipMemcpyAsync(t, h->f.t, b, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n) {
+  if (!h || !name) return fail(LW_ERR_ARG, "null argument");
+  const int64_t np = (int64_t)h->g.NX * h->g.NY * h->g.NZ;
+  const double *src = nullptr;
+  std::string s(name);
+  if (s == "nusgs") src = h->f.nu;
+  else if (s == "alsgs") src = h->f.al;
+  else if (s == "divgsum") src = h->f.div;
+  else if (s == "phi") src = h->f.phi;
+  else if (s == "qmass") src = h->f.qmass;
+  else if (s.size() == 8 && s.compare(0, 7, "nusgs1_") == 0 && s[7] >= '1' && s[7] <= '3') src = h->f.nu1[s[7] - '1'];
+  else if (s.size() == 8 && s.compare(0, 7, "alsgs1_") == 0 && s[7] >= '1' && s[7] <= '3') src = h->f.al1[s[7] - '1'];
+  else return fail(LW_ERR_ARG, "unknown array name");
+  if (!out) return np;
+  if (n < np) return fail(LW_ERR_ARG, "output buffer too small");
+  try {
+    HIPCHK(hipMemcpyAsync(out, src, sizeof(double) * np, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  } catch (int code) {
+    return code;
+  }
+  return np;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the step
+// ---------------------------------------------------------------------------
+static void stepinit(lw_handle *h) {
+  const lw_settings &st = h->st;
+  h->ntime += 1;
+  h->m = h->ntime;
+  for (int c = 0; c < h->ncomp; ++c)
+    if (h->ib[c].nbody > 0)
+      HIPCHK(hipMemsetAsync(h->ib[c].fcvavg, 0, sizeof(double) * h->ib[c].nbody, h->s));
+  if (st.idtopt == 0) h->dt = st.dt_size;
+  HIPCHK(hipMemsetAsync(h->f.redmax, 0, 8 * sizeof(unsigned long long), h->s));
+  launch_cfl(h->s, h->g, h->f, h->dt);
+  if (st.idtopt != 0) {
+    // the new dt depends on the CFL number: one host round trip per step
+    HIPCHK(hipMemcpyAsync(h->h_sumres, h->f.redmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    double c;
+    memcpy(&c, &h->h_sumres[0], 8);
+    h->cflmax = c;
+    if (h->cflmax != 0.) {
+      const double dtcfl = fmin(h->dt * st.cflfac / h->cflmax, h->dt * (0.80 + 0.20 * st.cflfac / h->cflmax));
+      if (st.idtopt == 1) h->dt = dtcfl;
+    }
+  }
+  const double eps_t = 1.0e-12 * fmax(1.0, fabs(h->time));
+  double dt_evt = h->dt;
+  if (st.ptb_tst > h->time + eps_t) dt_evt = fmin(dt_evt, st.ptb_tst - h->time);
+  if (st.avg_tst > h->time + eps_t) dt_evt = fmin(dt_evt, st.avg_tst - h->time);
+  if (st.tend > h->time + eps_t) dt_evt = fmin(dt_evt, st.tend - h->time);
+  if (dt_evt < h->dt) h->dt = dt_evt;
+}
+
+static void substepinit(lw_handle *h, int ms) {
+  Sub &sb = h->sub;
+  sb.msub = ms;
+  sb.dt = h->dt;
+  sb.gam = h->gamma[ms];
+  sb.ro = h->ro[ms];
+  sb.alpha = 0.5 * (h->gamma[ms] + h->ro[ms]);
+  sb.dtconst = h->dt * (h->gamma[ms] + h->ro[ms]);
+  sb.dtconsti = 1. / sb.dtconst;
+  sb.acoef = sb.alpha * h->dt / h->st.re;
+  sb.acoefi = 1. / sb.acoef;
+  sb.re = h->st.re;
+  sb.pr = h->st.pr;
+}
+
+static void phase_sgs(lw_handle *h) {
+  if (!h->les || h->sub.msub != 1) return;
+  const lw_settings &st = h->st;
+  const Geo &g = h->g;
+  Fld &f = h->f;
+  const size_t np = (size_t)g.NX * g.NY * g.NZ;
+  const bool zero_first = st.iread == 0 && h->ntime == 1;
+  // sgscalc (sgs.f90:288-332)
+  int slot = -1;
+  if (zero_first) HIPCHK(hipMemsetAsync(f.nu, 0, np * sizeof(double), h->s));
+  else if (st.idvmon == 1) {
+    launch_sgs_dvmg(h->s, g, f, h->sb, (int)st.filter);
+    launch_sgs_dvmg_coef(h->s, g, f, h->sb, h->nwall);
+    slot = 4;
+  } else {
+    launch_sgs_cvm(h->s, g, f, h->csgsts);
+  }
+  launch_sgs_finish(h->s, g, f, f.nu, f.nu1, 0, slot, st);
+  if (!h->heat) return;
+  // sgscalc_t (sgs.f90:1203-1247)
+  slot = -1;
+  if (zero_first) HIPCHK(hipMemsetAsync(f.al, 0, np * sizeof(double), h->s));
+  else if (st.idvmon == 1) {
+    launch_sgs_edm(h->s, g, f, h->sb, (int)st.filter);
+    launch_sgs_edm_coef(h->s, g, f, h->sb, h->nwall);
+    slot = 5;
+  } else {
+    launch_sgs_cedm(h->s, g, f, h->csgshf);
+  }
+  launch_sgs_finish(h->s, g, f, f.al, f.al1, 1, slot, st);
+}
+
+static void phase_rhsnlhs(lw_handle *h) {
+  const Geo &g = h->g;
+  Body b{};
+  launch_rhs_mom(h->s, g, h->sub, h->f, h->les);
+  if (h->st.ibmon != 0)
+    for (int c = 0; c < 3; ++c) launch_ibm(h->s, g, h->sub, h->f, h->ib, c);
+  launch_convbc_mom(h->s, g, h->sub, h->f, b, h->les);
+  launch_adi_mom(h->s, g, h->sub, h->f, 0, h->les);
+  launch_adi_mom(h->s, g, h->sub, h->f, 1, h->les);
+  if (g.n3m != 1) launch_adi_mom(h->s, g, h->sub, h->f, 2, h->les);
+  launch_retrv_mom(h->s, g, h->f);
+}
+
+static void phase_poisson(lw_handle *h) {
+  const Geo &g = h->g;
+  MG &mg = h->mg;
+  launch_fft_forward(h->s, g, mg, h->f);
+  launch_mg_zero(h->s, g, mg);
+  const double test1 = h->st.resid1 * h->sub.dtconsti;
+  const double test = test1 / (double)mg.n3mh * (double)g.n3m * 0.8;
+  unsigned long long m0 = 0, m1 = 0;
+  for (int kv = 1; kv <= mg.n3mh; ++kv) {
+    if (kv <= 64) m0 |= 1ull << (kv - 1);
+    else m1 |= 1ull << (kv - 65);
+  }
+  std::vector<int> cycles(mg.n3mh + 1, 0);
+  auto read_and_mask = [&]() {
+    HIPCHK(hipMemcpyAsync(h->h_sumres, mg.sumres, sizeof(unsigned long long) * mg.n3mh, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    for (int kv = 1; kv <= mg.n3mh; ++kv) {
+      double r;
+      memcpy(&r, &h->h_sumres[kv - 1], 8);
+      const bool act = kv <= 64 ? ((m0 >> (kv - 1)) & 1ull) : ((m1 >> (kv - 65)) & 1ull);
+      if (act && r < test) {
+        if (kv <= 64) m0 &= ~(1ull << (kv - 1));
+        else m1 &= ~(1ull << (kv - 65));
+      }
+    }
+  };
+  launch_mg_toplevel(h->s, g, mg, 0, 0.0, m0, m1);
+  read_and_mask();
+  int kk = 0;
+  while ((m0 | m1) && kk < mg.mgitr) {
+    ++kk;
+    for (int kv = 1; kv <= mg.n3mh; ++kv) {
+      const bool act = kv <= 64 ? ((m0 >> (kv - 1)) & 1ull) : ((m1 >> (kv - 65)) & 1ull);
+      if (act) cycles[kv] = kk;
+    }
+    for (int ilev = mg.nlev - 1; ilev >= 1; --ilev) {
+      launch_mg_relax(h->s, g, mg, ilev, 0.0, 1, m0, m1);
+      launch_mg_godown(h->s, g, mg, ilev, m0, m1);
+    }
+    launch_mg_relax(h->s, g, mg, 0, 0.0, mg.nbli, m0, m1);
+    for (int ilev = 0; ilev <= mg.nlev - 2; ++ilev) {
+      launch_mg_goup(h->s, g, mg, ilev, m0, m1);
+      launch_mg_relax(h->s, g, mg, ilev + 1, 1.0, 1, m0, m1);
+    }
+    launch_mg_toplevel(h->s, g, mg, 1, 1.0, m0, m1);
+    read_and_mask();
+  }
+  if (m0 | m1) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
+  int cmax = 0;
+  for (int kv = 1; kv <= mg.n3mh; ++kv) {
+    cmax = cycles[kv] > cmax ? cycles[kv] : cmax;
+    h->cyc_sum += cycles[kv];
+  }
+  if (cmax > h->cyc_max) h->cyc_max = cmax;
+  launch_fft_inverse(h->s, g, mg, h->f);
+}
+
+static void phase_energy(lw_handle *h) {
+  if (!h->heat) return;
+  const Geo &g = h->g;
+  Body b{};
+  launch_rhs_t(h->s, g, h->sub, h->f, h->les);
+  if (h->st.ibmon != 0 && h->st.iconjg == 0) launch_ibm(h->s, g, h->sub, h->f, h->ib, 3);
+  launch_convbc_t(h->s, g, h->sub, h->f, b, h->les, h->st);
+  launch_adi_t(h->s, g, h->sub, h->f, h->les);
+  launch_energy_bcs(h->s, g, h->f, b, h->st);
+}
+
+static void end_of_step(lw_handle *h) {
+  h->time = h->time + h->dt;
+  launch_diag(h->s, h->g, h->f, (int)(h->st.masson == 1 && h->st.ibmon == 1));
+}
+
+static void read_stats(lw_handle *h, lw_step_stats *stats) {
+  HIPCHK(hipMemcpyAsync(h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(h->h_red, h->f.red, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  double c[3];
+  memcpy(c, h->h_sumres + 8, 3 * 8);
+  h->cflmax = c[0];
+  h->dvmax = c[1];
+  h->qmmax = c[2];
+  if (h->les && h->st.idvmon == 1) {
+    h->csgsts = h->h_red[4];
+    if (h->heat) h->csgshf = h->h_red[5];
+  }
+  if (!stats) return;
+  stats->dt = h->dt;
+  stats->time = h->time;
+  stats->cflmax = h->cflmax;
+  stats->dvmax = h->dvmax;
+  stats->qmmax = h->qmmax;
+  stats->csgsts = h->csgsts;
+  stats->csgshf = h->csgshf;
+  stats->ntime = h->ntime;
+  stats->mg_cycles_max = h->cyc_max;
+  stats->mg_cycles_sum = h->cyc_sum;
+  stats->ms_sgs = h->ms[0];
+  stats->ms_momentum = h->ms[1];
+  stats->ms_poisson = h->ms[2];
+  stats->ms_energy = h->ms[3];
+}
+
+extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
+  if (!h || nsteps < 0) return fail(LW_ERR_ARG, "bad argument");
+  try {
+    for (int64_t n = 0; n < nsteps; ++n) {
+      h->cyc_max = 0;
+      h->cyc_sum = 0;
+      stepinit(h);
+      for (int ms = 1; ms <= 3; ++ms) {
+        substepinit(h, ms);
+        phase_sgs(h);
+        phase_rhsnlhs(h);
+        launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
+        phase_poisson(h);
+        launch_projection(h->s, h->g, h->sub, h->f);
+        if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
+        phase_energy(h);
+      }
+      end_of_step(h);
+    }
+    read_stats(h, stats);
+    if (!(h->dvmax == h->dvmax)) return fail(LW_ERR_NUMERIC, "non-finite divergence after the step");
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+extern "C" int lw_phase(lw_handle *h, int phase) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  try {
+    if (phase == 0) {
+      h->cyc_max = 0;
+      h->cyc_sum = 0;
+      stepinit(h);
+    } else if (phase >= 11 && phase <= 13) substepinit(h, phase - 10);
+    else if (phase == 1) phase_sgs(h);
+    else if (phase == 2) phase_rhsnlhs(h);
+    else if (phase == 3) launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
+    else if (phase == 4) phase_poisson(h);
+    else if (phase == 5) {
+      launch_projection(h->s, h->g, h->sub, h->f);
+      if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
+    } else if (phase == 6) phase_energy(h);
+    else if (phase == 7) {
+      end_of_step(h);
+      read_stats(h, nullptr);
+    } else return fail(LW_ERR_ARG, "unknown phase");
+    HIPCHK(hipStreamSynchronize(h->s));
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+extern "C" int lw_write_field(lw_handle *h, const char *path) {
+  if (!h || !path) return fail(LW_ERR_ARG, "null argument");
+  const size_t np = (size_t)h->g.NX * h->g.NY * h->g.NZ;
+  std::vector<double> u(np), v(np), w(np), p(np), t(h->heat ? np : 0);
+  int rc = lw_get_fields(h, u.data(), v.data(), w.data(), p.data(), h->heat ? t.data() : nullptr);
+  if (rc) return rc;
+  const lw_settings &s = h->st;
+  const int64_t n[3] = {h->g.n1, h->g.n2, h->g.n3};
+  const int64_t prdic[3] = {s.xprdic, s.yprdic, s.zprdic};
+  const int64_t bcs[6] = {s.xprdic ? 4 : 2, s.xprdic ? 4 : 3, s.bc_ybtm, s.bc_ytop, s.zprdic ? 4 : s.bc_zbtm,
+                          s.zprdic ? 4 : s.bc_ztop};
+  const int64_t ich[2] = {s.ich, s.iconjg};
+  return lw_write_fld(path, n, s.re, s.pr, s.gr, h->ihist, h->m, h->time, h->dt, prdic, bcs, ich, u.data(), v.data(),
+                      w.data(), p.data(), h->heat ? t.data() : nullptr);
+}

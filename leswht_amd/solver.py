@@ -1,0 +1,258 @@
+"""Python host of the MI355X time step: a ctypes binding of the C ABI in
+``include/leswht.h`` (library ``leswht_amd/_lib/libleswht.so``).
+
+The reference's operator API is the sequence of Fortran calls in
+``02_solver/f90/solver.f90``; :class:`Solver` mirrors it: construction does
+readsettings .. poisinit (``solver.f90:28-61``), :meth:`Solver.step` runs
+passes of the time loop (``solver.f90:85-181``) and :meth:`Solver.write_field`
+is writefield (``misc_hist.f90:212-252``). There is no CPU fallback: without
+the HIP library or a GPU every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libleswht.so")
+
+_D, _I = C.c_double, C.c_int64
+_PD, _PI = C.POINTER(C.c_double), C.POINTER(C.c_int64)
+
+
+class LwError(RuntimeError):
+    pass
+
+
+class lw_settings(C.Structure):
+    _fields_ = [(n, _D) for n in (
+        "re", "pr", "gr", "tflu_i", "tsol_i", "eps_ptr", "udrv_i", "dt_size", "cflfac",
+        "resid1", "wwsor", "csgsts", "csgshf", "tend", "ptb_tst", "avg_tst")] + \
+        [(n, _I) for n in (
+            "grdir", "ireset", "iread", "iavg", "ipzero", "nend", "nprint", "npriavg", "npin",
+            "idtopt", "nlev", "nbli", "ioldv", "mgitr", "imgsor", "iles", "insmdl", "itemdl",
+            "idvmon", "filter", "ibmon", "masson", "imovingon", "ihtrans", "idecomp",
+            "bc_ybtm", "bc_ytop", "bc_zbtm", "bc_ztop", "bc_t_ybtm", "bc_t_ytop", "bc_t_zbtm",
+            "bc_t_ztop", "ich", "iconjg", "xprdic", "yprdic", "zprdic", "iintp")] + \
+        [(n, _D) for n in ("val_t_ybtm", "val_t_ytop", "val_t_zbtm", "val_t_ztop")]
+
+
+GRID_D = ("x", "y", "z")
+GRID_I = ("ipv", "jpv", "kpv", "imv", "jmv", "kmv", "fixil", "fixjl", "fixkl",
+          "fixiu", "fixju", "fixku")
+GRID_M = ("c2cx", "c2cy", "c2cz", "c2cxi", "c2cyi", "c2czi", "f2fx", "f2fy", "f2fz",
+          "f2fxi", "f2fyi", "f2fzi", "xmp", "ymp", "zmp")
+
+
+class lw_grid(C.Structure):
+    _fields_ = [(n, _I) for n in ("n1", "n2", "n3", "n1m", "n2m", "n3m")] + \
+        [(n, _D) for n in ("xl", "yl", "zl")] + \
+        [(n, _PD) for n in GRID_D] + [(n, _PI) for n in GRID_I] + [(n, _PD) for n in GRID_M]
+
+
+class lw_tables(C.Structure):
+    _fields_ = [("nintp", _I * 4), ("nbody", _I * 4), ("ifc", _PI * 4), ("jfc", _PI * 4),
+                ("kfc", _PI * 4), ("intpindx", _PI * 4), ("geomfac", _PD * 4),
+                ("nzero", _I), ("inz", _PI), ("jnz", _PI), ("knz", _PI), ("nwall_dvm", _PI)]
+
+
+class lw_step_stats(C.Structure):
+    _fields_ = [(n, _D) for n in ("dt", "time", "cflmax", "dvmax", "qmmax", "csgsts", "csgshf")] + \
+        [(n, _I) for n in ("ntime", "mg_cycles_max", "mg_cycles_sum")] + \
+        [(n, _D) for n in ("ms_sgs", "ms_momentum", "ms_poisson", "ms_energy")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class lw_body(C.Structure):
+    _fields_ = [("kind", _I), ("p", _D * 4)]
+
+
+EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw_get_fields",
+           "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
+           "lw_device_bytes", "lw_last_error")
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the HIP library; raises when it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LwError("libleswht.so not built: run `python -m leswht_amd.build` (%s)" % path)
+    L = C.CDLL(path)
+    H = C.c_void_p
+    L.lw_create.argtypes = [C.POINTER(lw_settings), C.POINTER(lw_grid), C.POINTER(lw_tables),
+                            C.POINTER(lw_body), C.POINTER(H)]
+    L.lw_create_from_dir.argtypes = [C.c_char_p, C.POINTER(lw_body), C.POINTER(H)]
+    L.lw_destroy.argtypes = [H]
+    L.lw_set_fields.argtypes = [H, _PD, _PD, _PD, _PD, _PD, _I, _I, _D, _D]
+    L.lw_get_fields.argtypes = [H, _PD, _PD, _PD, _PD, _PD]
+    L.lw_step.argtypes = [H, _I, C.POINTER(lw_step_stats)]
+    L.lw_write_field.argtypes = [H, C.c_char_p]
+    L.lw_get_dims.argtypes = [H, _PI]
+    L.lw_get_array.argtypes = [H, C.c_char_p, _PD, _I]
+    L.lw_get_array.restype = _I
+    L.lw_phase.argtypes = [H, C.c_int]
+    L.lw_device_bytes.argtypes = [H]
+    L.lw_device_bytes.restype = _I
+    L.lw_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().lw_last_error().decode(errors="replace")
+        raise LwError("leswht error %d: %s" % (rc, msg))
+
+
+def _pd(a):
+    return a.ctypes.data_as(_PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_PI)
+
+
+def _at(arr, lo, dtype):
+    out = np.zeros(len(arr) + lo + 1, dtype=dtype)
+    out[lo:lo + len(arr)] = arr
+    return out
+
+
+def settings_struct(s) -> lw_settings:
+    st = lw_settings()
+    for name, _ in lw_settings._fields_:
+        setattr(st, name, getattr(s, {"imovingon": "imoving"}.get(name, name)))
+    return st
+
+
+class Solver:
+    """One case resident in HBM."""
+
+    def __init__(self, handle, heat: bool, dims, keep=None):
+        self._h = handle
+        self.heat = heat
+        self.n1, self.n2, self.n3 = dims
+        self._keep = keep
+
+    # -- construction ------------------------------------------------------
+    @classmethod
+    def from_case(cls, cd: dict, radius: float = 0.5) -> "Solver":
+        """From :func:`leswht_amd.case.load_case_dir` output (arrays as the
+        reference holds them after start-up)."""
+        L = load_library()
+        s, g = cd["settings"], cd["grid"]
+        keep = []
+        st = settings_struct(s)
+        gr = lw_grid()
+        for n in ("n1", "n2", "n3", "n1m", "n2m", "n3m"):
+            setattr(gr, n, g[n])
+        gr.xl, gr.yl, gr.zl = g["xl"], g["yl"], g["zl"]
+        for n in GRID_D:
+            a = _at(g[n], 1, np.float64); keep.append(a); setattr(gr, n, _pd(a))
+        for n in GRID_I:
+            a = _at(g[n], 1, np.int64); keep.append(a); setattr(gr, n, _pi(a))
+        for n in GRID_M:
+            a = np.ascontiguousarray(g[n], dtype=np.float64); keep.append(a); setattr(gr, n, _pd(a))
+        tb = lw_tables()
+        for ci, c in enumerate(("u", "v", "w", "t")):
+            if c not in cd["tables"]:
+                continue
+            t = cd["tables"][c]
+            tb.nintp[ci], tb.nbody[ci] = t["nintp"], t["nbody"]
+            fcp = np.ascontiguousarray(t["fcp"].reshape(-1, 3), dtype=np.int64)
+            cols = [np.ascontiguousarray(fcp[:, q]) for q in range(3)]
+            ix = np.ascontiguousarray(t["indx"].reshape(-1, 3), dtype=np.int64)
+            gf = np.ascontiguousarray(t["geomfac"].reshape(-1, 27), dtype=np.float64)
+            keep += cols + [ix, gf]
+            tb.ifc[ci], tb.jfc[ci], tb.kfc[ci] = (_pi(a) for a in cols)
+            tb.intpindx[ci], tb.geomfac[ci] = _pi(ix), _pd(gf)
+        nz = cd["nutzero"]
+        tb.nzero = nz["nzero"]
+        for n in ("inz", "jnz", "knz"):
+            a = np.ascontiguousarray(nz[n], dtype=np.int64); keep.append(a); setattr(tb, n, _pi(a))
+        nw = np.asfortranarray(nz["nwall"], dtype=np.int64); keep.append(nw)
+        tb.nwall_dvm = _pi(nw)
+        body = lw_body(0, (_D * 4)(radius, 0, 0, 0))
+        h = C.c_void_p()
+        _check(L.lw_create(C.byref(st), C.byref(gr), C.byref(tb), C.byref(body), C.byref(h)))
+        self = cls(h, bool(s.ihtrans), (g["n1"], g["n2"], g["n3"]))
+        f = cd["fields"]
+        self.set_fields(f["u"], f["v"], f["w"], f["p"], f.get("t"), cd["ihist"], cd["m"], cd["time"], cd["dt"])
+        return self
+
+    @classmethod
+    def from_dir(cls, solver_dir: str, radius: float = 0.5) -> "Solver":
+        """Reads a reference case directory (cwd of solver_exec)."""
+        L = load_library()
+        body = lw_body(0, (_D * 4)(radius, 0, 0, 0))
+        h = C.c_void_p()
+        _check(L.lw_create_from_dir(solver_dir.encode(), C.byref(body), C.byref(h)))
+        dims = (_I * 3)()
+        _check(L.lw_get_dims(h, C.cast(dims, _PI)))
+        from . import case as _case  # settings only, to know whether T is carried
+        s = _case.parse_settings(open(os.path.join(solver_dir, "settings.input")).read(),
+                                 open(os.path.join(solver_dir, "boundary.input")).read(),
+                                 open(os.path.join(solver_dir, "../output/ibmpre/ibmpre_prdic.bin")).read())
+        return cls(h, bool(s.ihtrans), tuple(dims))
+
+    # -- state -------------------------------------------------------------
+    def shape(self):
+        return (self.n1 + 1, self.n2 + 1, self.n3 + 1)
+
+    def set_fields(self, u, v, w, p, t=None, ihist=0, m=0, time=0.0, dt=0.0):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        tt = np.asfortranarray(t, dtype=np.float64) if t is not None else None
+        _check(load_library().lw_set_fields(self._h, *[_pd(a) for a in arrs],
+                                            _pd(tt) if tt is not None else None, ihist, m, time, dt))
+
+    def fields(self) -> dict:
+        sh = self.shape()
+        out = {k: np.zeros(sh, order="F") for k in ("u", "v", "w", "p", "t")}
+        _check(load_library().lw_get_fields(self._h, *[_pd(out[k]) for k in ("u", "v", "w", "p", "t")]))
+        if not self.heat:
+            out["t"] = None
+        return out
+
+    def array(self, name: str) -> np.ndarray:
+        L = load_library()
+        n = L.lw_get_array(self._h, name.encode(), None, 0)
+        if n < 0:
+            _check(int(n))
+        out = np.zeros(n)
+        _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
+        return out.reshape(self.shape(), order="F")
+
+    # -- time stepping -----------------------------------------------------
+    def step(self, n: int = 1) -> dict:
+        st = lw_step_stats()
+        _check(load_library().lw_step(self._h, n, C.byref(st)))
+        return st.as_dict()
+
+    def phase(self, ph: int):
+        _check(load_library().lw_phase(self._h, ph))
+
+    def write_field(self, path: str):
+        _check(load_library().lw_write_field(self._h, path.encode()))
+
+    def device_bytes(self) -> int:
+        return int(load_library().lw_device_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            load_library().lw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
