@@ -1,0 +1,137 @@
+"""Benchmark of the LESwHT time step on one MI355X.
+
+Workload: the 384x256x64 heated-cylinder LES (Re=3900, Pr=0.71, dynamic
+Vreman SGS with a global Germano coefficient, dynamic eddy diffusivity, box
+test filter in x and z, CFL-adaptive dt at CFL 0.5, seeded noise 1e-2 on the
+initial field). A step is one full RK3 time step (three substeps) of
+solver.f90:85-181 on data resident in HBM; the metric is cell-updates per
+second = n1m*n2m*n3m*steps / wall time.
+
+Prints one JSON line. ``roofline`` reports the dominant kernel's achieved HBM
+bandwidth from its algorithmic bytes and its event-timed average duration;
+``cpu_baseline`` times the reference solver (built from its own sources with
+amdflang, oracle/_ref/solver_exec) on the host cores of the same machine for
+two steps of the same case, read from its own per-step timer file.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import statistics
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from leswht_amd import case as C  # noqa: E402
+from leswht_amd.solver import Solver  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_spec(nend: int = 2) -> C.CaseSpec:
+    s = C.Settings(re=3900.0, pr=0.71, iles=1, insmdl=1, itemdl=0, idvmon=1, filter=3, idtopt=1,
+                   cflfac=0.5, dt_size=0.005, resid1=1.0e-6, nlev=5, nbli=3, mgitr=500, imgsor=170,
+                   nend=nend)
+    return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s)
+
+
+def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
+    """Reference solver on the host: two steps of the same case; per-step wall
+    time from its ftime.dat column 5 (misc_hist.f90:255-284)."""
+    import oracle  # checker-side: the reference binary lives under oracle/_ref
+
+    root = tempfile.mkdtemp(prefix="lw_cpu_")
+    try:
+        C.write_case(case, root)
+        t0 = time.time()
+        oracle.run_reference(root, threads=threads, timeout=1500)
+        wall = time.time() - t0
+        rows = [ln.split() for ln in open(os.path.join(root, "output/ftr/ftime.dat")) if ln.strip()]
+        step_s = [float(r[4]) for r in rows]
+        med = statistics.median(step_s)
+        n = spec.n1m * spec.n2m * spec.n3m
+        return {"value": n / med, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                "sample": "reference solver (amdflang -O3, OpenMP %d threads), same 384x256x64 LES case, "
+                          "%d RK3 steps, median of its own per-step timer (ftime.dat); process wall %.1f s "
+                          "incl. reading the text inputs" % (threads, len(step_s), wall),
+                "s_per_step": med}
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("LESWHT_CPU_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU time step "
+              "(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    spec = c2_spec()
+    t0 = time.time()
+    case = C.build_case(spec)
+    view = C.case_view(case)
+    t_build = time.time() - t0
+    sol = Solver.from_case(view)
+    sol.step(args.warmup)
+    sol.profile_reset()
+    t0 = time.perf_counter()
+    st = sol.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    prof = sol.profile()
+    n = spec.n1m * spec.n2m * spec.n3m
+    value = n * args.steps / elapsed
+    roof = None
+    if prof:
+        dom = max(prof, key=lambda r: r["ms_total"])
+        if dom["count"] and dom["bytes"]:
+            avg_s = dom["ms_total"] / dom["count"] / 1e3
+            ach = dom["bytes"] / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                    "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": dom["bytes"]}
+    out = {
+        "metric": "cell-updates/s per RK3 step (1 GPU)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded initial field, generated grid and IB tables)",
+        "config": {"workload": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64, dynamic Vreman "
+                               "+ dynamic eddy diffusivity, box test filter x/z, CFL 0.5",
+                   "cells": n, "nlev": spec.settings.nlev, "resid1": spec.settings.resid1,
+                   "mg_cycles_max_last_step": st["mg_cycles_max"], "dt_last": st["dt"],
+                   "case_build_s": round(t_build, 2)},
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            cb = cpu_baseline(spec, case, args.cpu_threads)
+            out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            out["speedup_vs_cpu_baseline"] = value / cb["value"]
+        except Exception as e:  # baseline is reported, never required
+            out["cpu_baseline"] = {"error": str(e)[:300]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -591,6 +591,11 @@ class Settings:
     val_t_ztop: float = 0.0
     ich: int = 0
     iconjg: int = 0
+    # ibmpre_prdic.bin
+    xprdic: int = 0
+    yprdic: int = 0
+    zprdic: int = 1
+    iintp: int = 1
 
     def settings_text(self) -> str:
         g = lambda v: repr(float(v))  # noqa: E731  full-precision reals
@@ -1026,3 +1031,38 @@ def apply_prefld(fl: dict, s: Settings, g: dict) -> dict:
         if t is not None:
             t[:, :, 0] = t[:, :, n3m]; t[:, :, n3] = t[:, :, 1]
     return dict(u=u, v=v, w=w, p=p, t=t)
+
+
+def case_view(case: Case) -> dict:
+    """The same dictionary :func:`load_case_dir` returns for the directory
+    :func:`write_case` would write, built in memory (the values pass through
+    the same text formatting, so they are bit-identical)."""
+    s = case.spec.settings
+    g = solver_metrics(case)
+    tabs = {}
+    comps = ("u", "v", "w") + (("t",) if s.ihtrans else ())
+    for c in comps:
+        t = case.tables
+        tabs[c] = dict(nintp=t.nintp[c], nbody=t.nintp[c] + t.ninner[c], fcp=t.fcp[c],
+                       indx=t.indx[c], geomfac=t.geomfac[c])
+    nz = dict(nzero=0, inz=np.zeros(0, np.int64), jnz=np.zeros(0, np.int64), knz=np.zeros(0, np.int64),
+              nwall=np.ones((g["n1m"], g["n2m"], g["n3m"]), np.int64))
+    if s.iles:
+        ii, jj, kk = np.nonzero(case.tables.iszero == 0)
+        nz.update(nzero=case.tables.nzero, inz=ii.astype(np.int64), jnz=jj.astype(np.int64),
+                  knz=kk.astype(np.int64), nwall=case.tables.iszero.copy())
+    n1, n2, n3 = g["n1"], g["n2"], g["n3"]
+    fl = dict(prdic=(0, 0, 1))
+    f = case.fields
+    u = f["u"].copy(); u[0] = 0.0
+    v = f["v"].copy(); v[:, 0, :] = 0.0
+    w = f["w"].copy(); w[:, :, 0] = 0.0
+    p = np.zeros_like(u)
+    p[1:n1, 1:n2, 1:n3] = f["p"][1:n1, 1:n2, 1:n3]
+    fl.update(u=u, v=v, w=w, p=p)
+    if s.ihtrans:
+        fl["t"] = f["t"]
+    st = dataclasses.replace(s, xprdic=0, yprdic=0, zprdic=1, iintp=1)
+    fields = apply_prefld(fl, st, g)
+    return dict(settings=st, grid=g, tables=tabs, nutzero=nz, fields=fields,
+                ihist=0, m=0, time=0.0, dt=s.dt_size)

@@ -243,6 +243,26 @@ class Solver:
     def write_field(self, path: str):
         _check(load_library().lw_write_field(self._h, path.encode()))
 
+    def profile_reset(self):
+        """Clears the per-kernel event timers (see :meth:`profile`)."""
+        L = load_library()
+        if hasattr(L, "lw_profile_reset"):
+            L.lw_profile_reset(self._h)
+
+    def profile(self) -> list:
+        """Per-kernel event timing since :meth:`profile_reset`: dicts with
+        name, count, ms_total and algorithmic bytes per launch."""
+        L = load_library()
+        if not hasattr(L, "lw_profile_count"):
+            return []
+        out = []
+        name = C.create_string_buffer(64)
+        cnt, ms, by = _I(), _D(), _D()
+        for q in range(L.lw_profile_count(self._h)):
+            L.lw_profile_get(self._h, q, name, 64, C.byref(cnt), C.byref(ms), C.byref(by))
+            out.append(dict(name=name.value.decode(), count=cnt.value, ms_total=ms.value, bytes=by.value))
+        return out
+
     def device_bytes(self) -> int:
         return int(load_library().lw_device_bytes(self._h))
 

@@ -1,0 +1,99 @@
+"""The oracle (CPU restatement) pinned against the reference's own outputs
+(golden fixtures written by the amdflang-built reference solver) and against
+closed forms and library routines for its building blocks."""
+import numpy as np
+import pytest
+
+from leswht_amd import case as C
+from tests.helpers import fld_regions, golden, golden_names, rel, spec_for
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_matches_reference_golden(name, oracle_lib):
+    import oracle
+    z, meta = golden(name)
+    spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
+    view = C.case_view(C.build_case(spec))
+    o = oracle.Oracle(view)
+    o.step(meta["nend"])
+    f = o.fields()
+    n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
+    for k, sl in fld_regions(n).items():
+        err = rel(z[k][sl], f[k][sl])
+        # the oracle restates the reference operation by operation; only libm
+        # calls and the FFT's radix split differ
+        assert err <= 1e-11, (name, k, err)
+
+
+def test_case_inputs_unchanged_since_fixtures():
+    """The case builder regenerates bit-identical inputs for the fixtures."""
+    import hashlib
+    import os
+    import tempfile
+    for name in golden_names():
+        z, meta = golden(name)
+        spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
+        root = tempfile.mkdtemp()
+        C.write_case(C.build_case(spec), root)
+        h = hashlib.sha256()
+        for rel_ in ("output/grid/grid.bin", "output/ibmpre/ibmpre_fcpts.bin", "output/ibmpre/ibmpre_fcpts_t.bin",
+                     "output/ibmpre/ibmpre_nutzero.bin", "02_solver/settings.input", "output/field/fld000000"):
+            p = os.path.join(root, rel_)
+            if os.path.exists(p):
+                h.update(open(p, "rb").read())
+        assert h.hexdigest() == meta["input_sha256"], name
+
+
+def _pd(a):
+    import ctypes
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@pytest.mark.parametrize("n", [2, 8, 16, 64, 12])
+def test_fft_conventions(n, oracle_lib):
+    """FFTE conventions (zfft1d.f:41-83): forward unnormalised e^{-2 pi i jk/n},
+    inverse = conj(FFT(conj(X))) / n; checked against numpy.fft."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    re, im = x.real.copy(), x.imag.copy()
+    oracle_lib.ora_fft_forward(n, _pd(re), _pd(im))
+    np.testing.assert_allclose(re + 1j * im, np.fft.fft(x), rtol=0, atol=1e-12 * n)
+    oracle_lib.ora_fft_inverse(n, _pd(re), _pd(im))
+    np.testing.assert_allclose(re + 1j * im, x, rtol=0, atol=1e-13 * n)
+
+
+def test_thomas_against_banded_solve(oracle_lib):
+    from scipy.linalg import solve_banded
+    rng = np.random.default_rng(1)
+    n = 37
+    a = rng.uniform(-1, 0, n); c = rng.uniform(-1, 0, n); b = 2.5 + rng.uniform(0, 1, n)
+    r = rng.standard_normal(n)
+    ab = np.zeros((3, n)); ab[0, 1:] = c[:-1]; ab[1] = b; ab[2, :-1] = a[1:]
+    want = solve_banded((1, 1), ab, r)
+    got = r.copy()
+    oracle_lib.ora_thomas(n, _pd(a), _pd(b), _pd(c), _pd(got))
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14)
+
+
+def test_periodic_thomas_against_dense_solve(oracle_lib):
+    rng = np.random.default_rng(2)
+    n = 16
+    a = rng.uniform(-1, 0, n); c = rng.uniform(-1, 0, n); b = 2.5 + rng.uniform(0, 1, n)
+    r = rng.standard_normal(n)
+    A = np.diag(b)
+    for j in range(n):
+        A[j, (j - 1) % n] += a[j]
+        A[j, (j + 1) % n] += c[j]
+    want = np.linalg.solve(A, r)
+    got = r.copy()
+    oracle_lib.ora_thomas_periodic(n, _pd(a), _pd(b), _pd(c), _pd(got))
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13)
+
+
+def test_oracle_rejects_out_of_scope(oracle_lib):
+    import oracle
+    spec = spec_for((32, 32, 8), "nosgs")
+    view = C.case_view(C.build_case(spec))
+    view["settings"].ich = 1
+    with pytest.raises(ValueError):
+        oracle.Oracle(view)
