@@ -62,18 +62,19 @@ struct IBComp {
 
 struct MG {
   int nlev, n1md, n3mh, nbli, mgitr;
-  long LD;       // leading dimension of MG planes: n1md + 1
-  long PLANE;    // (n1md + 1) * (n2 + 1)
-  long PCLD;     // n1 + 1
-  long PCPLANE;  // (n1 + 1) * (n2 + 1)
+  long LDJ;      // row length of every MG plane: n2 + 1 rounded up to 8
+  long PLANE;    // (n1md + 1) * LDJ, one wavenumber of the level stack
+  long PCPLANE;  // (n1 + 1) * LDJ, one wavenumber of the top level only
   int iimg[21][4];
   // coefficients
   double *aw, *ae, *as, *an, *coi1, *coi2, *cor1, *cor2;
   int *ipm, *imm;
-  double *ac, *gam, *bet;  // [kv][j][i] with LD, PLANE (real)
-  double2 *gg, *resd;      // [kv][j][i]
-  double2 *pc, *rhs;       // [kv][j][i] with PCLD, PCPLANE
-  unsigned long long *sumres;  // n3mh, max-norm bit patterns
+  double *ac, *gam, *bet;  // [kv][i][j], real
+  double2 *gg, *resd;      // [kv][i][j], all levels stacked along i
+  double2 *pc, *rhs;       // [kv][i][j], top level
+  unsigned long long *sumres;  // n3mh max-norm bit patterns, then tt (2 doubles)
+  int *done;               // n3mh: 0 iterating, 1 converged, 2 MGITR reached
+  int *cyc;                // n3mh: V-cycles started
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 

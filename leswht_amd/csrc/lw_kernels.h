@@ -62,14 +62,10 @@ void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, do
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
-// toplevel (poiss_z_ft.f90:302-380); id 0 or 1
-void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv,
-                        unsigned long long m0, unsigned long long m1);
-void launch_mg_relax(hipStream_t s, const Geo &g, const MG &mg, int ilev, double oldv,
-                     int iiter, unsigned long long m0, unsigned long long m1);
-void launch_mg_godown(hipStream_t s, const Geo &g, const MG &mg, int ilev,
-                      unsigned long long m0, unsigned long long m1);
-void launch_mg_goup(hipStream_t s, const Geo &g, const MG &mg, int ilev,
-                    unsigned long long m0, unsigned long long m1);
+// toplevel (poiss_z_ft.f90:302-380), id 0 or 1, ending with the per-wavenumber
+// convergence test against `test`
+void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test);
+// one V-cycle of every wavenumber still iterating (poiss_z_ft.f90:258-299)
+void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test);
 
 }  // namespace lw

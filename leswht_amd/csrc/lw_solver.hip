@@ -61,6 +61,8 @@ struct lw_handle {
   hipEvent_t ev[8];
   double ms[4] = {0, 0, 0, 0};
   unsigned long long *h_sumres = nullptr;  // pinned
+  int *h_flags = nullptr;                  // pinned copy of mg.done / mg.cyc
+  int mg_hint = 4;                         // V-cycles of the previous Poisson solve
   double *h_red = nullptr;                 // pinned
 
   template <class T>
@@ -275,18 +277,17 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.n3mh = n3m / 2 + 1;
   mg.nbli = (int)st.nbli;
   mg.mgitr = (int)st.mgitr;
-  mg.LD = mg.n1md + 1;
-  mg.PLANE = mg.LD * (n2 + 1);
-  mg.PCLD = n1 + 1;
-  mg.PCPLANE = mg.PCLD * (n2 + 1);
+  mg.LDJ = ((long)(n2 + 1) + 7) / 8 * 8;
+  mg.PLANE = (long)(mg.n1md + 1) * mg.LDJ;
+  mg.PCPLANE = (long)(n1 + 1) * mg.LDJ;
   const int n1md = mg.n1md, n3mh = mg.n3mh;
   std::vector<double> aw(n1md + 2), ae(n1md + 2), as(n2 + 2), an(n2 + 2), coi1(n1md + 2), coi2(n1md + 2),
       cor1(n1md + 2), cor2(n1md + 2), ak3(n3mh + 2);
   std::vector<int> ipm(n1md + 2), imm(n1md + 2);
   std::vector<double> ac((size_t)n3mh * mg.PLANE, 0.0), gam(ac.size(), 0.0), bet(ac.size(), 0.0);
-  auto AC = [&](int i, int j, int k) -> double & { return ac[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
-  auto GAMh = [&](int i, int j, int k) -> double & { return gam[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
-  auto BETh = [&](int i, int j, int k) -> double & { return bet[(size_t)(k - 1) * mg.PLANE + (size_t)j * mg.LD + i]; };
+  auto AC = [&](int i, int j, int k) -> double & { return ac[(size_t)(k - 1) * mg.PLANE + (size_t)i * mg.LDJ + j]; };
+  auto GAMh = [&](int i, int j, int k) -> double & { return gam[(size_t)(k - 1) * mg.PLANE + (size_t)i * mg.LDJ + j]; };
+  auto BETh = [&](int i, int j, int k) -> double & { return bet[(size_t)(k - 1) * mg.PLANE + (size_t)i * mg.LDJ + j]; };
   for (int i = 1; i <= n1m; ++i) {
     const int ip = ipv[i];
     aw[i] = (1.0 - fixil[i]) * c2cxi[i] * f2fxi[i];
@@ -388,6 +389,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);
+  mg.done = h->dalloc<int>(256);
+  mg.cyc = mg.done + 128;
   std::vector<double> twr(n3m), twi(n3m);
   const double pi2 = 8.0 * atan(1.0);
   for (int q = 0; q < n3m; ++q) {
@@ -453,6 +456,7 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     HIPCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
     setup(h, gr, t);
     // dttimeinit / rk3coefinit (misc_init.f90:1001-1044)
@@ -475,6 +479,7 @@ int lw_destroy(lw_handle *h) {
   for (int e = 0; e < 8; ++e)
     if (h->ev[e]) hipEventDestroy(h->ev[e]);
   if (h->h_sumres) hipHostFree(h->h_sumres);
+  if (h->h_flags) hipHostFree(h->h_flags);
   if (h->h_red) hipHostFree(h->h_red);
   if (h->s) hipStreamDestroy(h->s);
   delete h;
@@ -654,6 +659,11 @@ static void phase_rhsnlhs(lw_handle *h) {
   launch_retrv_mom(h->s, g, h->f);
 }
 
+// poisson (poiss_z_ft.f90:114-299): z-DFT, then V-cycles of every wavenumber
+// until its max-norm residual passes the test. Convergence is decided on the
+// device; the host enqueues batches of cycles (the first as long as the
+// previous solve needed) and reads the flags between batches. Cycles enqueued
+// after a wavenumber converged leave it untouched.
 static void phase_poisson(lw_handle *h) {
   const Geo &g = h->g;
   MG &mg = h->mg;
@@ -661,53 +671,34 @@ static void phase_poisson(lw_handle *h) {
   launch_mg_zero(h->s, g, mg);
   const double test1 = h->st.resid1 * h->sub.dtconsti;
   const double test = test1 / (double)mg.n3mh * (double)g.n3m * 0.8;
-  unsigned long long m0 = 0, m1 = 0;
-  for (int kv = 1; kv <= mg.n3mh; ++kv) {
-    if (kv <= 64) m0 |= 1ull << (kv - 1);
-    else m1 |= 1ull << (kv - 65);
-  }
-  std::vector<int> cycles(mg.n3mh + 1, 0);
-  auto read_and_mask = [&]() {
-    HIPCHK(hipMemcpyAsync(h->h_sumres, mg.sumres, sizeof(unsigned long long) * mg.n3mh, hipMemcpyDeviceToHost, h->s));
+  launch_mg_toplevel(h->s, g, mg, 0, 0.0, test);
+  int launched = 0, batch = h->mg_hint > 1 ? h->mg_hint : 1;
+  bool all_done = false;
+  while (!all_done && launched < mg.mgitr) {
+    if (batch > mg.mgitr - launched) batch = mg.mgitr - launched;
+    for (int c = 0; c < batch; ++c) launch_mg_cycle(h->s, g, mg, test);
+    launched += batch;
+    HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
-    for (int kv = 1; kv <= mg.n3mh; ++kv) {
-      double r;
-      memcpy(&r, &h->h_sumres[kv - 1], 8);
-      const bool act = kv <= 64 ? ((m0 >> (kv - 1)) & 1ull) : ((m1 >> (kv - 65)) & 1ull);
-      if (act && r < test) {
-        if (kv <= 64) m0 &= ~(1ull << (kv - 1));
-        else m1 &= ~(1ull << (kv - 65));
-      }
-    }
-  };
-  launch_mg_toplevel(h->s, g, mg, 0, 0.0, m0, m1);
-  read_and_mask();
-  int kk = 0;
-  while ((m0 | m1) && kk < mg.mgitr) {
-    ++kk;
-    for (int kv = 1; kv <= mg.n3mh; ++kv) {
-      const bool act = kv <= 64 ? ((m0 >> (kv - 1)) & 1ull) : ((m1 >> (kv - 65)) & 1ull);
-      if (act) cycles[kv] = kk;
-    }
-    for (int ilev = mg.nlev - 1; ilev >= 1; --ilev) {
-      launch_mg_relax(h->s, g, mg, ilev, 0.0, 1, m0, m1);
-      launch_mg_godown(h->s, g, mg, ilev, m0, m1);
-    }
-    launch_mg_relax(h->s, g, mg, 0, 0.0, mg.nbli, m0, m1);
-    for (int ilev = 0; ilev <= mg.nlev - 2; ++ilev) {
-      launch_mg_goup(h->s, g, mg, ilev, m0, m1);
-      launch_mg_relax(h->s, g, mg, ilev + 1, 1.0, 1, m0, m1);
-    }
-    launch_mg_toplevel(h->s, g, mg, 1, 1.0, m0, m1);
-    read_and_mask();
+    all_done = true;
+    for (int q = 0; q < mg.n3mh; ++q) all_done = all_done && h->h_flags[q] != 0;
+    batch = 2;
   }
-  if (m0 | m1) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
+  if (launched == 0) {  // every wavenumber passed on the first toplevel
+    HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  }
   int cmax = 0;
-  for (int kv = 1; kv <= mg.n3mh; ++kv) {
-    cmax = cycles[kv] > cmax ? cycles[kv] : cmax;
-    h->cyc_sum += cycles[kv];
+  bool limit = false;
+  for (int q = 0; q < mg.n3mh; ++q) {
+    const int c = h->h_flags[128 + q];
+    cmax = c > cmax ? c : cmax;
+    h->cyc_sum += c;
+    limit = limit || h->h_flags[q] != 1;
   }
+  if (limit) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
   if (cmax > h->cyc_max) h->cyc_max = cmax;
+  h->mg_hint = cmax;
   launch_fft_inverse(h->s, g, mg, h->f);
 }
 
