@@ -134,6 +134,20 @@ int lw_phase(lw_handle *h, int phase);
 /* device scratch footprint in bytes */
 int64_t lw_device_bytes(const lw_handle *h);
 
+/* Per-kernel device time. No counterpart in the reference, whose only timer is
+ * the per-step wall clock written to ftime.dat (misc_hist.f90:255-284).
+ * lw_profile_begin starts a fresh profile of the following lw_step / lw_phase
+ * calls: kernels = NULL or "" times every kernel, otherwise a comma-separated
+ * list of the names lw_profile_get reports. Each timed launch is bracketed by
+ * two events on the handle's stream. lw_profile_get returns, per kernel, the
+ * launch count, the summed event time and the summed algorithmic HBM bytes
+ * (each kernel's minimal reads and writes of its arrays). */
+int lw_profile_begin(lw_handle *h, const char *kernels);
+int lw_profile_end(lw_handle *h);
+int64_t lw_profile_count(const lw_handle *h);
+int lw_profile_get(const lw_handle *h, int64_t idx, char *name, int64_t cap,
+                   int64_t *launches, double *ms_total, double *bytes_total);
+
 const char *lw_last_error(void);
 
 #ifdef __cplusplus

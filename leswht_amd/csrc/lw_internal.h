@@ -74,7 +74,7 @@ struct MG {
   double2 *pc, *rhs;       // [kv][i][j], top level
   unsigned long long *sumres;  // n3mh max-norm bit patterns, then tt (2 doubles)
   int *done;               // n3mh: 0 iterating, 1 converged, 2 MGITR reached
-  int *cyc;                // n3mh: V-cycles started
+  int *cyc;                // n3mh: V-cycles completed
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 

@@ -7,6 +7,8 @@
 namespace lw {
 
 #define BLK3 dim3(64, 4, 1)
+// interior points, the unit of the per-launch byte counts below
+static inline double pts(const Geo &g) { return (double)g.n1m * g.n2m * g.n3m; }
 static inline dim3 grid3(int ni, int nj, int nk) {
   return dim3((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)(nk > 0 ? nk : 1));
 }
@@ -211,10 +213,12 @@ __global__ void k_rhs_w(Geo g, Sub sb, Fld f, int les) {
 }
 
 void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
-  hipLaunchKernelGGL(k_rhs_u, grid3(g.n1m - g.ib + 1, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
-  hipLaunchKernelGGL(k_rhs_v, grid3(g.n1m, g.n2m - g.jb + 1, g.n3m), BLK3, 0, s, g, sb, f, les);
+  // reads u v w p rk3o (+ three viscosities with LES), writes rhs rk3o rk3oo
+  const double b = pts(g) * 8.0 * (les ? 11 : 8);
+  LW_LAUNCH("k_rhs_u", b, 0, k_rhs_u, grid3(g.n1m - g.ib + 1, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
+  LW_LAUNCH("k_rhs_v", b, 0, k_rhs_v, grid3(g.n1m, g.n2m - g.jb + 1, g.n3m), BLK3, 0, s, g, sb, f, les);
   if (g.n3m != 1)
-    hipLaunchKernelGGL(k_rhs_w, grid3(g.n1m, g.n2m, g.n3m - g.kb + 1), BLK3, 0, s, g, sb, f, les);
+    LW_LAUNCH("k_rhs_w", b, 0, k_rhs_w, grid3(g.n1m, g.n2m, g.n3m - g.kb + 1), BLK3, 0, s, g, sb, f, les);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,63 +276,92 @@ void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp
   const double dti = 1. / sb.dt;
   IBComp &c = ib[comp];
   if (c.nintp > 0)
-    hipLaunchKernelGGL(k_ibm_gather, dim3((c.nintp + 255) / 256), dim3(256), 0, s, g, c, fld, f.rhs[comp]);
+    LW_LAUNCH("k_ibm_gather", c.nintp * (27.0 * 16 + 3 * 4 + 8 + 8), 0, k_ibm_gather, dim3((c.nintp + 255) / 256),
+              dim3(256), 0, s, g, c, fld, f.rhs[comp]);
   if (c.nbody > 0)
-    hipLaunchKernelGGL(k_ibm_scatter, dim3((c.nbody + 255) / 256), dim3(256), 0, s, g, c, fld,
-                       f.rhs[comp], dti, comp < 3 ? 1 : 0);
+    LW_LAUNCH("k_ibm_scatter", c.nbody * (3 * 4 + 8 + 8 + 8 + 8), 0, k_ibm_scatter, dim3((c.nbody + 255) / 256),
+              dim3(256), 0, s, g, c, fld, f.rhs[comp], dti, comp < 3 ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
-// convective outlet, slv_mmtm.f90:873-988 (convbc) + 991-1041 (rhsincorpbc):
-// one workgroup runs the three dependent reductions and the plane updates
+// convective outlet, slv_mmtm.f90:873-988 (convbc) + 991-1041 (rhsincorpbc).
+// Three dependent reductions over boundary planes: each reduction kernel
+// writes kObc block partials to f.obc, and every workgroup of the kernel that
+// needs the total re-sums the partials in one fixed order, so all of them hold
+// bit-identical totals without a one-workgroup launch in between. The
+// reference sums each plane serially; the fixed tree differs from it only by
+// rounding.
 // ---------------------------------------------------------------------------
-constexpr int kBC = 1024;
+constexpr int kObc = 64;  // workgroups of the outlet-plane kernels
+constexpr int kObcT = 256;
+// f.obc layout: [0..3][kObc] flux partials (u inlet, v walls, w ends, u
+// outlet), [4][kObc] outflow partials
 
-__device__ inline double plane_val(const double *a, const Geo &g, int j, int k) { return a[j + g.NY * k]; }
-
-__global__ __launch_bounds__(kBC) void k_convbc_mom(Geo g, Sub sb, Fld f, int les) {
-  __shared__ double sh[kBC / 64];
-  __shared__ double s_bc[4];
-  const int tid = threadIdx.x;
-  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
-  const double *u = f.u, *v = f.v, *w = f.w;
-  // qin: the reference sums each loop serially; here each loop is a fixed
-  // deterministic tree (rounding-level difference only)
-  double a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
-    a += u[P(g, 1, j, k)] * g.f2fy[j] * g.f2fz[k];
-  }
-  double qin = block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n1m * n3m; q += kBC) {
-    const int i = 1 + q % n1m, k = 1 + q / n1m;
-    a += (v[P(g, i, 1, k)] - v[P(g, i, n2, k)]) * g.f2fx[i] * g.f2fz[k];
-  }
-  qin = qin + block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n1m * n2m; q += kBC) {
-    const int i = 1 + q % n1m, j = 1 + q / n1m;
-    a += (w[P(g, i, j, 1)] - w[P(g, i, j, n3)]) * g.f2fx[i] * g.f2fy[j];
-  }
-  qin = qin + block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
-    a += u[P(g, n1, j, k)] * g.f2fy[j] * g.f2fz[k];
-  }
-  double ubar = block_sum<kBC>(a, sh);
-  if (tid == 0) {
-    s_bc[0] = qin;
-    s_bc[1] = ubar / (g.yl * g.zl);
-  }
+__device__ inline double obc_total(const double *part, double *sh) {
+  const double v = threadIdx.x < kObc ? part[threadIdx.x] : 0.0;
+  const double r = block_sum<kObcT>(v, sh);
+  if (threadIdx.x == 0) sh[kObcT / 64] = r;
   __syncthreads();
-  qin = s_bc[0];
-  ubar = s_bc[1];
-  const double ucoef = ubar * sb.dtconst * g.f2fxi[n1m];
-  const double vwcoef = ubar * sb.dtconst * g.c2cxi[n1];
-  a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
+  const double out = sh[kObcT / 64];
+  __syncthreads();
+  return out;
+}
+
+// boundary fluxes: qin = inlet + (bottom - top) + (front - back), and the
+// outlet-plane mean of u
+__global__ __launch_bounds__(kObcT) void k_obc_flux(Geo g, Fld f) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
+  const int str = kObc * kObcT, q0 = blockIdx.x * kObcT + threadIdx.x;
+  const double *u = f.u, *v = f.v, *w = f.w;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int q = q0; q < n2m * n3m; q += str) {
+    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    a0 += u[P(g, 1, j, k)] * g.f2fy[j] * g.f2fz[k];
+    a3 += u[P(g, n1, j, k)] * g.f2fy[j] * g.f2fz[k];
+  }
+  for (int q = q0; q < n1m * n3m; q += str) {
+    const int i = 1 + q % n1m, k = 1 + q / n1m;
+    a1 += (v[P(g, i, 1, k)] - v[P(g, i, n2, k)]) * g.f2fx[i] * g.f2fz[k];
+  }
+  for (int q = q0; q < n1m * n2m; q += str) {
+    const int i = 1 + q % n1m, j = 1 + q / n1m;
+    a2 += (w[P(g, i, j, 1)] - w[P(g, i, j, n3)]) * g.f2fx[i] * g.f2fy[j];
+  }
+  const double s0 = block_sum<kObcT>(a0, sh), s1 = block_sum<kObcT>(a1, sh), s2 = block_sum<kObcT>(a2, sh),
+               s3 = block_sum<kObcT>(a3, sh);
+  if (threadIdx.x == 0) {
+    f.obc[0 * kObc + blockIdx.x] = s0;
+    f.obc[1 * kObc + blockIdx.x] = s1;
+    f.obc[2 * kObc + blockIdx.x] = s2;
+    f.obc[3 * kObc + blockIdx.x] = s3;
+  }
+}
+
+struct ObcIn {
+  double qin, ubar;
+};
+__device__ inline ObcIn obc_in(const Geo &g, const Fld &f, double *sh) {
+  ObcIn r;
+  const double s0 = obc_total(f.obc, sh), s1 = obc_total(f.obc + kObc, sh), s2 = obc_total(f.obc + 2 * kObc, sh),
+               s3 = obc_total(f.obc + 3 * kObc, sh);
+  r.qin = s0;
+  r.qin = r.qin + s1;
+  r.qin = r.qin + s2;
+  r.ubar = s3 / (g.yl * g.zl);
+  return r;
+}
+
+// outlet velocities before the mass correction, and the outflow partials
+__global__ __launch_bounds__(kObcT) void k_obc_out_mom(Geo g, Sub sb, Fld f) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const ObcIn in = obc_in(g, f, sh);
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  const double *u = f.u, *v = f.v, *w = f.w;
+  const double ucoef = in.ubar * sb.dtconst * g.f2fxi[n1m];
+  const double vwcoef = in.ubar * sb.dtconst * g.c2cxi[n1];
+  double a = 0.0;
+  for (int q = blockIdx.x * kObcT + threadIdx.x; q < n2m * n3m; q += kObc * kObcT) {
     const int j = 1 + q % n2m, k = 1 + q / n2m;
     const long pb = j + g.NY * k;
     double uo, vo, wo;
@@ -346,48 +379,70 @@ __global__ __launch_bounds__(kBC) void k_convbc_mom(Geo g, Sub sb, Fld f, int le
     f.wout[pb] = wo;
     a += uo * g.f2fy[j] * g.f2fz[k];
   }
-  const double qout = block_sum<kBC>(a, sh);
-  if (tid == 0) s_bc[2] = qin / qout;
-  __syncthreads();
-  const double qratio = s_bc[2];
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
-    const long pb = j + g.NY * k;
-    if (!(f.mask[P(g, n1m, j, k)] & 64)) {
-      f.uout[pb] = f.uout[pb] * qratio;
-      f.vout[pb] = f.vout[pb] * qratio;
-      f.wout[pb] = f.wout[pb] * qratio;
-    }
-    f.duout[pb] = f.uout[pb] - u[P(g, n1, j, k)];
-    f.dvout[pb] = f.vout[pb] - v[P(g, n1, j, k)];
-    f.dwout[pb] = f.wout[pb] - w[P(g, n1, j, k)];
-  }
-  __syncthreads();
-  for (int j = tid; j <= n2; j += kBC) {  // z periodic ghosts of the deltas
-    f.duout[j + g.NY * n3] = f.duout[j + g.NY * 1];
-    f.duout[j] = f.duout[j + g.NY * n3m];
-    f.dvout[j + g.NY * n3] = f.dvout[j + g.NY * 1];
-    f.dvout[j] = f.dvout[j + g.NY * n3m];
-    f.dwout[j + g.NY * n3] = f.dwout[j + g.NY * 1];
-    f.dwout[j] = f.dwout[j + g.NY * n3m];
-  }
-  __syncthreads();
-  // rhsincorpbc (ICH = 0)
+  const double s = block_sum<kObcT>(a, sh);
+  if (threadIdx.x == 0) f.obc[4 * kObc + blockIdx.x] = s;
+  (void)n3m;
+}
+
+// mass correction qin/qout, the outlet deltas, and rhsincorpbc (ICH = 0)
+__global__ __launch_bounds__(kObcT) void k_obc_apply_mom(Geo g, Sub sb, Fld f, int les) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const ObcIn in = obc_in(g, f, sh);
+  const double qout = obc_total(f.obc + 4 * kObc, sh);
+  const double qratio = in.qin / qout;
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  const double *u = f.u, *v = f.v, *w = f.w;
   const double cre = les ? sb.re : 0., cre2 = les ? 2. * sb.re : 0.;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
+  for (int q = blockIdx.x * kObcT + threadIdx.x; q < n2m * n3m; q += kObc * kObcT) {
     const int j = 1 + q % n2m, k = 1 + q / n2m;
     const long pb = j + g.NY * k;
+    double uo = f.uout[pb], vo = f.vout[pb], wo = f.wout[pb];
+    if (!(f.mask[P(g, n1m, j, k)] & 64)) {
+      uo = uo * qratio;
+      vo = vo * qratio;
+      wo = wo * qratio;
+      f.uout[pb] = uo;
+      f.vout[pb] = vo;
+      f.wout[pb] = wo;
+    }
+    const double du = uo - u[P(g, n1, j, k)], dv = vo - v[P(g, n1, j, k)], dw = wo - w[P(g, n1, j, k)];
+    f.duout[pb] = du;
+    f.dvout[pb] = dv;
+    f.dwout[pb] = dw;
     const long c = P(g, n1m, j, k);
-    f.rhs[0][c] = f.rhs[0][c] - sb.acoef * g.ciu[n1m] * (1. + cre2 * f.nu[c]) * f.duout[pb];
+    f.rhs[0][c] = f.rhs[0][c] - sb.acoef * g.ciu[n1m] * (1. + cre2 * f.nu[c]) * du;
     if (j >= 2)
-      f.rhs[1][c] = f.rhs[1][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[2][P(g, n1, j, k)]) * f.dvout[pb];
+      f.rhs[1][c] = f.rhs[1][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[2][P(g, n1, j, k)]) * dv;
     if (k >= g.kb)
-      f.rhs[2][c] = f.rhs[2][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[1][P(g, n1, j, k)]) * f.dwout[pb];
+      f.rhs[2][c] = f.rhs[2][c] - sb.acoef * g.civw[n1m] * (1. + cre * f.nu1[1][P(g, n1, j, k)]) * dw;
+  }
+  (void)n3m;
+}
+
+// z periodic ghosts of the outlet deltas
+__global__ void k_obc_ghost(Geo g, double *d0, double *d1, double *d2) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j > g.n2) return;
+  double *d[3] = {d0, d1, d2};
+  for (int c = 0; c < 3; ++c) {
+    if (!d[c]) continue;
+    d[c][j + g.NY * g.n3] = d[c][j + g.NY * 1];
+    d[c][j] = d[c][j + g.NY * g.n3m];
   }
 }
 
+static void launch_obc_flux(hipStream_t s, const Geo &g, const Fld &f) {
+  const double b = ((double)g.n2m * g.n3m * 2 + (double)g.n1m * g.n3m * 2 + (double)g.n1m * g.n2m * 2) * 8.0;
+  LW_LAUNCH("k_obc_flux", b, 0, k_obc_flux, dim3(kObc), dim3(kObcT), 0, s, g, f);
+}
+
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &, int les) {
-  hipLaunchKernelGGL(k_convbc_mom, dim3(1), dim3(kBC), 0, s, g, sb, f, les);
+  const double pl = (double)g.n2m * g.n3m;
+  launch_obc_flux(s, g, f);
+  LW_LAUNCH("k_obc_out_mom", pl * 8.0 * 10, 0, k_obc_out_mom, dim3(kObc), dim3(kObcT), 0, s, g, sb, f);
+  LW_LAUNCH("k_obc_apply_mom", pl * 8.0 * 20, 0, k_obc_apply_mom, dim3(kObc), dim3(kObcT), 0, s, g, sb, f, les);
+  LW_LAUNCH("k_obc_ghost", (double)(g.n2 + 1) * 8.0 * 12, 0, k_obc_ghost, dim3((g.n2 + 64) / 64), dim3(64), 0, s, g,
+            f.duout, f.dvout, f.dwout);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,13 +670,20 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   const int ilo = C == 0 ? g.ib : 1;
   const int jlo = C == 1 ? g.jb : 1;
   const int klo = C == 2 ? g.kb : 1;
+  static const char *nz[4] = {"k_adi_z<u>", "k_adi_z<v>", "k_adi_z<w>", "k_adi_z<t>"};
+  static const char *ny[4] = {"k_adi_y<u>", "k_adi_y<v>", "k_adi_y<w>", "k_adi_y<t>"};
+  static const char *nx[4] = {"k_adi_x<u>", "k_adi_x<v>", "k_adi_x<w>", "k_adi_x<t>"};
+  void (*kz)(Geo, Sub, Fld, double, double, int, int) = k_adi_z<C>;
+  void (*ky)(Geo, Sub, Fld, double, double, int, int, int) = k_adi_y<C>;
+  void (*kx)(Geo, Sub, Fld, double, double, int, int, int) = k_adi_x<C>;
+  // z and y sweeps: viscosity and rhs in, rhs out; x sweep: also field in/out
   if (g.n3m != 1)
-    hipLaunchKernelGGL(k_adi_z<C>, grid3(g.n1m - ilo + 1, g.n2m - jlo + 1, 1), BLK3, 0, s, g, sb, f, cre,
-                       cre2, ilo, jlo);
-  hipLaunchKernelGGL(k_adi_y<C>, grid3(g.n1m - ilo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre, cre2,
-                     ilo, jlo, klo);
-  hipLaunchKernelGGL(k_adi_x<C>, grid3(g.n2m - jlo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre, cre2,
-                     ilo, jlo, klo);
+    LW_LAUNCH(nz[C], pts(g) * 24.0, 0, kz, grid3(g.n1m - ilo + 1, g.n2m - jlo + 1, 1), BLK3, 0, s, g, sb, f, cre,
+              cre2, ilo, jlo);
+  LW_LAUNCH(ny[C], pts(g) * 24.0, 0, ky, grid3(g.n1m - ilo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre,
+            cre2, ilo, jlo, klo);
+  LW_LAUNCH(nx[C], pts(g) * 32.0, 0, kx, grid3(g.n2m - jlo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre,
+            cre2, ilo, jlo, klo);
 }
 
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les) {
@@ -667,8 +729,10 @@ __global__ void k_zghost_uvw(Geo g, Fld f) {
 }
 
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f) {
-  hipLaunchKernelGGL(k_retrv_mom, dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);
-  hipLaunchKernelGGL(k_zghost_uvw, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+  LW_LAUNCH("k_retrv_mom", (double)(g.n2 + 1) * (g.n3 + 1) * 8.0 * 9, 0, k_retrv_mom,
+            dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);
+  LW_LAUNCH("k_zghost_uvw", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 12, 0, k_zghost_uvw,
+            dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -731,21 +795,30 @@ __global__ void k_divgs(Geo g, Sub sb, Fld f, int masson) {
 }
 
 void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson) {
-  hipLaunchKernelGGL(k_divgs, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, masson);
+  // u v w in, div out (+ qmass out with the mass source)
+  LW_LAUNCH("k_divgs", pts(g) * 8.0 * (masson ? 5 : 4), 0, k_divgs, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g,
+            sb, f, masson);
 }
 
 // ---------------------------------------------------------------------------
 // ucalc (slv_auxi.f90:233-416) and pcalc (419-493) fused; the outlet-plane
-// mean of phi is reduced first by one workgroup
+// mean of phi is reduced first (block partials into f.obc[5 * kObc..], then
+// one fixed-order sum)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBC) void k_phiref(Geo g, Fld f) {
-  __shared__ double sh[kBC / 64];
+__global__ __launch_bounds__(kObcT) void k_phiref(Geo g, Fld f) {
+  __shared__ double sh[kObcT / 64 + 1];
   double a = 0.0;
-  for (int q = threadIdx.x; q < g.n2m * g.n3m; q += kBC) {
+  for (int q = blockIdx.x * kObcT + threadIdx.x; q < g.n2m * g.n3m; q += kObc * kObcT) {
     const int j = 1 + q % g.n2m, k = 1 + q / g.n2m;
     a += f.phi[P(g, g.n1m, j, k)] * g.f2fy[j] * g.f2fz[k];
   }
-  const double s = block_sum<kBC>(a, sh);
+  const double s = block_sum<kObcT>(a, sh);
+  if (threadIdx.x == 0) f.obc[5 * kObc + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kObcT) void k_phiref_sum(Geo g, Fld f) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const double s = obc_total(f.obc + 5 * kObc, sh);
   if (threadIdx.x == 0) f.red[0] = s / (g.yl * g.zl);
 }
 
@@ -779,12 +852,19 @@ __global__ void k_zghost_p(Geo g, Fld f) {
 }
 
 void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f) {
-  hipLaunchKernelGGL(k_phiref, dim3(1), dim3(kBC), 0, s, g, f);
-  hipLaunchKernelGGL(k_ucalc_pcalc, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f);
-  hipLaunchKernelGGL(k_zghost_uvw, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
-  hipLaunchKernelGGL(k_zghost_p, dim3((g.n1m + 63) / 64, g.n2m), dim3(64), 0, s, g, f);
+  LW_LAUNCH("k_phiref", (double)g.n2m * g.n3m * 8.0, 0, k_phiref, dim3(kObc), dim3(kObcT), 0, s, g, f);
+  LW_LAUNCH("k_phiref_sum", kObc * 8.0, 0, k_phiref_sum, dim3(1), dim3(kObcT), 0, s, g, f);
+  // phi div in, u v w p in and out
+  LW_LAUNCH("k_ucalc_pcalc", pts(g) * 8.0 * 10, 0, k_ucalc_pcalc, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb,
+            f);
+  LW_LAUNCH("k_zghost_uvw", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 12, 0, k_zghost_uvw,
+            dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+  LW_LAUNCH("k_zghost_p", (double)g.n1m * g.n2m * 8.0 * 4, 0, k_zghost_p, dim3((g.n1m + 63) / 64, g.n2m), dim3(64),
+            0, s, g, f);
 }
 
+// ---------------------------------------------------------------------------
+// energy right-hand side, slv_engy.f90:29-198 (ICONJG = 0, IDECOMP = 0)
 // ---------------------------------------------------------------------------
 // energy right-hand side, slv_engy.f90:29-198 (ICONJG = 0, IDECOMP = 0)
 // ---------------------------------------------------------------------------
@@ -840,51 +920,25 @@ __global__ void k_rhs_t(Geo g, Sub sb, Fld f, int les) {
 }
 
 void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
-  hipLaunchKernelGGL(k_rhs_t, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
+  // reads t u v w rk3o (+ four diffusivities with LES), writes rhs rk3o rk3oo
+  LW_LAUNCH("k_rhs_t", pts(g) * 8.0 * (les ? 12 : 8), 0, k_rhs_t, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f,
+            les);
 }
 
-// convbc_t (slv_engy.f90:309-413) + rhsincorpbc_t (416-457), ICONJG = 0
-__global__ __launch_bounds__(kBC) void k_convbc_t(Geo g, Sub sb, Fld f, int les) {
-  __shared__ double sh[kBC / 64];
-  __shared__ double s_bc[4];
-  const int tid = threadIdx.x;
-  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
-  const double *u = f.u, *v = f.v, *w = f.w, *t = f.t;
+
+// ---------------------------------------------------------------------------
+// convbc_t (slv_engy.f90:309-413) + rhsincorpbc_t (416-457), ICONJG = 0,
+// with the plane reductions done as in the momentum outlet
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kObcT) void k_obc_out_t(Geo g, Sub sb, Fld f) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const ObcIn in = obc_in(g, f, sh);
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  const double *u = f.u, *t = f.t;
+  const double ucoef = in.ubar * sb.dtconst * g.f2fxi[n1m];
+  const double vwcoef = in.ubar * sb.dtconst * g.c2cxi[n1];
   double a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
-    a += u[P(g, 1, j, k)] * g.f2fy[j] * g.f2fz[k];
-  }
-  double qin = block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n1m * n3m; q += kBC) {
-    const int i = 1 + q % n1m, k = 1 + q / n1m;
-    a += (v[P(g, i, 1, k)] - v[P(g, i, n2, k)]) * g.f2fx[i] * g.f2fz[k];
-  }
-  qin = qin + block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n1m * n2m; q += kBC) {
-    const int i = 1 + q % n1m, j = 1 + q / n1m;
-    a += (w[P(g, i, j, 1)] - w[P(g, i, j, n3)]) * g.f2fx[i] * g.f2fy[j];
-  }
-  qin = qin + block_sum<kBC>(a, sh);
-  a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
-    a += u[P(g, n1, j, k)] * g.f2fy[j] * g.f2fz[k];
-  }
-  double ubar = block_sum<kBC>(a, sh);
-  if (tid == 0) {
-    s_bc[0] = qin;
-    s_bc[1] = ubar / (g.yl * g.zl);
-  }
-  __syncthreads();
-  qin = s_bc[0];
-  ubar = s_bc[1];
-  const double ucoef = ubar * sb.dtconst * g.f2fxi[n1m];
-  const double vwcoef = ubar * sb.dtconst * g.c2cxi[n1];
-  a = 0.0;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
+  for (int q = blockIdx.x * kObcT + threadIdx.x; q < n2m * n3m; q += kObc * kObcT) {
     const int j = 1 + q % n2m, k = 1 + q / n2m;
     const long pb = j + g.NY * k;
     const double uo = u[P(g, n1, j, k)] - ucoef * (u[P(g, n1, j, k)] - u[P(g, n1m, j, k)]);
@@ -895,34 +949,42 @@ __global__ __launch_bounds__(kBC) void k_convbc_t(Geo g, Sub sb, Fld f, int les)
       f.tout[pb] = t[P(g, n1, j, k)] - vwcoef * (t[P(g, n1, j, k)] - t[P(g, n1m, j, k)]);
     a += uo * g.f2fy[j] * g.f2fz[k];
   }
-  const double qout = block_sum<kBC>(a, sh);
-  if (tid == 0) s_bc[2] = fabs(qout) > 1.0e-14 ? qin / qout : 1.0;
-  __syncthreads();
-  const double qratio = s_bc[2];
-  for (int q = tid; q < n2m * n3m; q += kBC) {
+  const double s = block_sum<kObcT>(a, sh);
+  if (threadIdx.x == 0) f.obc[4 * kObc + blockIdx.x] = s;
+  (void)n3m;
+}
+
+__global__ __launch_bounds__(kObcT) void k_obc_apply_t(Geo g, Sub sb, Fld f, int les) {
+  __shared__ double sh[kObcT / 64 + 1];
+  const ObcIn in = obc_in(g, f, sh);
+  const double qout = obc_total(f.obc + 4 * kObc, sh);
+  const double qratio = fabs(qout) > 1.0e-14 ? in.qin / qout : 1.0;
+  const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m;
+  const double *t = f.t;
+  const double cre2 = les ? 2. * sb.re * sb.pr : 0.;
+  for (int q = blockIdx.x * kObcT + threadIdx.x; q < n2m * g.n3m; q += kObc * kObcT) {
     const int j = 1 + q % n2m, k = 1 + q / n2m;
     const long pb = j + g.NY * k;
-    if (!(f.mask[P(g, n1m, j, k)] & 64)) f.tout[pb] = f.tout[pb] * qratio;
-    f.dtout[pb] = f.tout[pb] - t[P(g, n1, j, k)];
-  }
-  __syncthreads();
-  for (int j = tid; j <= n2; j += kBC) {
-    f.dtout[j + g.NY * n3] = f.dtout[j + g.NY * 1];
-    f.dtout[j] = f.dtout[j + g.NY * n3m];
-  }
-  __syncthreads();
-  const double cre2 = les ? 2. * sb.re * sb.pr : 0.;
-  for (int q = tid; q < n2m * n3m; q += kBC) {
-    const int j = 1 + q % n2m, k = 1 + q / n2m;
+    double to = f.tout[pb];
+    if (!(f.mask[P(g, n1m, j, k)] & 64)) {
+      to = to * qratio;
+      f.tout[pb] = to;
+    }
+    const double dt = to - t[P(g, n1, j, k)];
+    f.dtout[pb] = dt;
     const long c = P(g, n1m, j, k);
-    f.rhs[3][c] = f.rhs[3][c] - sb.acoef * 1. * 1. * g.ciu[n1m] * (1. + cre2 * 1.0 * f.al[c]) *
-                                    f.dtout[j + g.NY * k];
+    f.rhs[3][c] = f.rhs[3][c] - sb.acoef * 1. * 1. * g.ciu[n1m] * (1. + cre2 * 1.0 * f.al[c]) * dt;
   }
 }
 
 void launch_convbc_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &, int les,
                      const lw_settings &) {
-  hipLaunchKernelGGL(k_convbc_t, dim3(1), dim3(kBC), 0, s, g, sb, f, les);
+  const double pl = (double)g.n2m * g.n3m;
+  launch_obc_flux(s, g, f);
+  LW_LAUNCH("k_obc_out_t", pl * 8.0 * 7, 0, k_obc_out_t, dim3(kObc), dim3(kObcT), 0, s, g, sb, f);
+  LW_LAUNCH("k_obc_apply_t", pl * 8.0 * 7, 0, k_obc_apply_t, dim3(kObc), dim3(kObcT), 0, s, g, sb, f, les);
+  LW_LAUNCH("k_obc_ghost", (double)(g.n2 + 1) * 8.0 * 4, 0, k_obc_ghost, dim3((g.n2 + 64) / 64), dim3(64), 0, s, g,
+            f.dtout, (double *)nullptr, (double *)nullptr);
 }
 
 // retrv_t, prdic_adj_t (periodic z) and wallbc_t (slv_engy.f90:612-812)
@@ -961,10 +1023,13 @@ __global__ void k_ywall_t(Geo g, Fld f, int bb, int bt, double vb, double vt) {
 }
 
 void launch_energy_bcs(hipStream_t s, const Geo &g, const Fld &f, const Body &, const lw_settings &st) {
-  hipLaunchKernelGGL(k_retrv_t, dim3((g.n2m + 63) / 64, g.n3m), dim3(64), 0, s, g, f);
-  hipLaunchKernelGGL(k_zghost_t, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
-  hipLaunchKernelGGL(k_inlet_t, dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f, st.tsol_i, st.tflu_i);
-  hipLaunchKernelGGL(k_ywall_t, dim3((g.n1 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f, (int)st.bc_t_ybtm,
+  LW_LAUNCH("k_retrv_t", (double)g.n2m * g.n3m * 8.0 * 3, 0, k_retrv_t, dim3((g.n2m + 63) / 64, g.n3m), dim3(64),
+            0, s, g, f);
+  LW_LAUNCH("k_zghost_t", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 4, 0, k_zghost_t, dim3((g.n1 + 64) / 64, g.n2 + 1),
+            dim3(64), 0, s, g, f);
+  LW_LAUNCH("k_inlet_t", (double)(g.n2 + 1) * (g.n3 + 1) * 8.0, 0, k_inlet_t, dim3((g.n2 + 64) / 64, g.n3 + 1),
+            dim3(64), 0, s, g, f, st.tsol_i, st.tflu_i);
+  LW_LAUNCH("k_ywall_t", (double)(g.n1 + 1) * (g.n3 + 1) * 8.0 * 4, 0, k_ywall_t, dim3((g.n1 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f, (int)st.bc_t_ybtm,
                      (int)st.bc_t_ytop, st.val_t_ybtm, st.val_t_ytop);
 }
 
@@ -972,61 +1037,72 @@ void launch_energy_bcs(hipStream_t s, const Geo &g, const Fld &f, const Body &, 
 // CFL (slv_auxi.f90:53-102), convergence_check / masscheck
 // (misc_init.f90:1047-1104), lagforce (slv_auxi.f90:519-570)
 // ---------------------------------------------------------------------------
-__global__ void k_cfl(Geo g, Fld f, double dt) {
+// block-wide max of non-negative values, then one atomic per workgroup
+__device__ inline void block_max_to(double v, unsigned long long *dst, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int t = threadIdx.y * 64 + threadIdx.x;
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  if (t == 0) atomic_max_nonneg(dst, fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3])));
+}
+
+// each thread walks k for one (i, j); one workgroup covers 64 x 4 columns
+__global__ __launch_bounds__(256) void k_cfl(Geo g, Fld f, double dt) {
+  __shared__ double sh[4];
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
   double c0 = 0.0;
   if (i <= g.n1m && j <= g.n2m) {
-    const long c = P(g, i, j, k);
-    const double c1 = fabs(f.u[c] + f.u[c + 1]) * 0.5 * g.f2fxi[i];
-    const double c2 = fabs(f.v[c] + f.v[c + g.NX]) * 0.5 * g.f2fyi[j];
-    const double c3 = fabs(f.w[c] + f.w[c + g.NX * g.NY]) * 0.5 * g.f2fzi[k];
-    c0 = (c1 + c2 + c3) * dt;
+    for (int k = 1; k <= g.n3m; ++k) {
+      const long c = P(g, i, j, k);
+      const double c1 = fabs(f.u[c] + f.u[c + 1]) * 0.5 * g.f2fxi[i];
+      const double c2 = fabs(f.v[c] + f.v[c + g.NX]) * 0.5 * g.f2fyi[j];
+      const double c3 = fabs(f.w[c] + f.w[c + g.NX * g.NY]) * 0.5 * g.f2fzi[k];
+      c0 = fmax(c0, (c1 + c2 + c3) * dt);
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) c0 = fmax(c0, __shfl_down(c0, o, 64));
-  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) atomic_max_nonneg(&f.redmax[0], c0);
+  block_max_to(c0, &f.redmax[0], sh);
 }
 
 void launch_cfl(hipStream_t s, const Geo &g, const Fld &f, double dt) {
-  hipLaunchKernelGGL(k_cfl, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, dt);
+  LW_LAUNCH("k_cfl", pts(g) * 24.0, 0, k_cfl, grid3(g.n1m, g.n2m, 1), BLK3, 0, s, g, f, dt);
 }
 
-__global__ void k_diag(Geo g, Fld f, int masson) {
+__global__ __launch_bounds__(256) void k_diag(Geo g, Fld f, int masson) {
+  __shared__ double sh[4];
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
   double dv = 0.0, qm = 0.0;
   if (i <= g.n1m && j <= g.n2m) {
-    const long c = P(g, i, j, k);
-    const double d1 = (f.u[c + 1] - f.u[c]) * g.f2fxi[i];
-    const double d2 = (f.v[c + g.NX] - f.v[c]) * g.f2fyi[j];
-    const double d3 = (f.w[c + g.NX * g.NY] - f.w[c]) * g.f2fzi[k];
-    const double dd = masson ? d1 + d2 + d3 - f.qmass[c] : d1 + d2 + d3;
-    dv = fabs(dd);
-    qm = masson ? fabs(f.qmass[c]) : 0.0;
+    for (int k = 1; k <= g.n3m; ++k) {
+      const long c = P(g, i, j, k);
+      const double d1 = (f.u[c + 1] - f.u[c]) * g.f2fxi[i];
+      const double d2 = (f.v[c + g.NX] - f.v[c]) * g.f2fyi[j];
+      const double d3 = (f.w[c + g.NX * g.NY] - f.w[c]) * g.f2fzi[k];
+      const double dd = masson ? d1 + d2 + d3 - f.qmass[c] : d1 + d2 + d3;
+      dv = fmax(dv, fabs(dd));
+      if (masson) qm = fmax(qm, fabs(f.qmass[c]));
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    dv = fmax(dv, __shfl_down(dv, o, 64));
-    qm = fmax(qm, __shfl_down(qm, o, 64));
-  }
-  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) {
-    atomic_max_nonneg(&f.redmax[1], dv);
-    atomic_max_nonneg(&f.redmax[2], qm);
-  }
+  block_max_to(dv, &f.redmax[1], sh);
+  __syncthreads();
+  block_max_to(qm, &f.redmax[2], sh);
 }
 
 void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson) {
-  hipLaunchKernelGGL(k_diag, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, masson);
+  LW_LAUNCH("k_diag", pts(g) * 8.0 * (masson ? 4 : 3), 0, k_diag, grid3(g.n1m, g.n2m, 1), BLK3, 0, s, g, f,
+            masson);
 }
 
+// lagforce (slv_auxi.f90:519-570): volume-weighted sums of the IB forcing of
+// u, v, w over the forcing points; block partials, then one fixed-order sum
 __global__ __launch_bounds__(256) void k_lagforce(Geo g, Sub sb, Fld f, IBComp u, IBComp v, IBComp w) {
   __shared__ double sh[4];
   const IBComp *cs[3] = {&u, &v, &w};
-  double acc[3] = {0, 0, 0};
   for (int c = 0; c < 3; ++c) {
     const IBComp &ib = *cs[c];
-    for (int n = threadIdx.x; n < ib.nbody; n += 256) {
+    double acc = 0.0;
+    for (int n = blockIdx.x * 256 + threadIdx.x; n < ib.nbody; n += gridDim.x * 256) {
       const int ii = ib.ifc[n], jj = ib.jfc[n], kk = ib.kfc[n];
       const long q = P(g, ii, jj, kk);
       const double rr = ib.dudtr[n] - (sb.gam * f.rk3o[c][q] + sb.ro * f.rk3oo[c][q]);
@@ -1034,17 +1110,27 @@ __global__ __launch_bounds__(256) void k_lagforce(Geo g, Sub sb, Fld f, IBComp u
       if (c == 0) vol = g.c2cx[ii] * g.f2fy[jj] * g.f2fz[kk];
       else if (c == 1) vol = g.f2fx[ii] * g.c2cy[jj] * g.f2fz[kk];
       else vol = g.f2fx[ii] * g.f2fy[jj] * g.c2cz[kk];
-      acc[c] += vol * rr;
+      acc += vol * rr;
     }
+    const double s = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) f.partial[c * kRedBlocks + blockIdx.x] = s;
   }
+}
+
+__global__ __launch_bounds__(256) void k_lagforce_sum(Sub sb, Fld f) {
+  __shared__ double sh[4];
   for (int c = 0; c < 3; ++c) {
-    const double s = block_sum<256>(acc[c], sh);
+    double a = 0.0;
+    for (int q = threadIdx.x; q < kRedBlocks; q += 256) a += f.partial[c * kRedBlocks + q];
+    const double s = block_sum<256>(a, sh);
     if (threadIdx.x == 0) f.red[8 + c] = (sb.msub == 1 ? 0.0 : f.red[8 + c]) + s;
   }
 }
 
 void launch_lagforce(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib) {
-  hipLaunchKernelGGL(k_lagforce, dim3(1), dim3(256), 0, s, g, sb, f, ib[0], ib[1], ib[2]);
+  LW_LAUNCH("k_lagforce", (double)(ib[0].nbody + ib[1].nbody + ib[2].nbody) * (3 * 4 + 8 * 5), 0, k_lagforce,
+            dim3(kRedBlocks), dim3(256), 0, s, g, sb, f, ib[0], ib[1], ib[2]);
+  LW_LAUNCH("k_lagforce_sum", 3.0 * kRedBlocks * 8, 0, k_lagforce_sum, dim3(1), dim3(256), 0, s, sb, f);
 }
 
 }  // namespace lw

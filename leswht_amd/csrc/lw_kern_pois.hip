@@ -112,34 +112,53 @@ __global__ void k_fft_inv_any(Geo g, MG mg, double *phi, int N) {
   }
 }
 
+// algorithmic bytes of the z-DFTs: the real column in, the half spectrum out
+static double fft_bytes(const Geo &g, const MG &mg) {
+  return (double)g.n1m * g.n2m * (8.0 * g.n3m + 16.0 * mg.n3mh);
+}
+
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
+  void (*k)(Geo, MG, const double *) = nullptr;
   switch (g.n3m) {
-    case 2: hipLaunchKernelGGL(k_fft_fwd<2>, gr, bl, 0, s, g, mg, f.div); break;
-    case 4: hipLaunchKernelGGL(k_fft_fwd<4>, gr, bl, 0, s, g, mg, f.div); break;
-    case 8: hipLaunchKernelGGL(k_fft_fwd<8>, gr, bl, 0, s, g, mg, f.div); break;
-    case 16: hipLaunchKernelGGL(k_fft_fwd<16>, gr, bl, 0, s, g, mg, f.div); break;
-    case 32: hipLaunchKernelGGL(k_fft_fwd<32>, gr, bl, 0, s, g, mg, f.div); break;
-    case 64: hipLaunchKernelGGL(k_fft_fwd<64>, gr, bl, 0, s, g, mg, f.div); break;
-    default: hipLaunchKernelGGL(k_fft_fwd_any, gr, bl, 0, s, g, mg, f.div, g.n3m); break;
+    case 2: k = k_fft_fwd<2>; break;
+    case 4: k = k_fft_fwd<4>; break;
+    case 8: k = k_fft_fwd<8>; break;
+    case 16: k = k_fft_fwd<16>; break;
+    case 32: k = k_fft_fwd<32>; break;
+    case 64: k = k_fft_fwd<64>; break;
+    default: break;
   }
+  if (k) LW_LAUNCH("k_fft_fwd", fft_bytes(g, mg), 0, k, gr, bl, 0, s, g, mg, f.div);
+  else LW_LAUNCH("k_fft_fwd_any", fft_bytes(g, mg), 0, k_fft_fwd_any, gr, bl, 0, s, g, mg, f.div, g.n3m);
 }
 
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
+  void (*k)(Geo, MG, double *) = nullptr;
   switch (g.n3m) {
-    case 2: hipLaunchKernelGGL(k_fft_inv<2>, gr, bl, 0, s, g, mg, f.phi); break;
-    case 4: hipLaunchKernelGGL(k_fft_inv<4>, gr, bl, 0, s, g, mg, f.phi); break;
-    case 8: hipLaunchKernelGGL(k_fft_inv<8>, gr, bl, 0, s, g, mg, f.phi); break;
-    case 16: hipLaunchKernelGGL(k_fft_inv<16>, gr, bl, 0, s, g, mg, f.phi); break;
-    case 32: hipLaunchKernelGGL(k_fft_inv<32>, gr, bl, 0, s, g, mg, f.phi); break;
-    case 64: hipLaunchKernelGGL(k_fft_inv<64>, gr, bl, 0, s, g, mg, f.phi); break;
-    default: hipLaunchKernelGGL(k_fft_inv_any, gr, bl, 0, s, g, mg, f.phi, g.n3m); break;
+    case 2: k = k_fft_inv<2>; break;
+    case 4: k = k_fft_inv<4>; break;
+    case 8: k = k_fft_inv<8>; break;
+    case 16: k = k_fft_inv<16>; break;
+    case 32: k = k_fft_inv<32>; break;
+    case 64: k = k_fft_inv<64>; break;
+    default: break;
   }
+  if (k) LW_LAUNCH("k_fft_inv", fft_bytes(g, mg), 0, k, gr, bl, 0, s, g, mg, f.phi);
+  else LW_LAUNCH("k_fft_inv_any", fft_bytes(g, mg), 0, k_fft_inv_any, gr, bl, 0, s, g, mg, f.phi, g.n3m);
 }
 
 // ---------------------------------------------------------------------------
-// bookkeeping kernels
+// multigrid (poiss_z_ft.f90:258-511)
+//
+// Launches per V-cycle: on each level going down the two zebra halves and one
+// residual+restriction kernel; NBLI pairs of halves on the coarsest level;
+// going up the prolongation (goup) is folded into the first half of the finer
+// level, which is the only reader of the prolonged rows before the second
+// half overwrites them; the top level ends with one kernel that forms the
+// residual, restricts it, applies the kv = 1 gauge shift and, in its last
+// workgroup, runs the convergence test of every wavenumber.
 // ---------------------------------------------------------------------------
 __global__ void k_mg_reset(MG mg, long n) {
   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -153,40 +172,52 @@ __global__ void k_mg_reset(MG mg, long n) {
 
 void launch_mg_zero(hipStream_t s, const Geo &, const MG &mg) {
   const long n = (long)mg.n3mh * mg.PCPLANE;
-  hipLaunchKernelGGL(k_mg_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mg, n);
+  LW_LAUNCH("k_mg_reset", 16.0 * n, 0, k_mg_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mg, n);
 }
 
-// start of a V-cycle: count it, or stop a wavenumber at the MGITR cap
-__global__ void k_mg_cycle_begin(MG mg) {
-  const int q = threadIdx.x;
-  if (q >= mg.n3mh || mg.done[q]) return;
-  if (mg.cyc[q] >= mg.mgitr) {
-    mg.done[q] = 2;  // iteration limit exceeded
-    return;
+// goup correction of a destination row as the reference forms it:
+// (dst + coi1*g(ic)) + coi2*g(ic+1), or dst + coi1*g(ic) on the last row
+struct Prolong {
+  const double2 *g0, *g1;
+  double c1, c2;
+  bool on, last;
+  __device__ inline double2 apply(double2 v, int j) const {
+    if (!on) return v;
+    return last ? cadd(v, rmul(c1, g0[j])) : cadd(cadd(v, rmul(c1, g0[j])), rmul(c2, g1[j]));
   }
-  mg.cyc[q] += 1;
-  mg.sumres[q] = 0ull;
-}
+};
 
-// after toplevel: the max-norm test (poiss_z_ft.f90:284) per wavenumber
-__global__ void k_mg_check(MG mg, double test) {
-  const int q = threadIdx.x;
-  if (q >= mg.n3mh || mg.done[q]) return;
-  const double r = __longlong_as_double((long long)mg.sumres[q]);
-  if (r < test) mg.done[q] = 1;
+__device__ inline Prolong make_prolong(const MG &mg, const double2 *gg, int ic, bool on, int cend) {
+  Prolong p;
+  p.on = on;
+  p.last = false;
+  p.g0 = p.g1 = nullptr;
+  p.c1 = p.c2 = 0.0;
+  if (on) {
+    p.g0 = ROW(gg, ic);
+    p.g1 = ROW(gg, mg.ipm[ic]);
+    p.c1 = mg.coi1[ic];
+    p.c2 = mg.coi2[ic];
+    p.last = ic == cend;
+  }
+  return p;
 }
 
 // ---------------------------------------------------------------------------
 // one zebra line: y-tridiagonal solve with the LU factors of line i
 // (trdiag1m, poiss_z_ft.f90:383-408). MODE selects the right-hand side:
-// 0 relax first half / later sweeps  resd - oldv*(aw*g(im) + ae*g(ip))
-// 1 relax second half                resd - aw*g(im) - ae*g(ip)
-// 2 toplevel first half              rhs - oldv*(aw*pc(im) + ae*pc(ip))
-// 3 toplevel second half             rhs - aw*(g(im)-tt) - ae*(g(ip)-tt),
+// 0 relax first half        resd - oldv*(aw*g(im) + ae*g(ip))
+// 1 relax second half       resd - aw*g(im) - ae*g(ip)
+// 2 toplevel first half     rhs - oldv*(aw*pc(im) + ae*pc(ip))
+// 3 toplevel second half    rhs - aw*(g(im)-tt) - ae*(g(ip)-tt),
 //   tt = g(1,n2m-1) for kv = 1 (the gauge of the singular mode), 0 otherwise
+// 4 as 0, the neighbours prolonged from level clev first (goup)
+// 5 as 2, the neighbours prolonged from level clev first (toplevel(1))
+// The neighbour rows of line t that receive a prolongation are the
+// destinations of coarse rows t-1 (when t > 0) and t of level clev.
 // ---------------------------------------------------------------------------
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, double *ttsave) {
+__global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
   const int lane = threadIdx.x;
   const int t = blockIdx.x * 4 + threadIdx.y;
   const int kv = 1 + blockIdx.y;
@@ -199,15 +230,25 @@ __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nline
   const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
   const int im = mg.imm[i], ip = mg.ipm[i];
   const double aw = mg.aw[i], ae = mg.ae[i];
-  const double2 *src = (MODE <= 1) ? ROW(mg.resd + pl, i) : ROW(mg.rhs + ppl, i);
-  const double2 *nbm = (MODE == 2) ? ROW(mg.pc + ppl, im) : ROW(gg, im);
-  const double2 *nbp = (MODE == 2) ? ROW(mg.pc + ppl, ip) : ROW(gg, ip);
+  constexpr bool TOP = MODE == 2 || MODE == 3 || MODE == 5;
+  constexpr bool PCN = MODE == 2 || MODE == 5;
+  constexpr bool PRO = MODE == 4 || MODE == 5;
+  const double2 *src = TOP ? ROW(mg.rhs + ppl, i) : ROW(mg.resd + pl, i);
+  const double2 *nbm = PCN ? ROW(mg.pc + ppl, im) : ROW(gg, im);
+  const double2 *nbp = PCN ? ROW(mg.pc + ppl, ip) : ROW(gg, ip);
+  Prolong pm, pp;
+  if (PRO) {
+    const int cb = mg.iimg[clev][1], ce = mg.iimg[clev][2];
+    pm = make_prolong(mg, gg, cb + t - 1, t > 0, ce);
+    pp = make_prolong(mg, gg, cb + t, true, ce);
+  }
   double2 tt = make_double2(0.0, 0.0);
   if (MODE == 3 && kv == 1) {
     tt = ROW(gg, 1)[n2m - 1];
     if (t == 0 && lane == 0) {
-      ttsave[0] = tt.x;
-      ttsave[1] = tt.y;
+      double *ts = (double *)(mg.sumres + mg.n3mh);
+      ts[0] = tt.x;
+      ts[1] = tt.y;
     }
   }
   const int j0 = 1 + lane * R;
@@ -220,6 +261,9 @@ __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nline
       const double2 s0 = src[j];
       if (MODE == 0 || MODE == 2) {
         rr[r] = csub(s0, rmul(oldv, cadd(rmul(aw, nbm[j]), rmul(ae, nbp[j]))));
+      } else if (PRO) {
+        const double2 vm = pm.apply(nbm[j], j), vp = pp.apply(nbp[j], j);
+        rr[r] = csub(s0, rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
       } else if (MODE == 1) {
         rr[r] = csub(csub(s0, rmul(aw, nbm[j])), rmul(ae, nbp[j]));
       } else {
@@ -306,7 +350,7 @@ __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nline
   }
 }
 
-typedef void (*LineKernel)(Geo, MG, int, int, double, double *);
+typedef void (*LineKernel)(Geo, MG, int, int, double, int);
 
 template <int MODE>
 static LineKernel pick_line(int R) {
@@ -326,151 +370,189 @@ static inline int lanes_r(int n2m) {
 }
 
 static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, int ibase, int nlines, double oldv,
-                         double *tts) {
+                         int clev) {
   if (nlines <= 0) return;
   const int R = lanes_r(g.n2m);
-  LineKernel k = mode == 0 ? pick_line<0>(R) : mode == 1 ? pick_line<1>(R) : mode == 2 ? pick_line<2>(R)
-                                                                                       : pick_line<3>(R);
-  hipLaunchKernelGGL(k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.n3mh), dim3(64, 4), 0, s, g, mg, ibase,
-                     nlines, oldv, tts);
+  LineKernel k;
+  switch (mode) {
+    case 0: k = pick_line<0>(R); break;
+    case 1: k = pick_line<1>(R); break;
+    case 2: k = pick_line<2>(R); break;
+    case 3: k = pick_line<3>(R); break;
+    case 4: k = pick_line<4>(R); break;
+    default: k = pick_line<5>(R); break;
+  }
+  static const char *names[6] = {"k_line_relax1", "k_line_relax2", "k_line_top1",
+                                 "k_line_top2",   "k_line_relax1_up", "k_line_top1_up"};
+  // per row: own right-hand side 16, one neighbour row 16 (each shared by two
+  // lines), bet 8, gam 8, solution 16; the prolonging modes also read one
+  // coarse row per line (16)
+  const double per_row = mode >= 4 ? 80.0 : 64.0;
+  LW_LAUNCH(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.n3mh),
+            dim3(64, 4), 0, s, g, mg, ibase, nlines, oldv, clev);
 }
 
-// kv = 1: shift the odd lines by tt after the second half (toplevel 339-352)
-__global__ void k_top_shift(Geo g, MG mg, const double *ttsave) {
-  if (mg.done[0]) return;
-  const int i = 1 + 2 * blockIdx.x;
-  const double2 tt = make_double2(ttsave[0], ttsave[1]);
-  double2 *row = ROW(mg.gg, i);
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += blockDim.x) row[j] = csub(row[j], tt);
+// residual of row i from the rows around it (poiss_z_ft.f90:360-369, 438-449)
+#define RESID_ROW(r0_, gi_, gm_, gp_, ac_, j_)        \
+  ({                                                  \
+    double2 r_ = (r0_);                               \
+    r_ = csub(r_, rmul(aw, (gm_)));                   \
+    r_ = csub(r_, rmul(ae, (gp_)));                   \
+    r_ = csub(r_, rmul(mg.as[j_], gi_[(j_) - 1]));    \
+    r_ = csub(r_, rmul(mg.an[j_], gi_[(j_) + 1]));    \
+    r_ = csub(r_, rmul((ac_)[j_], gi_[j_]));          \
+    r_;                                               \
+  })
+
+// going down (godown, poiss_z_ft.f90:438-458): residual of the fine rows
+// i0 = start + 2t and i0 + 1 in place, then the coarse row t from them
+__global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  const int t = blockIdx.x;
+  const long pl = (long)(kv - 1) * mg.PLANE;
+  const double2 *gg = mg.gg + pl;
+  const double *acb = mg.ac + pl;
+  double2 *rsb = mg.resd + pl;
+  const int ia = mg.iimg[ilev][1] + 2 * t, ib = ia + 1;
+  const int ic = mg.iimg[ilev - 1][1] + t;
+  const double2 *ga = ROW(gg, ia), *gb = ROW(gg, ib), *gam_ = ROW(gg, mg.imm[ia]), *gbp = ROW(gg, mg.ipm[ib]);
+  const double *aca = ROW(acb, ia), *acbb = ROW(acb, ib);
+  double2 *ra = ROW(rsb, ia), *rb = ROW(rsb, ib), *rc = ROW(rsb, ic);
+  const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+  const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
+  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
+    double aw = awa, ae = aea;
+    const double2 x = RESID_ROW(ra[j], ga, gam_[j], gb[j], aca, j);
+    aw = awb;
+    ae = aeb;
+    const double2 y = RESID_ROW(rb[j], gb, ga[j], gbp[j], acbb, j);
+    ra[j] = x;
+    rb[j] = y;
+    rc[j] = cadd(rmul(c1, x), rmul(c2, y));
+  }
 }
 
-// residual on the top level, max-norm per wavenumber, pc = gg
-__global__ __launch_bounds__(256) void k_top_resid(Geo g, MG mg) {
+// top level (toplevel, poiss_z_ft.f90:339-380): rows ia = 2t+1 (odd) and
+// ib = 2t+2. For kv = 1 the reference shifts every odd row of the top level
+// by tt before the residual; here the shift is applied as the rows are read
+// (the shifted odd rows reach pc; gg's odd rows are overwritten by the next
+// toplevel's first half before anything reads them). Residual into resd,
+// pc = solution, coarse row t, per-wavenumber max norm.
+__global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
   __shared__ double sh[4];
   const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int i = 1 + blockIdx.x;
-  const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
-  const double2 *gg = mg.gg + pl;
-  const double2 *gi = ROW(gg, i), *gm = ROW(gg, mg.imm[i]), *gp = ROW(gg, mg.ipm[i]);
-  const double2 *rh = ROW(mg.rhs + ppl, i);
-  const double *ac = ROW(mg.ac + pl, i);
-  double2 *rs = ROW(mg.resd + pl, i);
-  double2 *pc = ROW(mg.pc + ppl, i);
-  const double aw = mg.aw[i], ae = mg.ae[i];
-  double ar = 0.0;
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
-    double2 r = rh[j];
-    r = csub(r, rmul(aw, gm[j]));
-    r = csub(r, rmul(ae, gp[j]));
-    r = csub(r, rmul(mg.as[j], gi[j - 1]));
-    r = csub(r, rmul(mg.an[j], gi[j + 1]));
-    r = csub(r, rmul(ac[j], gi[j]));
-    rs[j] = r;
-    ar = fmax(ar, hypot(r.x, r.y));
-    pc[j] = gi[j];
-  }
-  for (int o = 32; o > 0; o >>= 1) ar = fmax(ar, __shfl_down(ar, o, 64));
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ar;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-    atomicMax(&mg.sumres[kv - 1], (unsigned long long)__double_as_longlong(m));
-  }
-}
-
-// restriction of level lf's residual into level lf-1 (toplevel 370-377,
-// godown 450-458)
-__global__ __launch_bounds__(256) void k_restrict(Geo g, MG mg, int lf) {
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int lc = lf - 1;
-  const int i = mg.iimg[lc][1] + blockIdx.x;
-  const int ic = mg.iimg[lf][1] + 2 * blockIdx.x;
-  double2 *base = mg.resd + (long)(kv - 1) * mg.PLANE;
-  double2 *rc = ROW(base, i);
-  const double2 *r0 = ROW(base, ic), *r1 = ROW(base, ic + 1);
-  const double c1 = mg.cor1[i], c2 = mg.cor2[i];
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) rc[j] = cadd(rmul(c1, r0[j]), rmul(c2, r1[j]));
-}
-
-// godown residual on level ilev, in place (poiss_z_ft.f90:438-449)
-__global__ __launch_bounds__(256) void k_lev_resid(Geo g, MG mg, int ilev) {
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int i = mg.iimg[ilev][1] + blockIdx.x;
-  const long pl = (long)(kv - 1) * mg.PLANE;
-  const double2 *gg = mg.gg + pl;
-  const double2 *gi = ROW(gg, i), *gm = ROW(gg, mg.imm[i]), *gp = ROW(gg, mg.ipm[i]);
-  const double *ac = ROW(mg.ac + pl, i);
-  double2 *rs = ROW(mg.resd + pl, i);
-  const double aw = mg.aw[i], ae = mg.ae[i];
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
-    double2 r = rs[j];
-    r = csub(r, rmul(aw, gm[j]));
-    r = csub(r, rmul(ae, gp[j]));
-    r = csub(r, rmul(mg.as[j], gi[j - 1]));
-    r = csub(r, rmul(mg.an[j], gi[j + 1]));
-    r = csub(r, rmul(ac[j], gi[j]));
-    rs[j] = r;
+  if (!mg.done[kv - 1]) {
+    const int t = blockIdx.x;
+    const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+    const double2 *gg = mg.gg + pl;
+    const int ia = 1 + 2 * t, ib = ia + 1;
+    const int nl = mg.nlev;
+    const int ic = mg.iimg[nl - 1][1] + t;
+    double2 tt = make_double2(0.0, 0.0);
+    const bool sh1 = kv == 1;
+    if (sh1) {
+      const double *ts = (const double *)(mg.sumres + mg.n3mh);
+      tt = make_double2(ts[0], ts[1]);
+    }
+    // neighbours: imm(ia) = 2t (even), ipm(ib) = 2t+3 (odd, shifted unless
+    // beyond the top level)
+    const int ibp = mg.ipm[ib];
+    const bool sbp = sh1 && ibp <= g.n1m;
+    const double2 *ga = ROW(gg, ia);
+    const double2 *gb = ROW(gg, ib), *gam_ = ROW(gg, mg.imm[ia]), *gbp = ROW(gg, ibp);
+    const double *aca = ROW(mg.ac + pl, ia), *acbb = ROW(mg.ac + pl, ib);
+    const double2 *rha = ROW(mg.rhs + ppl, ia), *rhb = ROW(mg.rhs + ppl, ib);
+    double2 *ra = ROW(mg.resd + pl, ia), *rb = ROW(mg.resd + pl, ib), *rc = ROW(mg.resd + pl, ic);
+    double2 *pa = ROW(mg.pc + ppl, ia), *pb = ROW(mg.pc + ppl, ib);
+    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
+    double ar = 0.0;
+    // the shift covers j = 1..n2m only; the ghost rows j = 0, n2 stay zero
+    auto GA = [&](int j) -> double2 { return (sh1 && j >= 1 && j <= g.n2m) ? csub(ga[j], tt) : ga[j]; };
+    for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
+      double aw = awa, ae = aea;
+      const double2 gaj = GA(j);
+      double2 x = rha[j];
+      x = csub(x, rmul(aw, gam_[j]));
+      x = csub(x, rmul(ae, gb[j]));
+      x = csub(x, rmul(mg.as[j], GA(j - 1)));
+      x = csub(x, rmul(mg.an[j], GA(j + 1)));
+      x = csub(x, rmul(aca[j], gaj));
+      aw = awb;
+      ae = aeb;
+      const double2 gbpj = sbp ? csub(gbp[j], tt) : gbp[j];
+      double2 y = rhb[j];
+      y = csub(y, rmul(aw, gaj));
+      y = csub(y, rmul(ae, gbpj));
+      y = csub(y, rmul(mg.as[j], gb[j - 1]));
+      y = csub(y, rmul(mg.an[j], gb[j + 1]));
+      y = csub(y, rmul(acbb[j], gb[j]));
+      ra[j] = x;
+      rb[j] = y;
+      rc[j] = cadd(rmul(c1, x), rmul(c2, y));
+      pa[j] = gaj;
+      pb[j] = gb[j];
+      ar = fmax(ar, fmax(hypot(x.x, x.y), hypot(y.x, y.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) ar = fmax(ar, __shfl_down(ar, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ar;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+      atomicMax(&mg.sumres[kv - 1], (unsigned long long)__double_as_longlong(m));
+    }
   }
 }
 
-// goup (poiss_z_ft.f90:411-435), or with topc the prolongation into pc of
-// toplevel(1) (312-327); evaluated left to right as the reference does
-__global__ __launch_bounds__(256) void k_goup(Geo g, MG mg, int ilev, int topc) {
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int i = mg.iimg[ilev][1] + blockIdx.x;
-  const bool last = i == mg.iimg[ilev][2];
-  const long pl = (long)(kv - 1) * mg.PLANE;
-  const double2 *gi = ROW(mg.gg + pl, i), *gp = ROW(mg.gg + pl, mg.ipm[i]);
-  double2 *dst = topc ? ROW(mg.pc + (long)(kv - 1) * mg.PCPLANE, 2 * (blockIdx.x + 1))
-                      : ROW(mg.gg + pl, mg.iimg[ilev + 1][1] + 1 + 2 * (int)blockIdx.x);
-  const double c1 = mg.coi1[i], c2 = mg.coi2[i];
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
-    if (!last) dst[j] = cadd(cadd(dst[j], rmul(c1, gi[j])), rmul(c2, gp[j]));
-    else dst[j] = cadd(dst[j], rmul(c1, gi[j]));
+// the convergence test after a toplevel (poiss_z_ft.f90:284-296): id = 1
+// counts a completed V-cycle; a wavenumber stops when its max norm is below
+// `test`, or when it has used MGITR cycles. Clears the norms for the next
+// toplevel.
+__global__ void k_mg_check(MG mg, double test, int id) {
+  for (int q = threadIdx.x; q < mg.n3mh; q += blockDim.x) {
+    if (!mg.done[q]) {
+      const int c = mg.cyc[q] + id;
+      mg.cyc[q] = c;
+      const double r = __longlong_as_double((long long)mg.sumres[q]);
+      if (r < test) mg.done[q] = 1;
+      else if (c >= mg.mgitr) mg.done[q] = 2;
+    }
+    mg.sumres[q] = 0ull;
   }
 }
-
-static inline double *tt_slot(const MG &mg) { return (double *)(mg.sumres + mg.n3mh); }
 
 static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
 
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test) {
   const int nl = mg.nlev;
-  const int nc = mg.iimg[nl - 1][2] - mg.iimg[nl - 1][1] + 1;
-  if (id == 1) hipLaunchKernelGGL(k_goup, rows(nc, mg.n3mh), dim3(256), 0, s, g, mg, nl - 1, 1);
-  launch_lines(s, g, mg, 2, 1, (g.n1m + 1) / 2, oldv, tt_slot(mg));
-  launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, tt_slot(mg));
-  hipLaunchKernelGGL(k_top_shift, dim3((unsigned)(g.n1m / 2)), dim3(256), 0, s, g, mg, (const double *)tt_slot(mg));
-  hipLaunchKernelGGL(k_top_resid, rows(g.n1m, mg.n3mh), dim3(256), 0, s, g, mg);
-  hipLaunchKernelGGL(k_restrict, rows(nc, mg.n3mh), dim3(256), 0, s, g, mg, nl);
-  hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(128), 0, s, mg, test);
+  launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
+  launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
+  // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
+  LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
+  LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id);
 }
 
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
-  hipLaunchKernelGGL(k_mg_cycle_begin, dim3(1), dim3(128), 0, s, mg);
-  auto relax = [&](int ilev, double oldv, int iiter) {
-    const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1;
-    for (int it = 0; it < iiter; ++it) {
-      launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, it == 0 ? oldv : 1.0, nullptr);
-      launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, nullptr);
-    }
-  };
+  auto nrow = [&](int l) { return mg.iimg[l][2] - mg.iimg[l][1] + 1; };
   for (int ilev = mg.nlev - 1; ilev >= 1; --ilev) {
-    relax(ilev, 0.0, 1);
-    const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1;
-    hipLaunchKernelGGL(k_lev_resid, rows(n, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
-    const int nc = mg.iimg[ilev - 1][2] - mg.iimg[ilev - 1][1] + 1;
-    hipLaunchKernelGGL(k_restrict, rows(nc, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
+    const int n = nrow(ilev);
+    launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
+    launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, 0);
+    // residual in/out 32, solution 16, ac 8 per fine point; coarse row 8 per fine point
+    LW_LAUNCH("k_lev_rr", 0, 64.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
   }
-  relax(0, 0.0, mg.nbli);
+  {
+    const int n = nrow(0);
+    for (int it = 0; it < mg.nbli; ++it) {
+      launch_lines(s, g, mg, 0, mg.iimg[0][1], (n + 1) / 2, it == 0 ? 0.0 : 1.0, 0);
+      launch_lines(s, g, mg, 1, mg.iimg[0][1] + 1, n / 2, 1.0, 0);
+    }
+  }
   for (int ilev = 0; ilev <= mg.nlev - 2; ++ilev) {
-    const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1;
-    hipLaunchKernelGGL(k_goup, rows(n, mg.n3mh), dim3(256), 0, s, g, mg, ilev, 0);
-    relax(ilev + 1, 1.0, 1);
+    const int n = nrow(ilev + 1);
+    launch_lines(s, g, mg, 4, mg.iimg[ilev + 1][1], (n + 1) / 2, 1.0, ilev);
+    launch_lines(s, g, mg, 1, mg.iimg[ilev + 1][1] + 1, n / 2, 1.0, 0);
   }
   launch_mg_toplevel(s, g, mg, 1, 1.0, test);
 }

@@ -263,28 +263,36 @@ __global__ void k_filter(Geo g, const double *in, double *out, int fx, int fz) {
   out[I3(g, i, j, k)] = r;
 }
 
-static void filter_many(hipStream_t s, const Geo &g, const SgsBuf &b, int lo, int hi, int filter) {
+static inline double pts(const Geo &g) { return (double)g.n1m * g.n2m * g.n3m; }
+
+// filters a[lo..hi] out of place; the filtered buffer takes the array's slot
+// and the old one becomes the spare
+static void filter_many(hipStream_t s, const Geo &g, SgsBuf &b, int lo, int hi, int filter) {
   const int fx = (filter == 1 || filter == 3) ? 1 : 0;
   const int fz = (filter == 2 || filter == 3) ? 1 : 0;
-  const size_t bytes = sizeof(double) * (size_t)g.n1m * g.n2m * g.n3m;
   for (int l = lo; l <= hi; ++l) {
-    hipLaunchKernelGGL(k_filter, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, b.a[l], b.tmp, fx, fz);
-    hipMemcpyAsync(b.a[l], b.tmp, bytes, hipMemcpyDeviceToDevice, s);
+    LW_LAUNCH("k_filter", pts(g) * 16.0, 0, k_filter, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, b.a[l], b.tmp, fx,
+              fz);
+    double *t = b.a[l];
+    b.a[l] = b.tmp;
+    b.tmp = t;
   }
 }
 
 // red slots: 4 = csgsts, 5 = csgshf
-void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter) {
-  hipLaunchKernelGGL(k_dvmg1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter) {
+  // u v w in; nu, alpha_ij (9) and the Vreman products (6) out
+  LW_LAUNCH("k_dvmg1", pts(g) * 8.0 * 19, 0, k_dvmg1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
   filter_many(s, g, b, 9, 14, filter);
   filter_many(s, g, b, 0, 8, filter);
-  hipLaunchKernelGGL(k_dvmg2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  // 15 filtered arrays and u v w in; M_ij (6), u_i (3) and u_i u_j (6) out
+  LW_LAUNCH("k_dvmg2", pts(g) * 8.0 * 33, 0, k_dvmg2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
   filter_many(s, g, b, 15, 23, filter);
 }
 
 void launch_sgs_dvmg_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
-  hipLaunchKernelGGL(k_dvmg3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
-  hipLaunchKernelGGL(k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -0.5, 4);
+  LW_LAUNCH("k_dvmg3", pts(g) * 8.0 * 16, 0, k_dvmg3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
+  LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -0.5, 4);
 }
 
 __global__ void k_cvm(Geo g, Fld f, double csgsts) {
@@ -299,7 +307,7 @@ __global__ void k_cvm(Geo g, Fld f, double csgsts) {
 }
 
 void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts) {
-  hipLaunchKernelGGL(k_cvm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgsts);
+  LW_LAUNCH("k_cvm", pts(g) * 32.0, 0, k_cvm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgsts);
 }
 
 // ---------------------------------------------------------------------------
@@ -386,19 +394,21 @@ __global__ __launch_bounds__(kRedThreads) void k_edm3(Geo g, Fld f, SgsBuf b, co
   }
 }
 
-void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter) {
-  hipLaunchKernelGGL(k_edm1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter) {
+  // u v w t in; alpha and 12 arrays out
+  LW_LAUNCH("k_edm1", pts(g) * 8.0 * 17, 0, k_edm1, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
   filter_many(s, g, b, 9, 11, filter);
   filter_many(s, g, b, 0, 8, filter);
-  hipLaunchKernelGGL(k_edm2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
+  // 12 filtered arrays and u v w t in; 12 arrays out
+  LW_LAUNCH("k_edm2", pts(g) * 8.0 * 28, 0, k_edm2, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, b);
   filter_many(s, g, b, 15, 17, filter);
   filter_many(s, g, b, 0, 2, filter);
   filter_many(s, g, b, 18, 20, filter);
 }
 
 void launch_sgs_edm_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
-  hipLaunchKernelGGL(k_edm3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
-  hipLaunchKernelGGL(k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -1.0, 5);
+  LW_LAUNCH("k_edm3", pts(g) * 8.0 * 13, 0, k_edm3, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, f, b, nwall);
+  LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -1.0, 5);
 }
 
 __global__ void k_cedm(Geo g, Fld f, double csgshf) {
@@ -415,7 +425,7 @@ __global__ void k_cedm(Geo g, Fld f, double csgshf) {
 }
 
 void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf) {
-  hipLaunchKernelGGL(k_cedm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgshf);
+  LW_LAUNCH("k_cedm", pts(g) * 32.0, 0, k_cedm, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, csgshf);
 }
 
 // ---------------------------------------------------------------------------
@@ -524,19 +534,24 @@ __global__ void k_edges(Geo g, const double *nu, double *o1, double *o2, double 
 
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1, int alp,
                        int scale_slot, const lw_settings &st) {
-  hipLaunchKernelGGL(k_scale_zero, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, nu, scale_slot);
-  hipLaunchKernelGGL(k_zghost_scalar, dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, nu);
+  LW_LAUNCH("k_scale_zero", pts(g) * 17.0, 0, k_scale_zero, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, nu,
+            scale_slot);
+  LW_LAUNCH("k_zghost_scalar", (double)(g.n1 + 1) * (g.n2 + 1) * 32.0, 0, k_zghost_scalar,
+            dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, nu);
   EdgeCtx e;
   e.ib = alp ? 1 : g.ib;
   e.jb = alp ? 1 : g.jb;
   e.kb = alp ? 1 : g.kb;
   e.ybwall = st.bc_ybtm == 0;
   e.ytwall = st.bc_ytop == 0;
-  hipLaunchKernelGGL(k_edges, grid3(g.n1 + 1, g.n2 + 1, g.n3 + 1), BLK3, 0, s, g, nu, nu1[0], nu1[1], nu1[2], e);
+  // nu in, three edge viscosities out
+  LW_LAUNCH("k_edges", (double)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1) * 32.0, 0, k_edges,
+            grid3(g.n1 + 1, g.n2 + 1, g.n3 + 1), BLK3, 0, s, g, nu, nu1[0], nu1[1], nu1[2], e);
 }
 
 void launch_zero_list(hipStream_t s, const Geo &g, double *nu, const int *lst, int n) {
-  if (n > 0) hipLaunchKernelGGL(k_zero_list, dim3((n + 255) / 256), dim3(256), 0, s, g, nu, lst, n);
+  if (n > 0)
+    LW_LAUNCH("k_zero_list", n * 20.0, 0, k_zero_list, dim3((n + 255) / 256), dim3(256), 0, s, g, nu, lst, n);
 }
 
 }  // namespace lw

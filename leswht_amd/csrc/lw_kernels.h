@@ -5,6 +5,24 @@
 
 namespace lw {
 
+// ---- per-launch event profiler (lw_solver.hip) ----
+// While a profile is active on the calling thread, every launch made through
+// LW_LAUNCH of a selected kernel is bracketed by two events on its stream and
+// recorded with its algorithmic HBM bytes: BYTES, plus BYTES_KV times the
+// number of wavenumbers still iterating in the multigrid cycle it belongs to.
+// BYTES and BYTES_KV are evaluated only while profiling.
+struct Prof;
+extern thread_local Prof *g_prof;
+bool prof_pre(const char *name, hipStream_t s);
+void prof_post(const char *name, hipStream_t s, double bytes, double bytes_kv);
+
+#define LW_LAUNCH(NAME, BYTES, BYTES_KV, KERNEL, GRID, BLOCK, SHM, STREAM, ...)       \
+  do {                                                                                \
+    const bool lw_p_ = ::lw::g_prof != nullptr && ::lw::prof_pre(NAME, STREAM);       \
+    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                \
+    if (lw_p_) ::lw::prof_post(NAME, STREAM, (double)(BYTES), (double)(BYTES_KV));    \
+  } while (0)
+
 struct Fld {
   double *u, *v, *w, *p, *t;
   double *rhs[4];
@@ -16,7 +34,8 @@ struct Fld {
   // outlet planes (0:n2, 0:n3), index j + NY*k
   double *uout, *vout, *wout, *tout, *duout, *dvout, *dwout, *dtout;
   // reduction scratch: red[0..63] doubles, partials[...] per block
-  double *red, *partial;
+  double *red, *partial;  // partial: 4 * kRedBlocks
+  double *obc;            // outlet-plane reduction partials (8 * 64)
   unsigned long long *redmax;  // [0] cfl, [1] dvmax, [2] qmmax
 };
 
@@ -51,9 +70,9 @@ struct SgsBuf {
   double *a[24];  // interior arrays (n1m*n2m*n3m each)
   double *tmp;
 };
-void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter);
+void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter);
 void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts);
-void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, int filter);
+void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter);
 void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf);
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1,
                        int alp, int scale_slot, const lw_settings &st);

@@ -11,6 +11,7 @@
 
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "lw_kernels.h"
@@ -36,6 +37,107 @@ static int fail(int code, const std::string &msg) {
       throw LW_ERR_DEVICE;                                                          \
     }                                                                               \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// per-launch event profiler
+// ---------------------------------------------------------------------------
+namespace lw {
+struct ProfRec {
+  int cls, cycle;
+  double bytes, bytes_kv;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false, all = true;
+  std::vector<std::string> sel;
+  std::vector<std::string> names;
+  std::unordered_map<std::string, int> idx;
+  std::vector<int64_t> launches;
+  std::vector<double> ms, bytes;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t pending = nullptr;
+  int cycle = -1;      // multigrid cycle being enqueued (0: the first toplevel)
+  size_t mg_first = 0; // first record of the Poisson solve being enqueued
+
+  hipEvent_t take() {
+    if (used == pool.size()) {
+      hipEvent_t e = nullptr;
+      HIPCHK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  int cls(const char *name) {
+    auto it = idx.find(name);
+    if (it != idx.end()) return it->second;
+    const int c = (int)names.size();
+    names.emplace_back(name);
+    idx.emplace(name, c);
+    launches.push_back(0);
+    ms.push_back(0.0);
+    bytes.push_back(0.0);
+    return c;
+  }
+  bool selected(const char *name) const {
+    if (all) return true;
+    for (const auto &n : sel)
+      if (n == name) return true;
+    return false;
+  }
+  // bytes of the per-wavenumber launches of the Poisson solve just enqueued;
+  // cyc[q] = V-cycles wavenumber q started
+  void resolve_mg(const int *cyc, int n3mh) {
+    for (size_t r = mg_first; r < recs.size(); ++r) {
+      ProfRec &p = recs[r];
+      if (p.bytes_kv == 0.0) continue;
+      int act = 0;
+      for (int q = 0; q < n3mh; ++q) act += (p.cycle == 0 || cyc[q] >= p.cycle) ? 1 : 0;
+      p.bytes += p.bytes_kv * act;
+      p.bytes_kv = 0.0;
+    }
+    cycle = -1;
+  }
+  // after the stream drained: fold the records into the per-kernel totals
+  void flush() {
+    for (const ProfRec &p : recs) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, p.a, p.b));
+      launches[p.cls] += 1;
+      ms[p.cls] += (double)t;
+      bytes[p.cls] += p.bytes;
+    }
+    recs.clear();
+    used = 0;
+    mg_first = 0;
+  }
+  ~Prof() {
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+  }
+};
+
+thread_local Prof *g_prof = nullptr;
+
+bool prof_pre(const char *name, hipStream_t s) {
+  if (!g_prof->selected(name)) return false;
+  g_prof->pending = g_prof->take();
+  HIPCHK(hipEventRecord(g_prof->pending, s));
+  return true;
+}
+
+void prof_post(const char *name, hipStream_t s, double bytes, double bytes_kv) {
+  hipEvent_t b = g_prof->take();
+  HIPCHK(hipEventRecord(b, s));
+  g_prof->recs.push_back(ProfRec{g_prof->cls(name), g_prof->cycle, bytes, bytes_kv, g_prof->pending, b});
+}
+}  // namespace lw
+
+// activates a handle's profile on this thread for the duration of one call
+struct ProfScope {
+  explicit ProfScope(Prof *p) { g_prof = (p && p->on) ? p : nullptr; }
+  ~ProfScope() { g_prof = nullptr; }
+};
 
 struct lw_handle {
   hipStream_t s = nullptr;
@@ -63,6 +165,7 @@ struct lw_handle {
   unsigned long long *h_sumres = nullptr;  // pinned
   int *h_flags = nullptr;                  // pinned copy of mg.done / mg.cyc
   int mg_hint = 4;                         // V-cycles of the previous Poisson solve
+  Prof *prof = nullptr;
   double *h_red = nullptr;                 // pinned
 
   template <class T>
@@ -215,7 +318,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.tout = h->dalloc<double>(nb); f.duout = h->dalloc<double>(nb); f.dvout = h->dalloc<double>(nb);
   f.dwout = h->dalloc<double>(nb); f.dtout = h->dalloc<double>(nb);
   f.red = h->dalloc<double>(64);
-  f.partial = h->dalloc<double>(2 * kRedBlocks);
+  f.partial = h->dalloc<double>(4 * kRedBlocks);
+  f.obc = h->dalloc<double>(8 * 64);
   f.redmax = h->dalloc<unsigned long long>(8);
   if (h->les) {
     for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
@@ -481,6 +585,7 @@ int lw_destroy(lw_handle *h) {
   if (h->h_sumres) hipHostFree(h->h_sumres);
   if (h->h_flags) hipHostFree(h->h_flags);
   if (h->h_red) hipHostFree(h->h_red);
+  delete h->prof;
   if (h->s) hipStreamDestroy(h->s);
   delete h;
   return LW_OK;
@@ -671,12 +776,19 @@ static void phase_poisson(lw_handle *h) {
   launch_mg_zero(h->s, g, mg);
   const double test1 = h->st.resid1 * h->sub.dtconsti;
   const double test = test1 / (double)mg.n3mh * (double)g.n3m * 0.8;
+  if (g_prof) {
+    g_prof->mg_first = g_prof->recs.size();
+    g_prof->cycle = 0;
+  }
   launch_mg_toplevel(h->s, g, mg, 0, 0.0, test);
   int launched = 0, batch = h->mg_hint > 1 ? h->mg_hint : 1;
   bool all_done = false;
   while (!all_done && launched < mg.mgitr) {
     if (batch > mg.mgitr - launched) batch = mg.mgitr - launched;
-    for (int c = 0; c < batch; ++c) launch_mg_cycle(h->s, g, mg, test);
+    for (int c = 0; c < batch; ++c) {
+      if (g_prof) g_prof->cycle = launched + c + 1;
+      launch_mg_cycle(h->s, g, mg, test);
+    }
     launched += batch;
     HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPCHK(hipStreamSynchronize(h->s));
@@ -697,6 +809,7 @@ static void phase_poisson(lw_handle *h) {
     limit = limit || h->h_flags[q] != 1;
   }
   if (limit) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
+  if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
   if (cmax > h->cyc_max) h->cyc_max = cmax;
   h->mg_hint = cmax;
   launch_fft_inverse(h->s, g, mg, h->f);
@@ -750,6 +863,7 @@ static void read_stats(lw_handle *h, lw_step_stats *stats) {
 
 extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
   if (!h || nsteps < 0) return fail(LW_ERR_ARG, "bad argument");
+  ProfScope ps(h->prof);
   try {
     for (int64_t n = 0; n < nsteps; ++n) {
       h->cyc_max = 0;
@@ -766,8 +880,13 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
         phase_energy(h);
       }
       end_of_step(h);
+      if (g_prof) {
+        HIPCHK(hipStreamSynchronize(h->s));
+        g_prof->flush();
+      }
     }
     read_stats(h, stats);
+    if (g_prof) g_prof->flush();
     if (!(h->dvmax == h->dvmax)) return fail(LW_ERR_NUMERIC, "non-finite divergence after the step");
     return LW_OK;
   } catch (int code) {
@@ -777,6 +896,7 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
 
 extern "C" int lw_phase(lw_handle *h, int phase) {
   if (!h) return fail(LW_ERR_ARG, "null handle");
+  ProfScope ps(h->prof);
   try {
     if (phase == 0) {
       h->cyc_max = 0;
@@ -796,10 +916,60 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
       read_stats(h, nullptr);
     } else return fail(LW_ERR_ARG, "unknown phase");
     HIPCHK(hipStreamSynchronize(h->s));
+    if (g_prof) g_prof->flush();
     return LW_OK;
   } catch (int code) {
     return code;
   }
+}
+
+extern "C" int lw_profile_begin(lw_handle *h, const char *kernels) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  try {
+    HIPCHK(hipStreamSynchronize(h->s));
+    delete h->prof;
+    h->prof = new Prof();
+    h->prof->on = true;
+    h->prof->all = kernels == nullptr || kernels[0] == 0;
+    if (!h->prof->all) {
+      std::string k(kernels), cur;
+      for (char c : k + ",") {
+        if (c == ',') {
+          if (!cur.empty()) h->prof->sel.push_back(cur);
+          cur.clear();
+        } else if (c != ' ') cur += c;
+      }
+    }
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+extern "C" int lw_profile_end(lw_handle *h) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  if (h->prof) h->prof->on = false;
+  return LW_OK;
+}
+
+extern "C" int64_t lw_profile_count(const lw_handle *h) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  return h->prof ? (int64_t)h->prof->names.size() : 0;
+}
+
+extern "C" int lw_profile_get(const lw_handle *h, int64_t idx, char *name, int64_t cap, int64_t *launches,
+                              double *ms_total, double *bytes_total) {
+  if (!h || !h->prof || idx < 0 || idx >= (int64_t)h->prof->names.size())
+    return fail(LW_ERR_ARG, "no such profile entry");
+  const Prof &p = *h->prof;
+  if (name && cap > 0) {
+    strncpy(name, p.names[idx].c_str(), (size_t)cap - 1);
+    name[cap - 1] = 0;
+  }
+  if (launches) *launches = p.launches[idx];
+  if (ms_total) *ms_total = p.ms[idx];
+  if (bytes_total) *bytes_total = p.bytes[idx];
+  return LW_OK;
 }
 
 extern "C" int lw_write_field(lw_handle *h, const char *path) {

@@ -74,7 +74,8 @@ class lw_body(C.Structure):
 
 EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw_get_fields",
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
-           "lw_device_bytes", "lw_last_error")
+           "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
+           "lw_profile_get", "lw_last_error")
 
 _lib = None
 
@@ -102,6 +103,11 @@ def load_library(path: str = LIB_PATH):
     L.lw_phase.argtypes = [H, C.c_int]
     L.lw_device_bytes.argtypes = [H]
     L.lw_device_bytes.restype = _I
+    L.lw_profile_begin.argtypes = [H, C.c_char_p]
+    L.lw_profile_end.argtypes = [H]
+    L.lw_profile_count.argtypes = [H]
+    L.lw_profile_count.restype = _I
+    L.lw_profile_get.argtypes = [H, _I, C.c_char_p, _I, _PI, _PD, _PD]
     L.lw_last_error.restype = C.c_char_p
     _lib = L
     return L
@@ -243,23 +249,25 @@ class Solver:
     def write_field(self, path: str):
         _check(load_library().lw_write_field(self._h, path.encode()))
 
-    def profile_reset(self):
-        """Clears the per-kernel event timers (see :meth:`profile`)."""
-        L = load_library()
-        if hasattr(L, "lw_profile_reset"):
-            L.lw_profile_reset(self._h)
+    def profile_begin(self, kernels=None):
+        """Starts a fresh per-kernel event profile of the following steps;
+        ``kernels`` None times every kernel, else an iterable of names."""
+        arg = None if not kernels else ",".join(kernels).encode()
+        _check(load_library().lw_profile_begin(self._h, arg))
+
+    def profile_end(self):
+        _check(load_library().lw_profile_end(self._h))
 
     def profile(self) -> list:
-        """Per-kernel event timing since :meth:`profile_reset`: dicts with
-        name, count, ms_total and algorithmic bytes per launch."""
+        """Per-kernel results of the current profile: dicts with name, count
+        (launches), ms_total (summed event time) and bytes (summed
+        algorithmic HBM bytes)."""
         L = load_library()
-        if not hasattr(L, "lw_profile_count"):
-            return []
         out = []
-        name = C.create_string_buffer(64)
+        name = C.create_string_buffer(96)
         cnt, ms, by = _I(), _D(), _D()
         for q in range(L.lw_profile_count(self._h)):
-            L.lw_profile_get(self._h, q, name, 64, C.byref(cnt), C.byref(ms), C.byref(by))
+            _check(L.lw_profile_get(self._h, q, name, 96, C.byref(cnt), C.byref(ms), C.byref(by)))
             out.append(dict(name=name.value.decode(), count=cnt.value, ms_total=ms.value, bytes=by.value))
         return out
 
