@@ -447,221 +447,269 @@ void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f,
 
 // ---------------------------------------------------------------------------
 // ADI line solves, slv_mmtm.f90:1044-1641 and slv_engy.f90:460-609.
-// One thread per line; lines are mapped so neighbouring threads touch
-// neighbouring addresses (i for the z and y sweeps). C = 0,1,2 momentum
-// component, 3 temperature.
+// One thread per line; for the z and y sweeps neighbouring threads hold
+// neighbouring i. C = 0,1,2 momentum component, 3 temperature.
 // ---------------------------------------------------------------------------
-struct Coef {
-  double a, b, c, g;
+// Coefficients of a line (the LHS assembly of slv_mmtm.f90:1044-1362 and
+// slv_engy.f90:460-609): a = am*(1 + cc*vis(lower face)),
+// c = cm*(1 + cc*vis(upper face)), b = bd - a - c, right-hand side bd*rhs,
+// with bd = 1/acoef (times Pr for temperature). LineCf names, per direction
+// D (0 x, 1 y, 2 z) and component C, the viscosity-like array, the offset
+// `lo` (-1 or 0, in points along the line) of its lower-face value, the
+// metric arrays and the multipliers.
+struct LineCf {
+  const double *vis, *am, *cm;
+  int lo;
+  double cc, bd;
 };
 
-template <int C>
-__device__ inline void coef_z(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
-                              int j, int k, Coef &o) {
-  const long q = P(g, i, j, k), dz = g.NX * g.NY;
-  double ak, ck, bd;
-  if (C == 0) {
-    ak = g.akuv[k] * (1. + cre * f.nu1[1][q]);
-    ck = g.ckuv[k] * (1. + cre * f.nu1[1][q + dz]);
-    bd = sb.acoefi;
-  } else if (C == 1) {
-    ak = g.akuv[k] * (1. + cre * f.nu1[0][q]);
-    ck = g.ckuv[k] * (1. + cre * f.nu1[0][q + dz]);
-    bd = sb.acoefi;
-  } else if (C == 2) {
-    ak = g.akw[k] * (1. + cre2 * f.nu[q - dz]);
-    ck = g.ckw[k] * (1. + cre2 * f.nu[q]);
-    bd = sb.acoefi;
+template <int D, int C>
+__device__ inline LineCf line_cf(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2) {
+  LineCf l;
+  l.bd = C == 3 ? sb.acoefi * sb.pr : sb.acoefi;
+  l.lo = 0;
+  l.cc = cre;
+  if (D == 0) {
+    l.am = C == 0 ? g.aiu : g.aivw;
+    l.cm = C == 0 ? g.ciu : g.civw;
+    if (C == 0) {
+      l.vis = f.nu;
+      l.lo = -1;
+      l.cc = cre2;
+    } else {
+      l.vis = C == 1 ? f.nu1[2] : C == 2 ? f.nu1[1] : f.al1[0];
+    }
+  } else if (D == 1) {
+    l.am = C == 1 ? g.ajv : g.ajuw;
+    l.cm = C == 1 ? g.cjv : g.cjuw;
+    if (C == 1) {
+      l.vis = f.nu;
+      l.lo = -1;
+      l.cc = cre2;
+    } else {
+      l.vis = C == 0 ? f.nu1[2] : C == 2 ? f.nu1[0] : f.al1[2];
+    }
   } else {
-    ak = g.akuv[k] * (1. * 1. + cre * 1.0 * f.al1[1][q]);
-    ck = g.ckuv[k] * (1. * 1. + cre * 1.0 * f.al1[1][q + dz]);
-    bd = sb.acoefi * sb.pr;
+    l.am = C == 2 ? g.akw : g.akuv;
+    l.cm = C == 2 ? g.ckw : g.ckuv;
+    if (C == 2) {
+      l.vis = f.nu;
+      l.lo = -1;
+      l.cc = cre2;
+    } else {
+      l.vis = C == 0 ? f.nu1[1] : C == 1 ? f.nu1[0] : f.al1[1];
+    }
   }
-  o.a = ak;
-  o.c = ck;
-  o.b = bd - ak - ck;
-  o.g = bd * f.rhs[C][q];
+  return l;
 }
 
-template <int C>
-__device__ inline void coef_y(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
-                              int j, int k, Coef &o) {
-  const long q = P(g, i, j, k), dy = g.NX;
-  double aj, cj, bd;
-  if (C == 0) {
-    aj = g.ajuw[j] * (1. + cre * f.nu1[2][q]);
-    cj = g.cjuw[j] * (1. + cre * f.nu1[2][q + dy]);
-    bd = sb.acoefi;
-  } else if (C == 1) {
-    aj = g.ajv[j] * (1. + cre2 * f.nu[q - dy]);
-    cj = g.cjv[j] * (1. + cre2 * f.nu[q]);
-    bd = sb.acoefi;
-  } else if (C == 2) {
-    aj = g.ajuw[j] * (1. + cre * f.nu1[0][q]);
-    cj = g.cjuw[j] * (1. + cre * f.nu1[0][q + dy]);
-    bd = sb.acoefi;
-  } else {
-    aj = g.ajuw[j] * (1. * 1. + cre * 1.0 * f.al1[2][q]);
-    cj = g.cjuw[j] * (1. * 1. + cre * 1.0 * f.al1[2][q + dy]);
-    bd = sb.acoefi * sb.pr;
-  }
-  o.a = aj;
-  o.c = cj;
-  o.b = bd - aj - cj;
-  o.g = bd * f.rhs[C][q];
-}
+// Each thread solves one line. The loads of a chunk of kAdiCh points are
+// issued together ahead of the recurrence over them (the recurrence itself is
+// the reference's, operation for operation), so a line waits for memory once
+// per chunk instead of once per point.
+constexpr int kAdiCh = 8;
 
-template <int C>
-__device__ inline void coef_x(const Geo &g, const Fld &f, const Sub &sb, double cre, double cre2, int i,
-                              int j, int k, Coef &o) {
-  const long q = P(g, i, j, k);
-  double ai, ci, bd;
-  if (C == 0) {
-    ai = g.aiu[i] * (1. + cre2 * f.nu[q - 1]);
-    ci = g.ciu[i] * (1. + cre2 * f.nu[q]);
-    bd = sb.acoefi;
-  } else if (C == 1) {
-    ai = g.aivw[i] * (1. + cre * f.nu1[2][q]);
-    ci = g.civw[i] * (1. + cre * f.nu1[2][q + 1]);
-    bd = sb.acoefi;
-  } else if (C == 2) {
-    ai = g.aivw[i] * (1. + cre * f.nu1[1][q]);
-    ci = g.civw[i] * (1. + cre * f.nu1[1][q + 1]);
-    bd = sb.acoefi;
-  } else {
-    ai = g.aivw[i] * (1. * 1. + cre * 1.0 * f.al1[0][q]);
-    ci = g.civw[i] * (1. * 1. + cre * 1.0 * f.al1[0][q + 1]);
-    bd = sb.acoefi * sb.pr;
+// non-periodic line along stride `st` from index lo_ to n (trdiag1/trdiag2,
+// slv_mmtm.f90:1365-1455): forward elimination into gm/r; the coefficient of
+// index l uses metrics am[l], cm[l]
+__device__ inline double line_forward(const LineCf &l, double *r, double *gm, long q0, long st, int lo_, int n) {
+  double bet, up, cprev;
+  {
+    const long q = q0 + lo_ * st;
+    const double a = l.am[lo_] * (1. + l.cc * l.vis[q + l.lo * st]);
+    const double c = l.cm[lo_] * (1. + l.cc * l.vis[q + (l.lo + 1) * st]);
+    const double b = l.bd - a - c;
+    bet = 1. / b;
+    up = (l.bd * r[q]) * bet;
+    cprev = c;
+    r[q] = up;
   }
-  o.a = ai;
-  o.c = ci;
-  o.b = bd - ai - ci;
-  o.g = bd * f.rhs[C][q];
+  for (int ib = lo_ + 1; ib <= n; ib += kAdiCh) {
+    double v[kAdiCh + 1], rv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u <= kAdiCh; ++u)
+      v[u] = ib + u <= n + 1 ? l.vis[q0 + (long)(ib + u + l.lo) * st] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) rv[u] = ib + u <= n ? r[q0 + (long)(ib + u) * st] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const int i = ib + u;
+      if (i > n) break;
+      const double a = l.am[i] * (1. + l.cc * v[u]);
+      const double c = l.cm[i] * (1. + l.cc * v[u + 1]);
+      const double b = l.bd - a - c;
+      const double gv = l.bd * rv[u];
+      const double gam = cprev * bet;
+      bet = 1. / (b - a * gam);
+      up = (gv - a * up) * bet;
+      const long q = q0 + (long)i * st;
+      gm[q] = gam;
+      r[q] = up;
+      cprev = c;
+    }
+  }
+  return up;
 }
 
 // periodic z line (trdiag3p, slv_mmtm.f90:1572-1641); result into rhs
 template <int C>
-__global__ void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo) {
+__global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo) {
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int j = jlo + blockIdx.y * 4 + threadIdx.y;
   if (i > g.n1m || j > g.n2m) return;
   const int n = g.n3m;
+  const LineCf l = line_cf<2, C>(g, f, sb, cre, cre2);
+  const long st = (long)g.NX * g.NY, q0 = P(g, i, j, 0);
   double *r = f.rhs[C];
   double *sq = f.s1, *ss = f.s2;
-  Coef cf;
-  coef_z<C>(g, f, sb, cre, cre2, i, j, n, cf);
-  const double fn = cf.g;
-  coef_z<C>(g, f, sb, cre, cre2, i, j, 1, cf);
-  const double binv = 1. / cf.b;
-  double qp = -cf.c * binv, sp = -cf.a * binv, fp = cf.g * binv;
-  long q1 = P(g, i, j, 1);
-  sq[q1] = qp;
-  ss[q1] = sp;
-  r[q1] = fp;
-  double an = 0, bn = 0, cn = 0;
-  for (int k = 2; k <= n; ++k) {
-    coef_z<C>(g, f, sb, cre, cre2, i, j, k, cf);
-    const double pn = 1. / (cf.b + cf.a * qp);
-    qp = -cf.c * pn;
-    sp = -cf.a * sp * pn;
-    fp = (cf.g - cf.a * fp) * pn;
-    const long q = P(g, i, j, k);
+  const double fn = l.bd * r[q0 + n * st];
+  double qp, sp, fp;
+  {
+    const long q = q0 + st;
+    const double a = l.am[1] * (1. + l.cc * l.vis[q + l.lo * st]);
+    const double c = l.cm[1] * (1. + l.cc * l.vis[q + (l.lo + 1) * st]);
+    const double b = l.bd - a - c;
+    const double binv = 1. / b;
+    qp = -c * binv;
+    sp = -a * binv;
+    fp = (l.bd * r[q]) * binv;
     sq[q] = qp;
     ss[q] = sp;
     r[q] = fp;
-    an = cf.a;
-    bn = cf.b;
-    cn = cf.c;
+  }
+  double an = 0, bn = 0, cn = 0;
+  for (int kb = 2; kb <= n; kb += kAdiCh) {
+    double v[kAdiCh + 1], rv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u <= kAdiCh; ++u)
+      v[u] = kb + u <= n + 1 ? l.vis[q0 + (long)(kb + u + l.lo) * st] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) rv[u] = kb + u <= n ? r[q0 + (long)(kb + u) * st] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const int k = kb + u;
+      if (k > n) break;
+      const double a = l.am[k] * (1. + l.cc * v[u]);
+      const double c = l.cm[k] * (1. + l.cc * v[u + 1]);
+      const double b = l.bd - a - c;
+      const double gv = l.bd * rv[u];
+      const double pn = 1. / (b + a * qp);
+      qp = -c * pn;
+      sp = -a * sp * pn;
+      fp = (gv - a * fp) * pn;
+      const long q = q0 + (long)k * st;
+      sq[q] = qp;
+      ss[q] = sp;
+      r[q] = fp;
+      an = a;
+      bn = b;
+      cn = c;
+    }
   }
   // backward pass: s(n) = 1, qe(n) = 0; qe overwrites f in place
   double snext = 1.0, qenext = 0.0;
-  for (int k = n - 1; k >= 1; --k) {
-    const long q = P(g, i, j, k);
-    const double qk = sq[q];
-    const double sk = ss[q] + qk * snext;
-    const double qek = r[q] + qk * qenext;
-    ss[q] = sk;
-    r[q] = qek;
-    snext = sk;
-    qenext = qek;
+  for (int kb = n - 1; kb >= 1; kb -= kAdiCh) {
+    double qv[kAdiCh], sv[kAdiCh], rv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const long q = q0 + (long)(kb - u) * st;
+      qv[u] = kb - u >= 1 ? sq[q] : 0.0;
+      sv[u] = kb - u >= 1 ? ss[q] : 0.0;
+      rv[u] = kb - u >= 1 ? r[q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      if (kb - u < 1) break;
+      const long q = q0 + (long)(kb - u) * st;
+      const double sk = sv[u] + qv[u] * snext;
+      const double qek = rv[u] + qv[u] * qenext;
+      ss[q] = sk;
+      r[q] = qek;
+      snext = sk;
+      qenext = qek;
+    }
   }
+  const long q1 = q0 + st, qn1 = q0 + (long)(n - 1) * st;
   const double qe1 = r[q1], s1 = ss[q1];
-  const long qn1 = P(g, i, j, n - 1);
   const double qenm = r[qn1], snm = ss[qn1];
   const double fnn = (fn - cn * qe1 - an * qenm) / (cn * s1 + an * snm + bn);
-  r[P(g, i, j, n)] = fnn;
-  for (int k = n - 1; k >= 1; --k) {
-    const long q = P(g, i, j, k);
-    r[q] = fnn * ss[q] + r[q];
+  r[q0 + (long)n * st] = fnn;
+  for (int kb = n - 1; kb >= 1; kb -= kAdiCh) {
+    double sv[kAdiCh], rv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const long q = q0 + (long)(kb - u) * st;
+      sv[u] = kb - u >= 1 ? ss[q] : 0.0;
+      rv[u] = kb - u >= 1 ? r[q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      if (kb - u < 1) break;
+      r[q0 + (long)(kb - u) * st] = fnn * sv[u] + rv[u];
+    }
   }
 }
 
 // non-periodic y line (trdiag2, slv_mmtm.f90:1415-1455); result into rhs
 template <int C>
-__global__ void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo, int klo) {
+__global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
+                                               int klo) {
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int k = klo + blockIdx.y * 4 + threadIdx.y;
   if (i > g.n1m || k > g.n3m) return;
+  const LineCf l = line_cf<1, C>(g, f, sb, cre, cre2);
+  const long st = g.NX, q0 = P(g, i, 0, k);
   double *r = f.rhs[C];
   double *gm = f.s1;
-  Coef cf;
-  coef_y<C>(g, f, sb, cre, cre2, i, jlo, k, cf);
-  double bet = 1. / cf.b;
-  double up = cf.g * bet;
-  double cprev = cf.c;
-  r[P(g, i, jlo, k)] = up;
-  for (int j = jlo + 1; j <= g.n2m; ++j) {
-    coef_y<C>(g, f, sb, cre, cre2, i, j, k, cf);
-    const double gam = cprev * bet;
-    bet = 1. / (cf.b - cf.a * gam);
-    up = (cf.g - cf.a * up) * bet;
-    const long q = P(g, i, j, k);
-    gm[q] = gam;
-    r[q] = up;
-    cprev = cf.c;
-  }
-  double unext = up;
-  for (int j = g.n2m - 1; j >= jlo; --j) {
-    const long q = P(g, i, j, k);
-    const double uu = r[q] - gm[q + g.NX] * unext;
-    r[q] = uu;
-    unext = uu;
+  double unext = line_forward(l, r, gm, q0, st, jlo, g.n2m);
+  for (int jb = g.n2m - 1; jb >= jlo; jb -= kAdiCh) {
+    double rv[kAdiCh], gv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const long q = q0 + (long)(jb - u) * st;
+      rv[u] = jb - u >= jlo ? r[q] : 0.0;
+      gv[u] = jb - u >= jlo ? gm[q + st] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      if (jb - u < jlo) break;
+      const double uu = rv[u] - gv[u] * unext;
+      r[q0 + (long)(jb - u) * st] = uu;
+      unext = uu;
+    }
   }
 }
 
 // non-periodic x line (trdiag1, slv_mmtm.f90:1365-1412); field += solution
 template <int C>
-__global__ void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo, int klo) {
+__global__ __launch_bounds__(256) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
+                                               int klo) {
   const int j = jlo + blockIdx.x * 64 + threadIdx.x;
   const int k = klo + blockIdx.y * 4 + threadIdx.y;
   if (j > g.n2m || k > g.n3m) return;
+  const LineCf l = line_cf<0, C>(g, f, sb, cre, cre2);
   double *fld = C == 0 ? f.u : C == 1 ? f.v : C == 2 ? f.w : f.t;
+  const long q0 = P(g, 0, j, k);
   double *r = f.rhs[C];
   double *gm = f.s1;
-  Coef cf;
-  coef_x<C>(g, f, sb, cre, cre2, ilo, j, k, cf);
-  double bet = 1. / cf.b;
-  double up = cf.g * bet;
-  double cprev = cf.c;
-  r[P(g, ilo, j, k)] = up;
-  for (int i = ilo + 1; i <= g.n1m; ++i) {
-    coef_x<C>(g, f, sb, cre, cre2, i, j, k, cf);
-    const double gam = cprev * bet;
-    bet = 1. / (cf.b - cf.a * gam);
-    up = (cf.g - cf.a * up) * bet;
-    const long q = P(g, i, j, k);
-    gm[q] = gam;
-    r[q] = up;
-    cprev = cf.c;
-  }
-  double unext = up;
-  long q = P(g, g.n1m, j, k);
-  fld[q] = unext + fld[q];
-  for (int i = g.n1m - 1; i >= ilo; --i) {
-    q = P(g, i, j, k);
-    const double uu = r[q] - gm[q + 1] * unext;
-    fld[q] = uu + fld[q];
-    unext = uu;
+  double unext = line_forward(l, r, gm, q0, 1, ilo, g.n1m);
+  fld[q0 + g.n1m] = unext + fld[q0 + g.n1m];
+  for (int ib = g.n1m - 1; ib >= ilo; ib -= kAdiCh) {
+    double rv[kAdiCh], gv[kAdiCh], fv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      const long q = q0 + ib - u;
+      rv[u] = ib - u >= ilo ? r[q] : 0.0;
+      gv[u] = ib - u >= ilo ? gm[q + 1] : 0.0;
+      fv[u] = ib - u >= ilo ? fld[q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      if (ib - u < ilo) break;
+      const double uu = rv[u] - gv[u] * unext;
+      fld[q0 + ib - u] = uu + fv[u];
+      unext = uu;
+    }
   }
 }
 

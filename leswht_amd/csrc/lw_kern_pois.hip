@@ -217,12 +217,8 @@ __device__ inline Prolong make_prolong(const MG &mg, const double2 *gg, int ic, 
 // destinations of coarse rows t-1 (when t > 0) and t of level clev.
 // ---------------------------------------------------------------------------
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
-  const int lane = threadIdx.x;
-  const int t = blockIdx.x * 4 + threadIdx.y;
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  if (t >= nlines) return;
+__device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase, int t, double oldv, int clev,
+                                  int lane) {
   const int n2m = g.n2m;
   const int i = ibase + 2 * t;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
@@ -350,6 +346,16 @@ __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nline
   }
 }
 
+// one wavefront per line, four lines per workgroup, all wavenumbers
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
+  const int t = blockIdx.x * 4 + threadIdx.y;
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  if (t >= nlines) return;
+  line_solve<R, MODE>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
+}
+
 typedef void (*LineKernel)(Geo, MG, int, int, double, int);
 
 template <int MODE>
@@ -406,10 +412,7 @@ static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, in
 
 // going down (godown, poiss_z_ft.f90:438-458): residual of the fine rows
 // i0 = start + 2t and i0 + 1 in place, then the coarse row t from them
-__global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int t = blockIdx.x;
+__device__ inline void lev_rr(const Geo &g, const MG &mg, int kv, int ilev, int t, int jfirst, int jstep) {
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 *gg = mg.gg + pl;
   const double *acb = mg.ac + pl;
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
   double2 *ra = ROW(rsb, ia), *rb = ROW(rsb, ib), *rc = ROW(rsb, ic);
   const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
   const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
-  for (int j = 1 + threadIdx.x; j <= g.n2m; j += 256) {
+  for (int j = jfirst; j <= g.n2m; j += jstep) {
     double aw = awa, ae = aea;
     const double2 x = RESID_ROW(ra[j], ga, gam_[j], gb[j], aca, j);
     aw = awb;
@@ -431,6 +434,80 @@ __global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
     rb[j] = y;
     rc[j] = cadd(rmul(c1, x), rmul(c2, y));
   }
+}
+
+__global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  lev_rr(g, mg, kv, ilev, blockIdx.x, 1 + threadIdx.x, 256);
+}
+
+// ---------------------------------------------------------------------------
+// the coarse levels 0..lc of a V-cycle in one launch: one workgroup of
+// kCoarseWaves wavefronts per wavenumber runs, in the reference's order, the
+// down-sweep from level lc (whose right-hand side the previous launch
+// restricted), the NBLI sweeps of level 0 and the up-sweep back to level lc.
+// Each stage ends at a workgroup barrier; the levels are small enough that
+// the barrier is far cheaper than a kernel boundary.
+// ---------------------------------------------------------------------------
+constexpr int kCoarseWaves = 16;
+
+template <int R, int MODE>
+__device__ inline void wg_sweep(const Geo &g, const MG &mg, int kv, int ibase, int nlines, double oldv, int clev) {
+  for (int t = threadIdx.y; t < nlines; t += kCoarseWaves)
+    line_solve<R, MODE>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
+  __syncthreads();
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, int lc) {
+  const int kv = 1 + blockIdx.x;
+  if (mg.done[kv - 1]) return;
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  for (int ilev = lc; ilev >= 1; --ilev) {
+    const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, b = mg.iimg[ilev][1];
+    wg_sweep<R, 0>(g, mg, kv, b, (n + 1) / 2, 0.0, 0);
+    wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+    for (int q = tid; q < (n / 2) * g.n2m; q += 64 * kCoarseWaves)
+      lev_rr(g, mg, kv, ilev, q / g.n2m, 1 + q % g.n2m, g.n2m);
+    __syncthreads();
+  }
+  {
+    const int n = mg.iimg[0][2] - mg.iimg[0][1] + 1, b = mg.iimg[0][1];
+    for (int it = 0; it < mg.nbli; ++it) {
+      wg_sweep<R, 0>(g, mg, kv, b, (n + 1) / 2, it == 0 ? 0.0 : 1.0, 0);
+      wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+    }
+  }
+  for (int ilev = 0; ilev < lc; ++ilev) {
+    const int n = mg.iimg[ilev + 1][2] - mg.iimg[ilev + 1][1] + 1, b = mg.iimg[ilev + 1][1];
+    wg_sweep<R, 4>(g, mg, kv, b, (n + 1) / 2, 1.0, ilev);
+    wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+  }
+}
+
+typedef void (*CoarseKernel)(Geo, MG, int);
+
+static CoarseKernel pick_coarse(int R) {
+  switch (R) {
+    case 1: return k_mg_coarse<1>;
+    case 2: return k_mg_coarse<2>;
+    case 4: return k_mg_coarse<4>;
+    case 8: return k_mg_coarse<8>;
+    default: return k_mg_coarse<16>;
+  }
+}
+
+static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
+  // per level and cycle: down and up sweeps 4 x 64 B, residual+restriction
+  // 64 B, per point; level 0: NBLI x 2 x 64 B
+  double per_kv = 0.0;
+  for (int l = 0; l <= mg.lc; ++l) {
+    const double pts = (double)(mg.iimg[l][2] - mg.iimg[l][1] + 1) * g.n2m;
+    per_kv += l == 0 ? pts * 64.0 * mg.nbli : pts * (64.0 * 2 + 64.0);
+  }
+  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m)), dim3((unsigned)mg.n3mh), dim3(64, kCoarseWaves),
+            0, s, g, mg, mg.lc);
 }
 
 // top level (toplevel, poiss_z_ft.f90:339-380): rows ia = 2t+1 (odd) and
@@ -535,21 +612,24 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
 
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
   auto nrow = [&](int l) { return mg.iimg[l][2] - mg.iimg[l][1] + 1; };
-  for (int ilev = mg.nlev - 1; ilev >= 1; --ilev) {
+  // levels above lc: one launch per zebra half, one per residual+restriction
+  for (int ilev = mg.nlev - 1; ilev >= 1 && ilev > mg.lc; --ilev) {
     const int n = nrow(ilev);
     launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
     launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, 0);
     // residual in/out 32, solution 16, ac 8 per fine point; coarse row 8 per fine point
     LW_LAUNCH("k_lev_rr", 0, 64.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
   }
-  {
+  if (mg.lc >= 0) {
+    launch_coarse(s, g, mg);
+  } else {
     const int n = nrow(0);
     for (int it = 0; it < mg.nbli; ++it) {
       launch_lines(s, g, mg, 0, mg.iimg[0][1], (n + 1) / 2, it == 0 ? 0.0 : 1.0, 0);
       launch_lines(s, g, mg, 1, mg.iimg[0][1] + 1, n / 2, 1.0, 0);
     }
   }
-  for (int ilev = 0; ilev <= mg.nlev - 2; ++ilev) {
+  for (int ilev = mg.lc < 0 ? 0 : mg.lc; ilev <= mg.nlev - 2; ++ilev) {
     const int n = nrow(ilev + 1);
     launch_lines(s, g, mg, 4, mg.iimg[ilev + 1][1], (n + 1) / 2, 1.0, ilev);
     launch_lines(s, g, mg, 1, mg.iimg[ilev + 1][1] + 1, n / 2, 1.0, 0);

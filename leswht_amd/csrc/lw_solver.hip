@@ -7,6 +7,7 @@
 // reference's expressions and uploaded.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -469,6 +470,13 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
       }
       coi1[iimg[ilev][2]] = 1.0;
     }
+    // coarse levels handled by one workgroup per wavenumber: up to 24 rows
+    // (LW_MG_COARSE_ROWS overrides the limit; 0 launches every level apart)
+    int coarse_rows = 24;
+    if (const char *e = getenv("LW_MG_COARSE_ROWS")) coarse_rows = atoi(e);
+    mg.lc = -1;
+    for (int ilev = 0; ilev <= nlev - 1; ++ilev)
+      if (iimg[ilev][3] <= coarse_rows) mg.lc = ilev;
     for (int ilev = nlev; ilev >= 0; --ilev)
       for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) {
         ipm[i] = i + 1;
