@@ -73,7 +73,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("LESWHT_CPU_THREADS", "16")))
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB five times (the known-size "
+                         "calibration of the FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.rand(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": n * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU time step "
               "(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)

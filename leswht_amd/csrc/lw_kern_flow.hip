@@ -1181,4 +1181,131 @@ void launch_lagforce(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, I
   LW_LAUNCH("k_lagforce_sum", 3.0 * kRedBlocks * 8, 0, k_lagforce_sum, dim3(1), dim3(256), 0, s, sb, f);
 }
 
+
+// ---------------------------------------------------------------------------
+// per-step diagnostics of the driver loop (solver.f90:158-172): the IB and
+// wall-shear parts of draglift (slv_auxi.f90:669-840), the boundary fluxes of
+// heatflux (slv_auxi.f90:573-666) and the bulk temperatures of writehistory
+// (misc_hist.f90:73-181). Each sum s is reduced by kDiagB workgroups into
+// block partials, then one workgroup sums every s in one fixed order into
+// red[kDiagRed + s]. The fluid tests of the boundary planes come from masks
+// computed on the host with the reference's funcbody test.
+// ---------------------------------------------------------------------------
+constexpr int kDiagB = 128;
+
+__device__ inline double diag_term(const Geo &g, const Fld &f, const DiagPlanes &dp, const IBComp *ib, int s,
+                                   long q, double re) {
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  if (s <= D_IBW) {  // -fcvavg * u-cell volume (the reference uses c2cx for all three)
+    const IBComp &c = ib[s];
+    const int ii = c.ifc[q], jj = c.jfc[q], kk = c.kfc[q];
+    return c.fcvavg[q] * (g.c2cx[ii] * g.f2fy[jj] * g.f2fz[kk]);
+  }
+  if (s <= D_YT_W) {  // y walls: one-sided second-order gradient at the wall
+    const bool top = s >= D_YT_U;
+    const int i = 1 + (int)(q % n1m), k = 1 + (int)(q / n1m);
+    if (!(top ? dp.wyt : dp.wyb)[i + (long)(n1 + 1) * k]) return 0.0;
+    const double dy1 = top ? 1.0 / g.c2cyi[n2] : 1.0 / g.c2cyi[1];
+    const double dy2 = top ? dy1 + 1.0 / g.c2cyi[n2m] : dy1 + 1.0 / g.c2cyi[2];
+    const int j1 = top ? n2m : 1, j2 = top ? n2m - 1 : 2;
+    const double *a = (s == D_YB_U || s == D_YT_U) ? f.u : f.w;
+    const double grad = (a[P(g, i, j1, k)] * (dy2 * dy2) - a[P(g, i, j2, k)] * (dy1 * dy1)) / (dy1 * dy2 * (dy2 - dy1));
+    const double tmp = grad * g.c2cx[i] * g.f2fz[k];
+    return tmp / re;
+  }
+  if (s <= D_ZT_V) {  // z walls
+    const bool top = s >= D_ZT_U;
+    const int i = 1 + (int)(q % n1m), j = 1 + (int)(q / n1m);
+    if (!(top ? dp.wzt : dp.wzb)[i + (long)(n1 + 1) * j]) return 0.0;
+    const double dz1 = top ? 1.0 / g.c2czi[n3] : 1.0 / g.c2czi[1];
+    const double dz2 = top ? dz1 + 1.0 / g.c2czi[n3m] : dz1 + 1.0 / g.c2czi[2];
+    const int k1 = top ? n3m : 1, k2 = top ? n3m - 1 : 2;
+    const double *a = (s == D_ZB_U || s == D_ZT_U) ? f.u : f.v;
+    const double grad = (a[P(g, i, j, k1)] * (dz2 * dz2) - a[P(g, i, j, k2)] * (dz1 * dz1)) / (dz1 * dz2 * (dz2 - dz1));
+    const double tmp = grad * g.c2cx[i] * g.f2fy[j];
+    return tmp / re;
+  }
+  if (s <= D_HXP) {  // x faces: u * T at the face * area
+    const bool hi = s == D_HXP;
+    const int j = 1 + (int)(q % n2m), k = 1 + (int)(q / n2m);
+    if (!(hi ? dp.hxp : dp.hxm)[j + (long)(n2 + 1) * k]) return 0.0;
+    const double area = g.f2fy[j] * g.f2fz[k];
+    const double tface = hi ? 0.5 * (f.t[P(g, n1m, j, k)] + f.t[P(g, n1, j, k)])
+                            : 0.5 * (f.t[P(g, 1, j, k)] + f.t[P(g, 0, j, k)]);
+    return f.u[P(g, hi ? n1 : 1, j, k)] * tface * area;
+  }
+  if (s <= D_HYP) {  // y faces
+    const bool hi = s == D_HYP;
+    const int i = 1 + (int)(q % n1m), k = 1 + (int)(q / n1m);
+    if (!(hi ? dp.hyp : dp.hym)[i + (long)(n1 + 1) * k]) return 0.0;
+    const double area = g.f2fx[i] * g.f2fz[k];
+    const double tface = hi ? 0.5 * (f.t[P(g, i, n2m, k)] + f.t[P(g, i, n2, k)])
+                            : 0.5 * (f.t[P(g, i, 1, k)] + f.t[P(g, i, 0, k)]);
+    return f.v[P(g, i, hi ? n2 : 1, k)] * tface * area;
+  }
+  if (s <= D_HZP) {  // z faces
+    const bool hi = s == D_HZP;
+    const int i = 1 + (int)(q % n1m), j = 1 + (int)(q / n1m);
+    if (!(hi ? dp.hzp : dp.hzm)[i + (long)(n1 + 1) * j]) return 0.0;
+    const double area = g.f2fx[i] * g.f2fy[j];
+    const double tface = hi ? 0.5 * (f.t[P(g, i, j, n3m)] + f.t[P(g, i, j, n3)])
+                            : 0.5 * (f.t[P(g, i, j, 1)] + f.t[P(g, i, j, 0)]);
+    return f.w[P(g, i, j, hi ? n3 : 1)] * tface * area;
+  }
+  // bulk temperatures: fluid / solid by the cell-centre test, u-cell volume
+  const int i = 1 + (int)(q % n1m), j = 1 + (int)((q / n1m) % n2m), k = 1 + (int)(q / ((long)n1m * n2m));
+  const long c = P(g, i, j, k);
+  const bool solid = f.mask[c] & M_SOLID_C;
+  const double cellvol = g.c2cx[i] * g.f2fy[j] * g.f2fz[k];
+  if (s == D_TFV) return solid ? 0.0 : cellvol;
+  if (s == D_TFI) return solid ? 0.0 : f.t[c] * cellvol;
+  if (s == D_TSV) return solid ? cellvol : 0.0;
+  return solid ? f.t[c] * cellvol : 0.0;
+}
+
+__device__ inline long diag_size(const Geo &g, const IBComp *ib, int s) {
+  if (s <= D_IBW) return ib[s].nbody;
+  if (s <= D_ZT_V) return (long)g.n1m * (s <= D_YT_W ? g.n3m : g.n2m);
+  if (s <= D_HXP) return (long)g.n2m * g.n3m;
+  if (s <= D_HYP) return (long)g.n1m * g.n3m;
+  if (s <= D_HZP) return (long)g.n1m * g.n2m;
+  return (long)g.n1m * g.n2m * g.n3m;
+}
+
+__global__ __launch_bounds__(256) void k_step_diag(Geo g, Fld f, IBComp u, IBComp v, IBComp w, DiagPlanes dp,
+                                                   unsigned active, double re) {
+  __shared__ double sh[4];
+  const int s = blockIdx.y;
+  if (!((active >> s) & 1u)) return;
+  const IBComp ib[3] = {u, v, w};
+  const long n = diag_size(g, ib, s);
+  double a = 0.0;
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)kDiagB * 256) a += diag_term(g, f, dp, ib, s, q, re);
+  const double r = block_sum<256>(a, sh);
+  if (threadIdx.x == 0) f.dpart[s * kDiagB + blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_step_diag_sum(Fld f, unsigned active) {
+  __shared__ double sh[4];
+  for (int s = 0; s < D_NSUM; ++s) {
+    if (!((active >> s) & 1u)) continue;
+    const double v = threadIdx.x < kDiagB ? f.dpart[s * kDiagB + threadIdx.x] : 0.0;
+    const double r = block_sum<256>(v, sh);
+    if (threadIdx.x == 0) f.red[kDiagRed + s] = r;
+  }
+}
+
+void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, const DiagPlanes &dp, unsigned active,
+                      double re) {
+  if (!active) return;
+  double bytes = 0.0;
+  if (g_prof)
+    for (int q = 0; q < D_NSUM; ++q)
+      if ((active >> q) & 1u) bytes += (q <= D_IBW ? 36.0 : q >= D_TFV ? 10.0 : 32.0) *
+                                       (q <= D_IBW ? (double)ib[q].nbody : q >= D_TFV ? pts(g) : 1e4);
+  LW_LAUNCH("k_step_diag", bytes, 0, k_step_diag, dim3(kDiagB, D_NSUM), dim3(256), 0, s, g, f, ib[0], ib[1], ib[2], dp,
+            active, re);
+  LW_LAUNCH("k_step_diag_sum", D_NSUM * kDiagB * 8.0, 0, k_step_diag_sum, dim3(1), dim3(256), 0, s, f, active);
+}
+
 }  // namespace lw

@@ -1,26 +1,33 @@
 // Kernel launchers of the LESwHT time step (gfx950). Each launcher enqueues
 // on the given stream; all arrays are device pointers.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "lw_internal.h"
 
 namespace lw {
 
 // ---- per-launch event profiler (lw_solver.hip) ----
 // While a profile is active on the calling thread, every launch made through
-// LW_LAUNCH of a selected kernel is bracketed by two events on its stream and
-// recorded with its algorithmic HBM bytes: BYTES, plus BYTES_KV times the
-// number of wavenumbers still iterating in the multigrid cycle it belongs to.
-// BYTES and BYTES_KV are evaluated only while profiling.
+// LW_LAUNCH of a selected kernel is launched with hipExtLaunchKernelGGL, whose
+// start and stop events take the dispatch's own begin and end timestamps on
+// its stream, and is recorded with its algorithmic HBM bytes: BYTES, plus
+// BYTES_KV times the number of wavenumbers still iterating in the multigrid
+// cycle it belongs to. BYTES and BYTES_KV are evaluated only while profiling.
 struct Prof;
 extern thread_local Prof *g_prof;
-bool prof_pre(const char *name, hipStream_t s);
-void prof_post(const char *name, hipStream_t s, double bytes, double bytes_kv);
+bool prof_take(const char *name, hipEvent_t *a, hipEvent_t *b);
+void prof_record(const char *name, hipEvent_t a, hipEvent_t b, double bytes, double bytes_kv);
 
-#define LW_LAUNCH(NAME, BYTES, BYTES_KV, KERNEL, GRID, BLOCK, SHM, STREAM, ...)       \
-  do {                                                                                \
-    const bool lw_p_ = ::lw::g_prof != nullptr && ::lw::prof_pre(NAME, STREAM);       \
-    hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                \
-    if (lw_p_) ::lw::prof_post(NAME, STREAM, (double)(BYTES), (double)(BYTES_KV));    \
+#define LW_LAUNCH(NAME, BYTES, BYTES_KV, KERNEL, GRID, BLOCK, SHM, STREAM, ...)                      \
+  do {                                                                                               \
+    hipEvent_t lw_a_ = nullptr, lw_b_ = nullptr;                                                     \
+    if (::lw::g_prof != nullptr && ::lw::prof_take(NAME, &lw_a_, &lw_b_)) {                          \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, lw_a_, lw_b_, 0u, __VA_ARGS__);        \
+      ::lw::prof_record(NAME, lw_a_, lw_b_, (double)(BYTES), (double)(BYTES_KV));                    \
+    } else {                                                                                         \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                             \
+    }                                                                                                \
   } while (0)
 
 struct Fld {
@@ -36,6 +43,7 @@ struct Fld {
   // reduction scratch: red[0..63] doubles, partials[...] per block
   double *red, *partial;  // partial: 4 * kRedBlocks
   double *obc;            // outlet-plane reduction partials (8 * 64)
+  double *dpart;          // per-step diagnostic partials (D_NSUM * 128)
   unsigned long long *redmax;  // [0] cfl, [1] dvmax, [2] qmmax
 };
 
@@ -64,6 +72,23 @@ void launch_energy_bcs(hipStream_t s, const Geo &g, const Fld &f, const Body &b,
 void launch_cfl(hipStream_t s, const Geo &g, const Fld &f, double dt);
 void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson);
 void launch_lagforce(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib);
+
+// per-step diagnostics: sums s of k_step_diag, results in red[kDiagRed + s]
+enum {
+  D_IBU = 0, D_IBV, D_IBW,                                  // IB forcing integrals (draglift)
+  D_YB_U, D_YB_W, D_YT_U, D_YT_W, D_ZB_U, D_ZB_V, D_ZT_U, D_ZT_V,  // wall shear (draglift)
+  D_HXM, D_HXP, D_HYM, D_HYP, D_HZM, D_HZP,                // boundary heat fluxes (heatflux)
+  D_TFV, D_TFI, D_TSV, D_TSI,                              // bulk temperatures (writehistory)
+  D_NSUM
+};
+constexpr int kDiagRed = 32;
+// fluid tests of the boundary planes (1 = funcbody >= 1e-10): walls indexed
+// i + (n1+1)*k or i + (n1+1)*j, x faces j + (n2+1)*k
+struct DiagPlanes {
+  const uint8_t *wyb, *wyt, *wzb, *wzt, *hxm, *hxp, *hym, *hyp, *hzm, *hzp;
+};
+void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, const DiagPlanes &dp, unsigned active,
+                      double re);
 
 // ---- subgrid scale (lw_kern_sgs.hip) ----
 struct SgsBuf {

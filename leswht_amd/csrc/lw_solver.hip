@@ -58,7 +58,6 @@ struct Prof {
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
   size_t used = 0;
-  hipEvent_t pending = nullptr;
   int cycle = -1;      // multigrid cycle being enqueued (0: the first toplevel)
   size_t mg_first = 0; // first record of the Poisson solve being enqueued
 
@@ -120,17 +119,15 @@ struct Prof {
 
 thread_local Prof *g_prof = nullptr;
 
-bool prof_pre(const char *name, hipStream_t s) {
+bool prof_take(const char *name, hipEvent_t *a, hipEvent_t *b) {
   if (!g_prof->selected(name)) return false;
-  g_prof->pending = g_prof->take();
-  HIPCHK(hipEventRecord(g_prof->pending, s));
+  *a = g_prof->take();
+  *b = g_prof->take();
   return true;
 }
 
-void prof_post(const char *name, hipStream_t s, double bytes, double bytes_kv) {
-  hipEvent_t b = g_prof->take();
-  HIPCHK(hipEventRecord(b, s));
-  g_prof->recs.push_back(ProfRec{g_prof->cls(name), g_prof->cycle, bytes, bytes_kv, g_prof->pending, b});
+void prof_record(const char *name, hipEvent_t a, hipEvent_t b, double bytes, double bytes_kv) {
+  g_prof->recs.push_back(ProfRec{g_prof->cls(name), g_prof->cycle, bytes, bytes_kv, a, b});
 }
 }  // namespace lw
 
