@@ -91,6 +91,27 @@ typedef struct lw_step_stats {
   double ms_sgs, ms_momentum, ms_poisson, ms_energy; /* device time, ms */
 } lw_step_stats;
 
+/* the per-step history quantities of the driver loop (solver.f90:150-172),
+ * for the step lw_step completed last:
+ *   fhist.dat  time dt cflmax dvmax qmmax      writehistory, misc_hist.f90:113
+ *   ftemp.dat  time tbulk[0..1] tbulk_avg[0..1] (IHTRANS; on history steps)
+ *                                              misc_hist.f90:145-178
+ *   fcdcl.dat  time cd[0..2] cd_avg[0..2]       draglift, slv_auxi.f90:669-840
+ *   fheatflux.dat time hflux[0..2] hflux_avg[0..2] heatflux, slv_auxi.f90:573-666
+ *   ftime.dat  the device seconds of the step's phases (the reference writes
+ *              its own wall clock, misc_hist.f90:255-284)
+ * written = 1 when mod(ntime, NPIN) = 0: the reference then advances ihist
+ * and writes the fhist/ftemp/fcdcl/fheatflux rows. Averages run from
+ * AVG_TST on, as in the reference, and are 0 before it. */
+typedef struct lw_history {
+  int64_t ntime, ihist, written;
+  double time, dt, cflmax, dvmax, qmmax;
+  double tbulk[2], tbulk_avg[2];
+  double cd[3], cd_avg[3];
+  double hflux[3], hflux_avg[3];
+  double s_sgs, s_rhsnlhs, s_poisson, s_energy;
+} lw_history;
+
 typedef struct lw_handle lw_handle;
 
 /* body level set used inside the step (convbc, nutzero, wallbc_t):
@@ -123,6 +144,11 @@ int lw_write_fld(const char *path, const int64_t n[3], double re, double pr, dou
 
 /* sizes: n[0..2] = n1, n2, n3 */
 int lw_get_dims(const lw_handle *h, int64_t n[3]);
+/* history quantities of the last completed step (see lw_history) */
+int lw_get_history(const lw_handle *h, lw_history *out);
+/* the settings the handle runs with (settings.input + boundary.input +
+ * ibmpre_prdic.bin as readsettings/readbcs hold them, misc_init.f90:19-158) */
+int lw_get_settings(const lw_handle *h, lw_settings *out);
 /* named internal array for phase checks ("nusgs", "nusgs1_1".."nusgs1_3",
  * "alsgs", "alsgs1_1".."alsgs1_3", "divgsum", "phi", "qmass");
  * n = capacity in doubles. Returns the element count or a negative code. */

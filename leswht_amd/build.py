@@ -1,4 +1,7 @@
-"""Builds the gfx950 library ``leswht_amd/_lib/libleswht.so`` with hipcc.
+"""Builds the gfx950 library ``leswht_amd/_lib/libleswht.so`` with hipcc, and
+the Fortran host ``leswht_amd/_lib/solver_exec`` (the drop-in for the
+reference executable, calling the library through ISO_C_BINDING) with
+amdflang when it is installed.
 
 FP64 parity with the reference needs IEEE evaluation in the written order,
 so every translation unit is compiled with -ffp-contract=off (no FMA
@@ -20,6 +23,9 @@ HIP_SOURCES = ["lw_solver.hip", "lw_kern_flow.hip", "lw_kern_sgs.hip", "lw_kern_
 CXX_SOURCES = ["lw_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("LESWHT_FC", "/opt/rocm/bin/amdflang")
+HOST = os.path.join(HERE, "host")
+EXE = os.path.join(OUT_DIR, "solver_exec")
 ARCH = os.environ.get("LESWHT_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
@@ -64,7 +70,28 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str | None:
+    """The Fortran solver_exec, linked against libleswht.so with an $ORIGIN
+    rpath so the pair can move together. Skipped when amdflang is absent."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("amdflang not found: solver_exec not built")
+        return None
+    mod_dir = os.path.join(OUT_DIR, "obj", "fmod")
+    os.makedirs(mod_dir, exist_ok=True)
+    tmp = EXE + ".tmp"
+    cmd = [FC, "-O2", "-module-dir", mod_dir,
+           os.path.join(HOST, "leswht_c.f90"), os.path.join(HOST, "solver_exec.f90"),
+           "-o", tmp, "-L" + OUT_DIR, "-lleswht", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, EXE)
+    return EXE
 
 
 if __name__ == "__main__":

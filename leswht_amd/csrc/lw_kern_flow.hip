@@ -1308,4 +1308,85 @@ void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, con
   LW_LAUNCH("k_step_diag_sum", D_NSUM * kDiagB * 8.0, 0, k_step_diag_sum, dim3(1), dim3(256), 0, s, f, active);
 }
 
+
+// the step's history values from the sums (draglift, slv_auxi.f90:700-840;
+// heatflux, 636-666; writehistory, misc_hist.f90:145-178) and their running
+// time averages from avg_tst on. hist: [0..2] cd, [3..5] cdavg, [6..8] heat
+// flux, [9..11] its average, [12..13] bulk T fluid/solid, [14..15] their
+// averages, [16..19] cd integral and duration, [20..23] heat-flux integral and
+// duration, [24..26] bulk-T integral and duration.
+__global__ void k_hist_update(Fld f, double *hist, unsigned active, int ibmon, int heat, int hstep, double time,
+                              double dt, double avg_tst, double vol) {
+  if (threadIdx.x != 0) return;
+  const double *S = f.red + kDiagRed;
+  const bool avg = time >= avg_tst;
+  if (ibmon) {
+    double cd[3] = {-S[D_IBU], -S[D_IBV], -S[D_IBW]};
+    for (int l = 0; l < 3; ++l) cd[l] = cd[l] + f.red[8 + l];
+    if ((active >> D_YB_U) & 1u) {
+      cd[0] = cd[0] + S[D_YB_U];
+      cd[2] = cd[2] + S[D_YB_W];
+    }
+    if ((active >> D_YT_U) & 1u) {
+      cd[0] = cd[0] + S[D_YT_U];
+      cd[2] = cd[2] + S[D_YT_W];
+    }
+    if ((active >> D_ZB_U) & 1u) {
+      cd[0] = cd[0] + S[D_ZB_U];
+      cd[1] = cd[1] + S[D_ZB_V];
+    }
+    if ((active >> D_ZT_U) & 1u) {
+      cd[0] = cd[0] + S[D_ZT_U];
+      cd[1] = cd[1] + S[D_ZT_V];
+    }
+    for (int l = 0; l < 3; ++l) {
+      cd[l] = cd[l] / vol;
+      hist[l] = cd[l];
+      hist[3 + l] = 0.0;
+    }
+    if (avg) {
+      for (int l = 0; l < 3; ++l) hist[16 + l] = hist[16 + l] + cd[l] * dt;
+      hist[19] = hist[19] + dt;
+      if (hist[19] > 0.0)
+        for (int l = 0; l < 3; ++l) hist[3 + l] = hist[16 + l] / hist[19];
+    }
+  }
+  if (heat) {
+    const double hf[3] = {S[D_HXP] - S[D_HXM], S[D_HYP] - S[D_HYM], S[D_HZP] - S[D_HZM]};
+    for (int l = 0; l < 3; ++l) {
+      hist[6 + l] = hf[l];
+      hist[9 + l] = 0.0;
+    }
+    if (avg) {
+      for (int l = 0; l < 3; ++l) hist[20 + l] = hist[20 + l] + hf[l] * dt;
+      hist[23] = hist[23] + dt;
+      if (hist[23] > 0.0)
+        for (int l = 0; l < 3; ++l) hist[9 + l] = hist[20 + l] / hist[23];
+    }
+    if (hstep) {
+      const double tf = S[D_TFV] > 0.0 ? S[D_TFI] / S[D_TFV] : 0.0;
+      const double ts = S[D_TSV] > 0.0 ? S[D_TSI] / S[D_TSV] : 0.0;
+      hist[12] = tf;
+      hist[13] = ts;
+      hist[14] = 0.0;
+      hist[15] = 0.0;
+      if (avg) {
+        hist[24] = hist[24] + tf * dt;
+        hist[25] = hist[25] + ts * dt;
+        hist[26] = hist[26] + dt;
+        if (hist[26] > 0.0) {
+          hist[14] = hist[24] / hist[26];
+          hist[15] = hist[25] / hist[26];
+        }
+      }
+    }
+  }
+}
+
+void launch_hist_update(hipStream_t s, const Fld &f, double *hist, unsigned active, int ibmon, int heat, int hstep,
+                        double time, double dt, double avg_tst, double vol) {
+  LW_LAUNCH("k_hist_update", 0, 0, k_hist_update, dim3(1), dim3(64), 0, s, f, hist, active, ibmon, heat, hstep, time,
+            dt, avg_tst, vol);
+}
+
 }  // namespace lw

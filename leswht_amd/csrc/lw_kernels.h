@@ -89,6 +89,8 @@ struct DiagPlanes {
 };
 void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, const DiagPlanes &dp, unsigned active,
                       double re);
+void launch_hist_update(hipStream_t s, const Fld &f, double *hist, unsigned active, int ibmon, int heat, int hstep,
+                        double time, double dt, double avg_tst, double vol);
 
 // ---- subgrid scale (lw_kern_sgs.hip) ----
 struct SgsBuf {

@@ -159,10 +159,16 @@ struct lw_handle {
   int64_t cyc_max = 0, cyc_sum = 0;
   Sub sub;
   hipEvent_t ev[8];
-  double ms[4] = {0, 0, 0, 0};
   unsigned long long *h_sumres = nullptr;  // pinned
   int *h_flags = nullptr;                  // pinned copy of mg.done / mg.cyc
   int mg_hint = 4;                         // V-cycles of the previous Poisson solve
+  DiagPlanes dplanes{};
+  double *hist = nullptr;  // device: the step's history values and their running averages
+  double h_hist[32] = {0};
+  int hist_written = 0;    // the last step was a history step (mod(ntime, npin) = 0)
+  hipEvent_t tev[3][8] = {};  // phase boundaries per substep: sgs, rhsnlhs, divgs+poisson, energy
+  double s_phase[4] = {0, 0, 0, 0};
+  bool tev_valid = false;  // tev hold a whole step recorded by lw_step
   Prof *prof = nullptr;
   double *h_red = nullptr;                 // pinned
 
@@ -318,6 +324,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.red = h->dalloc<double>(64);
   f.partial = h->dalloc<double>(4 * kRedBlocks);
   f.obc = h->dalloc<double>(8 * 64);
+  f.dpart = h->dalloc<double>(D_NSUM * 128);
+  h->hist = h->dalloc<double>(32);
   f.redmax = h->dalloc<unsigned long long>(8);
   if (h->les) {
     for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
@@ -370,6 +378,44 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     for (int j = 0; j <= n2; ++j)
       if (funcbody(h->body, h->xmp[0], h->ymp[j], h->zmp[k]) <= 1.e-10) mask[PIh(0, j, k)] |= 128;
   f.mask = h->upload(mask);
+  // fluid tests of the boundary planes used by draglift's wall terms and by
+  // heatflux (slv_auxi.f90:590-640, 733-829): funcbody >= 1e-10
+  {
+    auto fl = [&](double xx, double yy, double zz) -> uint8_t {
+      return funcbody(h->body, xx, yy, zz) >= 1.e-10 ? 1 : 0;
+    };
+    const std::vector<double> &X = h->x;
+    std::vector<double> Y = vd(gr->y, 1, n2), Z = vd(gr->z, 1, n3);
+    std::vector<uint8_t> wyb((size_t)(n1 + 1) * (n3 + 1)), wyt(wyb.size()), hym(wyb.size()), hyp(wyb.size());
+    std::vector<uint8_t> wzb((size_t)(n1 + 1) * (n2 + 1)), wzt(wzb.size()), hzm(wzb.size()), hzp(wzb.size());
+    std::vector<uint8_t> hxm((size_t)(n2 + 1) * (n3 + 1)), hxp(hxm.size());
+    for (int k = 1; k <= n3m; ++k)
+      for (int i = 1; i <= n1m; ++i) {
+        const size_t q = i + (size_t)(n1 + 1) * k;
+        wyb[q] = fl(X[i], h->ymp[1], h->zmp[k]);
+        wyt[q] = fl(X[i], h->ymp[n2m], h->zmp[k]);
+        hym[q] = fl(h->xmp[i], Y[1], h->zmp[k]);
+        hyp[q] = fl(h->xmp[i], Y[n2], h->zmp[k]);
+      }
+    for (int j = 1; j <= n2m; ++j)
+      for (int i = 1; i <= n1m; ++i) {
+        const size_t q = i + (size_t)(n1 + 1) * j;
+        wzb[q] = fl(X[i], h->ymp[j], h->zmp[1]);
+        wzt[q] = fl(X[i], h->ymp[j], h->zmp[n3m]);
+        hzm[q] = fl(h->xmp[i], h->ymp[j], Z[1]);
+        hzp[q] = fl(h->xmp[i], h->ymp[j], Z[n3]);
+      }
+    for (int k = 1; k <= n3m; ++k)
+      for (int j = 1; j <= n2m; ++j) {
+        const size_t q = j + (size_t)(n2 + 1) * k;
+        hxm[q] = fl(X[1], h->ymp[j], h->zmp[k]);
+        hxp[q] = fl(X[n1], h->ymp[j], h->zmp[k]);
+      }
+    DiagPlanes &dp = h->dplanes;
+    dp.wyb = h->upload(wyb); dp.wyt = h->upload(wyt); dp.wzb = h->upload(wzb); dp.wzt = h->upload(wzt);
+    dp.hxm = h->upload(hxm); dp.hxp = h->upload(hxp); dp.hym = h->upload(hym); dp.hyp = h->upload(hyp);
+    dp.hzm = h->upload(hzm); dp.hzp = h->upload(hzp);
+  }
 
   // ---- Poisson (poisinit, poiss_z_ft.f90:18-113 and coefmg 643-798) ----
   MG &mg = h->mg;
@@ -564,6 +610,8 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     h->csgshf = s->csgshf;
     HIPCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
+    for (int q = 0; q < 3; ++q)
+      for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->tev[q][e]));
     HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
     HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
@@ -587,6 +635,9 @@ int lw_destroy(lw_handle *h) {
   for (void *p : h->allocs) hipFree(p);
   for (int e = 0; e < 8; ++e)
     if (h->ev[e]) hipEventDestroy(h->ev[e]);
+  for (int q = 0; q < 3; ++q)
+    for (int e = 0; e < 8; ++e)
+      if (h->tev[q][e]) hipEventDestroy(h->tev[q][e]);
   if (h->h_sumres) hipHostFree(h->h_sumres);
   if (h->h_flags) hipHostFree(h->h_flags);
   if (h->h_red) hipHostFree(h->h_red);
@@ -831,15 +882,49 @@ static void phase_energy(lw_handle *h) {
   launch_energy_bcs(h->s, g, h->f, b, h->st);
 }
 
+// after the substeps (solver.f90:150-172): time update, convergence_check and
+// masscheck, the history counter, draglift, heatflux and the bulk
+// temperatures of writehistory
 static void end_of_step(lw_handle *h) {
+  const lw_settings &st = h->st;
   h->time = h->time + h->dt;
-  launch_diag(h->s, h->g, h->f, (int)(h->st.masson == 1 && h->st.ibmon == 1));
+  launch_diag(h->s, h->g, h->f, (int)(st.masson == 1 && st.ibmon == 1));
+  const int hstep = st.npin > 0 ? (h->ntime % st.npin) == 0 : 1;
+  if (hstep) h->ihist += 1;
+  h->hist_written = hstep;
+  unsigned act = 0;
+  if (st.ibmon == 1) {
+    act |= (1u << D_IBU) | (1u << D_IBV) | (1u << D_IBW);
+    if (st.bc_ybtm == 0) act |= (1u << D_YB_U) | (1u << D_YB_W);
+    if (st.bc_ytop == 0) act |= (1u << D_YT_U) | (1u << D_YT_W);
+    if (st.bc_zbtm == 0) act |= (1u << D_ZB_U) | (1u << D_ZB_V);
+    if (st.bc_ztop == 0) act |= (1u << D_ZT_U) | (1u << D_ZT_V);
+  }
+  if (h->heat) {
+    for (int q = D_HXM; q <= D_HZP; ++q) act |= 1u << q;
+    if (hstep)
+      for (int q = D_TFV; q <= D_TSI; ++q) act |= 1u << q;
+  }
+  launch_step_diag(h->s, h->g, h->f, h->ib, h->dplanes, act, st.re);
+  launch_hist_update(h->s, h->f, h->hist, act, st.ibmon == 1, h->heat, hstep, h->time, h->dt, st.avg_tst,
+                     h->g.xl * h->g.yl * h->g.zl);
 }
 
 static void read_stats(lw_handle *h, lw_step_stats *stats) {
   HIPCHK(hipMemcpyAsync(h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipMemcpyAsync(h->h_red, h->f.red, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(h->h_red + 16, h->hist, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
+  memcpy(h->h_hist, h->h_red + 16, 16 * sizeof(double));
+  for (int q = 0; q < 4; ++q) {
+    float a = 0.f, b = 0.f, c = 0.f;
+    if (h->tev_valid) {
+      HIPCHK(hipEventElapsedTime(&a, h->tev[0][2 * q], h->tev[0][2 * q + 1]));
+      HIPCHK(hipEventElapsedTime(&b, h->tev[1][2 * q], h->tev[1][2 * q + 1]));
+      HIPCHK(hipEventElapsedTime(&c, h->tev[2][2 * q], h->tev[2][2 * q + 1]));
+    }
+    h->s_phase[q] = ((double)a + (double)b + (double)c) * 1e-3;
+  }
   double c[3];
   memcpy(c, h->h_sumres + 8, 3 * 8);
   h->cflmax = c[0];
@@ -860,10 +945,10 @@ static void read_stats(lw_handle *h, lw_step_stats *stats) {
   stats->ntime = h->ntime;
   stats->mg_cycles_max = h->cyc_max;
   stats->mg_cycles_sum = h->cyc_sum;
-  stats->ms_sgs = h->ms[0];
-  stats->ms_momentum = h->ms[1];
-  stats->ms_poisson = h->ms[2];
-  stats->ms_energy = h->ms[3];
+  stats->ms_sgs = h->s_phase[0] * 1e3;
+  stats->ms_momentum = h->s_phase[1] * 1e3;
+  stats->ms_poisson = h->s_phase[2] * 1e3;
+  stats->ms_energy = h->s_phase[3] * 1e3;
 }
 
 extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
@@ -876,13 +961,23 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
       stepinit(h);
       for (int ms = 1; ms <= 3; ++ms) {
         substepinit(h, ms);
+        hipEvent_t *ev = h->tev[ms - 1];
+        HIPCHK(hipEventRecord(ev[0], h->s));
         phase_sgs(h);
+        HIPCHK(hipEventRecord(ev[1], h->s));
+        HIPCHK(hipEventRecord(ev[2], h->s));
         phase_rhsnlhs(h);
+        HIPCHK(hipEventRecord(ev[3], h->s));
+        HIPCHK(hipEventRecord(ev[4], h->s));
         launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
         phase_poisson(h);
+        HIPCHK(hipEventRecord(ev[5], h->s));
         launch_projection(h->s, h->g, h->sub, h->f);
         if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
+        HIPCHK(hipEventRecord(ev[6], h->s));
         phase_energy(h);
+        HIPCHK(hipEventRecord(ev[7], h->s));
+        if (ms == 3) h->tev_valid = true;
       }
       end_of_step(h);
       if (g_prof) {
@@ -926,6 +1021,40 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
   } catch (int code) {
     return code;
   }
+}
+
+extern "C" int lw_get_history(const lw_handle *h, lw_history *out) {
+  if (!h || !out) return fail(LW_ERR_ARG, "null argument");
+  memset(out, 0, sizeof(*out));
+  out->ntime = h->ntime;
+  out->ihist = h->ihist;
+  out->written = h->hist_written;
+  out->time = h->time;
+  out->dt = h->dt;
+  out->cflmax = h->cflmax;
+  out->dvmax = h->dvmax;
+  out->qmmax = h->qmmax;
+  for (int l = 0; l < 3; ++l) {
+    out->cd[l] = h->h_hist[l];
+    out->cd_avg[l] = h->h_hist[3 + l];
+    out->hflux[l] = h->h_hist[6 + l];
+    out->hflux_avg[l] = h->h_hist[9 + l];
+  }
+  out->tbulk[0] = h->h_hist[12];
+  out->tbulk[1] = h->h_hist[13];
+  out->tbulk_avg[0] = h->h_hist[14];
+  out->tbulk_avg[1] = h->h_hist[15];
+  out->s_sgs = h->s_phase[0];
+  out->s_rhsnlhs = h->s_phase[1];
+  out->s_poisson = h->s_phase[2];
+  out->s_energy = h->s_phase[3];
+  return LW_OK;
+}
+
+extern "C" int lw_get_settings(const lw_handle *h, lw_settings *out) {
+  if (!h || !out) return fail(LW_ERR_ARG, "null argument");
+  *out = h->st;
+  return LW_OK;
 }
 
 extern "C" int lw_profile_begin(lw_handle *h, const char *kernels) {

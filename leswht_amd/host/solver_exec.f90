@@ -1,0 +1,126 @@
+!=======================================================================
+! solver_exec on MI355X: the time loop of the LESwHT solver with the time
+! step on the GPU through the C ABI (include/leswht.h).
+!
+! Run, like the reference executable, with the solver directory (the one
+! holding settings.input) as the working directory (run_solver.sh:41). It
+! reads the same inputs (settings.input, boundary.input, ../output/ibmpre/*,
+! the grid file, the initial field; misc_init.f90:19-786) and writes the same
+! outputs: ../output/field/fld%07d at every NPRINT steps and at the end
+! (solver.f90:171,185; misc_hist.f90:212-252) and the history files
+! ../output/ftr/{fhist,ftemp,fcdcl,fheatflux,ftime}.dat with the reference's
+! records and formats (misc_hist.f90:2-181, 255-284; slv_auxi.f90:573-840).
+! Unlike the reference, an error ends the program with a non-zero status.
+!
+! The body is the circular cylinder of geometry/funcbody_example/
+! funcbody_cylinder.f90 (radius 0.5), or radius LW_BODY_RADIUS if set.
+!=======================================================================
+program solver_exec
+  use, intrinsic :: iso_c_binding
+  use leswht_c
+  implicit none
+
+  type(c_ptr) :: h
+  type(lw_settings) :: st
+  type(lw_history) :: hs
+  type(lw_step_stats) :: ss
+  type(lw_body) :: body
+  integer(c_int) :: rc
+  integer(8) :: clk0, clk1, rate
+  real(8) :: wall
+  character(len=64) :: env
+  integer :: elen, estat
+  logical :: heat, ibm
+
+  body%kind = 0
+  body%p = 0.0d0
+  body%p(1) = 0.5d0
+  call get_environment_variable('LW_BODY_RADIUS', env, elen, estat)
+  if (estat == 0 .and. elen > 0) read (env(1:elen), *) body%p(1)
+
+  rc = lw_create_from_dir('.'//c_null_char, body, h)
+  call check(rc, 'reading the case')
+  rc = lw_get_settings(h, st)
+  call check(rc, 'settings')
+  rc = lw_get_history(h, hs)
+  call check(rc, 'initial state')
+  heat = st%ihtrans == 1
+  ibm = st%ibmon == 1
+
+  call open_histories(st%iread == 1)
+  call system_clock(count_rate=rate)
+
+  do while ((hs%ntime < st%nend) .and. (hs%time < st%tend))
+    call system_clock(clk0)
+    rc = lw_step(h, 1_c_int64_t, ss)
+    call check(rc, 'time step')
+    call system_clock(clk1)
+    wall = dble(clk1 - clk0) / dble(rate)
+    rc = lw_get_history(h, hs)
+    call check(rc, 'history')
+
+    write (*, 320) hs%ntime, hs%time, hs%dt
+    write (*, 330) hs%cflmax, hs%dvmax, ss%mg_cycles_max
+320 format('--------------------------', i6, '  TIME=', f10.5, '  DT=', f12.8)
+330 format('  CFL=', es13.5, '  MAX DIV=', es13.5, '  MG CYCLES=', i5)
+
+    if (hs%written == 1) then
+      write (2000, 130) hs%time, hs%dt, hs%cflmax, hs%dvmax, hs%qmmax
+      if (heat) write (2012, 130) hs%time, hs%tbulk(1), hs%tbulk(2), hs%tbulk_avg(1), hs%tbulk_avg(2)
+      if (ibm) write (2001, 140) hs%time, hs%cd(1), hs%cd(2), hs%cd(3), &
+        hs%cd_avg(1), hs%cd_avg(2), hs%cd_avg(3)
+      if (heat) write (2002, 140) hs%time, hs%hflux(1), hs%hflux(2), hs%hflux(3), &
+        hs%hflux_avg(1), hs%hflux_avg(2), hs%hflux_avg(3)
+    end if
+    if (mod(hs%ntime, st%nprint) == 0) call write_field(hs%ihist)
+    write (2011, 130) hs%time, hs%s_sgs, hs%s_rhsnlhs, hs%s_poisson, wall
+  end do
+
+  call close_histories()
+  if (mod(hs%ntime, st%nprint) /= 0) call write_field(hs%ihist)
+  rc = lw_destroy(h)
+
+130 format(f13.5, 4es15.7)
+140 format(f13.5, 6es15.7)
+
+contains
+
+  subroutine check(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    if (code /= lw_ok) then
+      write (*, '(a,a,a,i0,a,a)') 'solver_exec: ', what, ' failed (', code, '): ', lw_error_text()
+      error stop 1
+    end if
+  end subroutine check
+
+  subroutine write_field(ihist)
+    integer(c_int64_t), intent(in) :: ihist
+    character(len=26) :: tname
+    write (tname, '(a,i7.7)') '../output/field/fld', ihist
+    rc = lw_write_field(h, trim(tname)//c_null_char)
+    call check(rc, 'writing '//trim(tname))
+  end subroutine write_field
+
+  ! ftrfiles(1): new files, or appended when the run continues a field
+  subroutine open_histories(append)
+    logical, intent(in) :: append
+    character(len=8) :: pos
+    pos = 'asis'
+    if (append) pos = 'append'
+    open (2000, file='../output/ftr/fhist.dat', position=trim(pos))
+    open (2001, file='../output/ftr/fcdcl.dat', position=trim(pos))
+    open (2002, file='../output/ftr/fheatflux.dat', position=trim(pos))
+    open (2012, file='../output/ftr/ftemp.dat', position=trim(pos))
+    open (2011, file='../output/ftr/ftime.dat', position=trim(pos))
+  end subroutine open_histories
+
+  subroutine close_histories()
+    close (2000)
+    close (2001)
+    close (2002)
+    close (2012)
+    close (2011)
+  end subroutine close_histories
+
+end program solver_exec

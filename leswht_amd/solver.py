@@ -72,10 +72,25 @@ class lw_body(C.Structure):
     _fields_ = [("kind", _I), ("p", _D * 4)]
 
 
+class lw_history(C.Structure):
+    _fields_ = [(n, _I) for n in ("ntime", "ihist", "written")] + \
+        [(n, _D) for n in ("time", "dt", "cflmax", "dvmax", "qmmax")] + \
+        [("tbulk", _D * 2), ("tbulk_avg", _D * 2), ("cd", _D * 3), ("cd_avg", _D * 3),
+         ("hflux", _D * 3), ("hflux_avg", _D * 3)] + \
+        [(n, _D) for n in ("s_sgs", "s_rhsnlhs", "s_poisson", "s_energy")]
+
+    def as_dict(self):
+        out = {}
+        for n, _ in self._fields_:
+            v = getattr(self, n)
+            out[n] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
 EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw_get_fields",
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
-           "lw_profile_get", "lw_last_error")
+           "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error")
 
 _lib = None
 
@@ -103,6 +118,8 @@ def load_library(path: str = LIB_PATH):
     L.lw_phase.argtypes = [H, C.c_int]
     L.lw_device_bytes.argtypes = [H]
     L.lw_device_bytes.restype = _I
+    L.lw_get_history.argtypes = [H, C.POINTER(lw_history)]
+    L.lw_get_settings.argtypes = [H, C.POINTER(lw_settings)]
     L.lw_profile_begin.argtypes = [H, C.c_char_p]
     L.lw_profile_end.argtypes = [H]
     L.lw_profile_count.argtypes = [H]
@@ -248,6 +265,19 @@ class Solver:
 
     def write_field(self, path: str):
         _check(load_library().lw_write_field(self._h, path.encode()))
+
+    def history(self) -> dict:
+        """History quantities of the last completed step: the values the
+        reference writes to fhist/ftemp/fcdcl/fheatflux.dat (``written`` is 1
+        on the steps it writes them) and the device seconds of the phases."""
+        h = lw_history()
+        _check(load_library().lw_get_history(self._h, C.byref(h)))
+        return h.as_dict()
+
+    def settings(self) -> dict:
+        st = lw_settings()
+        _check(load_library().lw_get_settings(self._h, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in lw_settings._fields_}
 
     def profile_begin(self, kernels=None):
         """Starts a fresh per-kernel event profile of the following steps;

@@ -2,7 +2,8 @@
 solver (oracle/_ref/solver_exec, compiled from the reference's own Fortran
 sources by oracle/Makefile) on tiny cylinder cases built by
 leswht_amd.case. Each fixture holds the case spec, a hash of the generated
-input text and the fld fields the reference wrote after NEND steps.
+input text, the fld fields the reference wrote after NEND steps and the rows
+of its history files (fhist, fcdcl, fheatflux, ftemp).
 
 Run in the build container (the reference sources are not on the GPU box):
     make -C oracle ref && python tests/golden/make_golden.py
@@ -60,12 +61,16 @@ def main():
         C.write_case(C.build_case(spec), root)
         oracle.run_reference(root, threads=1)
         fl = C.read_fld(os.path.join(root, "output/field/fld%07d" % c["nend"]))
-        hist = [ln.split() for ln in open(os.path.join(root, "output/ftr/fhist.dat")) if ln.strip()]
+        rows = {}
+        for key, fn in (("hist", "fhist"), ("cdcl", "fcdcl"), ("hflux", "fheatflux"), ("temp", "ftemp")):
+            path = os.path.join(root, "output/ftr/%s.dat" % fn)
+            rows[key] = np.array([[float(x) for x in ln.split()] for ln in open(path) if ln.strip()],
+                                 dtype=np.float64)
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
                             u=fl["u"], v=fl["v"], w=fl["w"], p=fl["p"], t=fl["t"],
-                            time=fl["time"], dt=fl["dt"], hist=np.array(hist, dtype=np.float64),
+                            time=fl["time"], dt=fl["dt"], ihist=fl["ihist"], **rows,
                             meta=json.dumps(dict(c, input_sha256=input_hash(root))))
-        print(name, "time", fl["time"], "dvmax", hist[-1][3])
+        print(name, "time", fl["time"], "dvmax", rows["hist"][-1][3])
         shutil.rmtree(root)
 
 

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from leswht_amd import solver
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,3 +33,17 @@ def test_errors_are_reported_not_thrown():
     rc = L.lw_create(None, None, None, None, ctypes.byref(h))
     assert rc == -1
     assert b"null" in L.lw_last_error()
+
+
+def test_fortran_host_reports_errors():
+    """solver_exec run where there is no case ends with a non-zero status and
+    the library's message (the reference would stop with status 0)."""
+    import subprocess
+    import tempfile
+    from leswht_amd.build import EXE
+    if not os.path.exists(EXE):
+        pytest.skip("solver_exec not built (amdflang absent at build time)")
+    with tempfile.TemporaryDirectory() as d:
+        res = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0
+    assert "failed" in (res.stdout + res.stderr)
