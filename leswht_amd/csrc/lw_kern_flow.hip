@@ -525,18 +525,23 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
     r[q] = up;
   }
   for (int ib = lo_ + 1; ib <= n; ib += kAdiCh) {
-    double v[kAdiCh + 1], rv[kAdiCh];
+    double v[kAdiCh + 1], rv[kAdiCh], am[kAdiCh], cm[kAdiCh];
 #pragma unroll
     for (int u = 0; u <= kAdiCh; ++u)
       v[u] = ib + u <= n + 1 ? l.vis[q0 + (long)(ib + u + l.lo) * st] : 0.0;
 #pragma unroll
-    for (int u = 0; u < kAdiCh; ++u) rv[u] = ib + u <= n ? r[q0 + (long)(ib + u) * st] : 0.0;
+    for (int u = 0; u < kAdiCh; ++u) {
+      const bool ok = ib + u <= n;
+      rv[u] = ok ? r[q0 + (long)(ib + u) * st] : 0.0;
+      am[u] = ok ? l.am[ib + u] : 0.0;
+      cm[u] = ok ? l.cm[ib + u] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
       const int i = ib + u;
       if (i > n) break;
-      const double a = l.am[i] * (1. + l.cc * v[u]);
-      const double c = l.cm[i] * (1. + l.cc * v[u + 1]);
+      const double a = am[u] * (1. + l.cc * v[u]);
+      const double c = cm[u] * (1. + l.cc * v[u + 1]);
       const double b = l.bd - a - c;
       const double gv = l.bd * rv[u];
       const double gam = cprev * bet;
@@ -579,18 +584,23 @@ __global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre,
   }
   double an = 0, bn = 0, cn = 0;
   for (int kb = 2; kb <= n; kb += kAdiCh) {
-    double v[kAdiCh + 1], rv[kAdiCh];
+    double v[kAdiCh + 1], rv[kAdiCh], am[kAdiCh], cm[kAdiCh];
 #pragma unroll
     for (int u = 0; u <= kAdiCh; ++u)
       v[u] = kb + u <= n + 1 ? l.vis[q0 + (long)(kb + u + l.lo) * st] : 0.0;
 #pragma unroll
-    for (int u = 0; u < kAdiCh; ++u) rv[u] = kb + u <= n ? r[q0 + (long)(kb + u) * st] : 0.0;
+    for (int u = 0; u < kAdiCh; ++u) {
+      const bool ok = kb + u <= n;
+      rv[u] = ok ? r[q0 + (long)(kb + u) * st] : 0.0;
+      am[u] = ok ? l.am[kb + u] : 0.0;
+      cm[u] = ok ? l.cm[kb + u] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
       const int k = kb + u;
       if (k > n) break;
-      const double a = l.am[k] * (1. + l.cc * v[u]);
-      const double c = l.cm[k] * (1. + l.cc * v[u + 1]);
+      const double a = am[u] * (1. + l.cc * v[u]);
+      const double c = cm[u] * (1. + l.cc * v[u + 1]);
       const double b = l.bd - a - c;
       const double gv = l.bd * rv[u];
       const double pn = 1. / (b + a * qp);
@@ -680,36 +690,145 @@ __global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre,
   }
 }
 
-// non-periodic x line (trdiag1, slv_mmtm.f90:1365-1412); field += solution
+// non-periodic x line (trdiag1, slv_mmtm.f90:1365-1412); field += solution.
+// x is the contiguous direction, so a thread per line would read addresses a
+// row apart. One wavefront takes 64 lines (consecutive (j, k)) and walks them
+// in chunks of kXch points: each chunk is loaded through LDS two lines at a
+// time (each half of the wavefront reads 32 consecutive doubles of one line),
+// every lane then runs the reference recurrence along its own line out of
+// LDS, and the results leave through LDS the same way.
+constexpr int kXch = 31;        // points per chunk
+constexpr int kXld = kXch + 2;  // LDS row stride (odd, conflict-free columns)
+
 template <int C>
-__global__ __launch_bounds__(256) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
-                                               int klo) {
-  const int j = jlo + blockIdx.x * 64 + threadIdx.x;
-  const int k = klo + blockIdx.y * 4 + threadIdx.y;
-  if (j > g.n2m || k > g.n3m) return;
+__global__ __launch_bounds__(64) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
+                                              int klo) {
+  __shared__ double V[64 * kXld], R[64 * kXld], G[64 * kXld], AM[kXch], CM[kXch];
+  const int lane = threadIdx.x;
+  const int nj = g.n2m - jlo + 1;
+  const int nlines = nj * (g.n3m - klo + 1);
+  const int line0 = blockIdx.x * 64;
+  const int nl = nlines - line0 < 64 ? nlines - line0 : 64;  // lines of this wavefront
+  const bool mine = lane < nl;
   const LineCf l = line_cf<0, C>(g, f, sb, cre, cre2);
   double *fld = C == 0 ? f.u : C == 1 ? f.v : C == 2 ? f.w : f.t;
-  const long q0 = P(g, 0, j, k);
   double *r = f.rhs[C];
   double *gm = f.s1;
-  double unext = line_forward(l, r, gm, q0, 1, ilo, g.n1m);
-  fld[q0 + g.n1m] = unext + fld[q0 + g.n1m];
-  for (int ib = g.n1m - 1; ib >= ilo; ib -= kAdiCh) {
-    double rv[kAdiCh], gv[kAdiCh], fv[kAdiCh];
+  const int n1m = g.n1m;
+  // row offset of this lane's line; line L's offset is lane L's (broadcast)
+  const long q0 = mine ? P(g, 0, jlo + (line0 + lane) % nj, klo + (line0 + lane) / nj) : 0;
+  // the row offsets of the lines this lane stages: line 2t + (lane >> 5)
+  long qs[32];
 #pragma unroll
-    for (int u = 0; u < kAdiCh; ++u) {
-      const long q = q0 + ib - u;
-      rv[u] = ib - u >= ilo ? r[q] : 0.0;
-      gv[u] = ib - u >= ilo ? gm[q + 1] : 0.0;
-      fv[u] = ib - u >= ilo ? fld[q] : 0.0;
+  for (int t = 0; t < 32; ++t) qs[t] = __shfl(q0, 2 * t + (lane >> 5), 64);
+  const int hl = lane >> 5, o = lane & 31;
+  double bet = 0.0, up = 0.0, cprev = 0.0;
+  if (mine) {  // first point, i = ilo
+    const long q = q0 + ilo;
+    const double a = l.am[ilo] * (1. + l.cc * l.vis[q + l.lo]);
+    const double c = l.cm[ilo] * (1. + l.cc * l.vis[q + l.lo + 1]);
+    const double b = l.bd - a - c;
+    bet = 1. / b;
+    up = (l.bd * r[q]) * bet;
+    cprev = c;
+    r[q] = up;
+  }
+  // forward elimination, chunks [ib, ib + cnt)
+  for (int ib = ilo + 1; ib <= n1m; ib += kXch) {
+    const int cnt = n1m - ib + 1 < kXch ? n1m - ib + 1 : kXch;
+    if (lane < cnt) {
+      AM[lane] = l.am[ib + lane];
+      CM[lane] = l.cm[ib + lane];
     }
+    // unconditional loads, 16 line pairs in flight at a time: lanes past the
+    // last line read row 0 of the first plane and points past the chunk read
+    // the next row, all inside the padded array; nothing beyond cnt is used
 #pragma unroll
-    for (int u = 0; u < kAdiCh; ++u) {
-      if (ib - u < ilo) break;
-      const double uu = rv[u] - gv[u] * unext;
-      fld[q0 + ib - u] = uu + fv[u];
-      unext = uu;
+    for (int t0 = 0; t0 < 32; t0 += 16) {
+      double tv[16], tr[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        tv[t] = l.vis[qs[t0 + t] + ib + l.lo + o];
+        tr[t] = r[qs[t0 + t] + ib + o];
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int L = 2 * (t0 + t) + hl;
+        V[L * kXld + o] = tv[t];
+        R[L * kXld + o] = tr[t];
+      }
     }
+    __syncthreads();
+    if (mine) {
+      const double *v = V + lane * kXld;
+      double *rr = R + lane * kXld, *gg = G + lane * kXld;
+#pragma unroll 4
+      for (int o = 0; o < cnt; ++o) {
+        const double a = AM[o] * (1. + l.cc * v[o]);
+        const double c = CM[o] * (1. + l.cc * v[o + 1]);
+        const double b = l.bd - a - c;
+        const double gv = l.bd * rr[o];
+        const double gam = cprev * bet;
+        bet = 1. / (b - a * gam);
+        up = (gv - a * up) * bet;
+        gg[o] = gam;
+        rr[o] = up;
+        cprev = c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int L = 2 * t + hl;
+      if (L < nl && o < cnt) {
+        gm[qs[t] + ib + o] = G[L * kXld + o];
+        r[qs[t] + ib + o] = R[L * kXld + o];
+      }
+    }
+    __syncthreads();
+  }
+  // back substitution, chunks [bot, top] from the top down; V holds the field
+  double unext = up;
+  if (mine) fld[q0 + n1m] = unext + fld[q0 + n1m];
+  for (int top = n1m - 1; top >= ilo; top -= kXch) {
+    const int bot = top - kXch + 1 > ilo ? top - kXch + 1 : ilo;
+    const int cnt = top - bot + 1;
+#pragma unroll
+    for (int t0 = 0; t0 < 32; t0 += 8) {
+      double tr[8], tg[8], tf[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const long qL = qs[t0 + t] + bot;
+        tr[t] = r[qL + o];
+        tg[t] = gm[qL + o + 1];
+        tf[t] = fld[qL + o];
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int L = 2 * (t0 + t) + hl;
+        R[L * kXld + o] = tr[t];
+        G[L * kXld + o] = tg[t];
+        V[L * kXld + o] = tf[t];
+      }
+    }
+    __syncthreads();
+    if (mine) {
+      const double *rr = R + lane * kXld, *gg = G + lane * kXld;
+      double *ff = V + lane * kXld;
+#pragma unroll 4
+      for (int o = cnt - 1; o >= 0; --o) {
+        const double uu = rr[o] - gg[o] * unext;
+        ff[o] = uu + ff[o];
+        unext = uu;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int L = 2 * t + hl;
+      if (L < nl && o < cnt) fld[qs[t] + bot + o] = V[L * kXld + o];
+    }
+    __syncthreads();
   }
 }
 
@@ -730,8 +849,9 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
               cre2, ilo, jlo);
   LW_LAUNCH(ny[C], pts(g) * 24.0, 0, ky, grid3(g.n1m - ilo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre,
             cre2, ilo, jlo, klo);
-  LW_LAUNCH(nx[C], pts(g) * 32.0, 0, kx, grid3(g.n2m - jlo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre,
-            cre2, ilo, jlo, klo);
+  const int xlines = (g.n2m - jlo + 1) * (g.n3m - klo + 1);
+  LW_LAUNCH(nx[C], pts(g) * 32.0, 0, kx, dim3((unsigned)((xlines + 63) / 64)), dim3(64), 0, s, g, sb, f, cre, cre2,
+            ilo, jlo, klo);
 }
 
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les) {

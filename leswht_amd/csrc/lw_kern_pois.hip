@@ -32,37 +32,86 @@ __device__ inline double2 shfl_down2(double2 v, int d) {
 // unnormalised; inverse by conjugation and 1/N. Direct sums over a column
 // held in registers; the column length is a template parameter.
 // ---------------------------------------------------------------------------
+// Tiles of 16 x 16 (i, j) columns per workgroup. A thread holds one column:
+// the real data are read along i (contiguous in the field), and the spectrum
+// leaves through LDS so that it is written along j (contiguous in the
+// multigrid planes), kFftKb wavenumbers at a time.
+constexpr int kFftT = 16;
+constexpr int kFftKb = 8;
+constexpr int kFftLd = kFftT + 1;  // padded LDS row
+
 template <int N>
-__global__ void k_fft_fwd(Geo g, MG mg, const double *div) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y;
-  if (i > g.n1m) return;
+__global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div) {
+  constexpr int NH = N / 2 + 1;
+  __shared__ double2 S[kFftKb * kFftT * kFftLd];
+  const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
+  const int i0 = 1 + blockIdx.x * kFftT, j0 = 1 + blockIdx.y * kFftT;
+  const int i = i0 + ti, j = j0 + tj;
+  const bool in = i <= g.n1m && j <= g.n2m;
+  const int oi = i0 + tj, oj = j0 + ti;  // writer of row oi, column oj
+  const bool oin = oi <= g.n1m && oj <= g.n2m;
   double x[N];
 #pragma unroll
-  for (int n = 0; n < N; ++n) x[n] = div[P(g, i, j, n + 1)];
+  for (int n = 0; n < N; ++n) x[n] = in ? div[P(g, i, j, n + 1)] : 0.0;
+  // the wavenumber loops stay rolled: the twiddle index (kv n) mod N is then
+  // uniform across the workgroup and read with scalar loads
+#pragma unroll 1
+  for (int kb = 0; kb < NH; kb += kFftKb) {
+#pragma unroll 1
+    for (int u = 0; u < kFftKb; ++u) {
+      const int kv = kb + u;
+      if (kv < NH) {
+        double sr = 0.0, si = 0.0;
 #pragma unroll
-  for (int kv = 0; kv <= N / 2; ++kv) {
-    double sr = 0.0, si = 0.0;
-#pragma unroll
-    for (int n = 0; n < N; ++n) {
-      const int m = (kv * n) % N;
-      sr += x[n] * mg.twr[m];
-      si += x[n] * mg.twi[m];
+        for (int n = 0; n < N; ++n) {
+          const int m = (kv * n) % N;
+          sr += x[n] * mg.twr[m];
+          si += x[n] * mg.twi[m];
+        }
+        S[(u * kFftT + tj) * kFftLd + ti] = make_double2(sr, si);
+      }
     }
-    mg.rhs[(long)kv * mg.PCPLANE + (long)i * mg.LDJ + j] = make_double2(sr, si);
+    __syncthreads();
+#pragma unroll 1
+    for (int u = 0; u < kFftKb; ++u) {
+      const int kv = kb + u;
+      if (kv < NH && oin) mg.rhs[(long)kv * mg.PCPLANE + (long)oi * mg.LDJ + oj] = S[(u * kFftT + ti) * kFftLd + tj];
+    }
+    __syncthreads();
   }
 }
 
 template <int N>
-__global__ void k_fft_inv(Geo g, MG mg, double *phi) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y;
-  if (i > g.n1m) return;
-  double2 X[N / 2 + 1];
+__global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
+  constexpr int NH = N / 2 + 1;
+  __shared__ double2 S[kFftKb * kFftT * kFftLd];
+  const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
+  const int i0 = 1 + blockIdx.x * kFftT, j0 = 1 + blockIdx.y * kFftT;
+  const int i = i0 + ti, j = j0 + tj;
+  const bool in = i <= g.n1m && j <= g.n2m;
+  const int oi = i0 + tj, oj = j0 + ti;  // reader of row oi, column oj
+  const bool oin = oi <= g.n1m && oj <= g.n2m;
+  double2 X[NH];
 #pragma unroll
-  for (int kv = 0; kv <= N / 2; ++kv) X[kv] = mg.pc[(long)kv * mg.PCPLANE + (long)i * mg.LDJ + j];
+  for (int kb = 0; kb < NH; kb += kFftKb) {
+#pragma unroll
+    for (int u = 0; u < kFftKb; ++u) {
+      const int kv = kb + u;
+      if (kv < NH)
+        S[(u * kFftT + ti) * kFftLd + tj] =
+            oin ? mg.pc[(long)kv * mg.PCPLANE + (long)oi * mg.LDJ + oj] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kFftKb; ++u) {
+      const int kv = kb + u;
+      if (kv < NH) X[kv] = S[(u * kFftT + tj) * kFftLd + ti];
+    }
+    __syncthreads();
+  }
+  if (!in) return;
   const double dn = 1.0 / (double)N;
-#pragma unroll
+#pragma unroll 1
   for (int n = 0; n < N; ++n) {
     double s = 0.0;
 #pragma unroll
@@ -119,6 +168,7 @@ static double fft_bytes(const Geo &g, const MG &mg) {
 
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
+  const dim3 gt((g.n1m + kFftT - 1) / kFftT, (g.n2m + kFftT - 1) / kFftT), bt(256);
   void (*k)(Geo, MG, const double *) = nullptr;
   switch (g.n3m) {
     case 2: k = k_fft_fwd<2>; break;
@@ -129,12 +179,13 @@ void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f)
     case 64: k = k_fft_fwd<64>; break;
     default: break;
   }
-  if (k) LW_LAUNCH("k_fft_fwd", fft_bytes(g, mg), 0, k, gr, bl, 0, s, g, mg, f.div);
+  if (k) LW_LAUNCH("k_fft_fwd", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.div);
   else LW_LAUNCH("k_fft_fwd_any", fft_bytes(g, mg), 0, k_fft_fwd_any, gr, bl, 0, s, g, mg, f.div, g.n3m);
 }
 
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
+  const dim3 gt((g.n1m + kFftT - 1) / kFftT, (g.n2m + kFftT - 1) / kFftT), bt(256);
   void (*k)(Geo, MG, double *) = nullptr;
   switch (g.n3m) {
     case 2: k = k_fft_inv<2>; break;
@@ -145,7 +196,7 @@ void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f)
     case 64: k = k_fft_inv<64>; break;
     default: break;
   }
-  if (k) LW_LAUNCH("k_fft_inv", fft_bytes(g, mg), 0, k, gr, bl, 0, s, g, mg, f.phi);
+  if (k) LW_LAUNCH("k_fft_inv", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.phi);
   else LW_LAUNCH("k_fft_inv_any", fft_bytes(g, mg), 0, k_fft_inv_any, gr, bl, 0, s, g, mg, f.phi, g.n3m);
 }
 
