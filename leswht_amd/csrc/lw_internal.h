@@ -69,7 +69,6 @@ struct MG {
   int iimg[21][4];
   // coefficients
   double *aw, *ae, *as, *an, *coi1, *coi2, *cor1, *cor2;
-  int *ipm, *imm;
   double *ac, *gam, *bet;  // [kv][i][j], real
   double2 *gg, *resd;      // [kv][i][j], all levels stacked along i
   double2 *pc, *rhs;       // [kv][i][j], top level

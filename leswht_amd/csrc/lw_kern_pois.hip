@@ -3,14 +3,19 @@
 // x-semicoarsened zebra-line multigrid whose lines are y-tridiagonal solves
 // with the precomputed LU factors bet/gam, then the inverse z-DFT.
 //
+// The x neighbours of row i are rows i-1 and i+1 on every level (imm, ipm of
+// coefmg, poiss_z_ft.f90:733-741; x-periodic grids, which rewire them, are
+// outside this path).
+//
 // Layout: every multigrid plane is stored [kv][i][j] with j contiguous, so a
 // y-line is one contiguous row. One wavefront solves one line: each lane owns
 // R consecutive rows, the forward and backward substitutions are first-order
-// linear recurrences, combined across lanes with a 6-step shuffle scan of
-// affine maps, and each lane then re-runs the reference recurrence on its
-// chunk from the exact carry-in. All wavenumbers advance together; a
-// wavenumber that met its max-norm test sets its flag on the device and every
-// later kernel skips it, so each keeps the reference's own V-cycle count.
+// linear recurrences, combined across lanes with a scan of affine maps (DPP
+// within 16-lane rows, then across rows from the row totals), and each lane
+// then re-runs the reference recurrence on its chunk from the carry-in.
+// All wavenumbers advance together; a wavenumber that met its max-norm test
+// sets its flag on the device and every later kernel skips it, so each keeps
+// the reference's own V-cycle count.
 #include "lw_kernels.h"
 
 namespace lw {
@@ -18,14 +23,45 @@ namespace lw {
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 rmul(double a, double2 b) { return make_double2(a * b.x, a * b.y); }
-__device__ inline double2 shfl_up2(double2 v, int d) {
-  return make_double2(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64));
-}
-__device__ inline double2 shfl_down2(double2 v, int d) {
-  return make_double2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
-}
 
 #define ROW(base, i_) ((base) + (long)(i_) * mg.LDJ)
+
+// DPP moves of a double (both halves with the same control; lanes whose
+// source is outside their 16-lane row receive `old`)
+template <int CTRL>
+__device__ inline double dpp_d(double v, double old) {
+  const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double rdlane(double v, int l) {
+  const long long vb = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)vb, l), hi = __builtin_amdgcn_readlane((int)(vb >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;  // row_shr:n
+constexpr int kShl1 = 0x101, kShl2 = 0x102, kShl4 = 0x104, kShl8 = 0x108;  // row_shl:n
+
+// prefix step of the affine maps (M, C) of the forward substitution: this
+// lane's map after the map of the lane D below it (identity across rows)
+#define FWD_STEP(CT)                                                             \
+  {                                                                              \
+    const double Mp = dpp_d<CT>(M, 1.0);                                         \
+    const double2 Cp = make_double2(dpp_d<CT>(Cc.x, 0.0), dpp_d<CT>(Cc.y, 0.0)); \
+    Cc = cadd(rmul(M, Cp), Cc);                                                  \
+    M = M * Mp;                                                                  \
+  }
+// suffix step of the back substitution's maps (N, D)
+#define BWD_STEP(CT)                                                           \
+  {                                                                            \
+    const double Np = dpp_d<CT>(Nn, 1.0);                                      \
+    const double2 Dp = make_double2(dpp_d<CT>(D.x, 0.0), dpp_d<CT>(D.y, 0.0)); \
+    D = cadd(rmul(Nn, Dp), D);                                                 \
+    Nn = Nn * Np;                                                              \
+  }
+
+
 
 // ---------------------------------------------------------------------------
 // z-DFT with FFTE's conventions (zfft1d.f:32-85): forward e^{-2 pi i k n/N}
@@ -246,7 +282,7 @@ __device__ inline Prolong make_prolong(const MG &mg, const double2 *gg, int ic, 
   p.c1 = p.c2 = 0.0;
   if (on) {
     p.g0 = ROW(gg, ic);
-    p.g1 = ROW(gg, mg.ipm[ic]);
+    p.g1 = ROW(gg, ic + 1);
     p.c1 = mg.coi1[ic];
     p.c2 = mg.coi2[ic];
     p.last = ic == cend;
@@ -275,7 +311,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
   double2 *gg = mg.gg + pl;
   const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
-  const int im = mg.imm[i], ip = mg.ipm[i];
+  const int im = i - 1, ip = i + 1;
   const double aw = mg.aw[i], ae = mg.ae[i];
   constexpr bool TOP = MODE == 2 || MODE == 3 || MODE == 5;
   constexpr bool PCN = MODE == 2 || MODE == 5;
@@ -299,71 +335,84 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
     }
   }
   const int j0 = 1 + lane * R;
+  // every load is issued unconditionally (rows past n2m read row n2m), so a
+  // line waits for memory once; values past n2m are masked out afterwards
   double2 rr[R];
-  double bt[R];
+  double bt[R], asj[R], gm[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
-    if (j <= n2m) {
-      const double2 s0 = src[j];
-      if (MODE == 0 || MODE == 2) {
-        rr[r] = csub(s0, rmul(oldv, cadd(rmul(aw, nbm[j]), rmul(ae, nbp[j]))));
-      } else if (PRO) {
-        const double2 vm = pm.apply(nbm[j], j), vp = pp.apply(nbp[j], j);
-        rr[r] = csub(s0, rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
-      } else if (MODE == 1) {
-        rr[r] = csub(csub(s0, rmul(aw, nbm[j])), rmul(ae, nbp[j]));
-      } else {
-        const double2 left = kv == 1 ? csub(nbm[j], tt) : nbm[j];
-        rr[r] = csub(csub(s0, rmul(aw, left)), rmul(ae, csub(nbp[j], tt)));
-      }
-      bt[r] = bet[j];
+    const int jc = j <= n2m ? j : n2m;
+    const double2 s0 = src[jc];
+    double2 v;
+    if (MODE == 0 || MODE == 2) {
+      v = csub(s0, rmul(oldv, cadd(rmul(aw, nbm[jc]), rmul(ae, nbp[jc]))));
+    } else if (PRO) {
+      const double2 vm = pm.apply(nbm[jc], jc), vp = pp.apply(nbp[jc], jc);
+      v = csub(s0, rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
+    } else if (MODE == 1) {
+      v = csub(csub(s0, rmul(aw, nbm[jc])), rmul(ae, nbp[jc]));
     } else {
-      rr[r] = make_double2(0.0, 0.0);
-      bt[r] = 0.0;
+      const double2 left = kv == 1 ? csub(nbm[jc], tt) : nbm[jc];
+      v = csub(csub(s0, rmul(aw, left)), rmul(ae, csub(nbp[jc], tt)));
     }
+    const double b = bet[jc];
+    const double a = mg.as[jc];
+    const double gnext = gam[jc < n2m ? jc + 1 : n2m];
+    // masks as factors (exact for finite values): a select would let the
+    // compiler move the loads into a branch per row again
+    const double okf = j <= n2m ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
+    rr[r] = rmul(okf, v);
+    bt[r] = okf * b;
+    asj[r] = a;
+    gm[r] = okg * gnext;
   }
   // forward substitution y_j = (rr_j - as_j y_{j-1}) bet_j as affine maps
+  // y -> M y + C composed over this lane's rows
   double M = 1.0;
   double2 Cc = make_double2(0.0, 0.0);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
     if (j <= n2m) {
-      const double m = j == 1 ? 0.0 : -(mg.as[j] * bt[r]);
+      const double m = j == 1 ? 0.0 : -(asj[r] * bt[r]);
       Cc = cadd(rmul(m, Cc), rmul(bt[r], rr[r]));
       M = m * M;
     }
   }
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double Mp = __shfl_up(M, d, 64);
-    const double2 Cp = shfl_up2(Cc, d);
-    if (lane >= d) {
-      Cc = cadd(rmul(M, Cp), Cc);
-      M = M * Mp;
-    }
+  // wavefront scan: within 16-lane rows by DPP, then across the four rows
+  // from their totals (lanes 15, 31, 47); yp = this lane's carry-in
+  FWD_STEP(kShr1)
+  FWD_STEP(kShr2)
+  FWD_STEP(kShr4)
+  FWD_STEP(kShr8)
+  double2 yp;
+  {
+    const double m0 = rdlane(M, 15), m1 = rdlane(M, 31), m2 = rdlane(M, 47);
+    const double2 c0 = make_double2(rdlane(Cc.x, 15), rdlane(Cc.y, 15));
+    const double2 c1 = make_double2(rdlane(Cc.x, 31), rdlane(Cc.y, 31));
+    const double2 c2 = make_double2(rdlane(Cc.x, 47), rdlane(Cc.y, 47));
+    const double2 p1 = cadd(rmul(m1, c0), c1);  // end of row 1
+    const double2 p2 = cadd(rmul(m2, p1), c2);  // end of row 2
+    (void)m0;
+    const int row = lane >> 4;
+    const double2 prev = row == 0 ? make_double2(0.0, 0.0) : row == 1 ? c0 : row == 2 ? p1 : p2;
+    Cc = cadd(rmul(M, prev), Cc);  // inclusive over the wavefront
+    yp = make_double2(dpp_d<kShr1>(Cc.x, 0.0), dpp_d<kShr1>(Cc.y, 0.0));
+    if ((lane & 15) == 0) yp = prev;
   }
-  double2 yp = shfl_up2(Cc, 1);
-  if (lane == 0) yp = make_double2(0.0, 0.0);
   double2 y[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
     if (j <= n2m) {
-      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(mg.as[j], yp)));
+      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(asj[r], yp)));
       y[r] = yp;
     } else {
       y[r] = make_double2(0.0, 0.0);
     }
   }
   // backward substitution x_j = y_j - gam_{j+1} x_{j+1}, x_n2m = y_n2m
-  double gm[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int j = j0 + r;
-    gm[r] = j < n2m ? gam[j + 1] : 0.0;
-  }
   double Nn = 1.0;
   double2 D = make_double2(0.0, 0.0);
 #pragma unroll
@@ -375,17 +424,24 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
       Nn = nj * Nn;
     }
   }
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double Np = __shfl_down(Nn, d, 64);
-    const double2 Dp = shfl_down2(D, d);
-    if (lane + d < 64) {
-      D = cadd(rmul(Nn, Dp), D);
-      Nn = Nn * Np;
-    }
+  BWD_STEP(kShl1)
+  BWD_STEP(kShl2)
+  BWD_STEP(kShl4)
+  BWD_STEP(kShl8)
+  double2 xn;
+  {
+    const double n1 = rdlane(Nn, 16), n2 = rdlane(Nn, 32);
+    const double2 d1 = make_double2(rdlane(D.x, 16), rdlane(D.y, 16));
+    const double2 d2 = make_double2(rdlane(D.x, 32), rdlane(D.y, 32));
+    const double2 d3 = make_double2(rdlane(D.x, 48), rdlane(D.y, 48));
+    const double2 s2 = cadd(rmul(n2, d3), d2);  // start of row 2
+    const double2 s1 = cadd(rmul(n1, s2), d1);  // start of row 1
+    const int row = lane >> 4;
+    const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
+    D = cadd(rmul(Nn, next), D);  // inclusive suffix over the wavefront
+    xn = make_double2(dpp_d<kShl1>(D.x, 0.0), dpp_d<kShl1>(D.y, 0.0));
+    if ((lane & 15) == 15) xn = next;
   }
-  double2 xn = shfl_down2(D, 1);
-  if (lane == 63) xn = make_double2(0.0, 0.0);
   double2 *out = ROW(gg, i);
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
@@ -470,7 +526,7 @@ __device__ inline void lev_rr(const Geo &g, const MG &mg, int kv, int ilev, int 
   double2 *rsb = mg.resd + pl;
   const int ia = mg.iimg[ilev][1] + 2 * t, ib = ia + 1;
   const int ic = mg.iimg[ilev - 1][1] + t;
-  const double2 *ga = ROW(gg, ia), *gb = ROW(gg, ib), *gam_ = ROW(gg, mg.imm[ia]), *gbp = ROW(gg, mg.ipm[ib]);
+  const double2 *ga = ROW(gg, ia), *gb = ROW(gg, ib), *gam_ = ROW(gg, ia - 1), *gbp = ROW(gg, ib + 1);
   const double *aca = ROW(acb, ia), *acbb = ROW(acb, ib);
   double2 *ra = ROW(rsb, ia), *rb = ROW(rsb, ib), *rc = ROW(rsb, ic);
   const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
@@ -501,7 +557,7 @@ __global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
 // Each stage ends at a workgroup barrier; the levels are small enough that
 // the barrier is far cheaper than a kernel boundary.
 // ---------------------------------------------------------------------------
-constexpr int kCoarseWaves = 16;
+constexpr int kCoarseWaves = 8;
 
 template <int R, int MODE>
 __device__ inline void wg_sweep(const Geo &g, const MG &mg, int kv, int ibase, int nlines, double oldv, int clev) {
@@ -585,10 +641,10 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
     }
     // neighbours: imm(ia) = 2t (even), ipm(ib) = 2t+3 (odd, shifted unless
     // beyond the top level)
-    const int ibp = mg.ipm[ib];
+    const int ibp = ib + 1;
     const bool sbp = sh1 && ibp <= g.n1m;
     const double2 *ga = ROW(gg, ia);
-    const double2 *gb = ROW(gg, ib), *gam_ = ROW(gg, mg.imm[ia]), *gbp = ROW(gg, ibp);
+    const double2 *gb = ROW(gg, ib), *gam_ = ROW(gg, ia - 1), *gbp = ROW(gg, ibp);
     const double *aca = ROW(mg.ac + pl, ia), *acbb = ROW(mg.ac + pl, ib);
     const double2 *rha = ROW(mg.rhs + ppl, ia), *rhb = ROW(mg.rhs + ppl, ib);
     double2 *ra = ROW(mg.resd + pl, ia), *rb = ROW(mg.resd + pl, ib), *rc = ROW(mg.resd + pl, ic);

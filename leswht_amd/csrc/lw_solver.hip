@@ -431,7 +431,6 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   const int n1md = mg.n1md, n3mh = mg.n3mh;
   std::vector<double> aw(n1md + 2), ae(n1md + 2), as(n2 + 2), an(n2 + 2), coi1(n1md + 2), coi2(n1md + 2),
       cor1(n1md + 2), cor2(n1md + 2), ak3(n3mh + 2);
-  std::vector<int> ipm(n1md + 2), imm(n1md + 2);
   std::vector<double> ac((size_t)n3mh * mg.PLANE, 0.0), gam(ac.size(), 0.0), bet(ac.size(), 0.0);
   auto AC = [&](int i, int j, int k) -> double & { return ac[(size_t)(k - 1) * mg.PLANE + (size_t)i * mg.LDJ + j]; };
   auto GAMh = [&](int i, int j, int k) -> double & { return gam[(size_t)(k - 1) * mg.PLANE + (size_t)i * mg.LDJ + j]; };
@@ -520,11 +519,6 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     mg.lc = -1;
     for (int ilev = 0; ilev <= nlev - 1; ++ilev)
       if (iimg[ilev][3] <= coarse_rows) mg.lc = ilev;
-    for (int ilev = nlev; ilev >= 0; --ilev)
-      for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) {
-        ipm[i] = i + 1;
-        imm[i] = i - 1;
-      }
     for (int ilev = 0; ilev <= nlev; ++ilev)
       for (int k = 1; k <= n3mh; ++k) {
         for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) BETh(i, 1, k) = 1.0 / AC(i, 1, k);
@@ -537,7 +531,6 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   }
   mg.aw = h->upload(aw); mg.ae = h->upload(ae); mg.as = h->upload(as); mg.an = h->upload(an);
   mg.coi1 = h->upload(coi1); mg.coi2 = h->upload(coi2); mg.cor1 = h->upload(cor1); mg.cor2 = h->upload(cor2);
-  mg.ipm = h->upload(ipm); mg.imm = h->upload(imm);
   mg.ac = h->upload(ac); mg.gam = h->upload(gam); mg.bet = h->upload(bet);
   mg.gg = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
   mg.resd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
