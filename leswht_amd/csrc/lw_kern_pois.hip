@@ -290,6 +290,99 @@ __device__ inline Prolong make_prolong(const MG &mg, const double2 *gg, int ic, 
   return p;
 }
 
+// forward and back substitution of one line whose right-hand side and LU
+// factors sit in registers: lane `lane` owns rows j0..j0+R-1 (j0 = 1 + lane*R);
+// rows past n2m carry zero factors and return x = 0
+template <int R>
+__device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], const double (&asj)[R],
+                                const double (&gm)[R], int j0, int n2m, int lane, double2 (&x)[R]) {
+  // forward substitution y_j = (rr_j - as_j y_{j-1}) bet_j as affine maps
+  // y -> M y + C composed over this lane's rows
+  double M = 1.0;
+  double2 Cc = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = j0 + r;
+    if (j <= n2m) {
+      const double m = j == 1 ? 0.0 : -(asj[r] * bt[r]);
+      Cc = cadd(rmul(m, Cc), rmul(bt[r], rr[r]));
+      M = m * M;
+    }
+  }
+  // wavefront scan: within 16-lane rows by DPP, then across the four rows
+  // from their totals (lanes 15, 31, 47); yp = this lane's carry-in
+  FWD_STEP(kShr1)
+  FWD_STEP(kShr2)
+  FWD_STEP(kShr4)
+  FWD_STEP(kShr8)
+  double2 yp;
+  {
+    const double m0 = rdlane(M, 15), m1 = rdlane(M, 31), m2 = rdlane(M, 47);
+    const double2 c0 = make_double2(rdlane(Cc.x, 15), rdlane(Cc.y, 15));
+    const double2 c1 = make_double2(rdlane(Cc.x, 31), rdlane(Cc.y, 31));
+    const double2 c2 = make_double2(rdlane(Cc.x, 47), rdlane(Cc.y, 47));
+    const double2 p1 = cadd(rmul(m1, c0), c1);  // end of row 1
+    const double2 p2 = cadd(rmul(m2, p1), c2);  // end of row 2
+    (void)m0;
+    const int row = lane >> 4;
+    const double2 prev = row == 0 ? make_double2(0.0, 0.0) : row == 1 ? c0 : row == 2 ? p1 : p2;
+    Cc = cadd(rmul(M, prev), Cc);  // inclusive over the wavefront
+    yp = make_double2(dpp_d<kShr1>(Cc.x, 0.0), dpp_d<kShr1>(Cc.y, 0.0));
+    if ((lane & 15) == 0) yp = prev;
+  }
+  double2 y[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = j0 + r;
+    if (j <= n2m) {
+      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(asj[r], yp)));
+      y[r] = yp;
+    } else {
+      y[r] = make_double2(0.0, 0.0);
+    }
+  }
+  // backward substitution x_j = y_j - gam_{j+1} x_{j+1}, x_n2m = y_n2m
+  double Nn = 1.0;
+  double2 D = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    const int j = j0 + r;
+    if (j <= n2m) {
+      const double nj = j == n2m ? 0.0 : -gm[r];
+      D = cadd(rmul(nj, D), y[r]);
+      Nn = nj * Nn;
+    }
+  }
+  BWD_STEP(kShl1)
+  BWD_STEP(kShl2)
+  BWD_STEP(kShl4)
+  BWD_STEP(kShl8)
+  double2 xn;
+  {
+    const double n1 = rdlane(Nn, 16), n2 = rdlane(Nn, 32);
+    const double2 d1 = make_double2(rdlane(D.x, 16), rdlane(D.y, 16));
+    const double2 d2 = make_double2(rdlane(D.x, 32), rdlane(D.y, 32));
+    const double2 d3 = make_double2(rdlane(D.x, 48), rdlane(D.y, 48));
+    const double2 s2 = cadd(rmul(n2, d3), d2);  // start of row 2
+    const double2 s1 = cadd(rmul(n1, s2), d1);  // start of row 1
+    const int row = lane >> 4;
+    const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
+    D = cadd(rmul(Nn, next), D);  // inclusive suffix over the wavefront
+    xn = make_double2(dpp_d<kShl1>(D.x, 0.0), dpp_d<kShl1>(D.y, 0.0));
+    if ((lane & 15) == 15) xn = next;
+  }
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    const int j = j0 + r;
+    if (j <= n2m) {
+      xn = j == n2m ? y[r] : csub(y[r], rmul(gm[r], xn));
+      x[r] = xn;
+    } else {
+      x[r] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // one zebra line: y-tridiagonal solve with the LU factors of line i
 // (trdiag1m, poiss_z_ft.f90:383-408). MODE selects the right-hand side:
@@ -367,90 +460,12 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
     asj[r] = a;
     gm[r] = okg * gnext;
   }
-  // forward substitution y_j = (rr_j - as_j y_{j-1}) bet_j as affine maps
-  // y -> M y + C composed over this lane's rows
-  double M = 1.0;
-  double2 Cc = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int j = j0 + r;
-    if (j <= n2m) {
-      const double m = j == 1 ? 0.0 : -(asj[r] * bt[r]);
-      Cc = cadd(rmul(m, Cc), rmul(bt[r], rr[r]));
-      M = m * M;
-    }
-  }
-  // wavefront scan: within 16-lane rows by DPP, then across the four rows
-  // from their totals (lanes 15, 31, 47); yp = this lane's carry-in
-  FWD_STEP(kShr1)
-  FWD_STEP(kShr2)
-  FWD_STEP(kShr4)
-  FWD_STEP(kShr8)
-  double2 yp;
-  {
-    const double m0 = rdlane(M, 15), m1 = rdlane(M, 31), m2 = rdlane(M, 47);
-    const double2 c0 = make_double2(rdlane(Cc.x, 15), rdlane(Cc.y, 15));
-    const double2 c1 = make_double2(rdlane(Cc.x, 31), rdlane(Cc.y, 31));
-    const double2 c2 = make_double2(rdlane(Cc.x, 47), rdlane(Cc.y, 47));
-    const double2 p1 = cadd(rmul(m1, c0), c1);  // end of row 1
-    const double2 p2 = cadd(rmul(m2, p1), c2);  // end of row 2
-    (void)m0;
-    const int row = lane >> 4;
-    const double2 prev = row == 0 ? make_double2(0.0, 0.0) : row == 1 ? c0 : row == 2 ? p1 : p2;
-    Cc = cadd(rmul(M, prev), Cc);  // inclusive over the wavefront
-    yp = make_double2(dpp_d<kShr1>(Cc.x, 0.0), dpp_d<kShr1>(Cc.y, 0.0));
-    if ((lane & 15) == 0) yp = prev;
-  }
-  double2 y[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int j = j0 + r;
-    if (j <= n2m) {
-      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(asj[r], yp)));
-      y[r] = yp;
-    } else {
-      y[r] = make_double2(0.0, 0.0);
-    }
-  }
-  // backward substitution x_j = y_j - gam_{j+1} x_{j+1}, x_n2m = y_n2m
-  double Nn = 1.0;
-  double2 D = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int r = R - 1; r >= 0; --r) {
-    const int j = j0 + r;
-    if (j <= n2m) {
-      const double nj = j == n2m ? 0.0 : -gm[r];
-      D = cadd(rmul(nj, D), y[r]);
-      Nn = nj * Nn;
-    }
-  }
-  BWD_STEP(kShl1)
-  BWD_STEP(kShl2)
-  BWD_STEP(kShl4)
-  BWD_STEP(kShl8)
-  double2 xn;
-  {
-    const double n1 = rdlane(Nn, 16), n2 = rdlane(Nn, 32);
-    const double2 d1 = make_double2(rdlane(D.x, 16), rdlane(D.y, 16));
-    const double2 d2 = make_double2(rdlane(D.x, 32), rdlane(D.y, 32));
-    const double2 d3 = make_double2(rdlane(D.x, 48), rdlane(D.y, 48));
-    const double2 s2 = cadd(rmul(n2, d3), d2);  // start of row 2
-    const double2 s1 = cadd(rmul(n1, s2), d1);  // start of row 1
-    const int row = lane >> 4;
-    const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
-    D = cadd(rmul(Nn, next), D);  // inclusive suffix over the wavefront
-    xn = make_double2(dpp_d<kShl1>(D.x, 0.0), dpp_d<kShl1>(D.y, 0.0));
-    if ((lane & 15) == 15) xn = next;
-  }
+  double2 x[R];
+  tri_regs<R>(rr, bt, asj, gm, j0, n2m, lane, x);
   double2 *out = ROW(gg, i);
 #pragma unroll
-  for (int r = R - 1; r >= 0; --r) {
-    const int j = j0 + r;
-    if (j <= n2m) {
-      xn = j == n2m ? y[r] : csub(y[r], rmul(gm[r], xn));
-      out[j] = xn;
-    }
-  }
+  for (int r = 0; r < R; ++r)
+    if (j0 + r <= n2m) out[j0 + r] = x[r];
 }
 
 // one wavefront per line, four lines per workgroup, all wavenumbers
@@ -593,6 +608,291 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// levels 1 and 0 of a V-cycle with the solution planes of both levels in
+// LDS. Wavefront w owns the level-1 lines 2w, 2w+1 and the level-0 line w
+// (level 1 has twice as many lines as level 0): their right-hand sides and
+// residuals stay in its registers from the first stage to the last, and only
+// the solutions, which the neighbouring lines read, pass through LDS. The
+// stages are those of k_mg_coarse with lc = 1, in the reference's order:
+//   relax(1, oldv 0): the first half has no neighbour term
+//   godown(1): residual of both lines, restriction into line w of level 0
+//   relax(0, NBLI sweeps, oldv 0 on the first)
+//   goup(0) folded into relax(1, oldv 1), as in k_line mode 4
+// Only the new level-1 solution leaves the kernel (the next finer level's
+// prolongation reads it); every other value is rebuilt by the next cycle.
+// LDS slot of row j0 + r of local line q for lane l: (q*R + r)*64 + l, so a
+// wavefront reads or writes one row index of a line as one contiguous block.
+// ---------------------------------------------------------------------------
+constexpr int kCoarseLdsWaves = 12;
+constexpr int kCoarseLdsBytes = 160 * 1024;
+
+template <int R>
+struct LineLU {
+  double bt[R], gm[R];
+};
+
+template <int R>
+__device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, LineLU<R> &c) {
+  const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = j0 + r;
+    const int jc = j <= n2m ? j : n2m;
+    const double okf = j <= n2m ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
+    c.bt[r] = okf * bet[jc];
+    c.gm[r] = okg * gam[jc < n2m ? jc + 1 : n2m];
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
+  extern __shared__ double2 lds[];
+  const int kv = 1 + blockIdx.x;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int n2m = g.n2m;
+  const int b1 = mg.iimg[1][1], n1 = mg.iimg[1][2] - b1 + 1;
+  const int b0 = mg.iimg[0][1], n0 = mg.iimg[0][2] - b0 + 1;
+  double2 *L1 = lds, *L0 = lds + (long)n1 * R * 64;
+  auto slot = [&](int q, int r) { return (q * R + r) * 64 + lane; };
+  const long pl = (long)(kv - 1) * mg.PLANE;
+  double2 *gg = mg.gg + pl;
+  const int la = 2 * w, lb = la + 1;  // local level-1 lines
+  const int ia = b1 + la, ib = ia + 1, ic = b0 + w;
+  const int j0 = 1 + lane * R;
+  const double2 z2 = make_double2(0.0, 0.0);
+
+  // registers held through the kernel: as, the level-0 line's right-hand
+  // side and LU factors. The level-1 residuals go to resd, as godown leaves
+  // them, and come back with the level-1 factors for the up-sweep (the
+  // register file holds twelve wavefronts)
+  double asj[R];
+  double2 resA[R], resB[R], res0[R];
+  LineLU<R> cA, cB, c0;
+  {
+    const double2 *ra = ROW(mg.resd + pl, ia), *rb = ROW(mg.resd + pl, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      const int jc = j <= n2m ? j : n2m;
+      const double okf = j <= n2m ? 1.0 : 0.0;
+      asj[r] = mg.as[jc];
+      resA[r] = rmul(okf, ra[jc]);
+      resB[r] = rmul(okf, rb[jc]);
+    }
+    load_lu<R>(mg, pl, ia, j0, n2m, cA);
+    load_lu<R>(mg, pl, ib, j0, n2m, cB);
+  }
+  double2 rr[R], x[R];
+
+  // relax(1), first half, oldv = 0: the right-hand side is resd itself
+#pragma unroll
+  for (int r = 0; r < R; ++r) rr[r] = resA[r];
+  tri_regs<R>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
+#pragma unroll
+  for (int r = 0; r < R; ++r) L1[slot(la, r)] = x[r];
+  __syncthreads();
+
+  // relax(1), second half: resd - aw*g(i-1) - ae*g(i+1)
+  {
+    const double aw = mg.aw[ib], ae = mg.ae[ib];
+    const bool hp = lb + 1 < n1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 gmv = L1[slot(la, r)];
+      const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
+    }
+    tri_regs<R>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) L1[slot(lb, r)] = x[r];
+  }
+  __syncthreads();
+
+  // godown(1): residuals of lines 2w and 2w+1, written to resd as godown
+  // leaves them, and restricted into line w of level 0 (res0). One line at a
+  // time: the scheduling barrier keeps the second line's loads from being
+  // hoisted over the first line's arithmetic.
+  {
+    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    double anj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) anj[r] = mg.an[j0 + r <= n2m ? j0 + r : n2m];
+    {
+      const double aw = mg.aw[ia], ae = mg.ae[ia];
+      const double *ac = ROW(mg.ac + pl, ia);
+      double2 *out = ROW(mg.resd + pl, ia);
+      const double2 lo = lane > 0 ? L1[(la * R + R - 1) * 64 + lane - 1] : z2;
+      const double2 hi = lane < 63 ? L1[(la * R) * 64 + lane + 1] : z2;
+      double2 gi[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) gi[r] = L1[slot(la, r)];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int jc = j0 + r <= n2m ? j0 + r : n2m;
+        const double2 gmv = la > 0 ? L1[slot(la - 1, r)] : z2;
+        const double2 gpv = L1[slot(lb, r)];
+        const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
+        double2 u = resA[r];
+        u = csub(u, rmul(aw, gmv));
+        u = csub(u, rmul(ae, gpv));
+        u = csub(u, rmul(asj[r], dn));
+        u = csub(u, rmul(anj[r], up));
+        u = csub(u, rmul(ac[jc], gi[r]));
+        resA[r] = u;
+        if (j0 + r <= n2m) out[j0 + r] = u;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      const double aw = mg.aw[ib], ae = mg.ae[ib];
+      const double *ac = ROW(mg.ac + pl, ib);
+      double2 *out = ROW(mg.resd + pl, ib);
+      const bool hp = lb + 1 < n1;
+      const double2 lo = lane > 0 ? L1[(lb * R + R - 1) * 64 + lane - 1] : z2;
+      const double2 hi = lane < 63 ? L1[(lb * R) * 64 + lane + 1] : z2;
+      double2 gi[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) gi[r] = L1[slot(lb, r)];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int jc = j0 + r <= n2m ? j0 + r : n2m;
+        const double2 gmv = L1[slot(la, r)];
+        const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+        const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
+        double2 v = resB[r];
+        v = csub(v, rmul(aw, gmv));
+        v = csub(v, rmul(ae, gpv));
+        v = csub(v, rmul(asj[r], dn));
+        v = csub(v, rmul(anj[r], up));
+        v = csub(v, rmul(ac[jc], gi[r]));
+        if (j0 + r <= n2m) out[j0 + r] = v;
+        res0[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, cadd(rmul(c1, resA[r]), rmul(c2, v)));
+      }
+    }
+  }
+
+  // relax(0, NBLI): first half on even lines, second half on odd lines
+  load_lu<R>(mg, pl, ic, j0, n2m, c0);
+  {
+    const double aw = mg.aw[ic], ae = mg.ae[ic];
+    const bool hm = w > 0, hp = w + 1 < n0;
+    for (int it = 0; it < mg.nbli; ++it) {
+      if ((w & 1) == 0) {
+        if (it == 0) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) rr[r] = res0[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double2 gmv = hm ? L0[slot(w - 1, r)] : z2;
+            const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
+            rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(res0[r], rmul(1.0, cadd(rmul(aw, gmv), rmul(ae, gpv)))));
+          }
+        }
+        tri_regs<R>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
+#pragma unroll
+        for (int r = 0; r < R; ++r) L0[slot(w, r)] = x[r];
+      }
+      __syncthreads();
+      if (w & 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double2 gmv = L0[slot(w - 1, r)];
+          const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
+          rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(res0[r], rmul(aw, gmv)), rmul(ae, gpv)));
+        }
+        tri_regs<R>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
+#pragma unroll
+        for (int r = 0; r < R; ++r) L0[slot(w, r)] = x[r];
+      }
+      __syncthreads();
+    }
+  }
+
+  // goup(0) + relax(1), first half: the neighbours 2w-1 and 2w+1 of line 2w
+  // are the destinations of level-0 rows w-1 and w, prolonged as they are read
+  load_lu<R>(mg, pl, ia, j0, n2m, cA);
+  {
+    const double2 *ra = ROW(mg.resd + pl, ia);
+#pragma unroll
+    for (int r = 0; r < R; ++r) resA[r] = ra[j0 + r <= n2m ? j0 + r : n2m];
+  }
+  {
+    const double aw = mg.aw[ia], ae = mg.ae[ia];
+    const bool hm = w > 0, plast = w + 1 == n0;
+    const double pm1 = hm ? mg.coi1[ic - 1] : 0.0, pm2 = hm ? mg.coi2[ic - 1] : 0.0;
+    const double pp1 = mg.coi1[ic], pp2 = mg.coi2[ic];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 vm = z2;
+      if (hm) vm = cadd(cadd(L1[slot(la - 1, r)], rmul(pm1, L0[slot(w - 1, r)])), rmul(pm2, L0[slot(w, r)]));
+      double2 vp = cadd(L1[slot(lb, r)], rmul(pp1, L0[slot(w, r)]));
+      if (!plast) vp = cadd(vp, rmul(pp2, L0[slot(w + 1, r)]));
+      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(resA[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp)))));
+      __builtin_amdgcn_sched_barrier(0);  // five rows of reads per row: one row at a time
+    }
+  }
+  // the new even rows are written where no wavefront of this stage reads
+  tri_regs<R>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
+  {
+    double2 *out = ROW(gg, ia);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      L1[slot(la, r)] = x[r];
+      if (j0 + r <= n2m) out[j0 + r] = x[r];
+    }
+  }
+  __syncthreads();
+
+  // relax(1), second half
+  load_lu<R>(mg, pl, ib, j0, n2m, cB);
+  {
+    const double2 *rb = ROW(mg.resd + pl, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r) resB[r] = rb[j0 + r <= n2m ? j0 + r : n2m];
+  }
+  {
+    const double aw = mg.aw[ib], ae = mg.ae[ib];
+    const bool hp = lb + 1 < n1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 gmv = L1[slot(la, r)];
+      const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
+    }
+    tri_regs<R>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
+    double2 *out = ROW(gg, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (j0 + r <= n2m) out[j0 + r] = x[r];
+  }
+}
+
+typedef void (*CoarseLdsKernel)(Geo, MG);
+
+static CoarseLdsKernel pick_coarse_lds(int R) {
+  switch (R) {
+    case 1: return k_mg_coarse_lds<1>;
+    case 2: return k_mg_coarse_lds<2>;
+    case 4: return k_mg_coarse_lds<4>;
+    default: return nullptr;
+  }
+}
+
+// LDS bytes of k_mg_coarse_lds, or 0 when the levels do not fit its shape
+// (lc = 1, at most kCoarseLdsWaves level-0 lines, R <= 4, planes in LDS)
+static long coarse_lds_bytes(const Geo &g, const MG &mg) {
+  if (mg.lc != 1) return 0;
+  const int R = lanes_r(g.n2m);
+  if (R > 4) return 0;
+  const int n1 = mg.iimg[1][2] - mg.iimg[1][1] + 1, n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
+  if (n1 != 2 * n0 || n0 < 1 || n0 > kCoarseLdsWaves) return 0;
+  const long bytes = (long)(n1 + n0) * R * 64 * (long)sizeof(double2);
+  return bytes <= kCoarseLdsBytes ? bytes : 0;
+}
+
 typedef void (*CoarseKernel)(Geo, MG, int);
 
 static CoarseKernel pick_coarse(int R) {
@@ -605,6 +905,14 @@ static CoarseKernel pick_coarse(int R) {
   }
 }
 
+// once per handle, before the first cycle: dynamic LDS beyond 64 KiB must be
+// allowed explicitly
+hipError_t prepare_mg_kernels(const Geo &g, const MG &mg) {
+  if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
+  return hipFuncSetAttribute((const void *)pick_coarse_lds(lanes_r(g.n2m)),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+}
+
 static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   // per level and cycle: down and up sweeps 4 x 64 B, residual+restriction
   // 64 B, per point; level 0: NBLI x 2 x 64 B
@@ -612,6 +920,13 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   for (int l = 0; l <= mg.lc; ++l) {
     const double pts = (double)(mg.iimg[l][2] - mg.iimg[l][1] + 1) * g.n2m;
     per_kv += l == 0 ? pts * 64.0 * mg.nbli : pts * (64.0 * 2 + 64.0);
+  }
+  const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
+  if (lds > 0) {
+    const int n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
+    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m)), dim3((unsigned)mg.n3mh), dim3(64, n0),
+              (unsigned)lds, s, g, mg);
+    return;
   }
   LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m)), dim3((unsigned)mg.n3mh), dim3(64, kCoarseWaves),
             0, s, g, mg, mg.lc);

@@ -519,6 +519,9 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     mg.lc = -1;
     for (int ilev = 0; ilev <= nlev - 1; ++ilev)
       if (iimg[ilev][3] <= coarse_rows) mg.lc = ilev;
+    // LW_MG_COARSE_LDS=0 keeps the coarse levels in global memory
+    mg.coarse_lds = 1;
+    if (const char *e = getenv("LW_MG_COARSE_LDS")) mg.coarse_lds = atoi(e) != 0;
     for (int ilev = 0; ilev <= nlev; ++ilev)
       for (int k = 1; k <= n3mh; ++k) {
         for (int i = iimg[ilev][1]; i <= iimg[ilev][2]; ++i) BETh(i, 1, k) = 1.0 / AC(i, 1, k);
@@ -533,6 +536,7 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.coi1 = h->upload(coi1); mg.coi2 = h->upload(coi2); mg.cor1 = h->upload(cor1); mg.cor2 = h->upload(cor2);
   mg.ac = h->upload(ac); mg.gam = h->upload(gam); mg.bet = h->upload(bet);
   mg.gg = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+  HIPCHK(lw::prepare_mg_kernels(g, mg));
   mg.resd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
   mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);

@@ -249,3 +249,21 @@ def test_gpu_post_projection_divergence(oracle_lib):
     dg = Solver.from_case(view).step(1)
     assert dg["dvmax"] <= 2.0 * do.dvmax + 1e-14, (dg["dvmax"], do.dvmax)
     assert dg["dvmax"] <= 5e-11
+
+
+@pytest.mark.parametrize("dims,nlev", [((96, 256, 8), 3), ((48, 128, 8), 2)])
+def test_gpu_lds_coarse_levels_equal_global_ones(dims, nlev, monkeypatch):
+    """Levels 1 and 0 with LDS-resident planes (4 and 2 rows per lane here)
+    give the fields of the global-memory coarse kernel, which the oracle
+    tests above pin: the two differ at most in the sign of exact zeros."""
+    view = _view(dims, "les", 2, nlev)
+    print("coarse lds", dims, flush=True)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("LW_MG_COARSE_LDS", flag)
+        g = Solver.from_case(view)
+        st = g.step(2)
+        out.append((st["mg_cycles_max"], st["mg_cycles_sum"], g.fields()))
+    assert out[0][:2] == out[1][:2]
+    for k in "uvwpt":
+        assert np.array_equal(out[0][2][k], out[1][2][k]), k
