@@ -72,6 +72,8 @@ struct MG {
   double *aw, *ae, *as, *an, *coi1, *coi2, *cor1, *cor2;
   double *ac, *gam, *bet;  // [kv][i][j], real
   double2 *gg, *resd;      // [kv][i][j], all levels stacked along i
+  double2 *gd, *rsu;       // fused kernels: down-sweep second halves, residuals
+  int fused;               // 1: one kernel per level and direction
   double2 *pc, *rhs;       // [kv][i][j], top level
   unsigned long long *sumres;  // n3mh max-norm bit patterns, then tt (2 doubles)
   int *done;               // n3mh: 0 iterating, 1 converged, 2 MGITR reached

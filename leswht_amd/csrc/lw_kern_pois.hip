@@ -905,14 +905,6 @@ static CoarseKernel pick_coarse(int R) {
   }
 }
 
-// once per handle, before the first cycle: dynamic LDS beyond 64 KiB must be
-// allowed explicitly
-hipError_t prepare_mg_kernels(const Geo &g, const MG &mg) {
-  if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
-  return hipFuncSetAttribute((const void *)pick_coarse_lds(lanes_r(g.n2m)),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
-}
-
 static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   // per level and cycle: down and up sweeps 4 x 64 B, residual+restriction
   // 64 B, per point; level 0: NBLI x 2 x 64 B
@@ -1021,14 +1013,389 @@ __global__ void k_mg_check(MG mg, double test, int id) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// fused level kernels: one launch per level and direction instead of one per
+// zebra half and one per residual. A workgroup owns a tile of T consecutive
+// lines of its level (T even, dividing the level) and recomputes the few
+// halo lines its last stage needs, so no workgroup waits for another; the
+// halo solves are the same computation as the owner's and give the same
+// values. Stages inside the workgroup meet at barriers, solutions pass
+// through LDS in the lane-major slot layout of k_mg_coarse_lds.
+//   k_mg_down(L): relax(L, oldv 0) + godown(L) (poiss_z_ft.f90:272-275)
+//   k_mg_up(L):   goup(L-1) + relax(L, oldv 1) (poiss_z_ft.f90:279-282)
+//   k_mg_top2:    second half of toplevel + residual + restriction + pc
+//                 (poiss_z_ft.f90:328-377)
+// The down-sweep's second-half lines, the rows goup adds to, go to mg.gd
+// rather than mg.gg, so the up kernel of a tile reads its neighbours' down
+// values while those neighbours write their up values to mg.gg.
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ inline void load_rows(const double2 *row, int j0, int n2m, double2 (&v)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = row[j0 + r <= n2m ? j0 + r : n2m];
+}
+
+template <int R>
+__device__ inline void store_rows(double2 *row, int j0, int n2m, const double2 (&v)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (j0 + r <= n2m) row[j0 + r] = v[r];
+}
+
+// tile size of the fused kernels on a level of n lines
+static inline int fused_tile(int n) { return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : 2; }
+
+// the residual at rows j0+r of the two lines whose LDS slots are qa, qa+1
+// (neighbours qa-1, qa+2), in the reference's order (poiss_z_ft.f90:440-443)
+#define TILE_AT(q_, r_) lds[((q_) * R + (r_)) * 64 + lane]
+#define TILE_DN(q_, r_) \
+  ((r_) > 0 ? lds[((q_) * R + (r_) - 1) * 64 + lane] : lane > 0 ? lds[((q_) * R + R - 1) * 64 + lane - 1] : z2)
+#define TILE_UP(q_, r_) \
+  ((r_) < R - 1 ? lds[((q_) * R + (r_) + 1) * 64 + lane] : lane < 63 ? lds[((q_) * R) * 64 + lane + 1] : z2)
+
+// godown (poiss_z_ft.f90:431-458) on a tile of level `ilev` >= 1: the relax
+// stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
+// outside the level as zeros); then residual and restriction per pair.
+template <int R>
+__global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T) {
+  extern __shared__ double2 lds[];
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
+  const int n2m = g.n2m, j0 = 1 + lane * R;
+  const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
+  const int f0 = blockIdx.x * T;
+  const long pl = (long)(kv - 1) * mg.PLANE;
+  const double2 z2 = make_double2(0.0, 0.0);
+  double asj[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  double2 x[R];
+
+  // first half (oldv 0, no neighbour term): line f0-2+2w for w <= T/2+1
+  const int e = f0 - 2 + 2 * w;
+  const bool eok = w <= T / 2 + 1 && e >= 0 && e < nL;
+  // second half: line f0-1+2w for w <= T/2; its inputs are loaded now
+  const int o = f0 - 1 + 2 * w;
+  const bool ook = w <= T / 2 && o >= 0;
+  double2 so[R];
+  LineLU<R> co;
+  if (ook) {
+    load_rows<R>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
+    load_lu<R>(mg, pl, bL + o, j0, n2m, co);
+  }
+  if (eok) {
+    double2 se[R];
+    LineLU<R> ce;
+    load_rows<R>(ROW(mg.resd + pl, bL + e), j0, n2m, se);
+    load_lu<R>(mg, pl, bL + e, j0, n2m, ce);
+    tri_regs<R>(se, ce.bt, asj, ce.gm, j0, n2m, lane, x);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+  if (w <= T / 2 + 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w, r) = x[r];
+  }
+  __syncthreads();
+
+  // second half: resd - aw*g(i-1) - ae*g(i+1)
+  if (ook) {
+    const double aw = mg.aw[bL + o], ae = mg.ae[bL + o];
+    double2 rr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(2 * w, r))), rmul(ae, TILE_AT(2 * w + 2, r)));
+    tri_regs<R>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+    if (w >= 1) store_rows<R>(ROW(mg.gd + pl, bL + o), j0, n2m, x);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+  if (w <= T / 2) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w + 1, r) = x[r];
+  }
+  __syncthreads();
+
+  // residual of lines a = f0+2p, b = a+1 in place, coarse row bC + f0/2 + p
+  const int U = (T / 2) * R;
+  for (int u = w; u < U; u += W) {
+    const int p = u / R, r = u % R;
+    const int j = j0 + r;
+    if (j > n2m) continue;
+    const int qa = 2 * p + 2, ia = bL + f0 + 2 * p, ib = ia + 1, ic = bC + f0 / 2 + p;
+    const double an = mg.an[j];
+    const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
+    double2 xa = ROW(mg.resd + pl, ia)[j];
+    xa = csub(xa, rmul(mg.aw[ia], TILE_AT(qa - 1, r)));
+    xa = csub(xa, rmul(mg.ae[ia], gb));
+    xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
+    xa = csub(xa, rmul(an, TILE_UP(qa, r)));
+    xa = csub(xa, rmul(ROW(mg.ac + pl, ia)[j], ga));
+    double2 xb = ROW(mg.resd + pl, ib)[j];
+    xb = csub(xb, rmul(mg.aw[ib], ga));
+    xb = csub(xb, rmul(mg.ae[ib], TILE_AT(qa + 2, r)));
+    xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
+    xb = csub(xb, rmul(an, TILE_UP(qa + 1, r)));
+    xb = csub(xb, rmul(ROW(mg.ac + pl, ib)[j], gb));
+    // the residual goes to rsu: neighbouring tiles still read resd rows of
+    // this level as their halo lines' right-hand sides
+    ROW(mg.rsu + pl, ia)[j] = xa;
+    ROW(mg.rsu + pl, ib)[j] = xb;
+    ROW(mg.resd + pl, ic)[j] = cadd(rmul(mg.cor1[ic], xa), rmul(mg.cor2[ic], xb));
+  }
+}
+
+// goup(ilev-1) + relax(ilev, oldv 1) on a tile: first half on lines f0+2w
+// (w <= T/2, the last one a halo) with its neighbours prolonged as read,
+// then the second half on the tile's odd lines
+template <int R>
+__global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
+  extern __shared__ double2 lds[];
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int n2m = g.n2m, j0 = 1 + lane * R;
+  const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1;
+  const int cb = mg.iimg[ilev - 1][1], ce = mg.iimg[ilev - 1][2];
+  const int f0 = blockIdx.x * T;
+  const long pl = (long)(kv - 1) * mg.PLANE;
+  const double2 z2 = make_double2(0.0, 0.0);
+  const double2 *gg = mg.gg + pl, *gd = mg.gd + pl;
+  double asj[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  double2 x[R];
+
+  const int o = f0 + 1 + 2 * w;
+  const bool ook = w < T / 2;
+  double2 so[R];
+  LineLU<R> co;
+  if (ook) {
+    load_rows<R>(ROW(mg.rsu + pl, bL + o), j0, n2m, so);
+    load_lu<R>(mg, pl, bL + o, j0, n2m, co);
+  }
+  const int e = f0 + 2 * w;
+  if (e < nL) {
+    const int i = bL + e;
+    const double aw = mg.aw[i], ae = mg.ae[i];
+    double2 rr[R];
+    LineLU<R> cl;
+    load_rows<R>(ROW(mg.rsu + pl, i), j0, n2m, rr);
+    load_lu<R>(mg, pl, i, j0, n2m, cl);
+    // line e-1 is the destination of coarse row c-1, line e+1 of row c
+    const int c = cb + e / 2;
+    const double2 *dm = ROW(gd, i - 1), *dp = ROW(gd, i + 1);
+    const double2 *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
+    const bool hm = e > 0, last = c == ce;
+    const double pm1 = hm ? mg.coi1[c - 1] : 0.0, pm2 = hm ? mg.coi2[c - 1] : 0.0;
+    const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jc = j0 + r <= n2m ? j0 + r : n2m;
+      const double2 vm = hm ? cadd(cadd(dm[jc], rmul(pm1, cm[jc])), rmul(pm2, c0[jc])) : z2;
+      double2 vp = cadd(dp[jc], rmul(pp1, c0[jc]));
+      if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
+      rr[r] = csub(rr[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp))));
+    }
+    tri_regs<R>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+    if (w < T / 2) store_rows<R>(ROW(mg.gg + pl, i), j0, n2m, x);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) TILE_AT(w, r) = x[r];
+  __syncthreads();
+
+  if (ook) {
+    const double aw = mg.aw[bL + o], ae = mg.ae[bL + o];
+    double2 rr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(w, r))), rmul(ae, TILE_AT(w + 1, r)));
+    tri_regs<R>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+    store_rows<R>(ROW(mg.gg + pl, bL + o), j0, n2m, x);
+  }
+}
+
+// second half of toplevel and everything after it on a tile of top lines
+// (0-based; rows 1 + line in the planes): the first-half lines f0-2+2w go to
+// LDS shifted by tt (kv = 1), the second-half lines f0-1+2w are solved, then
+// residual, restriction, pc and the max norm per pair (as k_top_rr).
+template <int R>
+__global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int T) {
+  extern __shared__ double2 lds[];
+  __shared__ double wmax[8];
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
+  const int n2m = g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
+  const int f0 = blockIdx.x * T;
+  const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+  const double2 z2 = make_double2(0.0, 0.0);
+  const double2 *gg = mg.gg + pl;
+  double asj[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  const double2 tt = kv == 1 ? ROW(gg, 1)[n2m - 1] : z2;
+  // a first-half line as the second half sees it (shifted, zero outside)
+  auto first_half = [&](int line, double2 (&v)[R]) {
+    if (line >= 0 && line < n1m) {
+      load_rows<R>(ROW(gg, 1 + line), j0, n2m, v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (kv == 1) v[r] = csub(v[r], tt);
+        if (j0 + r > n2m) v[r] = z2;  // the ghost rows past n2m read as zero
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = z2;
+    }
+  };
+  double2 x[R];
+  if (w <= T / 2 + 1) {
+    first_half(f0 - 2 + 2 * w, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w, r) = x[r];
+  }
+  const int o = f0 - 1 + 2 * w;
+  if (w <= T / 2) {
+    if (o >= 0) {
+      const int i = 1 + o;
+      double2 rr[R], gl[R], gr[R];
+      LineLU<R> cl;
+      load_rows<R>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
+      load_lu<R>(mg, pl, i, j0, n2m, cl);
+      first_half(o - 1, gl);
+      first_half(o + 1, gr);
+      const double aw = mg.aw[i], ae = mg.ae[i];
+#pragma unroll
+      for (int r = 0; r < R; ++r) rr[r] = csub(csub(rr[r], rmul(aw, gl[r])), rmul(ae, gr[r]));
+      tri_regs<R>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = z2;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w + 1, r) = x[r];
+  }
+  __syncthreads();
+
+  const int ic0 = mg.iimg[mg.nlev - 1][1] + f0 / 2;
+  double ar = 0.0;
+  const int U = (T / 2) * R;
+  for (int u = w; u < U; u += W) {
+    const int p = u / R, r = u % R;
+    const int j = j0 + r;
+    if (j > n2m) continue;
+    const int qa = 2 * p + 2, ia = 1 + f0 + 2 * p, ib = ia + 1, ic = ic0 + p;
+    const double an = mg.an[j];
+    const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
+    double2 xa = ROW(mg.rhs + ppl, ia)[j];
+    xa = csub(xa, rmul(mg.aw[ia], TILE_AT(qa - 1, r)));
+    xa = csub(xa, rmul(mg.ae[ia], gb));
+    xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
+    xa = csub(xa, rmul(an, TILE_UP(qa, r)));
+    xa = csub(xa, rmul(ROW(mg.ac + pl, ia)[j], ga));
+    double2 xb = ROW(mg.rhs + ppl, ib)[j];
+    xb = csub(xb, rmul(mg.aw[ib], ga));
+    xb = csub(xb, rmul(mg.ae[ib], TILE_AT(qa + 2, r)));
+    xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
+    xb = csub(xb, rmul(an, TILE_UP(qa + 1, r)));
+    xb = csub(xb, rmul(ROW(mg.ac + pl, ib)[j], gb));
+    ROW(mg.resd + pl, ic)[j] = cadd(rmul(mg.cor1[ic], xa), rmul(mg.cor2[ic], xb));
+    ROW(mg.pc + ppl, ia)[j] = ga;
+    ROW(mg.pc + ppl, ib)[j] = gb;
+    ar = fmax(ar, fmax(hypot(xa.x, xa.y), hypot(xb.x, xb.y)));
+  }
+  for (int s = 32; s > 0; s >>= 1) ar = fmax(ar, __shfl_down(ar, s, 64));
+  if (lane == 0) wmax[w] = ar;
+  __syncthreads();
+  if (w == 0 && lane == 0) {
+    double m = 0.0;
+    for (int q = 0; q < W; ++q) m = fmax(m, wmax[q]);
+    atomicMax(&mg.sumres[kv - 1], (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+#undef TILE_AT
+#undef TILE_DN
+#undef TILE_UP
+
+typedef void (*FusedLevelKernel)(Geo, MG, int, int);
+typedef void (*FusedTopKernel)(Geo, MG, int);
+
+static FusedLevelKernel pick_down(int R) {
+  switch (R) {
+    case 1: return k_mg_down<1>;
+    case 2: return k_mg_down<2>;
+    case 4: return k_mg_down<4>;
+    case 8: return k_mg_down<8>;
+    default: return k_mg_down<16>;
+  }
+}
+static FusedLevelKernel pick_up(int R) {
+  switch (R) {
+    case 1: return k_mg_up<1>;
+    case 2: return k_mg_up<2>;
+    case 4: return k_mg_up<4>;
+    case 8: return k_mg_up<8>;
+    default: return k_mg_up<16>;
+  }
+}
+static FusedTopKernel pick_top2(int R) {
+  switch (R) {
+    case 1: return k_mg_top2<1>;
+    case 2: return k_mg_top2<2>;
+    case 4: return k_mg_top2<4>;
+    case 8: return k_mg_top2<8>;
+    default: return k_mg_top2<16>;
+  }
+}
+
+// LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
+// T/2+1 (up), 64*R rows each
+static inline unsigned fused_lds(int R, int T, bool up) {
+  return (unsigned)((up ? T / 2 + 1 : T + 3) * R * 64 * (int)sizeof(double2));
+}
+
+static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
+  // relax: 2 x 48 B (rhs 16, bet 8, gam 8, half the lines' solutions 16);
+  // residual + restriction: 64 B per point
+  LW_LAUNCH("k_mg_down", 0, (48.0 * 2 + 64.0) * g.n2m * n, pick_down(R),
+            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T);
+}
+
+static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
+  // per line: rhs 16, bet 8, gam 8, solution 16; the first half also reads
+  // one down row and one coarse row per line (32)
+  LW_LAUNCH("k_mg_up", 0, (48.0 * 2 + 32.0) * g.n2m * n, pick_up(R), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+            dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
+}
+
+static void launch_top2(hipStream_t s, const Geo &g, const MG &mg) {
+  const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
+  // second half: rhs 16, bet 8, gam 8, neighbour 16 per point of half the
+  // lines; residual: rhs 16, solution 16, ac 8, pc 16 per point, coarse 8
+  LW_LAUNCH("k_mg_top2", 0, (48.0 + 64.0) * g.n2m * n, pick_top2(R), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+            dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, T);
+}
+
 static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
 
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test) {
   const int nl = mg.nlev;
   launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
-  launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
-  // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
-  LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
+  if (mg.fused) {
+    launch_top2(s, g, mg);
+  } else {
+    launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
+    // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
+    LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
+  }
   LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id);
 }
 
@@ -1037,6 +1404,10 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
   // levels above lc: one launch per zebra half, one per residual+restriction
   for (int ilev = mg.nlev - 1; ilev >= 1 && ilev > mg.lc; --ilev) {
     const int n = nrow(ilev);
+    if (mg.fused) {
+      launch_down(s, g, mg, ilev);
+      continue;
+    }
     launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
     launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, 0);
     // residual in/out 32, solution 16, ac 8 per fine point; coarse row 8 per fine point
@@ -1053,10 +1424,34 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
   }
   for (int ilev = mg.lc < 0 ? 0 : mg.lc; ilev <= mg.nlev - 2; ++ilev) {
     const int n = nrow(ilev + 1);
+    if (mg.fused) {
+      launch_up(s, g, mg, ilev + 1);
+      continue;
+    }
     launch_lines(s, g, mg, 4, mg.iimg[ilev + 1][1], (n + 1) / 2, 1.0, ilev);
     launch_lines(s, g, mg, 1, mg.iimg[ilev + 1][1] + 1, n / 2, 1.0, 0);
   }
   launch_mg_toplevel(s, g, mg, 1, 1.0, test);
+}
+
+// once per handle, before the first cycle: dynamic LDS beyond 64 KiB must be
+// allowed explicitly
+// the fused kernels need R <= 8 (n2m <= 512) to keep their tiles in LDS
+hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
+  const int R = lanes_r(g.n2m);
+  if (R > 8) mg.fused = 0;
+  if (mg.fused) {
+    const int big = (int)fused_lds(R, 8, false);
+    hipError_t e = hipFuncSetAttribute((const void *)pick_down(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void *)pick_up(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void *)pick_top2(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    if (e != hipSuccess) return e;
+  }
+  if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
+  return hipFuncSetAttribute((const void *)pick_coarse_lds(R), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kCoarseLdsBytes);
 }
 
 }  // namespace lw

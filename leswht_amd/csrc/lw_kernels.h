@@ -107,7 +107,7 @@ void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, do
 // ---- Poisson (lw_kern_pois.hip) ----
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
-hipError_t prepare_mg_kernels(const Geo &g, const MG &mg);
+hipError_t prepare_mg_kernels(const Geo &g, MG &mg);
 void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
 // toplevel (poiss_z_ft.f90:302-380), id 0 or 1, ending with the per-wavenumber
 // convergence test against `test`

@@ -536,8 +536,15 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.coi1 = h->upload(coi1); mg.coi2 = h->upload(coi2); mg.cor1 = h->upload(cor1); mg.cor2 = h->upload(cor2);
   mg.ac = h->upload(ac); mg.gam = h->upload(gam); mg.bet = h->upload(bet);
   mg.gg = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
-  HIPCHK(lw::prepare_mg_kernels(g, mg));
   mg.resd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+  // LW_MG_FUSED=0 runs every zebra half and residual as its own launch
+  mg.fused = 1;
+  if (const char *e = getenv("LW_MG_FUSED")) mg.fused = atoi(e) != 0;
+  if (mg.fused) {
+    mg.gd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+    mg.rsu = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+  }
+  HIPCHK(lw::prepare_mg_kernels(g, mg));
   mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);

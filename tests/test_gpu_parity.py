@@ -267,3 +267,26 @@ def test_gpu_lds_coarse_levels_equal_global_ones(dims, nlev, monkeypatch):
     assert out[0][:2] == out[1][:2]
     for k in "uvwpt":
         assert np.array_equal(out[0][2][k], out[1][2][k]), k
+
+
+@pytest.mark.parametrize("dims,nlev,coarse", [((96, 256, 8), 3, "24"), ((48, 128, 8), 2, "24"),
+                                               ((64, 32, 16), 2, "24"), ((48, 32, 16), 3, "0"),
+                                               ((24, 32, 8), 3, "0")])
+def test_gpu_fused_levels_equal_separate_launches(dims, nlev, coarse, monkeypatch):
+    """One kernel per level and direction (tiles with recomputed halo lines)
+    gives the fields of one launch per zebra half and residual, which the
+    oracle tests above pin: the two differ at most in the sign of exact
+    zeros. The shapes cover 1, 2 and 4 rows per lane and tiles of 8, 4 and 2
+    lines (coarse rows 0: every level runs the fused kernels)."""
+    view = _view(dims, "les", 2, nlev)
+    print("fused levels", dims, coarse, flush=True)
+    monkeypatch.setenv("LW_MG_COARSE_ROWS", coarse)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("LW_MG_FUSED", flag)
+        g = Solver.from_case(view)
+        st = g.step(2)
+        out.append((st["mg_cycles_max"], st["mg_cycles_sum"], g.fields()))
+    assert out[0][:2] == out[1][:2]
+    for k in "uvwpt":
+        assert np.array_equal(out[0][2][k], out[1][2][k]), k
