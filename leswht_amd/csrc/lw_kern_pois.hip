@@ -28,11 +28,11 @@ __device__ inline double2 rmul(double a, double2 b) { return make_double2(a * b.
 
 // DPP moves of a double (both halves with the same control; lanes whose
 // source is outside their 16-lane row receive `old`)
-template <int CTRL>
+template <int CTRL, int ROWS = 0xF>
 __device__ inline double dpp_d(double v, double old) {
   const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
-  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, ROWS, 0xF, false);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ inline double rdlane(double v, int l) {
@@ -42,6 +42,8 @@ __device__ inline double rdlane(double v, int l) {
 }
 constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;  // row_shr:n
 constexpr int kShl1 = 0x101, kShl2 = 0x102, kShl4 = 0x104, kShl8 = 0x108;  // row_shl:n
+constexpr int kBcast15 = 0x142, kBcast31 = 0x143;  // row_bcast:15, row_bcast:31
+constexpr int kWshl1 = 0x130, kWshr1 = 0x138;      // wave_shl:1, wave_shr:1
 
 // prefix step of the affine maps (M, C) of the forward substitution: this
 // lane's map after the map of the lane D below it (identity across rows)
@@ -51,6 +53,12 @@ constexpr int kShl1 = 0x101, kShl2 = 0x102, kShl4 = 0x104, kShl8 = 0x108;  // ro
     const double2 Cp = make_double2(dpp_d<CT>(Cc.x, 0.0), dpp_d<CT>(Cc.y, 0.0)); \
     Cc = cadd(rmul(M, Cp), Cc);                                                  \
     M = M * Mp;                                                                  \
+  }
+// carry from another row into the rows of mask RM: C = M*C(source) + C
+#define CROSS_STEP(CT, RM)                                                                   \
+  {                                                                                          \
+    const double2 Cp = make_double2(dpp_d<CT, RM>(Cc.x, 0.0), dpp_d<CT, RM>(Cc.y, 0.0));   \
+    Cc = cadd(rmul(M, Cp), Cc);                                                              \
   }
 // suffix step of the back substitution's maps (N, D)
 #define BWD_STEP(CT)                                                           \
@@ -293,7 +301,7 @@ __device__ inline Prolong make_prolong(const MG &mg, const double2 *gg, int ic, 
 // forward and back substitution of one line whose right-hand side and LU
 // factors sit in registers: lane `lane` owns rows j0..j0+R-1 (j0 = 1 + lane*R);
 // rows past n2m carry zero factors and return x = 0
-template <int R>
+template <int R, bool FULL>
 __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], const double (&asj)[R],
                                 const double (&gm)[R], int j0, int n2m, int lane, double2 (&x)[R]) {
   // forward substitution y_j = (rr_j - as_j y_{j-1}) bet_j as affine maps
@@ -303,38 +311,30 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
-    if (j <= n2m) {
+    if ((FULL || j <= n2m)) {
       const double m = j == 1 ? 0.0 : -(asj[r] * bt[r]);
       Cc = cadd(rmul(m, Cc), rmul(bt[r], rr[r]));
       M = m * M;
     }
   }
-  // wavefront scan: within 16-lane rows by DPP, then across the four rows
-  // from their totals (lanes 15, 31, 47); yp = this lane's carry-in
+  // wavefront scan: within 16-lane rows by DPP, then across the rows one
+  // row at a time: row 1 takes the total of row 0 (row_bcast:15 into row 1
+  // only), row 2 that of rows 0-1 (row_bcast:31 into row 2), row 3 that of
+  // rows 0-2 (row_bcast:15 into row 3). Rows outside a step's mask read the
+  // identity 0. yp = the inclusive value of the lane below (wave_shr:1).
   FWD_STEP(kShr1)
   FWD_STEP(kShr2)
   FWD_STEP(kShr4)
   FWD_STEP(kShr8)
-  double2 yp;
-  {
-    const double m0 = rdlane(M, 15), m1 = rdlane(M, 31), m2 = rdlane(M, 47);
-    const double2 c0 = make_double2(rdlane(Cc.x, 15), rdlane(Cc.y, 15));
-    const double2 c1 = make_double2(rdlane(Cc.x, 31), rdlane(Cc.y, 31));
-    const double2 c2 = make_double2(rdlane(Cc.x, 47), rdlane(Cc.y, 47));
-    const double2 p1 = cadd(rmul(m1, c0), c1);  // end of row 1
-    const double2 p2 = cadd(rmul(m2, p1), c2);  // end of row 2
-    (void)m0;
-    const int row = lane >> 4;
-    const double2 prev = row == 0 ? make_double2(0.0, 0.0) : row == 1 ? c0 : row == 2 ? p1 : p2;
-    Cc = cadd(rmul(M, prev), Cc);  // inclusive over the wavefront
-    yp = make_double2(dpp_d<kShr1>(Cc.x, 0.0), dpp_d<kShr1>(Cc.y, 0.0));
-    if ((lane & 15) == 0) yp = prev;
-  }
+  CROSS_STEP(kBcast15, 0x2)
+  CROSS_STEP(kBcast31, 0x4)
+  CROSS_STEP(kBcast15, 0x8)
+  double2 yp = make_double2(dpp_d<kWshr1>(Cc.x, 0.0), dpp_d<kWshr1>(Cc.y, 0.0));
   double2 y[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
-    if (j <= n2m) {
+    if ((FULL || j <= n2m)) {
       yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(asj[r], yp)));
       y[r] = yp;
     } else {
@@ -347,7 +347,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
     const int j = j0 + r;
-    if (j <= n2m) {
+    if ((FULL || j <= n2m)) {
       const double nj = j == n2m ? 0.0 : -gm[r];
       D = cadd(rmul(nj, D), y[r]);
       Nn = nj * Nn;
@@ -368,13 +368,12 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
     const int row = lane >> 4;
     const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
     D = cadd(rmul(Nn, next), D);  // inclusive suffix over the wavefront
-    xn = make_double2(dpp_d<kShl1>(D.x, 0.0), dpp_d<kShl1>(D.y, 0.0));
-    if ((lane & 15) == 15) xn = next;
+    xn = make_double2(dpp_d<kWshl1>(D.x, 0.0), dpp_d<kWshl1>(D.y, 0.0));
   }
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
     const int j = j0 + r;
-    if (j <= n2m) {
+    if ((FULL || j <= n2m)) {
       xn = j == n2m ? y[r] : csub(y[r], rmul(gm[r], xn));
       x[r] = xn;
     } else {
@@ -396,10 +395,10 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
 // The neighbour rows of line t that receive a prolongation are the
 // destinations of coarse rows t-1 (when t > 0) and t of level clev.
 // ---------------------------------------------------------------------------
-template <int R, int MODE>
+template <int R, int MODE, bool FULL>
 __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase, int t, double oldv, int clev,
                                   int lane) {
-  const int n2m = g.n2m;
+  const int n2m = FULL ? 64 * R : g.n2m;
   const int i = ibase + 2 * t;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
   double2 *gg = mg.gg + pl;
@@ -435,7 +434,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
-    const int jc = j <= n2m ? j : n2m;
+    const int jc = (FULL || j <= n2m) ? j : n2m;
     const double2 s0 = src[jc];
     double2 v;
     if (MODE == 0 || MODE == 2) {
@@ -454,41 +453,47 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
     const double gnext = gam[jc < n2m ? jc + 1 : n2m];
     // masks as factors (exact for finite values): a select would let the
     // compiler move the loads into a branch per row again
-    const double okf = j <= n2m ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
+    const double okf = (FULL || j <= n2m) ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
     rr[r] = rmul(okf, v);
     bt[r] = okf * b;
     asj[r] = a;
     gm[r] = okg * gnext;
   }
   double2 x[R];
-  tri_regs<R>(rr, bt, asj, gm, j0, n2m, lane, x);
+  tri_regs<R, FULL>(rr, bt, asj, gm, j0, n2m, lane, x);
   double2 *out = ROW(gg, i);
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (j0 + r <= n2m) out[j0 + r] = x[r];
+    if ((FULL || j0 + r <= n2m)) out[j0 + r] = x[r];
 }
 
 // one wavefront per line, four lines per workgroup, all wavenumbers
-template <int R, int MODE>
+template <int R, int MODE, bool FULL>
 __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
   const int t = blockIdx.x * 4 + threadIdx.y;
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   if (t >= nlines) return;
-  line_solve<R, MODE>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
+  line_solve<R, MODE, FULL>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
 }
 
 typedef void (*LineKernel)(Geo, MG, int, int, double, int);
 
-template <int MODE>
-static LineKernel pick_line(int R) {
+template <int MODE, bool F>
+static LineKernel pick_line_rf(int R) {
   switch (R) {
-    case 1: return k_line<1, MODE>;
-    case 2: return k_line<2, MODE>;
-    case 4: return k_line<4, MODE>;
-    case 8: return k_line<8, MODE>;
-    default: return k_line<16, MODE>;
+    case 1: return k_line<1, MODE, F>;
+    case 2: return k_line<2, MODE, F>;
+    case 4: return k_line<4, MODE, F>;
+    case 8: return k_line<8, MODE, F>;
+    default: return k_line<16, MODE, F>;
   }
+}
+
+// FULL: n2m == 64*R, every lane's rows exist and the row guards vanish
+template <int MODE>
+static LineKernel pick_line(int R, bool full) {
+  return full ? pick_line_rf<MODE, true>(R) : pick_line_rf<MODE, false>(R);
 }
 
 static inline int lanes_r(int n2m) {
@@ -497,18 +502,22 @@ static inline int lanes_r(int n2m) {
   return r;
 }
 
+// every lane's R rows exist (n2m == 64*R): the kernels drop the row guards
+static inline bool full_rows(const Geo &g) { return g.n2m == 64 * lanes_r(g.n2m); }
+
 static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, int ibase, int nlines, double oldv,
                          int clev) {
   if (nlines <= 0) return;
   const int R = lanes_r(g.n2m);
+  const bool full = full_rows(g);
   LineKernel k;
   switch (mode) {
-    case 0: k = pick_line<0>(R); break;
-    case 1: k = pick_line<1>(R); break;
-    case 2: k = pick_line<2>(R); break;
-    case 3: k = pick_line<3>(R); break;
-    case 4: k = pick_line<4>(R); break;
-    default: k = pick_line<5>(R); break;
+    case 0: k = pick_line<0>(R, full); break;
+    case 1: k = pick_line<1>(R, full); break;
+    case 2: k = pick_line<2>(R, full); break;
+    case 3: k = pick_line<3>(R, full); break;
+    case 4: k = pick_line<4>(R, full); break;
+    default: k = pick_line<5>(R, full); break;
   }
   static const char *names[6] = {"k_line_relax1", "k_line_relax2", "k_line_top1",
                                  "k_line_top2",   "k_line_relax1_up", "k_line_top1_up"};
@@ -574,22 +583,22 @@ __global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
 // ---------------------------------------------------------------------------
 constexpr int kCoarseWaves = 8;
 
-template <int R, int MODE>
+template <int R, int MODE, bool FULL>
 __device__ inline void wg_sweep(const Geo &g, const MG &mg, int kv, int ibase, int nlines, double oldv, int clev) {
   for (int t = threadIdx.y; t < nlines; t += kCoarseWaves)
-    line_solve<R, MODE>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
+    line_solve<R, MODE, FULL>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
   __syncthreads();
 }
 
-template <int R>
+template <int R, bool FULL>
 __global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, int lc) {
   const int kv = 1 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   const int tid = threadIdx.y * 64 + threadIdx.x;
   for (int ilev = lc; ilev >= 1; --ilev) {
     const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, b = mg.iimg[ilev][1];
-    wg_sweep<R, 0>(g, mg, kv, b, (n + 1) / 2, 0.0, 0);
-    wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+    wg_sweep<R, 0, FULL>(g, mg, kv, b, (n + 1) / 2, 0.0, 0);
+    wg_sweep<R, 1, FULL>(g, mg, kv, b + 1, n / 2, 1.0, 0);
     for (int q = tid; q < (n / 2) * g.n2m; q += 64 * kCoarseWaves)
       lev_rr(g, mg, kv, ilev, q / g.n2m, 1 + q % g.n2m, g.n2m);
     __syncthreads();
@@ -597,14 +606,14 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, i
   {
     const int n = mg.iimg[0][2] - mg.iimg[0][1] + 1, b = mg.iimg[0][1];
     for (int it = 0; it < mg.nbli; ++it) {
-      wg_sweep<R, 0>(g, mg, kv, b, (n + 1) / 2, it == 0 ? 0.0 : 1.0, 0);
-      wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+      wg_sweep<R, 0, FULL>(g, mg, kv, b, (n + 1) / 2, it == 0 ? 0.0 : 1.0, 0);
+      wg_sweep<R, 1, FULL>(g, mg, kv, b + 1, n / 2, 1.0, 0);
     }
   }
   for (int ilev = 0; ilev < lc; ++ilev) {
     const int n = mg.iimg[ilev + 1][2] - mg.iimg[ilev + 1][1] + 1, b = mg.iimg[ilev + 1][1];
-    wg_sweep<R, 4>(g, mg, kv, b, (n + 1) / 2, 1.0, ilev);
-    wg_sweep<R, 1>(g, mg, kv, b + 1, n / 2, 1.0, 0);
+    wg_sweep<R, 4, FULL>(g, mg, kv, b, (n + 1) / 2, 1.0, ilev);
+    wg_sweep<R, 1, FULL>(g, mg, kv, b + 1, n / 2, 1.0, 0);
   }
 }
 
@@ -632,26 +641,26 @@ struct LineLU {
   double bt[R], gm[R];
 };
 
-template <int R>
+template <int R, bool FULL>
 __device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, LineLU<R> &c) {
   const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
-    const int jc = j <= n2m ? j : n2m;
-    const double okf = j <= n2m ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
+    const int jc = (FULL || j <= n2m) ? j : n2m;
+    const double okf = (FULL || j <= n2m) ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
     c.bt[r] = okf * bet[jc];
     c.gm[r] = okg * gam[jc < n2m ? jc + 1 : n2m];
   }
 }
 
-template <int R>
+template <int R, bool FULL>
 __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = threadIdx.y;
-  const int n2m = g.n2m;
+  const int n2m = FULL ? 64 * R : g.n2m;
   const int b1 = mg.iimg[1][1], n1 = mg.iimg[1][2] - b1 + 1;
   const int b0 = mg.iimg[0][1], n0 = mg.iimg[0][2] - b0 + 1;
   double2 *L1 = lds, *L0 = lds + (long)n1 * R * 64;
@@ -675,21 +684,21 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = j0 + r;
-      const int jc = j <= n2m ? j : n2m;
-      const double okf = j <= n2m ? 1.0 : 0.0;
+      const int jc = (FULL || j <= n2m) ? j : n2m;
+      const double okf = (FULL || j <= n2m) ? 1.0 : 0.0;
       asj[r] = mg.as[jc];
       resA[r] = rmul(okf, ra[jc]);
       resB[r] = rmul(okf, rb[jc]);
     }
-    load_lu<R>(mg, pl, ia, j0, n2m, cA);
-    load_lu<R>(mg, pl, ib, j0, n2m, cB);
+    load_lu<R, FULL>(mg, pl, ia, j0, n2m, cA);
+    load_lu<R, FULL>(mg, pl, ib, j0, n2m, cB);
   }
   double2 rr[R], x[R];
 
   // relax(1), first half, oldv = 0: the right-hand side is resd itself
 #pragma unroll
   for (int r = 0; r < R; ++r) rr[r] = resA[r];
-  tri_regs<R>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
+  tri_regs<R, FULL>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
 #pragma unroll
   for (int r = 0; r < R; ++r) L1[slot(la, r)] = x[r];
   __syncthreads();
@@ -702,9 +711,9 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     for (int r = 0; r < R; ++r) {
       const double2 gmv = L1[slot(la, r)];
       const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
-      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
+      rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
     }
-    tri_regs<R>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
+    tri_regs<R, FULL>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
 #pragma unroll
     for (int r = 0; r < R; ++r) L1[slot(lb, r)] = x[r];
   }
@@ -718,7 +727,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
     double anj[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) anj[r] = mg.an[j0 + r <= n2m ? j0 + r : n2m];
+    for (int r = 0; r < R; ++r) anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
     {
       const double aw = mg.aw[ia], ae = mg.ae[ia];
       const double *ac = ROW(mg.ac + pl, ia);
@@ -730,7 +739,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(la, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int jc = j0 + r <= n2m ? j0 + r : n2m;
+        const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
         const double2 gmv = la > 0 ? L1[slot(la - 1, r)] : z2;
         const double2 gpv = L1[slot(lb, r)];
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
@@ -741,7 +750,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         u = csub(u, rmul(anj[r], up));
         u = csub(u, rmul(ac[jc], gi[r]));
         resA[r] = u;
-        if (j0 + r <= n2m) out[j0 + r] = u;
+        if ((FULL || j0 + r <= n2m)) out[j0 + r] = u;
       }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -757,7 +766,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(lb, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int jc = j0 + r <= n2m ? j0 + r : n2m;
+        const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
         const double2 gmv = L1[slot(la, r)];
         const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
@@ -767,14 +776,14 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         v = csub(v, rmul(asj[r], dn));
         v = csub(v, rmul(anj[r], up));
         v = csub(v, rmul(ac[jc], gi[r]));
-        if (j0 + r <= n2m) out[j0 + r] = v;
-        res0[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, cadd(rmul(c1, resA[r]), rmul(c2, v)));
+        if ((FULL || j0 + r <= n2m)) out[j0 + r] = v;
+        res0[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, cadd(rmul(c1, resA[r]), rmul(c2, v)));
       }
     }
   }
 
   // relax(0, NBLI): first half on even lines, second half on odd lines
-  load_lu<R>(mg, pl, ic, j0, n2m, c0);
+  load_lu<R, FULL>(mg, pl, ic, j0, n2m, c0);
   {
     const double aw = mg.aw[ic], ae = mg.ae[ic];
     const bool hm = w > 0, hp = w + 1 < n0;
@@ -788,10 +797,10 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
           for (int r = 0; r < R; ++r) {
             const double2 gmv = hm ? L0[slot(w - 1, r)] : z2;
             const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
-            rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(res0[r], rmul(1.0, cadd(rmul(aw, gmv), rmul(ae, gpv)))));
+            rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(res0[r], rmul(1.0, cadd(rmul(aw, gmv), rmul(ae, gpv)))));
           }
         }
-        tri_regs<R>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
+        tri_regs<R, FULL>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
 #pragma unroll
         for (int r = 0; r < R; ++r) L0[slot(w, r)] = x[r];
       }
@@ -801,9 +810,9 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         for (int r = 0; r < R; ++r) {
           const double2 gmv = L0[slot(w - 1, r)];
           const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
-          rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(res0[r], rmul(aw, gmv)), rmul(ae, gpv)));
+          rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(res0[r], rmul(aw, gmv)), rmul(ae, gpv)));
         }
-        tri_regs<R>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
+        tri_regs<R, FULL>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
 #pragma unroll
         for (int r = 0; r < R; ++r) L0[slot(w, r)] = x[r];
       }
@@ -813,11 +822,11 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 
   // goup(0) + relax(1), first half: the neighbours 2w-1 and 2w+1 of line 2w
   // are the destinations of level-0 rows w-1 and w, prolonged as they are read
-  load_lu<R>(mg, pl, ia, j0, n2m, cA);
+  load_lu<R, FULL>(mg, pl, ia, j0, n2m, cA);
   {
     const double2 *ra = ROW(mg.resd + pl, ia);
 #pragma unroll
-    for (int r = 0; r < R; ++r) resA[r] = ra[j0 + r <= n2m ? j0 + r : n2m];
+    for (int r = 0; r < R; ++r) resA[r] = ra[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   }
   {
     const double aw = mg.aw[ia], ae = mg.ae[ia];
@@ -830,28 +839,28 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       if (hm) vm = cadd(cadd(L1[slot(la - 1, r)], rmul(pm1, L0[slot(w - 1, r)])), rmul(pm2, L0[slot(w, r)]));
       double2 vp = cadd(L1[slot(lb, r)], rmul(pp1, L0[slot(w, r)]));
       if (!plast) vp = cadd(vp, rmul(pp2, L0[slot(w + 1, r)]));
-      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(resA[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp)))));
+      rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(resA[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp)))));
       __builtin_amdgcn_sched_barrier(0);  // five rows of reads per row: one row at a time
     }
   }
   // the new even rows are written where no wavefront of this stage reads
-  tri_regs<R>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
+  tri_regs<R, FULL>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
   {
     double2 *out = ROW(gg, ia);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       L1[slot(la, r)] = x[r];
-      if (j0 + r <= n2m) out[j0 + r] = x[r];
+      if ((FULL || j0 + r <= n2m)) out[j0 + r] = x[r];
     }
   }
   __syncthreads();
 
   // relax(1), second half
-  load_lu<R>(mg, pl, ib, j0, n2m, cB);
+  load_lu<R, FULL>(mg, pl, ib, j0, n2m, cB);
   {
     const double2 *rb = ROW(mg.resd + pl, ib);
 #pragma unroll
-    for (int r = 0; r < R; ++r) resB[r] = rb[j0 + r <= n2m ? j0 + r : n2m];
+    for (int r = 0; r < R; ++r) resB[r] = rb[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   }
   {
     const double aw = mg.aw[ib], ae = mg.ae[ib];
@@ -860,25 +869,30 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     for (int r = 0; r < R; ++r) {
       const double2 gmv = L1[slot(la, r)];
       const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
-      rr[r] = rmul(j0 + r <= n2m ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
+      rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
     }
-    tri_regs<R>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
+    tri_regs<R, FULL>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
     double2 *out = ROW(gg, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (j0 + r <= n2m) out[j0 + r] = x[r];
+      if ((FULL || j0 + r <= n2m)) out[j0 + r] = x[r];
   }
 }
 
 typedef void (*CoarseLdsKernel)(Geo, MG);
 
-static CoarseLdsKernel pick_coarse_lds(int R) {
+template <bool F>
+static CoarseLdsKernel pick_coarse_lds_rf(int R) {
   switch (R) {
-    case 1: return k_mg_coarse_lds<1>;
-    case 2: return k_mg_coarse_lds<2>;
-    case 4: return k_mg_coarse_lds<4>;
+    case 1: return k_mg_coarse_lds<1, F>;
+    case 2: return k_mg_coarse_lds<2, F>;
+    case 4: return k_mg_coarse_lds<4, F>;
     default: return nullptr;
   }
+}
+
+static CoarseLdsKernel pick_coarse_lds(int R, bool full) {
+  return full ? pick_coarse_lds_rf<true>(R) : pick_coarse_lds_rf<false>(R);
 }
 
 // LDS bytes of k_mg_coarse_lds, or 0 when the levels do not fit its shape
@@ -895,14 +909,19 @@ static long coarse_lds_bytes(const Geo &g, const MG &mg) {
 
 typedef void (*CoarseKernel)(Geo, MG, int);
 
-static CoarseKernel pick_coarse(int R) {
+template <bool F>
+static CoarseKernel pick_coarse_rf(int R) {
   switch (R) {
-    case 1: return k_mg_coarse<1>;
-    case 2: return k_mg_coarse<2>;
-    case 4: return k_mg_coarse<4>;
-    case 8: return k_mg_coarse<8>;
-    default: return k_mg_coarse<16>;
+    case 1: return k_mg_coarse<1, F>;
+    case 2: return k_mg_coarse<2, F>;
+    case 4: return k_mg_coarse<4, F>;
+    case 8: return k_mg_coarse<8, F>;
+    default: return k_mg_coarse<16, F>;
   }
+}
+
+static CoarseKernel pick_coarse(int R, bool full) {
+  return full ? pick_coarse_rf<true>(R) : pick_coarse_rf<false>(R);
 }
 
 static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
@@ -916,11 +935,11 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
     const int n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
-    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m)), dim3((unsigned)mg.n3mh), dim3(64, n0),
+    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.n3mh), dim3(64, n0),
               (unsigned)lds, s, g, mg);
     return;
   }
-  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m)), dim3((unsigned)mg.n3mh), dim3(64, kCoarseWaves),
+  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.n3mh), dim3(64, kCoarseWaves),
             0, s, g, mg, mg.lc);
 }
 
@@ -1029,17 +1048,17 @@ __global__ void k_mg_check(MG mg, double test, int id) {
 // rather than mg.gg, so the up kernel of a tile reads its neighbours' down
 // values while those neighbours write their up values to mg.gg.
 // ---------------------------------------------------------------------------
-template <int R>
+template <int R, bool FULL>
 __device__ inline void load_rows(const double2 *row, int j0, int n2m, double2 (&v)[R]) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = row[j0 + r <= n2m ? j0 + r : n2m];
+  for (int r = 0; r < R; ++r) v[r] = row[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
 }
 
-template <int R>
+template <int R, bool FULL>
 __device__ inline void store_rows(double2 *row, int j0, int n2m, const double2 (&v)[R]) {
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if (j0 + r <= n2m) row[j0 + r] = v[r];
+    if ((FULL || j0 + r <= n2m)) row[j0 + r] = v[r];
 }
 
 // tile size of the fused kernels on a level of n lines
@@ -1056,20 +1075,20 @@ static inline int fused_tile(int n) { return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : 2
 // godown (poiss_z_ft.f90:431-458) on a tile of level `ilev` >= 1: the relax
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
-template <int R>
+template <int R, bool FULL>
 __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
-  const int n2m = g.n2m, j0 = 1 + lane * R;
+  const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
   const int f0 = blockIdx.x * T;
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 z2 = make_double2(0.0, 0.0);
   double asj[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   double2 x[R];
 
   // first half (oldv 0, no neighbour term): line f0-2+2w for w <= T/2+1
@@ -1081,15 +1100,15 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
   double2 so[R];
   LineLU<R> co;
   if (ook) {
-    load_rows<R>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
-    load_lu<R>(mg, pl, bL + o, j0, n2m, co);
+    load_rows<R, FULL>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
+    load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
   if (eok) {
     double2 se[R];
     LineLU<R> ce;
-    load_rows<R>(ROW(mg.resd + pl, bL + e), j0, n2m, se);
-    load_lu<R>(mg, pl, bL + e, j0, n2m, ce);
-    tri_regs<R>(se, ce.bt, asj, ce.gm, j0, n2m, lane, x);
+    load_rows<R, FULL>(ROW(mg.resd + pl, bL + e), j0, n2m, se);
+    load_lu<R, FULL>(mg, pl, bL + e, j0, n2m, ce);
+    tri_regs<R, FULL>(se, ce.bt, asj, ce.gm, j0, n2m, lane, x);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1106,8 +1125,8 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
     double2 rr[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(2 * w, r))), rmul(ae, TILE_AT(2 * w + 2, r)));
-    tri_regs<R>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
-    if (w >= 1) store_rows<R>(ROW(mg.gd + pl, bL + o), j0, n2m, x);
+    tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+    if (w >= 1) store_rows<R, FULL>(ROW(mg.gd + pl, bL + o), j0, n2m, x);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1123,7 +1142,7 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
   for (int u = w; u < U; u += W) {
     const int p = u / R, r = u % R;
     const int j = j0 + r;
-    if (j > n2m) continue;
+    if (!FULL && j > n2m) continue;
     const int qa = 2 * p + 2, ia = bL + f0 + 2 * p, ib = ia + 1, ic = bC + f0 / 2 + p;
     const double an = mg.an[j];
     const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
@@ -1150,13 +1169,13 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
 // goup(ilev-1) + relax(ilev, oldv 1) on a tile: first half on lines f0+2w
 // (w <= T/2, the last one a halo) with its neighbours prolonged as read,
 // then the second half on the tile's odd lines
-template <int R>
+template <int R, bool FULL>
 __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = threadIdx.y;
-  const int n2m = g.n2m, j0 = 1 + lane * R;
+  const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1;
   const int cb = mg.iimg[ilev - 1][1], ce = mg.iimg[ilev - 1][2];
   const int f0 = blockIdx.x * T;
@@ -1165,7 +1184,7 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
   const double2 *gg = mg.gg + pl, *gd = mg.gd + pl;
   double asj[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   double2 x[R];
 
   const int o = f0 + 1 + 2 * w;
@@ -1173,8 +1192,8 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
   double2 so[R];
   LineLU<R> co;
   if (ook) {
-    load_rows<R>(ROW(mg.rsu + pl, bL + o), j0, n2m, so);
-    load_lu<R>(mg, pl, bL + o, j0, n2m, co);
+    load_rows<R, FULL>(ROW(mg.rsu + pl, bL + o), j0, n2m, so);
+    load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
   const int e = f0 + 2 * w;
   if (e < nL) {
@@ -1182,8 +1201,8 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
     const double aw = mg.aw[i], ae = mg.ae[i];
     double2 rr[R];
     LineLU<R> cl;
-    load_rows<R>(ROW(mg.rsu + pl, i), j0, n2m, rr);
-    load_lu<R>(mg, pl, i, j0, n2m, cl);
+    load_rows<R, FULL>(ROW(mg.rsu + pl, i), j0, n2m, rr);
+    load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
     // line e-1 is the destination of coarse row c-1, line e+1 of row c
     const int c = cb + e / 2;
     const double2 *dm = ROW(gd, i - 1), *dp = ROW(gd, i + 1);
@@ -1193,14 +1212,14 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
     const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int jc = j0 + r <= n2m ? j0 + r : n2m;
+      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
       const double2 vm = hm ? cadd(cadd(dm[jc], rmul(pm1, cm[jc])), rmul(pm2, c0[jc])) : z2;
       double2 vp = cadd(dp[jc], rmul(pp1, c0[jc]));
       if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
       rr[r] = csub(rr[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp))));
     }
-    tri_regs<R>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
-    if (w < T / 2) store_rows<R>(ROW(mg.gg + pl, i), j0, n2m, x);
+    tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+    if (w < T / 2) store_rows<R, FULL>(ROW(mg.gg + pl, i), j0, n2m, x);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1214,8 +1233,8 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
     double2 rr[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(w, r))), rmul(ae, TILE_AT(w + 1, r)));
-    tri_regs<R>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
-    store_rows<R>(ROW(mg.gg + pl, bL + o), j0, n2m, x);
+    tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+    store_rows<R, FULL>(ROW(mg.gg + pl, bL + o), j0, n2m, x);
   }
 }
 
@@ -1223,30 +1242,30 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
 // (0-based; rows 1 + line in the planes): the first-half lines f0-2+2w go to
 // LDS shifted by tt (kv = 1), the second-half lines f0-1+2w are solved, then
 // residual, restriction, pc and the max norm per pair (as k_top_rr).
-template <int R>
+template <int R, bool FULL>
 __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int T) {
   extern __shared__ double2 lds[];
   __shared__ double wmax[8];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
-  const int n2m = g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
+  const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
   const int f0 = blockIdx.x * T;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
   const double2 z2 = make_double2(0.0, 0.0);
   const double2 *gg = mg.gg + pl;
   double asj[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) asj[r] = mg.as[j0 + r <= n2m ? j0 + r : n2m];
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   const double2 tt = kv == 1 ? ROW(gg, 1)[n2m - 1] : z2;
   // a first-half line as the second half sees it (shifted, zero outside)
   auto first_half = [&](int line, double2 (&v)[R]) {
     if (line >= 0 && line < n1m) {
-      load_rows<R>(ROW(gg, 1 + line), j0, n2m, v);
+      load_rows<R, FULL>(ROW(gg, 1 + line), j0, n2m, v);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (kv == 1) v[r] = csub(v[r], tt);
-        if (j0 + r > n2m) v[r] = z2;  // the ghost rows past n2m read as zero
+        if (!FULL && j0 + r > n2m) v[r] = z2;  // the ghost rows past n2m read as zero
       }
     } else {
 #pragma unroll
@@ -1265,14 +1284,14 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
       const int i = 1 + o;
       double2 rr[R], gl[R], gr[R];
       LineLU<R> cl;
-      load_rows<R>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
-      load_lu<R>(mg, pl, i, j0, n2m, cl);
+      load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
+      load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
       first_half(o - 1, gl);
       first_half(o + 1, gr);
       const double aw = mg.aw[i], ae = mg.ae[i];
 #pragma unroll
       for (int r = 0; r < R; ++r) rr[r] = csub(csub(rr[r], rmul(aw, gl[r])), rmul(ae, gr[r]));
-      tri_regs<R>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+      tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1288,7 +1307,7 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
   for (int u = w; u < U; u += W) {
     const int p = u / R, r = u % R;
     const int j = j0 + r;
-    if (j > n2m) continue;
+    if (!FULL && j > n2m) continue;
     const int qa = 2 * p + 2, ia = 1 + f0 + 2 * p, ib = ia + 1, ic = ic0 + p;
     const double an = mg.an[j];
     const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
@@ -1326,32 +1345,47 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
 typedef void (*FusedLevelKernel)(Geo, MG, int, int);
 typedef void (*FusedTopKernel)(Geo, MG, int);
 
-static FusedLevelKernel pick_down(int R) {
+template <bool F>
+static FusedLevelKernel pick_down_rf(int R) {
   switch (R) {
-    case 1: return k_mg_down<1>;
-    case 2: return k_mg_down<2>;
-    case 4: return k_mg_down<4>;
-    case 8: return k_mg_down<8>;
-    default: return k_mg_down<16>;
+    case 1: return k_mg_down<1, F>;
+    case 2: return k_mg_down<2, F>;
+    case 4: return k_mg_down<4, F>;
+    case 8: return k_mg_down<8, F>;
+    default: return k_mg_down<16, F>;
   }
 }
-static FusedLevelKernel pick_up(int R) {
+
+static FusedLevelKernel pick_down(int R, bool full) {
+  return full ? pick_down_rf<true>(R) : pick_down_rf<false>(R);
+}
+template <bool F>
+static FusedLevelKernel pick_up_rf(int R) {
   switch (R) {
-    case 1: return k_mg_up<1>;
-    case 2: return k_mg_up<2>;
-    case 4: return k_mg_up<4>;
-    case 8: return k_mg_up<8>;
-    default: return k_mg_up<16>;
+    case 1: return k_mg_up<1, F>;
+    case 2: return k_mg_up<2, F>;
+    case 4: return k_mg_up<4, F>;
+    case 8: return k_mg_up<8, F>;
+    default: return k_mg_up<16, F>;
   }
 }
-static FusedTopKernel pick_top2(int R) {
+
+static FusedLevelKernel pick_up(int R, bool full) {
+  return full ? pick_up_rf<true>(R) : pick_up_rf<false>(R);
+}
+template <bool F>
+static FusedTopKernel pick_top2_rf(int R) {
   switch (R) {
-    case 1: return k_mg_top2<1>;
-    case 2: return k_mg_top2<2>;
-    case 4: return k_mg_top2<4>;
-    case 8: return k_mg_top2<8>;
-    default: return k_mg_top2<16>;
+    case 1: return k_mg_top2<1, F>;
+    case 2: return k_mg_top2<2, F>;
+    case 4: return k_mg_top2<4, F>;
+    case 8: return k_mg_top2<8, F>;
+    default: return k_mg_top2<16, F>;
   }
+}
+
+static FusedTopKernel pick_top2(int R, bool full) {
+  return full ? pick_top2_rf<true>(R) : pick_top2_rf<false>(R);
 }
 
 // LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
@@ -1364,7 +1398,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
   // relax: 2 x 48 B (rhs 16, bet 8, gam 8, half the lines' solutions 16);
   // residual + restriction: 64 B per point
-  LW_LAUNCH("k_mg_down", 0, (48.0 * 2 + 64.0) * g.n2m * n, pick_down(R),
+  LW_LAUNCH("k_mg_down", 0, (48.0 * 2 + 64.0) * g.n2m * n, pick_down(R, full_rows(g)),
             dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T);
 }
 
@@ -1372,7 +1406,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
   // per line: rhs 16, bet 8, gam 8, solution 16; the first half also reads
   // one down row and one coarse row per line (32)
-  LW_LAUNCH("k_mg_up", 0, (48.0 * 2 + 32.0) * g.n2m * n, pick_up(R), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+  LW_LAUNCH("k_mg_up", 0, (48.0 * 2 + 32.0) * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
             dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
@@ -1380,7 +1414,7 @@ static void launch_top2(hipStream_t s, const Geo &g, const MG &mg) {
   const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
   // second half: rhs 16, bet 8, gam 8, neighbour 16 per point of half the
   // lines; residual: rhs 16, solution 16, ac 8, pc 16 per point, coarse 8
-  LW_LAUNCH("k_mg_top2", 0, (48.0 + 64.0) * g.n2m * n, pick_top2(R), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+  LW_LAUNCH("k_mg_top2", 0, (48.0 + 64.0) * g.n2m * n, pick_top2(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
             dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, T);
 }
 
@@ -1442,15 +1476,15 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   if (R > 8) mg.fused = 0;
   if (mg.fused) {
     const int big = (int)fused_lds(R, 8, false);
-    hipError_t e = hipFuncSetAttribute((const void *)pick_down(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipError_t e = hipFuncSetAttribute((const void *)pick_down(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void *)pick_up(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+      e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void *)pick_top2(R), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+      e = hipFuncSetAttribute((const void *)pick_top2(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e != hipSuccess) return e;
   }
   if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
-  return hipFuncSetAttribute((const void *)pick_coarse_lds(R), hipFuncAttributeMaxDynamicSharedMemorySize,
+  return hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize,
                              kCoarseLdsBytes);
 }
 
