@@ -925,12 +925,14 @@ static CoarseKernel pick_coarse(int R, bool full) {
 }
 
 static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
-  // per level and cycle: down and up sweeps 4 x 64 B, residual+restriction
-  // 64 B, per point; level 0: NBLI x 2 x 64 B
+  // unique bytes per wavenumber: the restricted rows of level lc in (16)
+  // and its solution out (16) per point; bet and gam (16) on every level and
+  // ac (8) on the levels that form a residual; the levels below lc never
+  // need to leave the chip
   double per_kv = 0.0;
   for (int l = 0; l <= mg.lc; ++l) {
     const double pts = (double)(mg.iimg[l][2] - mg.iimg[l][1] + 1) * g.n2m;
-    per_kv += l == 0 ? pts * 64.0 * mg.nbli : pts * (64.0 * 2 + 64.0);
+    per_kv += pts * (16.0 + (l >= 1 ? 8.0 : 0.0) + (l == mg.lc ? 32.0 : 0.0));
   }
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
@@ -1080,7 +1082,7 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
-  const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
+  const int lane = threadIdx.x, w = threadIdx.y;
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
   const int f0 = blockIdx.x * T;
@@ -1091,20 +1093,34 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
   for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   double2 x[R];
 
-  // first half (oldv 0, no neighbour term): line f0-2+2w for w <= T/2+1
+  // wavefront w solves line e = f0-2+2w of the first half (w <= T/2+1) and
+  // line o = f0-1+2w of the second (w <= T/2); for 1 <= w <= T/2 these are
+  // the tile's pair w-1, whose residual it then forms from the right-hand
+  // sides still in its registers
   const int e = f0 - 2 + 2 * w;
   const bool eok = w <= T / 2 + 1 && e >= 0 && e < nL;
-  // second half: line f0-1+2w for w <= T/2; its inputs are loaded now
   const int o = f0 - 1 + 2 * w;
   const bool ook = w <= T / 2 && o >= 0;
-  double2 so[R];
+  const bool pair = w >= 1 && w <= T / 2;
+  double2 so[R], se[R];
   LineLU<R> co;
+  double acA[R], acB[R], anj[R];
   if (ook) {
     load_rows<R, FULL>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
+  if (pair) {
+    const double *aa = ROW(mg.ac + pl, bL + e), *ab = ROW(mg.ac + pl, bL + o);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
+      acA[r] = aa[jc];
+      acB[r] = ab[jc];
+      anj[r] = mg.an[jc];
+    }
+  }
+  // first half (oldv 0): the right-hand side is resd itself
   if (eok) {
-    double2 se[R];
     LineLU<R> ce;
     load_rows<R, FULL>(ROW(mg.resd + pl, bL + e), j0, n2m, se);
     load_lu<R, FULL>(mg, pl, bL + e, j0, n2m, ce);
@@ -1137,32 +1153,35 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
   }
   __syncthreads();
 
-  // residual of lines a = f0+2p, b = a+1 in place, coarse row bC + f0/2 + p
-  const int U = (T / 2) * R;
-  for (int u = w; u < U; u += W) {
-    const int p = u / R, r = u % R;
-    const int j = j0 + r;
-    if (!FULL && j > n2m) continue;
-    const int qa = 2 * p + 2, ia = bL + f0 + 2 * p, ib = ia + 1, ic = bC + f0 / 2 + p;
-    const double an = mg.an[j];
-    const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
-    double2 xa = ROW(mg.resd + pl, ia)[j];
-    xa = csub(xa, rmul(mg.aw[ia], TILE_AT(qa - 1, r)));
-    xa = csub(xa, rmul(mg.ae[ia], gb));
-    xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
-    xa = csub(xa, rmul(an, TILE_UP(qa, r)));
-    xa = csub(xa, rmul(ROW(mg.ac + pl, ia)[j], ga));
-    double2 xb = ROW(mg.resd + pl, ib)[j];
-    xb = csub(xb, rmul(mg.aw[ib], ga));
-    xb = csub(xb, rmul(mg.ae[ib], TILE_AT(qa + 2, r)));
-    xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
-    xb = csub(xb, rmul(an, TILE_UP(qa + 1, r)));
-    xb = csub(xb, rmul(ROW(mg.ac + pl, ib)[j], gb));
-    // the residual goes to rsu: neighbouring tiles still read resd rows of
-    // this level as their halo lines' right-hand sides
-    ROW(mg.rsu + pl, ia)[j] = xa;
-    ROW(mg.rsu + pl, ib)[j] = xb;
-    ROW(mg.resd + pl, ic)[j] = cadd(rmul(mg.cor1[ic], xa), rmul(mg.cor2[ic], xb));
+  // residual of the pair a = f0+2p, b = a+1 (p = w-1), restricted into the
+  // coarse row bC + f0/2 + p. The residual goes to rsu: neighbouring tiles
+  // still read resd rows of this level as their halo lines' right-hand sides.
+  if (pair) {
+    const int qa = 2 * w, ia = bL + e, ib = ia + 1, ic = bC + f0 / 2 + w - 1;
+    const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
+    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    double2 *ra = ROW(mg.rsu + pl, ia), *rb = ROW(mg.rsu + pl, ib), *rc = ROW(mg.resd + pl, ic);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      if (!FULL && j > n2m) continue;
+      const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
+      double2 xa = se[r];
+      xa = csub(xa, rmul(awa, TILE_AT(qa - 1, r)));
+      xa = csub(xa, rmul(aea, gb));
+      xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
+      xa = csub(xa, rmul(anj[r], TILE_UP(qa, r)));
+      xa = csub(xa, rmul(acA[r], ga));
+      double2 xb = so[r];
+      xb = csub(xb, rmul(awb, ga));
+      xb = csub(xb, rmul(aeb, TILE_AT(qa + 2, r)));
+      xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
+      xb = csub(xb, rmul(anj[r], TILE_UP(qa + 1, r)));
+      xb = csub(xb, rmul(acB[r], gb));
+      ra[j] = xa;
+      rb[j] = xb;
+      rc[j] = cadd(rmul(c1, xa), rmul(c2, xb));
+    }
   }
 }
 
@@ -1272,25 +1291,44 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
       for (int r = 0; r < R; ++r) v[r] = z2;
     }
   };
+  // wavefront w holds first-half line f0-2+2w (w <= T/2+1) and solves
+  // second-half line f0-1+2w (w <= T/2); for 1 <= w <= T/2 these are the
+  // tile's pair w-1, whose residual it then forms
+  const bool pair = w >= 1 && w <= T / 2;
+  const int o = f0 - 1 + 2 * w;
+  double2 sa[R], sb[R];
+  double acA[R], acB[R], anj[R];
+  if (pair) {
+    const int ia = o, ib = o + 1;  // rows of lines o-1 and o
+    const double2 *ra = ROW(mg.rhs + ppl, ia);
+    const double *aa = ROW(mg.ac + pl, ia), *ab = ROW(mg.ac + pl, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
+      sa[r] = ra[jc];
+      acA[r] = aa[jc];
+      acB[r] = ab[jc];
+      anj[r] = mg.an[jc];
+    }
+  }
   double2 x[R];
   if (w <= T / 2 + 1) {
     first_half(f0 - 2 + 2 * w, x);
 #pragma unroll
     for (int r = 0; r < R; ++r) TILE_AT(2 * w, r) = x[r];
   }
-  const int o = f0 - 1 + 2 * w;
   if (w <= T / 2) {
     if (o >= 0) {
       const int i = 1 + o;
       double2 rr[R], gl[R], gr[R];
       LineLU<R> cl;
-      load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
+      load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, sb);
       load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
       first_half(o - 1, gl);
       first_half(o + 1, gr);
       const double aw = mg.aw[i], ae = mg.ae[i];
 #pragma unroll
-      for (int r = 0; r < R; ++r) rr[r] = csub(csub(rr[r], rmul(aw, gl[r])), rmul(ae, gr[r]));
+      for (int r = 0; r < R; ++r) rr[r] = csub(csub(sb[r], rmul(aw, gl[r])), rmul(ae, gr[r]));
       tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
     } else {
 #pragma unroll
@@ -1301,32 +1339,36 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
   }
   __syncthreads();
 
-  const int ic0 = mg.iimg[mg.nlev - 1][1] + f0 / 2;
+  // residual of the pair (rows ia = f0+2p+1, ib = ia+1 with p = w-1),
+  // restriction, pc and the max norm
   double ar = 0.0;
-  const int U = (T / 2) * R;
-  for (int u = w; u < U; u += W) {
-    const int p = u / R, r = u % R;
-    const int j = j0 + r;
-    if (!FULL && j > n2m) continue;
-    const int qa = 2 * p + 2, ia = 1 + f0 + 2 * p, ib = ia + 1, ic = ic0 + p;
-    const double an = mg.an[j];
-    const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
-    double2 xa = ROW(mg.rhs + ppl, ia)[j];
-    xa = csub(xa, rmul(mg.aw[ia], TILE_AT(qa - 1, r)));
-    xa = csub(xa, rmul(mg.ae[ia], gb));
-    xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
-    xa = csub(xa, rmul(an, TILE_UP(qa, r)));
-    xa = csub(xa, rmul(ROW(mg.ac + pl, ia)[j], ga));
-    double2 xb = ROW(mg.rhs + ppl, ib)[j];
-    xb = csub(xb, rmul(mg.aw[ib], ga));
-    xb = csub(xb, rmul(mg.ae[ib], TILE_AT(qa + 2, r)));
-    xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
-    xb = csub(xb, rmul(an, TILE_UP(qa + 1, r)));
-    xb = csub(xb, rmul(ROW(mg.ac + pl, ib)[j], gb));
-    ROW(mg.resd + pl, ic)[j] = cadd(rmul(mg.cor1[ic], xa), rmul(mg.cor2[ic], xb));
-    ROW(mg.pc + ppl, ia)[j] = ga;
-    ROW(mg.pc + ppl, ib)[j] = gb;
-    ar = fmax(ar, fmax(hypot(xa.x, xa.y), hypot(xb.x, xb.y)));
+  if (pair) {
+    const int qa = 2 * w, ia = o, ib = o + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
+    const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
+    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(mg.pc + ppl, ia), *pb = ROW(mg.pc + ppl, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      if (!FULL && j > n2m) continue;
+      const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
+      double2 xa = sa[r];
+      xa = csub(xa, rmul(awa, TILE_AT(qa - 1, r)));
+      xa = csub(xa, rmul(aea, gb));
+      xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
+      xa = csub(xa, rmul(anj[r], TILE_UP(qa, r)));
+      xa = csub(xa, rmul(acA[r], ga));
+      double2 xb = sb[r];
+      xb = csub(xb, rmul(awb, ga));
+      xb = csub(xb, rmul(aeb, TILE_AT(qa + 2, r)));
+      xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
+      xb = csub(xb, rmul(anj[r], TILE_UP(qa + 1, r)));
+      xb = csub(xb, rmul(acB[r], gb));
+      rc[j] = cadd(rmul(c1, xa), rmul(c2, xb));
+      pa[j] = ga;
+      pb[j] = gb;
+      ar = fmax(ar, fmax(hypot(xa.x, xa.y), hypot(xb.x, xb.y)));
+    }
   }
   for (int s = 32; s > 0; s >>= 1) ar = fmax(ar, __shfl_down(ar, s, 64));
   if (lane == 0) wmax[w] = ar;
@@ -1396,25 +1438,28 @@ static inline unsigned fused_lds(int R, int T, bool up) {
 
 static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
-  // relax: 2 x 48 B (rhs 16, bet 8, gam 8, half the lines' solutions 16);
-  // residual + restriction: 64 B per point
-  LW_LAUNCH("k_mg_down", 0, (48.0 * 2 + 64.0) * g.n2m * n, pick_down(R, full_rows(g)),
+  // unique bytes per point of the level: reads resd 16, bet 8, gam 8, ac 8;
+  // writes the second-half solution 16 on half the lines (8), the residual
+  // 16 and the restricted row 16 per pair (8)
+  LW_LAUNCH("k_mg_down", 0, 72.0 * g.n2m * n, pick_down(R, full_rows(g)),
             dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T);
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
-  // per line: rhs 16, bet 8, gam 8, solution 16; the first half also reads
-  // one down row and one coarse row per line (32)
-  LW_LAUNCH("k_mg_up", 0, (48.0 * 2 + 32.0) * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+  // unique bytes per point of the level: reads residual 16, bet 8, gam 8,
+  // the down-sweep row on half the lines (8) and the coarse row per pair
+  // (8); writes the solution 16
+  LW_LAUNCH("k_mg_up", 0, 64.0 * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
             dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
 static void launch_top2(hipStream_t s, const Geo &g, const MG &mg) {
   const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
-  // second half: rhs 16, bet 8, gam 8, neighbour 16 per point of half the
-  // lines; residual: rhs 16, solution 16, ac 8, pc 16 per point, coarse 8
-  LW_LAUNCH("k_mg_top2", 0, (48.0 + 64.0) * g.n2m * n, pick_top2(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+  // unique bytes per top point: reads rhs 16, ac 8, bet and gam on the
+  // second-half lines (8), the first half's solution on the other half (8);
+  // writes pc 16 and the restricted row 16 per pair (8)
+  LW_LAUNCH("k_mg_top2", 0, 64.0 * g.n2m * n, pick_top2(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
             dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, T);
 }
 
