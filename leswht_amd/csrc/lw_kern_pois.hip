@@ -1257,6 +1257,50 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
   }
 }
 
+// first half of toplevel on a tile of top lines (0-based; rows 1 + line):
+// wavefront w solves line f0+2w with rhs - oldv*(aw*P(i-1) + ae*P(i+1)),
+// P = pc, plus for id = 1 the prolongation of level nlev-1 added as it is
+// read (INTERPOLATE & ADD, poiss_z_ft.f90:319-337; line 2c+1 is the
+// destination of coarse row c, the last coarse row with one term)
+template <int R, bool FULL>
+__global__ __launch_bounds__(64 * (8 / 2)) void k_mg_top1(Geo g, MG mg, int T, double oldv, int id) {
+  const int kv = 1 + blockIdx.y;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
+  const int f0 = blockIdx.x * T, e = f0 + 2 * w, i = 1 + e;
+  const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+  const double2 z2 = make_double2(0.0, 0.0);
+  const double2 *gg = mg.gg + pl, *pc = mg.pc + ppl;
+  const int cb = mg.iimg[mg.nlev - 1][1], ce = mg.iimg[mg.nlev - 1][2];
+  const int c = cb + e / 2;  // line e-1 <- coarse row c-1, line e+1 <- row c
+  const bool hm = e > 0, pro = id == 1, last = c == ce;
+  const double aw = mg.aw[i], ae = mg.ae[i];
+  const double pm1 = hm ? mg.coi1[c - 1] : 0.0, pm2 = hm ? mg.coi2[c - 1] : 0.0;
+  const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
+  const double2 *pm = ROW(pc, i - 1), *pp = ROW(pc, i + 1);
+  const double2 *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
+  double asj[R];
+  double2 rr[R], x[R];
+  LineLU<R> cl;
+  load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
+  load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
+    asj[r] = mg.as[jc];
+    double2 vm = hm ? pm[jc] : z2, vp = pp[jc];
+    if (pro) {
+      if (hm) vm = cadd(cadd(vm, rmul(pm1, cm[jc])), rmul(pm2, c0[jc]));
+      vp = cadd(vp, rmul(pp1, c0[jc]));
+      if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
+    }
+    rr[r] = csub(rr[r], rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
+  }
+  tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+  store_rows<R, FULL>(ROW(mg.gg + pl, i), j0, n2m, x);
+}
+
 // second half of toplevel and everything after it on a tile of top lines
 // (0-based; rows 1 + line in the planes): the first-half lines f0-2+2w go to
 // LDS shifted by tt (kv = 1), the second-half lines f0-1+2w are solved, then
@@ -1430,6 +1474,21 @@ static FusedTopKernel pick_top2(int R, bool full) {
   return full ? pick_top2_rf<true>(R) : pick_top2_rf<false>(R);
 }
 
+typedef void (*FusedTop1Kernel)(Geo, MG, int, double, int);
+template <bool F>
+static FusedTop1Kernel pick_top1_rf(int R) {
+  switch (R) {
+    case 1: return k_mg_top1<1, F>;
+    case 2: return k_mg_top1<2, F>;
+    case 4: return k_mg_top1<4, F>;
+    case 8: return k_mg_top1<8, F>;
+    default: return k_mg_top1<16, F>;
+  }
+}
+static FusedTop1Kernel pick_top1(int R, bool full) {
+  return full ? pick_top1_rf<true>(R) : pick_top1_rf<false>(R);
+}
+
 // LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
 // T/2+1 (up), 64*R rows each
 static inline unsigned fused_lds(int R, int T, bool up) {
@@ -1454,6 +1513,16 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
             dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
+static void launch_top1(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
+  const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
+  // per point of the first-half lines: rhs 16, bet 8, gam 8, solution 16;
+  // the neighbour rows of pc 16 (each shared by two lines) and for id = 1 a
+  // coarse row 16 (shared likewise)
+  const double per_row = id == 1 ? 80.0 : 64.0;
+  LW_LAUNCH(id == 1 ? "k_mg_top1_up" : "k_mg_top1", 0, per_row * g.n2m * (n / 2), pick_top1(R, full_rows(g)),
+            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2), 0, s, g, mg, T, oldv, id);
+}
+
 static void launch_top2(hipStream_t s, const Geo &g, const MG &mg) {
   const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
   // unique bytes per top point: reads rhs 16, ac 8, bet and gam on the
@@ -1467,10 +1536,11 @@ static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3
 
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test) {
   const int nl = mg.nlev;
-  launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
   if (mg.fused) {
+    launch_top1(s, g, mg, id, oldv);
     launch_top2(s, g, mg);
   } else {
+    launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
     launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
     // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
     LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
