@@ -84,9 +84,52 @@ constexpr int kFftT = 16;
 constexpr int kFftKb = 8;
 constexpr int kFftLd = kFftT + 1;  // padded LDS row
 
+// Radix-2 decimation-in-frequency FFT of length M held in registers:
+// natural order in, bit-reversed order out. w(q) = e^{-2 pi i q / N} from the
+// table (uniform index: scalar loads), M divides N, INV conjugates it. With
+// every loop unrolled the twiddle indices are compile-time constants and the
+// multiplications by w(0) = 1 drop out.
+template <int M, int N, bool INV>
+__device__ inline void fft_dif(double2 (&z)[M], const MG &mg) {
+#pragma unroll
+  for (int len = M; len >= 2; len >>= 1) {
+    const int half = len / 2;
+#pragma unroll
+    for (int st = 0; st < M; st += len) {
+#pragma unroll
+      for (int k = 0; k < half; ++k) {
+        const double2 a = z[st + k], b = z[st + k + half];
+        z[st + k] = cadd(a, b);
+        const double2 d = csub(a, b);
+        const int q = k * (M / len) * (N / M);
+        if (q == 0) {
+          z[st + k + half] = d;
+        } else {
+          const double wr = mg.twr[q], wi = INV ? -mg.twi[q] : mg.twi[q];
+          z[st + k + half] = make_double2(d.x * wr - d.y * wi, d.x * wi + d.y * wr);
+        }
+      }
+    }
+  }
+}
+
+__host__ __device__ constexpr int bitrev(int k, int M) {
+  int r = 0;
+  for (int b = 1; b < M; b <<= 1) r = (r << 1) | ((k & b) ? 1 : 0);
+  return r;
+}
+
+// Forward z-DFT of real columns with FFTE's conventions (zfft1d.f:32-85:
+// e^{-2 pi i k n/N}, unnormalised), kv = 0..N/2: the even and odd samples
+// form one complex sequence of length M = N/2 whose FFT is split into the
+// spectra of the two halves and recombined, X_k = E_k + w^k O_k. Tiles of
+// 16 x 16 (i, j) columns per workgroup, a column per thread: the real data
+// are read along i (contiguous in the field) and the spectrum leaves through
+// LDS so that it is written along j (contiguous in the multigrid planes),
+// kFftKb wavenumbers at a time.
 template <int N>
 __global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div) {
-  constexpr int NH = N / 2 + 1;
+  constexpr int M = N / 2, NH = N / 2 + 1;
   __shared__ double2 S[kFftKb * kFftT * kFftLd];
   const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
   const int i0 = 1 + blockIdx.x * kFftT, j0 = 1 + blockIdx.y * kFftT;
@@ -94,29 +137,43 @@ __global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div
   const bool in = i <= g.n1m && j <= g.n2m;
   const int oi = i0 + tj, oj = j0 + ti;  // writer of row oi, column oj
   const bool oin = oi <= g.n1m && oj <= g.n2m;
-  double x[N];
+  double2 z[M];
 #pragma unroll
-  for (int n = 0; n < N; ++n) x[n] = in ? div[P(g, i, j, n + 1)] : 0.0;
-  // the wavenumber loops stay rolled: the twiddle index (kv n) mod N is then
-  // uniform across the workgroup and read with scalar loads
-#pragma unroll 1
-  for (int kb = 0; kb < NH; kb += kFftKb) {
-#pragma unroll 1
-    for (int u = 0; u < kFftKb; ++u) {
-      const int kv = kb + u;
-      if (kv < NH) {
-        double sr = 0.0, si = 0.0;
+  for (int m = 0; m < M; ++m)
+    z[m] = in ? make_double2(div[P(g, i, j, 2 * m + 1)], div[P(g, i, j, 2 * m + 2)]) : make_double2(0.0, 0.0);
+  fft_dif<M, N, false>(z, mg);
+  // X_k for k = 0..M from Z_k and Z_{M-k} (Z_M = Z_0), in place by pairs
+  // (k, M-k): X_k goes where Z_k was (slot bitrev(k)), X_M to its own slot
+  double2 XM;
+  {
+    const double2 z0 = z[0];
+    z[0] = make_double2(z0.x + z0.y, 0.0);
+    XM = make_double2(z0.x - z0.y, 0.0);
+  }
 #pragma unroll
-        for (int n = 0; n < N; ++n) {
-          const int m = (kv * n) % N;
-          sr += x[n] * mg.twr[m];
-          si += x[n] * mg.twi[m];
-        }
-        S[(u * kFftT + tj) * kFftLd + ti] = make_double2(sr, si);
-      }
+  for (int k = 1; 2 * k <= M; ++k) {
+    const double2 zk = z[bitrev(k, M)], zc = z[bitrev(M - k, M)];
+    {
+      const double2 e = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y - zc.y));
+      const double2 o = make_double2(0.5 * (zk.y + zc.y), -0.5 * (zk.x - zc.x));
+      const double wr = mg.twr[k], wi = mg.twi[k];
+      z[bitrev(k, M)] = make_double2(e.x + (o.x * wr - o.y * wi), e.y + (o.x * wi + o.y * wr));
     }
+    if (2 * k < M) {
+      const double2 e = make_double2(0.5 * (zc.x + zk.x), 0.5 * (zc.y - zk.y));
+      const double2 o = make_double2(0.5 * (zc.y + zk.y), -0.5 * (zc.x - zk.x));
+      const double wr = mg.twr[M - k], wi = mg.twi[M - k];
+      z[bitrev(M - k, M)] = make_double2(e.x + (o.x * wr - o.y * wi), e.y + (o.x * wi + o.y * wr));
+    }
+  }
+  auto spec = [&](int k) -> double2 { return k == M ? XM : z[bitrev(k < M ? k : 0, M)]; };
+#pragma unroll
+  for (int kb = 0; kb < NH; kb += kFftKb) {
+#pragma unroll
+    for (int u = 0; u < kFftKb; ++u)
+      if (kb + u < NH) S[(u * kFftT + tj) * kFftLd + ti] = spec(kb + u);
     __syncthreads();
-#pragma unroll 1
+#pragma unroll
     for (int u = 0; u < kFftKb; ++u) {
       const int kv = kb + u;
       if (kv < NH && oin) mg.rhs[(long)kv * mg.PCPLANE + (long)oi * mg.LDJ + oj] = S[(u * kFftT + ti) * kFftLd + tj];
@@ -125,9 +182,13 @@ __global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div
   }
 }
 
+// Inverse z-DFT (conjugate twiddles, 1/N): from X_0..X_M the spectra of the
+// even and odd samples, E_k = (X_k + conj X_{M-k})/2 and
+// O_k = (X_k - conj X_{M-k}) conj(w^k)/2, give Z_k = E_k + i O_k, whose
+// inverse FFT of length M (scaled 1/M) holds x_{2m} + i x_{2m+1}.
 template <int N>
 __global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
-  constexpr int NH = N / 2 + 1;
+  constexpr int M = N / 2, NH = N / 2 + 1;
   __shared__ double2 S[kFftKb * kFftT * kFftLd];
   const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
   const int i0 = 1 + blockIdx.x * kFftT, j0 = 1 + blockIdx.y * kFftT;
@@ -135,7 +196,8 @@ __global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
   const bool in = i <= g.n1m && j <= g.n2m;
   const int oi = i0 + tj, oj = j0 + ti;  // reader of row oi, column oj
   const bool oin = oi <= g.n1m && oj <= g.n2m;
-  double2 X[NH];
+  // X_0..X_{M-1} in z, X_M apart; then Z_k in place by pairs (k, M-k)
+  double2 z[M], XM = make_double2(0.0, 0.0);
 #pragma unroll
   for (int kb = 0; kb < NH; kb += kFftKb) {
 #pragma unroll
@@ -149,23 +211,36 @@ __global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
 #pragma unroll
     for (int u = 0; u < kFftKb; ++u) {
       const int kv = kb + u;
-      if (kv < NH) X[kv] = S[(u * kFftT + tj) * kFftLd + ti];
+      if (kv < M) z[kv] = S[(u * kFftT + tj) * kFftLd + ti];
+      else if (kv == M) XM = S[(u * kFftT + tj) * kFftLd + ti];
     }
     __syncthreads();
   }
   if (!in) return;
-  const double dn = 1.0 / (double)N;
-#pragma unroll 1
-  for (int n = 0; n < N; ++n) {
-    double s = 0.0;
-#pragma unroll
-    for (int kv = 0; kv < N; ++kv) {
-      const int m = (kv * n) % N;
-      const double c = mg.twr[m], sn = -mg.twi[m];  // e^{+i theta}
-      if (kv <= N / 2) s += X[kv].x * c - X[kv].y * sn;
-      else s += X[N - kv].x * c + X[N - kv].y * sn;
+  auto zk_of = [&](const double2 xk, const double2 xc, int k) -> double2 {
+    const double2 e = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y - xc.y));
+    const double2 d = make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y + xc.y));
+    double2 o = d;
+    if (k != 0) {
+      const double wr = mg.twr[k], wi = -mg.twi[k];
+      o = make_double2(d.x * wr - d.y * wi, d.x * wi + d.y * wr);
     }
-    phi[P(g, i, j, n + 1)] = s * dn;
+    return make_double2(e.x - o.y, e.y + o.x);
+  };
+  z[0] = zk_of(z[0], XM, 0);
+#pragma unroll
+  for (int k = 1; 2 * k <= M; ++k) {
+    const double2 xk = z[k], xc = z[M - k];
+    z[k] = zk_of(xk, xc, k);
+    if (2 * k < M) z[M - k] = zk_of(xc, xk, M - k);
+  }
+  fft_dif<M, N, true>(z, mg);
+  const double dm = 1.0 / (double)M;
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const double2 v = z[bitrev(m, M)];
+    phi[P(g, i, j, 2 * m + 1)] = v.x * dm;
+    phi[P(g, i, j, 2 * m + 2)] = v.y * dm;
   }
 }
 
