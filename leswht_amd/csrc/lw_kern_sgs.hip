@@ -246,21 +246,32 @@ __device__ inline double xfilt(const Geo &g, const double *a, int i, int j, int 
   return g.cfx1[3 * i] * a[q - 1] + g.cfx1[3 * i + 1] * a[q] + g.cfx1[3 * i + 2] * a[q + 1];
 }
 
-__global__ void k_filter(Geo g, const double *in, double *out, int fx, int fz) {
+// A thread walks kFiltK planes of its (i, j) column up the periodic
+// z-direction and keeps the x-filtered values of planes k-1, k, k+1 in
+// registers, so an input value is read (kFiltK+2)/kFiltK times per launch
+// instead of three times.
+constexpr int kFiltK = 16;
+
+__global__ __launch_bounds__(256) void k_filter(Geo g, const double *__restrict__ in, double *__restrict__ out,
+                                                int fx, int fz) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int n3m = g.n3m;
+  const int k0 = 1 + blockIdx.z * kFiltK, k1 = k0 + kFiltK - 1 < n3m ? k0 + kFiltK - 1 : n3m;
   if (i > g.n1m || j > g.n2m) return;
-  double r;
-  if (fz) {
-    const int km = k == 1 ? g.n3m : k - 1;
-    const int kp = k == g.n3m ? 1 : k + 1;
-    r = g.cfz1[3 * k] * xfilt(g, in, i, j, km, fx) + g.cfz1[3 * k + 1] * xfilt(g, in, i, j, k, fx) +
-        g.cfz1[3 * k + 2] * xfilt(g, in, i, j, kp, fx);
-  } else {
-    r = xfilt(g, in, i, j, k, fx);
+  if (!fz) {
+#pragma unroll 4
+    for (int k = k0; k <= k1; ++k) out[I3(g, i, j, k)] = xfilt(g, in, i, j, k, fx);
+    return;
   }
-  out[I3(g, i, j, k)] = r;
+  double xm = xfilt(g, in, i, j, k0 == 1 ? n3m : k0 - 1, fx), x0 = xfilt(g, in, i, j, k0, fx);
+#pragma unroll 4
+  for (int k = k0; k <= k1; ++k) {
+    const double xp = xfilt(g, in, i, j, k == n3m ? 1 : k + 1, fx);
+    out[I3(g, i, j, k)] = g.cfz1[3 * k] * xm + g.cfz1[3 * k + 1] * x0 + g.cfz1[3 * k + 2] * xp;
+    xm = x0;
+    x0 = xp;
+  }
 }
 
 static inline double pts(const Geo &g) { return (double)g.n1m * g.n2m * g.n3m; }
@@ -271,8 +282,8 @@ static void filter_many(hipStream_t s, const Geo &g, SgsBuf &b, int lo, int hi, 
   const int fx = (filter == 1 || filter == 3) ? 1 : 0;
   const int fz = (filter == 2 || filter == 3) ? 1 : 0;
   for (int l = lo; l <= hi; ++l) {
-    LW_LAUNCH("k_filter", pts(g) * 16.0, 0, k_filter, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, b.a[l], b.tmp, fx,
-              fz);
+    LW_LAUNCH("k_filter", pts(g) * 16.0, 0, k_filter, grid3(g.n1m, g.n2m, (g.n3m + kFiltK - 1) / kFiltK), BLK3, 0, s,
+              g, b.a[l], b.tmp, fx, fz);
     double *t = b.a[l];
     b.a[l] = b.tmp;
     b.tmp = t;
