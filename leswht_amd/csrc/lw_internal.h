@@ -75,9 +75,11 @@ struct MG {
   double2 *gd, *rsu;       // fused kernels: down-sweep second halves, residuals
   int fused;               // 1: one kernel per level and direction
   double2 *pc, *rhs;       // [kv][i][j], top level
+  double2 *pc2;            // fused path: the other pc buffer (pcsel picks)
   unsigned long long *sumres;  // n3mh max-norm bit patterns, then tt (2 doubles)
   int *done;               // n3mh: 0 iterating, 1 converged, 2 MGITR reached
   int *cyc;                // n3mh: V-cycles completed
+  int *pcsel;              // n3mh: 1 when pc2 holds the current pc
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 

@@ -205,7 +205,8 @@ __global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
       const int kv = kb + u;
       if (kv < NH)
         S[(u * kFftT + ti) * kFftLd + tj] =
-            oin ? mg.pc[(long)kv * mg.PCPLANE + (long)oi * mg.LDJ + oj] : make_double2(0.0, 0.0);
+            oin ? (mg.pcsel[kv] ? mg.pc2 : mg.pc)[(long)kv * mg.PCPLANE + (long)oi * mg.LDJ + oj]
+                : make_double2(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll
@@ -272,7 +273,7 @@ __global__ void k_fft_inv_any(Geo g, MG mg, double *phi, int N) {
       const int m = (int)(((long)kv * n) % N);
       const double c = mg.twr[m], sn = -mg.twi[m];
       const int kk = kv <= N / 2 ? kv : N - kv;
-      const double2 X = mg.pc[(long)kk * mg.PCPLANE + (long)i * mg.LDJ + j];
+      const double2 X = (mg.pcsel[kk] ? mg.pc2 : mg.pc)[(long)kk * mg.PCPLANE + (long)i * mg.LDJ + j];
       if (kv <= N / 2) s += X.x * c - X.y * sn;
       else s += X.x * c + X.y * sn;
     }
@@ -336,6 +337,7 @@ __global__ void k_mg_reset(MG mg, long n) {
   if (q < mg.n3mh) {
     mg.done[q] = 0;
     mg.cyc[q] = 0;
+    mg.pcsel[q] = 0;
     mg.sumres[q] = 0ull;
   }
 }
@@ -1096,11 +1098,12 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
 // counts a completed V-cycle; a wavenumber stops when its max norm is below
 // `test`, or when it has used MGITR cycles. Clears the norms for the next
 // toplevel.
-__global__ void k_mg_check(MG mg, double test, int id) {
+__global__ void k_mg_check(MG mg, double test, int id, int flip) {
   for (int q = threadIdx.x; q < mg.n3mh; q += blockDim.x) {
     if (!mg.done[q]) {
       const int c = mg.cyc[q] + id;
       mg.cyc[q] = c;
+      if (flip) mg.pcsel[q] ^= 1;  // k_mg_top wrote the new pc to the other buffer
       const double r = __longlong_as_double((long long)mg.sumres[q]);
       if (r < test) mg.done[q] = 1;
       else if (c >= mg.mgitr) mg.done[q] = 2;
@@ -1119,8 +1122,8 @@ __global__ void k_mg_check(MG mg, double test, int id) {
 // through LDS in the lane-major slot layout of k_mg_coarse_lds.
 //   k_mg_down(L): relax(L, oldv 0) + godown(L) (poiss_z_ft.f90:272-275)
 //   k_mg_up(L):   goup(L-1) + relax(L, oldv 1) (poiss_z_ft.f90:279-282)
-//   k_mg_top2:    second half of toplevel + residual + restriction + pc
-//                 (poiss_z_ft.f90:328-377)
+//   k_mg_top:     toplevel: both halves, residual, restriction, pc
+//                 (poiss_z_ft.f90:302-380)
 // The down-sweep's second-half lines, the rows goup adds to, go to mg.gd
 // rather than mg.gg, so the up kernel of a tile reads its neighbours' down
 // values while those neighbours write their up values to mg.gg.
@@ -1332,140 +1335,143 @@ __global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int il
   }
 }
 
-// first half of toplevel on a tile of top lines (0-based; rows 1 + line):
-// wavefront w solves line f0+2w with rhs - oldv*(aw*P(i-1) + ae*P(i+1)),
-// P = pc, plus for id = 1 the prolongation of level nlev-1 added as it is
-// read (INTERPOLATE & ADD, poiss_z_ft.f90:319-337; line 2c+1 is the
-// destination of coarse row c, the last coarse row with one term)
+// first half of toplevel for line e (0-based; row 1 + e) reading the pc
+// planes `pcr`: rhs - oldv*(aw*P(e-1) + ae*P(e+1)), P = pc plus, for id = 1,
+// the prolongation of level nlev-1 added as it is read; the line's rhs rows
+// stay in `rh` for the residual
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (8 / 2)) void k_mg_top1(Geo g, MG mg, int T, double oldv, int id) {
-  const int kv = 1 + blockIdx.y;
-  if (mg.done[kv - 1]) return;
-  const int lane = threadIdx.x, w = threadIdx.y;
-  const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
-  const int f0 = blockIdx.x * T, e = f0 + 2 * w, i = 1 + e;
-  const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+__device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, const double2 *pcr, int e,
+                                 double oldv, int id, int j0, int n2m, int lane, const double (&asj)[R],
+                                 double2 (&rh)[R], double2 (&x)[R]) {
+  const int i = 1 + e;
   const double2 z2 = make_double2(0.0, 0.0);
-  const double2 *gg = mg.gg + pl, *pc = mg.pc + ppl;
+  const double2 *gg = mg.gg + pl;
   const int cb = mg.iimg[mg.nlev - 1][1], ce = mg.iimg[mg.nlev - 1][2];
   const int c = cb + e / 2;  // line e-1 <- coarse row c-1, line e+1 <- row c
   const bool hm = e > 0, pro = id == 1, last = c == ce;
   const double aw = mg.aw[i], ae = mg.ae[i];
   const double pm1 = hm ? mg.coi1[c - 1] : 0.0, pm2 = hm ? mg.coi2[c - 1] : 0.0;
   const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
-  const double2 *pm = ROW(pc, i - 1), *pp = ROW(pc, i + 1);
+  const double2 *pm = ROW(pcr, i - 1), *pp = ROW(pcr, i + 1);
   const double2 *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
-  double asj[R];
-  double2 rr[R], x[R];
   LineLU<R> cl;
-  load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rr);
+  load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rh);
   load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
+  double2 rr[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-    asj[r] = mg.as[jc];
     double2 vm = hm ? pm[jc] : z2, vp = pp[jc];
     if (pro) {
       if (hm) vm = cadd(cadd(vm, rmul(pm1, cm[jc])), rmul(pm2, c0[jc]));
       vp = cadd(vp, rmul(pp1, c0[jc]));
       if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
     }
-    rr[r] = csub(rr[r], rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
+    rr[r] = csub(rh[r], rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
   }
   tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
-  store_rows<R, FULL>(ROW(mg.gg + pl, i), j0, n2m, x);
 }
 
-// second half of toplevel and everything after it on a tile of top lines
-// (0-based; rows 1 + line in the planes): the first-half lines f0-2+2w go to
-// LDS shifted by tt (kv = 1), the second-half lines f0-1+2w are solved, then
-// residual, restriction, pc and the max norm per pair (as k_top_rr).
+// toplevel (poiss_z_ft.f90:302-380) in one launch per cycle: on a tile of
+// top lines, wavefront w solves first-half line f0-2+2w (w <= T/2+1) and
+// second-half line f0-1+2w (w <= T/2); for 1 <= w <= T/2 these are pair w-1,
+// whose residual, restriction, pc and max norm it then forms. For kv = 1 the
+// gauge tt = first-half line 0 at row n2m-1 shifts the first-half lines;
+// wavefront T/2+2 recomputes line 0 so every tile has it. The tiles read the
+// old pc and write the new one into the other buffer (mg.pcsel per
+// wavenumber says which holds the current pc; k_mg_check flips it), because
+// a tile's halo lines read pc rows its neighbours write.
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int T) {
+__global__ __launch_bounds__(64 * (8 / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
   extern __shared__ double2 lds[];
   __shared__ double wmax[8];
+  __shared__ double2 tts;
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
   const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
   const int f0 = blockIdx.x * T;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+  const int sel = mg.pcsel[kv - 1];
+  const double2 *pcr = (sel ? mg.pc2 : mg.pc) + ppl;
+  double2 *pcw = (sel ? mg.pc : mg.pc2) + ppl;
   const double2 z2 = make_double2(0.0, 0.0);
-  const double2 *gg = mg.gg + pl;
   double asj[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
-  const double2 tt = kv == 1 ? ROW(gg, 1)[n2m - 1] : z2;
-  // a first-half line as the second half sees it (shifted, zero outside)
-  auto first_half = [&](int line, double2 (&v)[R]) {
-    if (line >= 0 && line < n1m) {
-      load_rows<R, FULL>(ROW(gg, 1 + line), j0, n2m, v);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (kv == 1) v[r] = csub(v[r], tt);
-        if (!FULL && j0 + r > n2m) v[r] = z2;  // the ghost rows past n2m read as zero
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = z2;
-    }
-  };
-  // wavefront w holds first-half line f0-2+2w (w <= T/2+1) and solves
-  // second-half line f0-1+2w (w <= T/2); for 1 <= w <= T/2 these are the
-  // tile's pair w-1, whose residual it then forms
+
   const bool pair = w >= 1 && w <= T / 2;
-  const int o = f0 - 1 + 2 * w;
-  double2 sa[R], sb[R];
+  const int e = f0 - 2 + 2 * w, o = e + 1;
+  const bool eok = w <= T / 2 + 1 && e >= 0 && e < n1m;
+  const bool ook = w <= T / 2 && o >= 0;
+  double2 sa[R], sb[R], x[R];
+  LineLU<R> co;
   double acA[R], acB[R], anj[R];
+  if (ook) {
+    load_rows<R, FULL>(ROW(mg.rhs + ppl, 1 + o), j0, n2m, sb);
+    load_lu<R, FULL>(mg, pl, 1 + o, j0, n2m, co);
+  }
   if (pair) {
-    const int ia = o, ib = o + 1;  // rows of lines o-1 and o
-    const double2 *ra = ROW(mg.rhs + ppl, ia);
-    const double *aa = ROW(mg.ac + pl, ia), *ab = ROW(mg.ac + pl, ib);
+    const double *aa = ROW(mg.ac + pl, 1 + e), *ab = ROW(mg.ac + pl, 1 + o);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      sa[r] = ra[jc];
       acA[r] = aa[jc];
       acB[r] = ab[jc];
       anj[r] = mg.an[jc];
     }
   }
-  double2 x[R];
-  if (w <= T / 2 + 1) {
-    first_half(f0 - 2 + 2 * w, x);
+  if (eok) {
+    top_first<R, FULL>(g, mg, pl, ppl, pcr, e, oldv, id, j0, n2m, lane, asj, sa, x);
+  } else if (w == T / 2 + 2 && kv == 1) {
+    double2 rh[R];
+    top_first<R, FULL>(g, mg, pl, ppl, pcr, 0, oldv, id, j0, n2m, lane, asj, rh, x);
 #pragma unroll
-    for (int r = 0; r < R; ++r) TILE_AT(2 * w, r) = x[r];
+    for (int r = 0; r < R; ++r)
+      if (j0 + r == n2m - 1) tts = x[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+  __syncthreads();
+  // the first-half lines as the second half and the residual see them:
+  // shifted by tt on rows 1..n2m, the ghost rows past n2m zero
+  const double2 tt = kv == 1 ? tts : z2;
+  if (w <= T / 2 + 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 v = eok && kv == 1 ? csub(x[r], tt) : x[r];
+      if (!FULL && j0 + r > n2m) v = z2;
+      TILE_AT(2 * w, r) = v;
+    }
+  }
+  __syncthreads();
+
+  // second half: rhs - aw*(g(i-1)-tt) - ae*(g(i+1)-tt)
+  if (ook) {
+    const int i = 1 + o;
+    const double aw = mg.aw[i], ae = mg.ae[i];
+    double2 rr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rr[r] = csub(csub(sb[r], rmul(aw, TILE_AT(2 * w, r))), rmul(ae, TILE_AT(2 * w + 2, r)));
+    tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
   }
   if (w <= T / 2) {
-    if (o >= 0) {
-      const int i = 1 + o;
-      double2 rr[R], gl[R], gr[R];
-      LineLU<R> cl;
-      load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, sb);
-      load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
-      first_half(o - 1, gl);
-      first_half(o + 1, gr);
-      const double aw = mg.aw[i], ae = mg.ae[i];
-#pragma unroll
-      for (int r = 0; r < R; ++r) rr[r] = csub(csub(sb[r], rmul(aw, gl[r])), rmul(ae, gr[r]));
-      tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = z2;
-    }
 #pragma unroll
     for (int r = 0; r < R; ++r) TILE_AT(2 * w + 1, r) = x[r];
   }
   __syncthreads();
 
-  // residual of the pair (rows ia = f0+2p+1, ib = ia+1 with p = w-1),
-  // restriction, pc and the max norm
+  // residual of the pair (rows ia = 1+e, ib = ia+1), restriction, pc, norm
   double ar = 0.0;
   if (pair) {
-    const int qa = 2 * w, ia = o, ib = o + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
+    const int qa = 2 * w, ia = 1 + e, ib = ia + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
     const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
     const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
-    double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(mg.pc + ppl, ia), *pb = ROW(mg.pc + ppl, ib);
+    double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = j0 + r;
@@ -1504,7 +1510,6 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_top2(Geo g, MG mg, int 
 #undef TILE_UP
 
 typedef void (*FusedLevelKernel)(Geo, MG, int, int);
-typedef void (*FusedTopKernel)(Geo, MG, int);
 
 template <bool F>
 static FusedLevelKernel pick_down_rf(int R) {
@@ -1534,34 +1539,20 @@ static FusedLevelKernel pick_up_rf(int R) {
 static FusedLevelKernel pick_up(int R, bool full) {
   return full ? pick_up_rf<true>(R) : pick_up_rf<false>(R);
 }
+
+typedef void (*FusedTopKernel)(Geo, MG, int, double, int);
 template <bool F>
-static FusedTopKernel pick_top2_rf(int R) {
+static FusedTopKernel pick_top_rf(int R) {
   switch (R) {
-    case 1: return k_mg_top2<1, F>;
-    case 2: return k_mg_top2<2, F>;
-    case 4: return k_mg_top2<4, F>;
-    case 8: return k_mg_top2<8, F>;
-    default: return k_mg_top2<16, F>;
+    case 1: return k_mg_top<1, F>;
+    case 2: return k_mg_top<2, F>;
+    case 4: return k_mg_top<4, F>;
+    case 8: return k_mg_top<8, F>;
+    default: return k_mg_top<16, F>;
   }
 }
-
-static FusedTopKernel pick_top2(int R, bool full) {
-  return full ? pick_top2_rf<true>(R) : pick_top2_rf<false>(R);
-}
-
-typedef void (*FusedTop1Kernel)(Geo, MG, int, double, int);
-template <bool F>
-static FusedTop1Kernel pick_top1_rf(int R) {
-  switch (R) {
-    case 1: return k_mg_top1<1, F>;
-    case 2: return k_mg_top1<2, F>;
-    case 4: return k_mg_top1<4, F>;
-    case 8: return k_mg_top1<8, F>;
-    default: return k_mg_top1<16, F>;
-  }
-}
-static FusedTop1Kernel pick_top1(int R, bool full) {
-  return full ? pick_top1_rf<true>(R) : pick_top1_rf<false>(R);
+static FusedTopKernel pick_top(int R, bool full) {
+  return full ? pick_top_rf<true>(R) : pick_top_rf<false>(R);
 }
 
 // LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
@@ -1588,23 +1579,15 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
             dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
-static void launch_top1(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
+static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
   const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
-  // per point of the first-half lines: rhs 16, bet 8, gam 8, solution 16;
-  // the neighbour rows of pc 16 (each shared by two lines) and for id = 1 a
-  // coarse row 16 (shared likewise)
-  const double per_row = id == 1 ? 80.0 : 64.0;
-  LW_LAUNCH(id == 1 ? "k_mg_top1_up" : "k_mg_top1", 0, per_row * g.n2m * (n / 2), pick_top1(R, full_rows(g)),
-            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2), 0, s, g, mg, T, oldv, id);
-}
-
-static void launch_top2(hipStream_t s, const Geo &g, const MG &mg) {
-  const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
-  // unique bytes per top point: reads rhs 16, ac 8, bet and gam on the
-  // second-half lines (8), the first half's solution on the other half (8);
-  // writes pc 16 and the restricted row 16 per pair (8)
-  LW_LAUNCH("k_mg_top2", 0, 64.0 * g.n2m * n, pick_top2(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
-            dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, T);
+  // unique bytes per top point: reads rhs 16, bet 8, gam 8, ac 8, the old pc
+  // on half the lines (8) and for id = 1 the coarse row per pair (8); writes
+  // the new pc 16 and the restricted row per pair (8)
+  const double per_pt = id == 1 ? 80.0 : 72.0;
+  LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g)),
+            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T,
+            oldv, id);
 }
 
 static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
@@ -1612,15 +1595,14 @@ static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test) {
   const int nl = mg.nlev;
   if (mg.fused) {
-    launch_top1(s, g, mg, id, oldv);
-    launch_top2(s, g, mg);
+    launch_top(s, g, mg, id, oldv);
   } else {
     launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
     launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
     // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
     LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
   }
-  LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id);
+  LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id, mg.fused);
 }
 
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
@@ -1670,7 +1652,7 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void *)pick_top2(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+      e = hipFuncSetAttribute((const void *)pick_top(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e != hipSuccess) return e;
   }
   if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;

@@ -540,16 +540,20 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   // LW_MG_FUSED=0 runs every zebra half and residual as its own launch
   mg.fused = 1;
   if (const char *e = getenv("LW_MG_FUSED")) mg.fused = atoi(e) != 0;
+  HIPCHK(lw::prepare_mg_kernels(g, mg));  // clears fused where the tiles do not fit LDS
+  mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
+  mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   if (mg.fused) {
     mg.gd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
     mg.rsu = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
+    mg.pc2 = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
+  } else {
+    mg.pc2 = mg.pc;
   }
-  HIPCHK(lw::prepare_mg_kernels(g, mg));
-  mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
-  mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);
-  mg.done = h->dalloc<int>(256);
+  mg.done = h->dalloc<int>(384);
   mg.cyc = mg.done + 128;
+  mg.pcsel = mg.done + 256;
   std::vector<double> twr(n3m), twi(n3m);
   const double pi2 = 8.0 * atan(1.0);
   for (int q = 0; q < n3m; ++q) {
