@@ -524,26 +524,44 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
     cprev = c;
     r[q] = up;
   }
-  for (int ib = lo_ + 1; ib <= n; ib += kAdiCh) {
-    double v[kAdiCh + 1], rv[kAdiCh], am[kAdiCh], cm[kAdiCh];
+  // software pipeline: the next chunk's loads are in flight while this
+  // chunk's recurrence runs; indices past the line are clamped to its last
+  // point (read, never used)
+  double v[kAdiCh + 1], rv[kAdiCh], am[kAdiCh], cm[kAdiCh];
+  auto load = [&](int ib) {
 #pragma unroll
-    for (int u = 0; u <= kAdiCh; ++u)
-      v[u] = ib + u <= n + 1 ? l.vis[q0 + (long)(ib + u + l.lo) * st] : 0.0;
+    for (int u = 0; u <= kAdiCh; ++u) {
+      const int iv = ib + u <= n + 1 ? ib + u : n + 1;
+      v[u] = l.vis[q0 + (long)(iv + l.lo) * st];
+    }
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
-      const bool ok = ib + u <= n;
-      rv[u] = ok ? r[q0 + (long)(ib + u) * st] : 0.0;
-      am[u] = ok ? l.am[ib + u] : 0.0;
-      cm[u] = ok ? l.cm[ib + u] : 0.0;
+      const int iu = ib + u <= n ? ib + u : n;
+      rv[u] = r[q0 + (long)iu * st];
+      am[u] = l.am[iu];
+      cm[u] = l.cm[iu];
     }
+  };
+  load(lo_ + 1);
+  for (int ib = lo_ + 1; ib <= n; ib += kAdiCh) {
+    double cv[kAdiCh + 1], crv[kAdiCh], cam[kAdiCh], ccm[kAdiCh];
+#pragma unroll
+    for (int u = 0; u <= kAdiCh; ++u) cv[u] = v[u];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      crv[u] = rv[u];
+      cam[u] = am[u];
+      ccm[u] = cm[u];
+    }
+    if (ib + kAdiCh <= n) load(ib + kAdiCh);
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
       const int i = ib + u;
       if (i > n) break;
-      const double a = am[u] * (1. + l.cc * v[u]);
-      const double c = cm[u] * (1. + l.cc * v[u + 1]);
+      const double a = cam[u] * (1. + l.cc * cv[u]);
+      const double c = ccm[u] * (1. + l.cc * cv[u + 1]);
       const double b = l.bd - a - c;
-      const double gv = l.bd * rv[u];
+      const double gv = l.bd * crv[u];
       const double gam = cprev * bet;
       bet = 1. / (b - a * gam);
       up = (gv - a * up) * bet;
@@ -672,18 +690,29 @@ __global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre,
   double *r = f.rhs[C];
   double *gm = f.s1;
   double unext = line_forward(l, r, gm, q0, st, jlo, g.n2m);
-  for (int jb = g.n2m - 1; jb >= jlo; jb -= kAdiCh) {
-    double rv[kAdiCh], gv[kAdiCh];
+  // back substitution, pipelined like the forward pass (clamped indices)
+  double rv[kAdiCh], gv[kAdiCh];
+  auto load = [&](int jb) {
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
-      const long q = q0 + (long)(jb - u) * st;
-      rv[u] = jb - u >= jlo ? r[q] : 0.0;
-      gv[u] = jb - u >= jlo ? gm[q + st] : 0.0;
+      const long q = q0 + (long)(jb - u >= jlo ? jb - u : jlo) * st;
+      rv[u] = r[q];
+      gv[u] = gm[q + st];
     }
+  };
+  load(g.n2m - 1);
+  for (int jb = g.n2m - 1; jb >= jlo; jb -= kAdiCh) {
+    double crv[kAdiCh], cgv[kAdiCh];
+#pragma unroll
+    for (int u = 0; u < kAdiCh; ++u) {
+      crv[u] = rv[u];
+      cgv[u] = gv[u];
+    }
+    if (jb - kAdiCh >= jlo) load(jb - kAdiCh);
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
       if (jb - u < jlo) break;
-      const double uu = rv[u] - gv[u] * unext;
+      const double uu = crv[u] - cgv[u] * unext;
       r[q0 + (long)(jb - u) * st] = uu;
       unext = uu;
     }
