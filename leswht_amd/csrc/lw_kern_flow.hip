@@ -525,24 +525,25 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
     r[q] = up;
   }
   // software pipeline: the next chunk's loads are in flight while this
-  // chunk's recurrence runs; indices past the line are clamped to its last
-  // point (read, never used)
+  // chunk's recurrence runs. The loads run past the line's end (values
+  // unused): into the next plane of the padded arrays, whose planes k + 1 exist
+  // for every line (k <= n3m), and into the padded metric arrays. Addresses
+  // advance by pointer steps, so no per-point index arithmetic is needed.
+  const double *pv = l.vis + q0 + (long)(lo_ + 1 + l.lo) * st;
+  const double *pam = l.am + lo_ + 1, *pcm = l.cm + lo_ + 1;
+  double *pr = r + q0 + (long)(lo_ + 1) * st, *pg = gm + q0 + (long)(lo_ + 1) * st;
   double v[kAdiCh + 1], rv[kAdiCh], am[kAdiCh], cm[kAdiCh];
-  auto load = [&](int ib) {
+  auto load = [&](const double *lv, const double *lr, const double *la, const double *lc) {
 #pragma unroll
-    for (int u = 0; u <= kAdiCh; ++u) {
-      const int iv = ib + u <= n + 1 ? ib + u : n + 1;
-      v[u] = l.vis[q0 + (long)(iv + l.lo) * st];
-    }
+    for (int u = 0; u <= kAdiCh; ++u) v[u] = lv[u * st];
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
-      const int iu = ib + u <= n ? ib + u : n;
-      rv[u] = r[q0 + (long)iu * st];
-      am[u] = l.am[iu];
-      cm[u] = l.cm[iu];
+      rv[u] = lr[u * st];
+      am[u] = la[u];
+      cm[u] = lc[u];
     }
   };
-  load(lo_ + 1);
+  load(pv, pr, pam, pcm);
   for (int ib = lo_ + 1; ib <= n; ib += kAdiCh) {
     double cv[kAdiCh + 1], crv[kAdiCh], cam[kAdiCh], ccm[kAdiCh];
 #pragma unroll
@@ -553,11 +554,11 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
       cam[u] = am[u];
       ccm[u] = cm[u];
     }
-    if (ib + kAdiCh <= n) load(ib + kAdiCh);
+    const long sk = kAdiCh * st;
+    if (ib + kAdiCh <= n) load(pv + sk, pr + sk, pam + kAdiCh, pcm + kAdiCh);
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
-      const int i = ib + u;
-      if (i > n) break;
+      if (ib + u > n) break;
       const double a = cam[u] * (1. + l.cc * cv[u]);
       const double c = ccm[u] * (1. + l.cc * cv[u + 1]);
       const double b = l.bd - a - c;
@@ -565,11 +566,15 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
       const double gam = cprev * bet;
       bet = 1. / (b - a * gam);
       up = (gv - a * up) * bet;
-      const long q = q0 + (long)i * st;
-      gm[q] = gam;
-      r[q] = up;
+      pg[u * st] = gam;
+      pr[u * st] = up;
       cprev = c;
     }
+    pv += sk;
+    pr += sk;
+    pg += sk;
+    pam += kAdiCh;
+    pcm += kAdiCh;
   }
   return up;
 }
@@ -690,17 +695,19 @@ __global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre,
   double *r = f.rhs[C];
   double *gm = f.s1;
   double unext = line_forward(l, r, gm, q0, st, jlo, g.n2m);
-  // back substitution, pipelined like the forward pass (clamped indices)
+  // back substitution, pipelined like the forward pass; the loads run below
+  // the line's start into plane k-1 (k >= 1), values unused
+  double *pr = r + q0 + (long)(g.n2m - 1) * st;
+  const double *pg = gm + q0 + (long)g.n2m * st;
   double rv[kAdiCh], gv[kAdiCh];
-  auto load = [&](int jb) {
+  auto load = [&](const double *lr, const double *lg) {
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
-      const long q = q0 + (long)(jb - u >= jlo ? jb - u : jlo) * st;
-      rv[u] = r[q];
-      gv[u] = gm[q + st];
+      rv[u] = lr[-u * st];
+      gv[u] = lg[-u * st];
     }
   };
-  load(g.n2m - 1);
+  load(pr, pg);
   for (int jb = g.n2m - 1; jb >= jlo; jb -= kAdiCh) {
     double crv[kAdiCh], cgv[kAdiCh];
 #pragma unroll
@@ -708,14 +715,17 @@ __global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre,
       crv[u] = rv[u];
       cgv[u] = gv[u];
     }
-    if (jb - kAdiCh >= jlo) load(jb - kAdiCh);
+    const long sk = kAdiCh * st;
+    if (jb - kAdiCh >= jlo) load(pr - sk, pg - sk);
 #pragma unroll
     for (int u = 0; u < kAdiCh; ++u) {
       if (jb - u < jlo) break;
       const double uu = crv[u] - cgv[u] * unext;
-      r[q0 + (long)(jb - u) * st] = uu;
+      pr[-u * st] = uu;
       unext = uu;
     }
+    pr -= sk;
+    pg -= sk;
   }
 }
 

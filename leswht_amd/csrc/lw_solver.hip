@@ -246,8 +246,12 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   g.ib = st.xprdic == 1 ? 1 : 2;
   g.jb = st.yprdic == 1 ? 1 : 2;
   g.kb = st.zprdic == 1 ? 1 : 2;
-  std::vector<double> aiu(n1 + 2), ciu(n1 + 2), aivw(n1 + 2), civw(n1 + 2), ajv(n2 + 2), cjv(n2 + 2),
-      ajuw(n2 + 2), cjuw(n2 + 2), akw(n3 + 2), ckw(n3 + 2), akuv(n3 + 2), ckuv(n3 + 2);
+  // 32 zero entries past the end: the chunked line sweeps read up to two
+  // chunks beyond a line's last point (values never used)
+  const int pad = 32;
+  std::vector<double> aiu(n1 + 2 + pad), ciu(n1 + 2 + pad), aivw(n1 + 2 + pad), civw(n1 + 2 + pad),
+      ajv(n2 + 2 + pad), cjv(n2 + 2 + pad), ajuw(n2 + 2 + pad), cjuw(n2 + 2 + pad), akw(n3 + 2 + pad),
+      ckw(n3 + 2 + pad), akuv(n3 + 2 + pad), ckuv(n3 + 2 + pad);
   for (int ic = g.ib; ic <= n1m; ++ic) {
     const int im = imv[ic];
     aiu[ic] = -c2cxi[ic] * f2fxi[im];
