@@ -1142,7 +1142,15 @@ __device__ inline void store_rows(double2 *row, int j0, int n2m, const double2 (
 }
 
 // tile size of the fused kernels on a level of n lines
-static inline int fused_tile(int n) { return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : 2; }
+constexpr int kTileMax = 8;
+static inline int fused_tile(int n, int tmax) {
+  for (int t = tmax; t > 2; t >>= 1)
+    if (n % t == 0) return t;
+  return 2;
+}
+// tile sizes (lines per workgroup) of the down, up and top kernels;
+// LW_MG_TILES="d,u,t" overrides them (each 2, 4 or 8)
+static int g_tile_down = 4, g_tile_up = 4, g_tile_top = 8;
 
 // the residual at rows j0+r of the two lines whose LDS slots are qa, qa+1
 // (neighbours qa-1, qa+2), in the reference's order (poiss_z_ft.f90:440-443)
@@ -1156,7 +1164,7 @@ static inline int fused_tile(int n) { return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : 2
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T) {
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
@@ -1267,7 +1275,7 @@ __global__ __launch_bounds__(64 * (8 / 2 + 2)) void k_mg_down(Geo g, MG mg, int 
 // (w <= T/2, the last one a halo) with its neighbours prolonged as read,
 // then the second half on the tile's odd lines
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (8 / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
@@ -1382,9 +1390,9 @@ __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, 
 // wavenumber says which holds the current pc; k_mg_check flips it), because
 // a tile's halo lines read pc rows its neighbours write.
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (8 / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
   extern __shared__ double2 lds[];
-  __shared__ double wmax[8];
+  __shared__ double wmax[kTileMax / 2 + 3];
   __shared__ double2 tts;
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
@@ -1562,7 +1570,7 @@ static inline unsigned fused_lds(int R, int T, bool up) {
 }
 
 static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
-  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_down), R = lanes_r(g.n2m);
   // unique bytes per point of the level: reads resd 16, bet 8, gam 8, ac 8;
   // writes the second-half solution 16 on half the lines (8), the residual
   // 16 and the restricted row 16 per pair (8)
@@ -1571,7 +1579,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
-  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n), R = lanes_r(g.n2m);
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_up), R = lanes_r(g.n2m);
   // unique bytes per point of the level: reads residual 16, bet 8, gam 8,
   // the down-sweep row on half the lines (8) and the coarse row per pair
   // (8); writes the solution 16
@@ -1580,7 +1588,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
 }
 
 static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
-  const int n = g.n1m, T = fused_tile(n), R = lanes_r(g.n2m);
+  const int n = g.n1m, T = fused_tile(n, g_tile_top), R = lanes_r(g.n2m);
   // unique bytes per top point: reads rhs 16, bet 8, gam 8, ac 8, the old pc
   // on half the lines (8) and for id = 1 the coarse row per pair (8); writes
   // the new pc 16 and the restricted row per pair (8)
@@ -1645,9 +1653,18 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
 // the fused kernels need R <= 8 (n2m <= 512) to keep their tiles in LDS
 hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   const int R = lanes_r(g.n2m);
+  if (const char *e = getenv("LW_MG_TILES")) {
+    int d = 0, u = 0, t = 0;
+    if (sscanf(e, "%d,%d,%d", &d, &u, &t) == 3) {
+      auto ok = [](int v) { return v == 2 || v == 4 || v == 8; };
+      if (ok(d)) g_tile_down = d;
+      if (ok(u)) g_tile_up = u;
+      if (ok(t)) g_tile_top = t;
+    }
+  }
   if (R > 8) mg.fused = 0;
   if (mg.fused) {
-    const int big = (int)fused_lds(R, 8, false);
+    const int big = (int)fused_lds(R, kTileMax, false);
     hipError_t e = hipFuncSetAttribute((const void *)pick_down(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
