@@ -581,7 +581,8 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
 
 // periodic z line (trdiag3p, slv_mmtm.f90:1572-1641); result into rhs
 template <int C>
-__global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo) {
+__device__ inline void adi_z(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1;
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int j = jlo + blockIdx.y * 4 + threadIdx.y;
   if (i > g.n1m || j > g.n2m) return;
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre,
   const LineCf l = line_cf<2, C>(g, f, sb, cre, cre2);
   const long st = (long)g.NX * g.NY, q0 = P(g, i, j, 0);
   double *r = f.rhs[C];
-  double *sq = f.s1, *ss = f.s2;
+  double *sq = f.sa[C % 3], *ss = f.sb[C % 3];
   const double fn = l.bd * r[q0 + n * st];
   double qp, sp, fp;
   {
@@ -685,15 +686,15 @@ __global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre,
 
 // non-periodic y line (trdiag2, slv_mmtm.f90:1415-1455); result into rhs
 template <int C>
-__global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
-                                               int klo) {
+__device__ inline void adi_y(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int k = klo + blockIdx.y * 4 + threadIdx.y;
   if (i > g.n1m || k > g.n3m) return;
   const LineCf l = line_cf<1, C>(g, f, sb, cre, cre2);
   const long st = g.NX, q0 = P(g, i, 0, k);
   double *r = f.rhs[C];
-  double *gm = f.s1;
+  double *gm = f.sa[C % 3];
   double unext = line_forward(l, r, gm, q0, st, jlo, g.n2m);
   // back substitution, pipelined like the forward pass; the loads run below
   // the line's start into plane k-1 (k >= 1), values unused
@@ -740,19 +741,20 @@ constexpr int kXch = 31;        // points per chunk
 constexpr int kXld = kXch + 2;  // LDS row stride (odd, conflict-free columns)
 
 template <int C>
-__global__ __launch_bounds__(64) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2, int ilo, int jlo,
-                                              int klo) {
-  __shared__ double V[64 * kXld], R[64 * kXld], G[64 * kXld], AM[kXch], CM[kXch];
+__device__ inline void adi_x(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *V, double *R,
+                             double *G, double *AM, double *CM) {
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int lane = threadIdx.x;
   const int nj = g.n2m - jlo + 1;
   const int nlines = nj * (g.n3m - klo + 1);
   const int line0 = blockIdx.x * 64;
+  if (line0 >= nlines) return;  // the grid covers the component with the most lines
   const int nl = nlines - line0 < 64 ? nlines - line0 : 64;  // lines of this wavefront
   const bool mine = lane < nl;
   const LineCf l = line_cf<0, C>(g, f, sb, cre, cre2);
   double *fld = C == 0 ? f.u : C == 1 ? f.v : C == 2 ? f.w : f.t;
   double *r = f.rhs[C];
-  double *gm = f.s1;
+  double *gm = f.sa[C % 3];
   const int n1m = g.n1m;
   // row offset of this lane's line; line L's offset is lane L's (broadcast)
   const long q0 = mine ? P(g, 0, jlo + (line0 + lane) % nj, klo + (line0 + lane) / nj) : 0;
@@ -871,38 +873,76 @@ __global__ __launch_bounds__(64) void k_adi_x(Geo g, Sub sb, Fld f, double cre, 
   }
 }
 
-template <int C>
-static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
-  const int ilo = C == 0 ? g.ib : 1;
-  const int jlo = C == 1 ? g.jb : 1;
-  const int klo = C == 2 ? g.kb : 1;
-  static const char *nz[4] = {"k_adi_z<u>", "k_adi_z<v>", "k_adi_z<w>", "k_adi_z<t>"};
-  static const char *ny[4] = {"k_adi_y<u>", "k_adi_y<v>", "k_adi_y<w>", "k_adi_y<t>"};
-  static const char *nx[4] = {"k_adi_x<u>", "k_adi_x<v>", "k_adi_x<w>", "k_adi_x<t>"};
-  void (*kz)(Geo, Sub, Fld, double, double, int, int) = k_adi_z<C>;
-  void (*ky)(Geo, Sub, Fld, double, double, int, int, int) = k_adi_y<C>;
-  void (*kx)(Geo, Sub, Fld, double, double, int, int, int) = k_adi_x<C>;
-  // z and y sweeps: viscosity and rhs in, rhs out; x sweep: also field in/out
-  if (g.n3m != 1)
-    LW_LAUNCH(nz[C], pts(g) * 24.0, 0, kz, grid3(g.n1m - ilo + 1, g.n2m - jlo + 1, 1), BLK3, 0, s, g, sb, f, cre,
-              cre2, ilo, jlo);
-  LW_LAUNCH(ny[C], pts(g) * 24.0, 0, ky, grid3(g.n1m - ilo + 1, g.n3m - klo + 1, 1), BLK3, 0, s, g, sb, f, cre,
-            cre2, ilo, jlo, klo);
-  const int xlines = (g.n2m - jlo + 1) * (g.n3m - klo + 1);
-  LW_LAUNCH(nx[C], pts(g) * 32.0, 0, kx, dim3((unsigned)((xlines + 63) / 64)), dim3(64), 0, s, g, sb, f, cre, cre2,
-            ilo, jlo, klo);
+// One launch per direction for all momentum components at once (their
+// solves are independent: each has its own rhs, coefficients and scratch),
+// so the serial line recurrences of u, v and w overlap on the device.
+// blockIdx.z (z, y sweeps) or blockIdx.y (x sweep) is the component offset
+// from C0; C0 = 3 is the temperature alone.
+template <int C0>
+__global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  if constexpr (C0 == 3) {
+    adi_z<3>(g, sb, f, cre, cre2);
+  } else {
+    switch (blockIdx.z) {
+      case 0: adi_z<0>(g, sb, f, cre, cre2); break;
+      case 1: adi_z<1>(g, sb, f, cre, cre2); break;
+      default: adi_z<2>(g, sb, f, cre, cre2); break;
+    }
+  }
 }
 
-void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les) {
+template <int C0>
+__global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  if constexpr (C0 == 3) {
+    adi_y<3>(g, sb, f, cre, cre2);
+  } else {
+    switch (blockIdx.z) {
+      case 0: adi_y<0>(g, sb, f, cre, cre2); break;
+      case 1: adi_y<1>(g, sb, f, cre, cre2); break;
+      default: adi_y<2>(g, sb, f, cre, cre2); break;
+    }
+  }
+}
+
+template <int C0>
+__global__ __launch_bounds__(64) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  __shared__ double V[64 * kXld], R[64 * kXld], G[64 * kXld], AM[kXch], CM[kXch];
+  if constexpr (C0 == 3) {
+    adi_x<3>(g, sb, f, cre, cre2, V, R, G, AM, CM);
+  } else {
+    switch (blockIdx.y) {
+      case 0: adi_x<0>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+      case 1: adi_x<1>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+      default: adi_x<2>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+    }
+  }
+}
+
+// z, y, x sweeps of nc components from c0 (momentum: c0 = 0, nc = 3, or 2
+// when n3m = 1 and w has no equation; temperature: c0 = 3, nc = 1)
+template <int C0>
+static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, int nc) {
+  static const char *nz = C0 == 3 ? "k_adi_z<t>" : "k_adi_z<uvw>";
+  static const char *ny = C0 == 3 ? "k_adi_y<t>" : "k_adi_y<uvw>";
+  static const char *nx = C0 == 3 ? "k_adi_x<t>" : "k_adi_x<uvw>";
+  // the grids cover the widest component; z and y sweeps: viscosity and rhs
+  // in, rhs out; x sweep: also the field in and out (per component)
+  if (g.n3m != 1)
+    LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, k_adi_z<C0>, grid3(g.n1m, g.n2m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
+  LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, k_adi_y<C0>, grid3(g.n1m, g.n3m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
+  const int xlines = g.n2m * g.n3m;
+  LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_x<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64), 0,
+            s, g, sb, f, cre, cre2);
+}
+
+void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
   const double cre = (double)les * sb.re, cre2 = 2. * (double)les * sb.re;
-  if (comp == 0) adi3<0>(s, g, sb, f, cre, cre2);
-  else if (comp == 1) adi3<1>(s, g, sb, f, cre, cre2);
-  else adi3<2>(s, g, sb, f, cre, cre2);
+  adi3<0>(s, g, sb, f, cre, cre2, g.n3m != 1 ? 3 : 2);
 }
 
 void launch_adi_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
   const double cre = (double)les * sb.re * sb.pr;
-  adi3<3>(s, g, sb, f, cre, 0.0);
+  adi3<3>(s, g, sb, f, cre, 0.0, 1);
 }
 
 // retrv_uvw + prdic_adj_uvw(0) for periodic z (slv_mmtm.f90:1710-1857)

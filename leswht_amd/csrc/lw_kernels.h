@@ -36,7 +36,7 @@ struct Fld {
   double *rk3o[4], *rk3oo[4];
   double *nu, *nu1[3], *al, *al1[3];
   double *qmass, *div, *phi;
-  double *s1, *s2;  // ADI scratch (gam, or q and s of the periodic solve)
+  double *sa[3], *sb[3];  // ADI scratch per momentum component (gam, or q and s of the periodic solve)
   uint8_t *mask;
   // outlet planes (0:n2, 0:n3), index j + NY*k
   double *uout, *vout, *wout, *tout, *duout, *dvout, *dwout, *dtout;
@@ -60,7 +60,7 @@ constexpr int kRedThreads = 256;
 void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
 void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp);
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b, int les);
-void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int comp, int les);
+void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f);
 void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson);
 void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f);

@@ -320,8 +320,10 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.qmass = h->dalloc<double>(np);
   f.div = h->dalloc<double>(np);
   f.phi = h->dalloc<double>(np);
-  f.s1 = h->dalloc<double>(np);
-  f.s2 = h->dalloc<double>(np);
+  for (int c = 0; c < 3; ++c) {
+    f.sa[c] = h->dalloc<double>(np);
+    f.sb[c] = h->dalloc<double>(np);
+  }
   f.uout = h->dalloc<double>(nb); f.vout = h->dalloc<double>(nb); f.wout = h->dalloc<double>(nb);
   f.tout = h->dalloc<double>(nb); f.duout = h->dalloc<double>(nb); f.dvout = h->dalloc<double>(nb);
   f.dwout = h->dalloc<double>(nb); f.dtout = h->dalloc<double>(nb);
@@ -826,9 +828,7 @@ static void phase_rhsnlhs(lw_handle *h) {
   if (h->st.ibmon != 0)
     for (int c = 0; c < 3; ++c) launch_ibm(h->s, g, h->sub, h->f, h->ib, c);
   launch_convbc_mom(h->s, g, h->sub, h->f, b, h->les);
-  launch_adi_mom(h->s, g, h->sub, h->f, 0, h->les);
-  launch_adi_mom(h->s, g, h->sub, h->f, 1, h->les);
-  if (g.n3m != 1) launch_adi_mom(h->s, g, h->sub, h->f, 2, h->les);
+  launch_adi_mom(h->s, g, h->sub, h->f, h->les);
   launch_retrv_mom(h->s, g, h->f);
 }
 
