@@ -671,6 +671,7 @@ template <int R, bool FULL>
 __global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, int lc) {
   const int kv = 1 + blockIdx.x;
   if (mg.done[kv - 1]) return;
+  if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
   const int tid = threadIdx.y * 64 + threadIdx.x;
   for (int ilev = lc; ilev >= 1; --ilev) {
     const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, b = mg.iimg[ilev][1];
@@ -736,6 +737,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.x;
   if (mg.done[kv - 1]) return;
+  if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
   const int lane = threadIdx.x, w = threadIdx.y;
   const int n2m = FULL ? 64 * R : g.n2m;
   const int b1 = mg.iimg[1][1], n1 = mg.iimg[1][2] - b1 + 1;
@@ -1164,10 +1166,25 @@ static int g_tile_down = 4, g_tile_up = 4, g_tile_top = 8;
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T) {
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
+                                                                      int cyc) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
+  if (cyc >= 0) {
+    // the convergence test of the toplevel just done (k_mg_check's), folded
+    // into the first launch of the next cycle: every workgroup decides from
+    // the max norm, which no kernel of this launch writes; workgroup (0, kv)
+    // records it. An active wavenumber has completed `cyc` V-cycles.
+    const double r = __longlong_as_double((long long)mg.sumres[kv - 1]);
+    const int dn = r < test ? 1 : cyc >= mg.mgitr ? 2 : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+      mg.cyc[kv - 1] = cyc;
+      mg.pcsel[kv - 1] ^= 1;  // k_mg_top wrote the new pc to the other buffer
+      if (dn) mg.done[kv - 1] = dn;
+    }
+    if (dn) return;
+  }
   const int lane = threadIdx.x, w = threadIdx.y;
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
@@ -1518,9 +1535,10 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
 #undef TILE_UP
 
 typedef void (*FusedLevelKernel)(Geo, MG, int, int);
+typedef void (*FusedDownKernel)(Geo, MG, int, int, double, int);
 
 template <bool F>
-static FusedLevelKernel pick_down_rf(int R) {
+static FusedDownKernel pick_down_rf(int R) {
   switch (R) {
     case 1: return k_mg_down<1, F>;
     case 2: return k_mg_down<2, F>;
@@ -1530,7 +1548,7 @@ static FusedLevelKernel pick_down_rf(int R) {
   }
 }
 
-static FusedLevelKernel pick_down(int R, bool full) {
+static FusedDownKernel pick_down(int R, bool full) {
   return full ? pick_down_rf<true>(R) : pick_down_rf<false>(R);
 }
 template <bool F>
@@ -1569,13 +1587,14 @@ static inline unsigned fused_lds(int R, int T, bool up) {
   return (unsigned)((up ? T / 2 + 1 : T + 3) * R * 64 * (int)sizeof(double2));
 }
 
-static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
+static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, double test, int cyc) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_down), R = lanes_r(g.n2m);
   // unique bytes per point of the level: reads resd 16, bet 8, gam 8, ac 8;
   // writes the second-half solution 16 on half the lines (8), the residual
   // 16 and the restricted row 16 per pair (8)
   LW_LAUNCH("k_mg_down", 0, 72.0 * g.n2m * n, pick_down(R, full_rows(g)),
-            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T);
+            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T,
+            test, cyc);
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
@@ -1600,7 +1619,7 @@ static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double
 
 static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
 
-void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test) {
+void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test, bool check) {
   const int nl = mg.nlev;
   if (mg.fused) {
     launch_top(s, g, mg, id, oldv);
@@ -1610,16 +1629,25 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
     // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
     LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
   }
-  LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id, mg.fused);
+  if (check) LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id, mg.fused);
 }
 
-void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
+// whether the convergence test after a toplevel may move into the next
+// cycle's first launch: the fused path with a k_mg_down level above a coarse
+// kernel (which clears the max norms for the next toplevel)
+bool mg_folds_check(const MG &mg) { return mg.fused && mg.lc >= 0 && mg.nlev - 1 > mg.lc; }
+
+// one V-cycle. cyc >= 0: the first launch runs the convergence test of the
+// preceding toplevel, after which an active wavenumber has completed cyc
+// cycles (mg_folds_check); cyc < 0: a k_mg_check already ran it. check: end
+// the cycle with k_mg_check.
+void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int cyc, bool check) {
   auto nrow = [&](int l) { return mg.iimg[l][2] - mg.iimg[l][1] + 1; };
   // levels above lc: one launch per zebra half, one per residual+restriction
   for (int ilev = mg.nlev - 1; ilev >= 1 && ilev > mg.lc; --ilev) {
     const int n = nrow(ilev);
     if (mg.fused) {
-      launch_down(s, g, mg, ilev);
+      launch_down(s, g, mg, ilev, test, ilev == mg.nlev - 1 ? cyc : -1);
       continue;
     }
     launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
@@ -1645,7 +1673,7 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test) {
     launch_lines(s, g, mg, 4, mg.iimg[ilev + 1][1], (n + 1) / 2, 1.0, ilev);
     launch_lines(s, g, mg, 1, mg.iimg[ilev + 1][1] + 1, n / 2, 1.0, 0);
   }
-  launch_mg_toplevel(s, g, mg, 1, 1.0, test);
+  launch_mg_toplevel(s, g, mg, 1, 1.0, test, check);
 }
 
 // once per handle, before the first cycle: dynamic LDS beyond 64 KiB must be

@@ -111,8 +111,9 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg);
 void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
 // toplevel (poiss_z_ft.f90:302-380), id 0 or 1, ending with the per-wavenumber
 // convergence test against `test`
-void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test);
+void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test, bool check);
+bool mg_folds_check(const MG &mg);
 // one V-cycle of every wavenumber still iterating (poiss_z_ft.f90:258-299)
-void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test);
+void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int cyc, bool check);
 
 }  // namespace lw

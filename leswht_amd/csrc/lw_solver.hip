@@ -848,14 +848,19 @@ static void phase_poisson(lw_handle *h) {
     g_prof->mg_first = g_prof->recs.size();
     g_prof->cycle = 0;
   }
-  launch_mg_toplevel(h->s, g, mg, 0, 0.0, test);
+  // with the check folded, the test after a toplevel runs in the next
+  // cycle's first launch, and a k_mg_check only ends each batch
+  const bool fold = mg_folds_check(mg);
+  launch_mg_toplevel(h->s, g, mg, 0, 0.0, test, !fold);
   int launched = 0, batch = h->mg_hint > 1 ? h->mg_hint : 1;
   bool all_done = false;
   while (!all_done && launched < mg.mgitr) {
     if (batch > mg.mgitr - launched) batch = mg.mgitr - launched;
     for (int c = 0; c < batch; ++c) {
-      if (g_prof) g_prof->cycle = launched + c + 1;
-      launch_mg_cycle(h->s, g, mg, test);
+      const int m = launched + c + 1;  // this V-cycle's number in the solve
+      if (g_prof) g_prof->cycle = m;
+      const int cyc = fold && !(c == 0 && launched > 0) ? m - 1 : -1;
+      launch_mg_cycle(h->s, g, mg, test, cyc, !fold || c == batch - 1);
     }
     launched += batch;
     HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
