@@ -3,6 +3,7 @@
 // expression (built with -ffp-contract=off) so the HIP results agree with the
 // FP64 reference to rounding; the reference file:line is on each kernel.
 #include "lw_kernels.h"
+#include "lw_scan.h"
 
 namespace lw {
 
@@ -873,6 +874,110 @@ __device__ inline void adi_x(const Geo &g, const Sub &sb, const Fld &f, double c
   }
 }
 
+// y sweep with one half-wavefront per line (tri_line, lines of 32 lanes): a
+// workgroup stages the viscosity and right-hand side of kYL lines of
+// consecutive i (one plane k) through LDS, read along i; each half-wavefront
+// solves one line out of LDS with the line held in registers, and the
+// solutions leave through LDS the same way. No per-point scratch goes to HBM.
+constexpr int kYL = 16;
+
+__host__ __device__ inline int adi_y_ld(int n2) { return n2 + 1; }
+
+template <int R, bool FULL, int C>
+__device__ inline void adi_y_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+                                  double *tr) {
+  const LineCf l = line_cf<1, C>(g, f, sb, cre, cre2);
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
+  const int i0 = ilo + blockIdx.x * kYL, k = klo + blockIdx.y;
+  if (k > g.n3m) return;  // uniform: the grid covers the widest component
+  const int n1m = g.n1m, n2m = g.n2m, N = n2m - jlo + 1, LD = adi_y_ld(g.n2);
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  double *r = f.rhs[C];
+  for (int q = tid; q < kYL * (g.n2 + 1); q += 32 * kYL) {
+    const int li = q % kYL, j = q / kYL, i = i0 + li;
+    const long p = P(g, i <= n1m ? i : n1m, j, k);
+    tv[li * LD + j] = l.vis[p];
+    tr[li * LD + j] = r[p];
+  }
+  __syncthreads();
+  // two lines per wavefront, one per half (lines of 32 lanes)
+  const int lane = threadIdx.x, ll = lane & 31, li = 2 * threadIdx.y + (lane >> 5);
+  const double *v = tv + li * LD;
+  double *rr = tr + li * LD;
+  double a[R], b[R], c[R], ff[R], x[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int t = 1 + ll * R + u;
+    const int j = (FULL || t <= N) ? jlo + t - 1 : n2m;
+    a[u] = l.am[j] * (1. + l.cc * v[j + l.lo]);
+    c[u] = l.cm[j] * (1. + l.cc * v[j + l.lo + 1]);
+    b[u] = l.bd - a[u] - c[u];
+    ff[u] = l.bd * rr[j];
+  }
+  tri_line<R, FULL, 32>(a, b, c, ff, N, lane, x);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int t = 1 + ll * R + u;
+    if (FULL || t <= N) rr[jlo + t - 1] = x[u];
+  }
+  __syncthreads();
+  for (int q = tid; q < kYL * N; q += 32 * kYL) {
+    const int li2 = q % kYL, j = jlo + q / kYL, i = i0 + li2;
+    if (i <= n1m) r[P(g, i, j, k)] = tr[li2 * LD + j];
+  }
+}
+
+template <int R, int C>
+__device__ inline void adi_y_wave_c(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+                                    double *tr) {
+  const int jlo = C == 1 ? g.jb : 1;
+  if (g.n2m - jlo + 1 == 32 * R) adi_y_wave<R, true, C>(g, sb, f, cre, cre2, tv, tr);
+  else adi_y_wave<R, false, C>(g, sb, f, cre, cre2, tv, tr);
+}
+
+template <int R, int C0>
+__global__ __launch_bounds__(32 * kYL) void k_adi_yw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  extern __shared__ double yl[];
+  double *tv = yl, *tr = yl + kYL * adi_y_ld(g.n2);
+  if constexpr (C0 == 3) {
+    adi_y_wave_c<R, 3>(g, sb, f, cre, cre2, tv, tr);
+  } else {
+    switch (blockIdx.z) {
+      case 0: adi_y_wave_c<R, 0>(g, sb, f, cre, cre2, tv, tr); break;
+      case 1: adi_y_wave_c<R, 1>(g, sb, f, cre, cre2, tv, tr); break;
+      default: adi_y_wave_c<R, 2>(g, sb, f, cre, cre2, tv, tr); break;
+    }
+  }
+}
+
+typedef void (*AdiYwKernel)(Geo, Sub, Fld, double, double);
+template <int C0>
+static AdiYwKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
+  if (n2m <= 32) return k_adi_yw<1, C0>;
+  if (n2m <= 64) return k_adi_yw<2, C0>;
+  if (n2m <= 128) return k_adi_yw<4, C0>;
+  if (n2m <= 256) return k_adi_yw<8, C0>;
+  return nullptr;
+}
+static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * adi_y_ld(g.n2) * sizeof(double)); }
+
+// once per handle: the wave-per-line y sweep needs more than 64 KiB of LDS
+// at 256 rows; 0 = not used (more than 256 rows, or LW_ADI_WAVE=0)
+hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
+  *use_wave = 0;
+  if (const char *e = getenv("LW_ADI_WAVE"))
+    if (atoi(e) == 0) return hipSuccess;
+  if (!pick_adi_yw<0>(g.n2m) || adi_yw_lds(g) > 160 * 1024) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute((const void *)pick_adi_yw<0>(g.n2m),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)adi_yw_lds(g));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)pick_adi_yw<3>(g.n2m), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)adi_yw_lds(g));
+  if (e == hipSuccess) *use_wave = 1;
+  return e;
+}
+
 // One launch per direction for all momentum components at once (their
 // solves are independent: each has its own rhs, coefficients and scratch),
 // so the serial line recurrences of u, v and w overlap on the device.
@@ -929,7 +1034,12 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   // in, rhs out; x sweep: also the field in and out (per component)
   if (g.n3m != 1)
     LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, k_adi_z<C0>, grid3(g.n1m, g.n2m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
-  LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, k_adi_y<C0>, grid3(g.n1m, g.n3m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
+  if (g.adi_wave)
+    LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, pick_adi_yw<C0>(g.n2m),
+              dim3((unsigned)((g.n1m + kYL - 1) / kYL), (unsigned)g.n3m, (unsigned)nc), dim3(64, kYL / 2), adi_yw_lds(g), s,
+              g, sb, f, cre, cre2);
+  else
+    LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, k_adi_y<C0>, grid3(g.n1m, g.n3m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
   const int xlines = g.n2m * g.n3m;
   LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_x<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64), 0,
             s, g, sb, f, cre, cre2);

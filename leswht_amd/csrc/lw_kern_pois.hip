@@ -17,6 +17,7 @@
 // sets its flag on the device and every later kernel skips it, so each keeps
 // the reference's own V-cycle count.
 #include "lw_kernels.h"
+#include "lw_scan.h"
 
 namespace lw {
 
@@ -25,25 +26,6 @@ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x -
 __device__ inline double2 rmul(double a, double2 b) { return make_double2(a * b.x, a * b.y); }
 
 #define ROW(base, i_) ((base) + (long)(i_) * mg.LDJ)
-
-// DPP moves of a double (both halves with the same control; lanes whose
-// source is outside their 16-lane row receive `old`)
-template <int CTRL, int ROWS = 0xF>
-__device__ inline double dpp_d(double v, double old) {
-  const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
-  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, ROWS, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, ROWS, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-__device__ inline double rdlane(double v, int l) {
-  const long long vb = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)vb, l), hi = __builtin_amdgcn_readlane((int)(vb >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;  // row_shr:n
-constexpr int kShl1 = 0x101, kShl2 = 0x102, kShl4 = 0x104, kShl8 = 0x108;  // row_shl:n
-constexpr int kBcast15 = 0x142, kBcast31 = 0x143;  // row_bcast:15, row_bcast:31
-constexpr int kWshl1 = 0x130, kWshr1 = 0x138;      // wave_shl:1, wave_shr:1
 
 // prefix step of the affine maps (M, C) of the forward substitution: this
 // lane's map after the map of the lane D below it (identity across rows)

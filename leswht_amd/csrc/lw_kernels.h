@@ -61,6 +61,7 @@ void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, in
 void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp);
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b, int les);
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+hipError_t prepare_adi_kernels(const Geo &g, int *use_wave);
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f);
 void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson);
 void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f);

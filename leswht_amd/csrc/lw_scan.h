@@ -1,0 +1,208 @@
+// Wavefront-wide scans for line solves on gfx950: DPP moves of doubles and a
+// tridiagonal solve with one line per wavefront.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lw {
+
+// DPP moves of a double (both halves with the same control; lanes whose
+// source is outside their 16-lane row, or rows outside ROWS, receive `old`)
+template <int CTRL, int ROWS = 0xF>
+__device__ inline double dpp_d(double v, double old) {
+  const long long vb = __double_as_longlong(v), ob = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)ob, (int)vb, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double rdlane(double v, int l) {
+  const long long vb = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)vb, l), hi = __builtin_amdgcn_readlane((int)(vb >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;  // row_shr:n
+constexpr int kShl1 = 0x101, kShl2 = 0x102, kShl4 = 0x104, kShl8 = 0x108;  // row_shl:n
+constexpr int kBcast15 = 0x142, kBcast31 = 0x143;  // row_bcast:15, row_bcast:31
+constexpr int kWshl1 = 0x130, kWshr1 = 0x138;      // wave_shl:1, wave_shr:1
+
+// 2x2 matrices of the pivot recurrence, row-major (p, q; r, s)
+struct M22 {
+  double p, q, r, s;
+};
+__device__ inline M22 m22_mul(const M22 &A, const M22 &B) {  // A*B
+  return M22{A.p * B.p + A.q * B.r, A.p * B.q + A.q * B.s, A.r * B.p + A.s * B.r, A.r * B.q + A.s * B.s};
+}
+// scale by the power of two that brings the largest entry near 1 (exact)
+__device__ inline M22 m22_norm(const M22 &A) {
+  const double m = fmax(fmax(fabs(A.p), fabs(A.q)), fmax(fabs(A.r), fabs(A.s)));
+  if (m == 0.0) return A;
+  const int e = -__builtin_amdgcn_frexp_exp(m);
+  return M22{ldexp(A.p, e), ldexp(A.q, e), ldexp(A.r, e), ldexp(A.s, e)};
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ inline M22 m22_dpp(const M22 &A) {  // identity where nothing moves in
+  return M22{dpp_d<CTRL, ROWS>(A.p, 1.0), dpp_d<CTRL, ROWS>(A.q, 0.0), dpp_d<CTRL, ROWS>(A.r, 0.0),
+             dpp_d<CTRL, ROWS>(A.s, 1.0)};
+}
+
+// Solves a_t x_{t-1} + b_t x_t + c_t x_{t+1} = f_t, t = 1..N (a_1 and c_N
+// unused) with one line per wavefront: lane l owns rows t = 1 + l*R .. l*R+R
+// (N <= 64*R; FULL: N == 64*R). The arithmetic of a row is Thomas's
+// (trdiag1/trdiag2, slv_mmtm.f90:1365-1455): gam_t = c_{t-1} bet_{t-1},
+// bet_t = 1/(b_t - a_t gam_t), y_t = (f_t - a_t y_{t-1}) bet_t,
+// x_t = y_t - gam_{t+1} x_{t+1}. What a lane needs from the rows before it
+// comes from scans: the pivot d = 1/bet of its first row from the products
+// of the 2x2 maps (p_t, p_{t-1}) -> (b_t p_t - a_t c_{t-1} p_{t-1}, p_t),
+// d_t = p_t / p_{t-1} (normalised by exact powers of two), and y, x from the
+// affine maps of the two substitutions.
+// LANES = 32: each half of the wavefront solves its own line (lanes 0-31,
+// 32-63), with half the scan steps per line.
+template <int R, bool FULL, int LANES = 64>
+__device__ inline void tri_line(const double (&a)[R], const double (&b)[R], const double (&c)[R],
+                                const double (&f)[R], int N, int lane, double (&x)[R]) {
+  const int ll = LANES == 64 ? lane : (lane & 31);  // lane within the line
+  const int t0 = 1 + ll * R;
+  auto ok = [&](int r) { return FULL || t0 + r <= N; };
+  // c of the row before this lane's first (unused on a line's first lane)
+  const double cl = dpp_d<kWshr1>(c[R - 1], 0.0);
+  // pivot maps: this lane's product, then the exclusive scan over the lanes
+  M22 L{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ok(r)) {
+      const double cp = r == 0 ? cl : c[r - 1];
+      const double q = t0 + r == 1 ? 0.0 : -(a[r] * cp);
+      L = M22{b[r] * L.p + q * L.r, b[r] * L.q + q * L.s, L.p, L.q};
+    }
+  }
+  L = m22_norm(L);
+#define PIV_STEP(CT)                       \
+  {                                        \
+    const M22 B_ = m22_dpp<CT>(L);         \
+    L = m22_norm(m22_mul(L, B_));          \
+  }
+  PIV_STEP(kShr1)
+  PIV_STEP(kShr2)
+  PIV_STEP(kShr4)
+  PIV_STEP(kShr8)
+#undef PIV_STEP
+  if (LANES == 64) {
+    L = m22_norm(m22_mul(L, m22_dpp<kBcast15, 0x2>(L)));
+    L = m22_norm(m22_mul(L, m22_dpp<kBcast31, 0x4>(L)));
+    L = m22_norm(m22_mul(L, m22_dpp<kBcast15, 0x8>(L)));
+  } else {
+    L = m22_norm(m22_mul(L, m22_dpp<kBcast15, 0xA>(L)));  // rows 1 and 3 of each line
+  }
+  const M22 E = m22_dpp<kWshr1>(L);  // the rows before this lane (identity for its first lane)
+  // bet of the row before this lane's first: p_{t0-1} / p_{t0-2}, inverted
+  double bet = ll == 0 ? 0.0 : 1.0 / (E.p / E.r);
+  double bt[R], gm[R];
+  double cprev = cl;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = t0 + r;
+    if (ok(r)) {
+      if (t == 1) {
+        bet = 1.0 / b[r];
+      } else {
+        const double gam = cprev * bet;
+        bet = 1.0 / (b[r] - a[r] * gam);
+      }
+      bt[r] = bet;
+      gm[r] = t == N ? 0.0 : c[r] * bet;  // gam_{t+1}
+      cprev = c[r];
+    } else {
+      bt[r] = 0.0;
+      gm[r] = 0.0;
+    }
+  }
+  // forward substitution y_t = (f_t - a_t y_{t-1}) bet_t as affine maps
+  double M = 1.0, Cc = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ok(r)) {
+      const double m = t0 + r == 1 ? 0.0 : -(a[r] * bt[r]);
+      Cc = m * Cc + bt[r] * f[r];
+      M = m * M;
+    }
+  }
+#define FWD_R(CT)                       \
+  {                                     \
+    const double Mp = dpp_d<CT>(M, 1.0); \
+    const double Cp = dpp_d<CT>(Cc, 0.0); \
+    Cc = M * Cp + Cc;                   \
+    M = M * Mp;                         \
+  }
+  FWD_R(kShr1)
+  FWD_R(kShr2)
+  FWD_R(kShr4)
+  FWD_R(kShr8)
+#undef FWD_R
+  if (LANES == 64) {
+    Cc = M * dpp_d<kBcast15, 0x2>(Cc, 0.0) + Cc;
+    Cc = M * dpp_d<kBcast31, 0x4>(Cc, 0.0) + Cc;
+    Cc = M * dpp_d<kBcast15, 0x8>(Cc, 0.0) + Cc;
+  } else {
+    Cc = M * dpp_d<kBcast15, 0xA>(Cc, 0.0) + Cc;
+  }
+  double yp = dpp_d<kWshr1>(Cc, 0.0);
+  if (LANES == 32 && ll == 0) yp = 0.0;
+  double y[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ok(r)) {
+      yp = t0 + r == 1 ? f[r] * bt[r] : (f[r] - a[r] * yp) * bt[r];
+      y[r] = yp;
+    } else {
+      y[r] = 0.0;
+    }
+  }
+  // back substitution x_t = y_t - gam_{t+1} x_{t+1}
+  double Nn = 1.0, D = 0.0;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    if (ok(r)) {
+      const double nj = t0 + r == N ? 0.0 : -gm[r];
+      D = nj * D + y[r];
+      Nn = nj * Nn;
+    }
+  }
+#define BWD_R(CT)                        \
+  {                                      \
+    const double Np = dpp_d<CT>(Nn, 1.0); \
+    const double Dp = dpp_d<CT>(D, 0.0);  \
+    D = Nn * Dp + D;                     \
+    Nn = Nn * Np;                        \
+  }
+  BWD_R(kShl1)
+  BWD_R(kShl2)
+  BWD_R(kShl4)
+  BWD_R(kShl8)
+#undef BWD_R
+  if (LANES == 64) {
+    const double n1 = rdlane(Nn, 16), n2 = rdlane(Nn, 32);
+    const double d1 = rdlane(D, 16), d2 = rdlane(D, 32), d3 = rdlane(D, 48);
+    const double s2 = n2 * d3 + d2;  // start of row 2
+    const double s1 = n1 * s2 + d1;  // start of row 1
+    const int row = lane >> 4;
+    const double next = row == 3 ? 0.0 : row == 2 ? d3 : row == 1 ? s2 : s1;
+    D = Nn * next + D;
+  } else {
+    const double d1 = rdlane(D, 16), d3 = rdlane(D, 48);
+    const int row = lane >> 4;
+    const double next = row == 0 ? d1 : row == 2 ? d3 : 0.0;
+    D = Nn * next + D;
+  }
+  double xn = dpp_d<kWshl1>(D, 0.0);
+  if (LANES == 32 && ll == 31) xn = 0.0;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    if (ok(r)) {
+      xn = t0 + r == N ? y[r] : y[r] - gm[r] * xn;
+      x[r] = xn;
+    } else {
+      x[r] = 0.0;
+    }
+  }
+}
+
+}  // namespace lw

@@ -547,6 +547,7 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.fused = 1;
   if (const char *e = getenv("LW_MG_FUSED")) mg.fused = atoi(e) != 0;
   HIPCHK(lw::prepare_mg_kernels(g, mg));  // clears fused where the tiles do not fit LDS
+  HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
   mg.pc = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   mg.rhs = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
   if (mg.fused) {
