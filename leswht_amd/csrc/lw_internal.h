@@ -71,7 +71,9 @@ struct MG {
   int iimg[21][4];
   // coefficients
   double *aw, *ae, *as, *an, *coi1, *coi2, *cor1, *cor2;
-  double *ac, *gam, *bet;  // [kv][i][j], real
+  double *awe;             // aw + ae per line: ac = -((awe + as) + an) - ak3
+  double *ak3;             // modified z wavenumber squared per kv (ak3[1] = 0)
+  double *bet;             // [kv][i][j], real; gam(j+1) = an(j) bet(j)
   double2 *gg, *resd;      // [kv][i][j], all levels stacked along i
   double2 *gd, *rsu;       // fused kernels: down-sweep second halves, residuals
   int fused;               // 1: one kernel per level and direction

@@ -24,6 +24,10 @@ namespace lw {
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 rmul(double a, double2 b) { return make_double2(a * b.x, a * b.y); }
+// diagonal of the wavenumber-kv operator, ac(i,j,kv) (pmat and coefmg,
+// poiss_z_ft.f90:78-88, 696-703): the same additions in the same order as the
+// stored table, so the same bits (ak3(1) = 0 and x - 0 = x)
+__device__ inline double ac_of(double awe, double as, double an, double ak) { return -((awe + as) + an) - ak; }
 
 #define ROW(base, i_) ((base) + (long)(i_) * mg.LDJ)
 
@@ -461,7 +465,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
   const int i = ibase + 2 * t;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
   double2 *gg = mg.gg + pl;
-  const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
+  const double *bet = ROW(mg.bet + pl, i);
   const int im = i - 1, ip = i + 1;
   const double aw = mg.aw[i], ae = mg.ae[i];
   constexpr bool TOP = MODE == 2 || MODE == 3 || MODE == 5;
@@ -509,7 +513,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
     }
     const double b = bet[jc];
     const double a = mg.as[jc];
-    const double gnext = gam[jc < n2m ? jc + 1 : n2m];
+    const double gnext = mg.an[jc] * b;  // gam(j+1)
     // masks as factors (exact for finite values): a select would let the
     // compiler move the loads into a branch per row again
     const double okf = (FULL || j <= n2m) ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
@@ -581,23 +585,23 @@ static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, in
   static const char *names[6] = {"k_line_relax1", "k_line_relax2", "k_line_top1",
                                  "k_line_top2",   "k_line_relax1_up", "k_line_top1_up"};
   // per row: own right-hand side 16, one neighbour row 16 (each shared by two
-  // lines), bet 8, gam 8, solution 16; the prolonging modes also read one
-  // coarse row per line (16)
-  const double per_row = mode >= 4 ? 80.0 : 64.0;
+  // lines), bet 8, solution 16; the prolonging modes also read one coarse row
+  // per line (16)
+  const double per_row = mode >= 4 ? 72.0 : 56.0;
   LW_LAUNCH(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.n3mh),
             dim3(64, 4), 0, s, g, mg, ibase, nlines, oldv, clev);
 }
 
 // residual of row i from the rows around it (poiss_z_ft.f90:360-369, 438-449)
-#define RESID_ROW(r0_, gi_, gm_, gp_, ac_, j_)        \
-  ({                                                  \
-    double2 r_ = (r0_);                               \
-    r_ = csub(r_, rmul(aw, (gm_)));                   \
-    r_ = csub(r_, rmul(ae, (gp_)));                   \
-    r_ = csub(r_, rmul(mg.as[j_], gi_[(j_) - 1]));    \
-    r_ = csub(r_, rmul(mg.an[j_], gi_[(j_) + 1]));    \
-    r_ = csub(r_, rmul((ac_)[j_], gi_[j_]));          \
-    r_;                                               \
+#define RESID_ROW(r0_, gi_, gm_, gp_, awe_, ak_, j_)                          \
+  ({                                                                         \
+    double2 r_ = (r0_);                                                      \
+    r_ = csub(r_, rmul(aw, (gm_)));                                          \
+    r_ = csub(r_, rmul(ae, (gp_)));                                          \
+    r_ = csub(r_, rmul(mg.as[j_], gi_[(j_) - 1]));                           \
+    r_ = csub(r_, rmul(mg.an[j_], gi_[(j_) + 1]));                           \
+    r_ = csub(r_, rmul(ac_of(awe_, mg.as[j_], mg.an[j_], ak_), gi_[j_]));    \
+    r_;                                                                      \
   })
 
 // going down (godown, poiss_z_ft.f90:438-458): residual of the fine rows
@@ -605,21 +609,20 @@ static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, in
 __device__ inline void lev_rr(const Geo &g, const MG &mg, int kv, int ilev, int t, int jfirst, int jstep) {
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 *gg = mg.gg + pl;
-  const double *acb = mg.ac + pl;
   double2 *rsb = mg.resd + pl;
   const int ia = mg.iimg[ilev][1] + 2 * t, ib = ia + 1;
   const int ic = mg.iimg[ilev - 1][1] + t;
   const double2 *ga = ROW(gg, ia), *gb = ROW(gg, ib), *gam_ = ROW(gg, ia - 1), *gbp = ROW(gg, ib + 1);
-  const double *aca = ROW(acb, ia), *acbb = ROW(acb, ib);
+  const double awea = mg.awe[ia], aweb = mg.awe[ib], ak = mg.ak3[kv];
   double2 *ra = ROW(rsb, ia), *rb = ROW(rsb, ib), *rc = ROW(rsb, ic);
   const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
   const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
   for (int j = jfirst; j <= g.n2m; j += jstep) {
     double aw = awa, ae = aea;
-    const double2 x = RESID_ROW(ra[j], ga, gam_[j], gb[j], aca, j);
+    const double2 x = RESID_ROW(ra[j], ga, gam_[j], gb[j], awea, ak, j);
     aw = awb;
     ae = aeb;
-    const double2 y = RESID_ROW(rb[j], gb, ga[j], gbp[j], acbb, j);
+    const double2 y = RESID_ROW(rb[j], gb, ga[j], gbp[j], aweb, ak, j);
     ra[j] = x;
     rb[j] = y;
     rc[j] = cadd(rmul(c1, x), rmul(c2, y));
@@ -703,14 +706,15 @@ struct LineLU {
 
 template <int R, bool FULL>
 __device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, LineLU<R> &c) {
-  const double *bet = ROW(mg.bet + pl, i), *gam = ROW(mg.gam + pl, i);
+  const double *bet = ROW(mg.bet + pl, i);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
     const int jc = (FULL || j <= n2m) ? j : n2m;
     const double okf = (FULL || j <= n2m) ? 1.0 : 0.0, okg = j < n2m ? 1.0 : 0.0;
-    c.bt[r] = okf * bet[jc];
-    c.gm[r] = okg * gam[jc < n2m ? jc + 1 : n2m];
+    const double b = bet[jc];
+    c.bt[r] = okf * b;
+    c.gm[r] = okg * (mg.an[jc] * b);  // gam(j+1)
   }
 }
 
@@ -785,13 +789,12 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // time: the scheduling barrier keeps the second line's loads from being
   // hoisted over the first line's arithmetic.
   {
-    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic], ak = mg.ak3[kv];
     double anj[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
     {
-      const double aw = mg.aw[ia], ae = mg.ae[ia];
-      const double *ac = ROW(mg.ac + pl, ia);
+      const double aw = mg.aw[ia], ae = mg.ae[ia], awe = mg.awe[ia];
       double2 *out = ROW(mg.resd + pl, ia);
       const double2 lo = lane > 0 ? L1[(la * R + R - 1) * 64 + lane - 1] : z2;
       const double2 hi = lane < 63 ? L1[(la * R) * 64 + lane + 1] : z2;
@@ -809,15 +812,14 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         u = csub(u, rmul(ae, gpv));
         u = csub(u, rmul(asj[r], dn));
         u = csub(u, rmul(anj[r], up));
-        u = csub(u, rmul(ac[jc], gi[r]));
+        u = csub(u, rmul(ac_of(awe, asj[r], anj[r], ak), gi[r]));
         resA[r] = u;
         if ((FULL || j0 + r <= n2m)) out[j0 + r] = u;
       }
     }
     __builtin_amdgcn_sched_barrier(0);
     {
-      const double aw = mg.aw[ib], ae = mg.ae[ib];
-      const double *ac = ROW(mg.ac + pl, ib);
+      const double aw = mg.aw[ib], ae = mg.ae[ib], awe = mg.awe[ib];
       double2 *out = ROW(mg.resd + pl, ib);
       const bool hp = lb + 1 < n1;
       const double2 lo = lane > 0 ? L1[(lb * R + R - 1) * 64 + lane - 1] : z2;
@@ -836,7 +838,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         v = csub(v, rmul(ae, gpv));
         v = csub(v, rmul(asj[r], dn));
         v = csub(v, rmul(anj[r], up));
-        v = csub(v, rmul(ac[jc], gi[r]));
+        v = csub(v, rmul(ac_of(awe, asj[r], anj[r], ak), gi[r]));
         if ((FULL || j0 + r <= n2m)) out[j0 + r] = v;
         res0[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, cadd(rmul(c1, resA[r]), rmul(c2, v)));
       }
@@ -987,13 +989,12 @@ static CoarseKernel pick_coarse(int R, bool full) {
 
 static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   // unique bytes per wavenumber: the restricted rows of level lc in (16)
-  // and its solution out (16) per point; bet and gam (16) on every level and
-  // ac (8) on the levels that form a residual; the levels below lc never
-  // need to leave the chip
+  // and its solution out (16) per point; bet (8) on every level; the
+  // levels below lc never need to leave the chip
   double per_kv = 0.0;
   for (int l = 0; l <= mg.lc; ++l) {
     const double pts = (double)(mg.iimg[l][2] - mg.iimg[l][1] + 1) * g.n2m;
-    per_kv += pts * (16.0 + (l >= 1 ? 8.0 : 0.0) + (l == mg.lc ? 32.0 : 0.0));
+    per_kv += pts * (8.0 + (l == mg.lc ? 32.0 : 0.0));
   }
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
@@ -1034,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
     const bool sbp = sh1 && ibp <= g.n1m;
     const double2 *ga = ROW(gg, ia);
     const double2 *gb = ROW(gg, ib), *gam_ = ROW(gg, ia - 1), *gbp = ROW(gg, ibp);
-    const double *aca = ROW(mg.ac + pl, ia), *acbb = ROW(mg.ac + pl, ib);
+    const double awea = mg.awe[ia], aweb = mg.awe[ib], ak = mg.ak3[kv];
     const double2 *rha = ROW(mg.rhs + ppl, ia), *rhb = ROW(mg.rhs + ppl, ib);
     double2 *ra = ROW(mg.resd + pl, ia), *rb = ROW(mg.resd + pl, ib), *rc = ROW(mg.resd + pl, ic);
     double2 *pa = ROW(mg.pc + ppl, ia), *pb = ROW(mg.pc + ppl, ib);
@@ -1051,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
       x = csub(x, rmul(ae, gb[j]));
       x = csub(x, rmul(mg.as[j], GA(j - 1)));
       x = csub(x, rmul(mg.an[j], GA(j + 1)));
-      x = csub(x, rmul(aca[j], gaj));
+      x = csub(x, rmul(ac_of(awea, mg.as[j], mg.an[j], ak), gaj));
       aw = awb;
       ae = aeb;
       const double2 gbpj = sbp ? csub(gbp[j], tt) : gbp[j];
@@ -1060,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
       y = csub(y, rmul(ae, gbpj));
       y = csub(y, rmul(mg.as[j], gb[j - 1]));
       y = csub(y, rmul(mg.an[j], gb[j + 1]));
-      y = csub(y, rmul(acbb[j], gb[j]));
+      y = csub(y, rmul(ac_of(aweb, mg.as[j], mg.an[j], ak), gb[j]));
       ra[j] = x;
       rb[j] = y;
       rc[j] = cadd(rmul(c1, x), rmul(c2, y));
@@ -1195,13 +1196,13 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG m
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
   if (pair) {
-    const double *aa = ROW(mg.ac + pl, bL + e), *ab = ROW(mg.ac + pl, bL + o);
+    const double awa = mg.awe[bL + e], awb = mg.awe[bL + o], ak = mg.ak3[kv];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      acA[r] = aa[jc];
-      acB[r] = ab[jc];
       anj[r] = mg.an[jc];
+      acA[r] = ac_of(awa, asj[r], anj[r], ak);
+      acB[r] = ac_of(awb, asj[r], anj[r], ak);
     }
   }
   // first half (oldv 0): the right-hand side is resd itself
@@ -1419,13 +1420,13 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
     load_lu<R, FULL>(mg, pl, 1 + o, j0, n2m, co);
   }
   if (pair) {
-    const double *aa = ROW(mg.ac + pl, 1 + e), *ab = ROW(mg.ac + pl, 1 + o);
+    const double awa = mg.awe[1 + e], awb = mg.awe[1 + o], ak = mg.ak3[kv];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      acA[r] = aa[jc];
-      acB[r] = ab[jc];
       anj[r] = mg.an[jc];
+      acA[r] = ac_of(awa, asj[r], anj[r], ak);
+      acB[r] = ac_of(awb, asj[r], anj[r], ak);
     }
   }
   if (eok) {
@@ -1571,29 +1572,29 @@ static inline unsigned fused_lds(int R, int T, bool up) {
 
 static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, double test, int cyc) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_down), R = lanes_r(g.n2m);
-  // unique bytes per point of the level: reads resd 16, bet 8, gam 8, ac 8;
-  // writes the second-half solution 16 on half the lines (8), the residual
-  // 16 and the restricted row 16 per pair (8)
-  LW_LAUNCH("k_mg_down", 0, 72.0 * g.n2m * n, pick_down(R, full_rows(g)),
+  // unique bytes per point of the level: reads resd 16, bet 8; writes the
+  // second-half solution 16 on half the lines (8), the residual 16 and the
+  // restricted row 16 per pair (8)
+  LW_LAUNCH("k_mg_down", 0, 56.0 * g.n2m * n, pick_down(R, full_rows(g)),
             dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T,
             test, cyc);
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_up), R = lanes_r(g.n2m);
-  // unique bytes per point of the level: reads residual 16, bet 8, gam 8,
-  // the down-sweep row on half the lines (8) and the coarse row per pair
-  // (8); writes the solution 16
-  LW_LAUNCH("k_mg_up", 0, 64.0 * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
+  // unique bytes per point of the level: reads residual 16, bet 8, the
+  // down-sweep row on half the lines (8) and the coarse row per pair (8);
+  // writes the solution 16
+  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
             dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
 static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
   const int n = g.n1m, T = fused_tile(n, g_tile_top), R = lanes_r(g.n2m);
-  // unique bytes per top point: reads rhs 16, bet 8, gam 8, ac 8, the old pc
-  // on half the lines (8) and for id = 1 the coarse row per pair (8); writes
-  // the new pc 16 and the restricted row per pair (8)
-  const double per_pt = id == 1 ? 80.0 : 72.0;
+  // unique bytes per top point: reads rhs 16, bet 8, the old pc on half the
+  // lines (8) and for id = 1 the coarse row per pair (8); writes the new pc
+  // 16 and the restricted row per pair (8)
+  const double per_pt = id == 1 ? 64.0 : 56.0;
   LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g)),
             dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T,
             oldv, id);
@@ -1608,8 +1609,8 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
   } else {
     launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
     launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
-    // rhs 16, solution 16, ac 8, residual 16, pc 16 per point; coarse row 16 per pair
-    LW_LAUNCH("k_top_rr", 0, 80.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
+    // rhs 16, solution 16, residual 16, pc 16 per point; coarse row 16 per pair
+    LW_LAUNCH("k_top_rr", 0, 72.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
   }
   if (check) LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id, mg.fused);
 }
@@ -1634,8 +1635,8 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int
     }
     launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
     launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, 0);
-    // residual in/out 32, solution 16, ac 8 per fine point; coarse row 8 per fine point
-    LW_LAUNCH("k_lev_rr", 0, 64.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
+    // residual in/out 32, solution 16 per fine point; coarse row 8 per fine point
+    LW_LAUNCH("k_lev_rr", 0, 56.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
   }
   if (mg.lc >= 0) {
     launch_coarse(s, g, mg);

@@ -538,9 +538,28 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
           }
       }
   }
+  // The kernels form ac(i,j,kv) = -(((aw+ae)+as)+an) - ak3(kv) and
+  // gam(i,j+1,kv) = an(j) bet(i,j,kv) as they go, with the additions and the
+  // product the tables above were built from, instead of reading two more
+  // arrays of n3mh planes (ak3(1) = 0 and x - 0 = x). Only bet is stored.
+  std::vector<double> awe(n1md + 2, 0.0);
+  for (int i = 0; i <= n1md + 1; ++i) awe[i] = aw[i] + ae[i];
+  for (int ilev = 0; ilev <= nlev; ++ilev)
+    for (int k = 1; k <= n3mh; ++k)
+      for (int j = 1; j <= n2m; ++j)
+        for (int i = mg.iimg[ilev][1]; i <= mg.iimg[ilev][2]; ++i) {
+          const double a = -((awe[i] + as[j]) + an[j]) - ak3[k];
+          if (memcmp(&a, &AC(i, j, k), sizeof(double)) != 0) {
+            g_err = "Poisson diagonal differs from -(aw+ae+as+an) - ak3";
+            throw LW_ERR_ARG;
+          }
+        }
   mg.aw = h->upload(aw); mg.ae = h->upload(ae); mg.as = h->upload(as); mg.an = h->upload(an);
+  mg.awe = h->upload(awe); mg.ak3 = h->upload(ak3);
   mg.coi1 = h->upload(coi1); mg.coi2 = h->upload(coi2); mg.cor1 = h->upload(cor1); mg.cor2 = h->upload(cor2);
-  mg.ac = h->upload(ac); mg.gam = h->upload(gam); mg.bet = h->upload(bet);
+  mg.bet = h->upload(bet);
+  std::vector<double>().swap(ac);
+  std::vector<double>().swap(gam);
   mg.gg = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
   mg.resd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
   // LW_MG_FUSED=0 runs every zebra half and residual as its own launch
