@@ -962,19 +962,112 @@ static AdiYwKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
 }
 static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * adi_y_ld(g.n2) * sizeof(double)); }
 
-// once per handle: the wave-per-line y sweep needs more than 64 KiB of LDS
-// at 256 rows; 0 = not used (more than 256 rows, or LW_ADI_WAVE=0)
+// x sweep with one wavefront per line (tri_line, lines of 64 lanes, R rows
+// per lane): the line's viscosity and right-hand side are read along i
+// (contiguous) into LDS, each lane takes its R consecutive rows from there,
+// and the solution goes back through LDS to be added to the field along i.
+// Unlike the serial sweep, nothing but the field is written to HBM. LDS
+// index s + s / R spreads a lane's rows over the banks.
+constexpr int kXWL = 4;  // lines (wavefronts) per workgroup
+template <int R>
+__host__ __device__ constexpr int xw_ld() {
+  return 64 * (R + 1) + 2;
+}
+template <int R>
+__device__ inline int xw_p(int s) {
+  return s + s / R;
+}
+
+template <int R, bool FULL, int C>
+__device__ inline void adi_x_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int nj = g.n2m - jlo + 1, nlines = nj * (g.n3m - klo + 1);
+  if (blockIdx.x * kXWL >= nlines) return;  // uniform: the grid covers the component with the most lines
+  const int line = blockIdx.x * kXWL + w;
+  const bool mine = line < nlines;
+  const int ln = mine ? line : nlines - 1;  // a wavefront past the last line reads that line, stores nothing
+  const int N = g.n1m - ilo + 1;
+  const LineCf l = line_cf<0, C>(g, f, sb, cre, cre2);
+  double *fld = C == 0 ? f.u : C == 1 ? f.v : C == 2 ? f.w : f.t;
+  const double *r = f.rhs[C];
+  const long q0 = P(g, 0, jlo + ln % nj, klo + ln / nj) + ilo;  // row t = 1 of the line
+  double *V = lds + (long)w * 2 * xw_ld<R>(), *X = V + xw_ld<R>();
+  // V[s] = vis at row s + 1 + lo (s = 0..N): a_t reads V[t-1], c_t V[t]
+  for (int s = lane; s <= N; s += 64) V[xw_p<R>(s)] = l.vis[q0 + l.lo + s];
+  for (int s = lane; s < N; s += 64) X[xw_p<R>(s)] = r[q0 + s];
+  __syncthreads();
+  double a[R], b[R], c[R], ff[R], x[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int t = 1 + lane * R + u;
+    const int tc = (FULL || t <= N) ? t : N;
+    const int i = ilo + tc - 1;
+    a[u] = l.am[i] * (1. + l.cc * V[xw_p<R>(tc - 1)]);
+    c[u] = l.cm[i] * (1. + l.cc * V[xw_p<R>(tc)]);
+    b[u] = l.bd - a[u] - c[u];
+    ff[u] = l.bd * X[xw_p<R>(tc - 1)];
+  }
+  tri_line<R, FULL, 64>(a, b, c, ff, N, lane, x);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int t = 1 + lane * R + u;
+    if (FULL || t <= N) X[xw_p<R>(t - 1)] = x[u];
+  }
+  __syncthreads();
+  if (mine)
+    for (int s = lane; s < N; s += 64) fld[q0 + s] = X[xw_p<R>(s)] + fld[q0 + s];
+}
+
+template <int R, int C>
+__device__ inline void adi_x_wave_c(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
+  const int ilo = C == 0 ? g.ib : 1;
+  if (g.n1m - ilo + 1 == 64 * R) adi_x_wave<R, true, C>(g, sb, f, cre, cre2, lds);
+  else adi_x_wave<R, false, C>(g, sb, f, cre, cre2, lds);
+}
+
+template <int R, int C0>
+__global__ __launch_bounds__(64 * kXWL) void k_adi_xw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  __shared__ double xl[kXWL * 2 * xw_ld<R>()];
+  if constexpr (C0 == 3) {
+    adi_x_wave_c<R, 3>(g, sb, f, cre, cre2, xl);
+  } else {
+    switch (blockIdx.y) {
+      case 0: adi_x_wave_c<R, 0>(g, sb, f, cre, cre2, xl); break;
+      case 1: adi_x_wave_c<R, 1>(g, sb, f, cre, cre2, xl); break;
+      default: adi_x_wave_c<R, 2>(g, sb, f, cre, cre2, xl); break;
+    }
+  }
+}
+
+template <int C0>
+static AdiYwKernel pick_adi_xw(int n1m) {  // rows per lane on lines of 64 lanes
+  if (n1m <= 64) return k_adi_xw<1, C0>;
+  if (n1m <= 128) return k_adi_xw<2, C0>;
+  if (n1m <= 192) return k_adi_xw<3, C0>;
+  if (n1m <= 256) return k_adi_xw<4, C0>;
+  if (n1m <= 384) return k_adi_xw<6, C0>;
+  if (n1m <= 512) return k_adi_xw<8, C0>;
+  return nullptr;
+}
+
+// once per handle: which sweeps run with whole lines in registers (bit 1:
+// y, bit 2: x; LW_ADI_WAVE sets the mask, default 3). The y kernel needs
+// more than 64 KiB of LDS at 256 rows; lines longer than 256 (y) or 512 (x)
+// rows keep the serial sweeps.
 hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
+  int mask = 3;
+  if (const char *e = getenv("LW_ADI_WAVE")) mask = atoi(e);
   *use_wave = 0;
-  if (const char *e = getenv("LW_ADI_WAVE"))
-    if (atoi(e) == 0) return hipSuccess;
-  if (!pick_adi_yw<0>(g.n2m) || adi_yw_lds(g) > 160 * 1024) return hipSuccess;
+  if ((mask & 2) && pick_adi_xw<0>(g.n1m)) *use_wave |= 2;
+  if (!(mask & 1) || !pick_adi_yw<0>(g.n2m) || adi_yw_lds(g) > 160 * 1024) return hipSuccess;
   hipError_t e = hipFuncSetAttribute((const void *)pick_adi_yw<0>(g.n2m),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)adi_yw_lds(g));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void *)pick_adi_yw<3>(g.n2m), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)adi_yw_lds(g));
-  if (e == hipSuccess) *use_wave = 1;
+  if (e == hipSuccess) *use_wave |= 1;
   return e;
 }
 
@@ -1034,15 +1127,19 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   // in, rhs out; x sweep: also the field in and out (per component)
   if (g.n3m != 1)
     LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, k_adi_z<C0>, grid3(g.n1m, g.n2m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
-  if (g.adi_wave)
+  if (g.adi_wave & 1)
     LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, pick_adi_yw<C0>(g.n2m),
               dim3((unsigned)((g.n1m + kYL - 1) / kYL), (unsigned)g.n3m, (unsigned)nc), dim3(64, kYL / 2), adi_yw_lds(g), s,
               g, sb, f, cre, cre2);
   else
     LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, k_adi_y<C0>, grid3(g.n1m, g.n3m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
   const int xlines = g.n2m * g.n3m;
-  LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_x<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64), 0,
-            s, g, sb, f, cre, cre2);
+  if (g.adi_wave & 2)
+    LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, pick_adi_xw<C0>(g.n1m),
+              dim3((unsigned)((xlines + kXWL - 1) / kXWL), (unsigned)nc), dim3(64, kXWL), 0, s, g, sb, f, cre, cre2);
+  else
+    LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_x<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64),
+              0, s, g, sb, f, cre, cre2);
 }
 
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
