@@ -29,7 +29,7 @@ struct Geo {
   // SGS test-filter weights cf?1(n,-1:1) stored [3*i + d + 1]
   const double *cfx1, *cfy1, *cfz1;
   int ib, jb, kb;  // i_bgpx, j_bgpy, k_bgpz
-  int adi_wave;    // bit 1: y sweep, bit 2: x sweep with whole lines in registers
+  int adi_wave;    // sweeps with whole lines in registers: bit 1 y, bit 2 x, bit 4 z
 };
 
 __host__ __device__ inline long P(const Geo &g, int i, int j, int k) {

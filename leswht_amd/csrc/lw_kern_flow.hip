@@ -951,9 +951,9 @@ __global__ __launch_bounds__(32 * kYL) void k_adi_yw(Geo g, Sub sb, Fld f, doubl
   }
 }
 
-typedef void (*AdiYwKernel)(Geo, Sub, Fld, double, double);
+typedef void (*AdiWaveKernel)(Geo, Sub, Fld, double, double);
 template <int C0>
-static AdiYwKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
+static AdiWaveKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
   if (n2m <= 32) return k_adi_yw<1, C0>;
   if (n2m <= 64) return k_adi_yw<2, C0>;
   if (n2m <= 128) return k_adi_yw<4, C0>;
@@ -961,6 +961,78 @@ static AdiYwKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
   return nullptr;
 }
 static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * adi_y_ld(g.n2) * sizeof(double)); }
+
+// periodic z sweep with one half-wavefront per line (tri_per_line32, N =
+// n3m = 32*R rows): a workgroup stages the viscosity and right-hand side of
+// kZL lines of consecutive i (one row j) through LDS, plane by plane, and the
+// solutions go back to rhs the same way; no per-point scratch goes to HBM.
+constexpr int kZL = 16;
+
+template <int R, int C>
+__device__ inline void adi_z_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+                                  double *tr) {
+  constexpr int N = 32 * R, LD = N + 1;
+  const LineCf l = line_cf<2, C>(g, f, sb, cre, cre2);
+  const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1;
+  const int i0 = ilo + blockIdx.x * kZL, j = jlo + blockIdx.y;
+  if (j > g.n2m) return;  // uniform: the grid covers the widest component
+  const int n1m = g.n1m;
+  const long st = (long)g.NX * g.NY;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  double *r = f.rhs[C];
+  // tv[s] = vis at plane 1 + lo + s (s = 0..N): a_t reads tv[t-1], c_t tv[t]
+  for (int q = tid; q < kZL * (N + 1); q += 32 * kZL) {
+    const int li = q % kZL, s = q / kZL, i = i0 + li;
+    const long p0 = P(g, i <= n1m ? i : n1m, j, 0);
+    tv[li * LD + s] = l.vis[p0 + (long)(1 + l.lo + s) * st];
+    if (s < N) tr[li * LD + s] = r[p0 + (long)(1 + s) * st];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x, ll = lane & 31, li = 2 * threadIdx.y + (lane >> 5);
+  const double *v = tv + li * LD;
+  double *rr = tr + li * LD;
+  double a[R], b[R], c[R], ff[R], x[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int t = 1 + ll * R + u;
+    a[u] = l.am[t] * (1. + l.cc * v[t - 1]);
+    c[u] = l.cm[t] * (1. + l.cc * v[t]);
+    b[u] = l.bd - a[u] - c[u];
+    ff[u] = l.bd * rr[t - 1];
+  }
+  tri_per_line32<R>(a, b, c, ff, lane, x);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < R; ++u) rr[ll * R + u] = x[u];
+  __syncthreads();
+  for (int q = tid; q < kZL * N; q += 32 * kZL) {
+    const int li2 = q % kZL, s = q / kZL, i = i0 + li2;
+    if (i <= n1m) r[P(g, i, j, 0) + (long)(1 + s) * st] = tr[li2 * LD + s];
+  }
+}
+
+template <int R, int C0>
+__global__ __launch_bounds__(32 * kZL) void k_adi_zw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  __shared__ double zl[2 * kZL * (32 * R + 1)];
+  double *tv = zl, *tr = zl + kZL * (32 * R + 1);
+  if constexpr (C0 == 3) {
+    adi_z_wave<R, 3>(g, sb, f, cre, cre2, tv, tr);
+  } else {
+    switch (blockIdx.z) {
+      case 0: adi_z_wave<R, 0>(g, sb, f, cre, cre2, tv, tr); break;
+      case 1: adi_z_wave<R, 1>(g, sb, f, cre, cre2, tv, tr); break;
+      default: adi_z_wave<R, 2>(g, sb, f, cre, cre2, tv, tr); break;
+    }
+  }
+}
+
+template <int C0>
+static AdiWaveKernel pick_adi_zw(int n3m) {  // N = 32*R planes exactly
+  if (n3m == 32) return k_adi_zw<1, C0>;
+  if (n3m == 64) return k_adi_zw<2, C0>;
+  if (n3m == 128) return k_adi_zw<4, C0>;
+  return nullptr;
+}
 
 // x sweep with one wavefront per line (tri_line, lines of 64 lanes, R rows
 // per lane): the line's viscosity and right-hand side are read along i
@@ -1042,7 +1114,7 @@ __global__ __launch_bounds__(64 * kXWL) void k_adi_xw(Geo g, Sub sb, Fld f, doub
 }
 
 template <int C0>
-static AdiYwKernel pick_adi_xw(int n1m) {  // rows per lane on lines of 64 lanes
+static AdiWaveKernel pick_adi_xw(int n1m) {  // rows per lane on lines of 64 lanes
   if (n1m <= 64) return k_adi_xw<1, C0>;
   if (n1m <= 128) return k_adi_xw<2, C0>;
   if (n1m <= 192) return k_adi_xw<3, C0>;
@@ -1053,14 +1125,16 @@ static AdiYwKernel pick_adi_xw(int n1m) {  // rows per lane on lines of 64 lanes
 }
 
 // once per handle: which sweeps run with whole lines in registers (bit 1:
-// y, bit 2: x; LW_ADI_WAVE sets the mask, default 3). The y kernel needs
-// more than 64 KiB of LDS at 256 rows; lines longer than 256 (y) or 512 (x)
-// rows keep the serial sweeps.
+// y, bit 2: x, bit 4: z; LW_ADI_WAVE sets the mask, default 7). The y kernel
+// needs more than 64 KiB of LDS at 256 rows; lines longer than 256 (y) or
+// 512 (x) rows, and z lines of other than 32, 64 or 128 planes, keep the
+// serial sweeps.
 hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
-  int mask = 3;
+  int mask = 7;
   if (const char *e = getenv("LW_ADI_WAVE")) mask = atoi(e);
   *use_wave = 0;
   if ((mask & 2) && pick_adi_xw<0>(g.n1m)) *use_wave |= 2;
+  if ((mask & 4) && pick_adi_zw<0>(g.n3m)) *use_wave |= 4;
   if (!(mask & 1) || !pick_adi_yw<0>(g.n2m) || adi_yw_lds(g) > 160 * 1024) return hipSuccess;
   hipError_t e = hipFuncSetAttribute((const void *)pick_adi_yw<0>(g.n2m),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)adi_yw_lds(g));
@@ -1125,7 +1199,11 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   static const char *nx = C0 == 3 ? "k_adi_x<t>" : "k_adi_x<uvw>";
   // the grids cover the widest component; z and y sweeps: viscosity and rhs
   // in, rhs out; x sweep: also the field in and out (per component)
-  if (g.n3m != 1)
+  if (g.adi_wave & 4)
+    LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, pick_adi_zw<C0>(g.n3m),
+              dim3((unsigned)((g.n1m + kZL - 1) / kZL), (unsigned)g.n2m, (unsigned)nc), dim3(64, kZL / 2), 0, s, g, sb,
+              f, cre, cre2);
+  else if (g.n3m != 1)
     LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, k_adi_z<C0>, grid3(g.n1m, g.n2m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
   if (g.adi_wave & 1)
     LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, pick_adi_yw<C0>(g.n2m),

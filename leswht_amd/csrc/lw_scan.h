@@ -205,4 +205,140 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   }
 }
 
+// Periodic tridiagonal solve (trdiag3p, slv_mmtm.f90:1572-1641) of
+// a_t x_{t-1} + b_t x_t + c_t x_{t+1} = f_t, t = 1..N, x_0 = x_N and
+// x_{N+1} = x_1, on lines of 32 lanes (each half of the wavefront solves its
+// own line) with N = 32*R: lane l owns rows 1 + l*R .. l*R + R. The rows
+// follow the reference's recurrences: pn_t = 1/(b_t + a_t q_{t-1}),
+// q_t = -c_t pn_t, s_t = -a_t s_{t-1} pn_t, f_t = (f_t - a_t f_{t-1}) pn_t;
+// then from s_N = 1, qe_N = 0 down: s_t += q_t s_{t+1},
+// qe_t = f_t + q_t qe_{t+1}; x_N from rows 1, N-1 and N, and
+// x_t = x_N s_t + qe_t. Across lanes: the pivots from tri_line's scan of 2x2
+// maps, s and f from one affine scan (s is its product part, multiplier
+// -a_t pn_t), the backward pair from one scan with the shared multiplier q_t.
+template <int R>
+__device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R], const double (&c)[R],
+                                      const double (&f)[R], int lane, double (&x)[R]) {
+  constexpr int N = 32 * R;
+  const int ll = lane & 31, hb = lane & 32;
+  const int t0 = 1 + ll * R;
+  const double cl = dpp_d<kWshr1>(c[R - 1], 0.0);  // c of the row before this lane's first
+  M22 L{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double cp = r == 0 ? cl : c[r - 1];
+    const double q = t0 + r == 1 ? 0.0 : -(a[r] * cp);
+    L = M22{b[r] * L.p + q * L.r, b[r] * L.q + q * L.s, L.p, L.q};
+  }
+  L = m22_norm(L);
+  L = m22_norm(m22_mul(L, m22_dpp<kShr1>(L)));
+  L = m22_norm(m22_mul(L, m22_dpp<kShr2>(L)));
+  L = m22_norm(m22_mul(L, m22_dpp<kShr4>(L)));
+  L = m22_norm(m22_mul(L, m22_dpp<kShr8>(L)));
+  L = m22_norm(m22_mul(L, m22_dpp<kBcast15, 0xA>(L)));
+  const M22 E = m22_dpp<kWshr1>(L);
+  // q of the row before this lane's first (unused on a line's first lane)
+  double qprev = ll == 0 ? 0.0 : -(cl * (1.0 / (E.p / E.r)));
+  double pn[R], qv[R], mv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    pn[r] = t0 + r == 1 ? 1.0 / b[r] : 1.0 / (b[r] + a[r] * qprev);
+    qv[r] = -c[r] * pn[r];
+    mv[r] = -a[r] * pn[r];
+    qprev = qv[r];
+  }
+  // forward pair: y_t = m_t y_{t-1} + pn_t f_t, s_t = m_t s_{t-1} (s_0 = 1)
+  double M = 1.0, Cc = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    Cc = mv[r] * Cc + pn[r] * f[r];
+    M = mv[r] * M;
+  }
+#define PFWD(CT, ROWS)                           \
+  {                                              \
+    const double Mp = dpp_d<CT, ROWS>(M, 1.0);   \
+    const double Cp = dpp_d<CT, ROWS>(Cc, 0.0);  \
+    Cc = M * Cp + Cc;                            \
+    M = M * Mp;                                  \
+  }
+  PFWD(kShr1, 0xF)
+  PFWD(kShr2, 0xF)
+  PFWD(kShr4, 0xF)
+  PFWD(kShr8, 0xF)
+  PFWD(kBcast15, 0xA)
+#undef PFWD
+  double sp = dpp_d<kWshr1>(M, 1.0), yp = dpp_d<kWshr1>(Cc, 0.0);
+  if (ll == 0) {  // s_0 = 1, f_0 = 0: row 1 becomes s_1 = -a_1 pn_1, f_1 = f_1 pn_1
+    sp = 1.0;
+    yp = 0.0;
+  }
+  double sv[R], yv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    sv[r] = -(a[r] * sp) * pn[r];
+    yv[r] = (f[r] - a[r] * yp) * pn[r];
+    sp = sv[r];
+    yp = yv[r];
+  }
+  // backward pair: X_t = q_t X_{t+1} + (s_t | f_t) for t < N, X_N = (1 | 0)
+  double Nn = 1.0, Ds = 0.0, Dq = 0.0;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    const bool last = t0 + r == N;
+    const double nj = last ? 0.0 : qv[r];
+    Ds = nj * Ds + (last ? 1.0 : sv[r]);
+    Dq = nj * Dq + (last ? 0.0 : yv[r]);
+    Nn = nj * Nn;
+  }
+#define PBWD(CT)                                \
+  {                                             \
+    const double Np = dpp_d<CT>(Nn, 1.0);       \
+    const double Sp = dpp_d<CT>(Ds, 0.0);       \
+    const double Qp = dpp_d<CT>(Dq, 0.0);       \
+    Ds = Nn * Sp + Ds;                          \
+    Dq = Nn * Qp + Dq;                          \
+    Nn = Nn * Np;                               \
+  }
+  PBWD(kShl1)
+  PBWD(kShl2)
+  PBWD(kShl4)
+  PBWD(kShl8)
+#undef PBWD
+  {
+    const double s1 = rdlane(Ds, 16), s3 = rdlane(Ds, 48), q1 = rdlane(Dq, 16), q3 = rdlane(Dq, 48);
+    const int row = lane >> 4;
+    const double ns = row == 0 ? s1 : row == 2 ? s3 : 0.0, nq = row == 0 ? q1 : row == 2 ? q3 : 0.0;
+    Ds = Nn * ns + Ds;
+    Dq = Nn * nq + Dq;
+  }
+  double xs = dpp_d<kWshl1>(Ds, 0.0), xq = dpp_d<kWshl1>(Dq, 0.0);
+  if (ll == 31) {  // holds row N, which does not read its successor
+    xs = 0.0;
+    xq = 0.0;
+  }
+  double sf[R], qf[R];
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    if (t0 + r == N) {
+      xs = 1.0;
+      xq = 0.0;
+    } else {
+      xs = sv[r] + qv[r] * xs;
+      xq = yv[r] + qv[r] * xq;
+    }
+    sf[r] = xs;
+    qf[r] = xq;
+  }
+  // x_N = (f_N - c_N qe_1 - a_N qe_{N-1}) / (c_N s_1 + a_N s_{N-1} + b_N)
+  const int l1 = hb, lN = hb | 31;
+  const double s1 = __shfl(sf[0], l1, 64), qe1 = __shfl(qf[0], l1, 64);
+  const double sN1 = R >= 2 ? __shfl(sf[R >= 2 ? R - 2 : 0], lN, 64) : __shfl(sf[0], lN - 1, 64);
+  const double qN1 = R >= 2 ? __shfl(qf[R >= 2 ? R - 2 : 0], lN, 64) : __shfl(qf[0], lN - 1, 64);
+  const double aN = __shfl(a[R - 1], lN, 64), bN = __shfl(b[R - 1], lN, 64), cN = __shfl(c[R - 1], lN, 64);
+  const double fN = __shfl(f[R - 1], lN, 64);
+  const double xN = (fN - cN * qe1 - aN * qN1) / (cN * s1 + aN * sN1 + bN);
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = t0 + r == N ? xN : xN * sf[r] + qf[r];
+}
+
 }  // namespace lw

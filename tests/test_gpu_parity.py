@@ -293,21 +293,21 @@ def test_gpu_fused_levels_equal_separate_launches(dims, nlev, coarse, monkeypatc
 
 
 @pytest.mark.parametrize("dims,nlev", [((32, 256, 8), 2), ((48, 96, 8), 2), ((32, 32, 8), 2), ((384, 32, 8), 3),
-                                       ((128, 64, 8), 2)])
+                                       ((128, 64, 8), 2), ((32, 32, 32), 2), ((48, 32, 64), 2)])
 def test_gpu_wave_sweeps_match_serial_sweeps(dims, nlev, monkeypatch):
-    """The y sweep with one half-wavefront per line and the x sweep with one
-    wavefront per line (pivots from a scan of 2x2 maps, substitutions from
-    affine scans) against the serial Thomas sweeps, which the oracle tests
-    above pin. The two round differently, so the bar is relative L2 <= 1e-12
-    after two steps with the same multigrid cycle counts. y: 8 rows per lane
-    (256 = 32*8 for u and w, 255 for v on the wall-bounded y axis), 3 of 4
-    rows used (96), 1 row (32); x: 6 rows per lane (384 for v, w, T; 383 for
-    u, whose first point is the inflow boundary), 2 rows (128), 1 row (32,
-    48)."""
+    """The y and periodic z sweeps with one half-wavefront per line and the x
+    sweep with one wavefront per line (pivots from a scan of 2x2 maps,
+    substitutions from affine scans) against the serial sweeps, which the
+    oracle tests above pin. The two round differently, so the bar is relative
+    L2 <= 1e-12 after two steps with the same multigrid cycle counts. y: 8
+    rows per lane (256 = 32*8 for u and w, 255 for v on the wall-bounded y
+    axis), 3 of 4 rows used (96), 1 row (32); x: 6 rows per lane (384 for v,
+    w, T; 383 for u, whose first point is the inflow boundary), 2 rows (128),
+    1 row (32, 48); z: 1 and 2 rows per lane (32 and 64 planes)."""
     view = _view(dims, "les", 2, nlev)
     print("wave sweeps", dims, flush=True)
     out = []
-    for flag in ("0", "3"):
+    for flag in ("0", "7"):
         monkeypatch.setenv("LW_ADI_WAVE", flag)
         g = Solver.from_case(view)
         st = g.step(2)
