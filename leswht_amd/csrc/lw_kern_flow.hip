@@ -966,7 +966,7 @@ static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * ad
 // n3m = 32*R rows): a workgroup stages the viscosity and right-hand side of
 // kZL lines of consecutive i (one row j) through LDS, plane by plane, and the
 // solutions go back to rhs the same way; no per-point scratch goes to HBM.
-constexpr int kZL = 16;
+constexpr int kZL = 32;
 
 template <int R, int C>
 __device__ inline void adi_z_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
