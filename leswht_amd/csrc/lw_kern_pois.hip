@@ -24,6 +24,10 @@ namespace lw {
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 rmul(double a, double2 b) { return make_double2(a * b.x, a * b.y); }
+// a*b + c with one rounding per component: the scans' combination steps,
+// which have no counterpart in the reference's serial sweep, use it to
+// halve the dependent latency of each step
+__device__ inline double2 cfma(double a, double2 b, double2 c) { return make_double2(fma(a, b.x, c.x), fma(a, b.y, c.y)); }
 // diagonal of the wavenumber-kv operator, ac(i,j,kv) (pmat and coefmg,
 // poiss_z_ft.f90:78-88, 696-703): the same additions in the same order as the
 // stored table, so the same bits (ak3(1) = 0 and x - 0 = x)
@@ -37,21 +41,21 @@ __device__ inline double ac_of(double awe, double as, double an, double ak) { re
   {                                                                              \
     const double Mp = dpp_d<CT>(M, 1.0);                                         \
     const double2 Cp = make_double2(dpp_d<CT>(Cc.x, 0.0), dpp_d<CT>(Cc.y, 0.0)); \
-    Cc = cadd(rmul(M, Cp), Cc);                                                  \
+    Cc = cfma(M, Cp, Cc);                                                        \
     M = M * Mp;                                                                  \
   }
 // carry from another row into the rows of mask RM: C = M*C(source) + C
 #define CROSS_STEP(CT, RM)                                                                   \
   {                                                                                          \
     const double2 Cp = make_double2(dpp_d<CT, RM>(Cc.x, 0.0), dpp_d<CT, RM>(Cc.y, 0.0));   \
-    Cc = cadd(rmul(M, Cp), Cc);                                                              \
+    Cc = cfma(M, Cp, Cc);                                                                    \
   }
 // suffix step of the back substitution's maps (N, D)
 #define BWD_STEP(CT)                                                           \
   {                                                                            \
     const double Np = dpp_d<CT>(Nn, 1.0);                                      \
     const double2 Dp = make_double2(dpp_d<CT>(D.x, 0.0), dpp_d<CT>(D.y, 0.0)); \
-    D = cadd(rmul(Nn, Dp), D);                                                 \
+    D = cfma(Nn, Dp, D);                                                       \
     Nn = Nn * Np;                                                              \
   }
 
@@ -376,7 +380,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
     const int j = j0 + r;
     if ((FULL || j <= n2m)) {
       const double m = j == 1 ? 0.0 : -(asj[r] * bt[r]);
-      Cc = cadd(rmul(m, Cc), rmul(bt[r], rr[r]));
+      Cc = cfma(m, Cc, rmul(bt[r], rr[r]));
       M = m * M;
     }
   }
@@ -398,7 +402,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
   for (int r = 0; r < R; ++r) {
     const int j = j0 + r;
     if ((FULL || j <= n2m)) {
-      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], csub(rr[r], rmul(asj[r], yp)));
+      yp = j == 1 ? rmul(bt[r], rr[r]) : rmul(bt[r], cfma(-asj[r], yp, rr[r]));
       y[r] = yp;
     } else {
       y[r] = make_double2(0.0, 0.0);
@@ -412,7 +416,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
     const int j = j0 + r;
     if ((FULL || j <= n2m)) {
       const double nj = j == n2m ? 0.0 : -gm[r];
-      D = cadd(rmul(nj, D), y[r]);
+      D = cfma(nj, D, y[r]);
       Nn = nj * Nn;
     }
   }
@@ -426,18 +430,18 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
     const double2 d1 = make_double2(rdlane(D.x, 16), rdlane(D.y, 16));
     const double2 d2 = make_double2(rdlane(D.x, 32), rdlane(D.y, 32));
     const double2 d3 = make_double2(rdlane(D.x, 48), rdlane(D.y, 48));
-    const double2 s2 = cadd(rmul(n2, d3), d2);  // start of row 2
-    const double2 s1 = cadd(rmul(n1, s2), d1);  // start of row 1
+    const double2 s2 = cfma(n2, d3, d2);  // start of row 2
+    const double2 s1 = cfma(n1, s2, d1);  // start of row 1
     const int row = lane >> 4;
     const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
-    D = cadd(rmul(Nn, next), D);  // inclusive suffix over the wavefront
+    D = cfma(Nn, next, D);  // inclusive suffix over the wavefront
     xn = make_double2(dpp_d<kWshl1>(D.x, 0.0), dpp_d<kWshl1>(D.y, 0.0));
   }
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
     const int j = j0 + r;
     if ((FULL || j <= n2m)) {
-      xn = j == n2m ? y[r] : csub(y[r], rmul(gm[r], xn));
+      xn = j == n2m ? y[r] : cfma(-gm[r], xn, y[r]);
       x[r] = xn;
     } else {
       x[r] = make_double2(0.0, 0.0);

@@ -1,5 +1,8 @@
-// Wavefront-wide scans for line solves on gfx950: DPP moves of doubles and a
-// tridiagonal solve with one line per wavefront.
+// Wavefront-wide scans for line solves on gfx950: DPP moves of doubles and
+// tridiagonal solves with one line per wavefront or half-wavefront. The
+// scans reassociate the serial recurrences anyway, so their arithmetic uses
+// fused multiply-adds (one rounding, half the dependent latency of a
+// multiply followed by an add).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -29,7 +32,7 @@ struct M22 {
   double p, q, r, s;
 };
 __device__ inline M22 m22_mul(const M22 &A, const M22 &B) {  // A*B
-  return M22{A.p * B.p + A.q * B.r, A.p * B.q + A.q * B.s, A.r * B.p + A.s * B.r, A.r * B.q + A.s * B.s};
+  return M22{fma(A.p, B.p, A.q * B.r), fma(A.p, B.q, A.q * B.s), fma(A.r, B.p, A.s * B.r), fma(A.r, B.q, A.s * B.s)};
 }
 // scale by the power of two that brings the largest entry near 1 (exact)
 __device__ inline M22 m22_norm(const M22 &A) {
@@ -71,7 +74,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
     if (ok(r)) {
       const double cp = r == 0 ? cl : c[r - 1];
       const double q = t0 + r == 1 ? 0.0 : -(a[r] * cp);
-      L = M22{b[r] * L.p + q * L.r, b[r] * L.q + q * L.s, L.p, L.q};
+      L = M22{fma(b[r], L.p, q * L.r), fma(b[r], L.q, q * L.s), L.p, L.q};
     }
   }
   L = m22_norm(L);
@@ -105,7 +108,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
         bet = 1.0 / b[r];
       } else {
         const double gam = cprev * bet;
-        bet = 1.0 / (b[r] - a[r] * gam);
+        bet = 1.0 / fma(-a[r], gam, b[r]);
       }
       bt[r] = bet;
       gm[r] = t == N ? 0.0 : c[r] * bet;  // gam_{t+1}
@@ -121,7 +124,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   for (int r = 0; r < R; ++r) {
     if (ok(r)) {
       const double m = t0 + r == 1 ? 0.0 : -(a[r] * bt[r]);
-      Cc = m * Cc + bt[r] * f[r];
+      Cc = fma(m, Cc, bt[r] * f[r]);
       M = m * M;
     }
   }
@@ -129,7 +132,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   {                                     \
     const double Mp = dpp_d<CT>(M, 1.0); \
     const double Cp = dpp_d<CT>(Cc, 0.0); \
-    Cc = M * Cp + Cc;                   \
+    Cc = fma(M, Cp, Cc);                \
     M = M * Mp;                         \
   }
   FWD_R(kShr1)
@@ -138,11 +141,11 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   FWD_R(kShr8)
 #undef FWD_R
   if (LANES == 64) {
-    Cc = M * dpp_d<kBcast15, 0x2>(Cc, 0.0) + Cc;
-    Cc = M * dpp_d<kBcast31, 0x4>(Cc, 0.0) + Cc;
-    Cc = M * dpp_d<kBcast15, 0x8>(Cc, 0.0) + Cc;
+    Cc = fma(M, dpp_d<kBcast15, 0x2>(Cc, 0.0), Cc);
+    Cc = fma(M, dpp_d<kBcast31, 0x4>(Cc, 0.0), Cc);
+    Cc = fma(M, dpp_d<kBcast15, 0x8>(Cc, 0.0), Cc);
   } else {
-    Cc = M * dpp_d<kBcast15, 0xA>(Cc, 0.0) + Cc;
+    Cc = fma(M, dpp_d<kBcast15, 0xA>(Cc, 0.0), Cc);
   }
   double yp = dpp_d<kWshr1>(Cc, 0.0);
   if (LANES == 32 && ll == 0) yp = 0.0;
@@ -150,7 +153,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (ok(r)) {
-      yp = t0 + r == 1 ? f[r] * bt[r] : (f[r] - a[r] * yp) * bt[r];
+      yp = t0 + r == 1 ? f[r] * bt[r] : fma(-a[r], yp, f[r]) * bt[r];
       y[r] = yp;
     } else {
       y[r] = 0.0;
@@ -162,7 +165,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   for (int r = R - 1; r >= 0; --r) {
     if (ok(r)) {
       const double nj = t0 + r == N ? 0.0 : -gm[r];
-      D = nj * D + y[r];
+      D = fma(nj, D, y[r]);
       Nn = nj * Nn;
     }
   }
@@ -170,7 +173,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   {                                      \
     const double Np = dpp_d<CT>(Nn, 1.0); \
     const double Dp = dpp_d<CT>(D, 0.0);  \
-    D = Nn * Dp + D;                     \
+    D = fma(Nn, Dp, D);                  \
     Nn = Nn * Np;                        \
   }
   BWD_R(kShl1)
@@ -181,23 +184,23 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   if (LANES == 64) {
     const double n1 = rdlane(Nn, 16), n2 = rdlane(Nn, 32);
     const double d1 = rdlane(D, 16), d2 = rdlane(D, 32), d3 = rdlane(D, 48);
-    const double s2 = n2 * d3 + d2;  // start of row 2
-    const double s1 = n1 * s2 + d1;  // start of row 1
+    const double s2 = fma(n2, d3, d2);  // start of row 2
+    const double s1 = fma(n1, s2, d1);  // start of row 1
     const int row = lane >> 4;
     const double next = row == 3 ? 0.0 : row == 2 ? d3 : row == 1 ? s2 : s1;
-    D = Nn * next + D;
+    D = fma(Nn, next, D);
   } else {
     const double d1 = rdlane(D, 16), d3 = rdlane(D, 48);
     const int row = lane >> 4;
     const double next = row == 0 ? d1 : row == 2 ? d3 : 0.0;
-    D = Nn * next + D;
+    D = fma(Nn, next, D);
   }
   double xn = dpp_d<kWshl1>(D, 0.0);
   if (LANES == 32 && ll == 31) xn = 0.0;
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
     if (ok(r)) {
-      xn = t0 + r == N ? y[r] : y[r] - gm[r] * xn;
+      xn = t0 + r == N ? y[r] : fma(-gm[r], xn, y[r]);
       x[r] = xn;
     } else {
       x[r] = 0.0;
@@ -228,7 +231,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   for (int r = 0; r < R; ++r) {
     const double cp = r == 0 ? cl : c[r - 1];
     const double q = t0 + r == 1 ? 0.0 : -(a[r] * cp);
-    L = M22{b[r] * L.p + q * L.r, b[r] * L.q + q * L.s, L.p, L.q};
+    L = M22{fma(b[r], L.p, q * L.r), fma(b[r], L.q, q * L.s), L.p, L.q};
   }
   L = m22_norm(L);
   L = m22_norm(m22_mul(L, m22_dpp<kShr1>(L)));
@@ -242,7 +245,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   double pn[R], qv[R], mv[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    pn[r] = t0 + r == 1 ? 1.0 / b[r] : 1.0 / (b[r] + a[r] * qprev);
+    pn[r] = t0 + r == 1 ? 1.0 / b[r] : 1.0 / fma(a[r], qprev, b[r]);
     qv[r] = -c[r] * pn[r];
     mv[r] = -a[r] * pn[r];
     qprev = qv[r];
@@ -251,14 +254,14 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   double M = 1.0, Cc = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    Cc = mv[r] * Cc + pn[r] * f[r];
+    Cc = fma(mv[r], Cc, pn[r] * f[r]);
     M = mv[r] * M;
   }
 #define PFWD(CT, ROWS)                           \
   {                                              \
     const double Mp = dpp_d<CT, ROWS>(M, 1.0);   \
     const double Cp = dpp_d<CT, ROWS>(Cc, 0.0);  \
-    Cc = M * Cp + Cc;                            \
+    Cc = fma(M, Cp, Cc);                         \
     M = M * Mp;                                  \
   }
   PFWD(kShr1, 0xF)
@@ -276,7 +279,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     sv[r] = -(a[r] * sp) * pn[r];
-    yv[r] = (f[r] - a[r] * yp) * pn[r];
+    yv[r] = fma(-a[r], yp, f[r]) * pn[r];
     sp = sv[r];
     yp = yv[r];
   }
@@ -286,8 +289,8 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   for (int r = R - 1; r >= 0; --r) {
     const bool last = t0 + r == N;
     const double nj = last ? 0.0 : qv[r];
-    Ds = nj * Ds + (last ? 1.0 : sv[r]);
-    Dq = nj * Dq + (last ? 0.0 : yv[r]);
+    Ds = fma(nj, Ds, last ? 1.0 : sv[r]);
+    Dq = fma(nj, Dq, last ? 0.0 : yv[r]);
     Nn = nj * Nn;
   }
 #define PBWD(CT)                                \
@@ -295,8 +298,8 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
     const double Np = dpp_d<CT>(Nn, 1.0);       \
     const double Sp = dpp_d<CT>(Ds, 0.0);       \
     const double Qp = dpp_d<CT>(Dq, 0.0);       \
-    Ds = Nn * Sp + Ds;                          \
-    Dq = Nn * Qp + Dq;                          \
+    Ds = fma(Nn, Sp, Ds);                       \
+    Dq = fma(Nn, Qp, Dq);                       \
     Nn = Nn * Np;                               \
   }
   PBWD(kShl1)
@@ -308,8 +311,8 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
     const double s1 = rdlane(Ds, 16), s3 = rdlane(Ds, 48), q1 = rdlane(Dq, 16), q3 = rdlane(Dq, 48);
     const int row = lane >> 4;
     const double ns = row == 0 ? s1 : row == 2 ? s3 : 0.0, nq = row == 0 ? q1 : row == 2 ? q3 : 0.0;
-    Ds = Nn * ns + Ds;
-    Dq = Nn * nq + Dq;
+    Ds = fma(Nn, ns, Ds);
+    Dq = fma(Nn, nq, Dq);
   }
   double xs = dpp_d<kWshl1>(Ds, 0.0), xq = dpp_d<kWshl1>(Dq, 0.0);
   if (ll == 31) {  // holds row N, which does not read its successor
@@ -323,8 +326,8 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
       xs = 1.0;
       xq = 0.0;
     } else {
-      xs = sv[r] + qv[r] * xs;
-      xq = yv[r] + qv[r] * xq;
+      xs = fma(qv[r], xs, sv[r]);
+      xq = fma(qv[r], xq, yv[r]);
     }
     sf[r] = xs;
     qf[r] = xq;
@@ -338,7 +341,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   const double fN = __shfl(f[R - 1], lN, 64);
   const double xN = (fN - cN * qe1 - aN * qN1) / (cN * s1 + aN * sN1 + bN);
 #pragma unroll
-  for (int r = 0; r < R; ++r) x[r] = t0 + r == N ? xN : xN * sf[r] + qf[r];
+  for (int r = 0; r < R; ++r) x[r] = t0 + r == N ? xN : fma(xN, sf[r], qf[r]);
 }
 
 }  // namespace lw
