@@ -728,7 +728,13 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   const int kv = 1 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
-  const int lane = threadIdx.x, w = threadIdx.y;
+  // wavefront y takes level-0 line w (and level-1 lines 2w, 2w+1): the even
+  // lines on wavefronts 0.., the odd ones after them, so that the level-0
+  // half-sweeps, which run on every other line, spread their wavefronts
+  // over the four SIMDs instead of two (consecutive wavefronts of a
+  // workgroup go to consecutive SIMDs)
+  const int half = (blockDim.y + 1) / 2;
+  const int lane = threadIdx.x, w = threadIdx.y < half ? 2 * threadIdx.y : 2 * (threadIdx.y - half) + 1;
   const int n2m = FULL ? 64 * R : g.n2m;
   const int b1 = mg.iimg[1][1], n1 = mg.iimg[1][2] - b1 + 1;
   const int b0 = mg.iimg[0][1], n0 = mg.iimg[0][2] - b0 + 1;
