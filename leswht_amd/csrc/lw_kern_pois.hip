@@ -1424,20 +1424,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   const bool ook = w <= T / 2 && o >= 0;
   double2 sa[R], sb[R], x[R];
   LineLU<R> co;
-  double acA[R], acB[R], anj[R];
   if (ook) {
     load_rows<R, FULL>(ROW(mg.rhs + ppl, 1 + o), j0, n2m, sb);
     load_lu<R, FULL>(mg, pl, 1 + o, j0, n2m, co);
-  }
-  if (pair) {
-    const double awa = mg.awe[1 + e], awb = mg.awe[1 + o], ak = mg.ak3[kv];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      anj[r] = mg.an[jc];
-      acA[r] = ac_of(awa, asj[r], anj[r], ak);
-      acB[r] = ac_of(awb, asj[r], anj[r], ak);
-    }
   }
   if (eok) {
     top_first<R, FULL>(g, mg, pl, ppl, pcr, e, oldv, id, j0, n2m, lane, asj, sa, x);
@@ -1489,6 +1478,16 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
     const int qa = 2 * w, ia = 1 + e, ib = ia + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
     const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
     const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    double acA[R], acB[R], anj[R];
+    {
+      const double wea = mg.awe[ia], web = mg.awe[ib], ak = mg.ak3[kv];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
+        acA[r] = ac_of(wea, asj[r], anj[r], ak);
+        acB[r] = ac_of(web, asj[r], anj[r], ak);
+      }
+    }
     double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
