@@ -734,7 +734,8 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // over the four SIMDs instead of two (consecutive wavefronts of a
   // workgroup go to consecutive SIMDs)
   const int half = (blockDim.y + 1) / 2;
-  const int lane = threadIdx.x, w = threadIdx.y < half ? 2 * threadIdx.y : 2 * (threadIdx.y - half) + 1;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);  // uniform: scalar loads of line coefficients
+  const int lane = threadIdx.x, w = wy < half ? 2 * wy : 2 * (wy - half) + 1;
   const int n2m = FULL ? 64 * R : g.n2m;
   const int b1 = mg.iimg[1][1], n1 = mg.iimg[1][2] - b1 + 1;
   const int b0 = mg.iimg[0][1], n0 = mg.iimg[0][2] - b0 + 1;
@@ -1178,7 +1179,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG m
     }
     if (dn) return;
   }
-  const int lane = threadIdx.x, w = threadIdx.y;
+  // the wavefront index as a uniform value: line coefficients come in by
+  // scalar loads and row addresses stay in scalar registers
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
   const int f0 = blockIdx.x * T;
@@ -1289,7 +1292,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
-  const int lane = threadIdx.x, w = threadIdx.y;
+  // the wavefront index as a uniform value: line coefficients come in by
+  // scalar loads and row addresses stay in scalar registers
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1;
   const int cb = mg.iimg[ilev - 1][1], ce = mg.iimg[ilev - 1][2];
@@ -1406,7 +1411,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   __shared__ double2 tts;
   const int kv = 1 + blockIdx.y;
   if (mg.done[kv - 1]) return;
-  const int lane = threadIdx.x, w = threadIdx.y, W = blockDim.y;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y), W = blockDim.y;
   const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
   const int f0 = blockIdx.x * T;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
