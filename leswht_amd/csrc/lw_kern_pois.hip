@@ -814,7 +814,6 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(la, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
         const double2 gmv = la > 0 ? L1[slot(la - 1, r)] : z2;
         const double2 gpv = L1[slot(lb, r)];
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
@@ -840,7 +839,6 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(lb, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
         const double2 gmv = L1[slot(la, r)];
         const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
@@ -1160,7 +1158,7 @@ static int g_tile_down = 4, g_tile_up = 4, g_tile_top = 8;
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
                                                                       int cyc) {
   extern __shared__ double2 lds[];
   const int kv = 1 + blockIdx.y;
@@ -1201,22 +1199,19 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG m
   const int o = f0 - 1 + 2 * w;
   const bool ook = w <= T / 2 && o >= 0;
   const bool pair = w >= 1 && w <= T / 2;
+  // an(j) for the residual stage, through LDS (row j at j-1; rows past n2m
+  // read n2m), and the stages' line coefficients, all requested up front
+  double *san = (double *)(lds + (T + 3) * R * 64);
+  for (int q = lane + 64 * w; q < 64 * R; q += 64 * (int)blockDim.y) san[q] = mg.an[q + 1 <= n2m ? q + 1 : n2m];
+  const int iaq = pair ? bL + e : bL, ibq = ook ? bL + o : bL, icq = pair ? bC + f0 / 2 + w - 1 : bC;
+  const double aw2 = mg.aw[ibq], ae2 = mg.ae[ibq];
+  const double awa_ = mg.aw[iaq], aea_ = mg.ae[iaq], wea = mg.awe[iaq], web = mg.awe[ibq], ak = mg.ak3[kv];
+  const double c1 = mg.cor1[icq], c2 = mg.cor2[icq];
   double2 so[R], se[R];
   LineLU<R> co;
-  double acA[R], acB[R], anj[R];
   if (ook) {
     load_rows<R, FULL>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
-  }
-  if (pair) {
-    const double awa = mg.awe[bL + e], awb = mg.awe[bL + o], ak = mg.ak3[kv];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      anj[r] = mg.an[jc];
-      acA[r] = ac_of(awa, asj[r], anj[r], ak);
-      acB[r] = ac_of(awb, asj[r], anj[r], ak);
-    }
   }
   // first half (oldv 0): the right-hand side is resd itself
   if (eok) {
@@ -1236,10 +1231,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG m
 
   // second half: resd - aw*g(i-1) - ae*g(i+1)
   if (ook) {
-    const double aw = mg.aw[bL + o], ae = mg.ae[bL + o];
     double2 rr[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(2 * w, r))), rmul(ae, TILE_AT(2 * w + 2, r)));
+    for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw2, TILE_AT(2 * w, r))), rmul(ae2, TILE_AT(2 * w + 2, r)));
     tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
     if (w >= 1) store_rows<R, FULL>(ROW(mg.gd + pl, bL + o), j0, n2m, x);
   } else {
@@ -1257,8 +1251,14 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_mg_down(Geo g, MG m
   // still read resd rows of this level as their halo lines' right-hand sides.
   if (pair) {
     const int qa = 2 * w, ia = bL + e, ib = ia + 1, ic = bC + f0 / 2 + w - 1;
-    const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
-    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    const double awa = awa_, aea = aea_, awb = aw2, aeb = ae2;
+    double acA[R], acB[R], anj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      anj[r] = san[j0 - 1 + r];
+      acA[r] = ac_of(wea, asj[r], anj[r], ak);
+      acB[r] = ac_of(web, asj[r], anj[r], ak);
+    }
     double2 *ra = ROW(mg.rsu + pl, ia), *rb = ROW(mg.rsu + pl, ib), *rc = ROW(mg.resd + pl, ic);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -1427,6 +1427,15 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   const int e = f0 - 2 + 2 * w, o = e + 1;
   const bool eok = w <= T / 2 + 1 && e >= 0 && e < n1m;
   const bool ook = w <= T / 2 && o >= 0;
+  // an(j) for the residual stage, through LDS (row j at j-1; rows past n2m
+  // read n2m), and the stages' line coefficients, all requested up front
+  double *san = (double *)(lds + (T + 3) * R * 64);
+  for (int q = lane + 64 * w; q < 64 * R; q += 64 * W) san[q] = mg.an[q + 1 <= n2m ? q + 1 : n2m];
+  const int ia = 1 + e, ib = ia + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
+  const int icq = pair ? ic : mg.iimg[mg.nlev - 1][1], iaq = pair ? ia : 1, ibq = ook ? ib : 1;
+  const double aw2 = mg.aw[ibq], ae2 = mg.ae[ibq];
+  const double awa = mg.aw[iaq], aea = mg.ae[iaq], wea = mg.awe[iaq], web = mg.awe[ibq], ak = mg.ak3[kv];
+  const double c1 = mg.cor1[icq], c2 = mg.cor2[icq];
   double2 sa[R], sb[R], x[R];
   LineLU<R> co;
   if (ook) {
@@ -1461,11 +1470,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
 
   // second half: rhs - aw*(g(i-1)-tt) - ae*(g(i+1)-tt)
   if (ook) {
-    const int i = 1 + o;
-    const double aw = mg.aw[i], ae = mg.ae[i];
     double2 rr[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) rr[r] = csub(csub(sb[r], rmul(aw, TILE_AT(2 * w, r))), rmul(ae, TILE_AT(2 * w + 2, r)));
+    for (int r = 0; r < R; ++r) rr[r] = csub(csub(sb[r], rmul(aw2, TILE_AT(2 * w, r))), rmul(ae2, TILE_AT(2 * w + 2, r)));
     tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
   } else {
 #pragma unroll
@@ -1480,18 +1487,14 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   // residual of the pair (rows ia = 1+e, ib = ia+1), restriction, pc, norm
   double ar = 0.0;
   if (pair) {
-    const int qa = 2 * w, ia = 1 + e, ib = ia + 1, ic = mg.iimg[mg.nlev - 1][1] + f0 / 2 + w - 1;
-    const double awa = mg.aw[ia], aea = mg.ae[ia], awb = mg.aw[ib], aeb = mg.ae[ib];
-    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic];
+    const int qa = 2 * w;
+    const double awb = aw2, aeb = ae2;
     double acA[R], acB[R], anj[R];
-    {
-      const double wea = mg.awe[ia], web = mg.awe[ib], ak = mg.ak3[kv];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
-        acA[r] = ac_of(wea, asj[r], anj[r], ak);
-        acB[r] = ac_of(web, asj[r], anj[r], ak);
-      }
+    for (int r = 0; r < R; ++r) {
+      anj[r] = san[j0 - 1 + r];
+      acA[r] = ac_of(wea, asj[r], anj[r], ak);
+      acB[r] = ac_of(web, asj[r], anj[r], ak);
     }
     double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
 #pragma unroll
@@ -1580,8 +1583,11 @@ static FusedTopKernel pick_top(int R, bool full) {
 
 // LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
 // T/2+1 (up), 64*R rows each
+// dynamic LDS of the fused kernels: the tile's line slots and, for the down
+// and top kernels, the an(j) table their residual stage reads
 static inline unsigned fused_lds(int R, int T, bool up) {
-  return (unsigned)((up ? T / 2 + 1 : T + 3) * R * 64 * (int)sizeof(double2));
+  return (unsigned)(up ? (T / 2 + 1) * R * 64 * (int)sizeof(double2)
+                       : (T + 3) * R * 64 * (int)sizeof(double2) + 64 * R * (int)sizeof(double));
 }
 
 static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, double test, int cyc) {
