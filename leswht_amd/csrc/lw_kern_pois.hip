@@ -1135,6 +1135,24 @@ __device__ inline void store_rows(double2 *row, int j0, int n2m, const double2 (
     if ((FULL || j0 + r <= n2m)) row[j0 + r] = v[r];
 }
 
+// XCD-aware placement of the fused kernels' workgroups. gfx950 deals
+// consecutive workgroups of a launch to its 8 XCDs in turn, and each XCD has
+// its own L2; neighbouring tiles of a wavenumber read each other's halo rows
+// and coarse rows. Each wavenumber's tiles are cut into 8 runs of
+// c = ceil(ntile / 8) consecutive tiles, one run per XCD: workgroup b runs on
+// XCD x = b % 8 and takes tile x*c + (b/8) % c of wavenumber (b/8) / c
+// (tiles past the end exit). A wavenumber still spreads over all XCDs, which
+// matters when only one or two are left iterating.
+constexpr int kXcds = 8;
+__device__ inline void xcd_tile(int b, int ntile, int &kvi, int &tile) {
+  const int c = (ntile + kXcds - 1) / kXcds, slot = b / kXcds;
+  kvi = slot / c;
+  tile = (b % kXcds) * c + slot % c;
+}
+static inline dim3 xcd_grid(int ntile, int n3mh) {
+  return dim3((unsigned)(kXcds * ((ntile + kXcds - 1) / kXcds) * n3mh));
+}
+
 // tile size of the fused kernels on a level of n lines
 constexpr int kTileMax = 8;
 static inline int fused_tile(int n, int tmax) {
@@ -1161,7 +1179,11 @@ template <int R, bool FULL>
 __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
                                                                       int cyc) {
   extern __shared__ double2 lds[];
-  const int kv = 1 + blockIdx.y;
+  const int ntile = (mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1) / T;
+  int kvi, tile;
+  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  if (tile >= ntile) return;
+  const int kv = 1 + kvi;
   if (mg.done[kv - 1]) return;
   if (cyc >= 0) {
     // the convergence test of the toplevel just done (k_mg_check's), folded
@@ -1170,7 +1192,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
     // records it. An active wavenumber has completed `cyc` V-cycles.
     const double r = __longlong_as_double((long long)mg.sumres[kv - 1]);
     const int dn = r < test ? 1 : cyc >= mg.mgitr ? 2 : 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    if (tile == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
       mg.cyc[kv - 1] = cyc;
       mg.pcsel[kv - 1] ^= 1;  // k_mg_top wrote the new pc to the other buffer
       if (dn) mg.done[kv - 1] = dn;
@@ -1182,7 +1204,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
-  const int f0 = blockIdx.x * T;
+  const int f0 = tile * T;
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 z2 = make_double2(0.0, 0.0);
   double asj[R];
@@ -1290,7 +1312,11 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
 template <int R, bool FULL>
 __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
-  const int kv = 1 + blockIdx.y;
+  const int ntile = (mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1) / T;
+  int kvi, tile;
+  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  if (tile >= ntile) return;
+  const int kv = 1 + kvi;
   if (mg.done[kv - 1]) return;
   // the wavefront index as a uniform value: line coefficients come in by
   // scalar loads and row addresses stay in scalar registers
@@ -1298,7 +1324,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
   const int n2m = FULL ? 64 * R : g.n2m, j0 = 1 + lane * R;
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1;
   const int cb = mg.iimg[ilev - 1][1], ce = mg.iimg[ilev - 1][2];
-  const int f0 = blockIdx.x * T;
+  const int f0 = tile * T;
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 z2 = make_double2(0.0, 0.0);
   const double2 *gg = mg.gg + pl, *gd = mg.gd + pl;
@@ -1409,11 +1435,15 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   extern __shared__ double2 lds[];
   __shared__ double wmax[kTileMax / 2 + 3];
   __shared__ double2 tts;
-  const int kv = 1 + blockIdx.y;
+  const int ntile = g.n1m / T;
+  int kvi, tile;
+  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  if (tile >= ntile) return;
+  const int kv = 1 + kvi;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y), W = blockDim.y;
   const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
-  const int f0 = blockIdx.x * T;
+  const int f0 = tile * T;
   const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
   const int sel = mg.pcsel[kv - 1];
   const double2 *pcr = (sel ? mg.pc2 : mg.pc) + ppl;
@@ -1596,8 +1626,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, dou
   // second-half solution 16 on half the lines (8), the residual 16 and the
   // restricted row 16 per pair (8)
   LW_LAUNCH("k_mg_down", 0, 56.0 * g.n2m * n, pick_down(R, full_rows(g)),
-            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T,
-            test, cyc);
+            xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T, test, cyc);
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
@@ -1605,8 +1634,8 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   // unique bytes per point of the level: reads residual 16, bet 8, the
   // down-sweep row on half the lines (8) and the coarse row per pair (8);
   // writes the solution 16
-  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), dim3((unsigned)(n / T), (unsigned)mg.n3mh),
-            dim3(64, T / 2 + 1), fused_lds(R, T, true), s, g, mg, ilev, T);
+  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 1),
+            fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
 static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
@@ -1616,8 +1645,7 @@ static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double
   // 16 and the restricted row per pair (8)
   const double per_pt = id == 1 ? 64.0 : 56.0;
   LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g)),
-            dim3((unsigned)(n / T), (unsigned)mg.n3mh), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T,
-            oldv, id);
+            xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T, oldv, id);
 }
 
 static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
