@@ -63,6 +63,7 @@ struct IBComp {
 
 struct MG {
   int nlev, n1md, n3mh, nbli, mgitr;
+  int kv0, nkv;  // the wavenumbers kv0 .. kv0+nkv-1 (1-based) a launch covers
   int lc;        // levels 0..lc run in one launch per V-cycle (-1: none)
   int coarse_lds;  // 1: levels 1 and 0 with LDS-resident planes when they fit
   long LDJ;      // row length of every MG plane: n2 + 1 rounded up to 8

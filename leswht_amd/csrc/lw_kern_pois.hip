@@ -538,7 +538,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
 template <int R, int MODE, bool FULL>
 __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
   const int t = blockIdx.x * 4 + threadIdx.y;
-  const int kv = 1 + blockIdx.y;
+  const int kv = mg.kv0 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   if (t >= nlines) return;
   line_solve<R, MODE, FULL>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
@@ -592,7 +592,7 @@ static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, in
   // lines), bet 8, solution 16; the prolonging modes also read one coarse row
   // per line (16)
   const double per_row = mode >= 4 ? 72.0 : 56.0;
-  LW_LAUNCH(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.n3mh),
+  LW_LAUNCH(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.nkv),
             dim3(64, 4), 0, s, g, mg, ibase, nlines, oldv, clev);
 }
 
@@ -634,7 +634,7 @@ __device__ inline void lev_rr(const Geo &g, const MG &mg, int kv, int ilev, int 
 }
 
 __global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
-  const int kv = 1 + blockIdx.y;
+  const int kv = mg.kv0 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   lev_rr(g, mg, kv, ilev, blockIdx.x, 1 + threadIdx.x, 256);
 }
@@ -658,7 +658,7 @@ __device__ inline void wg_sweep(const Geo &g, const MG &mg, int kv, int ibase, i
 
 template <int R, bool FULL>
 __global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, int lc) {
-  const int kv = 1 + blockIdx.x;
+  const int kv = mg.kv0 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
   const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -725,7 +725,7 @@ __device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, Li
 template <int R, bool FULL>
 __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
   extern __shared__ double2 lds[];
-  const int kv = 1 + blockIdx.x;
+  const int kv = mg.kv0 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
   // wavefront y takes level-0 line w (and level-1 lines 2w, 2w+1): the even
@@ -1008,11 +1008,11 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
     const int n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
-    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.n3mh), dim3(64, n0),
+    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.nkv), dim3(64, n0),
               (unsigned)lds, s, g, mg);
     return;
   }
-  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.n3mh), dim3(64, kCoarseWaves),
+  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.nkv), dim3(64, kCoarseWaves),
             0, s, g, mg, mg.lc);
 }
 
@@ -1024,7 +1024,7 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
 // pc = solution, coarse row t, per-wavenumber max norm.
 __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
   __shared__ double sh[4];
-  const int kv = 1 + blockIdx.y;
+  const int kv = mg.kv0 + blockIdx.y;
   if (!mg.done[kv - 1]) {
     const int t = blockIdx.x;
     const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
@@ -1093,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
 // `test`, or when it has used MGITR cycles. Clears the norms for the next
 // toplevel.
 __global__ void k_mg_check(MG mg, double test, int id, int flip) {
-  for (int q = threadIdx.x; q < mg.n3mh; q += blockDim.x) {
+  for (int q = mg.kv0 - 1 + threadIdx.x; q < mg.kv0 - 1 + mg.nkv; q += blockDim.x) {
     if (!mg.done[q]) {
       const int c = mg.cyc[q] + id;
       mg.cyc[q] = c;
@@ -1149,8 +1149,8 @@ __device__ inline void xcd_tile(int b, int ntile, int &kvi, int &tile) {
   kvi = slot / c;
   tile = (b % kXcds) * c + slot % c;
 }
-static inline dim3 xcd_grid(int ntile, int n3mh) {
-  return dim3((unsigned)(kXcds * ((ntile + kXcds - 1) / kXcds) * n3mh));
+static inline dim3 xcd_grid(int ntile, int nkv) {
+  return dim3((unsigned)(kXcds * ((ntile + kXcds - 1) / kXcds) * nkv));
 }
 
 // tile size of the fused kernels on a level of n lines
@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
   int kvi, tile;
   xcd_tile(blockIdx.x, ntile, kvi, tile);
   if (tile >= ntile) return;
-  const int kv = 1 + kvi;
+  const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
   if (cyc >= 0) {
     // the convergence test of the toplevel just done (k_mg_check's), folded
@@ -1316,7 +1316,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
   int kvi, tile;
   xcd_tile(blockIdx.x, ntile, kvi, tile);
   if (tile >= ntile) return;
-  const int kv = 1 + kvi;
+  const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
   // the wavefront index as a uniform value: line coefficients come in by
   // scalar loads and row addresses stay in scalar registers
@@ -1439,7 +1439,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   int kvi, tile;
   xcd_tile(blockIdx.x, ntile, kvi, tile);
   if (tile >= ntile) return;
-  const int kv = 1 + kvi;
+  const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y), W = blockDim.y;
   const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
@@ -1626,7 +1626,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, dou
   // second-half solution 16 on half the lines (8), the residual 16 and the
   // restricted row 16 per pair (8)
   LW_LAUNCH("k_mg_down", 0, 56.0 * g.n2m * n, pick_down(R, full_rows(g)),
-            xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T, test, cyc);
+            xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T, test, cyc);
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
@@ -1634,7 +1634,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   // unique bytes per point of the level: reads residual 16, bet 8, the
   // down-sweep row on half the lines (8) and the coarse row per pair (8);
   // writes the solution 16
-  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 1),
+  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 1),
             fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
@@ -1645,10 +1645,10 @@ static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double
   // 16 and the restricted row per pair (8)
   const double per_pt = id == 1 ? 64.0 : 56.0;
   LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g)),
-            xcd_grid(n / T, mg.n3mh), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T, oldv, id);
+            xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T, oldv, id);
 }
 
-static inline dim3 rows(int n, int n3mh) { return dim3((unsigned)n, (unsigned)n3mh); }
+static inline dim3 rows(int n, int nkv) { return dim3((unsigned)n, (unsigned)nkv); }
 
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test, bool check) {
   const int nl = mg.nlev;
@@ -1658,7 +1658,7 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
     launch_lines(s, g, mg, id == 1 ? 5 : 2, 1, (g.n1m + 1) / 2, oldv, nl - 1);
     launch_lines(s, g, mg, 3, 2, g.n1m / 2, 1.0, 0);
     // rhs 16, solution 16, residual 16, pc 16 per point; coarse row 16 per pair
-    LW_LAUNCH("k_top_rr", 0, 72.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.n3mh), dim3(256), 0, s, g, mg);
+    LW_LAUNCH("k_top_rr", 0, 72.0 * g.n2m * g.n1m, k_top_rr, rows(g.n1m / 2, mg.nkv), dim3(256), 0, s, g, mg);
   }
   if (check) LW_LAUNCH("k_mg_check", 0, 0, k_mg_check, dim3(1), dim3(128), 0, s, mg, test, id, mg.fused);
 }
@@ -1684,7 +1684,7 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int
     launch_lines(s, g, mg, 0, mg.iimg[ilev][1], (n + 1) / 2, 0.0, 0);
     launch_lines(s, g, mg, 1, mg.iimg[ilev][1] + 1, n / 2, 1.0, 0);
     // residual in/out 32, solution 16 per fine point; coarse row 8 per fine point
-    LW_LAUNCH("k_lev_rr", 0, 56.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.n3mh), dim3(256), 0, s, g, mg, ilev);
+    LW_LAUNCH("k_lev_rr", 0, 56.0 * g.n2m * n, k_lev_rr, rows(n / 2, mg.nkv), dim3(256), 0, s, g, mg, ilev);
   }
   if (mg.lc >= 0) {
     launch_coarse(s, g, mg);

@@ -10,6 +10,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <thread>
+
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -44,7 +46,7 @@ static int fail(int code, const std::string &msg) {
 // ---------------------------------------------------------------------------
 namespace lw {
 struct ProfRec {
-  int cls, cycle;
+  int cls, cycle, kv0, nkv;
   double bytes, bytes_kv;
   hipEvent_t a, b;
 };
@@ -59,6 +61,7 @@ struct Prof {
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   int cycle = -1;      // multigrid cycle being enqueued (0: the first toplevel)
+  int kv0 = 1, nkv = 0;  // its wavenumbers kv0 .. kv0+nkv-1 (nkv 0: all)
   size_t mg_first = 0; // first record of the Poisson solve being enqueued
 
   hipEvent_t take() {
@@ -93,11 +96,14 @@ struct Prof {
       ProfRec &p = recs[r];
       if (p.bytes_kv == 0.0) continue;
       int act = 0;
-      for (int q = 0; q < n3mh; ++q) act += (p.cycle == 0 || cyc[q] >= p.cycle) ? 1 : 0;
+      const int q0 = p.nkv > 0 ? p.kv0 - 1 : 0, q1 = p.nkv > 0 ? p.kv0 - 1 + p.nkv : n3mh;
+      for (int q = q0; q < q1; ++q) act += (p.cycle == 0 || cyc[q] >= p.cycle) ? 1 : 0;
       p.bytes += p.bytes_kv * act;
       p.bytes_kv = 0.0;
     }
     cycle = -1;
+    kv0 = 1;
+    nkv = 0;
   }
   // after the stream drained: fold the records into the per-kernel totals
   void flush() {
@@ -127,7 +133,7 @@ bool prof_take(const char *name, hipEvent_t *a, hipEvent_t *b) {
 }
 
 void prof_record(const char *name, hipEvent_t a, hipEvent_t b, double bytes, double bytes_kv) {
-  g_prof->recs.push_back(ProfRec{g_prof->cls(name), g_prof->cycle, bytes, bytes_kv, a, b});
+  g_prof->recs.push_back(ProfRec{g_prof->cls(name), g_prof->cycle, g_prof->kv0, g_prof->nkv, bytes, bytes_kv, a, b});
 }
 }  // namespace lw
 
@@ -139,6 +145,8 @@ struct ProfScope {
 
 struct lw_handle {
   hipStream_t s = nullptr;
+  hipStream_t s2 = nullptr;  // the Poisson solve of kv = 1, beside the other wavenumbers
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   lw_settings st;
   lw_body body;
   int les = 0, heat = 0, ncomp = 3;
@@ -161,7 +169,10 @@ struct lw_handle {
   hipEvent_t ev[8];
   unsigned long long *h_sumres = nullptr;  // pinned
   int *h_flags = nullptr;                  // pinned copy of mg.done / mg.cyc
+  int *h_flags2 = nullptr;                 // the same, read on s2
   int mg_hint = 4;                         // V-cycles of the previous Poisson solve
+  int mg_hint1 = 4;                        // of its kv = 1 alone (two-stream solve)
+  int mg_split = 1;                        // kv = 1 on its own stream (LW_MG_SPLIT=0: one stream)
   DiagPlanes dplanes{};
   double *hist = nullptr;  // device: the step's history values and their running averages
   double h_hist[32] = {0};
@@ -429,6 +440,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.nlev = nlev;
   mg.n1md = 2 * n1m;
   mg.n3mh = n3m / 2 + 1;
+  mg.kv0 = 1;
+  mg.nkv = mg.n3mh;
   mg.nbli = (int)st.nbli;
   mg.mgitr = (int)st.mgitr;
   mg.LDJ = ((long)(n2 + 1) + 7) / 8 * 8;
@@ -643,11 +656,16 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     h->csgsts = s->csgsts;
     h->csgshf = s->csgshf;
     HIPCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->s2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    if (const char *e = getenv("LW_MG_SPLIT")) h->mg_split = atoi(e) != 0;
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     for (int q = 0; q < 3; ++q)
       for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->tev[q][e]));
     HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
     HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
+    HIPCHK(hipHostMalloc((void **)&h->h_flags2, 256 * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
     setup(h, gr, t);
     // dttimeinit / rk3coefinit (misc_init.f90:1001-1044)
@@ -666,6 +684,7 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
 int lw_destroy(lw_handle *h) {
   if (!h) return LW_OK;
   if (h->s) hipStreamSynchronize(h->s);
+  if (h->s2) hipStreamSynchronize(h->s2);
   for (void *p : h->allocs) hipFree(p);
   for (int e = 0; e < 8; ++e)
     if (h->ev[e]) hipEventDestroy(h->ev[e]);
@@ -674,8 +693,12 @@ int lw_destroy(lw_handle *h) {
       if (h->tev[q][e]) hipEventDestroy(h->tev[q][e]);
   if (h->h_sumres) hipHostFree(h->h_sumres);
   if (h->h_flags) hipHostFree(h->h_flags);
+  if (h->h_flags2) hipHostFree(h->h_flags2);
   if (h->h_red) hipHostFree(h->h_red);
   delete h->prof;
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->s2) hipStreamDestroy(h->s2);
   if (h->s) hipStreamDestroy(h->s);
   delete h;
   return LW_OK;
@@ -852,6 +875,93 @@ static void phase_rhsnlhs(lw_handle *h) {
   launch_retrv_mom(h->s, g, h->f);
 }
 
+// kv = 1 (the mean in z) needs several times the V-cycles of the other
+// wavenumbers (C2: 55-110 against 12-22), and alone it is latency-bound. Its
+// cycles therefore run on s2 beside the bandwidth-bound cycles of kv = 2..n3mh
+// on s, each side with its own batches and flags; the two sides touch
+// disjoint wavenumbers, and every wavenumber goes through the same kernels
+// and cycles as in one stream. The host polls both streams and gives a side
+// its next batch as soon as its flags say it needs one.
+static void poisson_split(lw_handle *h, double test, bool fold) {
+  const Geo &g = h->g;
+  MG a = h->mg, b = h->mg;
+  a.kv0 = 1;
+  a.nkv = 1;
+  b.kv0 = 2;
+  b.nkv = h->mg.n3mh - 1;
+  const int itr = h->mg.mgitr;
+  HIPCHK(hipEventRecord(h->ev_fork, h->s));
+  HIPCHK(hipStreamWaitEvent(h->s2, h->ev_fork, 0));
+  struct Side {
+    const MG *m;
+    hipStream_t s;
+    int *flags;
+    int launched, batch, pending;
+    bool done;
+  };
+  Side sd[2] = {{&a, h->s2, h->h_flags2, 0, h->mg_hint1 > 1 ? h->mg_hint1 : 1, 0, false},
+                {&b, h->s, h->h_flags, 0, h->mg_hint > 1 ? h->mg_hint : 1, 0, false}};
+  auto prof_side = [&](const Side &x, int cycle) {
+    if (!g_prof) return;
+    g_prof->cycle = cycle;
+    g_prof->kv0 = x.m->kv0;
+    g_prof->nkv = x.m->nkv;
+  };
+  for (Side &x : sd) {
+    prof_side(x, 0);
+    launch_mg_toplevel(x.s, g, *x.m, 0, 0.0, test, !fold);
+  }
+  // enqueue the next batch of every side that is idle and not done, one
+  // cycle of each side at a time, then its flags
+  auto enqueue = [&]() {
+    int n[2];
+    for (int k = 0; k < 2; ++k) {
+      Side &x = sd[k];
+      const bool go = !x.pending && !x.done && x.launched < itr;
+      n[k] = go ? (x.batch < itr - x.launched ? x.batch : itr - x.launched) : 0;
+    }
+    const int nmax = n[0] > n[1] ? n[0] : n[1];
+    for (int c = 0; c < nmax; ++c)
+      for (int k = 0; k < 2; ++k) {
+        Side &x = sd[k];
+        if (c >= n[k]) continue;
+        const int m = x.launched + c + 1;
+        prof_side(x, m);
+        const int cyc = fold && !(c == 0 && x.launched > 0) ? m - 1 : -1;
+        launch_mg_cycle(x.s, g, *x.m, test, cyc, !fold || c == n[k] - 1);
+      }
+    for (int k = 0; k < 2; ++k) {
+      Side &x = sd[k];
+      if (!n[k]) continue;
+      x.launched += n[k];
+      x.pending = 1;
+      x.batch = 2;
+      HIPCHK(hipMemcpyAsync(x.flags, h->mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, x.s));
+    }
+  };
+  enqueue();
+  while (sd[0].pending || sd[1].pending) {
+    bool progressed = false;
+    for (Side &x : sd) {
+      if (!x.pending) continue;
+      const hipError_t q = hipStreamQuery(x.s);
+      if (q == hipErrorNotReady) continue;
+      HIPCHK(q);
+      x.pending = 0;
+      bool d = true;
+      for (int i = x.m->kv0 - 1; i < x.m->kv0 - 1 + x.m->nkv; ++i) d = d && x.flags[i] != 0;
+      x.done = d;
+      progressed = true;
+    }
+    if (progressed) enqueue();
+    else std::this_thread::yield();
+  }
+  HIPCHK(hipEventRecord(h->ev_join, h->s2));
+  HIPCHK(hipStreamWaitEvent(h->s, h->ev_join, 0));
+  HIPCHK(hipMemcpyAsync(h->h_flags, h->mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+}
+
 // poisson (poiss_z_ft.f90:114-299): z-DFT, then V-cycles of every wavenumber
 // until its max-norm residual passes the test. Convergence is decided on the
 // device; the host enqueues batches of cycles (the first as long as the
@@ -871,40 +981,50 @@ static void phase_poisson(lw_handle *h) {
   // with the check folded, the test after a toplevel runs in the next
   // cycle's first launch, and a k_mg_check only ends each batch
   const bool fold = mg_folds_check(mg);
-  launch_mg_toplevel(h->s, g, mg, 0, 0.0, test, !fold);
-  int launched = 0, batch = h->mg_hint > 1 ? h->mg_hint : 1;
-  bool all_done = false;
-  while (!all_done && launched < mg.mgitr) {
-    if (batch > mg.mgitr - launched) batch = mg.mgitr - launched;
-    for (int c = 0; c < batch; ++c) {
-      const int m = launched + c + 1;  // this V-cycle's number in the solve
-      if (g_prof) g_prof->cycle = m;
-      const int cyc = fold && !(c == 0 && launched > 0) ? m - 1 : -1;
-      launch_mg_cycle(h->s, g, mg, test, cyc, !fold || c == batch - 1);
+  if (h->mg_split && mg.fused && mg.n3mh >= 2 && mg.mgitr > 0) {
+    poisson_split(h, test, fold);
+  } else {
+    launch_mg_toplevel(h->s, g, mg, 0, 0.0, test, !fold);
+    int launched = 0, batch = h->mg_hint > 1 ? h->mg_hint : 1;
+    bool all_done = false;
+    while (!all_done && launched < mg.mgitr) {
+      if (batch > mg.mgitr - launched) batch = mg.mgitr - launched;
+      for (int c = 0; c < batch; ++c) {
+        const int m = launched + c + 1;  // this V-cycle's number in the solve
+        if (g_prof) g_prof->cycle = m;
+        const int cyc = fold && !(c == 0 && launched > 0) ? m - 1 : -1;
+        launch_mg_cycle(h->s, g, mg, test, cyc, !fold || c == batch - 1);
+      }
+      launched += batch;
+      HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+      all_done = true;
+      for (int q = 0; q < mg.n3mh; ++q) all_done = all_done && h->h_flags[q] != 0;
+      batch = 2;
     }
-    launched += batch;
-    HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
-    all_done = true;
-    for (int q = 0; q < mg.n3mh; ++q) all_done = all_done && h->h_flags[q] != 0;
-    batch = 2;
+    if (launched == 0) {  // every wavenumber passed on the first toplevel
+      HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+    }
   }
-  if (launched == 0) {  // every wavenumber passed on the first toplevel
-    HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
-  }
-  int cmax = 0;
+  int cmax = 0, crest = 0;
   bool limit = false;
   for (int q = 0; q < mg.n3mh; ++q) {
     const int c = h->h_flags[128 + q];
     cmax = c > cmax ? c : cmax;
+    if (q > 0) crest = c > crest ? c : crest;
     h->cyc_sum += c;
     limit = limit || h->h_flags[q] != 1;
   }
   if (limit) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
   if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
   if (cmax > h->cyc_max) h->cyc_max = cmax;
-  h->mg_hint = cmax;
+  if (h->mg_split && mg.fused && mg.n3mh >= 2) {
+    h->mg_hint1 = h->h_flags[128];
+    h->mg_hint = crest;
+  } else {
+    h->mg_hint = cmax;
+  }
   launch_fft_inverse(h->s, g, mg, h->f);
 }
 
