@@ -40,7 +40,7 @@ __device__ inline double ac_of(double awe, double as, double an, double ak) { re
 #define FWD_STEP(CT)                                                             \
   {                                                                              \
     const double Mp = dpp_d<CT>(M, 1.0);                                         \
-    const double2 Cp = make_double2(dpp_d<CT>(Cc.x, 0.0), dpp_d<CT>(Cc.y, 0.0)); \
+    const double2 Cp = make_double2(dpp_z<CT>(Cc.x), dpp_z<CT>(Cc.y)); \
     Cc = cfma(M, Cp, Cc);                                                        \
     M = M * Mp;                                                                  \
   }
@@ -54,7 +54,7 @@ __device__ inline double ac_of(double awe, double as, double an, double ak) { re
 #define BWD_STEP(CT)                                                           \
   {                                                                            \
     const double Np = dpp_d<CT>(Nn, 1.0);                                      \
-    const double2 Dp = make_double2(dpp_d<CT>(D.x, 0.0), dpp_d<CT>(D.y, 0.0)); \
+    const double2 Dp = make_double2(dpp_z<CT>(D.x), dpp_z<CT>(D.y)); \
     D = cfma(Nn, Dp, D);                                                       \
     Nn = Nn * Np;                                                              \
   }
@@ -396,7 +396,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
   CROSS_STEP(kBcast15, 0x2)
   CROSS_STEP(kBcast31, 0x4)
   CROSS_STEP(kBcast15, 0x8)
-  double2 yp = make_double2(dpp_d<kWshr1>(Cc.x, 0.0), dpp_d<kWshr1>(Cc.y, 0.0));
+  double2 yp = make_double2(dpp_z<kWshr1>(Cc.x), dpp_z<kWshr1>(Cc.y));
   double2 y[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -435,7 +435,7 @@ __device__ inline void tri_regs(const double2 (&rr)[R], const double (&bt)[R], c
     const int row = lane >> 4;
     const double2 next = row == 3 ? make_double2(0.0, 0.0) : row == 2 ? d3 : row == 1 ? s2 : s1;
     D = cfma(Nn, next, D);  // inclusive suffix over the wavefront
-    xn = make_double2(dpp_d<kWshl1>(D.x, 0.0), dpp_d<kWshl1>(D.y, 0.0));
+    xn = make_double2(dpp_z<kWshl1>(D.x), dpp_z<kWshl1>(D.y));
   }
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {

@@ -17,6 +17,16 @@ __device__ inline double dpp_d(double v, double old) {
   const int hi = __builtin_amdgcn_update_dpp((int)(ob >> 32), (int)(vb >> 32), CTRL, ROWS, 0xF, false);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+// DPP move of a double with zero fill: lanes without a source (outside the
+// 16-lane row, or past the wavefront's end) read 0 (bound_ctrl), so no
+// identity value has to be placed in the destination first
+template <int CTRL>
+__device__ inline double dpp_z(double v) {
+  const long long vb = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)vb, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(vb >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 __device__ inline double rdlane(double v, int l) {
   const long long vb = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)vb, l), hi = __builtin_amdgcn_readlane((int)(vb >> 32), l);
@@ -43,8 +53,11 @@ __device__ inline M22 m22_norm(const M22 &A) {
 }
 template <int CTRL, int ROWS = 0xF>
 __device__ inline M22 m22_dpp(const M22 &A) {  // identity where nothing moves in
-  return M22{dpp_d<CTRL, ROWS>(A.p, 1.0), dpp_d<CTRL, ROWS>(A.q, 0.0), dpp_d<CTRL, ROWS>(A.r, 0.0),
-             dpp_d<CTRL, ROWS>(A.s, 1.0)};
+  if constexpr (ROWS == 0xF)
+    return M22{dpp_d<CTRL>(A.p, 1.0), dpp_z<CTRL>(A.q), dpp_z<CTRL>(A.r), dpp_d<CTRL>(A.s, 1.0)};
+  else
+    return M22{dpp_d<CTRL, ROWS>(A.p, 1.0), dpp_d<CTRL, ROWS>(A.q, 0.0), dpp_d<CTRL, ROWS>(A.r, 0.0),
+               dpp_d<CTRL, ROWS>(A.s, 1.0)};
 }
 
 // Solves a_t x_{t-1} + b_t x_t + c_t x_{t+1} = f_t, t = 1..N (a_1 and c_N
@@ -66,7 +79,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   const int t0 = 1 + ll * R;
   auto ok = [&](int r) { return FULL || t0 + r <= N; };
   // c of the row before this lane's first (unused on a line's first lane)
-  const double cl = dpp_d<kWshr1>(c[R - 1], 0.0);
+  const double cl = dpp_z<kWshr1>(c[R - 1]);
   // pivot maps: this lane's product, then the exclusive scan over the lanes
   M22 L{1.0, 0.0, 0.0, 1.0};
 #pragma unroll
@@ -131,7 +144,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
 #define FWD_R(CT)                       \
   {                                     \
     const double Mp = dpp_d<CT>(M, 1.0); \
-    const double Cp = dpp_d<CT>(Cc, 0.0); \
+    const double Cp = dpp_z<CT>(Cc); \
     Cc = fma(M, Cp, Cc);                \
     M = M * Mp;                         \
   }
@@ -147,7 +160,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
   } else {
     Cc = fma(M, dpp_d<kBcast15, 0xA>(Cc, 0.0), Cc);
   }
-  double yp = dpp_d<kWshr1>(Cc, 0.0);
+  double yp = dpp_z<kWshr1>(Cc);
   if (LANES == 32 && ll == 0) yp = 0.0;
   double y[R];
 #pragma unroll
@@ -172,7 +185,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
 #define BWD_R(CT)                        \
   {                                      \
     const double Np = dpp_d<CT>(Nn, 1.0); \
-    const double Dp = dpp_d<CT>(D, 0.0);  \
+    const double Dp = dpp_z<CT>(D);  \
     D = fma(Nn, Dp, D);                  \
     Nn = Nn * Np;                        \
   }
@@ -195,7 +208,7 @@ __device__ inline void tri_line(const double (&a)[R], const double (&b)[R], cons
     const double next = row == 0 ? d1 : row == 2 ? d3 : 0.0;
     D = fma(Nn, next, D);
   }
-  double xn = dpp_d<kWshl1>(D, 0.0);
+  double xn = dpp_z<kWshl1>(D);
   if (LANES == 32 && ll == 31) xn = 0.0;
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
@@ -225,7 +238,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   constexpr int N = 32 * R;
   const int ll = lane & 31, hb = lane & 32;
   const int t0 = 1 + ll * R;
-  const double cl = dpp_d<kWshr1>(c[R - 1], 0.0);  // c of the row before this lane's first
+  const double cl = dpp_z<kWshr1>(c[R - 1]);  // c of the row before this lane's first
   M22 L{1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -260,7 +273,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
 #define PFWD(CT, ROWS)                           \
   {                                              \
     const double Mp = dpp_d<CT, ROWS>(M, 1.0);   \
-    const double Cp = dpp_d<CT, ROWS>(Cc, 0.0);  \
+    const double Cp = ROWS == 0xF ? dpp_z<CT>(Cc) : dpp_d<CT, ROWS>(Cc, 0.0); \
     Cc = fma(M, Cp, Cc);                         \
     M = M * Mp;                                  \
   }
@@ -270,7 +283,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
   PFWD(kShr8, 0xF)
   PFWD(kBcast15, 0xA)
 #undef PFWD
-  double sp = dpp_d<kWshr1>(M, 1.0), yp = dpp_d<kWshr1>(Cc, 0.0);
+  double sp = dpp_d<kWshr1>(M, 1.0), yp = dpp_z<kWshr1>(Cc);
   if (ll == 0) {  // s_0 = 1, f_0 = 0: row 1 becomes s_1 = -a_1 pn_1, f_1 = f_1 pn_1
     sp = 1.0;
     yp = 0.0;
@@ -296,8 +309,8 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
 #define PBWD(CT)                                \
   {                                             \
     const double Np = dpp_d<CT>(Nn, 1.0);       \
-    const double Sp = dpp_d<CT>(Ds, 0.0);       \
-    const double Qp = dpp_d<CT>(Dq, 0.0);       \
+    const double Sp = dpp_z<CT>(Ds);       \
+    const double Qp = dpp_z<CT>(Dq);       \
     Ds = fma(Nn, Sp, Ds);                       \
     Dq = fma(Nn, Qp, Dq);                       \
     Nn = Nn * Np;                               \
@@ -314,7 +327,7 @@ __device__ inline void tri_per_line32(const double (&a)[R], const double (&b)[R]
     Ds = fma(Nn, ns, Ds);
     Dq = fma(Nn, nq, Dq);
   }
-  double xs = dpp_d<kWshl1>(Ds, 0.0), xq = dpp_d<kWshl1>(Dq, 0.0);
+  double xs = dpp_z<kWshl1>(Ds), xq = dpp_z<kWshl1>(Dq);
   if (ll == 31) {  // holds row N, which does not read its successor
     xs = 0.0;
     xq = 0.0;
