@@ -240,16 +240,13 @@ __global__ __launch_bounds__(kRedThreads) void k_coef(Fld f, int nblk, double sc
 
 // 3-point Simpson box test filter in x (i = 2..n1m-1, non-periodic) and z
 // (periodic), in that order (test_filter, sgs.f90:641-769). in -> out.
-__device__ inline double xfilt(const Geo &g, const double *a, int i, int j, int k, int fx) {
-  const long q = I3(g, i, j, k);
-  if (!fx || i < 2 || i > g.n1m - 1) return a[q];
-  return g.cfx1[3 * i] * a[q - 1] + g.cfx1[3 * i + 1] * a[q] + g.cfx1[3 * i + 2] * a[q + 1];
-}
-
 // A thread walks kFiltK planes of its (i, j) column up the periodic
 // z-direction and keeps the x-filtered values of planes k-1, k, k+1 in
 // registers, so an input value is read (kFiltK+2)/kFiltK times per launch
-// instead of three times.
+// instead of three times. Whether the column is x-filtered and its three
+// weights are decided once per thread; the column's loads are unconditional
+// (an unfiltered column reads its own point for both neighbours), so the
+// loads of later planes can be issued ahead of the arithmetic.
 constexpr int kFiltK = 16;
 
 __global__ __launch_bounds__(256) void k_filter(Geo g, const double *__restrict__ in, double *__restrict__ out,
@@ -259,16 +256,26 @@ __global__ __launch_bounds__(256) void k_filter(Geo g, const double *__restrict_
   const int n3m = g.n3m;
   const int k0 = 1 + blockIdx.z * kFiltK, k1 = k0 + kFiltK - 1 < n3m ? k0 + kFiltK - 1 : n3m;
   if (i > g.n1m || j > g.n2m) return;
+  const bool xi = fx && i >= 2 && i <= g.n1m - 1;
+  const int ic = xi ? i : 2;  // a weight row that exists
+  const double c0 = g.cfx1[3 * ic], c1 = g.cfx1[3 * ic + 1], c2 = g.cfx1[3 * ic + 2];
+  const long dm = xi ? -1 : 0, dp = xi ? 1 : 0;
+  const long q0 = I3(g, i, j, 1), st = (long)g.n1m * g.n2m;
+  auto X = [&](int k) {
+    const double *a = in + q0 + (long)(k - 1) * st;
+    const double am = a[dm], a0 = a[0], ap = a[dp];
+    return xi ? c0 * am + c1 * a0 + c2 * ap : a0;
+  };
   if (!fz) {
 #pragma unroll 4
-    for (int k = k0; k <= k1; ++k) out[I3(g, i, j, k)] = xfilt(g, in, i, j, k, fx);
+    for (int k = k0; k <= k1; ++k) out[q0 + (long)(k - 1) * st] = X(k);
     return;
   }
-  double xm = xfilt(g, in, i, j, k0 == 1 ? n3m : k0 - 1, fx), x0 = xfilt(g, in, i, j, k0, fx);
+  double xm = X(k0 == 1 ? n3m : k0 - 1), x0 = X(k0);
 #pragma unroll 4
   for (int k = k0; k <= k1; ++k) {
-    const double xp = xfilt(g, in, i, j, k == n3m ? 1 : k + 1, fx);
-    out[I3(g, i, j, k)] = g.cfz1[3 * k] * xm + g.cfz1[3 * k + 1] * x0 + g.cfz1[3 * k + 2] * xp;
+    const double xp = X(k == n3m ? 1 : k + 1);
+    out[q0 + (long)(k - 1) * st] = g.cfz1[3 * k] * xm + g.cfz1[3 * k + 1] * x0 + g.cfz1[3 * k + 2] * xp;
     xm = x0;
     x0 = xp;
   }
