@@ -1162,7 +1162,7 @@ static inline int fused_tile(int n, int tmax) {
 }
 // tile sizes (lines per workgroup) of the down, up and top kernels;
 // LW_MG_TILES="d,u,t" overrides them (each 2, 4 or 8)
-static int g_tile_down = 4, g_tile_up = 4, g_tile_top = 8;
+static int g_tile_down = 2, g_tile_up = 2, g_tile_top = 4;
 
 // the residual at rows j0+r of the two lines whose LDS slots are qa, qa+1
 // (neighbours qa-1, qa+2), in the reference's order (poiss_z_ft.f90:440-443)
