@@ -1160,9 +1160,6 @@ static inline int fused_tile(int n, int tmax) {
     if (n % t == 0) return t;
   return 2;
 }
-// tile sizes (lines per workgroup) of the down, up and top kernels;
-// LW_MG_TILES="d,u,t" overrides them (each 2, 4 or 8)
-static int g_tile_down = 2, g_tile_up = 2, g_tile_top = 4;
 
 // the residual at rows j0+r of the two lines whose LDS slots are qa, qa+1
 // (neighbours qa-1, qa+2), in the reference's order (poiss_z_ft.f90:440-443)
@@ -1621,7 +1618,7 @@ static inline unsigned fused_lds(int R, int T, bool up) {
 }
 
 static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, double test, int cyc) {
-  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_down), R = lanes_r(g.n2m);
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, mg.tile[0]), R = lanes_r(g.n2m);
   // unique bytes per point of the level: reads resd 16, bet 8; writes the
   // second-half solution 16 on half the lines (8), the residual 16 and the
   // restricted row 16 per pair (8)
@@ -1630,7 +1627,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, dou
 }
 
 static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
-  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, g_tile_up), R = lanes_r(g.n2m);
+  const int n = mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1, T = fused_tile(n, mg.tile[1]), R = lanes_r(g.n2m);
   // unique bytes per point of the level: reads residual 16, bet 8, the
   // down-sweep row on half the lines (8) and the coarse row per pair (8);
   // writes the solution 16
@@ -1639,7 +1636,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
 }
 
 static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv) {
-  const int n = g.n1m, T = fused_tile(n, g_tile_top), R = lanes_r(g.n2m);
+  const int n = g.n1m, T = fused_tile(n, mg.tile[2]), R = lanes_r(g.n2m);
   // unique bytes per top point: reads rhs 16, bet 8, the old pc on half the
   // lines (8) and for id = 1 the coarse row per pair (8); writes the new pc
   // 16 and the restricted row per pair (8)
@@ -1712,15 +1709,23 @@ void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int
 // the fused kernels need R <= 8 (n2m <= 512) to keep their tiles in LDS
 hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   const int R = lanes_r(g.n2m);
-  if (const char *e = getenv("LW_MG_TILES")) {
-    int d = 0, u = 0, t = 0;
-    if (sscanf(e, "%d,%d,%d", &d, &u, &t) == 3) {
-      auto ok = [](int v) { return v == 2 || v == 4 || v == 8; };
-      if (ok(d)) g_tile_down = d;
-      if (ok(u)) g_tile_up = u;
-      if (ok(t)) g_tile_top = t;
+  // tile sizes (lines per workgroup) of the down, up and top kernels: 2, 2, 4
+  // (LW_MG_TILES="d,u,t" overrides them, each 2, 4 or 8); tile1 for the
+  // kv = 1 side of the two-stream solve (LW_MG_TILES1, default the same)
+  auto parse = [](const char *var, int (&t)[3]) {
+    if (const char *e = getenv(var)) {
+      int v[3] = {0, 0, 0};
+      if (sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+        for (int q = 0; q < 3; ++q)
+          if (v[q] == 2 || v[q] == 4 || v[q] == 8) t[q] = v[q];
     }
-  }
+  };
+  mg.tile[0] = 2;
+  mg.tile[1] = 2;
+  mg.tile[2] = 4;
+  parse("LW_MG_TILES", mg.tile);
+  for (int q = 0; q < 3; ++q) mg.tile1[q] = mg.tile[q];
+  parse("LW_MG_TILES1", mg.tile1);
   if (R > 8) mg.fused = 0;
   if (mg.fused) {
     const int big = (int)fused_lds(R, kTileMax, false);

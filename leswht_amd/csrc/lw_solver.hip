@@ -887,6 +887,7 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
   MG a = h->mg, b = h->mg;
   a.kv0 = 1;
   a.nkv = 1;
+  for (int q = 0; q < 3; ++q) a.tile[q] = a.tile1[q];
   b.kv0 = 2;
   b.nkv = h->mg.n3mh - 1;
   const int itr = h->mg.mgitr;
