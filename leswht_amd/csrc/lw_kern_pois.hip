@@ -768,6 +768,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     }
     load_lu<R, FULL>(mg, pl, ia, j0, n2m, cA);
     load_lu<R, FULL>(mg, pl, ib, j0, n2m, cB);
+    load_lu<R, FULL>(mg, pl, ic, j0, n2m, c0);
   }
   double2 rr[R], x[R];
 
@@ -855,7 +856,6 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   }
 
   // relax(0, NBLI): first half on even lines, second half on odd lines
-  load_lu<R, FULL>(mg, pl, ic, j0, n2m, c0);
   {
     const double aw = mg.aw[ic], ae = mg.ae[ic];
     const bool hm = w > 0, hp = w + 1 < n0;
@@ -894,12 +894,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 
   // goup(0) + relax(1), first half: the neighbours 2w-1 and 2w+1 of line 2w
   // are the destinations of level-0 rows w-1 and w, prolonged as they are read
-  load_lu<R, FULL>(mg, pl, ia, j0, n2m, cA);
-  {
-    const double2 *ra = ROW(mg.resd + pl, ia);
-#pragma unroll
-    for (int r = 0; r < R; ++r) resA[r] = ra[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
-  }
+  // (cA and the level-1 residual resA are still in registers)
   {
     const double aw = mg.aw[ia], ae = mg.ae[ia];
     const bool hm = w > 0, plast = w + 1 == n0;
