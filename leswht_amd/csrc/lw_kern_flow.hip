@@ -1756,6 +1756,12 @@ __device__ inline long diag_size(const Geo &g, const IBComp *ib, int s) {
   return (long)g.n1m * g.n2m * g.n3m;
 }
 
+// The four bulk-temperature sums over the volume share one pass: the
+// workgroups of their four rows split the volume into 4*kDiagB chunks, and
+// each forms the partials of all four sums for its chunk (kDiagP partials
+// per sum; the other sums use the first kDiagB).
+constexpr int kDiagP = 4 * kDiagB;
+
 __global__ __launch_bounds__(256) void k_step_diag(Geo g, Fld f, IBComp u, IBComp v, IBComp w, DiagPlanes dp,
                                                    unsigned active, double re) {
   __shared__ double sh[4];
@@ -1763,17 +1769,39 @@ __global__ __launch_bounds__(256) void k_step_diag(Geo g, Fld f, IBComp u, IBCom
   if (!((active >> s) & 1u)) return;
   const IBComp ib[3] = {u, v, w};
   const long n = diag_size(g, ib, s);
+  if (s >= D_TFV) {
+    const int c = (s - D_TFV) * kDiagB + blockIdx.x;  // chunk 0 .. kDiagP-1
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long q = c * 256L + threadIdx.x; q < n; q += (long)kDiagP * 256) {
+      const int i = 1 + (int)(q % g.n1m), j = 1 + (int)((q / g.n1m) % g.n2m), k = 1 + (int)(q / ((long)g.n1m * g.n2m));
+      const long pc = P(g, i, j, k);
+      const bool solid = f.mask[pc] & M_SOLID_C;
+      const double cellvol = g.c2cx[i] * g.f2fy[j] * g.f2fz[k];
+      const double tv = f.t[pc] * cellvol;
+      a[0] += solid ? 0.0 : cellvol;  // D_TFV
+      a[1] += solid ? 0.0 : tv;       // D_TFI
+      a[2] += solid ? cellvol : 0.0;  // D_TSV
+      a[3] += solid ? tv : 0.0;       // D_TSI
+    }
+    for (int t = 0; t < 4; ++t) {
+      const double r = block_sum<256>(a[t], sh);
+      if (threadIdx.x == 0) f.dpart[(D_TFV + t) * kDiagP + c] = r;
+    }
+    return;
+  }
   double a = 0.0;
   for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)kDiagB * 256) a += diag_term(g, f, dp, ib, s, q, re);
   const double r = block_sum<256>(a, sh);
-  if (threadIdx.x == 0) f.dpart[s * kDiagB + blockIdx.x] = r;
+  if (threadIdx.x == 0) f.dpart[s * kDiagP + blockIdx.x] = r;
 }
 
 __global__ __launch_bounds__(256) void k_step_diag_sum(Fld f, unsigned active) {
   __shared__ double sh[4];
   for (int s = 0; s < D_NSUM; ++s) {
     if (!((active >> s) & 1u)) continue;
-    const double v = threadIdx.x < kDiagB ? f.dpart[s * kDiagB + threadIdx.x] : 0.0;
+    const int np = s >= D_TFV ? kDiagP : kDiagB;
+    double v = 0.0;
+    for (int q = threadIdx.x; q < np; q += 256) v += f.dpart[s * kDiagP + q];
     const double r = block_sum<256>(v, sh);
     if (threadIdx.x == 0) f.red[kDiagRed + s] = r;
   }
@@ -1785,11 +1813,11 @@ void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, con
   double bytes = 0.0;
   if (g_prof)
     for (int q = 0; q < D_NSUM; ++q)
-      if ((active >> q) & 1u) bytes += (q <= D_IBW ? 36.0 : q >= D_TFV ? 10.0 : 32.0) *
-                                       (q <= D_IBW ? (double)ib[q].nbody : q >= D_TFV ? pts(g) : 1e4);
+      if ((active >> q) & 1u)  // the four bulk-temperature sums: mask and T once (9 B per cell)
+        bytes += q <= D_IBW ? 36.0 * (double)ib[q].nbody : q == D_TFV ? 9.0 * pts(g) : q > D_TFV ? 0.0 : 32.0 * 1e4;
   LW_LAUNCH("k_step_diag", bytes, 0, k_step_diag, dim3(kDiagB, D_NSUM), dim3(256), 0, s, g, f, ib[0], ib[1], ib[2], dp,
             active, re);
-  LW_LAUNCH("k_step_diag_sum", D_NSUM * kDiagB * 8.0, 0, k_step_diag_sum, dim3(1), dim3(256), 0, s, f, active);
+  LW_LAUNCH("k_step_diag_sum", D_NSUM * kDiagP * 8.0, 0, k_step_diag_sum, dim3(1), dim3(256), 0, s, f, active);
 }
 
 

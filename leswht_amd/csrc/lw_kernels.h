@@ -43,7 +43,7 @@ struct Fld {
   // reduction scratch: red[0..63] doubles, partials[...] per block
   double *red, *partial;  // partial: 4 * kRedBlocks
   double *obc;            // outlet-plane reduction partials (8 * 64)
-  double *dpart;          // per-step diagnostic partials (D_NSUM * 128)
+  double *dpart;          // per-step diagnostic partials (D_NSUM * 512)
   unsigned long long *redmax;  // [0] cfl, [1] dvmax, [2] qmmax
 };
 

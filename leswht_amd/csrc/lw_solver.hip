@@ -341,7 +341,7 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.red = h->dalloc<double>(64);
   f.partial = h->dalloc<double>(4 * kRedBlocks);
   f.obc = h->dalloc<double>(8 * 64);
-  f.dpart = h->dalloc<double>(D_NSUM * 128);
+  f.dpart = h->dalloc<double>(D_NSUM * 512);
   h->hist = h->dalloc<double>(32);
   f.redmax = h->dalloc<unsigned long long>(8);
   if (h->les) {
