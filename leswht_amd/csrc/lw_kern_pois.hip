@@ -33,6 +33,15 @@ __device__ inline double2 cfma(double a, double2 b, double2 c) { return make_dou
 // stored table, so the same bits (ak3(1) = 0 and x - 0 = x)
 __device__ inline double ac_of(double awe, double as, double an, double ak) { return -((awe + as) + an) - ak; }
 
+// an LDS read of a row that may not exist: it reads one that does and
+// applies the condition to the value, so every read of a stage is one
+// unconditional 16-byte access and they issue together (a read under a
+// branch is split into two 8-byte reads, each waited for on its own)
+__device__ inline double2 lds_if(bool on, const double2 *p_on, const double2 *p_off) {
+  const double2 v = *(on ? p_on : p_off);
+  return on ? v : make_double2(0.0, 0.0);
+}
+
 #define ROW(base, i_) ((base) + (long)(i_) * mg.LDJ)
 
 // prefix step of the affine maps (M, C) of the forward substitution: this
@@ -787,7 +796,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double2 gmv = L1[slot(la, r)];
-      const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+      const double2 gpv = lds_if(hp, &L1[slot(lb + 1, r)], &L1[slot(lb, r)]);
       rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
     }
     tri_regs<R, FULL>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
@@ -808,14 +817,14 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     {
       const double aw = mg.aw[ia], ae = mg.ae[ia], awe = mg.awe[ia];
       double2 *out = ROW(mg.resd + pl, ia);
-      const double2 lo = lane > 0 ? L1[(la * R + R - 1) * 64 + lane - 1] : z2;
-      const double2 hi = lane < 63 ? L1[(la * R) * 64 + lane + 1] : z2;
+      const double2 lo = lds_if(lane > 0, &L1[(la * R + R - 1) * 64 + lane - 1], &L1[(la * R + R - 1) * 64 + lane]);
+      const double2 hi = lds_if(lane < 63, &L1[(la * R) * 64 + lane + 1], &L1[(la * R) * 64 + lane]);
       double2 gi[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(la, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double2 gmv = la > 0 ? L1[slot(la - 1, r)] : z2;
+        const double2 gmv = lds_if(la > 0, &L1[slot(la - 1, r)], &L1[slot(la, r)]);
         const double2 gpv = L1[slot(lb, r)];
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
         double2 u = resA[r];
@@ -833,15 +842,15 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       const double aw = mg.aw[ib], ae = mg.ae[ib], awe = mg.awe[ib];
       double2 *out = ROW(mg.resd + pl, ib);
       const bool hp = lb + 1 < n1;
-      const double2 lo = lane > 0 ? L1[(lb * R + R - 1) * 64 + lane - 1] : z2;
-      const double2 hi = lane < 63 ? L1[(lb * R) * 64 + lane + 1] : z2;
+      const double2 lo = lds_if(lane > 0, &L1[(lb * R + R - 1) * 64 + lane - 1], &L1[(lb * R + R - 1) * 64 + lane]);
+      const double2 hi = lds_if(lane < 63, &L1[(lb * R) * 64 + lane + 1], &L1[(lb * R) * 64 + lane]);
       double2 gi[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) gi[r] = L1[slot(lb, r)];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double2 gmv = L1[slot(la, r)];
-        const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+        const double2 gpv = lds_if(hp, &L1[slot(lb + 1, r)], &L1[slot(lb, r)]);
         const double2 dn = r > 0 ? gi[r > 0 ? r - 1 : 0] : lo, up = r < R - 1 ? gi[r < R - 1 ? r + 1 : 0] : hi;
         double2 v = resB[r];
         v = csub(v, rmul(aw, gmv));
@@ -867,8 +876,8 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         } else {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double2 gmv = hm ? L0[slot(w - 1, r)] : z2;
-            const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
+            const double2 gmv = lds_if(hm, &L0[slot(w - 1, r)], &L0[slot(w, r)]);
+            const double2 gpv = lds_if(hp, &L0[slot(w + 1, r)], &L0[slot(w, r)]);
             rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(res0[r], rmul(1.0, cadd(rmul(aw, gmv), rmul(ae, gpv)))));
           }
         }
@@ -881,7 +890,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const double2 gmv = L0[slot(w - 1, r)];
-          const double2 gpv = hp ? L0[slot(w + 1, r)] : z2;
+          const double2 gpv = lds_if(hp, &L0[slot(w + 1, r)], &L0[slot(w, r)]);
           rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(res0[r], rmul(aw, gmv)), rmul(ae, gpv)));
         }
         tri_regs<R, FULL>(rr, c0.bt, asj, c0.gm, j0, n2m, lane, x);
@@ -903,9 +912,12 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       double2 vm = z2;
-      if (hm) vm = cadd(cadd(L1[slot(la - 1, r)], rmul(pm1, L0[slot(w - 1, r)])), rmul(pm2, L0[slot(w, r)]));
+      const double2 l1m = lds_if(hm, &L1[slot(la - 1, r)], &L1[slot(la, r)]);
+      const double2 l0m = lds_if(hm, &L0[slot(w - 1, r)], &L0[slot(w, r)]), l0 = L0[slot(w, r)];
+      const double2 l0p = lds_if(!plast, &L0[slot(w + 1, r)], &L0[slot(w, r)]);
+      if (hm) vm = cadd(cadd(l1m, rmul(pm1, l0m)), rmul(pm2, l0));
       double2 vp = cadd(L1[slot(lb, r)], rmul(pp1, L0[slot(w, r)]));
-      if (!plast) vp = cadd(vp, rmul(pp2, L0[slot(w + 1, r)]));
+      if (!plast) vp = cadd(vp, rmul(pp2, l0p));
       rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(resA[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp)))));
       __builtin_amdgcn_sched_barrier(0);  // five rows of reads per row: one row at a time
     }
@@ -935,7 +947,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double2 gmv = L1[slot(la, r)];
-      const double2 gpv = hp ? L1[slot(lb + 1, r)] : z2;
+      const double2 gpv = lds_if(hp, &L1[slot(lb + 1, r)], &L1[slot(lb, r)]);
       rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
     }
     tri_regs<R, FULL>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
