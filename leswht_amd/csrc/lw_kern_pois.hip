@@ -756,6 +756,12 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   const int ia = b1 + la, ib = ia + 1, ic = b0 + w;
   const int j0 = 1 + lane * R;
   const double2 z2 = make_double2(0.0, 0.0);
+  // every stage's line coefficients requested up front (uniform: scalar loads)
+  const double awA = mg.aw[ia], aeA = mg.ae[ia], weA = mg.awe[ia];
+  const double awB = mg.aw[ib], aeB = mg.ae[ib], weB = mg.awe[ib];
+  const double aw0 = mg.aw[ic], ae0 = mg.ae[ic], cr1 = mg.cor1[ic], cr2 = mg.cor2[ic], akv = mg.ak3[kv];
+  const double ci1m = w > 0 ? mg.coi1[ic - 1] : 0.0, ci2m = w > 0 ? mg.coi2[ic - 1] : 0.0;
+  const double ci1 = mg.coi1[ic], ci2 = mg.coi2[ic];
 
   // registers held through the kernel: as, the level-0 line's right-hand
   // side and LU factors. The level-1 residuals go to resd, as godown leaves
@@ -791,7 +797,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 
   // relax(1), second half: resd - aw*g(i-1) - ae*g(i+1)
   {
-    const double aw = mg.aw[ib], ae = mg.ae[ib];
+    const double aw = awB, ae = aeB;
     const bool hp = lb + 1 < n1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -810,12 +816,12 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // time: the scheduling barrier keeps the second line's loads from being
   // hoisted over the first line's arithmetic.
   {
-    const double c1 = mg.cor1[ic], c2 = mg.cor2[ic], ak = mg.ak3[kv];
+    const double c1 = cr1, c2 = cr2, ak = akv;
     double anj[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
     {
-      const double aw = mg.aw[ia], ae = mg.ae[ia], awe = mg.awe[ia];
+      const double aw = awA, ae = aeA, awe = weA;
       double2 *out = ROW(mg.resd + pl, ia);
       const double2 lo = lds_if(lane > 0, &L1[(la * R + R - 1) * 64 + lane - 1], &L1[(la * R + R - 1) * 64 + lane]);
       const double2 hi = lds_if(lane < 63, &L1[(la * R) * 64 + lane + 1], &L1[(la * R) * 64 + lane]);
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     }
     __builtin_amdgcn_sched_barrier(0);
     {
-      const double aw = mg.aw[ib], ae = mg.ae[ib], awe = mg.awe[ib];
+      const double aw = awB, ae = aeB, awe = weB;
       double2 *out = ROW(mg.resd + pl, ib);
       const bool hp = lb + 1 < n1;
       const double2 lo = lds_if(lane > 0, &L1[(lb * R + R - 1) * 64 + lane - 1], &L1[(lb * R + R - 1) * 64 + lane]);
@@ -866,7 +872,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
 
   // relax(0, NBLI): first half on even lines, second half on odd lines
   {
-    const double aw = mg.aw[ic], ae = mg.ae[ic];
+    const double aw = aw0, ae = ae0;
     const bool hm = w > 0, hp = w + 1 < n0;
     for (int it = 0; it < mg.nbli; ++it) {
       if ((w & 1) == 0) {
@@ -905,10 +911,10 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // are the destinations of level-0 rows w-1 and w, prolonged as they are read
   // (cA and the level-1 residual resA are still in registers)
   {
-    const double aw = mg.aw[ia], ae = mg.ae[ia];
+    const double aw = awA, ae = aeA;
     const bool hm = w > 0, plast = w + 1 == n0;
-    const double pm1 = hm ? mg.coi1[ic - 1] : 0.0, pm2 = hm ? mg.coi2[ic - 1] : 0.0;
-    const double pp1 = mg.coi1[ic], pp2 = mg.coi2[ic];
+    const double pm1 = ci1m, pm2 = ci2m;
+    const double pp1 = ci1, pp2 = ci2;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       double2 vm = z2;
@@ -942,7 +948,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     for (int r = 0; r < R; ++r) resB[r] = rb[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
   }
   {
-    const double aw = mg.aw[ib], ae = mg.ae[ib];
+    const double aw = awB, ae = aeB;
     const bool hp = lb + 1 < n1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
