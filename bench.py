@@ -66,6 +66,58 @@ def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
         shutil.rmtree(root, ignore_errors=True)
 
 
+def serve_bench(args) -> dict:
+    """The served form: a seeded Poisson-arrival stream of independent cases
+    (sizes 64..256 x 64..192 x 8..32, Re 40..400, Pr 0.71 or 7, circle,
+    square or ellipse, plus a share of 384x256x64 LES cases) through the
+    native request loop with a memory budget. Then the same cases submitted
+    together, once through one slot (one case at a time, the plain form) and
+    once through all slots, for the aggregate-throughput ratio."""
+    from leswht_amd import serve
+    cases = serve.draw_cases(args.serve, seed=args.seed, rate=args.rate,
+                             steps=tuple(int(round(v * args.step_scale)) for v in (200, 2000)),
+                             large_share=0.1,
+                             large_steps=tuple(int(round(v * args.step_scale)) for v in (200, 2000)))
+    root = tempfile.mkdtemp(prefix="lw_serve_")
+    try:
+        t0 = time.time()
+        dirs = serve.prepare(cases, root)
+        t_prep = time.time() - t0
+        print("prepared %d case directories in %.1f s" % (len(cases), t_prep), file=sys.stderr, flush=True)
+        budget = int(args.budget_gb * (1 << 30))
+        stream = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=True)
+        print("stream: p50 %.3f s p99 %.3f s, %.3e cell-updates/s" % (
+            stream["p50"], stream["p99"], stream["throughput"]), file=sys.stderr, flush=True)
+        closed = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=False, write_fields=False)
+        print("all at once, %d slots: %.3e cell-updates/s" % (args.slots, closed["throughput"]),
+              file=sys.stderr, flush=True)
+        single = serve.run_stream(cases, dirs, budget, 1, honour_arrivals=False, write_fields=False)
+        print("all at once, 1 slot: %.3e cell-updates/s" % single["throughput"], file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    n_large = sum(c.large for c in cases)
+    return {
+        "metric": "p50/p99 time-to-solution of served cases; aggregate cell-updates/s (1 GPU)",
+        "value": stream["throughput"],
+        "unit": "cell-updates/s",
+        "p50_s": stream["p50"],
+        "p99_s": stream["p99"],
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded case stream: grids, bodies, IB tables and initial fields generated)",
+        "config": {"workload": "Poisson-arrival stream of %d cases at %.1f/s, %d of them 384x256x64 LES; "
+                               "step budgets U{200..2000} x %.3g" % (len(cases), args.rate, n_large,
+                                                                    args.step_scale),
+                   "slots": args.slots, "budget_gb": args.budget_gb, "seed": args.seed,
+                   "cell_updates": stream["cell_updates"], "prep_s": round(t_prep, 1),
+                   "peak_running": stream["info"]["peak_running"],
+                   "peak_reserved_gb": round(stream["info"]["peak_reserved_bytes"] / (1 << 30), 2)},
+        "closed_batch": {"slots": args.slots, "value": closed["throughput"], "one_slot": single["throughput"],
+                         "ratio": closed["throughput"] / single["throughput"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +128,18 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times (the known-size "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters), then exit")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="run the served form on a stream of this many cases instead of the time step")
+    ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
+    ap.add_argument("--step-scale", type=float, default=0.1,
+                    help="served form: factor on the step budgets U{200..2000}")
+    ap.add_argument("--slots", type=int, default=8, help="served form: cases on the device at once")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="served form: admission budget")
+    ap.add_argument("--seed", type=int, default=20261203)
     args = ap.parse_args()
+    if args.serve:
+        print(json.dumps(serve_bench(args)))
+        return
     if args.copy_probe_mib:
         import torch
         n = args.copy_probe_mib * (1 << 20) // 8

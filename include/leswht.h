@@ -114,12 +114,18 @@ typedef struct lw_history {
 
 typedef struct lw_handle lw_handle;
 
-/* body level set used inside the step (convbc, nutzero, wallbc_t):
- * kind 0 = circular cylinder sqrt(x^2+y^2)-p0 (funcbody_cylinder.f90:13) */
+/* body level set used inside the step (convbc, nutzero, wallbc_t), > 0 in
+ * fluid; the reference compiles one funcbody into the solver (Makefile:104-105):
+ * kind 0 = circular cylinder sqrt(x^2+y^2)-p0 (funcbody_cylinder.f90:13);
+ * kind 1 = square max(|x'|,|y'|)-p0 with (x',y') rotated by angle p1;
+ * kind 2 = ellipse (sqrt((x'/p0)^2+(y'/p1)^2)-1)*min(p0,p1), angle p2 */
 typedef struct lw_body {
   int64_t kind;
   double p[4];
 } lw_body;
+
+/* the level set of a body at one point (NULL: the cylinder of radius 0.5) */
+double lw_body_eval(const lw_body *b, double x, double y, double z);
 
 int lw_create(const lw_settings *s, const lw_grid *g, const lw_tables *t,
               const lw_body *body, lw_handle **out);
@@ -175,6 +181,55 @@ int lw_profile_get(const lw_handle *h, int64_t idx, char *name, int64_t cap,
                    int64_t *launches, double *ms_total, double *bytes_total);
 
 const char *lw_last_error(void);
+
+/* Upper bound of the device bytes lw_create allocates for a case of
+ * n1m x n2m x n3m cells with these settings (lw_device_bytes gives the
+ * actual figure once created). Negative on bad sizes. No counterpart in the
+ * reference, which allocates host arrays in alloinit (misc_init.f90:285-309). */
+int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m, int64_t n3m);
+/* settings.input, boundary.input and ibmpre_prdic.bin (readsettings/readbcs,
+ * misc_init.f90:19-158) and the cell counts of the grid file of a case
+ * directory, without reading tables or fields */
+int lw_probe_dir(const char *solver_dir, lw_settings *s, int64_t nm[3]);
+
+/* ---- request loop over independent cases on one GPU ----------------------
+ * The reference runs one case per solver_exec process (run_solver.sh:41); it
+ * has no counterpart to this. A server owns `slots` worker threads. Each
+ * request is a reference case directory (as for lw_create_from_dir) and a
+ * step count; requests are admitted first-come first-served when a slot is
+ * free and lw_estimate_device_bytes of the case fits what is left of the
+ * budget. An admitted case is created on the device, stepped on its own
+ * streams beside the other admitted cases, its final field is written with
+ * lw_write_field to fld_out (skipped when NULL or ""), and its memory freed.
+ * Times are seconds since lw_server_create. */
+#define LW_CASE_DONE 0
+#define LW_CASE_QUEUED 1
+#define LW_CASE_RUNNING 2
+typedef struct lw_case_result {
+  int64_t ticket;
+  int64_t status;          /* LW_CASE_*, or the negative code the case failed with */
+  int64_t cells, steps_done, mg_cycles_sum;
+  int64_t reserved_bytes;  /* the admission estimate */
+  int64_t device_bytes;    /* what lw_create allocated */
+  double t_submit, t_admit, t_done;
+  double time, dt;         /* simulated time and dt after the last step */
+} lw_case_result;
+typedef struct lw_server_info {
+  int64_t submitted, queued, running, slots;
+  int64_t reserved_bytes, peak_reserved_bytes, peak_running, budget_bytes;
+  double elapsed;
+} lw_server_info;
+typedef struct lw_server lw_server;
+int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out);
+int lw_server_submit(lw_server *s, const char *solver_dir, const lw_body *body, int64_t nsteps,
+                     const char *fld_out, int64_t *ticket);
+/* current state of a request (non-blocking) */
+int lw_server_poll(lw_server *s, int64_t ticket, lw_case_result *res);
+/* blocks until the request has finished or failed */
+int lw_server_wait(lw_server *s, int64_t ticket, lw_case_result *res);
+int lw_server_stats(lw_server *s, lw_server_info *info);
+/* runs every queued request to the end, then stops the workers */
+int lw_server_destroy(lw_server *s);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,23 @@ BODIES: Dict[str, Callable] = {
     "ellipse": body_ellipse,
 }
 
+# the solver library's lw_body kinds and the default parameters of each body
+BODY_KINDS: Dict[str, Tuple[int, Tuple[float, ...]]] = {
+    "cylinder": (0, (0.5,)),
+    "square": (1, (0.5, 0.0)),
+    "ellipse": (2, (0.5, 0.25, 0.0)),
+}
+
+
+def body_params(name: str, args: Tuple = ()) -> Tuple[int, Tuple[float, float, float, float]]:
+    """(kind, p[0..3]) of the library's lw_body for a named body with the
+    positional arguments its level-set function takes."""
+    kind, defaults = BODY_KINDS[name]
+    vals = list(args) + list(defaults[len(args):])
+    if len(vals) > len(defaults):
+        raise ValueError("too many arguments for body %r" % name)
+    return kind, tuple(float(v) for v in vals) + (0.0,) * (4 - len(vals))
+
 # ---------------------------------------------------------------------------
 # grid lines
 # ---------------------------------------------------------------------------
@@ -1065,4 +1082,5 @@ def case_view(case: Case) -> dict:
     st = dataclasses.replace(s, xprdic=0, yprdic=0, zprdic=1, iintp=1)
     fields = apply_prefld(fl, st, g)
     return dict(settings=st, grid=g, tables=tabs, nutzero=nz, fields=fields,
-                ihist=0, m=0, time=0.0, dt=s.dt_size)
+                ihist=0, m=0, time=0.0, dt=s.dt_size,
+                body=body_params(case.spec.body, case.spec.body_args))

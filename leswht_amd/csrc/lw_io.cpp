@@ -14,6 +14,9 @@
 
 #include "../../include/leswht.h"
 
+// the message lw_last_error returns (lw_solver.hip)
+void lw_set_last_error(const std::string &msg);
+
 namespace {
 
 thread_local std::string io_err;
@@ -381,7 +384,29 @@ extern "C" int lw_create_from_dir(const char *solver_dir, const lw_body *body, l
     return rc;
   } catch (const IoError &e) {
     io_err = e.msg;
+    lw_set_last_error(e.msg);
     fprintf(stderr, "leswht: %s\n", e.msg.c_str());
+    return LW_ERR_IO;
+  }
+}
+
+// settings.input, boundary.input and the grid header of a case directory,
+// without the tables and the field: what an admission decision needs
+extern "C" int lw_probe_dir(const char *solver_dir, lw_settings *s, int64_t nm[3]) {
+  if (!solver_dir || !s || !nm) return LW_ERR_ARG;
+  try {
+    Case c;
+    const std::string dir(solver_dir);
+    read_settings(c, dir);
+    ListReader r(join(dir, c.gridfile));
+    r.ints(3);
+    auto n = r.ints(3);
+    *s = c.s;
+    nm[0] = n[0]; nm[1] = n[1]; nm[2] = n[2];
+    return LW_OK;
+  } catch (const IoError &e) {
+    io_err = e.msg;
+    lw_set_last_error(e.msg);
     return LW_ERR_IO;
   }
 }

@@ -105,6 +105,16 @@ void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf);
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1,
                        int alp, int scale_slot, const lw_settings &st);
 
+// ---- time-averaged statistics (lw_kern_stats.hip) ----
+// running sums of field_avg (misc_hist.f90:287-472), interior arrays:
+// 0-2 u, v, w; 3-8 uiuj (11, 12, 13, 22, 23, 33); 9 p; 10 p^2; 11-13 vorticity;
+// 14-19 its products (xx, xy, xz, yy, yz, zz); 20 S:S; 21 T; 22 T^2
+constexpr int kAvgArrays = 23;
+struct AvgBuf {
+  double *a[kAvgArrays];
+};
+void launch_field_avg(hipStream_t s, const Geo &g, const Fld &f, const AvgBuf &b, double dt, int heat);
+
 // ---- Poisson (lw_kern_pois.hip) ----
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);

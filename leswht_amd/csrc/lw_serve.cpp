@@ -1,0 +1,217 @@
+// A request loop over independent cases sharing one GPU.
+//
+// The reference runs one case per process (run_solver.sh:41: solver_exec with
+// a case directory as cwd). Here a server takes such case directories as
+// requests while others run: each request is admitted when a slot is free and
+// its estimated device footprint fits the memory budget, then a worker thread
+// creates its handle (fields resident in HBM for the whole run), steps it on
+// the handle's own streams, writes the final fld file and frees the memory.
+// Kernels of admitted cases overlap on the device because every handle
+// launches on its own non-blocking streams from its own host thread; each case
+// runs exactly the kernels and the arithmetic it would run alone, so its
+// results do not depend on the mix.
+//
+// Admission is first-come first-served: the oldest queued request is admitted
+// next, and it waits (blocking those behind it) until enough of the budget
+// is free, so a large case is never starved by a stream of small ones.
+#include <stdint.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/leswht.h"
+
+void lw_set_last_error(const std::string &msg);
+
+namespace {
+
+struct Request {
+  std::string dir, fld_out;
+  lw_body body{};
+  int64_t nsteps = 0;
+  lw_case_result res{};
+  std::string err;
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct lw_server {
+  int64_t budget = 0;
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> queue;
+  std::vector<std::unique_ptr<Request>> all;  // index = ticket
+  int64_t reserved = 0, running = 0, peak_reserved = 0, peak_running = 0;
+  bool stopping = false;
+  std::vector<std::thread> workers;
+
+  void run_case(Request *r) {
+    lw_handle *h = nullptr;
+    int rc = lw_create_from_dir(r->dir.c_str(), &r->body, &h);
+    if (rc == LW_OK) {
+      r->res.device_bytes = lw_device_bytes(h);
+      lw_step_stats st{};
+      for (int64_t n = 0; n < r->nsteps && rc == LW_OK; ++n) {
+        rc = lw_step(h, 1, &st);
+        if (rc == LW_OK) {
+          r->res.steps_done = n + 1;
+          r->res.mg_cycles_sum += st.mg_cycles_sum;
+          r->res.time = st.time;
+          r->res.dt = st.dt;
+        }
+      }
+      if (rc == LW_OK && !r->fld_out.empty()) rc = lw_write_field(h, r->fld_out.c_str());
+    }
+    if (rc != LW_OK) r->err = lw_last_error();
+    if (h) lw_destroy(h);
+    r->res.status = rc == LW_OK ? LW_CASE_DONE : rc;
+  }
+
+  void worker() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_work.wait(lk, [&] {
+        return (!queue.empty() && reserved + queue.front()->res.reserved_bytes <= budget) ||
+               (stopping && queue.empty());
+      });
+      if (queue.empty()) return;
+      Request *r = queue.front();
+      queue.pop_front();
+      reserved += r->res.reserved_bytes;
+      running += 1;
+      if (reserved > peak_reserved) peak_reserved = reserved;
+      if (running > peak_running) peak_running = running;
+      r->res.status = LW_CASE_RUNNING;
+      r->res.t_admit = seconds_since(t0);
+      lk.unlock();
+      run_case(r);
+      lk.lock();
+      r->res.t_done = seconds_since(t0);
+      reserved -= r->res.reserved_bytes;
+      running -= 1;
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out) {
+  if (!out || mem_budget_bytes <= 0 || slots < 1 || slots > 64) {
+    lw_set_last_error("lw_server_create: budget must be > 0 and slots in 1..64");
+    return LW_ERR_ARG;
+  }
+  auto *s = new lw_server();
+  s->budget = mem_budget_bytes;
+  s->t0 = std::chrono::steady_clock::now();
+  for (int64_t q = 0; q < slots; ++q) s->workers.emplace_back([s] { s->worker(); });
+  *out = s;
+  return LW_OK;
+}
+
+int lw_server_submit(lw_server *s, const char *solver_dir, const lw_body *body, int64_t nsteps,
+                     const char *fld_out, int64_t *ticket) {
+  if (!s || !solver_dir || !ticket || nsteps < 0) {
+    lw_set_last_error("lw_server_submit: null argument or negative step count");
+    return LW_ERR_ARG;
+  }
+  auto r = std::make_unique<Request>();
+  r->res.t_submit = seconds_since(s->t0);
+  r->dir = solver_dir;
+  r->fld_out = fld_out ? fld_out : "";
+  if (body) r->body = *body; else r->body.p[0] = 0.5;
+  r->nsteps = nsteps;
+  lw_settings st{};
+  int64_t nm[3];
+  int rc = lw_probe_dir(solver_dir, &st, nm);
+  if (rc != LW_OK) return rc;
+  const int64_t est = lw_estimate_device_bytes(&st, nm[0], nm[1], nm[2]);
+  if (est < 0) return (int)est;
+  if (est > s->budget) {
+    lw_set_last_error("lw_server_submit: the case needs about " + std::to_string(est >> 20) +
+                      " MiB, more than the whole budget");
+    return LW_ERR_ARG;
+  }
+  r->res.cells = nm[0] * nm[1] * nm[2];
+  r->res.reserved_bytes = est;
+  r->res.status = LW_CASE_QUEUED;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) {
+    lw_set_last_error("lw_server_submit: server is shutting down");
+    return LW_ERR_ARG;
+  }
+  r->res.ticket = (int64_t)s->all.size();
+  *ticket = r->res.ticket;
+  s->queue.push_back(r.get());
+  s->all.push_back(std::move(r));
+  s->cv_work.notify_all();
+  return LW_OK;
+}
+
+int lw_server_poll(lw_server *s, int64_t ticket, lw_case_result *res) {
+  if (!s || !res) return LW_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (ticket < 0 || ticket >= (int64_t)s->all.size()) {
+    lw_set_last_error("lw_server_poll: unknown ticket");
+    return LW_ERR_ARG;
+  }
+  *res = s->all[ticket]->res;
+  if (res->status < 0) lw_set_last_error(s->all[ticket]->err);
+  return LW_OK;
+}
+
+int lw_server_wait(lw_server *s, int64_t ticket, lw_case_result *res) {
+  if (!s || !res) return LW_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (ticket < 0 || ticket >= (int64_t)s->all.size()) {
+    lw_set_last_error("lw_server_wait: unknown ticket");
+    return LW_ERR_ARG;
+  }
+  Request *r = s->all[ticket].get();
+  s->cv_done.wait(lk, [&] { return r->res.status != LW_CASE_QUEUED && r->res.status != LW_CASE_RUNNING; });
+  *res = r->res;
+  if (res->status < 0) lw_set_last_error(r->err);
+  return LW_OK;
+}
+
+int lw_server_stats(lw_server *s, lw_server_info *info) {
+  if (!s || !info) return LW_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  info->submitted = (int64_t)s->all.size();
+  info->queued = (int64_t)s->queue.size();
+  info->running = s->running;
+  info->reserved_bytes = s->reserved;
+  info->peak_reserved_bytes = s->peak_reserved;
+  info->peak_running = s->peak_running;
+  info->budget_bytes = s->budget;
+  info->slots = (int64_t)s->workers.size();
+  info->elapsed = seconds_since(s->t0);
+  return LW_OK;
+}
+
+int lw_server_destroy(lw_server *s) {
+  if (!s) return LW_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    s->cv_work.notify_all();
+  }
+  for (auto &t : s->workers) t.join();
+  delete s;
+  return LW_OK;
+}
+
+}  // extern "C"

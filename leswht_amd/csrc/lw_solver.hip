@@ -201,13 +201,23 @@ struct lw_handle {
   }
 };
 
+// The body's level set, > 0 in fluid, evaluated on the host at set-up (solid
+// masks, outlet and inlet cells, zero-viscosity cells). Kind 0 is the
+// reference's cylinder (geometry/funcbody_example/funcbody_cylinder.f90:13);
+// kinds 1 (square: half side p[0], angle p[1]) and 2 (ellipse: semi-axes
+// p[0], p[1], angle p[2]) are funcbody examples of the same form, evaluated
+// with the case builder's operations so the masks agree with its IB tables.
 static double funcbody(const lw_body &b, double x, double y, double z) {
   (void)z;
-  switch (b.kind) {
-    default:
-      // geometry/funcbody_example/funcbody_cylinder.f90:13
-      return sqrt(x * x + y * y) - b.p[0];
+  if (b.kind == 1 || b.kind == 2) {
+    const double ang = b.kind == 1 ? b.p[1] : b.p[2];
+    const double c = cos(ang), s = sin(ang);
+    const double xr = c * x + s * y, yr = (-s) * x + c * y;
+    if (b.kind == 1) return fmax(fabs(xr), fabs(yr)) - b.p[0];
+    const double ta = xr / b.p[0], tb = yr / b.p[1];
+    return (sqrt(ta * ta + tb * tb) - 1.0) * (b.p[0] < b.p[1] ? b.p[0] : b.p[1]);
   }
+  return sqrt(x * x + y * y) - b.p[0];
 }
 
 static std::vector<double> vd(const double *src, int64_t lo, int64_t hi) {
@@ -634,6 +644,11 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
 extern "C" {
 
 const char *lw_last_error(void) { return g_err.c_str(); }
+}  // extern "C"
+
+void lw_set_last_error(const std::string &msg) { g_err = msg; }
+
+extern "C" {
 
 int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body, lw_handle **out) {
   if (!s || !gr || !t || !out) return fail(LW_ERR_ARG, "null argument");
@@ -641,6 +656,8 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
   lw_handle *h = nullptr;
   try {
     check_scope(s, gr);
+    if (body && (body->kind < 0 || body->kind > 2 || !(body->p[0] > 0.0) || (body->kind == 2 && !(body->p[1] > 0.0))))
+      return fail(LW_ERR_ARG, "body: kind must be 0 (cylinder), 1 (square) or 2 (ellipse) with positive sizes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(LW_ERR_DEVICE, "no HIP device");
     h = new lw_handle();
@@ -711,6 +728,32 @@ int lw_get_dims(const lw_handle *h, int64_t n[3]) {
 }
 
 int64_t lw_device_bytes(const lw_handle *h) { return h ? h->bytes : 0; }
+
+double lw_body_eval(const lw_body *b, double x, double y, double z) {
+  lw_body c{};
+  c.p[0] = 0.5;
+  return funcbody(b ? *b : c, x, y, z);
+}
+
+// Upper bound of what lw_create allocates for these sizes, from the same
+// array counts as the allocations above: 34 padded FP64 arrays (30 without T),
+// 25 interior arrays with the dynamic SGS model, the multigrid plane stacks
+// (gg, resd, gd, rsu over all levels, pc, rhs, pc2 on the top level, bet) and
+// a margin for the masks, IB tables, 1-D metrics and reduction buffers.
+int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m, int64_t n3m) {
+  if (!s || n1m < 2 || n2m < 2 || n3m < 2) return fail(LW_ERR_ARG, "lw_estimate_device_bytes: bad sizes");
+  const int64_t np = (n1m + 2) * (n2m + 2) * (n3m + 2), ni = n1m * n2m * n3m;
+  const int64_t heat = s->ihtrans == 1 ? 1 : 0;
+  int64_t b = (30 + 4 * heat) * np * 8 + 2 * np;  // fields, mask and a byte of slack per cell
+  if (s->iles == 1) b += 25 * ni * 8 + ni * 8;       // SGS work arrays and nwall
+  const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
+  const int64_t n3mh = n3m / 2 + 1;
+  b += n3mh * (4 * plane + 3 * pcplane) * 16 + n3mh * plane * 8;
+  b += 8 * (n2m + 2) * (n3m + 2) * 8;                // outlet planes
+  // IB tables: at most every cell of a component a forcing point, but bodies
+  // cover a small share; 64 MB covers tables, metrics and partial sums
+  return b + b / 32 + ((int64_t)64 << 20);
+}
 
 int lw_set_fields(lw_handle *h, const double *u, const double *v, const double *w, const double *p, const double *t,
                   int64_t ihist, int64_t m, double time, double dt) {

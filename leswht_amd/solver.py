@@ -87,10 +87,32 @@ class lw_history(C.Structure):
         return out
 
 
+class lw_case_result(C.Structure):
+    _fields_ = [(n, _I) for n in ("ticket", "status", "cells", "steps_done", "mg_cycles_sum",
+                                  "reserved_bytes", "device_bytes")] + \
+        [(n, _D) for n in ("t_submit", "t_admit", "t_done", "time", "dt")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class lw_server_info(C.Structure):
+    _fields_ = [(n, _I) for n in ("submitted", "queued", "running", "slots", "reserved_bytes",
+                                  "peak_reserved_bytes", "peak_running", "budget_bytes")] + \
+        [("elapsed", _D)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+CASE_DONE, CASE_QUEUED, CASE_RUNNING = 0, 1, 2
+
 EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw_get_fields",
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
-           "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error")
+           "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
+           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_server_create", "lw_server_submit",
+           "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
 
 _lib = None
 
@@ -126,6 +148,17 @@ def load_library(path: str = LIB_PATH):
     L.lw_profile_count.restype = _I
     L.lw_profile_get.argtypes = [H, _I, C.c_char_p, _I, _PI, _PD, _PD]
     L.lw_last_error.restype = C.c_char_p
+    L.lw_estimate_device_bytes.argtypes = [C.POINTER(lw_settings), _I, _I, _I]
+    L.lw_estimate_device_bytes.restype = _I
+    L.lw_probe_dir.argtypes = [C.c_char_p, C.POINTER(lw_settings), _PI]
+    L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
+    L.lw_body_eval.restype = _D
+    L.lw_server_create.argtypes = [_I, _I, C.POINTER(H)]
+    L.lw_server_submit.argtypes = [H, C.c_char_p, C.POINTER(lw_body), _I, C.c_char_p, _PI]
+    L.lw_server_poll.argtypes = [H, _I, C.POINTER(lw_case_result)]
+    L.lw_server_wait.argtypes = [H, _I, C.POINTER(lw_case_result)]
+    L.lw_server_stats.argtypes = [H, C.POINTER(lw_server_info)]
+    L.lw_server_destroy.argtypes = [H]
     _lib = L
     return L
 
@@ -157,6 +190,74 @@ def settings_struct(s) -> lw_settings:
     return st
 
 
+def body_struct(body) -> lw_body:
+    """lw_body from (kind, params); None is the cylinder of radius 0.5."""
+    if body is None:
+        return lw_body(0, (_D * 4)(0.5, 0, 0, 0))
+    kind, p = body
+    p = tuple(p) + (0.0,) * (4 - len(p))
+    return lw_body(int(kind), (_D * 4)(*p[:4]))
+
+
+def estimate_device_bytes(settings, n1m: int, n2m: int, n3m: int) -> int:
+    """Upper bound of a case's device footprint (lw_estimate_device_bytes)."""
+    L = load_library()
+    st = settings_struct(settings)
+    b = L.lw_estimate_device_bytes(C.byref(st), n1m, n2m, n3m)
+    if b < 0:
+        _check(int(b))
+    return int(b)
+
+
+class Server:
+    """Request loop over independent cases on one GPU (lw_server_*): case
+    directories are submitted as they arrive, admitted first-come first-served
+    by free slot and memory budget, run concurrently on their own streams,
+    and written to ``fld_out`` when their steps are done."""
+
+    def __init__(self, budget_bytes: int, slots: int):
+        L = load_library()
+        self._s = C.c_void_p()
+        _check(L.lw_server_create(int(budget_bytes), int(slots), C.byref(self._s)))
+
+    def submit(self, solver_dir: str, nsteps: int, fld_out: str = None, body=None) -> int:
+        t = _I()
+        b = body_struct(body)
+        _check(load_library().lw_server_submit(self._s, solver_dir.encode(), C.byref(b), int(nsteps),
+                                               fld_out.encode() if fld_out else None, C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int) -> dict:
+        r = lw_case_result()
+        _check(load_library().lw_server_poll(self._s, ticket, C.byref(r)))
+        return r.as_dict()
+
+    def wait(self, ticket: int) -> dict:
+        """Blocks until the case is done; raises if it failed."""
+        r = lw_case_result()
+        _check(load_library().lw_server_wait(self._s, ticket, C.byref(r)))
+        if r.status != CASE_DONE:
+            _check(int(r.status))
+        return r.as_dict()
+
+    def stats(self) -> dict:
+        i = lw_server_info()
+        _check(load_library().lw_server_stats(self._s, C.byref(i)))
+        return i.as_dict()
+
+    def close(self):
+        """Runs every queued case to the end, then stops the workers."""
+        if self._s:
+            load_library().lw_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Solver:
     """One case resident in HBM."""
 
@@ -168,7 +269,7 @@ class Solver:
 
     # -- construction ------------------------------------------------------
     @classmethod
-    def from_case(cls, cd: dict, radius: float = 0.5) -> "Solver":
+    def from_case(cls, cd: dict, body=None) -> "Solver":
         """From :func:`leswht_amd.case.load_case_dir` output (arrays as the
         reference holds them after start-up)."""
         L = load_library()
@@ -204,7 +305,7 @@ class Solver:
             a = np.ascontiguousarray(nz[n], dtype=np.int64); keep.append(a); setattr(tb, n, _pi(a))
         nw = np.asfortranarray(nz["nwall"], dtype=np.int64); keep.append(nw)
         tb.nwall_dvm = _pi(nw)
-        body = lw_body(0, (_D * 4)(radius, 0, 0, 0))
+        body = body_struct(body if body is not None else cd.get("body"))
         h = C.c_void_p()
         _check(L.lw_create(C.byref(st), C.byref(gr), C.byref(tb), C.byref(body), C.byref(h)))
         self = cls(h, bool(s.ihtrans), (g["n1"], g["n2"], g["n3"]))
@@ -213,10 +314,12 @@ class Solver:
         return self
 
     @classmethod
-    def from_dir(cls, solver_dir: str, radius: float = 0.5) -> "Solver":
-        """Reads a reference case directory (cwd of solver_exec)."""
+    def from_dir(cls, solver_dir: str, body=None) -> "Solver":
+        """Reads a reference case directory (cwd of solver_exec). ``body`` is
+        (kind, params) as :func:`leswht_amd.case.body_params` gives it; None
+        is the reference's cylinder of radius 0.5."""
         L = load_library()
-        body = lw_body(0, (_D * 4)(radius, 0, 0, 0))
+        body = body_struct(body)
         h = C.c_void_p()
         _check(L.lw_create_from_dir(solver_dir.encode(), C.byref(body), C.byref(h)))
         dims = (_I * 3)()

@@ -75,6 +75,9 @@ def lib():
         L.ora_set_fields.argtypes = [C.c_void_p, _PD, _PD, _PD, _PD, _PD, _I, _I, _D, _D]
         L.ora_get_fields.argtypes = [C.c_void_p, _PD, _PD, _PD, _PD, _PD]
         L.ora_init.argtypes = [C.c_void_p]
+        L.ora_set_body.argtypes = [C.c_void_p, _I, C.POINTER(_D)]
+        L.ora_funcbody.argtypes = [C.c_void_p, _D, _D, _D]
+        L.ora_funcbody.restype = _D
         L.ora_step.argtypes = [C.c_void_p, C.POINTER(Diag)]
         L.ora_get_array.argtypes = [C.c_void_p, C.c_char_p, _PD]
         L.ora_get_array.restype = _I
@@ -167,6 +170,10 @@ class Oracle:
         self.h = L.ora_create(C.byref(self.st), C.byref(gr), C.byref(tb))
         if not self.h:
             raise ValueError("oracle rejected the configuration")
+        if cd.get("body") is not None:
+            kind, bp = cd["body"]
+            bp = tuple(bp) + (0.0,) * (4 - len(bp))
+            L.ora_set_body(self.h, int(kind), (_D * 4)(*bp[:4]))
         f = cd["fields"]
         arrs = [fortran_padded(f[k]) for k in ("u", "v", "w", "p")]
         tt = fortran_padded(f["t"]) if f.get("t") is not None else None

@@ -38,6 +38,8 @@ static inline double c_abs(cplx a) { return hypot(a.re, a.im); }
 
 struct ora {
   ora_settings s;
+  int64_t body_kind;
+  double body_p[4];
   int64_t n1, n2, n3, n1m, n2m, n3m, n1md, n3mh;
   double xl, yl, zl;
   double *x, *y, *z;
@@ -93,11 +95,30 @@ struct ora {
 /* ac/gam/bet (n1md, n2m, n3mh) */
 #define MIDX(i, j, k) (((i)-1) + o->n1md * (((j)-1) + o->n2m * (int64_t)((k)-1)))
 
-static double funcbody(double x, double y, double z, double t) {
+/* The body's level set, > 0 in fluid. Kind 0 is the reference's cylinder,
+ * geometry/funcbody_example/funcbody_cylinder.f90:13 (radius p[0], 0.5 by
+ * default); kinds 1 (square, half side p[0], angle p[1]) and 2 (ellipse,
+ * semi-axes p[0], p[1], angle p[2]) are further funcbody examples in the
+ * same form, evaluated as the case builder evaluates them. */
+static double funcbody(const ora *o, double x, double y, double z) {
   (void)z;
-  (void)t;
-  /* geometry/funcbody_example/funcbody_cylinder.f90:13 */
-  return sqrt(x * x + y * y) - 0.5;
+  const double *p = o->body_p;
+  if (o->body_kind == 1 || o->body_kind == 2) {
+    const double ang = o->body_kind == 1 ? p[1] : p[2];
+    const double c = cos(ang), s = sin(ang);
+    const double xr = c * x + s * y, yr = (-s) * x + c * y;
+    if (o->body_kind == 1) return fmax(fabs(xr), fabs(yr)) - p[0];
+    const double ta = xr / p[0], tb = yr / p[1];
+    return (sqrt(ta * ta + tb * tb) - 1.0) * (p[0] < p[1] ? p[0] : p[1]);
+  }
+  return sqrt(x * x + y * y) - p[0];
+}
+
+double ora_funcbody(const ora *o, double x, double y, double z) { return funcbody(o, x, y, z); }
+
+void ora_set_body(ora *o, int64_t kind, const double p[4]) {
+  o->body_kind = kind;
+  for (int q = 0; q < 4; ++q) o->body_p[q] = p[q];
 }
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -128,6 +149,7 @@ ora *ora_create(const ora_settings *s, const ora_grid *g, const ora_tables *t) {
   }
   ora *o = (ora *)xcalloc(1, sizeof(ora));
   o->s = *s;
+  o->body_p[0] = 0.5;
   o->n1 = g->n1; o->n2 = g->n2; o->n3 = g->n3;
   o->n1m = g->n1m; o->n2m = g->n2m; o->n3m = g->n3m;
   o->n1md = 2 * o->n1m;
@@ -713,7 +735,7 @@ static void nutzero(ora *o, double *nu) {
   for (int64_t k = 1; k <= o->n3m; ++k)
     for (int64_t j = 1; j <= o->n2m; ++j)
       for (int64_t i = 1; i <= o->n1m; ++i)
-        if (!(funcbody(o->xmp[i], o->ymp[j], o->zmp[k], o->time) >= 1.e-10)) F(nu, i, j, k) = 0.;
+        if (!(funcbody(o, o->xmp[i], o->ymp[j], o->zmp[k]) >= 1.e-10)) F(nu, i, j, k) = 0.;
 }
 
 /* sgs.f90:941-1133 nutinterpol (all = 1) / 1591-1773 alpinterpol (all = 0):
@@ -1214,7 +1236,7 @@ static void convbc(ora *o) {
   double qout = 0.;
   for (int64_t k = 1; k <= n3m; ++k)
     for (int64_t j = 1; j <= n2m; ++j) {
-      if (funcbody(o->xmp[n1m], o->ymp[j], o->zmp[k], o->time) <= 1.e-10) {
+      if (funcbody(o, o->xmp[n1m], o->ymp[j], o->zmp[k]) <= 1.e-10) {
         B2(o->uout, j, k) = F(u, n1m, j, k);
         B2(o->vout, j, k) = F(v, n1m, j, k);
         B2(o->wout, j, k) = F(w, n1m, j, k);
@@ -1228,7 +1250,7 @@ static void convbc(ora *o) {
   double qratio = qin / qout;
   for (int64_t k = 1; k <= n3m; ++k)
     for (int64_t j = 1; j <= n2m; ++j) {
-      if (funcbody(o->xmp[n1m], o->ymp[j], o->zmp[k], o->time) > 1.e-10) {
+      if (funcbody(o, o->xmp[n1m], o->ymp[j], o->zmp[k]) > 1.e-10) {
         B2(o->uout, j, k) = B2(o->uout, j, k) * qratio;
         B2(o->vout, j, k) = B2(o->vout, j, k) * qratio;
         B2(o->wout, j, k) = B2(o->wout, j, k) * qratio;
@@ -1925,7 +1947,7 @@ static void convbc_t(ora *o) {
     for (int64_t j = 1; j <= n2m; ++j) {
       double uo = F(u, n1, j, k) - ucoef * (F(u, n1, j, k) - F(u, n1m, j, k));
       B2(o->uout, j, k) = uo;
-      if (funcbody(o->xmp[n1m], o->ymp[j], o->zmp[k], o->time) <= 1.e-10)
+      if (funcbody(o, o->xmp[n1m], o->ymp[j], o->zmp[k]) <= 1.e-10)
         B2(o->tout, j, k) = F(t, n1m, j, k);
       else
         B2(o->tout, j, k) = F(t, n1, j, k) - vwcoef * (F(t, n1, j, k) - F(t, n1m, j, k));
@@ -1934,7 +1956,7 @@ static void convbc_t(ora *o) {
   double qratio = fabs(qout) > 1.0e-14 ? qin / qout : 1.0;
   for (int64_t k = 1; k <= n3m; ++k)
     for (int64_t j = 1; j <= n2m; ++j) {
-      if (funcbody(o->xmp[n1m], o->ymp[j], o->zmp[k], o->time) > 1.e-10)
+      if (funcbody(o, o->xmp[n1m], o->ymp[j], o->zmp[k]) > 1.e-10)
         B2(o->tout, j, k) = B2(o->tout, j, k) * qratio;
       B2(o->dtout, j, k) = B2(o->tout, j, k) - F(t, n1, j, k);
     }
@@ -2018,7 +2040,7 @@ static void energy_bcs(ora *o) {
     }
   for (int64_t k = 0; k <= n3; ++k)
     for (int64_t j = 0; j <= n2; ++j)
-      F(t, 0, j, k) = funcbody(o->xmp[0], o->ymp[j], o->zmp[k], o->time) <= 1.e-10 ? o->s.tsol_i
+      F(t, 0, j, k) = funcbody(o, o->xmp[0], o->ymp[j], o->zmp[k]) <= 1.e-10 ? o->s.tsol_i
                                                                                   : o->s.tflu_i;
   for (int64_t k = 0; k <= n3; ++k)
     for (int64_t i = 0; i <= n1; ++i) {

@@ -62,6 +62,11 @@ typedef struct ora ora;
 
 ora *ora_create(const ora_settings *s, const ora_grid *g, const ora_tables *t);
 void ora_destroy(ora *o);
+/* body level set used inside the step: kind 0 cylinder (radius p[0], the
+ * default 0.5), 1 square (half side p[0], angle p[1]), 2 ellipse (semi-axes
+ * p[0], p[1], angle p[2]) */
+void ora_set_body(ora *o, int64_t kind, const double p[4]);
+double ora_funcbody(const ora *o, double x, double y, double z);
 /* fields: padded (0:n1,0:n2,0:n3) arrays after prefld; t may be NULL. Also
  * sets ihist, m, time, dt as read from the restart file. */
 void ora_set_fields(ora *o, const double *u, const double *v, const double *w,
