@@ -148,6 +148,22 @@ int lw_write_fld(const char *path, const int64_t n[3], double re, double pr, dou
                  const int64_t bcs[6], const int64_t ich[2], const double *u, const double *v,
                  const double *w, const double *p, const double *t);
 
+/* Time-averaged statistics, field_avg (misc_hist.f90:287-527): with
+ * IAVG = 1, every step after AVG_TST adds dt-weighted cell-centre sums of
+ * u, v, w, u_i u_j, p, p^2, vorticity, its products, S:S (and T, T^2) on the
+ * device; every NPRIAVG steps they are written as fav%07d-%07d with the
+ * reference's records, listed in fav_manifest.dat, and restarted.
+ * lw_set_avg_dir names the directory (lw_create_from_dir sets
+ * <solver_dir>/../output/field_avg; NULL or "" writes no files, the sums still
+ * restart). lw_get_array reads the current sums ("uavg", "vavg", "wavg",
+ * "uiujavg_1".."uiujavg_6", "pavg", "p2avg", "voravg_1".."voravg_3",
+ * "vor2avg_1".."vor2avg_6", "ssavg", "tavg", "t2avg"; n1m*n2m*n3m values).
+ * lw_avg_finalize is field_avg_finalize (misc_hist.f90:530-697): the open
+ * interval's file, then the sum of every file in the manifest under
+ * combined/. */
+int lw_set_avg_dir(lw_handle *h, const char *dir);
+int lw_avg_finalize(lw_handle *h);
+
 /* sizes: n[0..2] = n1, n2, n3 */
 int lw_get_dims(const lw_handle *h, int64_t n[3]);
 /* history quantities of the last completed step (see lw_history) */

@@ -375,6 +375,9 @@ extern "C" int lw_create_from_dir(const char *solver_dir, const lw_body *body, l
     t.nwall_dvm = c.nwall.data();
     int rc = lw_create(&c.s, &g, &t, body, out);
     if (rc != 0) return rc;
+    // field_avg writes under ../output/field_avg relative to the solver's
+    // working directory (misc_hist.f90:484)
+    if (c.s.iavg == 1) lw_set_avg_dir(*out, join(dir, "../output/field_avg").c_str());
     rc = lw_set_fields(*out, c.u.data(), c.v.data(), c.w.data(), c.p.data(), c.s.ihtrans == 1 ? c.t.data() : nullptr,
                        c.ihist, c.m, c.time, c.dt);
     if (rc != 0) {

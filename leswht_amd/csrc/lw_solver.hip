@@ -182,6 +182,14 @@ struct lw_handle {
   bool tev_valid = false;  // tev hold a whole step recorded by lw_step
   Prof *prof = nullptr;
   double *h_red = nullptr;                 // pinned
+  // field_avg state (misc_hist.f90:287-527; mod_common avg_started,
+  // npriavg_count, timeinit, ihistinit); avg_dir empty: no fav files
+  AvgBuf avg{};
+  int navg = 0;
+  bool avg_started = false;
+  int64_t avg_count = 0, avg_ihistinit = 0;
+  double avg_timeinit = 0;
+  std::string avg_dir;
 
   template <class T>
   T *dalloc(size_t n) {
@@ -354,6 +362,10 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.dpart = h->dalloc<double>(D_NSUM * 512);
   h->hist = h->dalloc<double>(32);
   f.redmax = h->dalloc<unsigned long long>(8);
+  if (h->st.iavg == 1) {
+    h->navg = h->heat ? kAvgArrays : kAvgArrays - 2;
+    for (int l = 0; l < h->navg; ++l) h->avg.a[l] = h->dalloc<double>(ni);
+  }
   if (h->les) {
     for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
     h->sb.tmp = h->dalloc<double>(ni);
@@ -746,6 +758,7 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
   const int64_t heat = s->ihtrans == 1 ? 1 : 0;
   int64_t b = (30 + 4 * heat) * np * 8 + 2 * np;  // fields, mask and a byte of slack per cell
   if (s->iles == 1) b += 25 * ni * 8 + ni * 8;       // SGS work arrays and nwall
+  if (s->iavg == 1) b += (heat ? 23 : 21) * ni * 8;  // field_avg running sums
   const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
   const int64_t n3mh = n3m / 2 + 1;
   b += n3mh * (4 * plane + 3 * pcplane) * 16 + n3mh * plane * 8;
@@ -813,7 +826,27 @@ int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n) {
   else if (s == "qmass") src = h->f.qmass;
   else if (s.size() == 8 && s.compare(0, 7, "nusgs1_") == 0 && s[7] >= '1' && s[7] <= '3') src = h->f.nu1[s[7] - '1'];
   else if (s.size() == 8 && s.compare(0, 7, "alsgs1_") == 0 && s[7] >= '1' && s[7] <= '3') src = h->f.al1[s[7] - '1'];
-  else return fail(LW_ERR_ARG, "unknown array name");
+  else {
+    // running sums of field_avg: interior arrays (n1m, n2m, n3m)
+    static const char *names[kAvgArrays] = {"uavg", "vavg", "wavg", "uiujavg_1", "uiujavg_2", "uiujavg_3",
+                                             "uiujavg_4", "uiujavg_5", "uiujavg_6", "pavg", "p2avg", "voravg_1",
+                                             "voravg_2", "voravg_3", "vor2avg_1", "vor2avg_2", "vor2avg_3",
+                                             "vor2avg_4", "vor2avg_5", "vor2avg_6", "ssavg", "tavg", "t2avg"};
+    int l = 0;
+    while (l < kAvgArrays && s != names[l]) ++l;
+    if (l == kAvgArrays) return fail(LW_ERR_ARG, "unknown array name");
+    if (l >= h->navg) return fail(LW_ERR_ARG, "statistics not kept (IAVG = 0, or T without IHTRANS)");
+    const int64_t ni = (int64_t)h->g.n1m * h->g.n2m * h->g.n3m;
+    if (!out) return ni;
+    if (n < ni) return fail(LW_ERR_ARG, "output buffer too small");
+    try {
+      HIPCHK(hipMemcpyAsync(out, h->avg.a[l], sizeof(double) * ni, hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+    } catch (int code) {
+      return code;
+    }
+    return ni;
+  }
   if (!out) return np;
   if (n < np) return fail(LW_ERR_ARG, "output buffer too small");
   try {
@@ -1111,6 +1144,102 @@ static void end_of_step(lw_handle *h) {
                      h->g.xl * h->g.yl * h->g.zl);
 }
 
+// ---------------------------------------------------------------------------
+// time-averaged statistics: field_avg and field_avg_finalize
+// (misc_hist.f90:287-697); fav files in the reference's record layout
+// ---------------------------------------------------------------------------
+static void avg_zero(lw_handle *h) {
+  const size_t ni = (size_t)h->g.n1m * h->g.n2m * h->g.n3m;
+  for (int l = 0; l < h->navg; ++l) HIPCHK(hipMemsetAsync(h->avg.a[l], 0, ni * sizeof(double), h->s));
+}
+
+static std::string fav_name(int64_t i0, int64_t i1) {
+  char b[32];
+  snprintf(b, sizeof b, "fav%07lld-%07lld", (long long)i0, (long long)i1);
+  return b;
+}
+
+// one fav file: n1m n2m n3m re / timeinit timeend ihistinit ihistend (reals) /
+// u v w / uiuj(6) / p / p2 / vor(3) / vor2(6) / ss / [T / T2]
+// (misc_hist.f90:484-499); arr holds navg interior arrays, i fastest
+static void write_fav(const lw_handle *h, const std::string &path, const std::vector<double> &arr, double t0,
+                      double t1, int64_t i0, int64_t i1) {
+  FILE *fp = fopen(path.c_str(), "wb");
+  if (!fp) {
+    g_err = "cannot write " + path;
+    throw LW_ERR_IO;
+  }
+  auto rec = [&](const void *d, size_t len) {
+    const int32_t l = (int32_t)len;
+    fwrite(&l, 4, 1, fp);
+    fwrite(d, 1, len, fp);
+    fwrite(&l, 4, 1, fp);
+  };
+  const size_t ni = (size_t)h->g.n1m * h->g.n2m * h->g.n3m;
+  char hdr[32];
+  const int64_t n[3] = {h->g.n1m, h->g.n2m, h->g.n3m};
+  memcpy(hdr, n, 24);
+  memcpy(hdr + 24, &h->st.re, 8);
+  rec(hdr, 32);
+  const double tt[4] = {t0, t1, (double)i0, (double)i1};
+  rec(tt, 32);
+  // records of 1, 1, 1, 6, 1, 1, 3, 6, 1 (, 1, 1) arrays
+  static const int per[11] = {1, 1, 1, 6, 1, 1, 3, 6, 1, 1, 1};
+  int l = 0;
+  for (int r = 0; r < 11 && l < h->navg; ++r) {
+    rec(arr.data() + (size_t)l * ni, (size_t)per[r] * ni * 8);
+    l += per[r];
+  }
+  fclose(fp);
+}
+
+static void append_manifest(const lw_handle *h, const std::string &name) {
+  const std::string path = h->avg_dir + "/fav_manifest.dat";
+  FILE *fp = fopen(path.c_str(), "a");
+  if (!fp) {
+    g_err = "cannot append to " + path;
+    throw LW_ERR_IO;
+  }
+  fprintf(fp, "%s\n", name.c_str());
+  fclose(fp);
+}
+
+static std::vector<double> avg_download(lw_handle *h) {
+  const size_t ni = (size_t)h->g.n1m * h->g.n2m * h->g.n3m;
+  std::vector<double> a((size_t)h->navg * ni);
+  for (int l = 0; l < h->navg; ++l)
+    HIPCHK(hipMemcpyAsync(a.data() + (size_t)l * ni, h->avg.a[l], ni * 8, hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  return a;
+}
+
+// field_avg (misc_hist.f90:321-527), called after the step's time and ihist
+// updates with the step's dt
+static void field_avg(lw_handle *h) {
+  const lw_settings &st = h->st;
+  if (h->time <= st.avg_tst) return;
+  if (!h->avg_started) {
+    h->avg_started = true;
+    h->avg_count = 0;
+    h->avg_timeinit = st.avg_tst;
+    h->avg_ihistinit = h->ihist;
+    avg_zero(h);
+  }
+  launch_field_avg(h->s, h->g, h->f, h->avg, h->dt, h->heat);
+  h->avg_count += 1;
+  if (h->avg_count >= st.npriavg) {
+    if (!h->avg_dir.empty()) {
+      const std::string name = fav_name(h->avg_ihistinit, h->ihist);
+      write_fav(h, h->avg_dir + "/" + name, avg_download(h), h->avg_timeinit, h->time, h->avg_ihistinit, h->ihist);
+      append_manifest(h, name);
+    }
+    avg_zero(h);
+    h->avg_count = 0;
+    h->avg_timeinit = h->time;
+    h->avg_ihistinit = h->ihist;
+  }
+}
+
 static void read_stats(lw_handle *h, lw_step_stats *stats) {
   HIPCHK(hipMemcpyAsync(h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipMemcpyAsync(h->h_red, h->f.red, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
@@ -1152,6 +1281,103 @@ static void read_stats(lw_handle *h, lw_step_stats *stats) {
   stats->ms_energy = h->s_phase[3] * 1e3;
 }
 
+// records of a Fortran unformatted sequential file (4-byte markers)
+static bool read_records(const std::string &path, std::vector<std::vector<char>> &recs) {
+  FILE *fp = fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  recs.clear();
+  int32_t l = 0, l2 = 0;
+  bool ok = true;
+  while (fread(&l, 4, 1, fp) == 1) {
+    std::vector<char> r((size_t)l);
+    if (l < 0 || fread(r.data(), 1, (size_t)l, fp) != (size_t)l || fread(&l2, 4, 1, fp) != 1 || l2 != l) {
+      ok = false;
+      break;
+    }
+    recs.push_back(std::move(r));
+  }
+  fclose(fp);
+  return ok;
+}
+
+extern "C" int lw_set_avg_dir(lw_handle *h, const char *dir) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  h->avg_dir = dir ? dir : "";
+  return LW_OK;
+}
+
+// field_avg_finalize (misc_hist.f90:530-697): the open interval's file, then
+// the sum of every file the manifest lists, written under combined/
+extern "C" int lw_avg_finalize(lw_handle *h) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  if (h->st.iavg != 1 || !h->avg_started) return LW_OK;
+  try {
+    const bool files = !h->avg_dir.empty();
+    if (h->avg_count > 0 && files) {
+      const std::string name = fav_name(h->avg_ihistinit, h->ihist);
+      write_fav(h, h->avg_dir + "/" + name, avg_download(h), h->avg_timeinit, h->time, h->avg_ihistinit, h->ihist);
+      append_manifest(h, name);
+    }
+    avg_zero(h);
+    HIPCHK(hipStreamSynchronize(h->s));
+    if (!files) return LW_OK;
+    const size_t ni = (size_t)h->g.n1m * h->g.n2m * h->g.n3m;
+    std::vector<double> acc((size_t)h->navg * ni, 0.0);
+    FILE *mf = fopen((h->avg_dir + "/fav_manifest.dat").c_str(), "r");
+    if (!mf) return LW_OK;
+    bool have = false;
+    int64_t count = 0, ihistinit = 0, ihistend = 0;
+    double timeinit = 0, timeend = h->st.avg_tst;
+    char line[512];
+    std::vector<std::vector<char>> recs;
+    static const int per[11] = {1, 1, 1, 6, 1, 1, 3, 6, 1, 1, 1};
+    while (fgets(line, sizeof line, mf)) {
+      std::string nm(line);
+      while (!nm.empty() && (nm.back() == '\n' || nm.back() == '\r' || nm.back() == ' ')) nm.pop_back();
+      if (nm.empty()) continue;
+      if (!read_records(h->avg_dir + "/" + nm, recs)) continue;
+      if (recs.size() < 2 || recs[1].size() != 32) continue;
+      const int nrec = h->heat ? 11 : 9;
+      if ((int)recs.size() < 2 + nrec) {
+        fclose(mf);
+        g_err = "fav file " + nm + " has too few records";
+        throw LW_ERR_IO;
+      }
+      int l = 0;
+      for (int r = 0; r < nrec; ++r) {
+        if (recs[2 + r].size() != (size_t)per[r] * ni * 8) {
+          fclose(mf);
+          g_err = "fav file " + nm + " does not match the grid";
+          throw LW_ERR_IO;
+        }
+        const double *d = (const double *)recs[2 + r].data();
+        double *a = acc.data() + (size_t)l * ni;
+        for (size_t q = 0; q < (size_t)per[r] * ni; ++q) a[q] = a[q] + d[q];
+        l += per[r];
+      }
+      double tt[4];
+      memcpy(tt, recs[1].data(), 32);
+      count += 1;
+      if (!have) {
+        timeinit = tt[0];
+        ihistinit = (int64_t)llround(tt[2]);
+        have = true;
+      }
+      timeend = tt[1];
+      ihistend = (int64_t)llround(tt[3]);
+    }
+    fclose(mf);
+    if (!have || count <= 0) return LW_OK;
+    std::string dir = h->avg_dir + "/combined";
+    const std::string cmd = "mkdir -p '" + dir + "'";
+    if (system(cmd.c_str()) != 0) dir = h->avg_dir;
+    write_fav(h, dir + "/" + fav_name(ihistinit, ihistend), acc, timeinit, timeend, ihistinit, ihistend);
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
 extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
   if (!h || nsteps < 0) return fail(LW_ERR_ARG, "bad argument");
   ProfScope ps(h->prof);
@@ -1181,6 +1407,7 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
         if (ms == 3) h->tev_valid = true;
       }
       end_of_step(h);
+      if (h->st.iavg == 1) field_avg(h);
       if (g_prof) {
         HIPCHK(hipStreamSynchronize(h->s));
         g_prof->flush();

@@ -81,6 +81,12 @@ module leswht_c
       character(kind=c_char), dimension(*), intent(in) :: path
     end function lw_write_field
 
+    ! field_avg_finalize (misc_hist.f90:530-697)
+    integer(c_int) function lw_avg_finalize(h) bind(c, name='lw_avg_finalize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function lw_avg_finalize
+
     type(c_ptr) function lw_last_error() bind(c, name='lw_last_error')
       import :: c_ptr
     end function lw_last_error

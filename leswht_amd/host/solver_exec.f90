@@ -29,6 +29,7 @@ program solver_exec
   integer(8) :: clk0, clk1, rate
   real(8) :: wall
   character(len=64) :: env
+  character(len=68) :: benv
   integer :: elen, estat
   logical :: heat, ibm
 
@@ -37,6 +38,15 @@ program solver_exec
   body%p(1) = 0.5d0
   call get_environment_variable('LW_BODY_RADIUS', env, elen, estat)
   if (estat == 0 .and. elen > 0) read (env(1:elen), *) body%p(1)
+  ! LW_BODY = "kind p1 [p2 [p3]]": 0 cylinder (radius), 1 square (half side,
+  ! angle), 2 ellipse (semi-axes, angle), as lw_body in leswht.h
+  call get_environment_variable('LW_BODY', env, elen, estat)
+  if (estat == 0 .and. elen > 0) then
+    body%p = 0.0d0
+    ! the slash ends the list, so parameters not given stay 0
+    benv = env(1:elen)//' /'
+    read (benv, *, iostat=estat) body%kind, body%p
+  end if
 
   rc = lw_create_from_dir('.'//c_null_char, body, h)
   call check(rc, 'reading the case')
@@ -78,6 +88,10 @@ program solver_exec
 
   call close_histories()
   if (mod(hs%ntime, st%nprint) /= 0) call write_field(hs%ihist)
+  if (st%iavg == 1) then
+    rc = lw_avg_finalize(h)
+    call check(rc, 'field_avg_finalize')
+  end if
   rc = lw_destroy(h)
 
 130 format(f13.5, 4es15.7)

@@ -111,7 +111,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
-           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_server_create", "lw_server_submit",
+           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
 
 _lib = None
@@ -151,6 +151,8 @@ def load_library(path: str = LIB_PATH):
     L.lw_estimate_device_bytes.argtypes = [C.POINTER(lw_settings), _I, _I, _I]
     L.lw_estimate_device_bytes.restype = _I
     L.lw_probe_dir.argtypes = [C.c_char_p, C.POINTER(lw_settings), _PI]
+    L.lw_set_avg_dir.argtypes = [H, C.c_char_p]
+    L.lw_avg_finalize.argtypes = [H]
     L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
     L.lw_body_eval.restype = _D
     L.lw_server_create.argtypes = [_I, _I, C.POINTER(H)]
@@ -356,6 +358,27 @@ class Solver:
         out = np.zeros(n)
         _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
         return out.reshape(self.shape(), order="F")
+
+    # -- time-averaged statistics (field_avg) --------------------------------
+    AVG_NAMES = ("uavg", "vavg", "wavg") + tuple("uiujavg_%d" % q for q in range(1, 7)) + \
+        ("pavg", "p2avg") + tuple("voravg_%d" % q for q in range(1, 4)) + \
+        tuple("vor2avg_%d" % q for q in range(1, 7)) + ("ssavg", "tavg", "t2avg")
+
+    def avg_array(self, name: str) -> np.ndarray:
+        """A running sum of field_avg as an (n1m, n2m, n3m) array."""
+        L = load_library()
+        n = L.lw_get_array(self._h, name.encode(), None, 0)
+        if n < 0:
+            _check(int(n))
+        out = np.zeros(n)
+        _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
+        return out.reshape((self.n1 - 1, self.n2 - 1, self.n3 - 1), order="F")
+
+    def set_avg_dir(self, path):
+        _check(load_library().lw_set_avg_dir(self._h, path.encode() if path else None))
+
+    def avg_finalize(self):
+        _check(load_library().lw_avg_finalize(self._h))
 
     # -- time stepping -----------------------------------------------------
     def step(self, n: int = 1) -> dict:

@@ -82,6 +82,11 @@ struct ora {
   double cdavg_int[3], cdavg_dur, hflux_int[3], hflux_dur;
   double cd[3], heatflux[3];
   int64_t mg_cycles_max, mg_cycles_sum;
+  /* field_avg running sums (misc_hist.f90:302-304), interior arrays, in the
+   * order u v w, uiuj(6), p, p2, vor(3), vor2(6), ss, t, t2 */
+  double *avg[23];
+  int avg_started;
+  int64_t avg_count;
 };
 
 /* ---- indexing ------------------------------------------------------- */
@@ -150,6 +155,10 @@ ora *ora_create(const ora_settings *s, const ora_grid *g, const ora_tables *t) {
   ora *o = (ora *)xcalloc(1, sizeof(ora));
   o->s = *s;
   o->body_p[0] = 0.5;
+  if (s->iavg == 1) {
+    const int64_t nint = g->n1m * g->n2m * g->n3m;
+    for (int l = 0; l < 23; ++l) o->avg[l] = (double *)xcalloc(nint, 8);
+  }
   o->n1 = g->n1; o->n2 = g->n2; o->n3 = g->n3;
   o->n1m = g->n1m; o->n2m = g->n2m; o->n3m = g->n3m;
   o->n1md = 2 * o->n1m;
@@ -236,6 +245,7 @@ void ora_destroy(ora *o) {
   /* process-lifetime test helper: the few allocations are released with the
    * process; free the large ones explicitly */
   free(o->u); free(o->v); free(o->w); free(o->p); free(o->t);
+  for (int l = 0; l < 23; ++l) free(o->avg[l]);
   for (int c = 0; c < 4; ++c) { free(o->rhs1[c]); free(o->rk3o[c]); free(o->rk3oo[c]); }
   free(o->nusgs); free(o->alsgs);
   for (int c = 0; c < 3; ++c) { free(o->nusgs1[c]); free(o->alsgs1[c]); }
@@ -276,7 +286,17 @@ int64_t ora_get_array(const ora *o, const char *name, double *out) {
   else if (!strncmp(name, "nusgs1_", 7)) src = o->nusgs1[name[7] - '1'];
   else if (!strncmp(name, "alsgs1_", 7)) src = o->alsgs1[name[7] - '1'];
   else if (!strncmp(name, "rhs1_", 5)) src = o->rhs1[name[5] - '1'];
-  else return -1;
+  else {
+    static const char *names[23] = {"uavg", "vavg", "wavg", "uiujavg_1", "uiujavg_2", "uiujavg_3", "uiujavg_4",
+                                    "uiujavg_5", "uiujavg_6", "pavg", "p2avg", "voravg_1", "voravg_2", "voravg_3",
+                                    "vor2avg_1", "vor2avg_2", "vor2avg_3", "vor2avg_4", "vor2avg_5", "vor2avg_6",
+                                    "ssavg", "tavg", "t2avg"};
+    int l = 0;
+    while (l < 23 && strcmp(name, names[l])) ++l;
+    if (l == 23 || !o->avg[l]) return -1;
+    src = o->avg[l];
+    n = nint;
+  }
   if (out) memcpy(out, src, n * 8);
   return n;
 }
@@ -2154,6 +2174,137 @@ static void diagnostics(ora *o) {
   o->qmmax = qm;
 }
 
+/* ===================================================================== */
+/* misc_hist.f90:287-527 field_avg: the running sums only (the fav files   */
+/* are written by the library; here every NPRIAVG steps the sums restart)  */
+/* ===================================================================== */
+static void field_avg(ora *o) {
+  if (o->time <= o->s.avg_tst) return;
+  const int64_t n1 = o->n1, n2 = o->n2, n3 = o->n3, nint = o->n1m * o->n2m * o->n3m;
+  if (!o->avg_started) {
+    o->avg_started = 1;
+    o->avg_count = 0;
+    for (int l = 0; l < 23; ++l) memset(o->avg[l], 0, nint * 8);
+  }
+  const double dt = o->dt;
+  double **A = o->avg;
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) {
+        const double *u = o->u, *v = o->v, *w = o->w;
+        const double ucc = 0.5 * (F(u, i, j, k) + F(u, i + 1, j, k));
+        const double vcc = 0.5 * (F(v, i, j, k) + F(v, i, j + 1, k));
+        const double wcc = 0.5 * (F(w, i, j, k) + F(w, i, j, k + 1));
+        G(A[0], i, j, k) += ucc * dt;
+        G(A[1], i, j, k) += vcc * dt;
+        G(A[2], i, j, k) += wcc * dt;
+        G(A[3], i, j, k) += ucc * ucc * dt;
+        G(A[4], i, j, k) += ucc * vcc * dt;
+        G(A[5], i, j, k) += ucc * wcc * dt;
+        G(A[6], i, j, k) += vcc * vcc * dt;
+        G(A[7], i, j, k) += vcc * wcc * dt;
+        G(A[8], i, j, k) += wcc * wcc * dt;
+        if (o->s.ihtrans == 1) {
+          const double t = F(o->t, i, j, k);
+          G(A[21], i, j, k) += t * dt;
+          G(A[22], i, j, k) += t * t * dt;
+        }
+        const double p = F(o->p, i, j, k);
+        G(A[9], i, j, k) += p * dt;
+        G(A[10], i, j, k) += p * p * dt;
+      }
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) {
+        const double *u = o->u, *v = o->v, *w = o->w;
+        const double fju = (double)o->fixju[j], fjl = (double)o->fixjl[j], fku = (double)o->fixku[k],
+                     fkl = (double)o->fixkl[k], fiu = (double)o->fixiu[i], fil = (double)o->fixil[i];
+        double up, um, vp, vm, wp, wm;
+        const double vg11 = o->f2fxi[i] * (F(u, i + 1, j, k) - F(u, i, j, k));
+        up = o->c2cyi[j + 1] * 0.25 *
+                 (o->f2fy[j + 1] * (F(u, i, j, k) + F(u, i + 1, j, k)) +
+                  o->f2fy[j] * (F(u, i, j + 1, k) + F(u, i + 1, j + 1, k))) * (1. - fju) +
+             0.5 * (F(u, i, n2, k) + F(u, i + 1, n2, k)) * fju;
+        um = o->c2cyi[j] * 0.25 *
+                 (o->f2fy[j] * (F(u, i, j - 1, k) + F(u, i + 1, j - 1, k)) +
+                  o->f2fy[j - 1] * (F(u, i, j, k) + F(u, i + 1, j, k))) * (1. - fjl) +
+             0.5 * (F(u, i, 0, k) + F(u, i + 1, 0, k)) * fjl;
+        const double vg12 = o->f2fyi[j] * (up - um);
+        up = o->c2czi[k + 1] * 0.25 *
+                 (o->f2fz[k + 1] * (F(u, i, j, k) + F(u, i + 1, j, k)) +
+                  o->f2fz[k] * (F(u, i, j, k + 1) + F(u, i + 1, j, k + 1))) * (1. - fku) +
+             0.5 * (F(u, i, j, n3) + F(u, i + 1, j, n3)) * fku;
+        um = o->c2czi[k] * 0.25 *
+                 (o->f2fz[k] * (F(u, i, j, k - 1) + F(u, i + 1, j, k - 1)) +
+                  o->f2fz[k - 1] * (F(u, i, j, k) + F(u, i + 1, j, k))) * (1. - fkl) +
+             0.5 * (F(u, i, j, 0) + F(u, i + 1, j, 0)) * fkl;
+        const double vg13 = o->f2fzi[k] * (up - um);
+        vp = o->c2cxi[i + 1] * 0.25 *
+                 (o->f2fx[i + 1] * (F(v, i, j, k) + F(v, i, j + 1, k)) +
+                  o->f2fx[i] * (F(v, i + 1, j, k) + F(v, i + 1, j + 1, k))) * (1. - fiu) +
+             0.5 * (F(v, n1, j, k) + F(v, n1, j + 1, k)) * fiu;
+        vm = o->c2cxi[i] * 0.25 *
+                 (o->f2fx[i] * (F(v, i - 1, j, k) + F(v, i - 1, j + 1, k)) +
+                  o->f2fx[i - 1] * (F(v, i, j, k) + F(v, i, j + 1, k))) * (1. - fil) +
+             0.5 * (F(v, 0, j, k) + F(v, 0, j + 1, k)) * fil;
+        const double vg21 = o->f2fxi[i] * (vp - vm);
+        const double vg22 = o->f2fyi[j] * (F(v, i, j + 1, k) - F(v, i, j, k));
+        vp = o->c2czi[k + 1] * 0.25 *
+                 (o->f2fz[k + 1] * (F(v, i, j, k) + F(v, i, j + 1, k)) +
+                  o->f2fz[k] * (F(v, i, j, k + 1) + F(v, i, j + 1, k + 1))) * (1. - fku) +
+             0.5 * (F(v, i, j, n3) + F(v, i, j + 1, n3)) * fku;
+        vm = o->c2czi[k] * 0.25 *
+                 (o->f2fz[k] * (F(v, i, j, k - 1) + F(v, i, j + 1, k - 1)) +
+                  o->f2fz[k - 1] * (F(v, i, j, k) + F(v, i, j + 1, k))) * (1. - fkl) +
+             0.5 * (F(v, i, j, 0) + F(v, i, j + 1, 0)) * fkl;
+        const double vg23 = o->f2fzi[k] * (vp - vm);
+        wp = o->c2cxi[i + 1] * 0.25 *
+                 (o->f2fx[i + 1] * (F(w, i, j, k) + F(w, i, j, k + 1)) +
+                  o->f2fx[i] * (F(w, i + 1, j, k) + F(w, i + 1, j, k + 1))) * (1. - fiu) +
+             0.5 * (F(w, n1, j, k) + F(w, n1, j, k + 1)) * fiu;
+        wm = o->c2cxi[i] * 0.25 *
+                 (o->f2fx[i] * (F(w, i - 1, j, k) + F(w, i - 1, j, k + 1)) +
+                  o->f2fx[i - 1] * (F(w, i, j, k) + F(w, i, j, k + 1))) * (1. - fil) +
+             0.5 * (F(w, 0, j, k) + F(w, 0, j, k + 1)) * fil;
+        const double vg31 = o->f2fxi[i] * (wp - wm);
+        wp = o->c2cyi[j + 1] * 0.25 *
+                 (o->f2fy[j + 1] * (F(w, i, j, k) + F(w, i, j, k + 1)) +
+                  o->f2fy[j] * (F(w, i, j + 1, k) + F(w, i, j + 1, k + 1))) * (1. - fju) +
+             0.5 * (F(w, i, n2, k) + F(w, i, n2, k + 1)) * fju;
+        wm = o->c2cyi[j] * 0.25 *
+                 (o->f2fy[j] * (F(w, i, j - 1, k) + F(w, i, j - 1, k + 1)) +
+                  o->f2fy[j - 1] * (F(w, i, j, k) + F(w, i, j, k + 1))) * (1. - fjl) +
+             0.5 * (F(w, i, 0, k) + F(w, i, 0, k + 1)) * fjl;
+        const double vg32 = o->f2fyi[j] * (wp - wm);
+        const double vg33 = o->f2fzi[k] * (F(w, i, j, k + 1) - F(w, i, j, k));
+        double sr[7];
+        sr[1] = vg11;
+        sr[2] = 0.5 * (vg12 + vg21);
+        sr[3] = 0.5 * (vg13 + vg31);
+        sr[4] = vg22;
+        sr[5] = 0.5 * (vg23 + vg32);
+        sr[6] = vg33;
+        const double wx = vg32 - vg23, wy = vg13 - vg31, wz = vg21 - vg12;
+        G(A[11], i, j, k) += wx * dt;
+        G(A[12], i, j, k) += wy * dt;
+        G(A[13], i, j, k) += wz * dt;
+        G(A[14], i, j, k) += wx * wx * dt;
+        G(A[15], i, j, k) += wx * wy * dt;
+        G(A[16], i, j, k) += wx * wz * dt;
+        G(A[17], i, j, k) += wy * wy * dt;
+        G(A[18], i, j, k) += wy * wz * dt;
+        G(A[19], i, j, k) += wz * wz * dt;
+        const double srsr = 2. * (sr[2] * sr[2]) + sr[1] * sr[1] + 2. * (sr[3] * sr[3]) + sr[4] * sr[4] +
+                            2. * (sr[5] * sr[5]) + sr[6] * sr[6];
+        G(A[20], i, j, k) += srsr * dt;
+      }
+  o->avg_count += 1;
+  if (o->avg_count >= o->s.npriavg) {
+    for (int l = 0; l < 23; ++l) memset(o->avg[l], 0, nint * 8);
+    o->avg_count = 0;
+  }
+}
+
 void ora_step(ora *o, ora_diag *d) {
   ora_stepinit(o);
   o->subdt = 0.;
@@ -2173,6 +2324,7 @@ void ora_step(ora *o, ora_diag *d) {
   }
   o->time = o->time + o->dt;
   diagnostics(o);
+  if (o->s.iavg == 1) field_avg(o);
   if (d) {
     d->dt = o->dt; d->time = o->time; d->cflmax = o->cflmax; d->dvmax = o->dvmax;
     d->qmmax = o->qmmax; d->csgsts = o->csgsts; d->csgshf = o->csgshf;
