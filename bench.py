@@ -34,6 +34,17 @@ from leswht_amd.solver import Solver  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the multigrid kernels on this workload, averaged
+# over the launches of one step: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch) on `bench.py --no-cpu-baseline --steps 1 --warmup 1`,
+# scaled by what the same counters read for a 512 MiB device copy
+# (`bench.py --copy-probe-mib 512`: FETCH x2.000, WRITE x1.000)
+TRAFFIC = {
+    "k_mg_up": round((4027.3 * 2.0 + 2574.1) * 1024),
+    "k_mg_down": round((2444.4 * 2.0 + 5172.4) * 1024),
+    "k_mg_top_up": round((12754.6 * 2.0 + 13757.4) * 1024),
+}
+
 
 def c2_spec(nend: int = 2) -> C.CaseSpec:
     s = C.Settings(re=3900.0, pr=0.71, iles=1, insmdl=1, itemdl=0, idvmon=1, filter=3, idtopt=1,
@@ -128,6 +139,8 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times (the known-size "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters), then exit")
+    ap.add_argument("--dominant-only", action="store_true",
+                    help="time only the dominant kernel's launches in the timed region")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -168,9 +181,11 @@ def main():
     sol.step(1)
     table = sorted(sol.profile(), key=lambda r: -r["ms_total"])
     dom_name = table[0]["name"]
-    # timed region: K steps, with the dominant kernel's launches bracketed by
-    # events on the solver's stream
-    sol.profile_begin([dom_name])
+    # timed region: K steps, every launch timed by the start and end
+    # timestamps of its own dispatch on the stream it runs on (timing only
+    # the dominant kernel leaves it between untimed dispatches, whose waits
+    # then land in its start-to-end times)
+    sol.profile_begin(None if not args.dominant_only else [dom_name])
     t0 = time.perf_counter()
     st = sol.step(args.steps)
     elapsed = time.perf_counter() - t0
@@ -185,7 +200,7 @@ def main():
         per_launch = dom["bytes"] / dom["count"]
         ach = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom_name), "kernel": dom_name,
                 "launches": dom["count"], "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": round(per_launch)}
     step_ms = sum(r["ms_total"] for r in table)

@@ -208,6 +208,28 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
  * directory, without reading tables or fields */
 int lw_probe_dir(const char *solver_dir, lw_settings *s, int64_t nm[3]);
 
+/* ---- immersed-boundary tables built in the process ------------------------
+ * The pre-processor's table routines (ibm_body.f90:14-399 as called from
+ * preprocessing.py:124-276, IBM-INTERP ON) for a body chosen at run time:
+ * find_inout, findbdy_intp, geomfac_preset, geomfac_intp and, with les,
+ * find_zero_nu_sgs. Inputs are the grid lines of each direction (nx, ny, nz
+ * lines, the values grid.dat holds) and the periodic flags; the axes are
+ * formed as lib_setgrid.py:1-79 forms them. heat adds the T component.
+ * lw_ib_tables fills an lw_tables for lw_create (pointers valid while the
+ * lw_ib lives; geomfac as the solver reads it back from the %15.8f text);
+ * lw_ib_geomfac_raw gives the unrounded weights of component 0..3;
+ * lw_ib_write writes ibmpre_fcpts.bin, ibmpre_fcpts_t.bin, ibmpre_nutzero.bin
+ * and ibmpre_wallfdvm.bin with the pre-processor's formats
+ * (lib_preprocessing.py:223-338). */
+typedef struct lw_ib lw_ib;
+int lw_ib_build(const double *xlines, int64_t nx, const double *ylines, int64_t ny,
+                const double *zlines, int64_t nz, const int64_t periodic[3], const lw_body *body,
+                int64_t heat, int64_t les, lw_ib **out);
+int lw_ib_tables(const lw_ib *ib, lw_tables *t);
+int64_t lw_ib_geomfac_raw(const lw_ib *ib, int64_t comp, double *out, int64_t n);
+int lw_ib_write(const lw_ib *ib, const char *ibmpre_dir);
+int lw_ib_free(lw_ib *ib);
+
 /* ---- request loop over independent cases on one GPU ----------------------
  * The reference runs one case per solver_exec process (run_solver.sh:41); it
  * has no counterpart to this. A server owns `slots` worker threads. Each

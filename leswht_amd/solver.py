@@ -111,7 +111,8 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
-           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_server_create", "lw_server_submit",
+           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize",
+           "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
 
 _lib = None
@@ -151,6 +152,12 @@ def load_library(path: str = LIB_PATH):
     L.lw_estimate_device_bytes.argtypes = [C.POINTER(lw_settings), _I, _I, _I]
     L.lw_estimate_device_bytes.restype = _I
     L.lw_probe_dir.argtypes = [C.c_char_p, C.POINTER(lw_settings), _PI]
+    L.lw_ib_build.argtypes = [_PD, _I, _PD, _I, _PD, _I, _PI, C.POINTER(lw_body), _I, _I, C.POINTER(H)]
+    L.lw_ib_tables.argtypes = [H, C.POINTER(lw_tables)]
+    L.lw_ib_geomfac_raw.argtypes = [H, _I, _PD, _I]
+    L.lw_ib_geomfac_raw.restype = _I
+    L.lw_ib_write.argtypes = [H, C.c_char_p]
+    L.lw_ib_free.argtypes = [H]
     L.lw_set_avg_dir.argtypes = [H, C.c_char_p]
     L.lw_avg_finalize.argtypes = [H]
     L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
@@ -209,6 +216,66 @@ def estimate_device_bytes(settings, n1m: int, n2m: int, n3m: int) -> int:
     if b < 0:
         _check(int(b))
     return int(b)
+
+
+class IBTablesNative:
+    """Immersed-boundary tables built by the library (lw_ib_build) from the
+    grid lines of the three directions and a body (kind, params)."""
+
+    COMPS = ("u", "v", "w", "t")
+
+    def __init__(self, xlines, ylines, zlines, periodic=(0, 0, 1), body=None, heat=True, les=True):
+        L = load_library()
+        self._arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xlines, ylines, zlines)]
+        per = (_I * 3)(*[int(v) for v in periodic])
+        b = body_struct(body)
+        self._h = C.c_void_p()
+        x, y, z = self._arrs
+        _check(L.lw_ib_build(_pd(x), len(x), _pd(y), len(y), _pd(z), len(z), per, C.byref(b),
+                             int(heat), int(les), C.byref(self._h)))
+        self.heat, self.les = heat, les
+        self.shape = (len(x) - 1, len(y) - 1, len(z) - 1)  # cells
+
+    def tables(self) -> dict:
+        """Copies of the tables: per component nintp, nbody, fcp (nbody, 3),
+        indx (nintp, 3), geomfac (nintp, 3, 3, 3) as the solver reads it,
+        geomfac_raw; and nzero, iszero (cells, 0 = zero nu_t)."""
+        L = load_library()
+        t = lw_tables()
+        _check(L.lw_ib_tables(self._h, C.byref(t)))
+        out = {}
+        for ci, c in enumerate(self.COMPS[:4 if self.heat else 3]):
+            nb, ni = t.nbody[ci], t.nintp[ci]
+            fcp = np.stack([np.ctypeslib.as_array(p, (nb,)) if nb else np.zeros(0, np.int64)
+                            for p in (t.ifc[ci], t.jfc[ci], t.kfc[ci])], axis=1).astype(np.int64)
+            ix = np.ctypeslib.as_array(t.intpindx[ci], (ni * 3,)).reshape(ni, 3).copy() if ni else \
+                np.zeros((0, 3), np.int64)
+            gf = np.ctypeslib.as_array(t.geomfac[ci], (ni * 27,)).reshape(ni, 3, 3, 3).copy() if ni else \
+                np.zeros((0, 3, 3, 3))
+            raw = np.zeros(ni * 27)
+            if ni:
+                _check(0 if L.lw_ib_geomfac_raw(self._h, ci, _pd(raw), ni * 27) == ni * 27 else -1)
+            out[c] = dict(nintp=ni, nbody=nb, fcp=fcp, indx=ix, geomfac=gf, geomfac_raw=raw.reshape(ni, 3, 3, 3))
+        out["nzero"] = t.nzero
+        if self.les:
+            cells = self.shape
+            out["iszero"] = np.ctypeslib.as_array(t.nwall_dvm, (int(np.prod(cells)),)).reshape(
+                cells, order="F").copy()
+        return out
+
+    def write(self, ibmpre_dir: str):
+        _check(load_library().lw_ib_write(self._h, ibmpre_dir.encode()))
+
+    def close(self):
+        if self._h:
+            load_library().lw_ib_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:
