@@ -121,6 +121,7 @@ def serve_bench(args) -> dict:
                                "step budgets U{200..2000} x %.3g" % (len(cases), args.rate, n_large,
                                                                     args.step_scale),
                    "slots": args.slots, "budget_gb": args.budget_gb, "seed": args.seed,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "cell_updates": stream["cell_updates"], "prep_s": round(t_prep, 1),
                    "peak_running": stream["info"]["peak_running"],
                    "peak_reserved_gb": round(stream["info"]["peak_reserved_bytes"] / (1 << 30), 2)},
@@ -139,8 +140,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times (the known-size "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters), then exit")
-    ap.add_argument("--dominant-only", action="store_true",
-                    help="time only the dominant kernel's launches in the timed region")
+    ap.add_argument("--kernel-steps", type=int, default=3,
+                    help="steps after the timed region with every launch timed, for the roofline kernel's "
+                         "average duration (0: use the timed region's own timing of that kernel)")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -149,7 +151,13 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="served form: cases on the device at once")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served form: admission budget")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="served form: hardware queues per process (GPU_MAX_HW_QUEUES, at most 32); "
+                         "every case runs on two streams, which share the queues")
     args = ap.parse_args()
+    if args.hw_queues:
+        # read by the HIP runtime when it initialises, before any device call
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.hw_queues, 1), 32))
     if args.serve:
         print(json.dumps(serve_bench(args)))
         return
@@ -181,16 +189,25 @@ def main():
     sol.step(1)
     table = sorted(sol.profile(), key=lambda r: -r["ms_total"])
     dom_name = table[0]["name"]
-    # timed region: K steps, every launch timed by the start and end
-    # timestamps of its own dispatch on the stream it runs on (timing only
-    # the dominant kernel leaves it between untimed dispatches, whose waits
-    # then land in its start-to-end times)
-    sol.profile_begin(None if not args.dominant_only else [dom_name])
+    # timed region: K steps, with the dominant kernel's launches timed by the
+    # start and end timestamps of their own dispatches on the streams they run on
+    sol.profile_begin([dom_name])
     t0 = time.perf_counter()
     st = sol.step(args.steps)
     elapsed = time.perf_counter() - t0
-    prof = sol.profile()
+    prof_region = sol.profile()
     sol.profile_end()
+    # then a few steps with every launch timed: a timed dispatch that follows
+    # untimed ones on a stream also counts the time the stream spends
+    # reaching it, so the timed region's figure for a kernel of the two-stream
+    # Poisson solve runs high; timing every launch gives the kernel's own
+    # durations (rocprofv3's kernel trace agrees) at the cost of a slower step
+    prof = prof_region
+    if args.kernel_steps > 0:
+        sol.profile_begin()
+        sol.step(args.kernel_steps)
+        prof = sol.profile()
+        sol.profile_end()
     n = spec.n1m * spec.n2m * spec.n3m
     value = n * args.steps / elapsed
     roof = None
@@ -202,7 +219,12 @@ def main():
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom_name), "kernel": dom_name,
                 "launches": dom["count"], "avg_us": round(avg_s * 1e6, 2),
-                "bytes_per_launch": round(per_launch)}
+                "bytes_per_launch": round(per_launch),
+                "timing": "every launch timed, %d steps after the timed region" % args.kernel_steps
+                if args.kernel_steps > 0 else "timed region"}
+        reg = next((r for r in prof_region if r["name"] == dom_name), None)
+        if reg and reg["count"]:
+            roof["avg_us_timed_region"] = round(reg["ms_total"] / reg["count"] * 1e3, 2)
     step_ms = sum(r["ms_total"] for r in table)
     print("kernel table of one step (event time, algorithmic bytes):", file=sys.stderr)
     for r in table[:25]:
