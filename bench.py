@@ -87,7 +87,7 @@ def serve_bench(args) -> dict:
     from leswht_amd import serve
     cases = serve.draw_cases(args.serve, seed=args.seed, rate=args.rate,
                              steps=tuple(int(round(v * args.step_scale)) for v in (200, 2000)),
-                             large_share=0.1,
+                             large_share=args.large_share,
                              large_steps=tuple(int(round(v * args.step_scale)) for v in (200, 2000)))
     root = tempfile.mkdtemp(prefix="lw_serve_")
     try:
@@ -149,6 +149,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.1,
                     help="served form: factor on the step budgets U{200..2000}")
     ap.add_argument("--slots", type=int, default=8, help="served form: cases on the device at once")
+    ap.add_argument("--large-share", type=float, default=0.1,
+                    help="served form: share of 384x256x64 LES cases in the stream (0: the batch of small cases)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served form: admission budget")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--hw-queues", type=int, default=0,
@@ -193,7 +195,11 @@ def main():
     # start and end timestamps of their own dispatches on the streams they run on
     sol.profile_begin([dom_name])
     t0 = time.perf_counter()
-    st = sol.step(args.steps)
+    pois_ms, cycles = 0.0, 0
+    for _ in range(args.steps):
+        st = sol.step(1)
+        pois_ms += st["ms_poisson"]
+        cycles += st["mg_cycles_sum"]
     elapsed = time.perf_counter() - t0
     prof_region = sol.profile()
     sol.profile_end()
@@ -225,6 +231,18 @@ def main():
         reg = next((r for r in prof_region if r["name"] == dom_name), None)
         if reg and reg["count"]:
             roof["avg_us_timed_region"] = round(reg["ms_total"] / reg["count"] * 1e3, 2)
+    # Poisson HBM GB/s: the survey's byte model of divgs, the z-DFT pair and
+    # the multigrid (16 n1m n2m (11 n3mh + 22 cycles) per substep: the initial
+    # toplevel and 22 complex-plane passes per V-cycle), over the device time
+    # of divgs + poisson (phase events of every substep)
+    ncell = spec.n1m * spec.n2m * spec.n3m
+    n3mh = spec.n3m // 2 + 1
+    cpass = (spec.n3m + 2) / spec.n3m
+    pois_bytes = args.steps * 3 * 8.0 * ncell * (5 + 2 + 2 * cpass) + \
+        16.0 * spec.n1m * spec.n2m * (11 * n3mh * 3 * args.steps + 22 * cycles)
+    poisson = {"gbs": round(pois_bytes / (pois_ms / 1e3) / 1e9, 1), "ms_per_step": round(pois_ms / args.steps, 3),
+               "bytes_per_step": round(pois_bytes / args.steps), "mg_cycles_per_step": cycles / args.steps,
+               "frac": round(pois_bytes / (pois_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
     step_ms = sum(r["ms_total"] for r in table)
     print("kernel table of one step (event time, algorithmic bytes):", file=sys.stderr)
     for r in table[:25]:
@@ -250,6 +268,7 @@ def main():
                    "mg_cycles_max_last_step": st["mg_cycles_max"], "dt_last": st["dt"],
                    "case_build_s": round(t_build, 2)},
         "roofline": roof,
+        "poisson": poisson,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
