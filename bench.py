@@ -16,6 +16,7 @@ two steps of the same case, read from its own per-step timer file.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import shutil
@@ -53,6 +54,20 @@ def c2_spec(nend: int = 2) -> C.CaseSpec:
     return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s)
 
 
+def c1_spec(nend: int = 2) -> C.CaseSpec:
+    """The plain FP64 case: Re 100, 192x128x16, fixed dt 0.005, no SGS model."""
+    s = C.Settings(re=100.0, pr=0.71, iles=0, idtopt=0, dt_size=0.005, resid1=1.0e-6, nlev=4, nbli=3,
+                   mgitr=500, nend=nend)
+    return C.CaseSpec(192, 128, 16, dmin=0.01, noise=1.0e-3, settings=s)
+
+
+WORKLOADS = {
+    "c2": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64, dynamic Vreman + dynamic eddy "
+          "diffusivity, box test filter x/z, CFL 0.5",
+    "c1": "heated circular cylinder Re=100 Pr=0.71, 192x128x16, no SGS model, dt 0.005",
+}
+
+
 def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
     """Reference solver on the host: two steps of the same case; per-step wall
     time from its ftime.dat column 5 (misc_hist.f90:255-284)."""
@@ -60,6 +75,12 @@ def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
 
     root = tempfile.mkdtemp(prefix="lw_cpu_")
     try:
+        if spec.settings.iles == 0:
+            # the reference's ILES = 0 path reads LES arrays it never
+            # allocated; "no SGS" runs there as ILES = 1 with the model off
+            # and zero coefficients, which gives the same fields
+            st = dataclasses.replace(spec.settings, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0)
+            case = C.build_case(dataclasses.replace(spec, settings=st))
         C.write_case(case, root)
         t0 = time.time()
         oracle.run_reference(root, threads=threads, timeout=1500)
@@ -69,9 +90,10 @@ def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
         med = statistics.median(step_s)
         n = spec.n1m * spec.n2m * spec.n3m
         return {"value": n / med, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
-                "sample": "reference solver (amdflang -O3, OpenMP %d threads), same 384x256x64 LES case, "
+                "sample": "reference solver (amdflang -O3, OpenMP %d threads), same %dx%dx%d case, "
                           "%d RK3 steps, median of its own per-step timer (ftime.dat); process wall %.1f s "
-                          "incl. reading the text inputs" % (threads, len(step_s), wall),
+                          "incl. reading the text inputs" % (threads, spec.n1m, spec.n2m, spec.n3m,
+                                                             len(step_s), wall),
                 "s_per_step": med}
     finally:
         shutil.rmtree(root, ignore_errors=True)
@@ -143,6 +165,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=3,
                     help="steps after the timed region with every launch timed, for the roofline kernel's "
                          "average duration (0: use the timed region's own timing of that kernel)")
+    ap.add_argument("--config", choices=("c2", "c1"), default="c2",
+                    help="c2: the 384x256x64 LES the metric is quoted on; c1: the plain 192x128x16 case")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -178,7 +202,7 @@ def main():
               "(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
 
-    spec = c2_spec()
+    spec = c1_spec() if args.config == "c1" else c2_spec()
     t0 = time.time()
     case = C.build_case(spec)
     view = C.case_view(case)
@@ -223,7 +247,7 @@ def main():
         per_launch = dom["bytes"] / dom["count"]
         ach = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom_name), "kernel": dom_name,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom_name) if args.config == "c2" else None, "kernel": dom_name,
                 "launches": dom["count"], "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": round(per_launch),
                 "timing": "every launch timed, %d steps after the timed region" % args.kernel_steps
@@ -262,8 +286,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded initial field, generated grid and IB tables)",
-        "config": {"workload": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64, dynamic Vreman "
-                               "+ dynamic eddy diffusivity, box test filter x/z, CFL 0.5",
+        "config": {"workload": WORKLOADS[args.config],
                    "cells": n, "nlev": spec.settings.nlev, "resid1": spec.settings.resid1,
                    "mg_cycles_max_last_step": st["mg_cycles_max"], "dt_last": st["dt"],
                    "case_build_s": round(t_build, 2)},

@@ -179,6 +179,11 @@ int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n);
  * 0 stepinit, 10+m substepinit(m), 1 sgs, 2 rhsnlhs, 3 divgs, 4 poisson,
  * 5 ucalc+pcalc+lagforce, 6 energy, 7 end-of-step diagnostics */
 int lw_phase(lw_handle *h, int phase);
+/* per-handle switches, no counterpart in the reference: "mg_split" 1 solves
+ * kv = 1 of the Poisson equation on a second stream beside the other
+ * wavenumbers, 0 on the handle's one stream (default: 1 when n1m*n2m >= 65536);
+ * results are the same bits either way */
+int lw_set_option(lw_handle *h, const char *name, int64_t value);
 /* device scratch footprint in bytes */
 int64_t lw_device_bytes(const lw_handle *h);
 

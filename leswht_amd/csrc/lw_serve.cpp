@@ -47,7 +47,7 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
 }  // namespace
 
 struct lw_server {
-  int64_t budget = 0;
+  int64_t budget = 0, nslots = 1;
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -116,6 +116,7 @@ int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out) {
   }
   auto *s = new lw_server();
   s->budget = mem_budget_bytes;
+  s->nslots = slots;
   s->t0 = std::chrono::steady_clock::now();
   for (int64_t q = 0; q < slots; ++q) s->workers.emplace_back([s] { s->worker(); });
   *out = s;
@@ -197,7 +198,7 @@ int lw_server_stats(lw_server *s, lw_server_info *info) {
   info->peak_reserved_bytes = s->peak_reserved;
   info->peak_running = s->peak_running;
   info->budget_bytes = s->budget;
-  info->slots = (int64_t)s->workers.size();
+  info->slots = s->nslots;
   info->elapsed = seconds_since(s->t0);
   return LW_OK;
 }

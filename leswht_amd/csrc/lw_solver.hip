@@ -688,6 +688,10 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     HIPCHK(hipStreamCreateWithFlags(&h->s2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    // kv = 1 on its own stream pays on large planes (384 x 256: 4% per step);
+    // on small ones one stream is as fast alone and leaves the hardware
+    // queues to other cases (LW_MG_SPLIT or lw_set_option override)
+    h->mg_split = (int64_t)gr->n1m * gr->n2m >= 65536;
     if (const char *e = getenv("LW_MG_SPLIT")) h->mg_split = atoi(e) != 0;
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     for (int q = 0; q < 3; ++q)
@@ -1298,6 +1302,16 @@ static bool read_records(const std::string &path, std::vector<std::vector<char>>
   }
   fclose(fp);
   return ok;
+}
+
+extern "C" int lw_set_option(lw_handle *h, const char *name, int64_t value) {
+  if (!h || !name) return fail(LW_ERR_ARG, "null argument");
+  const std::string n(name);
+  if (n == "mg_split") {
+    h->mg_split = value != 0;
+    return LW_OK;
+  }
+  return fail(LW_ERR_ARG, "unknown option " + n);
 }
 
 extern "C" int lw_set_avg_dir(lw_handle *h, const char *dir) {

@@ -111,7 +111,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
-           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize",
+           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option",
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
 
@@ -158,6 +158,7 @@ def load_library(path: str = LIB_PATH):
     L.lw_ib_geomfac_raw.restype = _I
     L.lw_ib_write.argtypes = [H, C.c_char_p]
     L.lw_ib_free.argtypes = [H]
+    L.lw_set_option.argtypes = [H, C.c_char_p, _I]
     L.lw_set_avg_dir.argtypes = [H, C.c_char_p]
     L.lw_avg_finalize.argtypes = [H]
     L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
@@ -440,6 +441,9 @@ class Solver:
         out = np.zeros(n)
         _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
         return out.reshape((self.n1 - 1, self.n2 - 1, self.n3 - 1), order="F")
+
+    def set_option(self, name: str, value: int):
+        _check(load_library().lw_set_option(self._h, name.encode(), int(value)))
 
     def set_avg_dir(self, path):
         _check(load_library().lw_set_avg_dir(self._h, path.encode() if path else None))
