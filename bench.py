@@ -56,7 +56,7 @@ def c2_spec(nend: int = 2) -> C.CaseSpec:
 
 def c1_spec(nend: int = 2) -> C.CaseSpec:
     """The plain FP64 case: Re 100, 192x128x16, fixed dt 0.005, no SGS model."""
-    s = C.Settings(re=100.0, pr=0.71, iles=0, idtopt=0, dt_size=0.005, resid1=1.0e-6, nlev=4, nbli=3,
+    s = C.Settings(re=100.0, pr=0.71, iles=0, idtopt=0, dt_size=0.005, resid1=1.0e-6, nlev=5, nbli=3,
                    mgitr=500, nend=nend)
     return C.CaseSpec(192, 128, 16, dmin=0.01, noise=1.0e-3, settings=s)
 
@@ -167,6 +167,7 @@ def main():
                          "average duration (0: use the timed region's own timing of that kernel)")
     ap.add_argument("--config", choices=("c2", "c1"), default="c2",
                     help="c2: the 384x256x64 LES the metric is quoted on; c1: the plain 192x128x16 case")
+    ap.add_argument("--nlev", type=int, default=0, help="multigrid levels (0: the workload's own)")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -203,6 +204,8 @@ def main():
         sys.exit(2)
 
     spec = c1_spec() if args.config == "c1" else c2_spec()
+    if args.nlev:
+        spec = dataclasses.replace(spec, settings=dataclasses.replace(spec.settings, nlev=args.nlev))
     t0 = time.time()
     case = C.build_case(spec)
     view = C.case_view(case)

@@ -67,7 +67,7 @@ def test_gpu_phases_match_oracle(mode, oracle_lib):
 
 
 @pytest.mark.parametrize("dims,mode,nsteps,nlev", [((64, 32, 16), "les", 3, 2), ((48, 32, 16), "les_cfl", 4, 3),
-                                                   ((96, 64, 32), "les", 2, 3)])
+                                                   ((96, 64, 32), "les", 2, 3), ((96, 64, 16), "les", 2, 4)])
 def test_gpu_steps_match_oracle(dims, mode, nsteps, nlev):
     import oracle
     view = _view(dims, mode, nsteps, nlev)
