@@ -90,6 +90,8 @@ struct lw_ib {
 
 namespace {
 
+void add(std::string &s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
 // find_inout (ibm_body.f90:14-46) and findbdy_intp (49-123) for one
 // component on its staggered grid (xc, yc, zc indexed 0..N): body points in
 // k -> j -> i order, interpolation points first
@@ -246,10 +248,11 @@ void geomfac(const lw_ib &b, const std::vector<double> &xp, const std::vector<do
   // the solver reads the weights back from the %15.8f text
   // (lib_preprocessing.py:261; misc_init.f90:341-349)
   cp.gf.resize(cp.raw.size());
-  char buf[64];
+  std::string txt;
   for (size_t q = 0; q < cp.raw.size(); ++q) {
-    snprintf(buf, sizeof buf, " %15.8f ", cp.raw[q]);
-    cp.gf[q] = strtod(buf, nullptr);
+    txt.clear();
+    add(txt, " %15.8f ", cp.raw[q]);
+    cp.gf[q] = strtod(txt.c_str(), nullptr);
   }
 }
 
@@ -294,14 +297,21 @@ bool write_text(const std::string &path, const std::string &text) {
   return fclose(fp) == 0 && ok;
 }
 
-void add(std::string &s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void add(std::string &s, const char *fmt, ...) {
   char buf[256];
-  va_list ap;
+  va_list ap, ap2;
   va_start(ap, fmt);
+  va_copy(ap2, ap);
   const int n = vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  s.append(buf, (size_t)n);
+  if (n >= 0 && (size_t)n < sizeof buf) {
+    s.append(buf, (size_t)n);
+  } else if (n > 0) {  // a value wider than the buffer (e.g. a huge weight in %15.8f)
+    std::vector<char> big((size_t)n + 1);
+    vsnprintf(big.data(), big.size(), fmt, ap2);
+    s.append(big.data(), (size_t)n);
+  }
+  va_end(ap2);
 }
 
 // ibmpre_fcpts.bin (lib_preprocessing.py:223-277) for comps u, v, w, or
