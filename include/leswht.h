@@ -184,6 +184,18 @@ int lw_phase(lw_handle *h, int phase);
  * wavenumbers, 0 on the handle's one stream (default: 1 when n1m*n2m >= 65536);
  * results are the same bits either way */
 int lw_set_option(lw_handle *h, const char *name, int64_t value);
+/* Autotuner of the fused multigrid kernels' tile sizes (lines per workgroup
+ * of the down, up and top kernels; tile1 for kv = 1 on its own stream). Times
+ * reps Poisson solves of the current field's divergence per candidate and
+ * keeps the fastest set in the handle. Tiles never change the arithmetic, and
+ * the solve's inputs and outputs are recomputed in every substep, so steps
+ * after tuning give the same bits. No counterpart in the reference. */
+typedef struct lw_tune_result {
+  int64_t tile[3], tile1[3];  /* chosen (down, up, top) */
+  int64_t candidates;         /* sets timed besides the starting one */
+  double ms_default, ms_best; /* best solve time of the starting and chosen sets */
+} lw_tune_result;
+int lw_autotune(lw_handle *h, int64_t reps, lw_tune_result *out);
 /* device scratch footprint in bytes */
 int64_t lw_device_bytes(const lw_handle *h);
 

@@ -1285,6 +1285,88 @@ static void read_stats(lw_handle *h, lw_step_stats *stats) {
   stats->ms_energy = h->s_phase[3] * 1e3;
 }
 
+// ---------------------------------------------------------------------------
+// autotuner of the fused multigrid kernels' tile sizes (lines per workgroup
+// of the down, up and top kernels). Tiles change only how lines are grouped
+// into workgroups, never the arithmetic, so any choice gives the same bits.
+// The tuner times whole Poisson solves of the divergence of the current
+// field (the solve's own initial guess is zero, and divgs, the solve and
+// phi are recomputed in every substep, so the state the steps see is
+// unchanged), first for the main stream's tiles, then, with the kv = 1
+// split, for the kv = 1 side's.
+// ---------------------------------------------------------------------------
+extern "C" int lw_autotune(lw_handle *h, int64_t reps, lw_tune_result *out) {
+  if (!h || !out || reps < 1) return fail(LW_ERR_ARG, "lw_autotune: bad argument");
+  memset(out, 0, sizeof *out);
+  MG &mg = h->mg;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  try {
+    for (int q = 0; q < 3; ++q) {
+      out->tile[q] = mg.tile[q];
+      out->tile1[q] = mg.tile1[q];
+    }
+    if (!mg.fused) return LW_OK;  // separate launches: no tiles to tune
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const double dt0 = h->dt;
+    if (!(h->dt > 0.0)) h->dt = h->st.dt_size;
+    substepinit(h, 1);
+    launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
+    phase_poisson(h);  // warm-up; sets the batch hints
+    auto solve_ms = [&]() {
+      float best = 1e30f;
+      for (int64_t r = 0; r < reps; ++r) {
+        HIPCHK(hipEventRecord(e0, h->s));
+        phase_poisson(h);
+        HIPCHK(hipEventRecord(e1, h->s));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        best = t < best ? t : best;
+      }
+      return (double)best;
+    };
+    static const int cand[][3] = {{2, 2, 4}, {2, 2, 2}, {4, 4, 4}, {4, 4, 8}, {2, 2, 8},
+                                  {8, 8, 8}, {2, 4, 4}, {4, 2, 4}};
+    const int ncand = (int)(sizeof cand / sizeof cand[0]);
+    out->ms_default = solve_ms();
+    double best = out->ms_default;
+    const int split = h->mg_split && mg.n3mh >= 2 && mg.mgitr > 0;
+    for (int side = 0; side < 1 + split; ++side) {
+      int *t = side == 0 ? mg.tile : mg.tile1;
+      int keep[3] = {t[0], t[1], t[2]};
+      for (int c = 0; c < ncand; ++c) {
+        if (cand[c][0] == keep[0] && cand[c][1] == keep[1] && cand[c][2] == keep[2]) continue;
+        for (int q = 0; q < 3; ++q) t[q] = cand[c][q];
+        const double ms = solve_ms();
+        out->candidates += 1;
+        if (ms < best) {
+          best = ms;
+          for (int q = 0; q < 3; ++q) keep[q] = cand[c][q];
+        }
+      }
+      for (int q = 0; q < 3; ++q) t[q] = keep[q];
+    }
+    out->ms_best = best;
+    for (int q = 0; q < 3; ++q) {
+      out->tile[q] = mg.tile[q];
+      out->tile1[q] = mg.tile1[q];
+    }
+    h->dt = dt0;
+    h->mg_hint = 4;
+    h->mg_hint1 = 4;
+    h->cyc_max = 0;
+    h->cyc_sum = 0;
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    return LW_OK;
+  } catch (int code) {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return code;
+  }
+}
+
 // records of a Fortran unformatted sequential file (4-byte markers)
 static bool read_records(const std::string &path, std::vector<std::vector<char>> &recs) {
   FILE *fp = fopen(path.c_str(), "rb");

@@ -87,6 +87,14 @@ class lw_history(C.Structure):
         return out
 
 
+class lw_tune_result(C.Structure):
+    _fields_ = [("tile", _I * 3), ("tile1", _I * 3), ("candidates", _I), ("ms_default", _D), ("ms_best", _D)]
+
+    def as_dict(self):
+        return dict(tile=list(self.tile), tile1=list(self.tile1), candidates=self.candidates,
+                    ms_default=self.ms_default, ms_best=self.ms_best)
+
+
 class lw_case_result(C.Structure):
     _fields_ = [(n, _I) for n in ("ticket", "status", "cells", "steps_done", "mg_cycles_sum",
                                   "reserved_bytes", "device_bytes")] + \
@@ -111,7 +119,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_step", "lw_write_field", "lw_write_fld", "lw_get_dims", "lw_get_array", "lw_phase",
            "lw_device_bytes", "lw_profile_begin", "lw_profile_end", "lw_profile_count",
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
-           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option",
+           "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option", "lw_autotune",
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
 
@@ -159,6 +167,7 @@ def load_library(path: str = LIB_PATH):
     L.lw_ib_write.argtypes = [H, C.c_char_p]
     L.lw_ib_free.argtypes = [H]
     L.lw_set_option.argtypes = [H, C.c_char_p, _I]
+    L.lw_autotune.argtypes = [H, _I, C.POINTER(lw_tune_result)]
     L.lw_set_avg_dir.argtypes = [H, C.c_char_p]
     L.lw_avg_finalize.argtypes = [H]
     L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
@@ -441,6 +450,13 @@ class Solver:
         out = np.zeros(n)
         _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
         return out.reshape((self.n1 - 1, self.n2 - 1, self.n3 - 1), order="F")
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Picks the fused multigrid kernels' tile sizes by timing Poisson
+        solves of the current field (lw_autotune); results stay bitwise."""
+        r = lw_tune_result()
+        _check(load_library().lw_autotune(self._h, int(reps), C.byref(r)))
+        return r.as_dict()
 
     def set_option(self, name: str, value: int):
         _check(load_library().lw_set_option(self._h, name.encode(), int(value)))
