@@ -70,8 +70,24 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_serve(verbose)
     build_host(verbose)
     return LIB
+
+
+SERVE_EXE = os.path.join(OUT_DIR, "serve_exec")
+
+
+def build_serve(verbose: bool = False) -> str:
+    """serve_exec: the request loop over a manifest of case directories."""
+    tmp = SERVE_EXE + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HOST, "serve_exec.cpp"), "-o", tmp, "-L" + OUT_DIR, "-lleswht", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, SERVE_EXE)
+    return SERVE_EXE
 
 
 def build_host(verbose: bool = False) -> str | None:

@@ -102,3 +102,35 @@ def test_gpu_admission_keeps_to_the_budget():
     # first-come first-served: no case is admitted before an earlier one
     adm = [r["t_admit"] for r in out["results"]]
     assert adm == sorted(adm)
+
+
+def test_gpu_serve_exec_runs_a_manifest():
+    """serve_exec runs a manifest of case directories through the request
+    loop and writes each case's final field, the same bits as a Solver run of
+    the same directory."""
+    import subprocess
+    from leswht_amd.build import SERVE_EXE
+    if not os.path.exists(SERVE_EXE):
+        pytest.skip("serve_exec not built")
+    cases = MIX[1:3]
+    with tempfile.TemporaryDirectory() as root:
+        dirs = serve.prepare(cases, root)
+        man = os.path.join(root, "manifest")
+        with open(man, "w") as f:
+            for c, d in zip(cases, dirs):
+                kind, p = c.body
+                f.write("%s %d %d %s\n" % (d, c.nsteps, kind, " ".join(repr(v) for v in p)))
+        print("serve_exec", len(cases), flush=True)
+        res = subprocess.run([SERVE_EXE, man, "--slots", "2"], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        assert len([ln for ln in res.stdout.splitlines() if not ln.startswith("#")]) == len(cases)
+        for c, d in zip(cases, dirs):
+            got = C.read_fld(os.path.join(d, "..", "output", "field", "fld%07d" % c.nsteps), heat=True)
+            g = Solver.from_dir(d, body=c.body)
+            g.step(c.nsteps)
+            f = g.fields()
+            g.close()
+            n1, n2, n3 = g.n1, g.n2, g.n3
+            assert np.array_equal(got["u"][1:n1 + 1], f["u"][1:n1 + 1]), c.name
+            assert np.array_equal(got["p"][1:n1, 1:n2, 1:n3], f["p"][1:n1, 1:n2, 1:n3]), c.name
+            assert np.array_equal(got["t"][1:n1, 1:n2, 1:n3], f["t"][1:n1, 1:n2, 1:n3]), c.name

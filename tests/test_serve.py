@@ -152,3 +152,27 @@ def test_server_error_paths_without_running_a_case():
         info = srv.stats()
         assert info["submitted"] == 1 and info["slots"] == 2 and info["running"] == 0
         srv.close()
+
+
+def test_serve_exec_reports_manifest_and_case_errors():
+    """serve_exec (the manifest driver of the request loop): a malformed
+    manifest line stops it with status 2; a case that cannot be read is
+    reported and gives status 1 (no device work happens before that)."""
+    import subprocess
+    from leswht_amd.build import SERVE_EXE
+    from tests.helpers import spec_for
+    if not os.path.exists(SERVE_EXE):
+        pytest.skip("serve_exec not built")
+    with tempfile.TemporaryDirectory() as d:
+        man = os.path.join(d, "manifest")
+        with open(man, "w") as f:
+            f.write("# comment\nsome/dir notanumber\n")
+        res = subprocess.run([SERVE_EXE, man], capture_output=True, text=True, timeout=60)
+        assert res.returncode == 2 and "expected SOLVER_DIR NSTEPS" in res.stderr
+        sol = C.write_case(C.build_case(spec_for((16, 16, 4), "nosgs")), os.path.join(d, "case"))
+        os.remove(os.path.join(d, "case", "output", "field", "fld000000"))
+        with open(man, "w") as f:
+            f.write("%s 2\n%s 1 1 0.5 0.3\n" % (sol, os.path.join(d, "missing")))
+        res = subprocess.run([SERVE_EXE, man, "--slots", "2"], capture_output=True, text=True, timeout=60)
+        assert res.returncode == 1
+        assert "failed" in res.stderr or "cannot open" in res.stderr
