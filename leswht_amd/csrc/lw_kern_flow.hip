@@ -961,6 +961,14 @@ static AdiWaveKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lan
   return nullptr;
 }
 static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * adi_y_ld(g.n2) * sizeof(double)); }
+// LDS limit set on the instance pick_adi_yw chooses for n2m: the size its
+// longest line needs (n2m = 32 R rows). The attribute belongs to the kernel
+// instance, which every case with rows in the same range shares, so it is
+// set to the range's largest size and never lowered under a running case.
+static inline unsigned adi_yw_lds_bucket(int n2m) {
+  const int r = n2m <= 32 ? 1 : n2m <= 64 ? 2 : n2m <= 128 ? 4 : 8;
+  return (unsigned)(2 * kYL * adi_y_ld(32 * r + 1) * sizeof(double));
+}
 
 // periodic z sweep with one half-wavefront per line (tri_per_line32, N =
 // n3m = 32*R rows): a workgroup stages the viscosity and right-hand side of
@@ -1135,12 +1143,12 @@ hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
   *use_wave = 0;
   if ((mask & 2) && pick_adi_xw<0>(g.n1m)) *use_wave |= 2;
   if ((mask & 4) && pick_adi_zw<0>(g.n3m)) *use_wave |= 4;
-  if (!(mask & 1) || !pick_adi_yw<0>(g.n2m) || adi_yw_lds(g) > 160 * 1024) return hipSuccess;
+  if (!(mask & 1) || !pick_adi_yw<0>(g.n2m) || adi_yw_lds_bucket(g.n2m) > 160 * 1024) return hipSuccess;
+  const int lds = (int)adi_yw_lds_bucket(g.n2m);
   hipError_t e = hipFuncSetAttribute((const void *)pick_adi_yw<0>(g.n2m),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)adi_yw_lds(g));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)pick_adi_yw<3>(g.n2m), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)adi_yw_lds(g));
+    e = hipFuncSetAttribute((const void *)pick_adi_yw<3>(g.n2m), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e == hipSuccess) *use_wave |= 1;
   return e;
 }

@@ -57,26 +57,37 @@ struct lw_server {
   bool stopping = false;
   std::vector<std::thread> workers;
 
-  void run_case(Request *r) {
+  // Runs one admitted case without holding mu. Everything it learns goes
+  // into `out` and `err`, which the worker publishes under mu together with
+  // the completion time, so a poll or wait never sees a half-written result.
+  struct Outcome {
+    int rc = LW_OK;
+    int64_t steps_done = 0, mg_cycles_sum = 0, device_bytes = 0;
+    double time = 0.0, dt = 0.0;
+    std::string err;
+  };
+  static Outcome run_case(const Request &r) {
+    Outcome o;
     lw_handle *h = nullptr;
-    int rc = lw_create_from_dir(r->dir.c_str(), &r->body, &h);
-    if (rc == LW_OK) {
-      r->res.device_bytes = lw_device_bytes(h);
+    lw_body body = r.body;
+    o.rc = lw_create_from_dir(r.dir.c_str(), &body, &h);
+    if (o.rc == LW_OK) {
+      o.device_bytes = lw_device_bytes(h);
       lw_step_stats st{};
-      for (int64_t n = 0; n < r->nsteps && rc == LW_OK; ++n) {
-        rc = lw_step(h, 1, &st);
-        if (rc == LW_OK) {
-          r->res.steps_done = n + 1;
-          r->res.mg_cycles_sum += st.mg_cycles_sum;
-          r->res.time = st.time;
-          r->res.dt = st.dt;
+      for (int64_t n = 0; n < r.nsteps && o.rc == LW_OK; ++n) {
+        o.rc = lw_step(h, 1, &st);
+        if (o.rc == LW_OK) {
+          o.steps_done = n + 1;
+          o.mg_cycles_sum += st.mg_cycles_sum;
+          o.time = st.time;
+          o.dt = st.dt;
         }
       }
-      if (rc == LW_OK && !r->fld_out.empty()) rc = lw_write_field(h, r->fld_out.c_str());
+      if (o.rc == LW_OK && !r.fld_out.empty()) o.rc = lw_write_field(h, r.fld_out.c_str());
     }
-    if (rc != LW_OK) r->err = lw_last_error();
+    if (o.rc != LW_OK) o.err = lw_last_error();
     if (h) lw_destroy(h);
-    r->res.status = rc == LW_OK ? LW_CASE_DONE : rc;
+    return o;
   }
 
   void worker() {
@@ -96,9 +107,16 @@ struct lw_server {
       r->res.status = LW_CASE_RUNNING;
       r->res.t_admit = seconds_since(t0);
       lk.unlock();
-      run_case(r);
+      Outcome o = run_case(*r);
       lk.lock();
+      r->res.steps_done = o.steps_done;
+      r->res.mg_cycles_sum = o.mg_cycles_sum;
+      r->res.device_bytes = o.device_bytes;
+      r->res.time = o.time;
+      r->res.dt = o.dt;
+      r->err = o.err;
       r->res.t_done = seconds_since(t0);
+      r->res.status = o.rc == LW_OK ? LW_CASE_DONE : o.rc;
       reserved -= r->res.reserved_bytes;
       running -= 1;
       cv_done.notify_all();

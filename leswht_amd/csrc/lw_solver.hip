@@ -10,9 +10,9 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include <thread>
-
+#include <filesystem>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -40,6 +40,19 @@ static int fail(int code, const std::string &msg) {
       throw LW_ERR_DEVICE;                                                          \
     }                                                                               \
   } while (0)
+
+// A launch the device refuses (resources, LDS size) reports only through
+// the calling thread's last-error state. The step checks it after every
+// phase, so a kernel that did not run ends the call with LW_ERR_DEVICE
+// instead of leaving wrong fields. hipErrorNotReady is what the host's
+// polling of a busy stream leaves there, not a failure.
+static void check_launches(const char *where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess && e != hipErrorNotReady) {
+    g_err = std::string("HIP error: ") + hipGetErrorString(e) + " in " + where;
+    throw LW_ERR_DEVICE;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // per-launch event profiler
@@ -1465,8 +1478,9 @@ extern "C" int lw_avg_finalize(lw_handle *h) {
     fclose(mf);
     if (!have || count <= 0) return LW_OK;
     std::string dir = h->avg_dir + "/combined";
-    const std::string cmd = "mkdir -p '" + dir + "'";
-    if (system(cmd.c_str()) != 0) dir = h->avg_dir;
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    if (ec) dir = h->avg_dir;
     write_fav(h, dir + "/" + fav_name(ihistinit, ihistend), acc, timeinit, timeend, ihistinit, ihistend);
     return LW_OK;
   } catch (int code) {
@@ -1478,32 +1492,40 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
   if (!h || nsteps < 0) return fail(LW_ERR_ARG, "bad argument");
   ProfScope ps(h->prof);
   try {
+    (void)hipGetLastError();  // clear what earlier calls on this thread left
     for (int64_t n = 0; n < nsteps; ++n) {
       h->cyc_max = 0;
       h->cyc_sum = 0;
       stepinit(h);
+      check_launches("stepinit");
       for (int ms = 1; ms <= 3; ++ms) {
         substepinit(h, ms);
         hipEvent_t *ev = h->tev[ms - 1];
         HIPCHK(hipEventRecord(ev[0], h->s));
         phase_sgs(h);
+        check_launches("sgscalc");
         HIPCHK(hipEventRecord(ev[1], h->s));
         HIPCHK(hipEventRecord(ev[2], h->s));
         phase_rhsnlhs(h);
+        check_launches("rhsnlhs");
         HIPCHK(hipEventRecord(ev[3], h->s));
         HIPCHK(hipEventRecord(ev[4], h->s));
         launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
         phase_poisson(h);
+        check_launches("divgs/poisson");
         HIPCHK(hipEventRecord(ev[5], h->s));
         launch_projection(h->s, h->g, h->sub, h->f);
         if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
+        check_launches("ucalc/pcalc");
         HIPCHK(hipEventRecord(ev[6], h->s));
         phase_energy(h);
+        check_launches("rhsnlhs_t");
         HIPCHK(hipEventRecord(ev[7], h->s));
         if (ms == 3) h->tev_valid = true;
       }
       end_of_step(h);
       if (h->st.iavg == 1) field_avg(h);
+      check_launches("end of step");
       if (g_prof) {
         HIPCHK(hipStreamSynchronize(h->s));
         g_prof->flush();
@@ -1522,6 +1544,7 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
   if (!h) return fail(LW_ERR_ARG, "null handle");
   ProfScope ps(h->prof);
   try {
+    (void)hipGetLastError();
     if (phase == 0) {
       h->cyc_max = 0;
       h->cyc_sum = 0;
@@ -1539,6 +1562,7 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
       end_of_step(h);
       read_stats(h, nullptr);
     } else return fail(LW_ERR_ARG, "unknown phase");
+    check_launches("lw_phase");
     HIPCHK(hipStreamSynchronize(h->s));
     if (g_prof) g_prof->flush();
     return LW_OK;

@@ -25,7 +25,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import case as C
-from .solver import Server
+from .solver import CASE_DONE, Server
 
 
 @dataclasses.dataclass
@@ -117,18 +117,26 @@ def run_stream(cases: List[ServedCase], dirs: List[str], budget_bytes: int, slot
                 if wait > 0:
                     time.sleep(wait)
             tickets.append(srv.submit(d, c.nsteps, fld_path(d) if write_fields else None, body=c.body))
-        res = [srv.wait(t) for t in tickets]
+        res = [srv.wait(t, raise_on_error=False) for t in tickets]
         info = srv.stats()
     finally:
         srv.close()
     wall = time.perf_counter() - t0
-    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
-    updates = sum(r["cells"] * r["steps_done"] for r in res)
-    first = min(r["t_submit"] for r in res)
-    last = max(r["t_done"] for r in res)
+    # failed cases are counted and reported, and left out of the latency and
+    # throughput figures (their partial steps are not a solution)
+    done = [r for r in res if r["status"] == CASE_DONE]
+    failed = [dict(ticket=r["ticket"], status=r["status"], error=r.get("error", "")) for r in res
+              if r["status"] != CASE_DONE]
+    if not done:
+        return dict(results=res, info=info, wall=wall, p50=float("nan"), p99=float("nan"), cell_updates=0,
+                    throughput=0.0, failed=failed)
+    lat = np.array([r["t_done"] - r["t_submit"] for r in done])
+    updates = sum(r["cells"] * r["steps_done"] for r in done)
+    first = min(r["t_submit"] for r in done)
+    last = max(r["t_done"] for r in done)
     return dict(results=res, info=info, wall=wall, p50=float(np.percentile(lat, 50)),
                 p99=float(np.percentile(lat, 99)), cell_updates=updates,
-                throughput=updates / (last - first))
+                throughput=updates / (last - first), failed=failed)
 
 
 def run_alone(case: ServedCase, solver_dir: str, budget_bytes: Optional[int] = None) -> dict:

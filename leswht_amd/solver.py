@@ -182,6 +182,10 @@ def load_library(path: str = LIB_PATH):
     return L
 
 
+def last_error() -> str:
+    return load_library().lw_last_error().decode(errors="replace")
+
+
 def _check(rc: int):
     if rc != 0:
         msg = load_library().lw_last_error().decode(errors="replace")
@@ -311,12 +315,18 @@ class Server:
         _check(load_library().lw_server_poll(self._s, ticket, C.byref(r)))
         return r.as_dict()
 
-    def wait(self, ticket: int) -> dict:
-        """Blocks until the case is done; raises if it failed."""
+    def wait(self, ticket: int, raise_on_error: bool = True) -> dict:
+        """Blocks until the case is done. A case that failed raises, or with
+        ``raise_on_error=False`` comes back with its negative status and the
+        library's message under ``error``."""
         r = lw_case_result()
         _check(load_library().lw_server_wait(self._s, ticket, C.byref(r)))
         if r.status != CASE_DONE:
-            _check(int(r.status))
+            if raise_on_error:
+                _check(int(r.status))
+            d = r.as_dict()
+            d["error"] = last_error()
+            return d
         return r.as_dict()
 
     def stats(self) -> dict:

@@ -83,6 +83,51 @@ def test_gpu_steps_match_oracle(dims, mode, nsteps, nlev):
         assert rel(fo[k], fg[k]) <= TOL, k
 
 
+def test_gpu_c2_kernel_shapes_match_oracle():
+    """96x256x64 runs the headline workload's kernel instances: the
+    multigrid line solve at 256 rows (4 rows per lane, no row guards), the
+    64-point z-DFT pair, the y sweep at 8 rows per lane and the z sweep at 2.
+    Two LES steps at NLEV 3: fields within 1e-10 of the oracle with the same
+    cycle counts and dt."""
+    import oracle
+    view = _view((96, 256, 64), "les", 2, 3)
+    print("c2 kernel shapes 96x256x64", flush=True)
+    o = oracle.Oracle(view)
+    g = Solver.from_case(view)
+    do = o.step(2)
+    dg = g.step(2)
+    assert dg["mg_cycles_max"] == do.mg_cycles_max
+    assert dg["mg_cycles_sum"] == do.mg_cycles_sum
+    assert dg["dt"] == pytest.approx(do.dt, rel=1e-13)
+    fo, fg = o.fields(), g.fields()
+    for k in "uvwpt":
+        err = rel(fo[k], fg[k])
+        assert err <= TOL, (k, err)
+
+
+def test_gpu_full_c2_step_matches_oracle():
+    """One step of the 384x256x64 LES workload itself (domain [-15,30] x
+    [-15,15] x [0,pi], CFL-adaptive dt, NLEV 5, dynamic SGS): fields within
+    1e-10 of the oracle, identical cycle counts and dt."""
+    import oracle
+    from tests.helpers import c2_spec
+    view = C.case_view(C.build_case(c2_spec(1)))
+    print("c2 full step 384x256x64", flush=True)
+    g = Solver.from_case(view)
+    dg = g.step(1)
+    fg = g.fields()
+    g.close()
+    o = oracle.Oracle(view)
+    do = o.step(1)
+    assert dg["mg_cycles_max"] == do.mg_cycles_max
+    assert dg["mg_cycles_sum"] == do.mg_cycles_sum
+    assert dg["dt"] == pytest.approx(do.dt, rel=1e-13)
+    fo = o.fields()
+    for k in "uvwpt":
+        err = rel(fo[k], fg[k])
+        assert err <= TOL, (k, err)
+
+
 def test_gpu_full_c1_size_matches_oracle():
     """192x128x16 (the plain FP64 configuration's grid), one step."""
     import oracle

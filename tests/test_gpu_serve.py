@@ -45,6 +45,7 @@ def test_gpu_served_cases_equal_alone_and_in_the_mix():
             alone.append(serve.run_alone(c, d))
         print("mix of", len(MIX), flush=True)
         out = serve.run_stream(MIX, mix_dirs, budget_bytes=64 << 30, slots=len(MIX), honour_arrivals=False)
+        assert not out["failed"], out["failed"]
         assert out["info"]["peak_running"] >= 2  # the cases did overlap
         for c, a, m, da, dm in zip(MIX, alone, out["results"], alone_dirs, mix_dirs):
             assert a["steps_done"] == m["steps_done"] == c.nsteps
@@ -66,6 +67,32 @@ def test_gpu_served_cases_equal_alone_and_in_the_mix():
         assert np.array_equal(fa["u"][1:n1 + 1], f["u"][1:n1 + 1])
         assert np.array_equal(fa["p"][1:n1, 1:n2, 1:n3], f["p"][1:n1, 1:n2, 1:n3])
         assert np.array_equal(fa["t"][1:n1, 1:n2, 1:n3], f["t"][1:n1, 1:n2, 1:n3])
+
+
+def test_gpu_cases_sharing_a_sweep_instance_equal_alone_and_in_the_mix():
+    """A 256-row and a 192-row case run the same register-resident y-sweep
+    instance (8 rows per lane), whose LDS limit is a property of the kernel:
+    creating the smaller case while the larger one runs must not refuse the
+    larger one's sweeps. Both give the bits they give alone."""
+    pair = [serve.ServedCase("big", _spec((32, 256, 8), "cylinder", (0.5,), re=300.0), 6, 0.0),
+            serve.ServedCase("small", _spec((32, 192, 8), "ellipse", (0.5, 0.3, 0.8), re=200.0), 40, 0.0)]
+    with tempfile.TemporaryDirectory() as root:
+        alone_dirs = serve.prepare(pair, os.path.join(root, "alone"))
+        mix_dirs = serve.prepare(pair, os.path.join(root, "mix"))
+        alone = []
+        for c, d in zip(pair, alone_dirs):
+            print("alone", c.name, flush=True)
+            alone.append(serve.run_alone(c, d))
+        print("mix", flush=True)
+        out = serve.run_stream(pair, mix_dirs, budget_bytes=64 << 30, slots=2, honour_arrivals=False)
+        assert not out["failed"], out["failed"]
+        for c, a, m, da, dm in zip(pair, alone, out["results"], alone_dirs, mix_dirs):
+            assert a["status"] == m["status"] == 0, c.name
+            assert a["steps_done"] == m["steps_done"] == c.nsteps
+            assert a["mg_cycles_sum"] == m["mg_cycles_sum"], c.name
+            fa, fm = _fields(da), _fields(dm)
+            for k in ("u", "v", "w", "p", "t"):
+                assert np.array_equal(fa[k], fm[k]), (c.name, k)
 
 
 @pytest.mark.parametrize("body,args,iles", [("square", (0.5, 0.6), 1), ("ellipse", (0.5, 0.25, 1.2), 0)])
