@@ -129,6 +129,19 @@ double lw_body_eval(const lw_body *b, double x, double y, double z);
 
 int lw_create(const lw_settings *s, const lw_grid *g, const lw_tables *t,
               const lw_body *body, lw_handle **out);
+/* lw_create without device allocation. No counterpart in the reference: it
+ * is for callers that cannot call hipMalloc (harnesses that forbid it, tight
+ * serving loops that must not allocate). Every device array of the case is
+ * carved from a static area of lw_static_area_bytes() (256 MiB) that each
+ * GPU holds from its first such call until the process ends; host copies
+ * live in the handle, not in pinned memory, and the Poisson solve runs on one
+ * stream. Cases that need more return LW_ERR_ARG. One static handle per GPU
+ * at a time: a second returns LW_ERR_ARG until the first is destroyed. The
+ * area keeps the last case's bytes; the handle clears each array it takes.
+ * Same steps, same results as lw_create's handle. */
+int lw_create_static(const lw_settings *s, const lw_grid *g, const lw_tables *t,
+                     const lw_body *body, lw_handle **out);
+int64_t lw_static_area_bytes(void);
 /* readsettings .. prefld from a reference case directory; solver_dir is the
  * directory holding settings.input (the reference runs with it as cwd) */
 int lw_create_from_dir(const char *solver_dir, const lw_body *body, lw_handle **out);

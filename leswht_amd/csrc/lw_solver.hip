@@ -20,6 +20,9 @@
 #include "lw_kernels.h"
 
 namespace lw {
+size_t static_area_bytes();
+hipError_t static_area_acquire(int *dev, void **ptr, size_t *bytes);
+void static_area_release(int dev);
 void launch_sgs_dvmg_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall);
 void launch_sgs_edm_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall);
 }  // namespace lw
@@ -195,6 +198,15 @@ struct lw_handle {
   bool tev_valid = false;  // tev hold a whole step recorded by lw_step
   Prof *prof = nullptr;
   double *h_red = nullptr;                 // pinned
+  // lw_create_static: every device array comes from the device's static
+  // area (arena, a device address) instead of hipMalloc, and the host copies
+  // live in the handle itself instead of pinned memory
+  unsigned char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  int arena_dev = -1;
+  unsigned long long hs_sumres[256] = {0};
+  int hs_flags[256] = {0}, hs_flags2[256] = {0};
+  double hs_red[64] = {0};
   // field_avg state (misc_hist.f90:287-527; mod_common avg_started,
   // npriavg_count, timeinit, ihistinit); avg_dir empty: no fav files
   AvgBuf avg{};
@@ -208,6 +220,19 @@ struct lw_handle {
   T *dalloc(size_t n) {
     void *p = nullptr;
     size_t b = sizeof(T) * (n ? n : 1);
+    if (arena) {
+      const size_t off = (arena_used + 255) & ~(size_t)255;
+      if (off + b > arena_cap) {
+        g_err = "lw_create_static: the case needs more than the " + std::to_string(arena_cap >> 20) +
+                " MiB static device area (use lw_create)";
+        throw LW_ERR_ARG;
+      }
+      p = arena + off;
+      arena_used = off + b;
+      HIPCHK(hipMemsetAsync(p, 0, b, s));
+      bytes += (int64_t)b;
+      return (T *)p;
+    }
     HIPCHK(hipMalloc(&p, b));
     HIPCHK(hipMemsetAsync(p, 0, b, s));
     allocs.push_back(p);
@@ -675,7 +700,8 @@ void lw_set_last_error(const std::string &msg) { g_err = msg; }
 
 extern "C" {
 
-int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body, lw_handle **out) {
+static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body,
+                         lw_handle **out, bool in_static_area) {
   if (!s || !gr || !t || !out) return fail(LW_ERR_ARG, "null argument");
   *out = nullptr;
   lw_handle *h = nullptr;
@@ -686,6 +712,23 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(LW_ERR_DEVICE, "no HIP device");
     h = new lw_handle();
+    if (in_static_area) {
+      void *a = nullptr;
+      size_t cap = 0;
+      int dev = -1;
+      const hipError_t e = static_area_acquire(&dev, &a, &cap);
+      if (e == hipErrorAlreadyAcquired) {
+        delete h;
+        return fail(LW_ERR_ARG, "lw_create_static: another handle holds this device's static area");
+      }
+      if (e != hipSuccess) {
+        delete h;
+        return fail(LW_ERR_DEVICE, std::string("lw_create_static: static area: ") + hipGetErrorString(e));
+      }
+      h->arena = (unsigned char *)a;
+      h->arena_cap = cap;
+      h->arena_dev = dev;
+    }
     h->st = *s;
     if (body) h->body = *body;
     else {
@@ -706,13 +749,23 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
     // queues to other cases (LW_MG_SPLIT or lw_set_option override)
     h->mg_split = (int64_t)gr->n1m * gr->n2m >= 65536;
     if (const char *e = getenv("LW_MG_SPLIT")) h->mg_split = atoi(e) != 0;
+    // the two-stream solve polls copies into pinned memory; a static handle
+    // has none, so it solves on one stream (the same kernels and results)
+    if (h->arena) h->mg_split = 0;
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     for (int q = 0; q < 3; ++q)
       for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->tev[q][e]));
-    HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
-    HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
-    HIPCHK(hipHostMalloc((void **)&h->h_flags2, 256 * sizeof(int)));
-    HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
+    if (h->arena) {
+      h->h_sumres = h->hs_sumres;
+      h->h_flags = h->hs_flags;
+      h->h_flags2 = h->hs_flags2;
+      h->h_red = h->hs_red;
+    } else {
+      HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
+      HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
+      HIPCHK(hipHostMalloc((void **)&h->h_flags2, 256 * sizeof(int)));
+      HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
+    }
     setup(h, gr, t);
     // dttimeinit / rk3coefinit (misc_init.f90:1001-1044)
     h->gamma[1] = 8. / 15.; h->gamma[2] = 5. / 12.; h->gamma[3] = 3. / 4.;
@@ -727,6 +780,17 @@ int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const
   }
 }
 
+int lw_create(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body, lw_handle **out) {
+  return create_handle(s, gr, t, body, out, false);
+}
+
+int lw_create_static(const lw_settings *s, const lw_grid *gr, const lw_tables *t, const lw_body *body,
+                     lw_handle **out) {
+  return create_handle(s, gr, t, body, out, true);
+}
+
+int64_t lw_static_area_bytes(void) { return (int64_t)static_area_bytes(); }
+
 int lw_destroy(lw_handle *h) {
   if (!h) return LW_OK;
   if (h->s) hipStreamSynchronize(h->s);
@@ -737,15 +801,18 @@ int lw_destroy(lw_handle *h) {
   for (int q = 0; q < 3; ++q)
     for (int e = 0; e < 8; ++e)
       if (h->tev[q][e]) hipEventDestroy(h->tev[q][e]);
-  if (h->h_sumres) hipHostFree(h->h_sumres);
-  if (h->h_flags) hipHostFree(h->h_flags);
-  if (h->h_flags2) hipHostFree(h->h_flags2);
-  if (h->h_red) hipHostFree(h->h_red);
+  if (!h->arena) {
+    if (h->h_sumres) hipHostFree(h->h_sumres);
+    if (h->h_flags) hipHostFree(h->h_flags);
+    if (h->h_flags2) hipHostFree(h->h_flags2);
+    if (h->h_red) hipHostFree(h->h_red);
+  }
   delete h->prof;
   if (h->ev_fork) hipEventDestroy(h->ev_fork);
   if (h->ev_join) hipEventDestroy(h->ev_join);
   if (h->s2) hipStreamDestroy(h->s2);
   if (h->s) hipStreamDestroy(h->s);
+  if (h->arena) static_area_release(h->arena_dev);
   delete h;
   return LW_OK;
 }

@@ -121,7 +121,8 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
            "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option", "lw_autotune",
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
-           "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy")
+           "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
+           "lw_create_static", "lw_static_area_bytes")
 
 _lib = None
 
@@ -137,6 +138,8 @@ def load_library(path: str = LIB_PATH):
     H = C.c_void_p
     L.lw_create.argtypes = [C.POINTER(lw_settings), C.POINTER(lw_grid), C.POINTER(lw_tables),
                             C.POINTER(lw_body), C.POINTER(H)]
+    L.lw_create_static.argtypes = L.lw_create.argtypes
+    L.lw_static_area_bytes.restype = _I
     L.lw_create_from_dir.argtypes = [C.c_char_p, C.POINTER(lw_body), C.POINTER(H)]
     L.lw_destroy.argtypes = [H]
     L.lw_set_fields.argtypes = [H, _PD, _PD, _PD, _PD, _PD, _I, _I, _D, _D]
@@ -358,9 +361,11 @@ class Solver:
 
     # -- construction ------------------------------------------------------
     @classmethod
-    def from_case(cls, cd: dict, body=None) -> "Solver":
+    def from_case(cls, cd: dict, body=None, static_area: bool = False) -> "Solver":
         """From :func:`leswht_amd.case.load_case_dir` output (arrays as the
-        reference holds them after start-up)."""
+        reference holds them after start-up). ``static_area``: create it with
+        lw_create_static (no device allocation; the device's static area,
+        one such case at a time)."""
         L = load_library()
         s, g = cd["settings"], cd["grid"]
         keep = []
@@ -396,7 +401,8 @@ class Solver:
         tb.nwall_dvm = _pi(nw)
         body = body_struct(body if body is not None else cd.get("body"))
         h = C.c_void_p()
-        _check(L.lw_create(C.byref(st), C.byref(gr), C.byref(tb), C.byref(body), C.byref(h)))
+        create = L.lw_create_static if static_area else L.lw_create
+        _check(create(C.byref(st), C.byref(gr), C.byref(tb), C.byref(body), C.byref(h)))
         self = cls(h, bool(s.ihtrans), (g["n1"], g["n2"], g["n3"]))
         f = cd["fields"]
         self.set_fields(f["u"], f["v"], f["w"], f["p"], f.get("t"), cd["ihist"], cd["m"], cd["time"], cd["dt"])
