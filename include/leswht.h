@@ -181,6 +181,22 @@ int lw_avg_finalize(lw_handle *h);
 int lw_get_dims(const lw_handle *h, int64_t n[3]);
 /* history quantities of the last completed step (see lw_history) */
 int lw_get_history(const lw_handle *h, lw_history *out);
+/* tracer cells, the TRACE block of settings.input (misc_init.f90:47-52;
+ * lw_create_from_dir sets it from the file): ntrace <= 64 cells (i, j, k)
+ * each, traced every ntr steps by the host (solver.f90:171) */
+int lw_set_trace(lw_handle *h, int64_t ntrace, int64_t ntr, const int64_t *pts);
+/* ntrace, ntr and (pts not NULL) the 3*ntrace cell indices */
+int lw_trace_info(const lw_handle *h, int64_t *ntrace, int64_t *ntr, int64_t *pts);
+/* what tracer (misc_hist.f90:184-209) writes for the current fields:
+ * out[c*ntrace + n] for c = 0..3: u, v, w at the cell centre, then p;
+ * cap >= 4*ntrace */
+int lw_get_trace(lw_handle *h, double *out, int64_t cap);
+/* rows mg2d appends to mgftresiduemax.dat during the last step
+ * (poiss_z_ft.f90:293-297): for each substep, kv = 1..min(3, n3m/2+1) with
+ * KK (V-cycles done; MGITR+1 when the limit ended the solve) and RM (the
+ * final max-norm residual times dtconst). Returns the row count; copies at
+ * most cap rows into the arrays that are not NULL. */
+int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t *kk, double *rm, int64_t cap);
 /* the settings the handle runs with (settings.input + boundary.input +
  * ibmpre_prdic.bin as readsettings/readbcs hold them, misc_init.f90:19-158) */
 int lw_get_settings(const lw_handle *h, lw_settings *out);

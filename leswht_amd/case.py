@@ -593,6 +593,9 @@ class Settings:
     idecomp: int = 0
     gridfile: str = "../output/grid/grid.bin"
     prev_fld: str = "../output/field/fld000000"
+    # TRACE block: cells (i, j, k) traced every ntr steps (at most 64)
+    trace: tuple = ()
+    ntr: int = 1
     # boundary.input
     bc_ybtm: int = 1
     bc_ytop: int = 1
@@ -637,7 +640,8 @@ class Settings:
             f'"{self.gridfile}"',
             f'"{self.prev_fld}"',
             "TRACE   NTR",
-            "0       1",
+            f"{len(self.trace)} {self.ntr}",
+        ] + [f"{int(i)} {int(j)} {int(k)}" for i, j, k in self.trace] + [
             "",
         ]
         return "\n".join(lines)
@@ -947,6 +951,9 @@ def parse_settings(text: str, boundary: str, prdic: str) -> Settings:
     r.read(1)
     s.gridfile = r.read(1)[0][:25]
     s.prev_fld = r.read(1)[0][:25]
+    r.read(1)
+    v = r.read(2); ntrace, s.ntr = int(v[0]), int(v[1])
+    s.trace = tuple(tuple(int(x) for x in r.read(3)) for _ in range(ntrace))
     b = ListDirected(boundary)
     b.read(1); s.bc_ybtm, s.bc_ytop = (int(x) for x in b.read(2))
     b.read(1); s.bc_zbtm, s.bc_ztop = (int(x) for x in b.read(2))
@@ -1007,12 +1014,72 @@ def load_case_dir(root: str) -> dict:
             nz["knz"] = np.array([int(x) for x in r.read(n)], np.int64)
         mask = np.array(open(os.path.join(root, "output/ibmpre/ibmpre_wallfdvm.bin")).read().split(), np.int64)
         nz["nwall"] = mask.reshape(g["n3m"], g["n2m"], g["n1m"]).transpose(2, 1, 0).copy()
+    if s.iread == 0:
+        # makefld with the reference's compiled-in body, the cylinder
+        return dict(settings=s, grid=g, tables=tab, nutzero=nz, fields=makefld(s, g, body_cylinder),
+                    ihist=0, m=0, time=0.0, dt=s.dt_size)
     fl = read_fld(os.path.normpath(os.path.join(sol, s.prev_fld)), heat=bool(s.ihtrans))
     if fl["n"] != (g["n1"], g["n2"], g["n3"]):
         raise ValueError("PREVIOUS FIELD DOES NOT MATCH WITH THE GRID SYSTEM.")
     fields = apply_prefld(fl, s, g)
     return dict(settings=s, grid=g, tables=tab, nutzero=nz, fields=fields,
                 ihist=fl["ihist"], m=fl["m"], time=fl["time"], dt=fl["dt"])
+
+
+def makefld(s: Settings, g: dict, body: Callable, body_args: Tuple = ()) -> dict:
+    """The initial field of IREAD = 0 (makefld, misc_init.f90:443-607): u =
+    UDRV_I where the level set at the u point is >= 1e-10 (ICH = 0), v = w =
+    p = 0, u = 0 on the rows next to wall boundaries; T = TSOL_I, TFLU_I at
+    fluid cell centres of rows 1..n2m, the inlet ghost column by the level
+    set, y (and z) ghost rows by the thermal BCs. The u loop reads x(0), which
+    the reference never allocated (x(1:n1), mod_common.f90:124); x(0) = 0 is
+    taken here, and the u column i = 0 is read by no routine of the step."""
+    n1, n2, n3, n2m, n3m = g["n1"], g["n2"], g["n3"], g["n2m"], g["n3m"]
+    shape = (n1 + 1, n2 + 1, n3 + 1)
+    x0 = np.concatenate([[0.0], g["x"]])  # Fortran x(0..n1)
+    xu, yc, zc = np.meshgrid(x0, g["ymp"], g["zmp"], indexing="ij")
+    phi = body(xu, yc, zc, *body_args)
+    u = np.where(phi >= 1.e-10, s.udrv_i, 0.0)
+    if s.bc_ybtm == 0:
+        u[1:, 0:2, :] = 0.0
+    if s.bc_ytop == 0:
+        u[1:, n2m, :] = 0.0
+        u[1:, n2, :] = 0.0
+    if s.bc_zbtm == 0:
+        u[1:, :, 0:2] = 0.0
+    if s.bc_ztop == 0:
+        u[1:, :, n3m] = 0.0
+        u[1:, :, n3] = 0.0
+    out = dict(u=u, v=np.zeros(shape), w=np.zeros(shape), p=np.zeros(shape))
+    if not s.ihtrans:
+        return out
+    t = np.full(shape, float(s.tsol_i))
+    xc, yc, zc = np.meshgrid(g["xmp"], g["ymp"], g["zmp"], indexing="ij")
+    fluid = body(xc, yc, zc, *body_args) >= 1.e-10
+    inner = fluid[:, 1:n2m + 1, :]
+    t[:, 1:n2m + 1, :] = np.where(inner, s.tflu_i, t[:, 1:n2m + 1, :])
+    if s.xprdic == 0:
+        t[0] = np.where(fluid[0], s.tflu_i, s.tsol_i)
+    if s.yprdic == 0:
+        if s.bc_t_ybtm == 0:
+            t[:, 0, :] = s.val_t_ybtm
+        elif s.bc_t_ybtm == 1:
+            t[:, 0, :] = t[:, 1, :] - s.val_t_ybtm / g["c2cyi"][1]
+        if s.bc_t_ytop == 0:
+            t[:, n2, :] = s.val_t_ytop
+        elif s.bc_t_ytop == 1:
+            t[:, n2, :] = t[:, n2m, :] + s.val_t_ytop / g["c2cyi"][n2]
+    if s.zprdic == 0:
+        if s.bc_t_zbtm == 0:
+            t[:, :, 0] = s.val_t_zbtm
+        elif s.bc_t_zbtm == 1:
+            t[:, :, 0] = t[:, :, 1] - s.val_t_zbtm / g["c2czi"][1]
+        if s.bc_t_ztop == 0:
+            t[:, :, n3] = s.val_t_ztop
+        elif s.bc_t_ztop == 1:
+            t[:, :, n3] = t[:, :, n3m] + s.val_t_ztop / g["c2czi"][n3]
+    out["t"] = t
+    return out
 
 
 def apply_prefld(fl: dict, s: Settings, g: dict) -> dict:
@@ -1080,7 +1147,10 @@ def case_view(case: Case) -> dict:
     if s.ihtrans:
         fl["t"] = f["t"]
     st = dataclasses.replace(s, xprdic=0, yprdic=0, zprdic=1, iintp=1)
-    fields = apply_prefld(fl, st, g)
+    if s.iread == 0:
+        fields = makefld(st, g, BODIES[case.spec.body], case.spec.body_args)
+    else:
+        fields = apply_prefld(fl, st, g)
     return dict(settings=st, grid=g, tables=tabs, nutzero=nz, fields=fields,
                 ihist=0, m=0, time=0.0, dt=s.dt_size,
                 body=body_params(case.spec.body, case.spec.body_args))

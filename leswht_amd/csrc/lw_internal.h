@@ -85,6 +85,7 @@ struct MG {
   int *done;               // n3mh: 0 iterating, 1 converged, 2 MGITR reached
   int *cyc;                // n3mh: V-cycles completed
   int *pcsel;              // n3mh: 1 when pc2 holds the current pc
+  double *rfin;            // n3mh: the max norm of the toplevel that ended each wavenumber's solve
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 

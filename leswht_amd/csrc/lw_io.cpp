@@ -116,6 +116,9 @@ std::string trim25(std::string s) {
 struct Case {
   lw_settings s{};
   std::string gridfile, prev_fld;
+  // TRACE block (misc_init.f90:47-52)
+  int64_t ntrace = 0, ntr = 1;
+  std::vector<int64_t> trpts;
   // grid
   int64_t n1 = 0, n2 = 0, n3 = 0, n1m = 0, n2m = 0, n3m = 0;
   double xl = 0, yl = 0, zl = 0;
@@ -167,6 +170,15 @@ void read_settings(Case &c, const std::string &dir) {
   r.skip();
   c.gridfile = trim25(r.read(1)[0]);
   c.prev_fld = trim25(r.read(1)[0]);
+  r.skip();
+  b = r.ints(2);
+  c.ntrace = b[0];
+  c.ntr = b[1];
+  if (c.ntrace < 0 || c.ntrace > 64) throw IoError{"settings.input: NTRACE must be in 0..64 (trpts(64, 3))"};
+  for (int64_t n = 0; n < c.ntrace; ++n) {
+    b = r.ints(3);
+    c.trpts.insert(c.trpts.end(), b.begin(), b.end());
+  }
   ListReader bc(join(dir, "boundary.input"));
   bc.skip(); b = bc.ints(2); s.bc_ybtm = b[0]; s.bc_ytop = b[1];
   bc.skip(); b = bc.ints(2); s.bc_zbtm = b[0]; s.bc_ztop = b[1];
@@ -337,6 +349,66 @@ void read_field(Case &c, const std::string &dir) {
   }
 }
 
+// makefld (misc_init.f90:443-607), the initial field when IREAD = 0: u =
+// UDRV_I at fluid u points (the ICH = 0 profile), v = w = p = 0, no-slip
+// rows at wall boundaries; T = TSOL_I, then TFLU_I at fluid cell centres of
+// rows 1..n2m, the inlet ghost by the level set and the y (and z) ghosts by
+// the thermal BCs. The u loop starts at i = 0 and reads x(0), which the
+// reference never allocated (x is x(1:n1), mod_common.f90:124); the u ghost
+// column i = 0 is read by no routine of the step, and here x(0) is taken as 0.
+void make_field(Case &c, const lw_body *body) {
+  const lw_settings &s = c.s;
+  const int64_t NX = c.n1 + 1, NY = c.n2 + 1, NZ = c.n3 + 1;
+  const size_t np = (size_t)(NX * NY * NZ);
+  c.u.assign(np, 0.0); c.v.assign(np, 0.0); c.w.assign(np, 0.0); c.p.assign(np, 0.0);
+  auto P = [&](int64_t i, int64_t j, int64_t k) { return (size_t)(i + NX * (j + NY * k)); };
+  auto fb = [&](double x, double y, double z) { return lw_body_eval(body, x, y, z); };
+  c.ihist = 0;
+  c.m = 0;
+  c.time = 0.0;
+  c.dt = s.dt_size;
+  for (int64_t k = 0; k <= c.n3; ++k)
+    for (int64_t j = 0; j <= c.n2; ++j)
+      for (int64_t i = 0; i <= c.n1; ++i)
+        c.u[P(i, j, k)] = fb(c.x[i], c.ymp[j], c.zmp[k]) >= 1.e-10 ? s.udrv_i : 0.0;
+  for (int64_t k = 0; k <= c.n3; ++k)
+    for (int64_t i = 1; i <= c.n1; ++i) {
+      if (s.bc_ybtm == 0) c.u[P(i, 0, k)] = c.u[P(i, 1, k)] = 0.0;
+      if (s.bc_ytop == 0) c.u[P(i, c.n2m, k)] = c.u[P(i, c.n2, k)] = 0.0;
+    }
+  for (int64_t j = 0; j <= c.n2; ++j)
+    for (int64_t i = 1; i <= c.n1; ++i) {
+      if (s.bc_zbtm == 0) c.u[P(i, j, 0)] = c.u[P(i, j, 1)] = 0.0;
+      if (s.bc_ztop == 0) c.u[P(i, j, c.n3m)] = c.u[P(i, j, c.n3)] = 0.0;
+    }
+  if (s.ihtrans != 1) return;
+  c.t.assign(np, s.tsol_i);
+  for (int64_t k = 0; k <= c.n3; ++k)
+    for (int64_t j = 1; j <= c.n2m; ++j)
+      for (int64_t i = 0; i <= c.n1; ++i)
+        if (fb(c.xmp[i], c.ymp[j], c.zmp[k]) >= 1.e-10) c.t[P(i, j, k)] = s.tflu_i;
+  if (s.xprdic == 0)
+    for (int64_t k = 0; k <= c.n3; ++k)
+      for (int64_t j = 0; j <= c.n2; ++j)
+        c.t[P(0, j, k)] = fb(c.xmp[0], c.ymp[j], c.zmp[k]) >= 1.e-10 ? s.tflu_i : s.tsol_i;
+  if (s.yprdic == 0)
+    for (int64_t k = 0; k <= c.n3; ++k)
+      for (int64_t i = 0; i <= c.n1; ++i) {
+        if (s.bc_t_ybtm == 0) c.t[P(i, 0, k)] = s.val_t_ybtm;
+        else if (s.bc_t_ybtm == 1) c.t[P(i, 0, k)] = c.t[P(i, 1, k)] - s.val_t_ybtm / c.c2cyi[1];
+        if (s.bc_t_ytop == 0) c.t[P(i, c.n2, k)] = s.val_t_ytop;
+        else if (s.bc_t_ytop == 1) c.t[P(i, c.n2, k)] = c.t[P(i, c.n2m, k)] + s.val_t_ytop / c.c2cyi[c.n2];
+      }
+  if (s.zprdic == 0)
+    for (int64_t j = 0; j <= c.n2; ++j)
+      for (int64_t i = 0; i <= c.n1; ++i) {
+        if (s.bc_t_zbtm == 0) c.t[P(i, j, 0)] = s.val_t_zbtm;
+        else if (s.bc_t_zbtm == 1) c.t[P(i, j, 0)] = c.t[P(i, j, 1)] - s.val_t_zbtm / c.c2czi[1];
+        if (s.bc_t_ztop == 0) c.t[P(i, j, c.n3)] = s.val_t_ztop;
+        else if (s.bc_t_ztop == 1) c.t[P(i, j, c.n3)] = c.t[P(i, j, c.n3m)] + s.val_t_ztop / c.c2czi[c.n3];
+      }
+}
+
 }  // namespace
 
 extern "C" int lw_create_from_dir(const char *solver_dir, const lw_body *body, lw_handle **out) {
@@ -347,8 +419,10 @@ extern "C" int lw_create_from_dir(const char *solver_dir, const lw_body *body, l
     read_settings(c, dir);
     read_grid(c, dir);
     read_tables(c, dir);
-    if (c.s.iread != 1) throw IoError{"IREAD = 0 (makefld) is not supported: supply the initial fld file"};
-    read_field(c, dir);
+    lw_body cyl{};
+    cyl.p[0] = 0.5;
+    if (c.s.iread == 1) read_field(c, dir);
+    else make_field(c, body ? body : &cyl);
     lw_grid g{};
     g.n1 = c.n1; g.n2 = c.n2; g.n3 = c.n3; g.n1m = c.n1m; g.n2m = c.n2m; g.n3m = c.n3m;
     g.xl = c.xl; g.yl = c.yl; g.zl = c.zl;
@@ -380,6 +454,7 @@ extern "C" int lw_create_from_dir(const char *solver_dir, const lw_body *body, l
     if (c.s.iavg == 1) lw_set_avg_dir(*out, join(dir, "../output/field_avg").c_str());
     rc = lw_set_fields(*out, c.u.data(), c.v.data(), c.w.data(), c.p.data(), c.s.ihtrans == 1 ? c.t.data() : nullptr,
                        c.ihist, c.m, c.time, c.dt);
+    if (rc == 0) rc = lw_set_trace(*out, c.ntrace, c.ntr, c.trpts.data());
     if (rc != 0) {
       lw_destroy(*out);
       *out = nullptr;

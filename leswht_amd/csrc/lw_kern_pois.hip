@@ -1114,6 +1114,7 @@ __global__ void k_mg_check(MG mg, double test, int id, int flip) {
       const double r = __longlong_as_double((long long)mg.sumres[q]);
       if (r < test) mg.done[q] = 1;
       else if (c >= mg.mgitr) mg.done[q] = 2;
+      if (mg.done[q]) mg.rfin[q] = r;
     }
     mg.sumres[q] = 0ull;
   }
@@ -1205,7 +1206,10 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
     if (tile == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
       mg.cyc[kv - 1] = cyc;
       mg.pcsel[kv - 1] ^= 1;  // k_mg_top wrote the new pc to the other buffer
-      if (dn) mg.done[kv - 1] = dn;
+      if (dn) {
+        mg.done[kv - 1] = dn;
+        mg.rfin[kv - 1] = r;
+      }
     }
     if (dn) return;
   }

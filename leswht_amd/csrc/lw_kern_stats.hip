@@ -127,7 +127,24 @@ __global__ __launch_bounds__(256) void k_field_avg(Geo g, Fld f, AvgBuf b, doubl
   a[20][q] = a[20][q] + srsr * dt;
 }
 
+// tracer (misc_hist.f90:184-209): the cell-centre velocities and the
+// pressure of each traced cell (i, j, k) of n: out[c n + q] for c = u, v, w, p
+__global__ void k_trace(Geo g, Fld f, const int *pts, int n, double *out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = pts[3 * q], j = pts[3 * q + 1], k = pts[3 * q + 2];
+  out[q] = 0.5 * (f.u[P(g, i, j, k)] + f.u[P(g, i + 1, j, k)]);
+  out[n + q] = 0.5 * (f.v[P(g, i, j, k)] + f.v[P(g, i, j + 1, k)]);
+  out[2 * n + q] = 0.5 * (f.w[P(g, i, j, k)] + f.w[P(g, i, j, k + 1)]);
+  out[3 * n + q] = f.p[P(g, i, j, k)];
+}
+
 }  // namespace
+
+void launch_trace(hipStream_t s, const Geo &g, const Fld &f, const int *pts, int n, double *out) {
+  if (n <= 0) return;
+  LW_LAUNCH("k_trace", 0, 0, k_trace, dim3(1), dim3(64), 0, s, g, f, pts, n, out);
+}
 
 void launch_field_avg(hipStream_t s, const Geo &g, const Fld &f, const AvgBuf &b, double dt, int heat) {
   const double cells = (double)g.n1m * g.n2m * g.n3m;

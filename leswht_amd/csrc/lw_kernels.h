@@ -114,6 +114,8 @@ struct AvgBuf {
   double *a[kAvgArrays];
 };
 void launch_field_avg(hipStream_t s, const Geo &g, const Fld &f, const AvgBuf &b, double dt, int heat);
+// tracer: u, v, w at the centre and p of n <= 64 cells (pts: i, j, k each)
+void launch_trace(hipStream_t s, const Geo &g, const Fld &f, const int *pts, int n, double *out);
 
 // ---- Poisson (lw_kern_pois.hip) ----
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);

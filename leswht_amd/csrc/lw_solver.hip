@@ -192,6 +192,18 @@ struct lw_handle {
   DiagPlanes dplanes{};
   double *hist = nullptr;  // device: the step's history values and their running averages
   double h_hist[32] = {0};
+  // mgftresiduemax.dat rows of the last step (poiss_z_ft.f90:293-297): per
+  // substep, kv = 1..min(3, n3mh) with its cycle count and final max norm
+  int64_t mgres_n = 0;
+  int64_t mgres_kv[9] = {0}, mgres_kk[9] = {0};
+  double mgres_rm[9] = {0};
+  double *h_rfin = nullptr;  // pinned: rfin of kv = 1..3 after a solve
+  double hs_rfin[4] = {0};
+  // tracer (misc_hist.f90:184-209): cells whose centre values are traced
+  int64_t ntrace = 0, ntr = 1;
+  int64_t trpts[64 * 3] = {0};
+  int *d_trpts = nullptr;
+  double *d_trace = nullptr;
   int hist_written = 0;    // the last step was a history step (mod(ntime, npin) = 0)
   hipEvent_t tev[3][8] = {};  // phase boundaries per substep: sgs, rhsnlhs, divgs+poisson, energy
   double s_phase[4] = {0, 0, 0, 0};
@@ -399,6 +411,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   f.obc = h->dalloc<double>(8 * 64);
   f.dpart = h->dalloc<double>(D_NSUM * 512);
   h->hist = h->dalloc<double>(32);
+  h->d_trpts = h->dalloc<int>(64 * 3);
+  h->d_trace = h->dalloc<double>(64 * 4);
   f.redmax = h->dalloc<unsigned long long>(8);
   if (h->st.iavg == 1) {
     h->navg = h->heat ? kAvgArrays : kAvgArrays - 2;
@@ -653,6 +667,7 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.done = h->dalloc<int>(384);
   mg.cyc = mg.done + 128;
   mg.pcsel = mg.done + 256;
+  mg.rfin = h->dalloc<double>(128);
   std::vector<double> twr(n3m), twi(n3m);
   const double pi2 = 8.0 * atan(1.0);
   for (int q = 0; q < n3m; ++q) {
@@ -760,7 +775,9 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
       h->h_flags = h->hs_flags;
       h->h_flags2 = h->hs_flags2;
       h->h_red = h->hs_red;
+      h->h_rfin = h->hs_rfin;
     } else {
+      HIPCHK(hipHostMalloc((void **)&h->h_rfin, 4 * sizeof(double)));
       HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
       HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
       HIPCHK(hipHostMalloc((void **)&h->h_flags2, 256 * sizeof(int)));
@@ -806,6 +823,7 @@ int lw_destroy(lw_handle *h) {
     if (h->h_flags) hipHostFree(h->h_flags);
     if (h->h_flags2) hipHostFree(h->h_flags2);
     if (h->h_red) hipHostFree(h->h_red);
+    if (h->h_rfin) hipHostFree(h->h_rfin);
   }
   delete h->prof;
   if (h->ev_fork) hipEventDestroy(h->ev_fork);
@@ -1120,6 +1138,7 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
   HIPCHK(hipEventRecord(h->ev_join, h->s2));
   HIPCHK(hipStreamWaitEvent(h->s, h->ev_join, 0));
   HIPCHK(hipMemcpyAsync(h->h_flags, h->mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(h->h_rfin, h->mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
 }
 
@@ -1158,6 +1177,7 @@ static void phase_poisson(lw_handle *h) {
       }
       launched += batch;
       HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipMemcpyAsync(h->h_rfin, mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
       HIPCHK(hipStreamSynchronize(h->s));
       all_done = true;
       for (int q = 0; q < mg.n3mh; ++q) all_done = all_done && h->h_flags[q] != 0;
@@ -1165,6 +1185,7 @@ static void phase_poisson(lw_handle *h) {
     }
     if (launched == 0) {  // every wavenumber passed on the first toplevel
       HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipMemcpyAsync(h->h_rfin, mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
       HIPCHK(hipStreamSynchronize(h->s));
     }
   }
@@ -1178,6 +1199,17 @@ static void phase_poisson(lw_handle *h) {
     limit = limit || h->h_flags[q] != 1;
   }
   if (limit) fprintf(stderr, "leswht: ITERATION LIMIT EXCEEDED.\n");
+  // mgftresiduemax.dat: kv <= 3 with KK (the reference's loop index: the
+  // cycles done, or MGITR + 1 when the limit ended the loop) and the final
+  // max norm times dtconst
+  if (h->sub.msub == 1) h->mgres_n = 0;
+  for (int q = 0; q < mg.n3mh && q < 3; ++q) {
+    if (h->mgres_n >= 9) break;
+    const int64_t n = h->mgres_n++;
+    h->mgres_kv[n] = q + 1;
+    h->mgres_kk[n] = h->h_flags[q] == 2 ? (int64_t)mg.mgitr + 1 : h->h_flags[128 + q];
+    h->mgres_rm[n] = h->h_rfin[q] * h->sub.dtconst;
+  }
   if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
   if (cmax > h->cyc_max) h->cyc_max = cmax;
   if (h->mg_split && mg.fused && mg.n3mh >= 2) {
@@ -1664,6 +1696,75 @@ extern "C" int lw_get_history(const lw_handle *h, lw_history *out) {
   out->s_poisson = h->s_phase[2];
   out->s_energy = h->s_phase[3];
   return LW_OK;
+}
+
+// tracer cells from the TRACE block of settings.input (misc_init.f90:47-52):
+// the reference keeps at most 64 (trpts(64, 3), mod_common.f90:9) and reads
+// u(i+1), v(j+1), w(k+1), so each cell must have 0 <= i < n1, 0 <= j < n2,
+// 0 <= k < n3
+extern "C" int lw_set_trace(lw_handle *h, int64_t ntrace, int64_t ntr, const int64_t *pts) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  if (ntrace < 0 || ntrace > 64) return fail(LW_ERR_ARG, "lw_set_trace: NTRACE must be in 0..64");
+  if (ntrace > 0 && (!pts || ntr < 1)) return fail(LW_ERR_ARG, "lw_set_trace: points and NTR >= 1 required");
+  int ip[64 * 3];
+  for (int64_t n = 0; n < ntrace; ++n) {
+    const int64_t i = pts[3 * n], j = pts[3 * n + 1], k = pts[3 * n + 2];
+    if (i < 0 || i >= h->g.n1 || j < 0 || j >= h->g.n2 || k < 0 || k >= h->g.n3)
+      return fail(LW_ERR_ARG, "lw_set_trace: trace cell " + std::to_string(n + 1) + " outside the grid");
+    ip[3 * n] = (int)i; ip[3 * n + 1] = (int)j; ip[3 * n + 2] = (int)k;
+  }
+  try {
+    if (ntrace > 0) {
+      HIPCHK(hipMemcpyAsync(h->d_trpts, ip, sizeof(int) * 3 * ntrace, hipMemcpyHostToDevice, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+    }
+  } catch (int code) {
+    return code;
+  }
+  h->ntrace = ntrace;
+  h->ntr = ntrace > 0 ? ntr : 1;
+  for (int64_t q = 0; q < 3 * ntrace; ++q) h->trpts[q] = pts[q];
+  return LW_OK;
+}
+
+extern "C" int lw_trace_info(const lw_handle *h, int64_t *ntrace, int64_t *ntr, int64_t *pts) {
+  if (!h || !ntrace || !ntr) return fail(LW_ERR_ARG, "null argument");
+  *ntrace = h->ntrace;
+  *ntr = h->ntr;
+  if (pts)
+    for (int64_t q = 0; q < 3 * h->ntrace; ++q) pts[q] = h->trpts[q];
+  return LW_OK;
+}
+
+// the values tracer writes (misc_hist.f90:198-205) for the current fields:
+// out[c ntrace + n], c = u, v, w at the cell centre, then p
+extern "C" int lw_get_trace(lw_handle *h, double *out, int64_t cap) {
+  if (!h || (!out && h->ntrace > 0)) return fail(LW_ERR_ARG, "null argument");
+  if (cap < 4 * h->ntrace) return fail(LW_ERR_ARG, "lw_get_trace: output needs 4 * NTRACE values");
+  if (h->ntrace == 0) return LW_OK;
+  try {
+    (void)hipGetLastError();
+    launch_trace(h->s, h->g, h->f, h->d_trpts, (int)h->ntrace, h->d_trace);
+    check_launches("tracer");
+    HIPCHK(hipMemcpyAsync(out, h->d_trace, sizeof(double) * 4 * h->ntrace, hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    return LW_OK;
+  } catch (int code) {
+    return code;
+  }
+}
+
+// the mgftresiduemax.dat rows of the last step (poiss_z_ft.f90:293-297):
+// returns their number (3 substeps x min(3, n3m/2+1)); cap limits the copy
+extern "C" int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t *kk, double *rm, int64_t cap) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  const int64_t n = h->mgres_n < cap ? h->mgres_n : (cap > 0 ? cap : 0);
+  for (int64_t q = 0; q < n; ++q) {
+    if (kv) kv[q] = h->mgres_kv[q];
+    if (kk) kk[q] = h->mgres_kk[q];
+    if (rm) rm[q] = h->mgres_rm[q];
+  }
+  return h->mgres_n;
 }
 
 extern "C" int lw_get_settings(const lw_handle *h, lw_settings *out) {

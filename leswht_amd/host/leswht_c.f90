@@ -69,6 +69,30 @@ module leswht_c
       type(lw_history), intent(out) :: out
     end function lw_get_history
 
+    ! TRACE block and tracer (misc_init.f90:47-52, misc_hist.f90:184-209)
+    integer(c_int) function lw_trace_info(h, ntrace, ntr, pts) bind(c, name='lw_trace_info')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: ntrace, ntr
+      integer(c_int64_t), intent(out) :: pts(*)
+    end function lw_trace_info
+
+    integer(c_int) function lw_get_trace(h, out, cap) bind(c, name='lw_get_trace')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), value :: cap
+    end function lw_get_trace
+
+    ! the mgftresiduemax.dat rows of the last step (poiss_z_ft.f90:293-297)
+    integer(c_int64_t) function lw_get_mg_residuals(h, kv, kk, rm, cap) bind(c, name='lw_get_mg_residuals')
+      import :: c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: kv(*), kk(*)
+      real(c_double), intent(out) :: rm(*)
+      integer(c_int64_t), value :: cap
+    end function lw_get_mg_residuals
+
     integer(c_int) function lw_get_settings(h, out) bind(c, name='lw_get_settings')
       import :: c_int, c_ptr, lw_settings
       type(c_ptr), value :: h

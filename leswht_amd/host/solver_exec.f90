@@ -8,8 +8,11 @@
 ! the grid file, the initial field; misc_init.f90:19-786) and writes the same
 ! outputs: ../output/field/fld%07d at every NPRINT steps and at the end
 ! (solver.f90:171,185; misc_hist.f90:212-252) and the history files
-! ../output/ftr/{fhist,ftemp,fcdcl,fheatflux,ftime}.dat with the reference's
-! records and formats (misc_hist.f90:2-181, 255-284; slv_auxi.f90:573-840).
+! ../output/ftr/{fhist,ftemp,fcdcl,fheatflux,ftime}.dat, the tracer files
+! f{u,v,w,p}trace.dat and mgftresiduemax.dat with the reference's records and
+! formats (misc_hist.f90:2-209, 255-284; slv_auxi.f90:573-840;
+! poiss_z_ft.f90:49,293-297). With IREAD = 0 it makes the initial field
+! (makefld) and writes it as fld0000000 first, as the reference does.
 ! Unlike the reference, an error ends the program with a non-zero status.
 !
 ! The body is the circular cylinder of geometry/funcbody_example/
@@ -32,6 +35,9 @@ program solver_exec
   character(len=68) :: benv
   integer :: elen, estat
   logical :: heat, ibm
+  integer(c_int64_t) :: ntrace, ntr, nres, q
+  integer(c_int64_t) :: trpts(192), rkv(9), rkk(9)
+  real(c_double) :: trv(256), rrm(9)
 
   body%kind = 0
   body%p = 0.0d0
@@ -56,8 +62,14 @@ program solver_exec
   call check(rc, 'initial state')
   heat = st%ihtrans == 1
   ibm = st%ibmon == 1
+  rc = lw_trace_info(h, ntrace, ntr, trpts)
+  call check(rc, 'trace cells')
 
+  ! makefld is followed by writefield (solver.f90:50-53)
+  if (st%iread /= 1) call write_field(hs%ihist)
   call open_histories(st%iread == 1)
+  ! poisinit opens the residual log anew on every run (poiss_z_ft.f90:49)
+  open (77, file='../output/ftr/mgftresiduemax.dat')
   call system_clock(count_rate=rate)
 
   do while ((hs%ntime < st%nend) .and. (hs%time < st%tend))
@@ -68,6 +80,11 @@ program solver_exec
     wall = dble(clk1 - clk0) / dble(rate)
     rc = lw_get_history(h, hs)
     call check(rc, 'history')
+    nres = lw_get_mg_residuals(h, rkv, rkk, rrm, 9_c_int64_t)
+    do q = 1, min(nres, 9_c_int64_t)
+      write (77, 999) rkv(q), rkk(q), rrm(q)
+    end do
+999 format('KV=', i4, 3x, 'KK=', i4, 3x, 'RM=', es24.16)
 
     write (*, 320) hs%ntime, hs%time, hs%dt
     write (*, 330) hs%cflmax, hs%dvmax, ss%mg_cycles_max
@@ -81,6 +98,16 @@ program solver_exec
         hs%cd_avg(1), hs%cd_avg(2), hs%cd_avg(3)
       if (heat) write (2002, 140) hs%time, hs%hflux(1), hs%hflux(2), hs%hflux(3), &
         hs%hflux_avg(1), hs%hflux_avg(2), hs%hflux_avg(3)
+    end if
+    if (ntrace > 0) then
+      if (mod(hs%ntime, ntr) == 0) then
+        rc = lw_get_trace(h, trv, 256_c_int64_t)
+        call check(rc, 'tracer')
+        write (2005, 150) hs%time, trv(1:ntrace)
+        write (2006, 150) hs%time, trv(ntrace + 1:2 * ntrace)
+        write (2007, 150) hs%time, trv(2 * ntrace + 1:3 * ntrace)
+        write (2008, 150) hs%time, trv(3 * ntrace + 1:4 * ntrace)
+      end if
     end if
     if (mod(hs%ntime, st%nprint) == 0) call write_field(hs%ihist)
     write (2011, 130) hs%time, hs%s_sgs, hs%s_rhsnlhs, hs%s_poisson, wall
@@ -96,6 +123,7 @@ program solver_exec
 
 130 format(f13.5, 4es15.7)
 140 format(f13.5, 6es15.7)
+150 format(f15.7, 10000es15.7)
 
 contains
 
@@ -127,6 +155,12 @@ contains
     open (2002, file='../output/ftr/fheatflux.dat', position=trim(pos))
     open (2012, file='../output/ftr/ftemp.dat', position=trim(pos))
     open (2011, file='../output/ftr/ftime.dat', position=trim(pos))
+    if (ntrace /= 0) then
+      open (2005, file='../output/ftr/futrace.dat', position=trim(pos))
+      open (2006, file='../output/ftr/fvtrace.dat', position=trim(pos))
+      open (2007, file='../output/ftr/fwtrace.dat', position=trim(pos))
+      open (2008, file='../output/ftr/fptrace.dat', position=trim(pos))
+    end if
   end subroutine open_histories
 
   subroutine close_histories()
@@ -135,6 +169,13 @@ contains
     close (2002)
     close (2012)
     close (2011)
+    close (77)
+    if (ntrace /= 0) then
+      close (2005)
+      close (2006)
+      close (2007)
+      close (2008)
+    end if
   end subroutine close_histories
 
 end program solver_exec

@@ -122,7 +122,8 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option", "lw_autotune",
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
-           "lw_create_static", "lw_static_area_bytes")
+           "lw_create_static", "lw_static_area_bytes", "lw_set_trace", "lw_trace_info", "lw_get_trace",
+           "lw_get_mg_residuals")
 
 _lib = None
 
@@ -139,6 +140,11 @@ def load_library(path: str = LIB_PATH):
     L.lw_create.argtypes = [C.POINTER(lw_settings), C.POINTER(lw_grid), C.POINTER(lw_tables),
                             C.POINTER(lw_body), C.POINTER(H)]
     L.lw_create_static.argtypes = L.lw_create.argtypes
+    L.lw_set_trace.argtypes = [H, _I, _I, _PI]
+    L.lw_trace_info.argtypes = [H, _PI, _PI, _PI]
+    L.lw_get_trace.argtypes = [H, _PD, _I]
+    L.lw_get_mg_residuals.argtypes = [H, _PI, _PI, _PD, _I]
+    L.lw_get_mg_residuals.restype = _I
     L.lw_static_area_bytes.restype = _I
     L.lw_create_from_dir.argtypes = [C.c_char_p, C.POINTER(lw_body), C.POINTER(H)]
     L.lw_destroy.argtypes = [H]
@@ -406,6 +412,8 @@ class Solver:
         self = cls(h, bool(s.ihtrans), (g["n1"], g["n2"], g["n3"]))
         f = cd["fields"]
         self.set_fields(f["u"], f["v"], f["w"], f["p"], f.get("t"), cd["ihist"], cd["m"], cd["time"], cd["dt"])
+        if getattr(s, "trace", ()):
+            self.set_trace(s.trace, s.ntr)
         return self
 
     @classmethod
@@ -502,6 +510,35 @@ class Solver:
         h = lw_history()
         _check(load_library().lw_get_history(self._h, C.byref(h)))
         return h.as_dict()
+
+    def set_trace(self, cells, ntr: int = 1):
+        """Tracer cells (i, j, k), traced every ``ntr`` steps (lw_set_trace)."""
+        pts = np.ascontiguousarray(np.asarray(cells, dtype=np.int64).reshape(-1))
+        _check(load_library().lw_set_trace(self._h, len(pts) // 3, int(ntr), _pi(pts) if len(pts) else None))
+
+    def trace_info(self):
+        """(ntrace, ntr, cells) of the handle."""
+        n, r = _I(), _I()
+        pts = np.zeros(192, dtype=np.int64)
+        _check(load_library().lw_trace_info(self._h, C.byref(n), C.byref(r), _pi(pts)))
+        return n.value, r.value, pts[:3 * n.value].reshape(-1, 3)
+
+    def trace(self) -> dict:
+        """What tracer writes for the current fields: u, v, w at the traced
+        cell centres and p there (arrays of ntrace values)."""
+        n = self.trace_info()[0]
+        out = np.zeros(max(4 * n, 1))
+        _check(load_library().lw_get_trace(self._h, _pd(out), len(out)))
+        return {c: out[q * n:(q + 1) * n].copy() for q, c in enumerate("uvwp")}
+
+    def mg_residuals(self):
+        """The mgftresiduemax.dat rows of the last step: list of (kv, kk, rm)."""
+        L = load_library()
+        kv, kk, rm = np.zeros(9, np.int64), np.zeros(9, np.int64), np.zeros(9)
+        n = L.lw_get_mg_residuals(self._h, _pi(kv), _pi(kk), _pd(rm), 9)
+        if n < 0:
+            _check(int(n))
+        return [(int(kv[q]), int(kk[q]), float(rm[q])) for q in range(min(n, 9))]
 
     def settings(self) -> dict:
         st = lw_settings()

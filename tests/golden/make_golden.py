@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 from leswht_amd import case as C  # noqa: E402
 import oracle  # noqa: E402
+from tests.helpers import spec_for  # noqa: E402
 
 CASES = {
     # no SGS: ILES=1 with zero constant coefficients (the reference's ILES=0
@@ -31,17 +32,15 @@ CASES = {
     "cyl32_nosgs_5": dict(dims=(32, 32, 8), mode="nosgs", nend=5),
     "cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3),
     "cyl48_les_cfl_4": dict(dims=(48, 32, 16), mode="les_cfl", nend=4),
+    # constant-coefficient Vreman and eddy diffusivity (sgs_cvm, sgs_cedm)
+    "cyl32_const_3": dict(dims=(32, 32, 8), mode="les_const", nend=3),
+    # the configurations' domain and near-body spacing, five multigrid levels
+    "cyl192_dom_nlev5_3": dict(dims=(192, 64, 16), mode="dom_les", nend=3, nlev=5),
+    # 100 steps of the dynamic LES
+    "cyl32_les_100": dict(dims=(32, 32, 8), mode="les", nend=100),
+    # makefld (IREAD = 0) and the tracer every 2 steps
+    "cyl32_iread0_trace_4": dict(dims=(32, 32, 8), mode="les_trace_iread0", nend=4),
 }
-
-
-def spec_for(dims, mode, nend):
-    if mode == "nosgs":
-        s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=2)
-    elif mode == "les":
-        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=2, re=3900.0)
-    else:
-        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=3, re=3900.0, idtopt=1, cflfac=0.5)
-    return C.CaseSpec(*dims, xlim=(-3.0, 6.0), ylim=(-3.0, 3.0), zlen=1.0, dmin=0.05, noise=1e-2, settings=s)
 
 
 def input_hash(root):
@@ -54,25 +53,48 @@ def input_hash(root):
     return h.hexdigest()
 
 
-def main():
+def text_rows(path):
+    return np.array([[float(x) for x in ln.split()] for ln in open(path) if ln.strip()], dtype=np.float64)
+
+
+def mg_rows(path):
+    """mgftresiduemax.dat: 'KV=  k   KK=  n   RM= r' per line."""
+    rows = []
+    for ln in open(path):
+        t = ln.replace("=", " ").split()
+        if len(t) == 6:
+            rows.append((float(t[1]), float(t[3]), float(t[5].replace("D", "E"))))
+    return np.array(rows, dtype=np.float64).reshape(-1, 3)
+
+
+def main(names=None):
     for name, c in CASES.items():
-        spec = spec_for(c["dims"], c["mode"], c["nend"])
+        if names and name not in names:
+            continue
+        spec = spec_for(c["dims"], c["mode"], c["nend"], c.get("nlev"))
         root = tempfile.mkdtemp(prefix="lw_gold_")
         C.write_case(C.build_case(spec), root)
         oracle.run_reference(root, threads=1)
         fl = C.read_fld(os.path.join(root, "output/field/fld%07d" % c["nend"]))
         rows = {}
         for key, fn in (("hist", "fhist"), ("cdcl", "fcdcl"), ("hflux", "fheatflux"), ("temp", "ftemp")):
-            path = os.path.join(root, "output/ftr/%s.dat" % fn)
-            rows[key] = np.array([[float(x) for x in ln.split()] for ln in open(path) if ln.strip()],
-                                 dtype=np.float64)
+            rows[key] = text_rows(os.path.join(root, "output/ftr/%s.dat" % fn))
+        rows["mgres"] = mg_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat"))
+        extra = {}
+        if spec.settings.trace:
+            for q in "uvwp":
+                extra["trace_" + q] = text_rows(os.path.join(root, "output/ftr/f%strace.dat" % q))
+        if spec.settings.iread == 0:
+            f0 = C.read_fld(os.path.join(root, "output/field/fld0000000"))
+            for q in ("u", "v", "w", "p", "t"):
+                extra["init_" + q] = f0[q]
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
                             u=fl["u"], v=fl["v"], w=fl["w"], p=fl["p"], t=fl["t"],
-                            time=fl["time"], dt=fl["dt"], ihist=fl["ihist"], **rows,
+                            time=fl["time"], dt=fl["dt"], ihist=fl["ihist"], **rows, **extra,
                             meta=json.dumps(dict(c, input_sha256=input_hash(root))))
-        print(name, "time", fl["time"], "dvmax", rows["hist"][-1][3])
+        print(name, "time", fl["time"], "dvmax", rows["hist"][-1][3], "mg rows", len(rows["mgres"]))
         shutil.rmtree(root)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

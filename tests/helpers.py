@@ -9,11 +9,25 @@ from leswht_amd import case as C
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+TRACE_CELLS = ((5, 6, 2), (20, 16, 4), (28, 25, 7))
+
+
 def spec_for(dims, mode, nend=1, nlev=None):
+    if mode == "dom_les":
+        # the configurations' domain [-15,30] x [-15,15] x [0,pi], dmin 0.01
+        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
+                       cflfac=0.5)
+        return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
     if mode == "nosgs":
         s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2)
     elif mode == "les":
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 2, re=3900.0)
+    elif mode == "les_const":
+        s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.025, csgshf=0.003, resid1=1e-8, nlev=nlev or 2,
+                       re=3900.0)
+    elif mode == "les_trace_iread0":
+        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 2, re=3900.0, iread=0,
+                       trace=TRACE_CELLS, ntr=2)
     elif mode == "les_cfl":
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 3, re=3900.0, idtopt=1, cflfac=0.5)
     else:

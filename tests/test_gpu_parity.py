@@ -22,7 +22,7 @@ def _view(dims, mode, nend=1, nlev=None):
 @pytest.mark.parametrize("name", golden_names())
 def test_gpu_matches_reference_golden(name):
     z, meta = golden(name)
-    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"])
+    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
     print("golden", name, flush=True)
     g = Solver.from_case(view)
     st = g.step(meta["nend"])
@@ -188,17 +188,55 @@ def test_gpu_rejects_out_of_scope_configuration():
 
 
 def _hist_rows(sol, nend):
-    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": []}
+    """Per step: the history rows, the mgftresiduemax.dat rows and, every NTR
+    steps, the tracer rows, as the reference writes them."""
+    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": []}
+    ntrace, ntr, _ = sol.trace_info()
+    trows = {c: [] for c in "uvwp"}
     for _ in range(nend):
         sol.step(1)
         h = sol.history()
+        rows["mgres"] += [list(r) for r in sol.mg_residuals()]
+        if ntrace and h["ntime"] % ntr == 0:
+            tr = sol.trace()
+            for c in "uvwp":
+                trows[c].append([h["time"]] + list(tr[c]))
         if not h["written"]:
             continue
         rows["hist"].append([h["time"], h["dt"], h["cflmax"], h["dvmax"], h["qmmax"]])
         rows["cdcl"].append([h["time"]] + h["cd"] + h["cd_avg"])
         rows["hflux"].append([h["time"]] + h["hflux"] + h["hflux_avg"])
         rows["temp"].append([h["time"]] + h["tbulk"] + h["tbulk_avg"])
-    return {k: np.array(v) for k, v in rows.items()}
+    out = {k: np.array(v) for k, v in rows.items()}
+    if ntrace:
+        out.update({"trace_" + c: np.array(v) for c, v in trows.items()})
+    return out
+
+
+def _check_mgres(name, ref, got):
+    """mgftresiduemax.dat: the same wavenumbers and cycle counts (KK = MGITR+1
+    where the limit ended a solve), and the final max-norm residual times
+    dtconst. The residual is the small difference of A phi and the
+    right-hand side, both about 1/dx^2 larger than it on a grid with 0.01
+    spacing, so the last digits of phi, which round differently, move it by
+    up to 3.4e-4 relative there (1e-7 on the small-domain grids); the bar is
+    1e-3. Wavenumbers whose right-hand side is rounding noise (a field
+    without z variation) end near 1e-14: below 1e-12 the values are compared
+    as zeros. The cycle counts, which decide every solve, are exact."""
+    assert ref.shape == got.shape, (name, ref.shape, got.shape)
+    assert np.array_equal(ref[:, :2], got[:, :2]), (name, "kv/kk")
+    np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=1e-3, atol=1e-12, err_msg=name)
+
+
+def _check_trace(name, z, got):
+    """f{u,v,w,p}trace.dat rows (f15.7 time, es15.7 values: 8 digits)."""
+    for c in "uvwp":
+        ref = z["trace_" + c]
+        g = got["trace_" + c]
+        assert ref.shape == g.shape, (name, c, ref.shape, g.shape)
+        assert np.all(np.abs(ref[:, 0] - g[:, 0]) <= 1e-7), (name, c, "time")
+        scale = np.maximum(np.abs(ref[:, 1:]), 1e-30)
+        assert np.all(np.abs(ref[:, 1:] - g[:, 1:]) <= 1e-6 * scale + 1e-12), (name, c)
 
 
 def _check_rows(name, key, ref, got):
@@ -223,22 +261,35 @@ def test_gpu_history_matches_reference_golden(name):
     """fhist, fcdcl, fheatflux and ftemp rows of every step against the
     reference's own history files (draglift, heatflux, writehistory)."""
     z, meta = golden(name)
-    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"])
+    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
     print("history", name, flush=True)
     rows = _hist_rows(Solver.from_case(view), meta["nend"])
     for key in ("hist", "cdcl", "hflux", "temp"):
         _check_rows(name, key, z[key], rows[key])
+    _check_mgres(name, z["mgres"], rows["mgres"])
+    if "trace_u" in z.files:
+        _check_trace(name, z, rows)
 
 
-def test_gpu_fortran_host_matches_reference():
+def _mg_file_rows(path):
+    rows = []
+    for ln in open(path):
+        t = ln.replace("=", " ").split()
+        if len(t) == 6:
+            rows.append((float(t[1]), float(t[3]), float(t[5])))
+    return np.array(rows).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4"])
+def test_gpu_fortran_host_matches_reference(name):
     """The Fortran solver_exec (ISO_C_BINDING host over the C ABI) run on the
-    reference's case directory writes the reference's fld file and history
-    rows."""
+    reference's case directory writes the reference's fld file, history rows,
+    mgftresiduemax.dat, and with a TRACE block the tracer files; with IREAD =
+    0 it makes the initial field and writes it as fld0000000 first."""
     import subprocess
     from leswht_amd.build import EXE
     if not os.path.exists(EXE):
         pytest.skip("solver_exec not built (amdflang absent at build time)")
-    name = "cyl32_les_3"
     z, meta = golden(name)
     spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
     with tempfile.TemporaryDirectory() as root:
@@ -260,6 +311,16 @@ def test_gpu_fortran_host_matches_reference():
             _check_rows(name, key, z[key], got)
         tim = [ln.split() for ln in open(os.path.join(root, "output/ftr/ftime.dat")) if ln.strip()]
         assert len(tim) == meta["nend"] and all(len(r) == 5 for r in tim)
+        _check_mgres(name, z["mgres"], _mg_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat")))
+        if "trace_u" in z.files:
+            got = {"trace_" + c: np.array([[float(x) for x in ln.split()] for ln in
+                                           open(os.path.join(root, "output/ftr/f%strace.dat" % c)) if ln.strip()])
+                   for c in "uvwp"}
+            _check_trace(name, z, got)
+        if "init_u" in z.files:
+            f0 = C.read_fld(os.path.join(root, "output/field/fld0000000"))
+            for k, sl in fld_regions(n).items():
+                assert np.array_equal(z["init_" + k][sl], f0[k][sl]), ("fld0000000", k)
 
 
 def test_gpu_100_steps_match_oracle(oracle_lib):

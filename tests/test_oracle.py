@@ -12,17 +12,30 @@ from tests.helpers import fld_regions, golden, golden_names, rel, spec_for
 def test_oracle_matches_reference_golden(name, oracle_lib):
     import oracle
     z, meta = golden(name)
-    spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
+    spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
     view = C.case_view(C.build_case(spec))
     o = oracle.Oracle(view)
     o.step(meta["nend"])
     f = o.fields()
     n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
+    # the oracle restates the reference operation by operation; only libm
+    # calls and the FFT's radix split differ, so a few steps stay within
+    # 1e-11 (1e-10 over 100 steps, where the bar for 100 steps is 1e-6)
+    bar = 1e-11 if meta["nend"] <= 10 else 1e-10
     for k, sl in fld_regions(n).items():
         err = rel(z[k][sl], f[k][sl])
-        # the oracle restates the reference operation by operation; only libm
-        # calls and the FFT's radix split differ
-        assert err <= 1e-11, (name, k, err)
+        assert err <= bar, (name, k, err)
+
+
+def test_makefld_matches_reference_initial_field():
+    """IREAD = 0: the case builder's makefld gives the field the reference
+    wrote as fld0000000 (misc_init.f90:443-607, solver.f90:50-53) bit for bit."""
+    z, meta = golden("cyl32_iread0_trace_4")
+    view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])))
+    f = view["fields"]
+    n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
+    for k, sl in fld_regions(n).items():
+        assert np.array_equal(z["init_" + k][sl], f[k][sl]), k
 
 
 def test_case_inputs_unchanged_since_fixtures():
@@ -32,7 +45,7 @@ def test_case_inputs_unchanged_since_fixtures():
     import tempfile
     for name in golden_names():
         z, meta = golden(name)
-        spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
+        spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
         root = tempfile.mkdtemp()
         C.write_case(C.build_case(spec), root)
         h = hashlib.sha256()
