@@ -35,16 +35,23 @@ from leswht_amd.solver import Solver  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the multigrid kernels on this workload, averaged
-# over the launches of one step: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (KiB per dispatch) on `bench.py --no-cpu-baseline --steps 1 --warmup 1`,
-# scaled by what the same counters read for a 512 MiB device copy
-# (`bench.py --copy-probe-mib 512`: FETCH x2.000, WRITE x1.000)
-TRAFFIC = {
-    "k_mg_up": round((4027.3 * 2.0 + 2574.1) * 1024),
-    "k_mg_down": round((2444.4 * 2.0 + 5172.4) * 1024),
-    "k_mg_top_up": round((12754.6 * 2.0 + 13757.4) * 1024),
-}
+PROFILES = os.path.join(HERE, "profiles")
+
+
+def traffic_from_profile(kernel: str, config: str):
+    """HBM bytes per launch of ``kernel`` measured by the FETCH_SIZE and
+    WRITE_SIZE counters (rocprofv3 --pmc, scaled by the same counters on a
+    device copy of known size), from the committed profile summary of this
+    workload, whose line ``traffic <kernel>: <bytes> bytes per launch`` the
+    summary's own commands produce. None when no summary holds it."""
+    path = os.path.join(PROFILES, "%s_%s.md" % (kernel, WORKLOAD_TAGS[config]))
+    if not os.path.exists(path):
+        return None, None
+    for ln in open(path):
+        t = ln.strip().split()
+        if len(t) >= 3 and t[0] == "traffic" and t[1] == kernel + ":":
+            return int(t[2]), os.path.relpath(path, HERE)
+    return None, None
 
 
 def c2_spec(nend: int = 2) -> C.CaseSpec:
@@ -61,42 +68,107 @@ def c1_spec(nend: int = 2) -> C.CaseSpec:
     return C.CaseSpec(192, 128, 16, dmin=0.01, noise=1.0e-3, settings=s)
 
 
+WORKLOAD_TAGS = {"c2": "c2_384x256x64", "c1": "c1_192x128x16"}
+
 WORKLOADS = {
-    "c2": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64, dynamic Vreman + dynamic eddy "
-          "diffusivity, box test filter x/z, CFL 0.5",
-    "c1": "heated circular cylinder Re=100 Pr=0.71, 192x128x16, no SGS model, dt 0.005",
+    "c2": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64 on [-15,30]x[-15,15]x[0,pi], dynamic "
+          "Vreman + dynamic eddy diffusivity, box test filter x/z, CFL 0.5, NLEV 5; x and y lines clustered "
+          "by a two-sided sinh at the body centre (spacing 0.0155 x, 0.021 y at the surface)",
+    "c1": "heated circular cylinder Re=100 Pr=0.71, 192x128x16 on [-15,30]x[-15,15]x[0,pi], no SGS model, "
+          "dt 0.005, NLEV 5; sinh-clustered lines (spacing 0.030 x, 0.046 y at the surface)",
 }
 
 
-def cpu_baseline(spec: C.CaseSpec, case, threads: int) -> dict:
-    """Reference solver on the host: two steps of the same case; per-step wall
-    time from its ftime.dat column 5 (misc_hist.f90:255-284)."""
+SERIAL_REF_FILES = "slv_mmtm.f90, slv_engy.f90 and slv_auxi.f90"
+
+
+def _mg_file_rows(path):
+    rows = []
+    for ln in open(path):
+        t = ln.replace("=", " ").split()
+        if len(t) == 6:
+            rows.append((int(t[1]), int(t[3])))
+    return rows
+
+
+def cpu_baseline(spec: C.CaseSpec, case, fld_path: str, nsteps: int, threads: int, gpu: dict) -> dict:
+    """The reference solver on the host over the same steps the GPU timed
+    first: it restarts (IREAD = 1) from the fld file the GPU wrote at the
+    start of its timed window and runs ``nsteps`` steps; per-step wall time
+    and the SGS / RHSNLHS / POISSON split from its own timer file ftime.dat
+    (misc_hist.f90:255-284), its multigrid cycle counts of kv = 1..3 from
+    mgftresiduemax.dat (poiss_z_ft.f90:293-297) against the GPU's."""
     import oracle  # checker-side: the reference binary lives under oracle/_ref
 
     root = tempfile.mkdtemp(prefix="lw_cpu_")
     try:
-        if spec.settings.iles == 0:
+        st = dataclasses.replace(spec.settings, iread=1, ireset=0, nend=nsteps, nprint=100000)
+        if st.iles == 0:
             # the reference's ILES = 0 path reads LES arrays it never
             # allocated; "no SGS" runs there as ILES = 1 with the model off
             # and zero coefficients, which gives the same fields
-            st = dataclasses.replace(spec.settings, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0)
+            st = dataclasses.replace(st, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0)
             case = C.build_case(dataclasses.replace(spec, settings=st))
+        else:
+            case = dataclasses.replace(case, spec=dataclasses.replace(spec, settings=st))
         C.write_case(case, root)
+        shutil.copyfile(fld_path, os.path.join(root, "output/field/fld000000"))
         t0 = time.time()
         oracle.run_reference(root, threads=threads, timeout=1500)
         wall = time.time() - t0
-        rows = [ln.split() for ln in open(os.path.join(root, "output/ftr/ftime.dat")) if ln.strip()]
-        step_s = [float(r[4]) for r in rows]
-        med = statistics.median(step_s)
-        n = spec.n1m * spec.n2m * spec.n3m
-        return {"value": n / med, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
-                "sample": "reference solver (amdflang -O3, OpenMP %d threads), same %dx%dx%d case, "
-                          "%d RK3 steps, median of its own per-step timer (ftime.dat); process wall %.1f s "
-                          "incl. reading the text inputs" % (threads, spec.n1m, spec.n2m, spec.n3m,
-                                                             len(step_s), wall),
-                "s_per_step": med}
+        rows = [[float(x) for x in ln.split()] for ln in open(os.path.join(root, "output/ftr/ftime.dat"))
+                if ln.strip()]
+        kk = _mg_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat"))
     finally:
         shutil.rmtree(root, ignore_errors=True)
+    step_s = [r[4] for r in rows]
+    med = statistics.median(step_s)
+    n = spec.n1m * spec.n2m * spec.n3m
+    gpu_kk = [(kv, k) for kv, k, _ in gpu["mg_rows"][:len(kk)]]
+    return {
+        "value": n / med, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+        "sample": "reference solver built from its own sources (amdflang -O3; OpenMP with %d threads, except "
+                  "%s, compiled serial: flang rejects or mis-handles their OpenMP clauses), restarted from "
+                  "the fld the GPU wrote at the start of its timed window, over the same %d steps; median of "
+                  "its own per-step timer (ftime.dat); process wall %.1f s incl. reading the text inputs"
+                  % (threads, SERIAL_REF_FILES, len(step_s), wall),
+        "s_per_step": med,
+        "phase_s_per_step": {"sgs": statistics.median(r[1] for r in rows),
+                             "rhsnlhs": statistics.median(r[2] for r in rows),
+                             "poisson": statistics.median(r[3] for r in rows)},
+        "same_steps": {"steps": len(step_s), "cpu_s": sum(step_s), "gpu_s": gpu["s"],
+                       "ratio": sum(step_s) / gpu["s"] if gpu["s"] else None,
+                       "mg_cycles_kv123_rows": len(kk),
+                       "mg_cycles_kv123_rows_differing": sum(a != b for a, b in zip(kk, gpu_kk)) + abs(len(kk) - len(gpu_kk)),
+                       "mg_cycles_kv1_per_step_cpu": sum(k for kv, k in kk if kv == 1) / max(len(step_s), 1),
+                       "mg_cycles_kv1_per_step_gpu": sum(k for kv, k in gpu_kk if kv == 1) / max(len(step_s), 1)},
+        "host": {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))},
+    }
+
+
+def serve_leg(args) -> dict:
+    """A short served stream beside the time step: seeded Poisson arrivals of
+    small cases (config 4's sizes, bodies, Re and Pr) and a share of
+    384x256x64 LES cases through the native request loop; p50/p99
+    time-to-solution and aggregate cell-updates/s of the cases that finished."""
+    from leswht_amd import serve
+    cases = serve.draw_cases(args.serve_leg, seed=args.seed, rate=args.rate,
+                             steps=tuple(int(round(v * args.serve_leg_scale)) for v in (200, 2000)),
+                             large_share=args.large_share,
+                             large_steps=tuple(int(round(v * args.serve_leg_scale)) for v in (200, 2000)))
+    root = tempfile.mkdtemp(prefix="lw_serve_")
+    try:
+        t0 = time.time()
+        dirs = serve.prepare(cases, root)
+        t_prep = time.time() - t0
+        out = serve.run_stream(cases, dirs, int(args.budget_gb * (1 << 30)), args.slots, honour_arrivals=True,
+                               write_fields=False)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    return {"cases": len(cases), "large_cases": sum(c.large for c in cases), "rate_per_s": args.rate,
+            "step_budgets": "U{200..2000} x %g" % args.serve_leg_scale, "slots": args.slots,
+            "p50_s": out["p50"], "p99_s": out["p99"], "cell_updates_per_s": out["throughput"],
+            "failed": len(out["failed"]), "prep_s": round(t_prep, 1)}
 
 
 def serve_bench(args) -> dict:
@@ -149,6 +221,7 @@ def serve_bench(args) -> dict:
                    "peak_reserved_gb": round(stream["info"]["peak_reserved_bytes"] / (1 << 30), 2)},
         "closed_batch": {"slots": args.slots, "value": closed["throughput"], "one_slot": single["throughput"],
                          "ratio": closed["throughput"] / single["throughput"]},
+        "failed": len(stream["failed"]) + len(closed["failed"]) + len(single["failed"]),
     }
 
 
@@ -158,7 +231,18 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("LESWHT_CPU_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("LESWHT_CPU_THREADS", os.environ.get("OMP_NUM_THREADS", "16"))),
+                    help="OpenMP threads of the reference solver (default: OMP_NUM_THREADS, else 16)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="steps of the reference run: the first steps of the GPU's timed window")
+    ap.add_argument("--later-at", type=int, default=200,
+                    help="step after which a later window is timed (0: none)")
+    ap.add_argument("--later-steps", type=int, default=10, help="steps of the later window")
+    ap.add_argument("--serve-leg", type=int, default=16,
+                    help="cases of the short served stream reported beside the step (0: none)")
+    ap.add_argument("--serve-leg-scale", type=float, default=0.05,
+                    help="short served stream: factor on the step budgets U{200..2000}")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB five times (the known-size "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -221,15 +305,25 @@ def main():
     sol.step(1)
     table = sorted(sol.profile(), key=lambda r: -r["ms_total"])
     dom_name = table[0]["name"]
+    # the state at the start of the timed window, for the reference run over
+    # the same steps (written before the clock starts)
+    work = tempfile.mkdtemp(prefix="lw_bench_")
+    fld_start = os.path.join(work, "fld_start")
+    sol.write_field(fld_start)
     # timed region: K steps, with the dominant kernel's launches timed by the
     # start and end timestamps of their own dispatches on the streams they run on
     sol.profile_begin([dom_name])
-    t0 = time.perf_counter()
+    step_s, mg_rows = [], []
     pois_ms, cycles = 0.0, 0
-    for _ in range(args.steps):
+    t0 = time.perf_counter()
+    for q in range(args.steps):
+        ts = time.perf_counter()
         st = sol.step(1)
+        step_s.append(time.perf_counter() - ts)
         pois_ms += st["ms_poisson"]
         cycles += st["mg_cycles_sum"]
+        if q < args.cpu_steps:
+            mg_rows += sol.mg_residuals()
     elapsed = time.perf_counter() - t0
     prof_region = sol.profile()
     sol.profile_end()
@@ -246,18 +340,33 @@ def main():
         sol.profile_end()
     n = spec.n1m * spec.n2m * spec.n3m
     value = n * args.steps / elapsed
+    done_steps = args.warmup + args.steps + max(args.kernel_steps, 0)
+    later = None
+    if args.later_at > 0 and args.later_steps > 0:
+        # out of the impulsive start: the same case further on
+        if args.later_at > done_steps:
+            sol.step(args.later_at - done_steps)
+        cyc, t1 = 0, time.perf_counter()
+        for _ in range(args.later_steps):
+            cyc += sol.step(1)["mg_cycles_sum"]
+        lt = time.perf_counter() - t1
+        later = {"after_step": max(args.later_at, done_steps), "steps": args.later_steps,
+                 "ms_per_step": lt / args.later_steps * 1e3, "value": n * args.later_steps / lt,
+                 "mg_cycles_per_step": cyc / args.later_steps}
     roof = None
     dom = next((r for r in prof if r["name"] == dom_name), None)
     if dom and dom["count"] and dom["bytes"]:
         avg_s = dom["ms_total"] / dom["count"] / 1e3
         per_launch = dom["bytes"] / dom["count"]
         ach = per_launch / avg_s / 1e9
+        traffic, traffic_src = traffic_from_profile(dom_name, args.config)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom_name) if args.config == "c2" else None, "kernel": dom_name,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
                 "launches": dom["count"], "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": round(per_launch),
                 "timing": "every launch timed, %d steps after the timed region" % args.kernel_steps
-                if args.kernel_steps > 0 else "timed region"}
+                if args.kernel_steps > 0 else "timed region",
+                "traffic_source": traffic_src}
         reg = next((r for r in prof_region if r["name"] == dom_name), None)
         if reg and reg["count"]:
             roof["avg_us_timed_region"] = round(reg["ms_total"] / reg["count"] * 1e3, 2)
@@ -279,6 +388,7 @@ def main():
         gbs = r["bytes"] / (r["ms_total"] / 1e3) / 1e9 if r["ms_total"] else 0.0
         print("  %-24s %6d launches %9.3f ms %5.1f%% %8.1f GB/s" % (
             r["name"], r["count"], r["ms_total"], 100 * r["ms_total"] / step_ms, gbs), file=sys.stderr)
+    sol.close()
     out = {
         "metric": "cell-updates/s per RK3 step (1 GPU)",
         "value": value,
@@ -298,17 +408,27 @@ def main():
                    "case_build_s": round(t_build, 2), "mg_tiles": tune},
         "roofline": roof,
         "poisson": poisson,
+        "later_window": later,
+        "serve": None,
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if args.serve_leg > 0:
         try:
-            cb = cpu_baseline(spec, case, args.cpu_threads)
-            out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
-            out["speedup_vs_cpu_baseline"] = value / cb["value"]
+            out["serve"] = serve_leg(args)
+        except Exception as e:  # reported beside the step, never required
+            out["serve"] = {"error": str(e)[:300]}
+    if not args.no_cpu_baseline:
+        k = min(args.cpu_steps, args.steps)
+        gpu_same = {"s": sum(step_s[:k]), "mg_rows": mg_rows}
+        try:
+            cb = cpu_baseline(spec, case, fld_start, k, args.cpu_threads, gpu_same)
+            out["cpu_baseline"] = {q: cb[q] for q in ("value", "unit", "cores", "kind", "sample", "phase_s_per_step",
+                                                      "same_steps", "host")}
+            out["speedup_vs_cpu_baseline"] = cb["same_steps"]["ratio"]
         except Exception as e:  # baseline is reported, never required
             out["cpu_baseline"] = {"error": str(e)[:300]}
+    shutil.rmtree(work, ignore_errors=True)
     print(json.dumps(out))
-
 
 if __name__ == "__main__":
     main()
