@@ -55,6 +55,12 @@ typedef struct lw_settings {
   int64_t bc_ybtm, bc_ytop, bc_zbtm, bc_ztop, bc_t_ybtm, bc_t_ytop, bc_t_zbtm,
       bc_t_ztop, ich, iconjg, xprdic, yprdic, zprdic, iintp;
   double val_t_ybtm, val_t_ytop, val_t_zbtm, val_t_ztop;
+  /* the Poisson solver; the reference links one (02_solver/Makefile:23-34)
+   * and settings.input does not name it: 0 = poiss_z_ft (z-DFT, then a 2-D
+   * multigrid per wavenumber), 1 = poiss_mgrd (BiCGSTAB preconditioned by
+   * one 3-D multigrid V-cycle; lw_create_from_dir takes it from the
+   * environment variable LW_POISSON = "mgrd") */
+  int64_t poisson;
 } lw_settings;
 
 /* grid.bin as readgeom holds it (misc_init.f90:204-251) */
@@ -197,6 +203,14 @@ int lw_get_trace(lw_handle *h, double *out, int64_t cap);
  * final max-norm residual times dtconst). Returns the row count; copies at
  * most cap rows into the arrays that are not NULL. */
 int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t *kk, double *rm, int64_t cap);
+/* with poisson = 1: the rows poiss_mgrd.f90 writes during the last step,
+ * per BiCGSTAB iteration to poiss_itr.dat (:185-186: the substep, the
+ * iteration, max |residual| * dtconst and the last cell i, j, k holding it,
+ * loc[3 q .. 3 q + 2]) and per solve to ftrpoittr.dat (:221-222: the
+ * iteration count, MGITR+1 when the limit ended it, and the residual).
+ * Return the row counts; copy at most cap rows. */
+int64_t lw_get_bicg_log(const lw_handle *h, int64_t *sub, int64_t *icg, double *rm, int64_t *loc, int64_t cap);
+int64_t lw_get_bicg_solves(const lw_handle *h, int64_t *icg, double *rm, int64_t cap);
 /* the settings the handle runs with (settings.input + boundary.input +
  * ibmpre_prdic.bin as readsettings/readbcs hold them, misc_init.f90:19-158) */
 int lw_get_settings(const lw_handle *h, lw_settings *out);

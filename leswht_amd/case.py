@@ -611,6 +611,9 @@ class Settings:
     val_t_ztop: float = 0.0
     ich: int = 0
     iconjg: int = 0
+    # the Poisson solver the reference links (02_solver/Makefile:23-34), not
+    # in any input file: 0 poiss_z_ft, 1 poiss_mgrd
+    poisson: int = 0
     # ibmpre_prdic.bin
     xprdic: int = 0
     yprdic: int = 0

@@ -89,4 +89,22 @@ struct MG {
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 
+// The 3-D multigrid-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90).
+// Levels are stacked along i and k as in the reference: level l holds the
+// 1-based stacked lines iimg[l][0]..iimg[l][1] (iimg[l][2] of them) and
+// kkmg[l][0]..[1]; the finest level is nlev (1..n1m, 1..n3m).
+struct M3 {
+  int nlev, n1md, n3md, nbli;
+  int iimg[21][3], kkmg[21][3];
+  long GX, GY;  // gi(i, j, k) at i + GX (j + GY k): GX = n1md + 1, GY = n2 + 1
+  // coefficients with the reference's 1-based index (aw(i) at aw[i])
+  const double *aw, *ae, *as, *an, *ab, *af, *fidw, *fkdw, *fiup, *fkup;
+  const int *ih1, *ih2, *kh1, *kh2, *il1, *il2, *kl1, *kl2, *ipm, *imm, *kpm, *kmm;
+  const double *bet;    // (n1md, n2m, n3md), i fastest; gam(j) = an(j-1) bet(j-1)
+  double *gi, *resid;   // (0:n1md, 0:n2, 0:n3md) and (n1md, n2m, n3md)
+  double *p, *s, *y, *z, *v, *t, *res;  // BiCGSTAB vectors, the fields' padded layout
+  double *sc;           // scalars, see lw_kern_mgrd.hip
+  double *part;         // block partials: 4 * kRedBlocks sums, then kRedBlocks (max, index) pairs
+};
+
 }  // namespace lw

@@ -5,6 +5,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <fstream>
@@ -190,6 +191,10 @@ void read_settings(Case &c, const std::string &dir) {
   ListReader pr(join(dir, "../output/ibmpre/ibmpre_prdic.bin"));
   b = pr.ints(4);
   s.xprdic = b[0]; s.yprdic = b[1]; s.zprdic = b[2]; s.iintp = b[3];
+  // the Poisson solver is the reference's link-time choice (Makefile:23-34),
+  // named by no input file: LW_POISSON = mgrd selects poiss_mgrd
+  const char *ps = getenv("LW_POISSON");
+  s.poisson = (ps && strcmp(ps, "mgrd") == 0) ? 1 : 0;
 }
 
 template <class T>

@@ -129,4 +129,11 @@ bool mg_folds_check(const MG &mg);
 // one V-cycle of every wavenumber still iterating (poiss_z_ft.f90:258-299)
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int cyc, bool check);
 
+// ---- the 3-D MG-preconditioned BiCGSTAB Poisson solver (lw_kern_mgrd.hip) ----
+// scalars of M3::sc the host reads after an iteration
+constexpr int kM3Resm = 8, kM3Loc = 9;
+void launch_mgrd_init(hipStream_t s, const Geo &g, const M3 &m, const Fld &f, double test1);
+void launch_mgrd_iteration(hipStream_t s, const Geo &g, const M3 &m, const Fld &f, int first);
+void launch_mgrd_finish(hipStream_t s, const Geo &g, const M3 &m, const Fld &f);
+
 }  // namespace lw

@@ -199,6 +199,10 @@ struct lw_handle {
   double mgres_rm[9] = {0};
   double *h_rfin = nullptr;  // pinned: rfin of kv = 1..3 after a solve
   double hs_rfin[4] = {0};
+  // poiss_mgrd: the solver's state and the log of the last step
+  M3 m3{};
+  std::vector<int64_t> bicg_sub, bicg_icg, bicg_loc, solve_icg;
+  std::vector<double> bicg_rm, solve_rm;
   // tracer (misc_hist.f90:184-209): cells whose centre values are traced
   int64_t ntrace = 0, ntr = 1;
   int64_t trpts[64 * 3] = {0};
@@ -292,6 +296,202 @@ static std::vector<int> vi(const int64_t *src, int64_t lo, int64_t hi) {
 // ---------------------------------------------------------------------------
 // set-up: lhsinit, poisinit/pmat/mwavenumber/coefmg, sgsfilterinit
 // ---------------------------------------------------------------------------
+// poisinit, mgcoef and tridcoef of poiss_mgrd.f90 (678-993): the 1-D
+// coefficients of every level (levels stacked along i and k), the
+// restriction and prolongation weights and indices, and the TDMA pivots of
+// every y-line of every level, in the reference's expressions and order.
+// At a level's first and last line the reference multiplies a quotient by
+// 0 (iwest, ieast, kback, kforw) whose divisor holds a centre of another
+// level or one not yet set; that coefficient is 0 here.
+static void setup_mgrd(lw_handle *h, const lw_grid *gr) {
+  const lw_settings &st = h->st;
+  const Geo &g = h->g;
+  M3 &m = h->m3;
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  const int nlev = (int)st.nlev, n1md = 2 * n1m, n3md = 2 * n3m;
+  m.nlev = nlev;
+  m.n1md = n1md;
+  m.n3md = n3md;
+  m.nbli = (int)st.nbli;
+  m.GX = n1md + 1;
+  m.GY = n2 + 1;
+  auto c2cxi = vd(gr->c2cxi, 0, n1), c2cyi = vd(gr->c2cyi, 0, n2), c2czi = vd(gr->c2czi, 0, n3);
+  auto f2fxi = vd(gr->f2fxi, 0, n1), f2fyi = vd(gr->f2fyi, 0, n2), f2fzi = vd(gr->f2fzi, 0, n3);
+  auto zmp = vd(gr->zmp, 0, n3);
+  auto x = vd(gr->x, 1, n1), z = vd(gr->z, 1, n3);
+  auto ipv = vi(gr->ipv, 1, n1m), jpv = vi(gr->jpv, 1, n2m), kpv = vi(gr->kpv, 1, n3m);
+  auto fixil = vi(gr->fixil, 1, n1m), fixjl = vi(gr->fixjl, 1, n2m), fixkl = vi(gr->fixkl, 1, n3m);
+  auto fixiu = vi(gr->fixiu, 1, n1m), fixju = vi(gr->fixju, 1, n2m), fixku = vi(gr->fixku, 1, n3m);
+  std::vector<double> aw(n1md + 2, 0.0), ae(n1md + 2, 0.0), as(n2m + 2, 0.0), an(n2m + 2, 0.0), ab(n3md + 2, 0.0),
+      af(n3md + 2, 0.0), fidw(n1md + 2, 0.0), fiup(n1md + 2, 0.0), fkdw(n3md + 2, 0.0), fkup(n3md + 2, 0.0);
+  std::vector<int> ih1(n1md + 2, 0), ih2(n1md + 2, 0), il1(n1md + 2, 0), il2(n1md + 2, 0), ipm(n1md + 2, 0),
+      imm(n1md + 2, 0), kh1(n3md + 2, 0), kh2(n3md + 2, 0), kl1(n3md + 2, 0), kl2(n3md + 2, 0), kpm(n3md + 2, 0),
+      kmm(n3md + 2, 0);
+  for (int i = 1; i <= n1m; ++i) {
+    const int ip = ipv[i];
+    aw[i] = (1.0 - fixil[i]) * c2cxi[i] * f2fxi[i];
+    ae[i] = (1.0 - fixiu[i]) * c2cxi[ip] * f2fxi[i];
+  }
+  for (int j = 1; j <= n2m; ++j) {
+    const int jp = jpv[j];
+    as[j] = (1.0 - fixjl[j]) * c2cyi[j] * f2fyi[j];
+    an[j] = (1.0 - fixju[j]) * c2cyi[jp] * f2fyi[j];
+  }
+  for (int k = 1; k <= n3m; ++k) {
+    const int kp = kpv[k];
+    ab[k] = (1.0 - fixkl[k]) * c2czi[k] * f2fzi[k];
+    af[k] = (1.0 - fixku[k]) * c2czi[kp] * f2fzi[k];
+  }
+  // level ranges: [l][0] first, [1] last, [2] count (1-based stacked lines)
+  int(*iimg)[3] = m.iimg, (*kkmg)[3] = m.kkmg;
+  iimg[nlev][0] = 1; iimg[nlev][1] = n1m; iimg[nlev][2] = n1m;
+  kkmg[nlev][0] = 1; kkmg[nlev][1] = n3m; kkmg[nlev][2] = n3m;
+  for (int l = nlev - 1; l >= 0; --l) {
+    iimg[l][0] = iimg[l + 1][0] + iimg[l + 1][2];
+    iimg[l][2] = (n1m / (1 << nlev)) * (1 << l);
+    iimg[l][1] = iimg[l][0] + iimg[l][2] - 1;
+    kkmg[l][0] = kkmg[l + 1][0] + kkmg[l + 1][2];
+    kkmg[l][2] = (n3m / (1 << nlev)) * (1 << l);
+    kkmg[l][1] = kkmg[l][0] + kkmg[l][2] - 1;
+  }
+  std::vector<double> xmpm(n1md + 2, 0.0), zmpm(n3md + 2, 0.0);
+  std::vector<int> iwest(n1md + 2, 0), ieast(n1md + 2, 0), kback(n3md + 2, 0), kforw(n3md + 2, 0);
+  for (int i = 1; i <= n1m; ++i) {
+    xmpm[i] = h->xmp[i];
+    iwest[i] = 1 - 1 / i;
+    ieast[i] = 1 - i / n1m;
+  }
+  for (int k = 1; k <= n3m; ++k) {
+    zmpm[k] = zmp[k];
+    kback[k] = 1 - 1 / k;
+    kforw[k] = 1 - k / n3m;
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int ibg = iimg[l][0], iend = iimg[l][1], isp = 1 << (nlev - l);
+    const int kbg = kkmg[l][0], kend = kkmg[l][1], ksp = 1 << (nlev - l);
+    for (int i = ibg, ic = 1; i <= iend; ++i, ++ic) {
+      xmpm[i] = 0.5 * (x[(ic - 1) * isp + 1] + x[ic * isp + 1]);
+      iwest[i] = 1 - ibg / i;
+      ieast[i] = 1 - i / iend;
+    }
+    for (int k = kbg, kc = 1; k <= kend; ++k, ++kc) {
+      zmpm[k] = 0.5 * (z[(kc - 1) * ksp + 1] + z[kc * ksp + 1]);
+      kback[k] = 1 - kbg / k;
+      kforw[k] = 1 - k / kend;
+    }
+    for (int i = ibg, ic = 1; i <= iend; ++i, ++ic) {
+      const double w = x[ic * isp + 1] - x[(ic - 1) * isp + 1];
+      aw[i] = iwest[i] ? 1.0 / ((xmpm[i] - xmpm[i - 1]) * w) : 0.0;
+      ae[i] = ieast[i] ? 1.0 / ((xmpm[i + 1] - xmpm[i]) * w) : 0.0;
+    }
+    for (int k = kbg, kc = 1; k <= kend; ++k, ++kc) {
+      const double w = z[kc * ksp + 1] - z[(kc - 1) * ksp + 1];
+      ab[k] = kback[k] ? 1.0 / ((zmpm[k] - zmpm[k - 1]) * w) : 0.0;
+      af[k] = kforw[k] ? 1.0 / ((zmpm[k + 1] - zmpm[k]) * w) : 0.0;
+    }
+  }
+  for (int l = 0; l <= nlev - 1; ++l) {
+    int ibgh = iimg[l + 1][0];
+    for (int i = iimg[l][0]; i <= iimg[l][1]; ++i, ibgh += 2) {
+      fidw[i] = (xmpm[i] - xmpm[ibgh]) / (xmpm[ibgh + 1] - xmpm[ibgh]);
+      ih1[i] = ibgh;
+      ih2[i] = ibgh + 1;
+    }
+    int kbgh = kkmg[l + 1][0];
+    for (int k = kkmg[l][0]; k <= kkmg[l][1]; ++k, kbgh += 2) {
+      fkdw[k] = (zmpm[k] - zmpm[kbgh]) / (zmpm[kbgh + 1] - zmpm[kbgh]);
+      kh1[k] = kbgh;
+      kh2[k] = kbgh + 1;
+    }
+  }
+  for (int l = 1; l <= nlev; ++l) {
+    int ibgl = iimg[l - 1][0];
+    for (int i = iimg[l][0]; i <= iimg[l][1]; i += 2, ibgl += 1) {
+      fiup[i] = (xmpm[i] - xmpm[ibgl - 1]) / (xmpm[ibgl] - xmpm[ibgl - 1]);
+      fiup[i + 1] = (xmpm[i + 1] - xmpm[ibgl]) / (xmpm[ibgl + 1] - xmpm[ibgl]);
+      il1[i] = ibgl - 1;
+      il2[i] = ibgl;
+      il1[i + 1] = ibgl;
+      il2[i + 1] = ibgl + 1;
+    }
+    fiup[iimg[l][0]] = 1.0;
+    fiup[iimg[l][1]] = 0.0;
+    int kbgl = kkmg[l - 1][0];
+    for (int k = kkmg[l][0]; k <= kkmg[l][1]; k += 2, kbgl += 1) {
+      fkup[k] = (zmpm[k] - zmpm[kbgl - 1]) / (zmpm[kbgl] - zmpm[kbgl - 1]);
+      fkup[k + 1] = (zmpm[k + 1] - zmpm[kbgl]) / (zmpm[kbgl + 1] - zmpm[kbgl]);
+      kl1[k] = kbgl - 1;
+      kl2[k] = kbgl;
+      kl1[k + 1] = kbgl;
+      kl2[k + 1] = kbgl + 1;
+    }
+    fkup[kkmg[l][0]] = 1.0;
+    fkup[kkmg[l][1]] = 0.0;
+  }
+  for (int l = nlev; l >= 0; --l) {
+    for (int k = kkmg[l][0]; k <= kkmg[l][1]; ++k) { kpm[k] = k + 1; kmm[k] = k - 1; }
+    for (int i = iimg[l][0]; i <= iimg[l][1]; ++i) { ipm[i] = i + 1; imm[i] = i - 1; }
+  }
+  if (st.zprdic == 1) {
+    for (int l = nlev; l >= 0; --l) {
+      kpm[kkmg[l][1]] = kkmg[l][0];
+      kmm[kkmg[l][0]] = kkmg[l][1];
+    }
+    for (int l = nlev - 1; l >= 0; --l) {
+      const int kbg = kkmg[l][0], kend = kkmg[l][1], ksp = 1 << (nlev - l);
+      const double vdz = zmpm[kbg] - z[1] + z[n3] - zmpm[kend];
+      const double sdz_kbg = z[1 + ksp] - z[1], sdz_kend = z[n3] - z[n3 - ksp];
+      ab[kbg] = 1.0 / (vdz * sdz_kbg);
+      af[kend] = 1.0 / (vdz * sdz_kend);
+    }
+    for (int l = 1; l <= nlev; ++l) {
+      const int kbg = kkmg[l][0], kend = kkmg[l][1], kbgl = kkmg[l - 1][0], kendl = kkmg[l - 1][1];
+      const double dz = zmpm[kbgl] - z[1] + z[n3] - zmpm[kendl];
+      const double dz1 = zmpm[kend] - zmpm[kendl];
+      const double dz2 = z[n3] - zmpm[kendl] + zmpm[kbg] - z[1];
+      fkup[kbg] = dz2 / dz;
+      fkup[kend] = dz1 / dz;
+      kl1[kbg] = kendl;
+      kl2[kbg] = kbgl;
+      kl1[kend] = kendl;
+      kl2[kend] = kbgl;
+    }
+  }
+  // tridcoef: pivots of every line; gam(j) = an(j-1) bet(j-1) is formed
+  // again by the kernels with the same product
+  std::vector<double> bet((size_t)n1md * n2m * n3md, 0.0);
+  auto B = [&](int i, int j, int k) -> double & {
+    return bet[(size_t)(i - 1) + (size_t)n1md * ((size_t)(j - 1) + (size_t)n2m * (size_t)(k - 1))];
+  };
+  for (int l = 0; l <= nlev; ++l)
+    for (int k = kkmg[l][0]; k <= kkmg[l][1]; ++k)
+      for (int i = iimg[l][0]; i <= iimg[l][1]; ++i) {
+        const double acc1 = -1.0 * (aw[i] + ae[i] + as[1] + an[1] + ab[k] + af[k]);
+        B(i, 1, k) = 1.0 / acc1;
+        for (int j = 2; j <= n2m; ++j) {
+          const double acc = -1.0 * (aw[i] + ae[i] + as[j] + an[j] + ab[k] + af[k]);
+          const double gam = an[j - 1] * B(i, j - 1, k);
+          B(i, j, k) = 1.0 / (acc - as[j] * gam);
+        }
+      }
+  m.aw = h->upload(aw); m.ae = h->upload(ae); m.as = h->upload(as); m.an = h->upload(an);
+  m.ab = h->upload(ab); m.af = h->upload(af);
+  m.fidw = h->upload(fidw); m.fkdw = h->upload(fkdw); m.fiup = h->upload(fiup); m.fkup = h->upload(fkup);
+  m.ih1 = h->upload(ih1); m.ih2 = h->upload(ih2); m.kh1 = h->upload(kh1); m.kh2 = h->upload(kh2);
+  m.il1 = h->upload(il1); m.il2 = h->upload(il2); m.kl1 = h->upload(kl1); m.kl2 = h->upload(kl2);
+  m.ipm = h->upload(ipm); m.imm = h->upload(imm); m.kpm = h->upload(kpm); m.kmm = h->upload(kmm);
+  m.bet = h->upload(bet);
+  // gi and resid start at 0 and every value read from them later is finite
+  m.gi = h->dalloc<double>((size_t)m.GX * m.GY * (n3md + 1));
+  m.resid = h->dalloc<double>((size_t)n1md * n2m * n3md);
+  const size_t np = (size_t)g.NX * g.NY * g.NZ;
+  m.p = h->dalloc<double>(np); m.s = h->dalloc<double>(np); m.y = h->dalloc<double>(np);
+  m.z = h->dalloc<double>(np); m.v = h->dalloc<double>(np); m.t = h->dalloc<double>(np);
+  m.res = h->dalloc<double>(np);
+  m.sc = h->dalloc<double>(16);
+  m.part = h->dalloc<double>(4 * kRedBlocks);
+}
+
 static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   const lw_settings &st = h->st;
   Geo &g = h->g;
@@ -508,6 +708,11 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     dp.hzm = h->upload(hzm); dp.hzp = h->upload(hzp);
   }
 
+  if (st.poisson == 1) {
+    setup_mgrd(h, gr);
+    HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
+    return;
+  }
   // ---- Poisson (poisinit, poiss_z_ft.f90:18-113 and coefmg 643-798) ----
   MG &mg = h->mg;
   const int nlev = (int)st.nlev;
@@ -685,7 +890,7 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
     throw LW_ERR_ARG;
   };
   if (s->xprdic || s->yprdic) bad("x- or y-periodic grids are not supported (poiss_z_ft path, inflow/outflow in x)");
-  if (s->zprdic != 1) bad("the z direction must be periodic (poiss_z_ft)");
+  if (s->zprdic != 1) bad("the z direction must be periodic (the step's z sweeps and poiss_z_ft)");
   if (s->ich != 0) bad("ICH != 0 (channel driving) is not supported");
   if (s->iconjg) bad("ICONJG = 1 (conjugate heat transfer) is not supported");
   if (s->imovingon) bad("IMOVING = 1 is not supported");
@@ -697,10 +902,23 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
   if (s->nlev < 1 || s->nlev > 19) bad("NLEV must be in 1..19");
   if (g->n1m % (1 << s->nlev) != 0) bad("n1m must be divisible by 2**NLEV");
   if (g->n1m % 2 != 0) bad("n1m must be even");
-  if (s->imgsor < g->n3m / 2 + 1) bad("IMGSOR < n3m/2+1 (line-SOR wavenumbers) is not supported");
-  if (g->n3m / 2 + 1 > 128) bad("n3m must be at most 254");
   if (g->n3m < 2) bad("n3m must be at least 2");
   if (s->nbli < 1) bad("NBLI must be >= 1");
+  if (s->poisson != 0 && s->poisson != 1) bad("poisson must be 0 (poiss_z_ft) or 1 (poiss_mgrd)");
+  if (s->poisson == 1) {
+    // poiss_mgrd coarsens z as well; with IOLDV = 1 its V-cycle starts from
+    // an uninitialised local array (poiss_mgrd.f90:236, 239)
+    if (s->ioldv != 0) bad("poiss_mgrd: IOLDV must be 0");
+    if (g->n3m % (1 << s->nlev) != 0) bad("poiss_mgrd: n3m must be divisible by 2**NLEV");
+    const int64_t nz0 = g->n3m >> s->nlev;
+    // an odd number of z lines above 1 on the coarsest level puts two
+    // neighbouring lines in one zebra half; the reference's result then
+    // depends on the order its loop visits them
+    if (nz0 > 1 && nz0 % 2 == 1) bad("poiss_mgrd: n3m / 2**NLEV must be 1 or even");
+  } else {
+    if (s->imgsor < g->n3m / 2 + 1) bad("IMGSOR < n3m/2+1 (line-SOR wavenumbers) is not supported");
+    if (g->n3m / 2 + 1 > 128) bad("n3m must be at most 254");
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -861,9 +1079,15 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
   int64_t b = (30 + 4 * heat) * np * 8 + 2 * np;  // fields, mask and a byte of slack per cell
   if (s->iles == 1) b += 25 * ni * 8 + ni * 8;       // SGS work arrays and nwall
   if (s->iavg == 1) b += (heat ? 23 : 21) * ni * 8;  // field_avg running sums
-  const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
-  const int64_t n3mh = n3m / 2 + 1;
-  b += n3mh * (4 * plane + 3 * pcplane) * 16 + n3mh * plane * 8;
+  if (s->poisson == 1) {
+    // poiss_mgrd: gi, resid and the pivots over the stacked levels, and the
+    // seven BiCGSTAB vectors
+    b += (2 * n1m + 1) * (n2m + 2) * (2 * n3m + 1) * 8 + 2 * (2 * n1m) * n2m * (2 * n3m) * 8 + 7 * np * 8;
+  } else {
+    const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
+    const int64_t n3mh = n3m / 2 + 1;
+    b += n3mh * (4 * plane + 3 * pcplane) * 16 + n3mh * plane * 8;
+  }
   b += 8 * (n2m + 2) * (n3m + 2) * 8;                // outlet planes
   // IB tables: at most every cell of a component a forcing point, but bodies
   // cover a small share; 64 MB covers tables, metrics and partial sums
@@ -1147,7 +1371,49 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
 // device; the host enqueues batches of cycles (the first as long as the
 // previous solve needed) and reads the flags between batches. Cycles enqueued
 // after a wavenumber converged leave it untouched.
+// poisson of poiss_mgrd.f90 (25-224): BiCGSTAB iterations on the device,
+// one host read of the iteration's max norm after each; the log rows of
+// poiss_itr.dat and ftrpoittr.dat are kept for the host
+static void phase_poisson_mgrd(lw_handle *h) {
+  const Geo &g = h->g;
+  const M3 &m = h->m3;
+  const double test1 = h->st.resid1 * h->sub.dtconsti;
+  if (h->sub.msub == 1) {
+    h->bicg_sub.clear(); h->bicg_icg.clear(); h->bicg_rm.clear(); h->bicg_loc.clear();
+    h->solve_icg.clear(); h->solve_rm.clear();
+  }
+  launch_mgrd_init(h->s, g, m, h->f, test1);
+  const int64_t itr = h->st.mgitr;
+  int64_t icg = 1;
+  double resm = 0.0;
+  for (; icg <= itr; ++icg) {
+    launch_mgrd_iteration(h->s, g, m, h->f, icg == 1);
+    HIPCHK(hipMemcpyAsync(h->h_rfin, m.sc + kM3Resm, 2 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    resm = h->h_rfin[0];
+    const int64_t at = (int64_t)h->h_rfin[1], ni = g.n1m, nij = ni * g.n2m;
+    h->bicg_sub.push_back(h->sub.msub);
+    h->bicg_icg.push_back(icg);
+    h->bicg_rm.push_back(resm * h->sub.dtconst);
+    h->bicg_loc.push_back(at % ni + 1);
+    h->bicg_loc.push_back((at % nij) / ni + 1);
+    h->bicg_loc.push_back(at / nij + 1);
+    if (resm < test1) break;
+  }
+  if (icg > itr) fprintf(stderr, "leswht: === MUTLGRID : NOT CONVERGED ===\n");
+  launch_mgrd_finish(h->s, g, m, h->f);
+  h->solve_icg.push_back(icg);
+  h->solve_rm.push_back(resm * h->sub.dtconst);
+  if (icg > h->cyc_max) h->cyc_max = icg;
+  h->cyc_sum += icg;
+  h->mgres_n = 0;
+}
+
 static void phase_poisson(lw_handle *h) {
+  if (h->st.poisson == 1) {
+    phase_poisson_mgrd(h);
+    return;
+  }
   const Geo &g = h->g;
   MG &mg = h->mg;
   launch_fft_forward(h->s, g, mg, h->f);
@@ -1417,7 +1683,8 @@ extern "C" int lw_autotune(lw_handle *h, int64_t reps, lw_tune_result *out) {
       out->tile[q] = mg.tile[q];
       out->tile1[q] = mg.tile1[q];
     }
-    if (!mg.fused) return LW_OK;  // separate launches: no tiles to tune
+    // separate launches, or the 3-D solver: no tiles to tune
+    if (!mg.fused || h->st.poisson == 1) return LW_OK;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     const double dt0 = h->dt;
@@ -1765,6 +2032,30 @@ extern "C" int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t 
     if (rm) rm[q] = h->mgres_rm[q];
   }
   return h->mgres_n;
+}
+
+extern "C" int64_t lw_get_bicg_log(const lw_handle *h, int64_t *sub, int64_t *icg, double *rm, int64_t *loc,
+                                   int64_t cap) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  const int64_t n = (int64_t)h->bicg_icg.size();
+  for (int64_t q = 0; q < n && q < cap; ++q) {
+    if (sub) sub[q] = h->bicg_sub[q];
+    if (icg) icg[q] = h->bicg_icg[q];
+    if (rm) rm[q] = h->bicg_rm[q];
+    if (loc)
+      for (int c = 0; c < 3; ++c) loc[3 * q + c] = h->bicg_loc[3 * q + c];
+  }
+  return n;
+}
+
+extern "C" int64_t lw_get_bicg_solves(const lw_handle *h, int64_t *icg, double *rm, int64_t cap) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  const int64_t n = (int64_t)h->solve_icg.size();
+  for (int64_t q = 0; q < n && q < cap; ++q) {
+    if (icg) icg[q] = h->solve_icg[q];
+    if (rm) rm[q] = h->solve_rm[q];
+  }
+  return n;
 }
 
 extern "C" int lw_get_settings(const lw_handle *h, lw_settings *out) {

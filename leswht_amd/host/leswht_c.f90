@@ -18,6 +18,7 @@ module leswht_c
     integer(c_int64_t) :: bc_ybtm, bc_ytop, bc_zbtm, bc_ztop, bc_t_ybtm, bc_t_ytop, bc_t_zbtm, &
                           bc_t_ztop, ich, iconjg, xprdic, yprdic, zprdic, iintp
     real(c_double) :: val_t_ybtm, val_t_ytop, val_t_zbtm, val_t_ztop
+    integer(c_int64_t) :: poisson
   end type lw_settings
 
   ! per-step results (lw_step_stats)
@@ -92,6 +93,23 @@ module leswht_c
       real(c_double), intent(out) :: rm(*)
       integer(c_int64_t), value :: cap
     end function lw_get_mg_residuals
+
+    ! the poiss_itr.dat and ftrpoittr.dat rows of the last step (poiss_mgrd.f90:185,221)
+    integer(c_int64_t) function lw_get_bicg_log(h, sub, icg, rm, loc, cap) bind(c, name='lw_get_bicg_log')
+      import :: c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: sub(*), icg(*), loc(*)
+      real(c_double), intent(out) :: rm(*)
+      integer(c_int64_t), value :: cap
+    end function lw_get_bicg_log
+
+    integer(c_int64_t) function lw_get_bicg_solves(h, icg, rm, cap) bind(c, name='lw_get_bicg_solves')
+      import :: c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: icg(*)
+      real(c_double), intent(out) :: rm(*)
+      integer(c_int64_t), value :: cap
+    end function lw_get_bicg_solves
 
     integer(c_int) function lw_get_settings(h, out) bind(c, name='lw_get_settings')
       import :: c_int, c_ptr, lw_settings

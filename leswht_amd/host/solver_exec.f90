@@ -37,7 +37,12 @@ program solver_exec
   logical :: heat, ibm
   integer(c_int64_t) :: ntrace, ntr, nres, q
   integer(c_int64_t) :: trpts(192), rkv(9), rkk(9)
-  real(c_double) :: trv(256), rrm(9)
+  real(c_double) :: trv(256), rrm(9), t_begin
+  integer(c_int64_t), parameter :: maxit = 3 * 4096
+  integer(c_int64_t), allocatable :: bsub(:), bicg(:), bloc(:)
+  real(c_double), allocatable :: brm(:)
+  integer(c_int64_t) :: sicg(3)
+  real(c_double) :: srm(3)
 
   body%kind = 0
   body%p = 0.0d0
@@ -68,11 +73,20 @@ program solver_exec
   ! makefld is followed by writefield (solver.f90:50-53)
   if (st%iread /= 1) call write_field(hs%ihist)
   call open_histories(st%iread == 1)
-  ! poisinit opens the residual log anew on every run (poiss_z_ft.f90:49)
-  open (77, file='../output/ftr/mgftresiduemax.dat')
+  ! poisinit opens the solver's logs anew on every run: poiss_z_ft its
+  ! residual log (poiss_z_ft.f90:49), poiss_mgrd its iteration and solve logs
+  ! (poiss_mgrd.f90:710-711)
+  if (st%poisson == 1) then
+    allocate (bsub(maxit), bicg(maxit), bloc(3 * maxit), brm(maxit))
+    open (77, file='../output/ftr/poiss_itr.dat')
+    open (78, file='../output/ftr/ftrpoittr.dat')
+  else
+    open (77, file='../output/ftr/mgftresiduemax.dat')
+  end if
   call system_clock(count_rate=rate)
 
   do while ((hs%ntime < st%nend) .and. (hs%time < st%tend))
+    t_begin = hs%time
     call system_clock(clk0)
     rc = lw_step(h, 1_c_int64_t, ss)
     call check(rc, 'time step')
@@ -80,11 +94,24 @@ program solver_exec
     wall = dble(clk1 - clk0) / dble(rate)
     rc = lw_get_history(h, hs)
     call check(rc, 'history')
-    nres = lw_get_mg_residuals(h, rkv, rkk, rrm, 9_c_int64_t)
-    do q = 1, min(nres, 9_c_int64_t)
-      write (77, 999) rkv(q), rkk(q), rrm(q)
-    end do
+    if (st%poisson == 1) then
+      nres = lw_get_bicg_log(h, bsub, bicg, brm, bloc, maxit)
+      do q = 1, min(nres, maxit)
+        write (77, 102) bicg(q), brm(q), bloc(3 * q - 2), bloc(3 * q - 1), bloc(3 * q)
+      end do
+      nres = lw_get_bicg_solves(h, sicg, srm, 3_c_int64_t)
+      do q = 1, min(nres, 3_c_int64_t)
+        write (78, 103) t_begin, hs%dt, sicg(q), srm(q)
+      end do
+    else
+      nres = lw_get_mg_residuals(h, rkv, rkk, rrm, 9_c_int64_t)
+      do q = 1, min(nres, 9_c_int64_t)
+        write (77, 999) rkv(q), rkk(q), rrm(q)
+      end do
+    end if
 999 format('KV=', i4, 3x, 'KK=', i4, 3x, 'RM=', es24.16)
+102 format(i4, es15.7, 3i4)
+103 format(2f13.5, i5, es15.7)
 
     write (*, 320) hs%ntime, hs%time, hs%dt
     write (*, 330) hs%cflmax, hs%dvmax, ss%mg_cycles_max
@@ -170,6 +197,7 @@ contains
     close (2012)
     close (2011)
     close (77)
+    if (st%poisson == 1) close (78)
     if (ntrace /= 0) then
       close (2005)
       close (2006)

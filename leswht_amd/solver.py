@@ -37,7 +37,8 @@ class lw_settings(C.Structure):
             "idvmon", "filter", "ibmon", "masson", "imovingon", "ihtrans", "idecomp",
             "bc_ybtm", "bc_ytop", "bc_zbtm", "bc_ztop", "bc_t_ybtm", "bc_t_ytop", "bc_t_zbtm",
             "bc_t_ztop", "ich", "iconjg", "xprdic", "yprdic", "zprdic", "iintp")] + \
-        [(n, _D) for n in ("val_t_ybtm", "val_t_ytop", "val_t_zbtm", "val_t_ztop")]
+        [(n, _D) for n in ("val_t_ybtm", "val_t_ytop", "val_t_zbtm", "val_t_ztop")] + \
+        [("poisson", _I)]
 
 
 GRID_D = ("x", "y", "z")
@@ -123,7 +124,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
            "lw_create_static", "lw_static_area_bytes", "lw_set_trace", "lw_trace_info", "lw_get_trace",
-           "lw_get_mg_residuals")
+           "lw_get_mg_residuals", "lw_get_bicg_log", "lw_get_bicg_solves")
 
 _lib = None
 
@@ -145,6 +146,10 @@ def load_library(path: str = LIB_PATH):
     L.lw_get_trace.argtypes = [H, _PD, _I]
     L.lw_get_mg_residuals.argtypes = [H, _PI, _PI, _PD, _I]
     L.lw_get_mg_residuals.restype = _I
+    L.lw_get_bicg_log.argtypes = [H, _PI, _PI, _PD, _PI, _I]
+    L.lw_get_bicg_log.restype = _I
+    L.lw_get_bicg_solves.argtypes = [H, _PI, _PD, _I]
+    L.lw_get_bicg_solves.restype = _I
     L.lw_static_area_bytes.restype = _I
     L.lw_create_from_dir.argtypes = [C.c_char_p, C.POINTER(lw_body), C.POINTER(H)]
     L.lw_destroy.argtypes = [H]
@@ -539,6 +544,23 @@ class Solver:
         if n < 0:
             _check(int(n))
         return [(int(kv[q]), int(kk[q]), float(rm[q])) for q in range(min(n, 9))]
+
+    def bicg_log(self):
+        """With the 3-D Poisson solver: the rows of the last step, per
+        BiCGSTAB iteration (substep, iteration, residual * dtconst, i, j, k)
+        and per solve (iterations, residual * dtconst)."""
+        L = load_library()
+        cap = 3 * 4096
+        sub, icg, loc = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(3 * cap, np.int64)
+        rm = np.zeros(cap)
+        n = L.lw_get_bicg_log(self._h, _pi(sub), _pi(icg), _pd(rm), _pi(loc), cap)
+        if n < 0:
+            _check(int(n))
+        it = [(int(sub[q]), int(icg[q]), float(rm[q]), int(loc[3 * q]), int(loc[3 * q + 1]), int(loc[3 * q + 2]))
+              for q in range(min(n, cap))]
+        si, sr = np.zeros(3, np.int64), np.zeros(3)
+        ns = L.lw_get_bicg_solves(self._h, _pi(si), _pd(sr), 3)
+        return it, [(int(si[q]), float(sr[q])) for q in range(min(ns, 3))]
 
     def settings(self) -> dict:
         st = lw_settings()

@@ -89,6 +89,12 @@ def lib():
         L.ora_fft_inverse.argtypes = [C.c_int, _PD, _PD]
         L.ora_thomas.argtypes = [_I, _PD, _PD, _PD, _PD]
         L.ora_thomas_periodic.argtypes = [_I, _PD, _PD, _PD, _PD]
+        L.ora_set_poisson.argtypes = [C.c_void_p, _I]
+        L.ora_set_poisson.restype = C.c_int
+        L.ora_bicg_log.argtypes = [C.c_void_p, _PI, _PI, _PD, _PI, _I]
+        L.ora_bicg_log.restype = _I
+        L.ora_bicg_solves.argtypes = [C.c_void_p, _PI, _PD]
+        L.ora_bicg_solves.restype = _I
         _lib = L
     return _lib
 
@@ -125,7 +131,7 @@ def fortran_1d(arr, lo):
 class Oracle:
     """One oracle instance built from :func:`leswht_amd.case.load_case_dir`."""
 
-    def __init__(self, cd: dict):
+    def __init__(self, cd: dict, poisson: str = "z_ft"):
         L = lib()
         self._keep = []
         s = cd["settings"]
@@ -180,6 +186,24 @@ class Oracle:
         L.ora_set_fields(self.h, *[_pd(a) for a in arrs], _pd(tt) if tt is not None else None,
                          cd["ihist"], cd["m"], cd["time"], cd["dt"])
         L.ora_init(self.h)
+        if poisson != "z_ft":
+            if poisson != "mgrd" or L.ora_set_poisson(self.h, 1) != 0:
+                raise ValueError("oracle: Poisson path %r not available for this case" % poisson)
+
+    def bicg_log(self):
+        """Per BiCGSTAB iteration of the last step: (substep, iteration,
+        residual * dtconst, i, j, k); per solve: (iteration, residual *
+        dtconst)."""
+        L = lib()
+        cap = 3 * 4096
+        sub, icg, loc = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(3 * cap, np.int64)
+        rm = np.zeros(cap)
+        n = L.ora_bicg_log(self.h, _pi(sub), _pi(icg), _pd(rm), _pi(loc), cap)
+        it = [(int(sub[q]), int(icg[q]), float(rm[q]), int(loc[3 * q]), int(loc[3 * q + 1]), int(loc[3 * q + 2]))
+              for q in range(n)]
+        si, sr = np.zeros(3, np.int64), np.zeros(3)
+        ns = L.ora_bicg_solves(self.h, _pi(si), _pd(sr))
+        return it, [(int(si[q]), float(sr[q])) for q in range(ns)]
 
     def step(self, n: int = 1):
         d = Diag()
@@ -216,14 +240,18 @@ class Oracle:
             pass
 
 
-def run_reference(root: str, threads: int = 1, timeout: int = 600) -> str:
-    """Run the amdflang-built reference solver (oracle/_ref/solver_exec) in
-    ``root``/02_solver. Returns its stdout."""
-    if not os.path.exists(REF_EXEC):
-        raise FileNotFoundError(REF_EXEC)
+REF_EXEC_MGRD = os.path.join(HERE, "_ref", "solver_exec_mgrd")
+
+
+def run_reference(root: str, threads: int = 1, timeout: int = 600, exe: str = REF_EXEC) -> str:
+    """Run the amdflang-built reference solver (oracle/_ref/solver_exec, or
+    ``exe``, e.g. REF_EXEC_MGRD) in ``root``/02_solver. Returns its stdout."""
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    REF_EXEC_ = exe
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_DYNAMIC="FALSE",
                OMP_STACKSIZE="1G", OMP_PROC_BIND="close")
-    res = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + REF_EXEC],
+    res = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + REF_EXEC_],
                          cwd=os.path.join(root, "02_solver"), env=env, capture_output=True,
                          text=True, timeout=timeout)
     if res.returncode != 0:

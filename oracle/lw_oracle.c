@@ -87,6 +87,10 @@ struct ora {
   double *avg[23];
   int avg_started;
   int64_t avg_count;
+  /* Poisson path: 0 poiss_z_ft, 1 poiss_mgrd (the reference picks one at
+   * link time, 02_solver/Makefile:23-34) */
+  int poiss_kind;
+  struct mgrd *m3;
 };
 
 /* ---- indexing ------------------------------------------------------- */
@@ -2145,7 +2149,8 @@ void ora_phase_sgs(ora *o) {
 }
 void ora_phase_rhsnlhs(ora *o) { rhsnlhs(o); }
 void ora_phase_divgs(ora *o) { divgs(o); }
-void ora_phase_poisson(ora *o) { poisson(o); }
+static void poisson_any(ora *o);
+void ora_phase_poisson(ora *o) { poisson_any(o); }
 void ora_phase_ucalc_pcalc(ora *o) {
   ucalc(o);
   pcalc(o);
@@ -2315,7 +2320,7 @@ void ora_step(ora *o, ora_diag *d) {
     ora_phase_sgs(o);
     rhsnlhs(o);
     divgs(o);
-    poisson(o);
+    poisson_any(o);
     if (o->mg_cycles_max > cyc_max) cyc_max = o->mg_cycles_max;
     ucalc(o);
     pcalc(o);
@@ -2330,4 +2335,502 @@ void ora_step(ora *o, ora_diag *d) {
     d->qmmax = o->qmmax; d->csgsts = o->csgsts; d->csgshf = o->csgshf;
     d->ntime = o->ntime; d->mg_cycles_max = cyc_max; d->mg_cycles_sum = o->mg_cycles_sum;
   }
+}
+
+/* ===================================================================== */
+/* poiss_mgrd.f90: BiCGSTAB preconditioned by one 3-D multigrid V-cycle   */
+/* (x and z coarsened, zebra y-line relaxation with TDMA), the           */
+/* alternative Poisson path without the z-DFT.                            */
+/* ===================================================================== */
+struct mgrd {
+  int64_t n1md, n3md, nlev;
+  int64_t iimg[21][4], kkmg[21][4];
+  /* 1-based: aw, ae, fidw, fiup, ih*, il*, ipm, imm (n1md); as, an (n2m);
+   * ab, af, fkdw, fkup, kh*, kl*, kpm, kmm (n3md) */
+  double *aw, *ae, *as, *an, *ab, *af, *fidw, *fkdw, *fiup, *fkup;
+  int64_t *ih1, *ih2, *kh1, *kh2, *il1, *il2, *kl1, *kl2, *ipm, *imm, *kpm, *kmm;
+  double *gam, *bet;          /* (n1md, n2m, n3md) */
+  double *gi, *resid;         /* (0:n1md, 0:n2, 0:n3md), (n1md, n2m, n3md) */
+  double *p_cg, *s_cg, *y_cg, *z_cg, *v_cg, *t_cg, *res_cg; /* padded (0:n1, 0:n2, 0:n3) */
+  /* log of the last step: per BiCGSTAB iteration (poiss_itr.dat) and per
+   * solve (ftrpoittr.dat) */
+  int64_t nit, it_sub[3 * 4096], it_icg[3 * 4096], it_loc[3 * 4096 * 3];
+  double it_rm[3 * 4096];
+  int64_t nsolve, sv_icg[3];
+  double sv_rm[3];
+};
+
+#define M3I(i, j, k) (((i)-1) + m->n1md * (((j)-1) + o->n2m * (int64_t)((k)-1)))
+#define M3G(i, j, k) ((i) + (m->n1md + 1) * ((j) + (o->n2 + 1) * (int64_t)(k)))
+
+/* poisinit (poiss_mgrd.f90:678-716), mgcoef (719-960), tridcoef (963-993) */
+static void mgrd_init(ora *o) {
+  struct mgrd *m = (struct mgrd *)xcalloc(1, sizeof(struct mgrd));
+  o->m3 = m;
+  const int64_t n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, nlev = o->s.nlev;
+  m->n1md = 2 * n1m;
+  m->n3md = 2 * n3m;
+  m->nlev = nlev;
+  const int64_t n1md = m->n1md, n3md = m->n3md;
+  m->aw = xcalloc(n1md + 1, 8); m->ae = xcalloc(n1md + 1, 8);
+  m->as = xcalloc(n2m + 1, 8); m->an = xcalloc(n2m + 1, 8);
+  m->ab = xcalloc(n3md + 1, 8); m->af = xcalloc(n3md + 1, 8);
+  m->fidw = xcalloc(n1md + 1, 8); m->fiup = xcalloc(n1md + 1, 8);
+  m->fkdw = xcalloc(n3md + 1, 8); m->fkup = xcalloc(n3md + 1, 8);
+  m->ih1 = xcalloc(n1md + 1, 8); m->ih2 = xcalloc(n1md + 1, 8);
+  m->il1 = xcalloc(n1md + 1, 8); m->il2 = xcalloc(n1md + 1, 8);
+  m->ipm = xcalloc(n1md + 1, 8); m->imm = xcalloc(n1md + 1, 8);
+  m->kh1 = xcalloc(n3md + 1, 8); m->kh2 = xcalloc(n3md + 1, 8);
+  m->kl1 = xcalloc(n3md + 1, 8); m->kl2 = xcalloc(n3md + 1, 8);
+  m->kpm = xcalloc(n3md + 1, 8); m->kmm = xcalloc(n3md + 1, 8);
+  m->gam = xcalloc((size_t)n1md * n2m * n3md, 8);
+  m->bet = xcalloc((size_t)n1md * n2m * n3md, 8);
+  m->gi = xcalloc((size_t)(n1md + 1) * (o->n2 + 1) * (n3md + 1), 8);
+  m->resid = xcalloc((size_t)n1md * n2m * n3md, 8);
+  const size_t np = (size_t)NX1 * NY1 * NZ1;
+  m->p_cg = xcalloc(np, 8); m->s_cg = xcalloc(np, 8); m->y_cg = xcalloc(np, 8);
+  m->z_cg = xcalloc(np, 8); m->v_cg = xcalloc(np, 8); m->t_cg = xcalloc(np, 8);
+  m->res_cg = xcalloc(np, 8);
+  /* poisinit */
+  for (int64_t i = 1; i <= n1m; ++i) {
+    int64_t ip = o->ipv[i];
+    m->aw[i] = (1.0 - o->fixil[i]) * o->c2cxi[i] * o->f2fxi[i];
+    m->ae[i] = (1.0 - o->fixiu[i]) * o->c2cxi[ip] * o->f2fxi[i];
+  }
+  for (int64_t j = 1; j <= n2m; ++j) {
+    int64_t jp = o->jpv[j];
+    m->as[j] = (1.0 - o->fixjl[j]) * o->c2cyi[j] * o->f2fyi[j];
+    m->an[j] = (1.0 - o->fixju[j]) * o->c2cyi[jp] * o->f2fyi[j];
+  }
+  for (int64_t k = 1; k <= n3m; ++k) {
+    int64_t kp = o->kpv[k];
+    m->ab[k] = (1.0 - o->fixkl[k]) * o->c2czi[k] * o->f2fzi[k];
+    m->af[k] = (1.0 - o->fixku[k]) * o->c2czi[kp] * o->f2fzi[k];
+  }
+  /* mgcoef: level index ranges */
+  int64_t (*iimg)[4] = m->iimg, (*kkmg)[4] = m->kkmg;
+  iimg[nlev][1] = 1; iimg[nlev][2] = n1m; iimg[nlev][3] = n1m;
+  kkmg[nlev][1] = 1; kkmg[nlev][2] = n3m; kkmg[nlev][3] = n3m;
+  for (int64_t l = nlev - 1; l >= 0; --l) {
+    iimg[l][1] = iimg[l + 1][1] + iimg[l + 1][3];
+    iimg[l][3] = (n1m / (1LL << nlev)) * (1LL << l);
+    iimg[l][2] = iimg[l][1] + iimg[l][3] - 1;
+    kkmg[l][1] = kkmg[l + 1][1] + kkmg[l + 1][3];
+    kkmg[l][3] = (n3m / (1LL << nlev)) * (1LL << l);
+    kkmg[l][2] = kkmg[l][1] + kkmg[l][3] - 1;
+  }
+  double *xmpm = xcalloc(n1md + 1, 8), *zmpm = xcalloc(n3md + 1, 8);
+  int64_t *iwest = xcalloc(n1md + 1, 8), *ieast = xcalloc(n1md + 1, 8);
+  int64_t *kback = xcalloc(n3md + 1, 8), *kforw = xcalloc(n3md + 1, 8);
+  const double *x = o->x, *z = o->z;
+  for (int64_t i = 1; i <= n1m; ++i) {
+    xmpm[i] = o->xmp[i];
+    iwest[i] = 1 - 1 / i;
+    ieast[i] = 1 - i / n1m;
+  }
+  for (int64_t k = 1; k <= n3m; ++k) {
+    zmpm[k] = o->zmp[k];
+    kback[k] = 1 - 1 / k;
+    kforw[k] = 1 - k / n3m;
+  }
+  for (int64_t l = nlev - 1; l >= 0; --l) {
+    const int64_t ibg = iimg[l][1], iend = iimg[l][2], isp = 1LL << (nlev - l);
+    const int64_t kbg = kkmg[l][1], kend = kkmg[l][2], ksp = 1LL << (nlev - l);
+    int64_t ic = 0;
+    for (int64_t i = ibg; i <= iend; ++i) {
+      ic += 1;
+      xmpm[i] = 0.5 * (x[(ic - 1) * isp + 1] + x[ic * isp + 1]);
+      iwest[i] = 1 - ibg / i;
+      ieast[i] = 1 - i / iend;
+    }
+    int64_t kc = 0;
+    for (int64_t k = kbg; k <= kend; ++k) {
+      kc += 1;
+      zmpm[k] = 0.5 * (z[(kc - 1) * ksp + 1] + z[kc * ksp + 1]);
+      kback[k] = 1 - kbg / k;
+      kforw[k] = 1 - k / kend;
+    }
+    ic = 0;
+    for (int64_t i = ibg; i <= iend; ++i) {
+      ic += 1;
+      /* at a level's first and last line the factor is 0 and the
+       * reference divides by a distance to a centre of another level, or to
+       * one not yet set: 0 here, as 0 times any finite quotient */
+      const double w = x[ic * isp + 1] - x[(ic - 1) * isp + 1];
+      m->aw[i] = iwest[i] ? 1.0 / ((xmpm[i] - xmpm[i - 1]) * w) : 0.0;
+      m->ae[i] = ieast[i] ? 1.0 / ((xmpm[i + 1] - xmpm[i]) * w) : 0.0;
+    }
+    kc = 0;
+    for (int64_t k = kbg; k <= kend; ++k) {
+      kc += 1;
+      const double w = z[kc * ksp + 1] - z[(kc - 1) * ksp + 1];
+      m->ab[k] = kback[k] ? 1.0 / ((zmpm[k] - zmpm[k - 1]) * w) : 0.0;
+      m->af[k] = kforw[k] ? 1.0 / ((zmpm[k + 1] - zmpm[k]) * w) : 0.0;
+    }
+  }
+  /* restriction */
+  for (int64_t l = 0; l <= nlev - 1; ++l) {
+    int64_t ibgh = iimg[l + 1][1];
+    for (int64_t i = iimg[l][1]; i <= iimg[l][2]; ++i) {
+      m->fidw[i] = (xmpm[i] - xmpm[ibgh]) / (xmpm[ibgh + 1] - xmpm[ibgh]);
+      m->ih1[i] = ibgh;
+      m->ih2[i] = ibgh + 1;
+      ibgh += 2;
+    }
+    int64_t kbgh = kkmg[l + 1][1];
+    for (int64_t k = kkmg[l][1]; k <= kkmg[l][2]; ++k) {
+      m->fkdw[k] = (zmpm[k] - zmpm[kbgh]) / (zmpm[kbgh + 1] - zmpm[kbgh]);
+      m->kh1[k] = kbgh;
+      m->kh2[k] = kbgh + 1;
+      kbgh += 2;
+    }
+  }
+  /* prolongation */
+  for (int64_t l = 1; l <= nlev; ++l) {
+    int64_t ibgl = iimg[l - 1][1];
+    for (int64_t i = iimg[l][1]; i <= iimg[l][2]; i += 2) {
+      m->fiup[i] = (xmpm[i] - xmpm[ibgl - 1]) / (xmpm[ibgl] - xmpm[ibgl - 1]);
+      m->fiup[i + 1] = (xmpm[i + 1] - xmpm[ibgl]) / (xmpm[ibgl + 1] - xmpm[ibgl]);
+      m->il1[i] = ibgl - 1;
+      m->il2[i] = ibgl;
+      m->il1[i + 1] = ibgl;
+      m->il2[i + 1] = ibgl + 1;
+      ibgl += 1;
+    }
+    m->fiup[iimg[l][1]] = 1.0;
+    m->fiup[iimg[l][2]] = 0.0;
+    int64_t kbgl = kkmg[l - 1][1];
+    for (int64_t k = kkmg[l][1]; k <= kkmg[l][2]; k += 2) {
+      m->fkup[k] = (zmpm[k] - zmpm[kbgl - 1]) / (zmpm[kbgl] - zmpm[kbgl - 1]);
+      m->fkup[k + 1] = (zmpm[k + 1] - zmpm[kbgl]) / (zmpm[kbgl + 1] - zmpm[kbgl]);
+      m->kl1[k] = kbgl - 1;
+      m->kl2[k] = kbgl;
+      m->kl1[k + 1] = kbgl;
+      m->kl2[k + 1] = kbgl + 1;
+      kbgl += 1;
+    }
+    m->fkup[kkmg[l][1]] = 1.0;
+    m->fkup[kkmg[l][2]] = 0.0;
+  }
+  for (int64_t l = nlev; l >= 0; --l) {
+    for (int64_t k = kkmg[l][1]; k <= kkmg[l][2]; ++k) { m->kpm[k] = k + 1; m->kmm[k] = k - 1; }
+    for (int64_t i = iimg[l][1]; i <= iimg[l][2]; ++i) { m->ipm[i] = i + 1; m->imm[i] = i - 1; }
+  }
+  /* z periodic (x periodic is outside the restated scope) */
+  if (o->s.zprdic == 1) {
+    for (int64_t l = nlev; l >= 0; --l) {
+      m->kpm[kkmg[l][2]] = kkmg[l][1];
+      m->kmm[kkmg[l][1]] = kkmg[l][2];
+    }
+    const int64_t n3 = o->n3;
+    for (int64_t l = nlev - 1; l >= 0; --l) {
+      const int64_t kbg = kkmg[l][1], kend = kkmg[l][2], ksp = 1LL << (nlev - l);
+      const double vdz = zmpm[kbg] - z[1] + z[n3] - zmpm[kend];
+      const double sdz_kbg = z[1 + ksp] - z[1], sdz_kend = z[n3] - z[n3 - ksp];
+      m->ab[kbg] = 1.0 / (vdz * sdz_kbg);
+      m->af[kend] = 1.0 / (vdz * sdz_kend);
+    }
+    for (int64_t l = 1; l <= nlev; ++l) {
+      const int64_t kbg = kkmg[l][1], kend = kkmg[l][2], kbgl = kkmg[l - 1][1], kendl = kkmg[l - 1][2];
+      const double dz = zmpm[kbgl] - z[1] + z[n3] - zmpm[kendl];
+      const double dz1 = zmpm[kend] - zmpm[kendl];
+      const double dz2 = z[n3] - zmpm[kendl] + zmpm[kbg] - z[1];
+      m->fkup[kbg] = dz2 / dz;
+      m->fkup[kend] = dz1 / dz;
+      m->kl1[kbg] = kendl;
+      m->kl2[kbg] = kbgl;
+      m->kl1[kend] = kendl;
+      m->kl2[kend] = kbgl;
+    }
+  }
+  free(xmpm); free(zmpm); free(iwest); free(ieast); free(kback); free(kforw);
+  /* tridcoef */
+  for (int64_t l = 0; l <= nlev; ++l) {
+    for (int64_t k = kkmg[l][1]; k <= kkmg[l][2]; ++k)
+      for (int64_t i = iimg[l][1]; i <= iimg[l][2]; ++i) {
+        const double acc = -1.0 * (m->aw[i] + m->ae[i] + m->as[1] + m->an[1] + m->ab[k] + m->af[k]);
+        m->bet[M3I(i, 1, k)] = 1.0 / acc;
+      }
+    for (int64_t k = kkmg[l][1]; k <= kkmg[l][2]; ++k)
+      for (int64_t j = 2; j <= n2m; ++j)
+        for (int64_t i = iimg[l][1]; i <= iimg[l][2]; ++i) {
+          const double acc = -1.0 * (m->aw[i] + m->ae[i] + m->as[j] + m->an[j] + m->ab[k] + m->af[k]);
+          m->gam[M3I(i, j, k)] = m->an[j - 1] * m->bet[M3I(i, j - 1, k)];
+          m->bet[M3I(i, j, k)] = 1.0 / (acc - m->as[j] * m->gam[M3I(i, j, k)]);
+        }
+  }
+}
+
+static inline double m3_acc(const struct mgrd *m, int64_t i, int64_t j, int64_t k) {
+  return -1.0 * (m->aw[i] + m->ae[i] + m->as[j] + m->an[j] + m->ab[k] + m->af[k]);
+}
+
+/* one zebra half of relax/toplevel (poiss_mgrd.f90:298-344, 500-591): the
+ * lines (i, k) with i = ibg + mod(kpm(k), 2), +2 (half 0) or ibg + mod(k, 2)
+ * (half 1), right-hand sides from src (toplevel: rhs on the finest level;
+ * relax: resid) minus w * the x and z neighbours, then the y TDMA */
+static void m3_half(ora *o, int64_t l, int half, double w, const double *rhs_pad) {
+  struct mgrd *m = o->m3;
+  const int64_t n2m = o->n2m;
+  double *gi = m->gi;
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k) {
+    const int64_t is = m->iimg[l][1] + (half == 0 ? m->kpm[k] % 2 : k % 2);
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t i = is; i <= m->iimg[l][2]; i += 2) {
+        const double r = rhs_pad ? F(rhs_pad, i, j, k) : m->resid[M3I(i, j, k)];
+        const double nb = m->aw[i] * gi[M3G(m->imm[i], j, k)] + m->ae[i] * gi[M3G(m->ipm[i], j, k)] +
+                          m->ab[k] * gi[M3G(i, j, m->kmm[k])] + m->af[k] * gi[M3G(i, j, m->kpm[k])];
+        gi[M3G(i, j, k)] = r - w * nb;
+      }
+  }
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k) {
+    const int64_t is = m->iimg[l][1] + (half == 0 ? m->kpm[k] % 2 : k % 2);
+    for (int64_t i = is; i <= m->iimg[l][2]; i += 2) gi[M3G(i, 1, k)] = gi[M3G(i, 1, k)] * m->bet[M3I(i, 1, k)];
+  }
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k) {
+    const int64_t is = m->iimg[l][1] + (half == 0 ? m->kpm[k] % 2 : k % 2);
+    for (int64_t j = 2; j <= n2m; ++j)
+      for (int64_t i = is; i <= m->iimg[l][2]; i += 2)
+        gi[M3G(i, j, k)] = (gi[M3G(i, j, k)] - m->as[j] * gi[M3G(i, j - 1, k)]) * m->bet[M3I(i, j, k)];
+  }
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k) {
+    const int64_t is = m->iimg[l][1] + (half == 0 ? m->kpm[k] % 2 : k % 2);
+    for (int64_t j = n2m - 1; j >= 1; --j)
+      for (int64_t i = is; i <= m->iimg[l][2]; i += 2)
+        gi[M3G(i, j, k)] = gi[M3G(i, j, k)] - m->gam[M3I(i, j + 1, k)] * gi[M3G(i, j + 1, k)];
+  }
+}
+
+/* the residual of level l: rhs (or resid itself) minus A gi */
+static double m3_residual(ora *o, int64_t l, const double *rhs_pad) {
+  struct mgrd *m = o->m3;
+  const double *gi = m->gi;
+  double resm = 0.0;
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = m->iimg[l][1]; i <= m->iimg[l][2]; ++i) {
+        const double acc = m3_acc(m, i, j, k);
+        const double r = rhs_pad ? F(rhs_pad, i, j, k) : m->resid[M3I(i, j, k)];
+        const double v = r - acc * gi[M3G(i, j, k)] - m->aw[i] * gi[M3G(m->imm[i], j, k)] -
+                         m->ae[i] * gi[M3G(m->ipm[i], j, k)] - m->as[j] * gi[M3G(i, j - 1, k)] -
+                         m->an[j] * gi[M3G(i, j + 1, k)] - m->ab[k] * gi[M3G(i, j, m->kmm[k])] -
+                         m->af[k] * gi[M3G(i, j, m->kpm[k])];
+        m->resid[M3I(i, j, k)] = v;
+        if (fabs(v) > resm) resm = fabs(v);
+      }
+  return resm;
+}
+
+/* godown (poiss_mgrd.f90:435-458): restriction into level l */
+static void m3_godown(ora *o, int64_t l) {
+  struct mgrd *m = o->m3;
+  double *r = m->resid;
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = m->iimg[l][1]; i <= m->iimg[l][2]; ++i)
+        r[M3I(i, j, k)] = (1.0 - m->fidw[i]) * (1.0 - m->fkdw[k]) * r[M3I(m->ih1[i], j, m->kh1[k])] +
+                          m->fidw[i] * (1.0 - m->fkdw[k]) * r[M3I(m->ih2[i], j, m->kh1[k])] +
+                          (1.0 - m->fidw[i]) * m->fkdw[k] * r[M3I(m->ih1[i], j, m->kh2[k])] +
+                          m->fidw[i] * m->fkdw[k] * r[M3I(m->ih2[i], j, m->kh2[k])];
+}
+
+/* goup (poiss_mgrd.f90:461-486): the checkerboard lines of level l take the
+ * bilinear (x, z) interpolation of level l-1 */
+static void m3_goup(ora *o, int64_t l) {
+  struct mgrd *m = o->m3;
+  double *gi = m->gi;
+  for (int64_t k = m->kkmg[l][1]; k <= m->kkmg[l][2]; ++k) {
+    const int64_t is = m->iimg[l][1] + k % 2;
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = is; i <= m->iimg[l][2]; i += 2)
+        gi[M3G(i, j, k)] = gi[M3G(i, j, k)] +
+                           (1.0 - m->fiup[i]) * (1.0 - m->fkup[k]) * gi[M3G(m->il1[i], j, m->kl1[k])] +
+                           m->fiup[i] * (1.0 - m->fkup[k]) * gi[M3G(m->il2[i], j, m->kl1[k])] +
+                           (1.0 - m->fiup[i]) * m->fkup[k] * gi[M3G(m->il1[i], j, m->kl2[k])] +
+                           m->fiup[i] * m->fkup[k] * gi[M3G(m->il2[i], j, m->kl2[k])];
+  }
+}
+
+/* relax (poiss_mgrd.f90:489-675) */
+static void m3_relax(ora *o, int64_t l, int64_t iter, int64_t iold) {
+  m3_half(o, l, 0, (double)iold, NULL);
+  m3_half(o, l, 1, 1.0, NULL);
+  for (int64_t q = 1; q <= iter - 1; ++q) {
+    m3_half(o, l, 0, 1.0, NULL);
+    m3_half(o, l, 1, 1.0, NULL);
+  }
+}
+
+/* toplevel (poiss_mgrd.f90:271-404) */
+static void m3_toplevel(ora *o, const double *rhs, int64_t iold) {
+  struct mgrd *m = o->m3;
+  const int64_t L = m->nlev;
+  m3_half(o, L, 0, (double)iold, rhs);
+  m3_half(o, L, 1, 1.0, rhs);
+  m3_residual(o, L, rhs);
+  if (L == 0) return;
+  m3_godown(o, L - 1);
+}
+
+/* mg3d (poiss_mgrd.f90:227-268): one V-cycle from gi = 0 (the reference's
+ * automatic array holds whatever the stack held; every value it contributes
+ * there is multiplied by a zero coefficient) */
+static void mg3d(ora *o, double *sol, const double *rhs) {
+  struct mgrd *m = o->m3;
+  const int64_t L = m->nlev;
+  memset(m->gi, 0, (size_t)(m->n1md + 1) * (o->n2 + 1) * (m->n3md + 1) * 8);
+  memset(m->resid, 0, (size_t)m->n1md * o->n2m * m->n3md * 8);
+  m3_toplevel(o, rhs, 0);
+  if (L != 0) {
+    for (int64_t l = L - 1; l >= 1; --l) {
+      m3_relax(o, l, 1, 0);
+      m3_residual(o, l, NULL);
+      m3_godown(o, l - 1);
+    }
+    m3_relax(o, 0, o->s.nbli, 0);
+    for (int64_t l = 1; l <= L - 1; ++l) {
+      m3_goup(o, l);
+      m3_relax(o, l, 1, 1);
+    }
+    m3_goup(o, L);
+  }
+  m3_toplevel(o, rhs, 1);
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) F(sol, i, j, k) = m->gi[M3G(i, j, k)];
+}
+
+/* v = A y on the finest level (poiss_mgrd.f90:107-118) */
+static void m3_apply(ora *o, double *v, const double *y) {
+  struct mgrd *m = o->m3;
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) {
+        const double acc = m3_acc(m, i, j, k);
+        F(v, i, j, k) = m->aw[i] * F(y, m->imm[i], j, k) + m->ae[i] * F(y, m->ipm[i], j, k) +
+                        m->as[j] * F(y, i, j - 1, k) + m->an[j] * F(y, i, j + 1, k) +
+                        m->ab[k] * F(y, i, j, m->kmm[k]) + m->af[k] * F(y, i, j, m->kpm[k]) + acc * F(y, i, j, k);
+      }
+}
+
+/* poisson (poiss_mgrd.f90:25-224), IOLDV = 0 (phi = 0 every substep,
+ * solver.f90:113-117) */
+static void poisson_mgrd(ora *o) {
+  struct mgrd *m = o->m3;
+  const size_t np = (size_t)NX1 * NY1 * NZ1;
+  double *phi = o->phi, *dv = o->divgsum;
+  memset(phi, 0, np * 8);
+  memset(m->p_cg, 0, np * 8); memset(m->s_cg, 0, np * 8); memset(m->y_cg, 0, np * 8);
+  memset(m->z_cg, 0, np * 8); memset(m->v_cg, 0, np * 8); memset(m->t_cg, 0, np * 8);
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) F(m->res_cg, i, j, k) = F(dv, i, j, k);
+  double rho_ol = 1.0, alp = 1.0, w = 1.0, resm = 0.0;
+  int64_t icg;
+  if (o->msub == 1) { m->nit = 0; m->nsolve = 0; }
+  for (icg = 1; icg <= o->s.mgitr; ++icg) {
+    double rho = 0.0;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i) rho = rho + F(dv, i, j, k) * F(m->res_cg, i, j, k);
+    const double bet = rho / rho_ol * alp / w;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i)
+          F(m->p_cg, i, j, k) = F(m->res_cg, i, j, k) + bet * (F(m->p_cg, i, j, k) - w * F(m->v_cg, i, j, k));
+    mg3d(o, m->y_cg, m->p_cg);
+    m3_apply(o, m->v_cg, m->y_cg);
+    double rv = 0.0;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i) rv = rv + F(dv, i, j, k) * F(m->v_cg, i, j, k);
+    alp = rho / rv;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i)
+          F(m->s_cg, i, j, k) = F(m->res_cg, i, j, k) - alp * F(m->v_cg, i, j, k);
+    mg3d(o, m->z_cg, m->s_cg);
+    m3_apply(o, m->t_cg, m->z_cg);
+    double bttt = 0.0, uppp = 0.0;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i) {
+          const double t = F(m->t_cg, i, j, k);
+          uppp = uppp + t * F(m->s_cg, i, j, k);
+          bttt = bttt + t * t;
+        }
+    w = uppp / bttt;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i)
+          F(phi, i, j, k) = F(phi, i, j, k) + alp * F(m->y_cg, i, j, k) + w * F(m->z_cg, i, j, k);
+    resm = 0.0;
+    int64_t im = 0, jm = 0, km = 0;
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j)
+        for (int64_t i = 1; i <= o->n1m; ++i) {
+          const double r = F(m->s_cg, i, j, k) - w * F(m->t_cg, i, j, k);
+          F(m->res_cg, i, j, k) = r;
+          resm = fmax(resm, fabs(r));
+          if (resm == fabs(r)) { im = i; jm = j; km = k; }
+        }
+    rho_ol = rho;
+    if (m->nit < 3 * 4096) {
+      const int64_t q = m->nit++;
+      m->it_sub[q] = o->msub; m->it_icg[q] = icg; m->it_rm[q] = resm * o->dtconst;
+      m->it_loc[3 * q] = im; m->it_loc[3 * q + 1] = jm; m->it_loc[3 * q + 2] = km;
+    }
+    if (resm < o->test1) break;
+  }
+  if (icg > o->s.mgitr) fprintf(stderr, "oracle: BiCGSTAB not converged\n");
+  const double phiref = F(phi, 1, o->n2m, 1);
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i) F(phi, i, j, k) = F(phi, i, j, k) - phiref;
+  if (m->nsolve < 3) {
+    m->sv_icg[m->nsolve] = icg;
+    m->sv_rm[m->nsolve] = resm * o->dtconst;
+    m->nsolve += 1;
+  }
+  if (icg > o->mg_cycles_max) o->mg_cycles_max = icg;
+  o->mg_cycles_sum += icg;
+}
+
+static void poisson_any(ora *o) {
+  if (o->poiss_kind == 1) {
+    o->mg_cycles_max = 0;
+    poisson_mgrd(o);
+  } else {
+    poisson(o);
+  }
+}
+
+int ora_set_poisson(ora *o, int64_t kind) {
+  if (kind != 0 && kind != 1) return -1;
+  if (kind == 1) {
+    const int64_t nl = o->s.nlev;
+    if (o->s.ioldv != 0 || o->s.xprdic != 0 || nl < 0 || (o->n1m % (1LL << nl)) != 0 ||
+        (o->n3m % (1LL << nl)) != 0 || o->n1m / (1LL << nl) < 1 || o->n3m / (1LL << nl) < 1)
+      return -1;
+    if (!o->m3) mgrd_init(o);
+  }
+  o->poiss_kind = (int)kind;
+  return 0;
+}
+
+int64_t ora_bicg_log(const ora *o, int64_t *sub, int64_t *icg, double *rm, int64_t *loc, int64_t cap) {
+  const struct mgrd *m = o->m3;
+  if (!m) return 0;
+  for (int64_t q = 0; q < m->nit && q < cap; ++q) {
+    sub[q] = m->it_sub[q]; icg[q] = m->it_icg[q]; rm[q] = m->it_rm[q];
+    for (int c = 0; c < 3; ++c) loc[3 * q + c] = m->it_loc[3 * q + c];
+  }
+  return m->nit;
+}
+
+int64_t ora_bicg_solves(const ora *o, int64_t *icg, double *rm) {
+  const struct mgrd *m = o->m3;
+  if (!m) return 0;
+  for (int64_t q = 0; q < m->nsolve; ++q) { icg[q] = m->sv_icg[q]; rm[q] = m->sv_rm[q]; }
+  return m->nsolve;
 }

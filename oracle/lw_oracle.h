@@ -93,6 +93,17 @@ void ora_phase_energy(ora *o);
 void ora_stepinit(ora *o);
 
 /* building blocks exposed for unit tests */
+/* Poisson path: 0 = poiss_z_ft (default), 1 = poiss_mgrd (BiCGSTAB with a
+ * 3-D multigrid preconditioner; IOLDV = 0, z periodic or not, x not
+ * periodic). Returns 0, or -1 when the case is outside that scope. */
+int ora_set_poisson(ora *o, int64_t kind);
+/* the BiCGSTAB log of the last step: per iteration (substep, iteration,
+ * max |residual| * dtconst, its cell i, j, k as poiss_itr.dat holds them);
+ * per solve (the final iteration number, MGITR+1 if not converged, and
+ * residual, as ftrpoittr.dat holds them) */
+int64_t ora_bicg_log(const ora *o, int64_t *sub, int64_t *icg, double *rm, int64_t *loc, int64_t cap);
+int64_t ora_bicg_solves(const ora *o, int64_t *icg, double *rm);
+
 void ora_fft_forward(int n, double *re, double *im);   /* e^{-2 pi i jk/n} */
 void ora_fft_inverse(int n, double *re, double *im);   /* conj trick, 1/n */
 void ora_thomas(int64_t n, const double *a, const double *b, const double *c,

@@ -44,6 +44,14 @@ def c2_spec(nend=1):
     return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s)
 
 
+def golden_view(meta):
+    """The case view of a golden fixture, with the Poisson solver it ran."""
+    view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))))
+    if meta.get("poisson") == "mgrd":
+        view["settings"].poisson = 1
+    return view
+
+
 def golden(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     meta = json.loads(str(z["meta"]))

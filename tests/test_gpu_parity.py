@@ -9,7 +9,7 @@ import pytest
 
 from leswht_amd import case as C
 from leswht_amd.solver import Solver
-from tests.helpers import fld_regions, golden, golden_names, rel, spec_for
+from tests.helpers import fld_regions, golden, golden_names, golden_view, rel, spec_for
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -22,7 +22,7 @@ def _view(dims, mode, nend=1, nlev=None):
 @pytest.mark.parametrize("name", golden_names())
 def test_gpu_matches_reference_golden(name):
     z, meta = golden(name)
-    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
+    view = golden_view(meta)
     print("golden", name, flush=True)
     g = Solver.from_case(view)
     st = g.step(meta["nend"])
@@ -128,6 +128,27 @@ def test_gpu_full_c2_step_matches_oracle():
         assert err <= TOL, (k, err)
 
 
+@pytest.mark.parametrize("dims,mode,nlev,nsteps", [((64, 64, 16), "les", 3, 2), ((96, 64, 32), "les_cfl", 4, 2)])
+def test_gpu_bicgstab_path_matches_oracle(dims, mode, nlev, nsteps):
+    """The 3-D MG-preconditioned BiCGSTAB Poisson path (poiss_mgrd.f90)
+    against the oracle's restatement on shapes the fixtures do not cover:
+    fields within 1e-10, the same iteration count in every solve."""
+    import oracle
+    view = _view(dims, mode, nsteps, nlev)
+    view["settings"].poisson = 1
+    print("bicgstab", dims, mode, nlev, flush=True)
+    o = oracle.Oracle(view, poisson="mgrd")
+    g = Solver.from_case(view)
+    for _ in range(nsteps):
+        o.step(1)
+        g.step(1)
+        assert [q[0] for q in g.bicg_log()[1]] == [q[0] for q in o.bicg_log()[1]]
+    fo, fg = o.fields(), g.fields()
+    for k in "uvwpt":
+        err = rel(fo[k], fg[k])
+        assert err <= TOL, (k, err)
+
+
 def test_gpu_full_c1_size_matches_oracle():
     """192x128x16 (the plain FP64 configuration's grid), one step."""
     import oracle
@@ -190,13 +211,18 @@ def test_gpu_rejects_out_of_scope_configuration():
 def _hist_rows(sol, nend):
     """Per step: the history rows, the mgftresiduemax.dat rows and, every NTR
     steps, the tracer rows, as the reference writes them."""
-    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": []}
+    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": [], "bicg_itr": [], "bicg_solve": []}
     ntrace, ntr, _ = sol.trace_info()
     trows = {c: [] for c in "uvwp"}
+    mgrd = sol.settings()["poisson"] == 1
     for _ in range(nend):
         sol.step(1)
         h = sol.history()
         rows["mgres"] += [list(r) for r in sol.mg_residuals()]
+        if mgrd:
+            it, sv = sol.bicg_log()
+            rows["bicg_itr"] += [[r[1], r[2], r[3], r[4], r[5]] for r in it]
+            rows["bicg_solve"] += [[q[0], q[1]] for q in sv]
         if ntrace and h["ntime"] % ntr == 0:
             tr = sol.trace()
             for c in "uvwp":
@@ -226,6 +252,27 @@ def _check_mgres(name, ref, got):
     assert ref.shape == got.shape, (name, ref.shape, got.shape)
     assert np.array_equal(ref[:, :2], got[:, :2]), (name, "kv/kk")
     np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=1e-3, atol=1e-12, err_msg=name)
+
+
+def _check_bicg(name, z, got):
+    """poiss_mgrd's logs: the same iteration count in every solve
+    (ftrpoittr.dat) and the same sequence of iterations (poiss_itr.dat), with
+    the residuals to 1e-3 (the dot products are block sums here, serial
+    loops there, and the residual is a small difference of large terms).
+    The cell holding the largest residual may differ where two cells hold
+    values equal to rounding: at most a tenth of the rows may differ."""
+    ref = z["bicg_solve"]
+    g = np.array(got["bicg_solve"], dtype=np.float64).reshape(-1, 2)
+    assert len(ref) == len(g), (name, len(ref), len(g))
+    assert np.array_equal(ref[:, 2], g[:, 0]), (name, "iterations per solve")
+    np.testing.assert_allclose(g[:, 1], ref[:, 3], rtol=1e-3, atol=1e-12, err_msg=name)
+    ri = z["bicg_itr"]
+    gi = np.array(got["bicg_itr"], dtype=np.float64).reshape(-1, 5)
+    assert ri.shape == gi.shape, (name, ri.shape, gi.shape)
+    assert np.array_equal(ri[:, 0], gi[:, 0]), (name, "iteration sequence")
+    np.testing.assert_allclose(gi[:, 1], ri[:, 1], rtol=1e-3, atol=1e-12, err_msg=name)
+    differ = np.sum(np.any(ri[:, 2:] != gi[:, 2:], axis=1))
+    assert differ <= len(ri) // 10, (name, "argmax cells differ in", differ, "rows")
 
 
 def _check_trace(name, z, got):
@@ -261,12 +308,15 @@ def test_gpu_history_matches_reference_golden(name):
     """fhist, fcdcl, fheatflux and ftemp rows of every step against the
     reference's own history files (draglift, heatflux, writehistory)."""
     z, meta = golden(name)
-    view = _view(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
+    view = golden_view(meta)
     print("history", name, flush=True)
     rows = _hist_rows(Solver.from_case(view), meta["nend"])
     for key in ("hist", "cdcl", "hflux", "temp"):
         _check_rows(name, key, z[key], rows[key])
-    _check_mgres(name, z["mgres"], rows["mgres"])
+    if "mgres" in z.files:
+        _check_mgres(name, z["mgres"], rows["mgres"])
+    if "bicg_solve" in z.files:
+        _check_bicg(name, z, rows)
     if "trace_u" in z.files:
         _check_trace(name, z, rows)
 
@@ -280,7 +330,7 @@ def _mg_file_rows(path):
     return np.array(rows).reshape(-1, 3)
 
 
-@pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4"])
+@pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4", "mgrd_cyl32_les_3"])
 def test_gpu_fortran_host_matches_reference(name):
     """The Fortran solver_exec (ISO_C_BINDING host over the C ABI) run on the
     reference's case directory writes the reference's fld file, history rows,
@@ -295,8 +345,12 @@ def test_gpu_fortran_host_matches_reference(name):
     with tempfile.TemporaryDirectory() as root:
         C.write_case(C.build_case(spec), root)
         print("fortran host", name, flush=True)
+        env = dict(os.environ)
+        env.pop("LW_POISSON", None)
+        if meta.get("poisson") == "mgrd":
+            env["LW_POISSON"] = "mgrd"  # the reference's link-time choice of poiss_mgrd
         res = subprocess.run([EXE], cwd=os.path.join(root, "02_solver"), capture_output=True, text=True,
-                             timeout=120)
+                             timeout=120, env=env)
         assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
         fl = C.read_fld(os.path.join(root, "output/field/fld%07d" % int(z["ihist"])))
         assert fl["ihist"] == int(z["ihist"])
@@ -311,7 +365,14 @@ def test_gpu_fortran_host_matches_reference(name):
             _check_rows(name, key, z[key], got)
         tim = [ln.split() for ln in open(os.path.join(root, "output/ftr/ftime.dat")) if ln.strip()]
         assert len(tim) == meta["nend"] and all(len(r) == 5 for r in tim)
-        _check_mgres(name, z["mgres"], _mg_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat")))
+        if "mgres" in z.files:
+            _check_mgres(name, z["mgres"], _mg_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat")))
+        if "bicg_solve" in z.files:
+            txt = lambda f: np.array([[float(x) for x in ln.split()] for ln in  # noqa: E731
+                                      open(os.path.join(root, "output/ftr", f)) if ln.strip()])
+            sv, it = txt("ftrpoittr.dat"), txt("poiss_itr.dat")
+            _check_bicg(name, z, {"bicg_solve": sv[:, 2:4], "bicg_itr": it})
+            assert np.all(np.abs(sv[:, :2] - z["bicg_solve"][:, :2]) <= 1e-5)
         if "trace_u" in z.files:
             got = {"trace_" + c: np.array([[float(x) for x in ln.split()] for ln in
                                            open(os.path.join(root, "output/ftr/f%strace.dat" % c)) if ln.strip()])

@@ -14,7 +14,7 @@ def test_oracle_matches_reference_golden(name, oracle_lib):
     z, meta = golden(name)
     spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))
     view = C.case_view(C.build_case(spec))
-    o = oracle.Oracle(view)
+    o = oracle.Oracle(view, poisson=meta.get("poisson", "z_ft"))
     o.step(meta["nend"])
     f = o.fields()
     n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
@@ -25,6 +25,31 @@ def test_oracle_matches_reference_golden(name, oracle_lib):
     for k, sl in fld_regions(n).items():
         err = rel(z[k][sl], f[k][sl])
         assert err <= bar, (name, k, err)
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names() if n.startswith("mgrd_")])
+def test_oracle_bicgstab_logs_match_reference(name, oracle_lib):
+    """poiss_mgrd's logs: per solve (ftrpoittr.dat: iterations and final
+    residual * dtconst) over every step, per iteration of the last step
+    (poiss_itr.dat: residual and the cell holding it). The oracle's sums run
+    in the reference's order, so the rows agree to the digits printed."""
+    import oracle
+    z, meta = golden(name)
+    view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))))
+    o = oracle.Oracle(view, poisson="mgrd")
+    solves = []
+    for _ in range(meta["nend"]):
+        o.step(1)
+        it, sv = o.bicg_log()
+        solves += sv
+    ref = z["bicg_solve"]
+    assert len(solves) == len(ref)
+    assert [s[0] for s in solves] == [int(r[2]) for r in ref]
+    np.testing.assert_allclose([s[1] for s in solves], ref[:, 3], rtol=1e-6)
+    last = z["bicg_itr"][-len(it):]
+    assert [r[1] for r in it] == [int(r[0]) for r in last]
+    np.testing.assert_allclose([r[2] for r in it], last[:, 1], rtol=1e-6)
+    assert [tuple(r[3:]) for r in it] == [tuple(int(v) for v in r[2:]) for r in last]
 
 
 def test_makefld_matches_reference_initial_field():
