@@ -4,6 +4,8 @@
 // sgs_cedm 1250-1294), the Simpson box test filter (test_filter 641-769),
 // zeroing in the body (nutzero/alpzero) and edge interpolation
 // (nutinterpol 941-1133, alpinterpol 1591-1773).
+#include <stdlib.h>
+
 #include "lw_kernels.h"
 
 namespace lw {
@@ -295,6 +297,258 @@ static void filter_many(hipStream_t s, const Geo &g, SgsBuf &b, int lo, int hi, 
     b.a[l] = b.tmp;
     b.tmp = t;
   }
+}
+
+// ---------------------------------------------------------------------------
+// The dynamic models in one pass each. The quantities the reference filters
+// (sgs.f90:405-638 Vreman: alpha_ij, nu S_ij, u_i, u_i u_j; 1297-1478 eddy
+// diffusivity: S_ij, dT/dx_i, alpha dT/dx_i, T, u_i, u_i T) are formed per
+// cell in registers and filtered on the fly instead of going through 24
+// interior arrays and one filter launch per array: a wavefront owns 62
+// consecutive cells of a row j (lanes 1..62; lanes 0 and 63 compute the x
+// neighbours), takes each cell's quantities from the lanes next to it for
+// the x filter and marches kSgsK planes up the periodic z-direction, adding
+// each plane's x-filtered values into the z-filtered sums of the planes
+// below, above and at it in the order test_filter adds them (641-769). The
+// Germano products of each cell (593-618, 1447-1462) go into the thread's
+// sums; workgroup sums go to f.partial for k_coef. The filtered values are
+// the reference's to the bit; the sums differ from a serial loop's by the
+// order of the additions only.
+// ---------------------------------------------------------------------------
+constexpr int kSgsK = 16;   // planes a wavefront marches
+constexpr int kSgsOwn = 62; // cells of a row a wavefront owns
+
+struct SgsTile {
+  int i, ic, j, k0, k1, lane;
+  bool own, xi;
+  double cx0, cx1, cx2;
+};
+
+__device__ inline bool sgs_tile(const Geo &g, long t, int fx, SgsTile &T) {
+  const int nti = (g.n1m + kSgsOwn - 1) / kSgsOwn, ntk = (g.n3m + kSgsK - 1) / kSgsK;
+  if (t >= (long)nti * g.n2m * ntk) return false;
+  const int ti = (int)(t % nti);
+  const long rest = t / nti;
+  T.j = 1 + (int)(rest % g.n2m);
+  const int tk = (int)(rest / g.n2m);
+  T.lane = threadIdx.x & 63;
+  T.i = ti * kSgsOwn + T.lane;
+  T.ic = T.i < 1 ? 1 : (T.i > g.n1m ? g.n1m : T.i);
+  T.own = T.lane >= 1 && T.lane <= kSgsOwn && T.i <= g.n1m;
+  T.xi = fx && T.i >= 2 && T.i <= g.n1m - 1;
+  const int ir = T.xi ? T.i : 2;
+  T.cx0 = g.cfx1[3 * ir];
+  T.cx1 = g.cfx1[3 * ir + 1];
+  T.cx2 = g.cfx1[3 * ir + 2];
+  T.k0 = 1 + tk * kSgsK;
+  T.k1 = T.k0 + kSgsK - 1 < g.n3m ? T.k0 + kSgsK - 1 : g.n3m;
+  return true;
+}
+
+// the x filter, in place, of NQ quantities of the lanes' cells (every lane
+// of the wavefront calls it)
+template <int NQ>
+__device__ inline void sgs_xfilter(const SgsTile &T, double (&q)[NQ]) {
+#pragma unroll
+  for (int l = 0; l < NQ; ++l) {
+    const double am = __shfl_up(q[l], 1, 64), ap = __shfl_down(q[l], 1, 64);
+    q[l] = T.xi ? T.cx0 * am + T.cx1 * q[l] + T.cx2 * ap : q[l];
+  }
+}
+
+// Vreman quantities of a cell: alpha_ij (9), nu S_ij (6), u_i (3), u_i u_j
+// (6) (sgs.f90:434-470, 538-566); returns nu
+__device__ inline double dvmg_cell(const Geo &g, const Fld &f, int i, int j, int k, double (&q)[24]) {
+  double alp[9], str[6];
+  velgrad(g, f, i, j, k, alp, str);
+  const double r = vreman(alp, g.f2fx[i] * g.f2fx[i], g.f2fy[j] * g.f2fy[j], g.f2fz[k] * g.f2fz[k]);
+#pragma unroll
+  for (int l = 0; l < 9; ++l) q[l] = alp[l];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) q[9 + l] = r * str[l];
+  const long c = P(g, i, j, k);
+  const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
+  const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
+  const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
+  q[15] = u1; q[16] = u2; q[17] = u3;
+  q[18] = u1 * u1; q[19] = u1 * u2; q[20] = u1 * u3;
+  q[21] = u2 * u2; q[22] = u2 * u3; q[23] = u3 * u3;
+  return r;
+}
+
+// the Germano products of a cell from its filtered quantities (k_dvmg2's M,
+// k_dvmg3's L and sums)
+__device__ inline void dvmg_germano(const Geo &g, int i, int j, int k, const double (&F)[24], const int64_t *nwall,
+                                    double &lm, double &mm) {
+  double alp[9], str[6];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) alp[l] = F[l];
+  str[0] = alp[0];
+  str[1] = 0.5 * (alp[3] + alp[1]);
+  str[2] = 0.5 * (alp[6] + alp[2]);
+  str[3] = alp[4];
+  str[4] = 0.5 * (alp[7] + alp[5]);
+  str[5] = alp[8];
+  const double dx = fwidth(g.f2fx[i - 1], g.f2fx[i], g.f2fx[i + 1], g.fixiu[i], g.fixil[i]);
+  const double dy = fwidth(g.f2fy[j - 1], g.f2fy[j], g.f2fy[j + 1], g.fixju[j], g.fixjl[j]);
+  const double dz = fwidth(g.f2fz[k - 1], g.f2fz[k], g.f2fz[k + 1], g.fixku[k], g.fixkl[k]);
+  const double ami = vreman(alp, dx * dx, dy * dy, dz * dz);
+  const double m1 = ami * str[0] - F[9], m2 = ami * str[1] - F[10], m3 = ami * str[2] - F[11],
+               m4 = ami * str[3] - F[12], m5 = ami * str[4] - F[13], m6 = ami * str[5] - F[14];
+  const double U1 = F[15], U2 = F[16], U3 = F[17];
+  const double l1 = F[18] - U1 * U1, l2 = F[19] - U1 * U2, l3 = F[20] - U1 * U3, l4 = F[21] - U2 * U2,
+               l5 = F[22] - U2 * U3, l6 = F[23] - U3 * U3;
+  const double volume = g.f2fx[i] * g.f2fy[j] * g.f2fz[k];
+  const double nw = (double)nwall[I3(g, i, j, k)];
+  lm += (2. * l2 * m2 + l1 * m1 + 2. * l3 * m3 + l4 * m4 + 2. * l5 * m5 + l6 * m6) * nw * volume;
+  mm += (2. * (m2 * m2) + m1 * m1 + 2. * (m3 * m3) + m4 * m4 + 2. * (m5 * m5) + m6 * m6) * nw * volume;
+}
+
+// eddy-diffusivity quantities of a cell: S_ij (6), dT/dx_i (3), alpha
+// dT/dx_i (3), T, u_i (3), u_i T (3) (sgs.f90:1325-1345, 1395-1430);
+// returns alpha
+__device__ inline double edm_cell(const Geo &g, const Fld &f, int i, int j, int k, double (&q)[19]) {
+  double d[9], s[6], alp[3];
+  velgrad(g, f, i, j, k, d, s);
+  temgrad(g, f, i, j, k, alp);
+  const double del = pow(g.f2fx[i] * g.f2fy[j] * g.f2fz[k], 1. / 3.);
+  double sss = s[0] * s[0] + 2 * (s[1] * s[1]) + 2 * (s[2] * s[2]) + s[3] * s[3] + 2 * (s[4] * s[4]) + s[5] * s[5];
+  sss = fmax(sss, 1.0e-16);
+  const double al = (del * del) * sqrt(2. * sss);
+#pragma unroll
+  for (int l = 0; l < 6; ++l) q[l] = s[l];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    q[6 + l] = alp[l];
+    q[9 + l] = al * alp[l];
+  }
+  const long c = P(g, i, j, k);
+  const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
+  const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
+  const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
+  const double tc = f.t[c];
+  q[12] = tc;
+  q[13] = u1; q[14] = u2; q[15] = u3;
+  q[16] = u1 * tc; q[17] = u2 * tc; q[18] = u3 * tc;
+  return al;
+}
+
+__device__ inline void edm_germano(const Geo &g, int i, int j, int k, const double (&F)[19], const int64_t *nwall,
+                                   double &lm, double &mm) {
+  const double d1 = fwidth(g.f2fx[i - 1], g.f2fx[i], g.f2fx[i + 1], g.fixiu[i], g.fixil[i]);
+  const double d2 = fwidth(g.f2fy[j - 1], g.f2fy[j], g.f2fy[j + 1], g.fixju[j], g.fixjl[j]);
+  const double d3 = fwidth(g.f2fz[k - 1], g.f2fz[k], g.f2fz[k + 1], g.fixku[k], g.fixkl[k]);
+  const double del = pow(d1 * d2 * d3, 1. / 3.);
+  const double a1 = F[0], a2 = F[1], a3 = F[2], a4 = F[3], a5 = F[4], a6 = F[5];
+  double sss = a1 * a1 + 2 * (a2 * a2) + 2 * (a3 * a3) + a4 * a4 + 2 * (a5 * a5) + a6 * a6;
+  sss = fmax(sss, 1.0e-16);
+  const double ami = (del * del) * sqrt(2. * sss);
+  const double m1 = ami * F[6] - F[9], m2 = ami * F[7] - F[10], m3 = ami * F[8] - F[11];
+  const double l1 = F[16] - F[13] * F[12], l2 = F[17] - F[14] * F[12], l3 = F[18] - F[15] * F[12];
+  const double volume = g.f2fx[i] * g.f2fy[j] * g.f2fz[k];
+  const double nw = (double)nwall[I3(g, i, j, k)];
+  lm += (l1 * m1 + l2 * m2 + l3 * m3) * nw * volume;
+  mm += (m1 * m1 + m2 * m2 + m3 * m3) * nw * volume;
+}
+
+// MODEL 0: dynamic Vreman (writes nu), 1: dynamic eddy diffusivity (writes
+// alpha). Workgroups of 4 wavefronts walk the tiles; each workgroup's two
+// sums go to f.partial[2 b], f.partial[2 b + 1].
+template <int MODEL, int NQ>
+__global__ __launch_bounds__(256) void k_sgs_dyn(Geo g, Fld f, const int64_t *nwall, int fx, int fz) {
+  __shared__ double sh[4];
+  double lm = 0.0, mm = 0.0;
+  const int w = threadIdx.x >> 6;
+  SgsTile T;
+  for (long t = (long)blockIdx.x * 4 + w; sgs_tile(g, t, fx, T); t += (long)gridDim.x * 4) {
+    const int n3m = g.n3m;
+    // z sums: A of the plane awaiting its last term, B of the plane awaiting
+    // its middle one
+    double A[NQ], B[NQ];
+    double x[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) A[l] = B[l] = 0.0;
+    const int kb = fz ? T.k0 - 1 : T.k0, ke = fz ? T.k1 + 1 : T.k1;
+    for (int kk = kb; kk <= ke; ++kk) {
+      const int k = kk < 1 ? n3m : (kk > n3m ? 1 : kk);
+      double v;
+      if constexpr (MODEL == 0) v = dvmg_cell(g, f, T.ic, T.j, k, x);
+      else v = edm_cell(g, f, T.ic, T.j, k, x);
+      if (T.own && kk >= T.k0 && kk <= T.k1) (MODEL == 0 ? f.nu : f.al)[P(g, T.i, T.j, k)] = v;
+      sgs_xfilter<NQ>(T, x);
+      if (!fz) {
+        if (T.own) {
+          if constexpr (MODEL == 0) dvmg_germano(g, T.i, T.j, k, x, nwall, lm, mm);
+          else edm_germano(g, T.i, T.j, k, x, nwall, lm, mm);
+        }
+        continue;
+      }
+      // the plane below (kk - 1) is complete with this plane's term
+      if (kk >= T.k0 + 1) {
+        const int km = kk - 1;
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) A[l] = A[l] + g.cfz1[3 * km + 2] * x[l];
+        if (T.own) {
+          if constexpr (MODEL == 0) dvmg_germano(g, T.i, T.j, km, A, nwall, lm, mm);
+          else edm_germano(g, T.i, T.j, km, A, nwall, lm, mm);
+        }
+      }
+      // this plane's middle term; it then awaits its last
+      if (kk >= T.k0 && kk <= T.k1) {
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) A[l] = B[l] + g.cfz1[3 * k + 1] * x[l];
+      }
+      // the first term of the plane above
+      if (kk + 1 <= T.k1) {
+        const int kp = kk + 1;
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) B[l] = g.cfz1[3 * kp] * x[l];
+      }
+    }
+  }
+  // workgroup sums (fixed shape)
+  for (int o = 32; o > 0; o >>= 1) {
+    lm += __shfl_down(lm, o, 64);
+    mm += __shfl_down(mm, o, 64);
+  }
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) sh[w] = lm;
+  __syncthreads();
+  if (threadIdx.x == 0) f.partial[2 * blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  __syncthreads();
+  if (lane == 0) sh[w] = mm;
+  __syncthreads();
+  if (threadIdx.x == 0) f.partial[2 * blockIdx.x + 1] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+bool sgs_fused_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("LW_SGS_FUSED");
+    return !(e && atoi(e) == 0);
+  }();
+  return on;
+}
+
+static void launch_sgs_dyn(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int model,
+                           double scale, int slot) {
+  const int fx = (filter == 1 || filter == 3) ? 1 : 0;
+  const int fz = (filter == 2 || filter == 3) ? 1 : 0;
+  const double cells = pts(g);
+  if (model == 0)
+    // u v w read (with their x and z neighbour planes), nu and nwall: about 6 arrays
+    LW_LAUNCH("k_sgs_dvmg", cells * 8.0 * 6, 0, (k_sgs_dyn<0, 24>), dim3(kRedBlocks), dim3(256), 0, s, g, f, nwall,
+              fx, fz);
+  else
+    LW_LAUNCH("k_sgs_edm", cells * 8.0 * 7, 0, (k_sgs_dyn<1, 19>), dim3(kRedBlocks), dim3(256), 0, s, g, f, nwall,
+              fx, fz);
+  LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, scale, slot);
+}
+
+void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter) {
+  launch_sgs_dyn(s, g, f, nwall, filter, 0, -0.5, 4);
+}
+void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter) {
+  launch_sgs_dyn(s, g, f, nwall, filter, 1, -1.0, 5);
 }
 
 // red slots: 4 = csgsts, 5 = csgshf

@@ -100,6 +100,12 @@ struct SgsBuf {
 };
 void launch_sgs_dvmg(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter);
 void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts);
+// the dynamic models in one pass each, filtering on the fly (no SgsBuf);
+// coefficient into red[4] (Vreman) or red[5] (eddy diffusivity).
+// LW_SGS_FUSED=0 selects the separate kernels and filter launches instead.
+bool sgs_fused_enabled();
+void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter);
+void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter);
 void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter);
 void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf);
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1,

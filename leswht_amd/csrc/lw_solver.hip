@@ -189,6 +189,7 @@ struct lw_handle {
   int mg_hint = 4;                         // V-cycles of the previous Poisson solve
   int mg_hint1 = 4;                        // of its kv = 1 alone (two-stream solve)
   int mg_split = 1;                        // kv = 1 on its own stream (LW_MG_SPLIT=0: one stream)
+  int sgs_fused = 1;                       // dynamic SGS in one pass (LW_SGS_FUSED=0: separate kernels)
   DiagPlanes dplanes{};
   double *hist = nullptr;  // device: the step's history values and their running averages
   double h_hist[32] = {0};
@@ -619,8 +620,10 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     for (int l = 0; l < h->navg; ++l) h->avg.a[l] = h->dalloc<double>(ni);
   }
   if (h->les) {
-    for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
-    h->sb.tmp = h->dalloc<double>(ni);
+    if (!h->sgs_fused) {  // the separate-kernel path keeps 24 interior arrays and a spare
+      for (int l = 0; l < 24; ++l) h->sb.a[l] = h->dalloc<double>(ni);
+      h->sb.tmp = h->dalloc<double>(ni);
+    }
     std::vector<int64_t> nw(tb->nwall_dvm, tb->nwall_dvm + ni);
     h->nwall = h->upload(nw);
   }
@@ -985,6 +988,7 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
     // the two-stream solve polls copies into pinned memory; a static handle
     // has none, so it solves on one stream (the same kernels and results)
     if (h->arena) h->mg_split = 0;
+    h->sgs_fused = sgs_fused_enabled() ? 1 : 0;
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     for (int q = 0; q < 3; ++q)
       for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->tev[q][e]));
@@ -1077,7 +1081,7 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
   const int64_t np = (n1m + 2) * (n2m + 2) * (n3m + 2), ni = n1m * n2m * n3m;
   const int64_t heat = s->ihtrans == 1 ? 1 : 0;
   int64_t b = (30 + 4 * heat) * np * 8 + 2 * np;  // fields, mask and a byte of slack per cell
-  if (s->iles == 1) b += 25 * ni * 8 + ni * 8;       // SGS work arrays and nwall
+  if (s->iles == 1) b += (sgs_fused_enabled() ? 0 : 25 * ni * 8) + ni * 8;  // SGS work arrays, nwall
   if (s->iavg == 1) b += (heat ? 23 : 21) * ni * 8;  // field_avg running sums
   if (s->poisson == 1) {
     // poiss_mgrd: gi, resid and the pivots over the stacked levels, and the
@@ -1245,8 +1249,12 @@ static void phase_sgs(lw_handle *h) {
   int slot = -1;
   if (zero_first) HIPCHK(hipMemsetAsync(f.nu, 0, np * sizeof(double), h->s));
   else if (st.idvmon == 1) {
-    launch_sgs_dvmg(h->s, g, f, h->sb, (int)st.filter);
-    launch_sgs_dvmg_coef(h->s, g, f, h->sb, h->nwall);
+    if (h->sgs_fused) {
+      launch_sgs_dvmg_fused(h->s, g, f, h->nwall, (int)st.filter);
+    } else {
+      launch_sgs_dvmg(h->s, g, f, h->sb, (int)st.filter);
+      launch_sgs_dvmg_coef(h->s, g, f, h->sb, h->nwall);
+    }
     slot = 4;
   } else {
     launch_sgs_cvm(h->s, g, f, h->csgsts);
@@ -1257,8 +1265,12 @@ static void phase_sgs(lw_handle *h) {
   slot = -1;
   if (zero_first) HIPCHK(hipMemsetAsync(f.al, 0, np * sizeof(double), h->s));
   else if (st.idvmon == 1) {
-    launch_sgs_edm(h->s, g, f, h->sb, (int)st.filter);
-    launch_sgs_edm_coef(h->s, g, f, h->sb, h->nwall);
+    if (h->sgs_fused) {
+      launch_sgs_edm_fused(h->s, g, f, h->nwall, (int)st.filter);
+    } else {
+      launch_sgs_edm(h->s, g, f, h->sb, (int)st.filter);
+      launch_sgs_edm_coef(h->s, g, f, h->sb, h->nwall);
+    }
     slot = 5;
   } else {
     launch_sgs_cedm(h->s, g, f, h->csgshf);
