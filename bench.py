@@ -252,6 +252,9 @@ def main():
     ap.add_argument("--config", choices=("c2", "c1"), default="c2",
                     help="c2: the 384x256x64 LES the metric is quoted on; c1: the plain 192x128x16 case")
     ap.add_argument("--nlev", type=int, default=0, help="multigrid levels (0: the workload's own)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="store the multigrid level planes below the top level in FP32 (arithmetic, the top "
+                         "level and the convergence test stay FP64); not the default, and not the metric's form")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the multigrid tile sizes with the library's autotuner before the warmup")
     ap.add_argument("--serve", type=int, default=0,
@@ -297,6 +300,8 @@ def main():
     view = C.case_view(case)
     t_build = time.time() - t0
     sol = Solver.from_case(view)
+    if args.mixed:
+        sol.set_option("mg_fp32", 1)
     tune = sol.autotune(reps=2) if args.autotune else None
     # untimed warmup; its last step times every kernel once to find the one
     # with the most device time
@@ -400,12 +405,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32 multigrid levels" if args.mixed else "f64",
         "data": "synthetic (seeded initial field, generated grid and IB tables)",
         "config": {"workload": WORKLOADS[args.config],
                    "cells": n, "nlev": spec.settings.nlev, "resid1": spec.settings.resid1,
                    "mg_cycles_max_last_step": st["mg_cycles_max"], "dt_last": st["dt"],
-                   "case_build_s": round(t_build, 2), "mg_tiles": tune},
+                   "case_build_s": round(t_build, 2), "mg_tiles": tune,
+                   "mg_level_storage": "f32" if args.mixed else "f64"},
         "roofline": roof,
         "poisson": poisson,
         "later_window": later,

@@ -225,7 +225,12 @@ int lw_phase(lw_handle *h, int phase);
 /* per-handle switches, no counterpart in the reference: "mg_split" 1 solves
  * kv = 1 of the Poisson equation on a second stream beside the other
  * wavenumbers, 0 on the handle's one stream (default: 1 when n1m*n2m >= 65536);
- * results are the same bits either way */
+ * results are the same bits either way. "mg_fp32" 1 selects the
+ * mixed-precision form of the multigrid: the level planes below the top
+ * level hold FP32 values while the top level's relaxation, residual,
+ * convergence test and solution stay FP64 (also set at creation by the
+ * environment variable LW_MG_FP32=1); results then differ from the FP64
+ * form by about the solver tolerance. */
 int lw_set_option(lw_handle *h, const char *name, int64_t value);
 /* Autotuner of the fused multigrid kernels' tile sizes (lines per workgroup
  * of the down, up and top kernels; tile1 for kv = 1 on its own stream). Times

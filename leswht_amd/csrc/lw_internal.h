@@ -78,6 +78,7 @@ struct MG {
   double2 *gg, *resd;      // [kv][i][j], all levels stacked along i
   double2 *gd, *rsu;       // fused kernels: down-sweep second halves, residuals
   int fused;               // 1: one kernel per level and direction
+  int f32;                 // 1: the level planes (gg, resd, gd, rsu) hold FP32 values (mixed precision)
   int tile[3], tile1[3];   // lines per workgroup of the fused down, up, top kernels (tile1: kv = 1 side)
   double2 *pc, *rhs;       // [kv][i][j], top level
   double2 *pc2;            // fused path: the other pc buffer (pcsel picks)

@@ -33,6 +33,18 @@ __device__ inline double2 cfma(double a, double2 b, double2 c) { return make_dou
 // stored table, so the same bits (ak3(1) = 0 and x - 0 = x)
 __device__ inline double ac_of(double awe, double as, double an, double ak) { return -((awe + as) + an) - ak; }
 
+// The level planes below the top (gg, resd, gd, rsu) are stored as FP64
+// (double2) or, in the mixed-precision form, as FP32 (float2): values are
+// rounded to FP32 when stored and widened when loaded, and every operation
+// stays FP64. The top level (rhs, pc) and the LU pivots stay FP64.
+__device__ inline double2 ld2(const double2 *p) { return *p; }
+__device__ inline double2 ld2(const float2 *p) {
+  const float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ inline void st2(double2 *p, double2 v) { *p = v; }
+__device__ inline void st2(float2 *p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
 // an LDS read of a row that may not exist: it reads one that does and
 // applies the condition to the value, so every read of a stage is one
 // unconditional 16-byte access and they issue together (a read under a
@@ -731,7 +743,7 @@ __device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, Li
   }
 }
 
-template <int R, bool FULL>
+template <int R, bool FULL, typename S>
 __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
   extern __shared__ double2 lds[];
   const int kv = mg.kv0 + blockIdx.x;
@@ -751,7 +763,8 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   double2 *L1 = lds, *L0 = lds + (long)n1 * R * 64;
   auto slot = [&](int q, int r) { return (q * R + r) * 64 + lane; };
   const long pl = (long)(kv - 1) * mg.PLANE;
-  double2 *gg = mg.gg + pl;
+  S *gg = (S *)mg.gg + pl;
+  S *resd = (S *)mg.resd + pl;
   const int la = 2 * w, lb = la + 1;  // local level-1 lines
   const int ia = b1 + la, ib = ia + 1, ic = b0 + w;
   const int j0 = 1 + lane * R;
@@ -771,15 +784,15 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   double2 resA[R], resB[R], res0[R];
   LineLU<R> cA, cB, c0;
   {
-    const double2 *ra = ROW(mg.resd + pl, ia), *rb = ROW(mg.resd + pl, ib);
+    const S *ra = ROW(resd, ia), *rb = ROW(resd, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = j0 + r;
       const int jc = (FULL || j <= n2m) ? j : n2m;
       const double okf = (FULL || j <= n2m) ? 1.0 : 0.0;
       asj[r] = mg.as[jc];
-      resA[r] = rmul(okf, ra[jc]);
-      resB[r] = rmul(okf, rb[jc]);
+      resA[r] = rmul(okf, ld2(ra + jc));
+      resB[r] = rmul(okf, ld2(rb + jc));
     }
     load_lu<R, FULL>(mg, pl, ia, j0, n2m, cA);
     load_lu<R, FULL>(mg, pl, ib, j0, n2m, cB);
@@ -822,7 +835,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
     for (int r = 0; r < R; ++r) anj[r] = mg.an[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
     {
       const double aw = awA, ae = aeA, awe = weA;
-      double2 *out = ROW(mg.resd + pl, ia);
+      S *out = ROW(resd, ia);
       const double2 lo = lds_if(lane > 0, &L1[(la * R + R - 1) * 64 + lane - 1], &L1[(la * R + R - 1) * 64 + lane]);
       const double2 hi = lds_if(lane < 63, &L1[(la * R) * 64 + lane + 1], &L1[(la * R) * 64 + lane]);
       double2 gi[R];
@@ -840,13 +853,13 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         u = csub(u, rmul(anj[r], up));
         u = csub(u, rmul(ac_of(awe, asj[r], anj[r], ak), gi[r]));
         resA[r] = u;
-        if ((FULL || j0 + r <= n2m)) out[j0 + r] = u;
+        if ((FULL || j0 + r <= n2m)) st2(out + j0 + r, u);
       }
     }
     __builtin_amdgcn_sched_barrier(0);
     {
       const double aw = awB, ae = aeB, awe = weB;
-      double2 *out = ROW(mg.resd + pl, ib);
+      S *out = ROW(resd, ib);
       const bool hp = lb + 1 < n1;
       const double2 lo = lds_if(lane > 0, &L1[(lb * R + R - 1) * 64 + lane - 1], &L1[(lb * R + R - 1) * 64 + lane]);
       const double2 hi = lds_if(lane < 63, &L1[(lb * R) * 64 + lane + 1], &L1[(lb * R) * 64 + lane]);
@@ -864,7 +877,7 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
         v = csub(v, rmul(asj[r], dn));
         v = csub(v, rmul(anj[r], up));
         v = csub(v, rmul(ac_of(awe, asj[r], anj[r], ak), gi[r]));
-        if ((FULL || j0 + r <= n2m)) out[j0 + r] = v;
+        if ((FULL || j0 + r <= n2m)) st2(out + j0 + r, v);
         res0[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, cadd(rmul(c1, resA[r]), rmul(c2, v)));
       }
     }
@@ -931,11 +944,11 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // the new even rows are written where no wavefront of this stage reads
   tri_regs<R, FULL>(rr, cA.bt, asj, cA.gm, j0, n2m, lane, x);
   {
-    double2 *out = ROW(gg, ia);
+    S *out = ROW(gg, ia);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       L1[slot(la, r)] = x[r];
-      if ((FULL || j0 + r <= n2m)) out[j0 + r] = x[r];
+      if ((FULL || j0 + r <= n2m)) st2(out + j0 + r, x[r]);
     }
   }
   __syncthreads();
@@ -943,9 +956,9 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   // relax(1), second half
   load_lu<R, FULL>(mg, pl, ib, j0, n2m, cB);
   {
-    const double2 *rb = ROW(mg.resd + pl, ib);
+    const S *rb = ROW(resd, ib);
 #pragma unroll
-    for (int r = 0; r < R; ++r) resB[r] = rb[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
+    for (int r = 0; r < R; ++r) resB[r] = ld2(rb + ((FULL || j0 + r <= n2m) ? j0 + r : n2m));
   }
   {
     const double aw = awB, ae = aeB;
@@ -957,27 +970,28 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
       rr[r] = rmul((FULL || j0 + r <= n2m) ? 1.0 : 0.0, csub(csub(resB[r], rmul(aw, gmv)), rmul(ae, gpv)));
     }
     tri_regs<R, FULL>(rr, cB.bt, asj, cB.gm, j0, n2m, lane, x);
-    double2 *out = ROW(gg, ib);
+    S *out = ROW(gg, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if ((FULL || j0 + r <= n2m)) out[j0 + r] = x[r];
+      if ((FULL || j0 + r <= n2m)) st2(out + j0 + r, x[r]);
   }
 }
 
 typedef void (*CoarseLdsKernel)(Geo, MG);
 
-template <bool F>
+template <bool F, typename S>
 static CoarseLdsKernel pick_coarse_lds_rf(int R) {
   switch (R) {
-    case 1: return k_mg_coarse_lds<1, F>;
-    case 2: return k_mg_coarse_lds<2, F>;
-    case 4: return k_mg_coarse_lds<4, F>;
+    case 1: return k_mg_coarse_lds<1, F, S>;
+    case 2: return k_mg_coarse_lds<2, F, S>;
+    case 4: return k_mg_coarse_lds<4, F, S>;
     default: return nullptr;
   }
 }
 
-static CoarseLdsKernel pick_coarse_lds(int R, bool full) {
-  return full ? pick_coarse_lds_rf<true>(R) : pick_coarse_lds_rf<false>(R);
+static CoarseLdsKernel pick_coarse_lds(int R, bool full, bool f32 = false) {
+  if (f32) return full ? pick_coarse_lds_rf<true, float2>(R) : pick_coarse_lds_rf<false, float2>(R);
+  return full ? pick_coarse_lds_rf<true, double2>(R) : pick_coarse_lds_rf<false, double2>(R);
 }
 
 // LDS bytes of k_mg_coarse_lds, or 0 when the levels do not fit its shape
@@ -1021,7 +1035,7 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
     const int n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
-    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.nkv), dim3(64, n0),
+    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g), mg.f32), dim3((unsigned)mg.nkv), dim3(64, n0),
               (unsigned)lds, s, g, mg);
     return;
   }
@@ -1136,17 +1150,17 @@ __global__ void k_mg_check(MG mg, double test, int id, int flip) {
 // rather than mg.gg, so the up kernel of a tile reads its neighbours' down
 // values while those neighbours write their up values to mg.gg.
 // ---------------------------------------------------------------------------
-template <int R, bool FULL>
-__device__ inline void load_rows(const double2 *row, int j0, int n2m, double2 (&v)[R]) {
+template <int R, bool FULL, typename T>
+__device__ inline void load_rows(const T *row, int j0, int n2m, double2 (&v)[R]) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = row[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
+  for (int r = 0; r < R; ++r) v[r] = ld2(row + ((FULL || j0 + r <= n2m) ? j0 + r : n2m));
 }
 
-template <int R, bool FULL>
-__device__ inline void store_rows(double2 *row, int j0, int n2m, const double2 (&v)[R]) {
+template <int R, bool FULL, typename T>
+__device__ inline void store_rows(T *row, int j0, int n2m, const double2 (&v)[R]) {
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    if ((FULL || j0 + r <= n2m)) row[j0 + r] = v[r];
+    if ((FULL || j0 + r <= n2m)) st2(row + j0 + r, v[r]);
 }
 
 // XCD-aware placement of the fused kernels' workgroups. gfx950 deals
@@ -1186,7 +1200,7 @@ static inline int fused_tile(int n, int tmax) {
 // godown (poiss_z_ft.f90:431-458) on a tile of level `ilev` >= 1: the relax
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
-template <int R, bool FULL>
+template <int R, bool FULL, typename S>
 __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
                                                                       int cyc) {
   extern __shared__ double2 lds[];
@@ -1220,6 +1234,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
   const int bL = mg.iimg[ilev][1], nL = mg.iimg[ilev][2] - bL + 1, bC = mg.iimg[ilev - 1][1];
   const int f0 = tile * T;
   const long pl = (long)(kv - 1) * mg.PLANE;
+  S *resdS = (S *)mg.resd + pl, *gdS = (S *)mg.gd + pl, *rsuS = (S *)mg.rsu + pl;
   const double2 z2 = make_double2(0.0, 0.0);
   double asj[R];
 #pragma unroll
@@ -1246,13 +1261,13 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
   double2 so[R], se[R];
   LineLU<R> co;
   if (ook) {
-    load_rows<R, FULL>(ROW(mg.resd + pl, bL + o), j0, n2m, so);
+    load_rows<R, FULL>(ROW(resdS, bL + o), j0, n2m, so);
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
   // first half (oldv 0): the right-hand side is resd itself
   if (eok) {
     LineLU<R> ce;
-    load_rows<R, FULL>(ROW(mg.resd + pl, bL + e), j0, n2m, se);
+    load_rows<R, FULL>(ROW(resdS, bL + e), j0, n2m, se);
     load_lu<R, FULL>(mg, pl, bL + e, j0, n2m, ce);
     tri_regs<R, FULL>(se, ce.bt, asj, ce.gm, j0, n2m, lane, x);
   } else {
@@ -1271,7 +1286,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
 #pragma unroll
     for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw2, TILE_AT(2 * w, r))), rmul(ae2, TILE_AT(2 * w + 2, r)));
     tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
-    if (w >= 1) store_rows<R, FULL>(ROW(mg.gd + pl, bL + o), j0, n2m, x);
+    if (w >= 1) store_rows<R, FULL>(ROW(gdS, bL + o), j0, n2m, x);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1295,7 +1310,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
       acA[r] = ac_of(wea, asj[r], anj[r], ak);
       acB[r] = ac_of(web, asj[r], anj[r], ak);
     }
-    double2 *ra = ROW(mg.rsu + pl, ia), *rb = ROW(mg.rsu + pl, ib), *rc = ROW(mg.resd + pl, ic);
+    S *ra = ROW(rsuS, ia), *rb = ROW(rsuS, ib), *rc = ROW(resdS, ic);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = j0 + r;
@@ -1313,9 +1328,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
       xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
       xb = csub(xb, rmul(anj[r], TILE_UP(qa + 1, r)));
       xb = csub(xb, rmul(acB[r], gb));
-      ra[j] = xa;
-      rb[j] = xb;
-      rc[j] = cadd(rmul(c1, xa), rmul(c2, xb));
+      st2(ra + j, xa);
+      st2(rb + j, xb);
+      st2(rc + j, cadd(rmul(c1, xa), rmul(c2, xb)));
     }
   }
 }
@@ -1323,7 +1338,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
 // goup(ilev-1) + relax(ilev, oldv 1) on a tile: first half on lines f0+2w
 // (w <= T/2, the last one a halo) with its neighbours prolonged as read,
 // then the second half on the tile's odd lines
-template <int R, bool FULL>
+template <int R, bool FULL, typename S>
 __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
   extern __shared__ double2 lds[];
   const int ntile = (mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1) / T;
@@ -1341,7 +1356,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
   const int f0 = tile * T;
   const long pl = (long)(kv - 1) * mg.PLANE;
   const double2 z2 = make_double2(0.0, 0.0);
-  const double2 *gg = mg.gg + pl, *gd = mg.gd + pl;
+  const S *gg = (const S *)mg.gg + pl, *gd = (const S *)mg.gd + pl;
+  S *ggW = (S *)mg.gg + pl;
+  const S *rsuS = (const S *)mg.rsu + pl;
   double asj[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
@@ -1352,7 +1369,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
   double2 so[R];
   LineLU<R> co;
   if (ook) {
-    load_rows<R, FULL>(ROW(mg.rsu + pl, bL + o), j0, n2m, so);
+    load_rows<R, FULL>(ROW(rsuS, bL + o), j0, n2m, so);
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
   const int e = f0 + 2 * w;
@@ -1361,25 +1378,25 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
     const double aw = mg.aw[i], ae = mg.ae[i];
     double2 rr[R];
     LineLU<R> cl;
-    load_rows<R, FULL>(ROW(mg.rsu + pl, i), j0, n2m, rr);
+    load_rows<R, FULL>(ROW(rsuS, i), j0, n2m, rr);
     load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
     // line e-1 is the destination of coarse row c-1, line e+1 of row c
     const int c = cb + e / 2;
-    const double2 *dm = ROW(gd, i - 1), *dp = ROW(gd, i + 1);
-    const double2 *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
+    const S *dm = ROW(gd, i - 1), *dp = ROW(gd, i + 1);
+    const S *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
     const bool hm = e > 0, last = c == ce;
     const double pm1 = hm ? mg.coi1[c - 1] : 0.0, pm2 = hm ? mg.coi2[c - 1] : 0.0;
     const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
-      const double2 vm = hm ? cadd(cadd(dm[jc], rmul(pm1, cm[jc])), rmul(pm2, c0[jc])) : z2;
-      double2 vp = cadd(dp[jc], rmul(pp1, c0[jc]));
-      if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
+      const double2 vm = hm ? cadd(cadd(ld2(dm + jc), rmul(pm1, ld2(cm + jc))), rmul(pm2, ld2(c0 + jc))) : z2;
+      double2 vp = cadd(ld2(dp + jc), rmul(pp1, ld2(c0 + jc)));
+      if (!last) vp = cadd(vp, rmul(pp2, ld2(c1 + jc)));
       rr[r] = csub(rr[r], rmul(1.0, cadd(rmul(aw, vm), rmul(ae, vp))));
     }
     tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
-    if (w < T / 2) store_rows<R, FULL>(ROW(mg.gg + pl, i), j0, n2m, x);
+    if (w < T / 2) store_rows<R, FULL>(ROW(ggW, i), j0, n2m, x);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
@@ -1394,7 +1411,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
 #pragma unroll
     for (int r = 0; r < R; ++r) rr[r] = csub(csub(so[r], rmul(aw, TILE_AT(w, r))), rmul(ae, TILE_AT(w + 1, r)));
     tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
-    store_rows<R, FULL>(ROW(mg.gg + pl, bL + o), j0, n2m, x);
+    store_rows<R, FULL>(ROW(ggW, bL + o), j0, n2m, x);
   }
 }
 
@@ -1402,13 +1419,13 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
 // planes `pcr`: rhs - oldv*(aw*P(e-1) + ae*P(e+1)), P = pc plus, for id = 1,
 // the prolongation of level nlev-1 added as it is read; the line's rhs rows
 // stay in `rh` for the residual
-template <int R, bool FULL>
+template <int R, bool FULL, typename S>
 __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, const double2 *pcr, int e,
                                  double oldv, int id, int j0, int n2m, int lane, const double (&asj)[R],
                                  double2 (&rh)[R], double2 (&x)[R]) {
   const int i = 1 + e;
   const double2 z2 = make_double2(0.0, 0.0);
-  const double2 *gg = mg.gg + pl;
+  const S *gg = (const S *)mg.gg + pl;
   const int cb = mg.iimg[mg.nlev - 1][1], ce = mg.iimg[mg.nlev - 1][2];
   const int c = cb + e / 2;  // line e-1 <- coarse row c-1, line e+1 <- row c
   const bool hm = e > 0, pro = id == 1, last = c == ce;
@@ -1416,7 +1433,7 @@ __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, 
   const double pm1 = hm ? mg.coi1[c - 1] : 0.0, pm2 = hm ? mg.coi2[c - 1] : 0.0;
   const double pp1 = mg.coi1[c], pp2 = mg.coi2[c];
   const double2 *pm = ROW(pcr, i - 1), *pp = ROW(pcr, i + 1);
-  const double2 *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
+  const S *cm = ROW(gg, c - 1), *c0 = ROW(gg, c), *c1 = ROW(gg, c < ce ? c + 1 : c);
   LineLU<R> cl;
   load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, rh);
   load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
@@ -1426,9 +1443,9 @@ __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, 
     const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
     double2 vm = hm ? pm[jc] : z2, vp = pp[jc];
     if (pro) {
-      if (hm) vm = cadd(cadd(vm, rmul(pm1, cm[jc])), rmul(pm2, c0[jc]));
-      vp = cadd(vp, rmul(pp1, c0[jc]));
-      if (!last) vp = cadd(vp, rmul(pp2, c1[jc]));
+      if (hm) vm = cadd(cadd(vm, rmul(pm1, ld2(cm + jc))), rmul(pm2, ld2(c0 + jc)));
+      vp = cadd(vp, rmul(pp1, ld2(c0 + jc)));
+      if (!last) vp = cadd(vp, rmul(pp2, ld2(c1 + jc)));
     }
     rr[r] = csub(rh[r], rmul(oldv, cadd(rmul(aw, vm), rmul(ae, vp))));
   }
@@ -1444,7 +1461,7 @@ __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, 
 // old pc and write the new one into the other buffer (mg.pcsel per
 // wavenumber says which holds the current pc; k_mg_check flips it), because
 // a tile's halo lines read pc rows its neighbours write.
-template <int R, bool FULL>
+template <int R, bool FULL, typename S>
 __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
   extern __shared__ double2 lds[];
   __shared__ double wmax[kTileMax / 2 + 3];
@@ -1487,10 +1504,10 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
     load_lu<R, FULL>(mg, pl, 1 + o, j0, n2m, co);
   }
   if (eok) {
-    top_first<R, FULL>(g, mg, pl, ppl, pcr, e, oldv, id, j0, n2m, lane, asj, sa, x);
+    top_first<R, FULL, S>(g, mg, pl, ppl, pcr, e, oldv, id, j0, n2m, lane, asj, sa, x);
   } else if (w == T / 2 + 2 && kv == 1) {
     double2 rh[R];
-    top_first<R, FULL>(g, mg, pl, ppl, pcr, 0, oldv, id, j0, n2m, lane, asj, rh, x);
+    top_first<R, FULL, S>(g, mg, pl, ppl, pcr, 0, oldv, id, j0, n2m, lane, asj, rh, x);
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (j0 + r == n2m - 1) tts = x[r];
@@ -1540,7 +1557,8 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
       acA[r] = ac_of(wea, asj[r], anj[r], ak);
       acB[r] = ac_of(web, asj[r], anj[r], ak);
     }
-    double2 *rc = ROW(mg.resd + pl, ic), *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
+    S *rc = ROW((S *)mg.resd + pl, ic);
+    double2 *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = j0 + r;
@@ -1558,7 +1576,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
       xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
       xb = csub(xb, rmul(anj[r], TILE_UP(qa + 1, r)));
       xb = csub(xb, rmul(acB[r], gb));
-      rc[j] = cadd(rmul(c1, xa), rmul(c2, xb));
+      st2(rc + j, cadd(rmul(c1, xa), rmul(c2, xb)));
       pa[j] = ga;
       pb[j] = gb;
       ar = fmax(ar, fmax(hypot(xa.x, xa.y), hypot(xb.x, xb.y)));
@@ -1581,48 +1599,51 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
 typedef void (*FusedLevelKernel)(Geo, MG, int, int);
 typedef void (*FusedDownKernel)(Geo, MG, int, int, double, int);
 
-template <bool F>
+template <bool F, typename S>
 static FusedDownKernel pick_down_rf(int R) {
   switch (R) {
-    case 1: return k_mg_down<1, F>;
-    case 2: return k_mg_down<2, F>;
-    case 4: return k_mg_down<4, F>;
-    case 8: return k_mg_down<8, F>;
-    default: return k_mg_down<16, F>;
+    case 1: return k_mg_down<1, F, S>;
+    case 2: return k_mg_down<2, F, S>;
+    case 4: return k_mg_down<4, F, S>;
+    case 8: return k_mg_down<8, F, S>;
+    default: return k_mg_down<16, F, S>;
   }
 }
 
-static FusedDownKernel pick_down(int R, bool full) {
-  return full ? pick_down_rf<true>(R) : pick_down_rf<false>(R);
+static FusedDownKernel pick_down(int R, bool full, bool f32 = false) {
+  if (f32) return full ? pick_down_rf<true, float2>(R) : pick_down_rf<false, float2>(R);
+  return full ? pick_down_rf<true, double2>(R) : pick_down_rf<false, double2>(R);
 }
-template <bool F>
+template <bool F, typename S>
 static FusedLevelKernel pick_up_rf(int R) {
   switch (R) {
-    case 1: return k_mg_up<1, F>;
-    case 2: return k_mg_up<2, F>;
-    case 4: return k_mg_up<4, F>;
-    case 8: return k_mg_up<8, F>;
-    default: return k_mg_up<16, F>;
+    case 1: return k_mg_up<1, F, S>;
+    case 2: return k_mg_up<2, F, S>;
+    case 4: return k_mg_up<4, F, S>;
+    case 8: return k_mg_up<8, F, S>;
+    default: return k_mg_up<16, F, S>;
   }
 }
 
-static FusedLevelKernel pick_up(int R, bool full) {
-  return full ? pick_up_rf<true>(R) : pick_up_rf<false>(R);
+static FusedLevelKernel pick_up(int R, bool full, bool f32 = false) {
+  if (f32) return full ? pick_up_rf<true, float2>(R) : pick_up_rf<false, float2>(R);
+  return full ? pick_up_rf<true, double2>(R) : pick_up_rf<false, double2>(R);
 }
 
 typedef void (*FusedTopKernel)(Geo, MG, int, double, int);
-template <bool F>
+template <bool F, typename S>
 static FusedTopKernel pick_top_rf(int R) {
   switch (R) {
-    case 1: return k_mg_top<1, F>;
-    case 2: return k_mg_top<2, F>;
-    case 4: return k_mg_top<4, F>;
-    case 8: return k_mg_top<8, F>;
-    default: return k_mg_top<16, F>;
+    case 1: return k_mg_top<1, F, S>;
+    case 2: return k_mg_top<2, F, S>;
+    case 4: return k_mg_top<4, F, S>;
+    case 8: return k_mg_top<8, F, S>;
+    default: return k_mg_top<16, F, S>;
   }
 }
-static FusedTopKernel pick_top(int R, bool full) {
-  return full ? pick_top_rf<true>(R) : pick_top_rf<false>(R);
+static FusedTopKernel pick_top(int R, bool full, bool f32 = false) {
+  if (f32) return full ? pick_top_rf<true, float2>(R) : pick_top_rf<false, float2>(R);
+  return full ? pick_top_rf<true, double2>(R) : pick_top_rf<false, double2>(R);
 }
 
 // LDS of the fused kernels for tile T: T+3 line slots (down, top2) or
@@ -1639,7 +1660,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, dou
   // unique bytes per point of the level: reads resd 16, bet 8; writes the
   // second-half solution 16 on half the lines (8), the residual 16 and the
   // restricted row 16 per pair (8)
-  LW_LAUNCH("k_mg_down", 0, 56.0 * g.n2m * n, pick_down(R, full_rows(g)),
+  LW_LAUNCH("k_mg_down", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_down(R, full_rows(g), mg.f32),
             xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T, test, cyc);
 }
 
@@ -1648,7 +1669,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   // unique bytes per point of the level: reads residual 16, bet 8, the
   // down-sweep row on half the lines (8) and the coarse row per pair (8);
   // writes the solution 16
-  LW_LAUNCH("k_mg_up", 0, 56.0 * g.n2m * n, pick_up(R, full_rows(g)), xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 1),
+  LW_LAUNCH("k_mg_up", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_up(R, full_rows(g), mg.f32), xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 1),
             fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
@@ -1657,8 +1678,9 @@ static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double
   // unique bytes per top point: reads rhs 16, bet 8, the old pc on half the
   // lines (8) and for id = 1 the coarse row per pair (8); writes the new pc
   // 16 and the restricted row per pair (8)
-  const double per_pt = id == 1 ? 64.0 : 56.0;
-  LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g)),
+  // (the coarse rows take 4 and 4 in the mixed-precision form)
+  const double per_pt = (id == 1 ? 64.0 : 56.0) - (mg.f32 ? (id == 1 ? 8.0 : 4.0) : 0.0);
+  LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g), mg.f32),
             xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T, oldv, id);
 }
 
@@ -1746,16 +1768,32 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   if (R > 8) mg.fused = 0;
   if (mg.fused) {
     const int big = (int)fused_lds(R, kTileMax, false);
-    hipError_t e = hipFuncSetAttribute((const void *)pick_down(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void *)pick_top(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize, big);
+    hipError_t e = hipSuccess;
+    for (int f32 = 0; f32 <= 1 && e == hipSuccess; ++f32) {
+      e = hipFuncSetAttribute((const void *)pick_down(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              big);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                big);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)pick_top(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                big);
+    }
     if (e != hipSuccess) return e;
   }
   if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
-  return hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             kCoarseLdsBytes);
+  hipError_t e = hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), false),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), true),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+}
+
+// the mixed-precision form needs every kernel that touches the level planes
+// to read them as FP32: the fused level kernels and the LDS-resident coarse
+// kernel (or no coarse kernel)
+bool mg_mixed_supported(const Geo &g, const MG &mg) {
+  return mg.fused && (mg.lc < 0 || (mg.coarse_lds && coarse_lds_bytes(g, mg) > 0));
 }
 
 }  // namespace lw

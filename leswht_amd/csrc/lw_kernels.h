@@ -127,6 +127,7 @@ void launch_trace(hipStream_t s, const Geo &g, const Fld &f, const int *pts, int
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f);
 hipError_t prepare_mg_kernels(const Geo &g, MG &mg);
+bool mg_mixed_supported(const Geo &g, const MG &mg);
 void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
 // toplevel (poiss_z_ft.f90:302-380), id 0 or 1, ending with the per-wavenumber
 // convergence test against `test`

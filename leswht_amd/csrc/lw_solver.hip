@@ -493,6 +493,8 @@ static void setup_mgrd(lw_handle *h, const lw_grid *gr) {
   m.part = h->dalloc<double>(4 * kRedBlocks);
 }
 
+static void set_mg_fp32(lw_handle *h, bool on);
+
 static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   const lw_settings &st = h->st;
   Geo &g = h->g;
@@ -1010,6 +1012,8 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
     h->gamma[1] = 8. / 15.; h->gamma[2] = 5. / 12.; h->gamma[3] = 3. / 4.;
     h->ro[1] = 0.; h->ro[2] = -17. / 60.; h->ro[3] = -5. / 12.;
     h->dt = s->dt_size;
+    if (const char *e = getenv("LW_MG_FP32"))
+      if (atoi(e) != 0) set_mg_fp32(h, true);
     HIPCHK(hipStreamSynchronize(h->s));
     *out = h;
     return LW_OK;
@@ -1193,6 +1197,28 @@ int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n) {
 // ---------------------------------------------------------------------------
 // the step
 // ---------------------------------------------------------------------------
+// the mixed-precision form: the multigrid level planes below the top level
+// hold FP32 values (the coarse-grid corrections), while the top level's
+// relaxation, residual, convergence test and solution stay FP64, so every
+// V-cycle is a defect correction in FP64 with an FP32 coarse-grid
+// correction. Switching clears the planes: the other form's bit patterns are
+// not values of this one.
+static void set_mg_fp32(lw_handle *h, bool on) {
+  MG &mg = h->mg;
+  if (on && (h->st.poisson != 0 || !mg_mixed_supported(h->g, mg))) {
+    g_err = "mg_fp32: needs the z-DFT multigrid path with its fused kernels and LDS-resident coarse levels";
+    throw LW_ERR_ARG;
+  }
+  if ((int)on == mg.f32) return;
+  const size_t b = sizeof(double2) * (size_t)mg.n3mh * mg.PLANE;
+  HIPCHK(hipMemsetAsync(mg.gg, 0, b, h->s));
+  HIPCHK(hipMemsetAsync(mg.resd, 0, b, h->s));
+  if (mg.gd) HIPCHK(hipMemsetAsync(mg.gd, 0, b, h->s));
+  if (mg.rsu) HIPCHK(hipMemsetAsync(mg.rsu, 0, b, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  mg.f32 = on ? 1 : 0;
+}
+
 static void stepinit(lw_handle *h) {
   const lw_settings &st = h->st;
   h->ntime += 1;
@@ -1782,6 +1808,14 @@ extern "C" int lw_set_option(lw_handle *h, const char *name, int64_t value) {
   const std::string n(name);
   if (n == "mg_split") {
     h->mg_split = value != 0;
+    return LW_OK;
+  }
+  if (n == "mg_fp32") {
+    try {
+      set_mg_fp32(h, value != 0);
+    } catch (int code) {
+      return code;
+    }
     return LW_OK;
   }
   return fail(LW_ERR_ARG, "unknown option " + n);
