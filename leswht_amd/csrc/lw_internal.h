@@ -87,6 +87,8 @@ struct MG {
   int *cyc;                // n3mh: V-cycles completed
   int *pcsel;              // n3mh: 1 when pc2 holds the current pc
   double *rfin;            // n3mh: the max norm of the toplevel that ended each wavenumber's solve
+  unsigned *kbar;          // k_mg_kv1: the grid barrier's arrival counter
+  int *kerr;               // k_mg_kv1: set when a barrier wait ran out
   double *twr, *twi;       // z-DFT twiddles cos/sin(-2 pi m / n3m), m = 0..n3m-1
 };
 

@@ -744,9 +744,9 @@ __device__ inline void load_lu(const MG &mg, long pl, int i, int j0, int n2m, Li
 }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
+__device__ __forceinline__ void mg_coarse_lds_body(const Geo &g, const MG &mg, int kvi) {
   extern __shared__ double2 lds[];
-  const int kv = mg.kv0 + blockIdx.x;
+  const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
   if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
   // wavefront y takes level-0 line w (and level-1 lines 2w, 2w+1): the even
@@ -977,6 +977,11 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, M
   }
 }
 
+template <int R, bool FULL, typename S>
+__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
+  mg_coarse_lds_body<R, FULL, S>(g, mg, blockIdx.x);
+}
+
 typedef void (*CoarseLdsKernel)(Geo, MG);
 
 template <bool F, typename S>
@@ -1183,7 +1188,10 @@ static inline dim3 xcd_grid(int ntile, int nkv) {
 
 // tile size of the fused kernels on a level of n lines
 constexpr int kTileMax = 8;
-static inline int fused_tile(int n, int tmax) {
+// most wavefronts a workgroup of a multigrid kernel has (k_mg_kv1: the
+// level-0 lines of k_mg_coarse_lds)
+constexpr int kWaveMax = 16;
+__host__ __device__ inline int fused_tile(int n, int tmax) {
   for (int t = tmax; t > 2; t >>= 1)
     if (n % t == 0) return t;
   return 2;
@@ -1201,12 +1209,11 @@ static inline int fused_tile(int n, int tmax) {
 // stages write the tile's lines f0-2 .. f0+T to LDS slots 0 .. T+2 (lines
 // outside the level as zeros); then residual and restriction per pair.
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
-                                                                      int cyc) {
+__device__ __forceinline__ void mg_down_body(const Geo &g, const MG &mg, int ilev, int T, double test, int cyc, int bid) {
   extern __shared__ double2 lds[];
   const int ntile = (mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1) / T;
   int kvi, tile;
-  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  xcd_tile(bid, ntile, kvi, tile);
   if (tile >= ntile) return;
   const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
@@ -1335,15 +1342,21 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_wave
   }
 }
 
+template <int R, bool FULL, typename S>
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
+                                                                      int cyc) {
+  mg_down_body<R, FULL, S>(g, mg, ilev, T, test, cyc, blockIdx.x);
+}
+
 // goup(ilev-1) + relax(ilev, oldv 1) on a tile: first half on lines f0+2w
 // (w <= T/2, the last one a halo) with its neighbours prolonged as read,
 // then the second half on the tile's odd lines
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
+__device__ __forceinline__ void mg_up_body(const Geo &g, const MG &mg, int ilev, int T, int bid) {
   extern __shared__ double2 lds[];
   const int ntile = (mg.iimg[ilev][2] - mg.iimg[ilev][1] + 1) / T;
   int kvi, tile;
-  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  xcd_tile(bid, ntile, kvi, tile);
   if (tile >= ntile) return;
   const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
@@ -1372,8 +1385,9 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
     load_rows<R, FULL>(ROW(rsuS, bL + o), j0, n2m, so);
     load_lu<R, FULL>(mg, pl, bL + o, j0, n2m, co);
   }
+  // wavefronts past T/2 (a launch with more of them) take no line
   const int e = f0 + 2 * w;
-  if (e < nL) {
+  if (w <= T / 2 && e < nL) {
     const int i = bL + e;
     const double aw = mg.aw[i], ae = mg.ae[i];
     double2 rr[R];
@@ -1401,8 +1415,10 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = z2;
   }
+  if (w <= T / 2) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) TILE_AT(w, r) = x[r];
+    for (int r = 0; r < R; ++r) TILE_AT(w, r) = x[r];
+  }
   __syncthreads();
 
   if (ook) {
@@ -1413,6 +1429,11 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg,
     tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
     store_rows<R, FULL>(ROW(ggW, bL + o), j0, n2m, x);
   }
+}
+
+template <int R, bool FULL, typename S>
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
+  mg_up_body<R, FULL, S>(g, mg, ilev, T, blockIdx.x);
 }
 
 // first half of toplevel for line e (0-based; row 1 + e) reading the pc
@@ -1462,13 +1483,13 @@ __device__ inline void top_first(const Geo &g, const MG &mg, long pl, long ppl, 
 // wavenumber says which holds the current pc; k_mg_check flips it), because
 // a tile's halo lines read pc rows its neighbours write.
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
+__device__ __forceinline__ void mg_top_body(const Geo &g, const MG &mg, int T, double oldv, int id, int bid) {
   extern __shared__ double2 lds[];
-  __shared__ double wmax[kTileMax / 2 + 3];
+  __shared__ double wmax[kWaveMax];
   __shared__ double2 tts;
   const int ntile = g.n1m / T;
   int kvi, tile;
-  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  xcd_tile(bid, ntile, kvi, tile);
   if (tile >= ntile) return;
   const int kv = mg.kv0 + kvi;
   if (mg.done[kv - 1]) return;
@@ -1592,9 +1613,122 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   }
 }
 
+template <int R, bool FULL, typename S>
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
+  mg_top_body<R, FULL, S>(g, mg, T, oldv, id, blockIdx.x);
+}
+
 #undef TILE_AT
 #undef TILE_DN
 #undef TILE_UP
+
+// ---------------------------------------------------------------------------
+// Every V-cycle of kv = 1 in one launch (mg.kv0 = 1, mg.nkv = 1). One
+// wavenumber cannot fill the chip, and as one launch per level its cycle
+// time is set by the gaps between launches rather than by its work. Here a
+// grid of co-resident workgroups runs the stages of each cycle in the order
+// of launch_mg_cycle (down-sweeps, the LDS coarse levels, up-sweeps,
+// toplevel), every stage the device code of its own kernel (so the same
+// values), with a grid-wide barrier in place of each kernel boundary. The
+// loop ends when the convergence test folded into the first down-stage has
+// set the wavenumber's flag (every workgroup reads the flag after that
+// stage's barrier, so all leave together), or after ncyc cycles, when
+// workgroup 0 runs k_mg_check's test on the last toplevel. Every barrier
+// wait is bounded: a workgroup that waits longer than tlim ticks of the
+// constant-rate clock, or sees another's failure, sets *mg.kerr and leaves,
+// so the launch always ends; the host turns the flag into an error.
+// ---------------------------------------------------------------------------
+__device__ inline bool grid_barrier(const MG &mg, unsigned target, long long tlim) {
+  __shared__ int ok;
+  // the workgroup barrier does not wait for global stores and atomics:
+  // every wavefront waits for its own first (vmcnt(0))
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    // release at device scope: the workgroup's writes are visible to every
+    // XCD before its arrival counts
+    __hip_atomic_fetch_add(mg.kbar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int good = 1;
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(mg.kbar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(mg.kerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || wall_clock64() - t0 > tlim) {
+        good = 0;
+        __hip_atomic_store(mg.kerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    // acquire at device scope: the other workgroups' writes become visible
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__device__ inline int nrow_of(const MG &mg, int l) { return mg.iimg[l][2] - mg.iimg[l][1] + 1; }
+__device__ inline int xcd_blocks(int ntile) { return kXcds * ((ntile + kXcds - 1) / kXcds); }
+
+template <int R, bool FULL, typename S>
+__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_kv1(Geo g, MG mg, double test, int ncyc, long long tlim) {
+  const int nb = gridDim.x;
+  unsigned target = 0;
+  bool alive = true, conv = false;
+  for (int c = 0; c < ncyc && alive && !conv; ++c) {
+    // down-sweeps; the first runs the test of the toplevel before it
+    for (int ilev = mg.nlev - 1; ilev > mg.lc && alive; --ilev) {
+      const int T = fused_tile(nrow_of(mg, ilev), mg.tile[0]), nblk = xcd_blocks(nrow_of(mg, ilev) / T);
+      for (int b = blockIdx.x; b < nblk; b += nb) {
+        __syncthreads();  // the previous tile's last LDS reads are done
+        mg_down_body<R, FULL, S>(g, mg, ilev, T, test, ilev == mg.nlev - 1 ? c : -1, b);
+      }
+      alive = grid_barrier(mg, (++target) * nb, tlim);
+      if (alive && ilev == mg.nlev - 1 && __hip_atomic_load(&mg.done[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        conv = true;
+        break;
+      }
+    }
+    if (!alive || conv) break;
+    if (blockIdx.x == 0) mg_coarse_lds_body<R, FULL, S>(g, mg, 0);
+    alive = grid_barrier(mg, (++target) * nb, tlim);
+    for (int ilev = mg.lc + 1; ilev <= mg.nlev - 1 && alive; ++ilev) {
+      const int T = fused_tile(nrow_of(mg, ilev), mg.tile[1]), nblk = xcd_blocks(nrow_of(mg, ilev) / T);
+      for (int b = blockIdx.x; b < nblk; b += nb) {
+        __syncthreads();
+        mg_up_body<R, FULL, S>(g, mg, ilev, T, b);
+      }
+      alive = grid_barrier(mg, (++target) * nb, tlim);
+    }
+    if (!alive) break;
+    {
+      const int T = fused_tile(g.n1m, mg.tile[2]), nblk = xcd_blocks(g.n1m / T);
+      for (int b = blockIdx.x; b < nblk; b += nb) {
+        __syncthreads();
+        mg_top_body<R, FULL, S>(g, mg, T, 1.0, 1, b);
+      }
+      alive = grid_barrier(mg, (++target) * nb, tlim);
+    }
+  }
+  // ncyc cycles without the flag: the test of the last toplevel (k_mg_check
+  // with id 1 and the pc buffers flipped)
+  if (alive && !conv && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    if (!mg.done[0]) {
+      const int c = mg.cyc[0] + 1;
+      mg.cyc[0] = c;
+      mg.pcsel[0] ^= 1;
+      const double r = __longlong_as_double((long long)mg.sumres[0]);
+      if (r < test) mg.done[0] = 1;
+      else if (c >= mg.mgitr) mg.done[0] = 2;
+      if (mg.done[0]) mg.rfin[0] = r;
+    }
+    mg.sumres[0] = 0ull;
+  }
+}
+
+__global__ void k_mg_kv1_reset(MG mg) {
+  *mg.kbar = 0u;
+  *mg.kerr = 0;
+}
 
 typedef void (*FusedLevelKernel)(Geo, MG, int, int);
 typedef void (*FusedDownKernel)(Geo, MG, int, int, double, int);
@@ -1704,6 +1838,53 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
 // kernel (which clears the max norms for the next toplevel)
 bool mg_folds_check(const MG &mg) { return mg.fused && mg.lc >= 0 && mg.nlev - 1 > mg.lc; }
 
+typedef void (*Kv1Kernel)(Geo, MG, double, int, long long);
+template <bool F, typename S>
+static Kv1Kernel pick_kv1_rf(int R) {
+  switch (R) {
+    case 1: return k_mg_kv1<1, F, S>;
+    case 2: return k_mg_kv1<2, F, S>;
+    default: return k_mg_kv1<4, F, S>;  // the LDS coarse levels need R <= 4
+  }
+}
+static Kv1Kernel pick_kv1(int R, bool full, bool f32) {
+  if (f32) return full ? pick_kv1_rf<true, float2>(R) : pick_kv1_rf<false, float2>(R);
+  return full ? pick_kv1_rf<true, double2>(R) : pick_kv1_rf<false, double2>(R);
+}
+
+// k_mg_kv1's workgroup is the level-0 lines' wavefronts of the LDS coarse
+// kernel; every tile kernel's stages must fit in that many
+static int kv1_waves(const MG &a) { return a.iimg[0][2] - a.iimg[0][1] + 1; }
+
+bool mg_kv1_supported(const Geo &g, const MG &a) {
+  if (!a.fused || !a.coarse_lds || a.lc != 1 || a.kv0 != 1 || a.nkv != 1 || !mg_folds_check(a)) return false;
+  if (lanes_r(g.n2m) > 4 || coarse_lds_bytes(g, a) <= 0) return false;
+  int need = fused_tile(g.n1m, a.tile[2]) / 2 + 3;
+  for (int l = a.lc + 1; l <= a.nlev - 1; ++l) {
+    const int n = a.iimg[l][2] - a.iimg[l][1] + 1;
+    need = std::max(need, std::max(fused_tile(n, a.tile[0]) / 2 + 2, fused_tile(n, a.tile[1]) / 2 + 1));
+  }
+  const int nw = kv1_waves(a);
+  return nw >= need && nw <= kCoarseLdsWaves;
+}
+
+void launch_mg_kv1(hipStream_t s, const Geo &g, const MG &a, double test, int ncyc, long long tlim) {
+  const int R = lanes_r(g.n2m);
+  // dynamic LDS: the largest stage's; grid: the most tiles a stage has
+  unsigned lds = (unsigned)coarse_lds_bytes(g, a);
+  int tiles = g.n1m / fused_tile(g.n1m, a.tile[2]);
+  lds = std::max(lds, fused_lds(R, fused_tile(g.n1m, a.tile[2]), false));
+  for (int l = a.lc + 1; l <= a.nlev - 1; ++l) {
+    const int n = a.iimg[l][2] - a.iimg[l][1] + 1, td = fused_tile(n, a.tile[0]), tu = fused_tile(n, a.tile[1]);
+    lds = std::max(lds, std::max(fused_lds(R, td, false), fused_lds(R, tu, true)));
+    tiles = std::max(tiles, std::max(n / td, n / tu));
+  }
+  const unsigned grid = (unsigned)(kXcds * ((tiles + kXcds - 1) / kXcds));
+  LW_LAUNCH("k_mg_kv1_reset", 0, 0, k_mg_kv1_reset, dim3(1), dim3(1), 0, s, a);
+  LW_LAUNCH("k_mg_kv1", 0, 0, pick_kv1(R, full_rows(g), a.f32 != 0), dim3(grid), dim3(64, kv1_waves(a)), lds, s, g, a,
+            test, ncyc, tlim);
+}
+
 // one V-cycle. cyc >= 0: the first launch runs the convergence test of the
 // preceding toplevel, after which an active wavenumber has completed cyc
 // cycles (mg_folds_check); cyc < 0: a k_mg_check already ran it. check: end
@@ -1784,9 +1965,20 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
   hipError_t e = hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), false),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), true),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), true),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+  if (R > 4) return e;
+  // k_mg_kv1 also holds a few static LDS words (the top stage's norms, the
+  // barrier's flag): the dynamic part may take the rest
+  for (int f32 = 0; f32 < 2 && e == hipSuccess; ++f32) {
+    const void *fn = (const void *)pick_kv1(R, full_rows(g), f32 != 0);
+    hipFuncAttributes at{};
+    e = hipFuncGetAttributes(&at, fn);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes - (int)at.sharedSizeBytes);
+  }
+  return e;
 }
 
 // the mixed-precision form needs every kernel that touches the level planes

@@ -133,6 +133,12 @@ void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
 // convergence test against `test`
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test, bool check);
 bool mg_folds_check(const MG &mg);
+// kv = 1 of the two-stream solve in one launch (k_mg_kv1): whether the
+// tiles and levels of `mg` (kv0 = 1, nkv = 1) allow it, and the launch of up
+// to ncyc V-cycles after the first toplevel, each barrier wait bounded by
+// tlim ticks of the constant-rate clock
+bool mg_kv1_supported(const Geo &g, const MG &mg);
+void launch_mg_kv1(hipStream_t s, const Geo &g, const MG &mg, double test, int ncyc, long long tlim);
 // one V-cycle of every wavenumber still iterating (poiss_z_ft.f90:258-299)
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int cyc, bool check);
 

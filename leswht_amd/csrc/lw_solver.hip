@@ -188,6 +188,13 @@ struct lw_handle {
   int *h_flags2 = nullptr;                 // the same, read on s2
   int mg_hint = 4;                         // V-cycles of the previous Poisson solve
   int mg_hint1 = 4;                        // of its kv = 1 alone (two-stream solve)
+  // kv = 1 of the two-stream solve as one launch of k_mg_kv1 (LW_MG_KV1=1
+  // or option "mg_kv1" 1; off by default: on MI355X its stages and grid
+  // barriers take longer than the launches per level they replace), its
+  // tiles (LW_MG_KV1_TILES) and the bound on each barrier wait in clock ticks
+  int mg_kv1 = 0;
+  int kv1_tiles[3] = {8, 8, 8};
+  long long kv1_tlim = 0;
   int mg_split = 1;                        // kv = 1 on its own stream (LW_MG_SPLIT=0: one stream)
   int sgs_fused = 1;                       // dynamic SGS in one pass (LW_SGS_FUSED=0: separate kernels)
   DiagPlanes dplanes{};
@@ -222,7 +229,7 @@ struct lw_handle {
   size_t arena_cap = 0, arena_used = 0;
   int arena_dev = -1;
   unsigned long long hs_sumres[256] = {0};
-  int hs_flags[256] = {0}, hs_flags2[256] = {0};
+  int hs_flags[256] = {0}, hs_flags2[512] = {0};
   double hs_red[64] = {0};
   // field_avg state (misc_hist.f90:287-527; mod_common avg_started,
   // npriavg_count, timeinit, ihistinit); avg_dir empty: no fav files
@@ -874,9 +881,11 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     mg.pc2 = mg.pc;
   }
   mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);
-  mg.done = h->dalloc<int>(384);
+  mg.done = h->dalloc<int>(392);
   mg.cyc = mg.done + 128;
   mg.pcsel = mg.done + 256;
+  mg.kbar = (unsigned *)(mg.done + 384);
+  mg.kerr = mg.done + 385;
   mg.rfin = h->dalloc<double>(128);
   std::vector<double> twr(n3m), twi(n3m);
   const double pi2 = 8.0 * atan(1.0);
@@ -990,6 +999,20 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
     // the two-stream solve polls copies into pinned memory; a static handle
     // has none, so it solves on one stream (the same kernels and results)
     if (h->arena) h->mg_split = 0;
+    if (const char *e = getenv("LW_MG_KV1")) h->mg_kv1 = atoi(e) != 0;
+    if (const char *e = getenv("LW_MG_KV1_TILES")) {
+      int v[3] = {0, 0, 0};
+      if (sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3)
+        for (int q = 0; q < 3; ++q)
+          if (v[q] == 2 || v[q] == 4 || v[q] == 8) h->kv1_tiles[q] = v[q];
+    }
+    {
+      // 2 s of the constant-rate clock (its rate in kHz)
+      int dev = 0, khz = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      h->kv1_tlim = 2000LL * (long long)(khz > 0 ? khz : 100000);
+    }
     h->sgs_fused = sgs_fused_enabled() ? 1 : 0;
     for (int e = 0; e < 8; ++e) HIPCHK(hipEventCreate(&h->ev[e]));
     for (int q = 0; q < 3; ++q)
@@ -1004,7 +1027,7 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
       HIPCHK(hipHostMalloc((void **)&h->h_rfin, 4 * sizeof(double)));
       HIPCHK(hipHostMalloc((void **)&h->h_sumres, 256 * sizeof(unsigned long long)));
       HIPCHK(hipHostMalloc((void **)&h->h_flags, 256 * sizeof(int)));
-      HIPCHK(hipHostMalloc((void **)&h->h_flags2, 256 * sizeof(int)));
+      HIPCHK(hipHostMalloc((void **)&h->h_flags2, 512 * sizeof(int)));
       HIPCHK(hipHostMalloc((void **)&h->h_red, 64 * sizeof(double)));
     }
     setup(h, gr, t);
@@ -1328,6 +1351,17 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
   a.kv0 = 1;
   a.nkv = 1;
   for (int q = 0; q < 3; ++q) a.tile[q] = a.tile1[q];
+  // kv = 1 as one launch of k_mg_kv1 when its shape allows (the check must
+  // be folded: the launch starts with the test of the first toplevel)
+  bool kv1 = false;
+  if (h->mg_kv1 && fold) {
+    MG p = a;
+    for (int q = 0; q < 3; ++q) p.tile[q] = h->kv1_tiles[q];
+    if (mg_kv1_supported(g, p)) {
+      a = p;
+      kv1 = true;
+    }
+  }
   b.kv0 = 2;
   b.nkv = h->mg.n3mh - 1;
   const int itr = h->mg.mgitr;
@@ -1361,11 +1395,18 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
       const bool go = !x.pending && !x.done && x.launched < itr;
       n[k] = go ? (x.batch < itr - x.launched ? x.batch : itr - x.launched) : 0;
     }
+    // k_mg_kv1 runs every cycle kv = 1 may need in its one launch, first
+    if (kv1 && n[0] > 0) {
+      n[0] = itr - sd[0].launched;
+      prof_side(sd[0], 1);
+      launch_mg_kv1(sd[0].s, g, a, test, n[0], h->kv1_tlim);
+      HIPCHK(hipMemcpyAsync(sd[0].flags + 400, a.kerr, sizeof(int), hipMemcpyDeviceToHost, sd[0].s));
+    }
     const int nmax = n[0] > n[1] ? n[0] : n[1];
     for (int c = 0; c < nmax; ++c)
       for (int k = 0; k < 2; ++k) {
         Side &x = sd[k];
-        if (c >= n[k]) continue;
+        if (c >= n[k] || (k == 0 && kv1)) continue;
         const int m = x.launched + c + 1;
         prof_side(x, m);
         const int cyc = fold && !(c == 0 && x.launched > 0) ? m - 1 : -1;
@@ -1389,6 +1430,10 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
       if (q == hipErrorNotReady) continue;
       HIPCHK(q);
       x.pending = 0;
+      if (kv1 && x.m == &a && x.flags[400] != 0) {
+        g_err = "k_mg_kv1: a grid barrier wait ran out (the workgroups were not co-resident)";
+        throw LW_ERR_DEVICE;
+      }
       bool d = true;
       for (int i = x.m->kv0 - 1; i < x.m->kv0 - 1 + x.m->nkv; ++i) d = d && x.flags[i] != 0;
       x.done = d;
@@ -1808,6 +1853,10 @@ extern "C" int lw_set_option(lw_handle *h, const char *name, int64_t value) {
   const std::string n(name);
   if (n == "mg_split") {
     h->mg_split = value != 0;
+    return LW_OK;
+  }
+  if (n == "mg_kv1") {
+    h->mg_kv1 = value != 0;
     return LW_OK;
   }
   if (n == "mg_fp32") {
