@@ -203,6 +203,15 @@ int lw_get_trace(lw_handle *h, double *out, int64_t cap);
  * final max-norm residual times dtconst). Returns the row count; copies at
  * most cap rows into the arrays that are not NULL. */
 int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t *kk, double *rm, int64_t cap);
+/* every row the z-DFT solver appends to mgftresiduemax.dat during the last
+ * step, in the file's order: per substep, mg2d's rows (kind 0: kv <= 3, KK,
+ * RM as lw_get_mg_residuals gives them; 'KV=',i4,3x,'KK=',i4,3x,'RM=',es24.16)
+ * and then, for every kv above IMGSOR in turn, gsor2d's row of each sweep of
+ * its loop (kind 1: KK the sweep, RG its max-norm residual times dtconst
+ * times n3m/2+1; 'KV=',i3,3x,'KK=',i3,3x,'RG=',es24.16;
+ * poiss_z_ft.f90:293-297, 584). Returns the row count; copies at most cap
+ * rows into the arrays that are not NULL. */
+int64_t lw_get_residual_rows(const lw_handle *h, int64_t *kind, int64_t *kv, int64_t *kk, double *val, int64_t cap);
 /* with poisson = 1: the rows poiss_mgrd.f90 writes during the last step,
  * per BiCGSTAB iteration to poiss_itr.dat (:185-186: the substep, the
  * iteration, max |residual| * dtconst and the last cell i, j, k holding it,

@@ -1618,6 +1618,169 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
   mg_top_body<R, FULL, S>(g, mg, T, oldv, id, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// gsor2d (poiss_z_ft.f90:513-587) for the wavenumbers above IMGSOR: zebra
+// y-line over-relaxation of the top level alone, one launch per sweep, each
+// sweep ending with resid3's max norm (:590-616). The tiles are k_mg_top's:
+// wavefront w solves first-half line f0-2+2w (w <= T/2+1) and second-half
+// line f0-1+2w (w <= T/2), and for 1 <= w <= T/2 forms the residual of the
+// pair w-1 from the relaxed values in LDS. A line's relaxed value is
+// u = ww*x + (wgt*ww2)*u_old, u_old from the current pc buffer; the new
+// values go to the other buffer (the check flips pcsel). wgt is gsor2d's
+// oldv on the sweep before its loop (0: u starts at zero) and 1 in it; it
+// also scales the first half's x coupling, which the second half's never is:
+// rhs - wgt*(aw*u(i-1) + ae*u(i+1)) against rhs - (aw*u(i-1) + ae*u(i+1)).
+// ---------------------------------------------------------------------------
+template <int R, bool FULL>
+__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_sor(Geo g, MG mg, int T, double wgt, double ww) {
+  extern __shared__ double2 lds[];
+  __shared__ double wmax[kWaveMax];
+  const int ntile = g.n1m / T;
+  int kvi, tile;
+  xcd_tile(blockIdx.x, ntile, kvi, tile);
+  if (tile >= ntile) return;
+  const int kv = mg.kv0 + kvi;
+  if (mg.done[kv - 1]) return;
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y), W = blockDim.y;
+  const int n2m = FULL ? 64 * R : g.n2m, n1m = g.n1m, j0 = 1 + lane * R;
+  const int f0 = tile * T;
+  const long pl = (long)(kv - 1) * mg.PLANE, ppl = (long)(kv - 1) * mg.PCPLANE;
+  const int sel = mg.pcsel[kv - 1];
+  const double2 *pcr = (sel ? mg.pc2 : mg.pc) + ppl;
+  double2 *pcw = (sel ? mg.pc : mg.pc2) + ppl;
+  const double2 z2 = make_double2(0.0, 0.0);
+  const double cw = wgt * (1.0 - ww);
+  double asj[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) asj[r] = mg.as[(FULL || j0 + r <= n2m) ? j0 + r : n2m];
+
+  const bool pair = w >= 1 && w <= T / 2;
+  const int e = f0 - 2 + 2 * w, o = e + 1;
+  const bool eok = w <= T / 2 + 1 && e >= 0 && e < n1m;
+  const bool ook = w <= T / 2 && o >= 0;
+  double *san = (double *)(lds + (T + 3) * R * 64);
+  for (int q = lane + 64 * w; q < 64 * R; q += 64 * W) san[q] = mg.an[q + 1 <= n2m ? q + 1 : n2m];
+  const int ia = 1 + e, ib = ia + 1;
+  const int iaq = pair ? ia : 1, ibq = ook ? ib : 1;
+  const double aw2 = mg.aw[ibq], ae2 = mg.ae[ibq];
+  const double awa = mg.aw[iaq], aea = mg.ae[iaq], wea = mg.awe[iaq], web = mg.awe[ibq], ak = mg.ak3[kv];
+  double2 sa[R], sb[R], ub[R], x[R];
+  LineLU<R> co;
+  if (ook) {
+    load_rows<R, FULL>(ROW(mg.rhs + ppl, 1 + o), j0, n2m, sb);
+    load_rows<R, FULL>(ROW(pcr, 1 + o), j0, n2m, ub);
+    load_lu<R, FULL>(mg, pl, 1 + o, j0, n2m, co);
+  }
+  // first half: line i = 1 + e
+  if (eok) {
+    const int i = ia;
+    const double aw = mg.aw[i], ae = mg.ae[i];
+    const double2 *pm = ROW(pcr, i - 1), *pp = ROW(pcr, i + 1), *p0 = ROW(pcr, i);
+    LineLU<R> cl;
+    load_rows<R, FULL>(ROW(mg.rhs + ppl, i), j0, n2m, sa);
+    load_lu<R, FULL>(mg, pl, i, j0, n2m, cl);
+    double2 rr[R], u0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int jc = (FULL || j0 + r <= n2m) ? j0 + r : n2m;
+      const double2 vm = e > 0 ? pm[jc] : z2, vp = pp[jc];
+      u0[r] = p0[jc];
+      rr[r] = csub(sa[r], rmul(wgt, cadd(rmul(aw, vm), rmul(ae, vp))));
+    }
+    tri_regs<R, FULL>(rr, cl.bt, asj, cl.gm, j0, n2m, lane, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = cadd(rmul(ww, x[r]), rmul(cw, u0[r]));
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+  if (w <= T / 2 + 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w, r) = (!FULL && j0 + r > n2m) ? z2 : x[r];
+  }
+  __syncthreads();
+
+  // second half: line i = 1 + o, its neighbours relaxed above
+  if (ook) {
+    double2 rr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rr[r] = csub(sb[r], cadd(rmul(aw2, TILE_AT(2 * w, r)), rmul(ae2, TILE_AT(2 * w + 2, r))));
+    tri_regs<R, FULL>(rr, co.bt, asj, co.gm, j0, n2m, lane, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = cadd(rmul(ww, x[r]), rmul(cw, ub[r]));
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = z2;
+  }
+  if (w <= T / 2) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) TILE_AT(2 * w + 1, r) = (!FULL && j0 + r > n2m) ? z2 : x[r];
+  }
+  __syncthreads();
+
+  // resid3 of the pair (rows ia, ib), the new u and the max norm
+  double ar = 0.0;
+  if (pair) {
+    const int qa = 2 * w;
+    double acA[R], acB[R], anj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      anj[r] = san[j0 - 1 + r];
+      acA[r] = ac_of(wea, asj[r], anj[r], ak);
+      acB[r] = ac_of(web, asj[r], anj[r], ak);
+    }
+    double2 *pa = ROW(pcw, ia), *pb = ROW(pcw, ib);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = j0 + r;
+      if (!FULL && j > n2m) continue;
+      const double2 ga = TILE_AT(qa, r), gb = TILE_AT(qa + 1, r);
+      double2 xa = sa[r];
+      xa = csub(xa, rmul(awa, TILE_AT(qa - 1, r)));
+      xa = csub(xa, rmul(aea, gb));
+      xa = csub(xa, rmul(asj[r], TILE_DN(qa, r)));
+      xa = csub(xa, rmul(anj[r], TILE_UP(qa, r)));
+      xa = csub(xa, rmul(acA[r], ga));
+      double2 xb = sb[r];
+      xb = csub(xb, rmul(aw2, ga));
+      xb = csub(xb, rmul(ae2, TILE_AT(qa + 2, r)));
+      xb = csub(xb, rmul(asj[r], TILE_DN(qa + 1, r)));
+      xb = csub(xb, rmul(anj[r], TILE_UP(qa + 1, r)));
+      xb = csub(xb, rmul(acB[r], gb));
+      pa[j] = ga;
+      pb[j] = gb;
+      ar = fmax(ar, fmax(hypot(xa.x, xa.y), hypot(xb.x, xb.y)));
+    }
+  }
+  for (int s = 32; s > 0; s >>= 1) ar = fmax(ar, __shfl_down(ar, s, 64));
+  if (lane == 0) wmax[w] = ar;
+  __syncthreads();
+  if (w == 0 && lane == 0) {
+    double m = 0.0;
+    for (int q = 0; q < W; ++q) m = fmax(m, wmax[q]);
+    atomicMax(&mg.sumres[kv - 1], (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// the test after a sweep of gsor2d (poiss_z_ft.f90:551-552, 585-587): as
+// k_mg_check, and for the sweeps of the loop (id = 1) the max norm of sweep
+// c of wavenumber q is kept at log[(c-1)*ld + q] for the residual rows
+__global__ void k_sor_check(MG mg, double test, int id, double *log, int ld) {
+  for (int q = mg.kv0 - 1 + threadIdx.x; q < mg.kv0 - 1 + mg.nkv; q += blockDim.x) {
+    if (!mg.done[q]) {
+      const int c = mg.cyc[q] + id;
+      mg.cyc[q] = c;
+      mg.pcsel[q] ^= 1;
+      const double r = __longlong_as_double((long long)mg.sumres[q]);
+      if (id == 1) log[(long)(c - 1) * ld + q] = r;
+      if (r < test) mg.done[q] = 1;
+      else if (c >= mg.mgitr) mg.done[q] = 2;
+      if (mg.done[q]) mg.rfin[q] = r;
+    }
+    mg.sumres[q] = 0ull;
+  }
+}
+
 #undef TILE_AT
 #undef TILE_DN
 #undef TILE_UP
@@ -1868,6 +2031,29 @@ bool mg_kv1_supported(const Geo &g, const MG &a) {
   return nw >= need && nw <= kCoarseLdsWaves;
 }
 
+typedef void (*SorKernel)(Geo, MG, int, double, double);
+template <bool F>
+static SorKernel pick_sor_rf(int R) {
+  switch (R) {
+    case 1: return k_sor<1, F>;
+    case 2: return k_sor<2, F>;
+    case 4: return k_sor<4, F>;
+    case 8: return k_sor<8, F>;
+    default: return k_sor<16, F>;
+  }
+}
+static SorKernel pick_sor(int R, bool full) { return full ? pick_sor_rf<true>(R) : pick_sor_rf<false>(R); }
+
+void launch_sor_sweep(hipStream_t s, const Geo &g, const MG &m, double wgt, double ww, double test, int id, double *log,
+                      int ld) {
+  const int n = g.n1m, T = fused_tile(n, m.tile[2]), R = lanes_r(g.n2m);
+  // per top point: rhs 16, bet 8, the old u on every line 16 (and its
+  // neighbours on the first half, from L2) read; the new u 16 written
+  LW_LAUNCH("k_sor", 0, 56.0 * g.n2m * n, pick_sor(R, full_rows(g)), xcd_grid(n / T, m.nkv), dim3(64, T / 2 + 2),
+            fused_lds(R, T, false), s, g, m, T, wgt, ww);
+  LW_LAUNCH("k_sor_check", 0, 0, k_sor_check, dim3(1), dim3(128), 0, s, m, test, id, log, ld);
+}
+
 void launch_mg_kv1(hipStream_t s, const Geo &g, const MG &a, double test, int ncyc, long long tlim) {
   const int R = lanes_r(g.n2m);
   // dynamic LDS: the largest stage's; grid: the most tiles a stage has
@@ -1947,6 +2133,12 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   for (int q = 0; q < 3; ++q) mg.tile1[q] = mg.tile[q];
   parse("LW_MG_TILES1", mg.tile1);
   if (R > 8) mg.fused = 0;
+  if (R <= 8) {
+    // the line-SOR sweeps take the top kernel's tiles
+    hipError_t e = hipFuncSetAttribute((const void *)pick_sor(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)fused_lds(R, kTileMax, false));
+    if (e != hipSuccess) return e;
+  }
   if (mg.fused) {
     const int big = (int)fused_lds(R, kTileMax, false);
     hipError_t e = hipSuccess;

@@ -139,6 +139,11 @@ bool mg_folds_check(const MG &mg);
 // tlim ticks of the constant-rate clock
 bool mg_kv1_supported(const Geo &g, const MG &mg);
 void launch_mg_kv1(hipStream_t s, const Geo &g, const MG &mg, double test, int ncyc, long long tlim);
+// one sweep of gsor2d over the wavenumbers of `mg` (kv0.., nkv) with its
+// test: wgt 0 for the sweep before the loop (id 0), 1 in the loop (id 1,
+// which keeps each active wavenumber's max norm at log[(sweep-1)*ld + q])
+void launch_sor_sweep(hipStream_t s, const Geo &g, const MG &mg, double wgt, double ww, double test, int id, double *log,
+                      int ld);
 // one V-cycle of every wavenumber still iterating (poiss_z_ft.f90:258-299)
 void launch_mg_cycle(hipStream_t s, const Geo &g, const MG &mg, double test, int cyc, bool check);
 

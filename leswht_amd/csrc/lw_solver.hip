@@ -205,6 +205,15 @@ struct lw_handle {
   int64_t mgres_n = 0;
   int64_t mgres_kv[9] = {0}, mgres_kk[9] = {0};
   double mgres_rm[9] = {0};
+  // every row of mgftresiduemax.dat of the last step in file order: mg2d's
+  // (kind 0, RM) and gsor2d's per sweep (kind 1, RG; poiss_z_ft.f90:584)
+  std::vector<int64_t> rrow_kind, rrow_kv, rrow_kk;
+  std::vector<double> rrow_val;
+  // gsor2d for kv > IMGSOR: the max norm of every sweep of its loop on the
+  // device, [sweep][kv]; the host copy; the sweeps of the last solve
+  double *sorlog = nullptr;
+  std::vector<double> h_sorlog;
+  int sor_hint = 1;
   double *h_rfin = nullptr;  // pinned: rfin of kv = 1..3 after a solve
   double hs_rfin[4] = {0};
   // poiss_mgrd: the solver's state and the log of the last step
@@ -732,7 +741,9 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.n1md = 2 * n1m;
   mg.n3mh = n3m / 2 + 1;
   mg.kv0 = 1;
-  mg.nkv = mg.n3mh;
+  // the multigrid takes kv = 1..IMGSOR, line-SOR (gsor2d) the rest
+  // (poiss_z_ft.f90:186-190)
+  mg.nkv = (int)std::min<int64_t>(st.imgsor, mg.n3mh);
   mg.nbli = (int)st.nbli;
   mg.mgitr = (int)st.mgitr;
   mg.LDJ = ((long)(n2 + 1) + 7) / 8 * 8;
@@ -876,10 +887,12 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   if (mg.fused) {
     mg.gd = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
     mg.rsu = h->dalloc<double2>((size_t)n3mh * mg.PLANE);
-    mg.pc2 = h->dalloc<double2>((size_t)n3mh * mg.PCPLANE);
-  } else {
-    mg.pc2 = mg.pc;
   }
+  // the fused kernels and the line-SOR sweeps read one pc buffer and write
+  // the other (their tiles read rows their neighbours write)
+  const bool sor = mg.nkv < n3mh;
+  mg.pc2 = mg.fused || sor ? h->dalloc<double2>((size_t)n3mh * mg.PCPLANE) : mg.pc;
+  if (sor) h->sorlog = h->dalloc<double>((size_t)std::max(mg.mgitr, 1) * n3mh);
   mg.sumres = h->dalloc<unsigned long long>(n3mh + 4);
   mg.done = h->dalloc<int>(392);
   mg.cyc = mg.done + 128;
@@ -930,7 +943,10 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
     // depends on the order its loop visits them
     if (nz0 > 1 && nz0 % 2 == 1) bad("poiss_mgrd: n3m / 2**NLEV must be 1 or even");
   } else {
-    if (s->imgsor < g->n3m / 2 + 1) bad("IMGSOR < n3m/2+1 (line-SOR wavenumbers) is not supported");
+    // kv = 1, the singular mean mode, needs the multigrid's gauge shift
+    if (s->imgsor < 1) bad("IMGSOR must be at least 1");
+    if (s->imgsor < g->n3m / 2 + 1 && (g->n2m + 63) / 64 > 8)
+      bad("line-SOR wavenumbers (IMGSOR < n3m/2+1) need n2m <= 512");
     if (g->n3m / 2 + 1 > 128) bad("n3m must be at most 254");
   }
 }
@@ -1363,7 +1379,7 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
     }
   }
   b.kv0 = 2;
-  b.nkv = h->mg.n3mh - 1;
+  b.nkv = h->mg.nkv - 1;
   const int itr = h->mg.mgitr;
   HIPCHK(hipEventRecord(h->ev_fork, h->s));
   HIPCHK(hipStreamWaitEvent(h->s2, h->ev_fork, 0));
@@ -1449,6 +1465,62 @@ static void poisson_split(lw_handle *h, double test, bool fold) {
   HIPCHK(hipStreamSynchronize(h->s));
 }
 
+// gsor2d (poiss_z_ft.f90:513-587) for kv = IMGSOR+1..n3mh, after the
+// multigrid wavenumbers: the sweep before its loop, then batches of sweeps
+// (the first as long as the previous solve needed), each with its test on
+// the device, flags read between batches; then the sweeps' max norms for
+// the RG rows of mgftresiduemax.dat, kv by kv as the reference's loop over
+// kv writes them
+static void poisson_sor(lw_handle *h, double test) {
+  const Geo &g = h->g;
+  MG m = h->mg;
+  m.kv0 = h->mg.nkv + 1;
+  m.nkv = h->mg.n3mh - h->mg.nkv;
+  const double ww = h->st.wwsor;
+  const int itr = m.mgitr, ld = m.n3mh;
+  auto prof = [&](int sweep) {
+    if (!g_prof) return;
+    g_prof->cycle = sweep;
+    g_prof->kv0 = m.kv0;
+    g_prof->nkv = m.nkv;
+  };
+  prof(0);
+  launch_sor_sweep(h->s, g, m, 0.0, ww, test, 0, h->sorlog, ld);
+  int launched = 0, batch = h->sor_hint > 1 ? h->sor_hint : 1;
+  for (;;) {
+    const int n = std::min(batch, itr - launched);
+    for (int c = 0; c < n; ++c) {
+      prof(launched + c + 1);
+      launch_sor_sweep(h->s, g, m, 1.0, ww, test, 1, h->sorlog, ld);
+    }
+    launched += n;
+    HIPCHK(hipMemcpyAsync(h->h_flags, m.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(h->h_rfin, m.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+    bool all = true;
+    for (int q = m.kv0 - 1; q < m.kv0 - 1 + m.nkv; ++q) all = all && h->h_flags[q] != 0;
+    if (all || launched >= itr) break;
+    batch = 4;
+  }
+  int kmax = 0;
+  for (int q = m.kv0 - 1; q < m.kv0 - 1 + m.nkv; ++q) kmax = std::max(kmax, h->h_flags[128 + q]);
+  h->sor_hint = kmax;
+  if (kmax > 0) {
+    h->h_sorlog.resize((size_t)kmax * ld);
+    HIPCHK(hipMemcpyAsync(h->h_sorlog.data(), h->sorlog, h->h_sorlog.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  }
+  // RG = errmax * dtconst * float(n3mh) (poiss_z_ft.f90:584)
+  for (int q = m.kv0 - 1; q < m.kv0 - 1 + m.nkv; ++q)
+    for (int kk = 1; kk <= h->h_flags[128 + q]; ++kk) {
+      h->rrow_kind.push_back(1);
+      h->rrow_kv.push_back(q + 1);
+      h->rrow_kk.push_back(kk);
+      h->rrow_val.push_back(h->h_sorlog[(size_t)(kk - 1) * ld + q] * h->sub.dtconst * (double)m.n3mh);
+    }
+}
+
 // poisson (poiss_z_ft.f90:114-299): z-DFT, then V-cycles of every wavenumber
 // until its max-norm residual passes the test. Convergence is decided on the
 // device; the host enqueues batches of cycles (the first as long as the
@@ -1510,7 +1582,9 @@ static void phase_poisson(lw_handle *h) {
   // with the check folded, the test after a toplevel runs in the next
   // cycle's first launch, and a k_mg_check only ends each batch
   const bool fold = mg_folds_check(mg);
-  if (h->mg_split && mg.fused && mg.n3mh >= 2 && mg.mgitr > 0) {
+  const int nmg = mg.nkv;  // kv = 1..nmg by multigrid, the rest by line-SOR
+  if (g_prof) g_prof->nkv = nmg;
+  if (h->mg_split && mg.fused && nmg >= 2 && mg.mgitr > 0) {
     poisson_split(h, test, fold);
   } else {
     launch_mg_toplevel(h->s, g, mg, 0, 0.0, test, !fold);
@@ -1529,7 +1603,7 @@ static void phase_poisson(lw_handle *h) {
       HIPCHK(hipMemcpyAsync(h->h_rfin, mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
       HIPCHK(hipStreamSynchronize(h->s));
       all_done = true;
-      for (int q = 0; q < mg.n3mh; ++q) all_done = all_done && h->h_flags[q] != 0;
+      for (int q = 0; q < nmg; ++q) all_done = all_done && h->h_flags[q] != 0;
       batch = 2;
     }
     if (launched == 0) {  // every wavenumber passed on the first toplevel
@@ -1538,12 +1612,23 @@ static void phase_poisson(lw_handle *h) {
       HIPCHK(hipStreamSynchronize(h->s));
     }
   }
+  if (h->sub.msub == 1) {
+    h->rrow_kind.clear(); h->rrow_kv.clear(); h->rrow_kk.clear(); h->rrow_val.clear();
+  }
+  // mg2d's rows come first: its wavenumbers precede gsor2d's
+  for (int q = 0; q < nmg && q < 3; ++q) {
+    h->rrow_kind.push_back(0);
+    h->rrow_kv.push_back(q + 1);
+    h->rrow_kk.push_back(h->h_flags[q] == 2 ? (int64_t)mg.mgitr + 1 : h->h_flags[128 + q]);
+    h->rrow_val.push_back(h->h_rfin[q] * h->sub.dtconst);
+  }
+  if (nmg < mg.n3mh) poisson_sor(h, test);
   int cmax = 0, crest = 0;
   bool limit = false;
   for (int q = 0; q < mg.n3mh; ++q) {
     const int c = h->h_flags[128 + q];
     cmax = c > cmax ? c : cmax;
-    if (q > 0) crest = c > crest ? c : crest;
+    if (q > 0 && q < nmg) crest = c > crest ? c : crest;
     h->cyc_sum += c;
     limit = limit || h->h_flags[q] != 1;
   }
@@ -1552,7 +1637,7 @@ static void phase_poisson(lw_handle *h) {
   // cycles done, or MGITR + 1 when the limit ended the loop) and the final
   // max norm times dtconst
   if (h->sub.msub == 1) h->mgres_n = 0;
-  for (int q = 0; q < mg.n3mh && q < 3; ++q) {
+  for (int q = 0; q < nmg && q < 3; ++q) {
     if (h->mgres_n >= 9) break;
     const int64_t n = h->mgres_n++;
     h->mgres_kv[n] = q + 1;
@@ -1561,7 +1646,7 @@ static void phase_poisson(lw_handle *h) {
   }
   if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
   if (cmax > h->cyc_max) h->cyc_max = cmax;
-  if (h->mg_split && mg.fused && mg.n3mh >= 2) {
+  if (h->mg_split && mg.fused && nmg >= 2) {
     h->mg_hint1 = h->h_flags[128];
     h->mg_hint = crest;
   } else {
@@ -2127,6 +2212,20 @@ extern "C" int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t 
     if (rm) rm[q] = h->mgres_rm[q];
   }
   return h->mgres_n;
+}
+
+extern "C" int64_t lw_get_residual_rows(const lw_handle *h, int64_t *kind, int64_t *kv, int64_t *kk, double *val,
+                                        int64_t cap) {
+  if (!h) return fail(LW_ERR_ARG, "null handle");
+  const int64_t tot = (int64_t)h->rrow_kv.size();
+  const int64_t n = tot < cap ? tot : (cap > 0 ? cap : 0);
+  for (int64_t q = 0; q < n; ++q) {
+    if (kind) kind[q] = h->rrow_kind[q];
+    if (kv) kv[q] = h->rrow_kv[q];
+    if (kk) kk[q] = h->rrow_kk[q];
+    if (val) val[q] = h->rrow_val[q];
+  }
+  return tot;
 }
 
 extern "C" int64_t lw_get_bicg_log(const lw_handle *h, int64_t *sub, int64_t *icg, double *rm, int64_t *loc,

@@ -94,6 +94,16 @@ module leswht_c
       integer(c_int64_t), value :: cap
     end function lw_get_mg_residuals
 
+    ! every mgftresiduemax.dat row of the last step in file order: kind 0
+    ! mg2d's RM rows, kind 1 gsor2d's RG rows (poiss_z_ft.f90:293-297, 584)
+    integer(c_int64_t) function lw_get_residual_rows(h, kind, kv, kk, val, cap) bind(c, name='lw_get_residual_rows')
+      import :: c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: kind(*), kv(*), kk(*)
+      real(c_double), intent(out) :: val(*)
+      integer(c_int64_t), value :: cap
+    end function lw_get_residual_rows
+
     ! the poiss_itr.dat and ftrpoittr.dat rows of the last step (poiss_mgrd.f90:185,221)
     integer(c_int64_t) function lw_get_bicg_log(h, sub, icg, rm, loc, cap) bind(c, name='lw_get_bicg_log')
       import :: c_ptr, c_int64_t, c_double

@@ -36,8 +36,10 @@ program solver_exec
   integer :: elen, estat
   logical :: heat, ibm
   integer(c_int64_t) :: ntrace, ntr, nres, q
-  integer(c_int64_t) :: trpts(192), rkv(9), rkk(9)
-  real(c_double) :: trv(256), rrm(9), t_begin
+  integer(c_int64_t) :: trpts(192)
+  real(c_double) :: trv(256), t_begin
+  integer(c_int64_t), allocatable :: rkind(:), rkv(:), rkk(:)
+  real(c_double), allocatable :: rrm(:)
   integer(c_int64_t), parameter :: maxit = 3 * 4096
   integer(c_int64_t), allocatable :: bsub(:), bicg(:), bloc(:)
   real(c_double), allocatable :: brm(:)
@@ -81,6 +83,7 @@ program solver_exec
     open (77, file='../output/ftr/poiss_itr.dat')
     open (78, file='../output/ftr/ftrpoittr.dat')
   else
+    allocate (rkind(64), rkv(64), rkk(64), rrm(64))
     open (77, file='../output/ftr/mgftresiduemax.dat')
   end if
   call system_clock(count_rate=rate)
@@ -104,12 +107,23 @@ program solver_exec
         write (78, 103) t_begin, hs%dt, sicg(q), srm(q)
       end do
     else
-      nres = lw_get_mg_residuals(h, rkv, rkk, rrm, 9_c_int64_t)
-      do q = 1, min(nres, 9_c_int64_t)
-        write (77, 999) rkv(q), rkk(q), rrm(q)
+      ! mg2d's rows, then gsor2d's per sweep (IMGSOR < n3m/2+1)
+      nres = lw_get_residual_rows(h, rkind, rkv, rkk, rrm, 0_c_int64_t)
+      if (nres > size(rkv)) then
+        deallocate (rkind, rkv, rkk, rrm)
+        allocate (rkind(nres), rkv(nres), rkk(nres), rrm(nres))
+      end if
+      nres = lw_get_residual_rows(h, rkind, rkv, rkk, rrm, int(size(rkv), c_int64_t))
+      do q = 1, min(nres, int(size(rkv), c_int64_t))
+        if (rkind(q) == 0) then
+          write (77, 999) rkv(q), rkk(q), rrm(q)
+        else
+          write (77, 998) rkv(q), rkk(q), rrm(q)
+        end if
       end do
     end if
 999 format('KV=', i4, 3x, 'KK=', i4, 3x, 'RM=', es24.16)
+998 format('KV=', i3, 3x, 'KK=', i3, 3x, 'RG=', es24.16)
 102 format(i4, es15.7, 3i4)
 103 format(2f13.5, i5, es15.7)
 

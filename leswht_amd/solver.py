@@ -124,7 +124,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
            "lw_create_static", "lw_static_area_bytes", "lw_set_trace", "lw_trace_info", "lw_get_trace",
-           "lw_get_mg_residuals", "lw_get_bicg_log", "lw_get_bicg_solves")
+           "lw_get_mg_residuals", "lw_get_residual_rows", "lw_get_bicg_log", "lw_get_bicg_solves")
 
 _lib = None
 
@@ -146,6 +146,8 @@ def load_library(path: str = LIB_PATH):
     L.lw_get_trace.argtypes = [H, _PD, _I]
     L.lw_get_mg_residuals.argtypes = [H, _PI, _PI, _PD, _I]
     L.lw_get_mg_residuals.restype = _I
+    L.lw_get_residual_rows.argtypes = [H, _PI, _PI, _PI, _PD, _I]
+    L.lw_get_residual_rows.restype = _I
     L.lw_get_bicg_log.argtypes = [H, _PI, _PI, _PD, _PI, _I]
     L.lw_get_bicg_log.restype = _I
     L.lw_get_bicg_solves.argtypes = [H, _PI, _PD, _I]
@@ -535,6 +537,19 @@ class Solver:
         out = np.zeros(max(4 * n, 1))
         _check(load_library().lw_get_trace(self._h, _pd(out), len(out)))
         return {c: out[q * n:(q + 1) * n].copy() for q, c in enumerate("uvwp")}
+
+    def residual_rows(self):
+        """Every mgftresiduemax.dat row of the last step in file order: list of
+        (kind, kv, kk, value), kind 0 for mg2d's RM rows and 1 for gsor2d's RG
+        rows (one per sweep of its loop, kv above IMGSOR)."""
+        L = load_library()
+        n = int(L.lw_get_residual_rows(self._h, None, None, None, None, 0))
+        if n < 0:
+            _check(n)
+        kind, kv, kk = (np.zeros(max(n, 1), np.int64) for _ in range(3))
+        val = np.zeros(max(n, 1))
+        L.lw_get_residual_rows(self._h, _pi(kind), _pi(kv), _pi(kk), _pd(val), n)
+        return [(int(kind[q]), int(kv[q]), int(kk[q]), float(val[q])) for q in range(n)]
 
     def mg_residuals(self):
         """The mgftresiduemax.dat rows of the last step: list of (kv, kk, rm)."""

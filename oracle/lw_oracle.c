@@ -1753,6 +1753,63 @@ static int64_t mg2d(mgctx *m, double test) {
   return o->s.mgitr;
 }
 
+/* resid3 (poiss_z_ft.f90:590-616): the max norm of rhs - A u on the top
+   level, with the stored diagonal ac */
+static double resid3(mgctx *m) {
+  ora *o = m->o;
+  double errmax = 0.0;
+  for (int64_t j = 1; j <= o->n2m; ++j)
+    for (int64_t i = 1; i <= o->n1m; ++i) {
+      cplx r = RHS(i, j);
+      r = c_sub(r, c_rmul(o->aw[i], PC(o->imm[i], j)));
+      r = c_sub(r, c_rmul(o->ae[i], PC(o->ipm[i], j)));
+      r = c_sub(r, c_rmul(o->as[j], PC(i, j - 1)));
+      r = c_sub(r, c_rmul(o->an[j], PC(i, j + 1)));
+      r = c_sub(r, c_rmul(AC(i, j), PC(i, j)));
+      double ar = c_abs(r);
+      errmax = errmax > ar ? errmax : ar;
+    }
+  return errmax;
+}
+
+/* one zebra line over-relaxation of the top level (the two halves of
+   gsor2d, poiss_z_ft.f90:527-557 with wgt = oldv, 565-597 with wgt = 1):
+   the first half's x coupling is scaled by wgt, the second's is not, and
+   both relax as u = ww*x + (wgt*ww2)*u */
+static void sor_sweep(mgctx *m, double wgt) {
+  ora *o = m->o;
+  int64_t n1m = o->n1m, n2m = o->n2m;
+  double ww = o->s.wwsor, ww2 = 1.0 - ww, cw = wgt * ww2;
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t i = 1; i <= n1m; i += 2)
+      GG(i, j) = c_sub(RHS(i, j), c_rmul(wgt, c_add(c_rmul(o->aw[i], PC(o->imm[i], j)),
+                                                     c_rmul(o->ae[i], PC(o->ipm[i], j)))));
+  trdiag1m(m, 1, n1m);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t i = 1; i <= n1m; i += 2) PC(i, j) = c_add(c_rmul(ww, GG(i, j)), c_rmul(cw, PC(i, j)));
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t i = 2; i <= n1m; i += 2)
+      GG(i, j) = c_sub(RHS(i, j), c_add(c_rmul(o->aw[i], PC(o->imm[i], j)), c_rmul(o->ae[i], PC(o->ipm[i], j))));
+  trdiag1m(m, 2, n1m);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t i = 2; i <= n1m; i += 2) PC(i, j) = c_add(c_rmul(ww, GG(i, j)), c_rmul(cw, PC(i, j)));
+}
+
+/* gsor2d (poiss_z_ft.f90:513-587) with oldv = 0 as poisson calls it: one
+   sweep from u = 0, then up to MGITR sweeps until resid3's max norm passes
+   the test; returns the sweeps of the loop (0 when the first passed) */
+static int64_t gsor2d(mgctx *m, double test) {
+  ora *o = m->o;
+  sor_sweep(m, 0.0);
+  if (resid3(m) < test) return 0;
+  for (int64_t kk = 1; kk <= o->s.mgitr; ++kk) {
+    sor_sweep(m, 1.0);
+    if (resid3(m) < test) return kk;
+  }
+  fprintf(stderr, "oracle: ITERATION LIMIT EXCEEDED (kv=%lld, line-SOR)\n", (long long)m->kv);
+  return o->s.mgitr;
+}
+
 static void poisson(ora *o) {
   int64_t n1 = o->n1, n2 = o->n2, n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n3mh = o->n3mh;
   memset(o->phi, 0, NX1 * NY1 * NZ1 * 8);
@@ -1778,11 +1835,7 @@ static void poisson(ora *o) {
     memset(m.ggii, 0, (size_t)(o->n1md + 1) * (n2 + 1) * sizeof(cplx));
     m.kv = kkk;
     m.rhs = &CC(1, 1, kkk);
-    if (kkk > o->s.imgsor) {
-      fprintf(stderr, "oracle: line-SOR wavenumbers (IMGSOR < n3m/2+1) are not restated\n");
-      abort();
-    }
-    int64_t cyc = mg2d(&m, test);
+    int64_t cyc = kkk > o->s.imgsor ? gsor2d(&m, test) : mg2d(&m, test);
     if (cyc > o->mg_cycles_max) o->mg_cycles_max = cyc;
     o->mg_cycles_sum += cyc;
     for (int64_t j = 1; j <= n2m; ++j)

@@ -40,6 +40,8 @@ CASES = {
     "cyl32_les_100": dict(dims=(32, 32, 8), mode="les", nend=100),
     # makefld (IREAD = 0) and the tracer every 2 steps
     "cyl32_iread0_trace_4": dict(dims=(32, 32, 8), mode="les_trace_iread0", nend=4),
+    # line-SOR (gsor2d) for the wavenumbers above IMGSOR = 2, with its RG rows
+    "cyl32_sor_3": dict(dims=(32, 32, 8), mode="les_sor", nend=3),
     # the 3-D MG-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90) linked
     # in place of poiss_z_ft.f90 (oracle/_ref/solver_exec_mgrd)
     "mgrd_cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3, poisson="mgrd"),

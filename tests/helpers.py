@@ -28,6 +28,9 @@ def spec_for(dims, mode, nend=1, nlev=None):
     elif mode == "les_trace_iread0":
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 2, re=3900.0, iread=0,
                        trace=TRACE_CELLS, ntr=2)
+    elif mode == "les_sor":
+        # kv = 1, 2 by multigrid, kv = 3..n3m/2+1 by line-SOR (gsor2d)
+        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 2, re=3900.0, imgsor=2, wwsor=1.8)
     elif mode == "les_cfl":
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 3, re=3900.0, idtopt=1, cflfac=0.5)
     else:
