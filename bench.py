@@ -198,6 +198,17 @@ def serve_bench(args) -> dict:
               file=sys.stderr, flush=True)
         single = serve.run_stream(cases, dirs, budget, 1, honour_arrivals=False, write_fields=False)
         print("all at once, 1 slot: %.3e cell-updates/s" % single["throughput"], file=sys.stderr, flush=True)
+        mixed = None
+        if args.mixed:
+            # the served mixed-precision form: the same batch through all
+            # slots with the FP32 multigrid levels wherever a case allows them
+            os.environ["LW_MG_FP32"] = "1"
+            try:
+                mixed = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=False, write_fields=False)
+            finally:
+                del os.environ["LW_MG_FP32"]
+            print("all at once, %d slots, mixed precision: %.3e cell-updates/s" % (args.slots, mixed["throughput"]),
+                  file=sys.stderr, flush=True)
     finally:
         shutil.rmtree(root, ignore_errors=True)
     n_large = sum(c.large for c in cases)
@@ -221,7 +232,11 @@ def serve_bench(args) -> dict:
                    "peak_reserved_gb": round(stream["info"]["peak_reserved_bytes"] / (1 << 30), 2)},
         "closed_batch": {"slots": args.slots, "value": closed["throughput"], "one_slot": single["throughput"],
                          "ratio": closed["throughput"] / single["throughput"]},
-        "failed": len(stream["failed"]) + len(closed["failed"]) + len(single["failed"]),
+        "closed_batch_mixed": None if mixed is None else {
+            "slots": args.slots, "value": mixed["throughput"],
+            "ratio_vs_one_slot_fp64": mixed["throughput"] / single["throughput"]},
+        "failed": len(stream["failed"]) + len(closed["failed"]) + len(single["failed"]) +
+                  (len(mixed["failed"]) if mixed else 0),
     }
 
 

@@ -1051,8 +1051,10 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
     h->gamma[1] = 8. / 15.; h->gamma[2] = 5. / 12.; h->gamma[3] = 3. / 4.;
     h->ro[1] = 0.; h->ro[2] = -17. / 60.; h->ro[3] = -5. / 12.;
     h->dt = s->dt_size;
+    // LW_MG_FP32=1 asks for the mixed-precision multigrid wherever the case
+    // allows it (the option "mg_fp32" insists and reports when it does not)
     if (const char *e = getenv("LW_MG_FP32"))
-      if (atoi(e) != 0) set_mg_fp32(h, true);
+      if (atoi(e) != 0 && h->st.poisson == 0 && mg_mixed_supported(h->g, h->mg)) set_mg_fp32(h, true);
     HIPCHK(hipStreamSynchronize(h->s));
     *out = h;
     return LW_OK;
