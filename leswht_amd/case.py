@@ -755,6 +755,11 @@ class CaseSpec:
     noise: float = 1.0e-3
     seed: int = 20261203
     settings: Settings = dataclasses.field(default_factory=Settings)
+    # "body": sinh-clustered x and y lines around the body, inflow/outflow in
+    # x; "channel": uniform lines, x and z periodic, the y ends the settings'
+    # boundaries (walls with BC_Y = 0), the body (if it reaches the domain)
+    # immersed as usual
+    layout: str = "body"
 
 
 @dataclasses.dataclass
@@ -770,19 +775,37 @@ class Case:
 
 def build_case(spec: CaseSpec) -> Case:
     s = spec.settings
-    xl = gridlines_sinh(spec.n1m, spec.xlim[0], spec.xlim[1], spec.dmin)
-    yl = gridlines_sinh(spec.n2m, spec.ylim[0], spec.ylim[1], spec.dmin)
+    channel = spec.layout == "channel"
+    if channel:
+        xl = gridlines_uniform(spec.n1m, spec.xlim[0], spec.xlim[1])
+        yl = gridlines_uniform(spec.n2m, spec.ylim[0], spec.ylim[1])
+    else:
+        xl = gridlines_sinh(spec.n1m, spec.xlim[0], spec.xlim[1], spec.dmin)
+        yl = gridlines_sinh(spec.n2m, spec.ylim[0], spec.ylim[1], spec.dmin)
     zl = gridlines_uniform(spec.n3m, 0.0, spec.zlen)
     xl, yl, zl = _fmt15(xl), _fmt15(yl), _fmt15(zl)
     lengths = tuple(float("%23.15f" % v) for v in (
         spec.xlim[1] - spec.xlim[0], spec.ylim[1] - spec.ylim[0], spec.zlen))
-    ax = make_axis(xl, periodic=False)
+    ax = make_axis(xl, periodic=channel)
     ay = make_axis(yl, periodic=False)
     az = make_axis(zl, periodic=True)
     fn = BODIES[spec.body]
     body = (lambda x, y, z: fn(x, y, z, *spec.body_args)) if spec.body_args else fn
     tables = build_ib_tables(body, ax, ay, az, heat=bool(s.ihtrans), les=bool(s.iles))
     fields = initial_fields(spec, ax, ay, az, body)
+    if channel:
+        # no-slip walls at the y ends: the wall-tangential ghosts and the
+        # wall-normal face velocities are zero (the solver keeps them)
+        n2 = ay.ncell + 1
+        for q in ("u", "w"):
+            if s.bc_ybtm == 0:
+                fields[q][:, 0, :] = 0.0
+            if s.bc_ytop == 0:
+                fields[q][:, n2, :] = 0.0
+        if s.bc_ybtm == 0:
+            fields["v"][:, 1, :] = 0.0
+        if s.bc_ytop == 0:
+            fields["v"][:, n2, :] = 0.0
     return Case(spec, ax, ay, az, lengths, tables, fields)
 
 
@@ -862,7 +885,7 @@ def write_case(case: Case, root: str) -> str:
     with open(os.path.join(root, "output/grid/grid.bin"), "w") as f:
         f.write(grid_bin_text(case.ax, case.ay, case.az, case.lengths))
     with open(os.path.join(root, "output/ibmpre/ibmpre_prdic.bin"), "w") as f:
-        f.write("%d %d %d %d" % (0, 0, 1, 1))
+        f.write("%d %d %d %d" % (case_prdic(case.spec) + (1,)))
     with open(os.path.join(root, "output/ibmpre/ibmpre_fcpts.bin"), "w") as f:
         f.write(fcpts_text(case.tables))
     if s.ihtrans:
@@ -878,8 +901,9 @@ def write_case(case: Case, root: str) -> str:
             f.write(wallfdvm_text(case.tables))
     n = (case.ax.ncell + 1, case.ay.ncell + 1, case.az.ncell + 1)
     fl = case.fields
+    xp = case_prdic(case.spec)[0]
     write_fld(os.path.join(root, "output/field/fld000000"), n, s.re, s.pr, s.gr,
-              0, 0, 0.0, s.dt_size, (0, 0, 1), (2, 3, s.bc_ybtm, s.bc_ytop, 4, 4),
+              0, 0, 0.0, s.dt_size, case_prdic(case.spec), (4 if xp else 2, 4 if xp else 3, s.bc_ybtm, s.bc_ytop, 4, 4),
               s.ich, s.iconjg, fl["u"], fl["v"], fl["w"], fl["p"],
               fl["t"] if s.ihtrans else None)
     return sol
@@ -1120,6 +1144,11 @@ def apply_prefld(fl: dict, s: Settings, g: dict) -> dict:
     return dict(u=u, v=v, w=w, p=p, t=t)
 
 
+def case_prdic(spec: CaseSpec) -> Tuple[int, int, int]:
+    """The periodicity flags of ibmpre_prdic.bin: z always, x for a channel."""
+    return (1 if spec.layout == "channel" else 0, 0, 1)
+
+
 def case_view(case: Case) -> dict:
     """The same dictionary :func:`load_case_dir` returns for the directory
     :func:`write_case` would write, built in memory (the values pass through
@@ -1139,7 +1168,8 @@ def case_view(case: Case) -> dict:
         nz.update(nzero=case.tables.nzero, inz=ii.astype(np.int64), jnz=jj.astype(np.int64),
                   knz=kk.astype(np.int64), nwall=case.tables.iszero.copy())
     n1, n2, n3 = g["n1"], g["n2"], g["n3"]
-    fl = dict(prdic=(0, 0, 1))
+    xp, yp, zp = case_prdic(case.spec)
+    fl = dict(prdic=(xp, yp, zp))
     f = case.fields
     u = f["u"].copy(); u[0] = 0.0
     v = f["v"].copy(); v[:, 0, :] = 0.0
@@ -1149,7 +1179,7 @@ def case_view(case: Case) -> dict:
     fl.update(u=u, v=v, w=w, p=p)
     if s.ihtrans:
         fl["t"] = f["t"]
-    st = dataclasses.replace(s, xprdic=0, yprdic=0, zprdic=1, iintp=1)
+    st = dataclasses.replace(s, xprdic=xp, yprdic=yp, zprdic=zp, iintp=1)
     if s.iread == 0:
         fields = makefld(st, g, BODIES[case.spec.body], case.spec.body_args)
     else:

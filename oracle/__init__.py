@@ -186,8 +186,12 @@ class Oracle:
         L.ora_set_fields(self.h, *[_pd(a) for a in arrs], _pd(tt) if tt is not None else None,
                          cd["ihist"], cd["m"], cd["time"], cd["dt"])
         L.ora_init(self.h)
+        # an x-periodic case takes poiss_ftft, the reference Makefile's default
+        if poisson == "z_ft" and cd["settings"].xprdic == 1:
+            poisson = "ftft"
         if poisson != "z_ft":
-            if poisson != "mgrd" or L.ora_set_poisson(self.h, 1) != 0:
+            kind = {"mgrd": 1, "ftft": 2}.get(poisson, -1)
+            if kind < 0 or L.ora_set_poisson(self.h, kind) != 0:
                 raise ValueError("oracle: Poisson path %r not available for this case" % poisson)
 
     def bicg_log(self):
@@ -230,7 +234,9 @@ class Oracle:
         if n == np.prod(sh):
             return out.reshape(sh, order="F")
         g = self.g
-        return out.reshape((g["n1m"], g["n2m"], g["n3m"]), order="F")
+        if n == g["n1m"] * g["n2m"] * g["n3m"]:
+            return out.reshape((g["n1m"], g["n2m"], g["n3m"]), order="F")
+        return out  # scalars: "cmfr" (the fcmfr.dat row), "pmi"
 
     def __del__(self):
         try:
@@ -241,6 +247,7 @@ class Oracle:
 
 
 REF_EXEC_MGRD = os.path.join(HERE, "_ref", "solver_exec_mgrd")
+REF_EXEC_FTFT = os.path.join(HERE, "_ref", "solver_exec_ftft")
 
 
 def run_reference(root: str, threads: int = 1, timeout: int = 600, exe: str = REF_EXEC) -> str:

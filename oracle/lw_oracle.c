@@ -91,6 +91,12 @@ struct ora {
    * link time, 02_solver/Makefile:23-34) */
   int poiss_kind;
   struct mgrd *m3;
+  /* channel (ICH != 0, mod_common.f90:77): pmi(0:4), pmiavg, the PI
+   * controller's integral (meanpg's SAVE variable), the fcmfr.dat row of the
+   * last step (time, qflux, ubulk, pmiavg, cmfravg(1:3)) and its sums */
+  double pmi[5], pmiavg, err_integral, cmfr[7], cmfravg_int[3], cmfravg_dur;
+  /* poiss_ftft (poisinit, poiss_ftft.f90:164-204): ak1(n1m), ak3(n3mh) */
+  double *ak1f, *ak3f;
 };
 
 /* ---- indexing ------------------------------------------------------- */
@@ -151,8 +157,10 @@ static void *xcalloc(size_t n, size_t sz) {
   } while (0)
 
 ora *ora_create(const ora_settings *s, const ora_grid *g, const ora_tables *t) {
-  if (s->xprdic || s->yprdic || s->ich != 0 || s->iconjg || s->imovingon ||
-      s->idecomp || s->gr != 0.0) {
+  /* x periodicity is restated for the channel: no heat, no dynamic SGS */
+  const int chan_ok = s->xprdic == 1 && s->ihtrans == 0 && (s->iles == 0 || s->idvmon == 0);
+  if ((s->xprdic && !chan_ok) || s->yprdic || (s->ich != 0 && !s->xprdic) || s->ich < 0 || s->ich > 2 ||
+      s->iconjg || s->imovingon || s->idecomp || s->gr != 0.0) {
     fprintf(stderr, "oracle: configuration outside the restated scope\n");
     return NULL;
   }
@@ -290,6 +298,8 @@ int64_t ora_get_array(const ora *o, const char *name, double *out) {
   else if (!strncmp(name, "nusgs1_", 7)) src = o->nusgs1[name[7] - '1'];
   else if (!strncmp(name, "alsgs1_", 7)) src = o->alsgs1[name[7] - '1'];
   else if (!strncmp(name, "rhs1_", 5)) src = o->rhs1[name[5] - '1'];
+  else if (!strcmp(name, "cmfr")) { src = o->cmfr; n = 7; }
+  else if (!strcmp(name, "pmi")) { src = o->pmi; n = 5; }
   else {
     static const char *names[23] = {"uavg", "vavg", "wavg", "uiujavg_1", "uiujavg_2", "uiujavg_3", "uiujavg_4",
                                     "uiujavg_5", "uiujavg_6", "pavg", "p2avg", "voravg_1", "voravg_2", "voravg_3",
@@ -768,6 +778,12 @@ static void interpol_edges(ora *o, double *nu, double *nu1[3], int alp) {
   int64_t n1 = o->n1, n2 = o->n2, n3 = o->n3, n1m = o->n1m, n2m = o->n2m, n3m = o->n3m;
   int64_t npad = NX1 * NY1 * NZ1;
   for (int c = 0; c < 3; ++c) memset(nu1[c], 0, npad * 8);
+  if (o->s.xprdic == 1)
+    for (int64_t k = 0; k <= n3; ++k)
+      for (int64_t j = 0; j <= n2; ++j) {
+        F(nu, 0, j, k) = F(nu, n1m, j, k);
+        F(nu, n1, j, k) = F(nu, 1, j, k);
+      }
   /* z periodic ghosts */
   for (int64_t j = 0; j <= n2; ++j)
     for (int64_t i = 0; i <= n1; ++i) {
@@ -794,10 +810,15 @@ static void interpol_edges(ora *o, double *nu, double *nu1[3], int alp) {
         F(nu1[2], i, j, k) = o->c2cxi[i] * o->c2cyi[j] * 0.25 *
                              (f2fx[i - 1] * (f2fy[j - 1] * F(nu, i, j, k) + f2fy[j] * F(nu, i, j - 1, k)) +
                               f2fx[i] * (f2fy[j - 1] * F(nu, i - 1, j, k) + f2fy[j] * F(nu, i - 1, j - 1, k)));
-  /* x boundaries (bc_xbtm = 2, bc_xtop = 3: never walls) */
+  /* x boundaries: periodic copies (sgs.f90:1032-1036), or inlet/outlet
+   * (bc_xbtm = 2, bc_xtop = 3: never walls) */
   for (int64_t j = jb; j <= n2m; ++j)
     for (int64_t k = kb; k <= n3m; ++k) {
       int64_t jm = o->jmv[j], km = o->kmv[k];
+      if (o->s.xprdic == 1) {
+        for (int c = 0; c < 3; ++c) F(nu1[c], n1, j, k) = F(nu1[c], 1, j, k);
+        continue;
+      }
       F(nu1[1], 1, j, k) = o->c2czi[k] * 0.5 * (f2fz[k] * F(nu, 2, j, km) + f2fz[km] * F(nu, 2, j, k));
       F(nu1[2], 1, j, k) = o->c2cyi[j] * 0.5 * (f2fy[j] * F(nu, 2, jm, k) + f2fy[jm] * F(nu, 2, j, k));
       F(nu1[1], n1, j, k) = o->c2czi[k] * 0.5 * (f2fz[k] * F(nu, n1m, j, km) + f2fz[km] * F(nu, n1m, j, k));
@@ -1058,7 +1079,8 @@ static void rhs(ora *o) {
         }
         F(o->rhs1[0], i, j, k) =
             dt * (gam * rk3x + ro * G(o->rk3o[0], i, j, k) + 2. * alpha * alx -
-                  2. * alpha * (F(p, i, j, k) - F(p, i - 1, j, k)) * c2cxi[i] - 2. * alpha * 0.0 * 0.0);
+                  2. * alpha * (F(p, i, j, k) - F(p, i - 1, j, k)) * c2cxi[i] -
+                  2. * alpha * o->pmi[0] * (double)(o->s.ich != 0 ? 1 : 0));
         G(o->rk3oo[0], i, j, k) = G(o->rk3o[0], i, j, k);
         G(o->rk3o[0], i, j, k) = rk3x;
       }
@@ -1242,6 +1264,13 @@ static void convbc(ora *o) {
   int64_t n1 = o->n1, n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n2 = o->n2, n3 = o->n3;
   const double *u = o->u, *v = o->v, *w = o->w;
 #define B2(a, j, k) (a)[(j) + NY1 * (int64_t)(k)]
+  if (o->s.ich == 1) {
+    /* constant flow rate: no outlet (slv_mmtm.f90:980-984) */
+    memset(o->uout, 0, NY1 * NZ1 * 8);
+    memset(o->vout, 0, NY1 * NZ1 * 8);
+    memset(o->wout, 0, NY1 * NZ1 * 8);
+    return;
+  }
   double qin = 0.;
   for (int64_t k = 1; k <= n3m; ++k)
     for (int64_t j = 1; j <= n2m; ++j) qin = F(u, 1, j, k) * o->f2fy[j] * o->f2fz[k] + qin;
@@ -1433,11 +1462,34 @@ static void lhs_mom(ora *o, int comp) {
         b[i - ilo] = acoefi - ai - ci;
         g[i - ilo] = acoefi * F(r, i, j, k);
       }
-      ora_thomas(n1m - ilo + 1, a, b, c, g);
+      /* trdiag1p for x periodicity (slv_mmtm.f90:1501-1570) */
+      if (o->s.xprdic == 1) ora_thomas_periodic(n1m - ilo + 1, a, b, c, g);
+      else ora_thomas(n1m - ilo + 1, a, b, c, g);
       for (int64_t i = ilo; i <= n1m; ++i) F(fld, i, j, k) = g[i - ilo] + F(fld, i, j, k);
     }
   }
   free(a); free(b); free(c); free(g);
+}
+
+/* the x-periodic ghost planes of u, v, w as prdic_adj_uvw and ucalc copy
+ * them (slv_mmtm.f90:1755-1786, slv_auxi.f90:388-412) */
+static void xghost_uvw(ora *o) {
+  int64_t n1 = o->n1, n1m = o->n1m, n2 = o->n2, n3 = o->n3;
+  for (int64_t k = 0; k <= n3; ++k)
+    for (int64_t j = 0; j <= n2; ++j) {
+      F(o->u, 0, j, k) = F(o->u, n1m, j, k);
+      F(o->u, n1, j, k) = F(o->u, 1, j, k);
+    }
+  for (int64_t k = 0; k <= n3; ++k)
+    for (int64_t j = 1; j <= n2; ++j) {
+      F(o->v, 0, j, k) = F(o->v, n1m, j, k);
+      F(o->v, n1, j, k) = F(o->v, 1, j, k);
+    }
+  for (int64_t k = 1; k <= n3; ++k)
+    for (int64_t j = 0; j <= n2; ++j) {
+      F(o->w, 0, j, k) = F(o->w, n1m, j, k);
+      F(o->w, n1, j, k) = F(o->w, 1, j, k);
+    }
 }
 
 /* slv_mmtm.f90:1710-1857 retrv_uvw + prdic_adj_uvw(0) */
@@ -1449,6 +1501,7 @@ static void retrv_prdic_uvw(ora *o) {
     for (int64_t j = 2; j <= o->n2m; ++j) F(o->v, n1, j, k) = B2(o->vout, j, k);
   for (int64_t k = o->k_bgpz; k <= n3m; ++k)
     for (int64_t j = 1; j <= o->n2m; ++j) F(o->w, n1, j, k) = B2(o->wout, j, k);
+  if (o->s.xprdic == 1) xghost_uvw(o);
   for (int64_t j = 0; j <= n2; ++j)
     for (int64_t i = 1; i <= n1; ++i) {
       F(o->u, i, j, 0) = F(o->u, i, j, n3m);
@@ -1472,7 +1525,7 @@ static void rhsnlhs(ora *o) {
   rhs(o);
   if (o->s.ibmon != 0) rhs_ibm(o);
   convbc(o);
-  rhsincorpbc(o);
+  if (o->s.ich != 1) rhsincorpbc(o);
   lhs_mom(o, 0);
   lhs_mom(o, 1);
   if (o->n3m != 1) lhs_mom(o, 2);
@@ -1891,6 +1944,7 @@ static void ucalc(ora *o) {
       F(o->w, i, j, 0) = F(o->w, i, j, n3m);
       F(o->w, i, j, n3) = F(o->w, i, j, 1);
     }
+  if (o->s.xprdic == 1) xghost_uvw(o);
 }
 
 static void pcalc(ora *o) {
@@ -1913,6 +1967,92 @@ static void pcalc(ora *o) {
       F(o->p, i, j, 0) = F(o->p, i, j, o->n3m);
       F(o->p, i, j, o->n3) = F(o->p, i, j, 1);
     }
+  if (o->s.xprdic == 1)
+    for (int64_t k = 1; k <= o->n3m; ++k)
+      for (int64_t j = 1; j <= o->n2m; ++j) {
+        F(o->p, 0, j, k) = F(o->p, o->n1m, j, k);
+        F(o->p, o->n1, j, k) = F(o->p, 1, j, k);
+      }
+}
+
+/* meanpg (slv_mmtm.f90:2-177): the mean pressure gradient of the channel
+ * from the wall shear at the y walls and the IB forcing of u, over the
+ * domain volume; then ICH = 1 adds the PI controller on the bulk velocity
+ * (its integral kept between calls), ICH = 2 fixes it at -UDRV_I */
+static void meanpg(ora *o) {
+  int64_t n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n2 = o->n2;
+  double flowvol = 0.0, qvol_current = 0.0, u_bulk = 0.0;
+  for (int64_t k = 1; k <= n3m; ++k)
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t i = 1; i <= n1m; ++i)
+        if (funcbody(o, o->x[i], o->ymp[j], o->zmp[k]) >= 1.e-10) {
+          flowvol = flowvol + o->c2cx[i] * o->f2fy[j] * o->f2fz[k];
+          qvol_current = qvol_current + F(o->u, i, j, k) * o->c2cx[i] * o->f2fy[j] * o->f2fz[k];
+        }
+  u_bulk = flowvol > 0.0 ? qvol_current / flowvol : 0.0;
+  for (int q = 0; q < 5; ++q) o->pmi[q] = 0.;
+  if (o->s.bc_ybtm == 0) {
+    double tmp = 0., dy1 = 1.0 / o->c2cyi[1], dy2 = dy1 + 1.0 / o->c2cyi[2];
+    for (int64_t k = 1; k <= n3m; ++k)
+      for (int64_t i = 1; i <= n1m; ++i)
+        if (funcbody(o, o->x[i], o->ymp[1], o->zmp[k]) >= 1.e-10) {
+          double g = (F(o->u, i, 1, k) * (dy2 * dy2) - F(o->u, i, 2, k) * (dy1 * dy1)) / (dy1 * dy2 * (dy2 - dy1));
+          tmp = tmp + g * o->c2cx[i] * o->f2fz[k] / o->s.re;
+        }
+    o->pmi[1] = o->pmi[1] + tmp;
+  }
+  if (o->s.bc_ytop == 0) {
+    double tmp = 0., dy1 = 1.0 / o->c2cyi[n2], dy2 = dy1 + 1.0 / o->c2cyi[n2m];
+    for (int64_t k = 1; k <= n3m; ++k)
+      for (int64_t i = 1; i <= n1m; ++i)
+        if (funcbody(o, o->x[i], o->ymp[n2m], o->zmp[k]) >= 1.e-10) {
+          double g = (F(o->u, i, n2m, k) * (dy2 * dy2) - F(o->u, i, n2m - 1, k) * (dy1 * dy1)) /
+                     (dy1 * dy2 * (dy2 - dy1));
+          tmp = tmp + g * o->c2cx[i] * o->f2fz[k] / o->s.re;
+        }
+    o->pmi[2] = o->pmi[2] + tmp;
+  }
+  /* z walls (pmi(3), pmi(4)) need a non-periodic z, outside this path */
+  o->pmi[0] = o->pmi[1] + o->pmi[2] + o->pmi[3] + o->pmi[4];
+  double tmp = 0.;
+  for (int64_t n = 0; n < o->nbody[0]; ++n)
+    tmp = tmp + o->fcv[0][n] * o->c2cx[o->ifc[0][n]] * o->f2fy[o->jfc[0][n]] * o->f2fz[o->kfc[0][n]];
+  o->pmi[0] = o->pmi[0] + tmp;
+  double volume = o->xl * o->yl * o->zl;
+  for (int q = 0; q < 5; ++q) o->pmi[q] = o->pmi[q] / volume;
+  if (o->s.ich == 1) {
+    if (o->ntime > 0 && o->dt > 0.0) {
+      double err_u = o->s.udrv_i - u_bulk;
+      o->err_integral = o->err_integral + err_u * o->dtconst;
+      o->pmi[0] = o->pmi[0] - (1.0 * err_u + 0.1 * o->err_integral);
+    }
+  } else if (o->s.ich == 2) {
+    o->pmi[0] = -o->s.udrv_i;
+  }
+}
+
+/* writehistory's fcmfr.dat row (misc_hist.f90:91-137) */
+static void channel_history(ora *o) {
+  double flowvol = 0.0, qvol_u = 0.0;
+  for (int64_t k = 1; k <= o->n3m; ++k)
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      for (int64_t i = 1; i <= o->n1m; ++i)
+        if (funcbody(o, o->x[i], o->ymp[j], o->zmp[k]) >= 1.e-10) {
+          qvol_u = qvol_u + F(o->u, i, j, k) * o->c2cx[i] * o->f2fy[j] * o->f2fz[k];
+          flowvol = flowvol + o->c2cx[i] * o->f2fy[j] * o->f2fz[k];
+        }
+  double qflux = qvol_u / o->xl, ubulk = flowvol > 0.0 ? qvol_u / flowvol : 0.0;
+  double avg[3] = {0., 0., 0.};
+  if (o->time >= o->s.avg_tst) {
+    o->cmfravg_int[0] = o->cmfravg_int[0] + qflux * o->dt;
+    o->cmfravg_int[1] = o->cmfravg_int[1] + ubulk * o->dt;
+    o->cmfravg_int[2] = o->cmfravg_int[2] + o->pmiavg * o->dt;
+    o->cmfravg_dur = o->cmfravg_dur + o->dt;
+    if (o->cmfravg_dur > 0.0)
+      for (int q = 0; q < 3; ++q) avg[q] = o->cmfravg_int[q] / o->cmfravg_dur;
+  }
+  o->cmfr[0] = o->time; o->cmfr[1] = qflux; o->cmfr[2] = ubulk; o->cmfr[3] = o->pmiavg;
+  for (int q = 0; q < 3; ++q) o->cmfr[4 + q] = avg[q];
 }
 
 /* slv_auxi.f90:519-570 lagforce */
@@ -2149,6 +2289,7 @@ static void rhsnlhs_t(ora *o) {
 void ora_stepinit(ora *o) {
   o->ntime = o->ntime + 1;
   o->m = o->ntime;
+  if (o->s.ich != 0) o->pmiavg = 0.;
   for (int c = 0; c < o->ncomp; ++c) memset(o->fcvavg[c], 0, o->nbody[c] * 8);
   if (o->s.idtopt == 0) o->dt = o->s.dt_size;
   double cflm = 0.;
@@ -2181,11 +2322,16 @@ void ora_substep_begin(ora *o, int substep) {
   o->test1 = o->s.resid1 * o->dtconsti;
   o->acoef = o->alpha * o->dt / o->s.re;
   o->acoefi = 1. / o->acoef;
+  o->pmiavg = o->pmiavg + o->pmi[0] * 2. * o->alpha;
   o->subdt = o->subdt + o->dtconst;
   o->msub = substep;
 }
 
+static void meanpg(ora *o);
 void ora_init(ora *o) {
+  /* solver.f90:54: the channel's first mean pressure gradient, from the
+   * loaded field, before dttimeinit (ntime is still 0: no controller term) */
+  if (o->s.ich != 0) meanpg(o);
   dttimeinit(o);
   rk3coefinit(o);
   lhsinit(o);
@@ -2207,6 +2353,7 @@ void ora_phase_poisson(ora *o) { poisson_any(o); }
 void ora_phase_ucalc_pcalc(ora *o) {
   ucalc(o);
   pcalc(o);
+  if (o->s.ich != 0) meanpg(o);
   if (o->s.ibmon == 1) lagforce(o);
 }
 void ora_phase_energy(ora *o) {
@@ -2377,11 +2524,13 @@ void ora_step(ora *o, ora_diag *d) {
     if (o->mg_cycles_max > cyc_max) cyc_max = o->mg_cycles_max;
     ucalc(o);
     pcalc(o);
+    if (o->s.ich != 0) meanpg(o);
     if (o->s.ibmon == 1) lagforce(o);
     if (o->s.ihtrans == 1) rhsnlhs_t(o);
   }
   o->time = o->time + o->dt;
   diagnostics(o);
+  if (o->s.ich != 0) channel_history(o);
   if (o->s.iavg == 1) field_avg(o);
   if (d) {
     d->dt = o->dt; d->time = o->time; d->cflmax = o->cflmax; d->dvmax = o->dvmax;
@@ -2849,17 +2998,131 @@ static void poisson_mgrd(ora *o) {
   o->mg_cycles_sum += icg;
 }
 
+/* ===================================================================== */
+/* poiss_ftft.f90: FFT in x and z, then a y-tridiagonal solve per         */
+/* wavenumber pair (the reference Makefile's default, x-periodic grids)    */
+/* ===================================================================== */
+/* poisinit (poiss_ftft.f90:164-204): modified wavenumbers */
+static void ftft_init(ora *o) {
+  int64_t n1m = o->n1m, n3m = o->n3m, n3mh = o->n3mh, n1mh = n1m / 2 + 1;
+  o->ak1f = (double *)xcalloc(n1m + 2, 8);
+  o->ak3f = (double *)xcalloc(n3mh + 2, 8);
+  double *ai3 = (double *)xcalloc(n3mh + 2, 8), *ai1 = (double *)xcalloc(n1m + 2, 8);
+  double pi = 2.0 * asin(1.0);
+  for (int64_t k = 1; k <= n3mh; ++k) ai3[k] = (double)(k - 1) * 2.0 * pi;
+  ai3[1] = 0.0;
+  for (int64_t i = 1; i <= n1m; ++i) ai1[i] = (double)(i - 1) * 2.0 * pi;
+  ai1[1] = 0.0;
+  double sdzis = o->f2fzi[1], sdxis = o->f2fxi[1];
+  for (int64_t kk = 1; kk <= n3mh; ++kk) o->ak3f[kk] = 2.0 * (1.0 - cos(ai3[kk] / (double)n3m)) * sdzis * sdzis;
+  for (int64_t kk = 1; kk <= n1mh; ++kk) o->ak1f[kk] = 2.0 * (1.0 - cos(ai1[kk] / (double)n1m)) * sdxis * sdxis;
+  for (int64_t kk = n1m; kk >= n1mh + 1; --kk) o->ak1f[kk] = o->ak1f[n1m + 2 - kk];
+  free(ai3); free(ai1);
+}
+
+/* ctrdiag (poiss_ftft.f90:207-244) on one line j = 1..n: real a, b, c,
+ * complex r in place */
+static void ctrdiag_line(int64_t n, const double *a, const double *b, const double *c, cplx *r) {
+  double *gam = (double *)xcalloc(n + 1, 8);
+  double bet = 1.0 / b[1];
+  r[1] = c_rmul(bet, r[1]);
+  for (int64_t j = 2; j <= n; ++j) {
+    gam[j] = c[j - 1] * bet;
+    bet = 1.0 / (b[j] - a[j] * gam[j]);
+    r[j] = c_rmul(bet, c_sub(r[j], c_rmul(a[j], r[j - 1])));
+  }
+  for (int64_t j = n - 1; j >= 1; --j) r[j] = c_sub(r[j], c_rmul(gam[j + 1], r[j + 1]));
+  free(gam);
+}
+
+/* poisson (poiss_ftft.f90:21-161) */
+static void poisson_ftft(ora *o) {
+  int64_t n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n3mh = o->n3mh;
+  cplx *cc = (cplx *)xcalloc((size_t)n3mh * n1m * n2m, sizeof(cplx));
+#define CF3(k, i, j) cc[((k)-1) + n3mh * (((i)-1) + n1m * (int64_t)((j)-1))]
+  double *re = (double *)xcalloc(n3m > n1m ? n3m : n1m, 8), *im = (double *)xcalloc(n3m > n1m ? n3m : n1m, 8);
+  cplx *zz = (cplx *)xcalloc((size_t)n3m * n1m, sizeof(cplx));
+  for (int64_t j = 1; j <= n2m; ++j) {
+    for (int64_t i = 1; i <= n1m; ++i) {
+      for (int64_t k = 1; k <= n3m; ++k) { re[k - 1] = F(o->divgsum, i, j, k); im[k - 1] = 0.; }
+      ora_fft_forward((int)n3m, re, im);
+      for (int64_t k = 1; k <= n3m; ++k) zz[(k - 1) + n3m * (i - 1)] = (cplx){re[k - 1], im[k - 1]};
+    }
+    for (int64_t k = 1; k <= n3mh; ++k) {
+      for (int64_t i = 1; i <= n1m; ++i) { re[i - 1] = zz[(k - 1) + n3m * (i - 1)].re; im[i - 1] = zz[(k - 1) + n3m * (i - 1)].im; }
+      ora_fft_forward((int)n1m, re, im);
+      for (int64_t i = 1; i <= n1m; ++i) CF3(k, i, j) = (cplx){re[i - 1], im[i - 1]};
+    }
+  }
+  double *a = (double *)xcalloc(n2m + 1, 8), *b = (double *)xcalloc(n2m + 1, 8), *c = (double *)xcalloc(n2m + 1, 8);
+  cplx *r = (cplx *)xcalloc(n2m + 1, sizeof(cplx));
+  for (int64_t k = 1; k <= n3mh; ++k)
+    for (int64_t i = 1; i <= n1m; ++i) {
+      for (int64_t j = 1; j <= n2m; ++j) {
+        int64_t jjp = o->jpv[j];
+        double fl = (double)o->fixjl[j], fu = (double)o->fixju[j];
+        a[j] = o->f2fyi[j] * o->c2cyi[j] * (1.0 - fl);
+        c[j] = o->f2fyi[j] * o->c2cyi[jjp] * (1.0 - fu);
+        b[j] = -((a[j] + c[j] + o->ak1f[i] + o->ak3f[k]) * (1.0 - fl) * (1.0 - fu) +
+                 (o->f2fyi[1] * o->c2cyi[2] + o->ak1f[i] + o->ak3f[k]) * fl +
+                 (o->f2fyi[n2m] * o->c2cyi[n2m] + o->ak1f[i] + o->ak3f[k]) * fu);
+        r[j] = CF3(k, i, j);
+      }
+      /* the mean mode fixes phi at (i, j) = (1, n2m) (poiss_ftft.f90:95-104, 108-114) */
+      const int ref = k == 1 && i == 1;
+      cplx rref = {0., 0.};
+      double bref = 0., aref = 0.;
+      if (ref) {
+        rref = r[n2m]; bref = b[n2m]; aref = a[n2m];
+        r[n2m] = (cplx){0., 0.}; b[n2m] = 1.0; a[n2m] = 0.0; c[n2m] = 0.0;
+      }
+      ctrdiag_line(n2m, a, b, c, r);
+      if (ref) {
+        cplx t = c_add(c_rmul(-aref, r[n2m - 1]), rref);
+        cplx ph = {t.re / bref, t.im / bref};
+        for (int64_t j = 1; j <= n2m; ++j) r[j] = c_sub(r[j], ph);
+        r[n2m] = (cplx){0., 0.};
+      }
+      for (int64_t j = 1; j <= n2m; ++j) CF3(k, i, j) = r[j];
+    }
+  memset(o->phi, 0, NX1 * NY1 * NZ1 * 8);
+  for (int64_t j = 1; j <= n2m; ++j) {
+    for (int64_t k = 1; k <= n3mh; ++k) {
+      for (int64_t i = 1; i <= n1m; ++i) { re[i - 1] = CF3(k, i, j).re; im[i - 1] = CF3(k, i, j).im; }
+      ora_fft_inverse((int)n1m, re, im);
+      for (int64_t i = 1; i <= n1m; ++i) zz[(k - 1) + n3m * (i - 1)] = (cplx){re[i - 1], im[i - 1]};
+    }
+    for (int64_t i = 1; i <= n1m; ++i) {
+      for (int64_t k = n3mh + 1; k <= n3m; ++k) {
+        cplx q = zz[(n3m + 2 - k - 1) + n3m * (i - 1)];
+        zz[(k - 1) + n3m * (i - 1)] = (cplx){q.re, -q.im};
+      }
+      for (int64_t k = 1; k <= n3m; ++k) { re[k - 1] = zz[(k - 1) + n3m * (i - 1)].re; im[k - 1] = zz[(k - 1) + n3m * (i - 1)].im; }
+      ora_fft_inverse((int)n3m, re, im);
+      for (int64_t k = 1; k <= n3m; ++k) F(o->phi, i, j, k) = re[k - 1];
+    }
+  }
+#undef CF3
+  free(cc); free(re); free(im); free(zz); free(a); free(b); free(c); free(r);
+}
+
 static void poisson_any(ora *o) {
   if (o->poiss_kind == 1) {
     o->mg_cycles_max = 0;
     poisson_mgrd(o);
+  } else if (o->poiss_kind == 2) {
+    o->mg_cycles_max = 0;
+    poisson_ftft(o);
   } else {
     poisson(o);
   }
 }
 
 int ora_set_poisson(ora *o, int64_t kind) {
-  if (kind != 0 && kind != 1) return -1;
+  if (kind < 0 || kind > 2) return -1;
+  /* poiss_ftft needs x periodicity; the other two an inflow/outflow x */
+  if ((kind == 2) != (o->s.xprdic == 1)) return -1;
+  if (kind == 2 && !o->ak1f) ftft_init(o);
   if (kind == 1) {
     const int64_t nl = o->s.nlev;
     if (o->s.ioldv != 0 || o->s.xprdic != 0 || nl < 0 || (o->n1m % (1LL << nl)) != 0 ||

@@ -42,6 +42,11 @@ CASES = {
     "cyl32_iread0_trace_4": dict(dims=(32, 32, 8), mode="les_trace_iread0", nend=4),
     # line-SOR (gsor2d) for the wavenumbers above IMGSOR = 2, with its RG rows
     "cyl32_sor_3": dict(dims=(32, 32, 8), mode="les_sor", nend=3),
+    # plane channels, x periodic, with poiss_ftft.f90 (the reference Makefile's
+    # default, oracle/_ref/solver_exec_ftft): constant pressure gradient and
+    # constant flow rate, with their fcmfr.dat rows
+    "chan32_cpg_3": dict(dims=(32, 32, 8), mode="chan_cpg", nend=3, poisson="ftft"),
+    "chan32_cfr_4": dict(dims=(32, 32, 8), mode="chan_cfr", nend=4, poisson="ftft"),
     # the 3-D MG-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90) linked
     # in place of poiss_z_ft.f90 (oracle/_ref/solver_exec_mgrd)
     "mgrd_cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3, poisson="mgrd"),
@@ -81,7 +86,9 @@ def main(names=None):
         root = tempfile.mkdtemp(prefix="lw_gold_")
         C.write_case(C.build_case(spec), root)
         mgrd = c.get("poisson") == "mgrd"
-        oracle.run_reference(root, threads=1, exe=oracle.REF_EXEC_MGRD if mgrd else oracle.REF_EXEC)
+        ftft = c.get("poisson") == "ftft"
+        exe = oracle.REF_EXEC_MGRD if mgrd else oracle.REF_EXEC_FTFT if ftft else oracle.REF_EXEC
+        oracle.run_reference(root, threads=1, exe=exe)
         fl = C.read_fld(os.path.join(root, "output/field/fld%07d" % c["nend"]))
         rows = {}
         for key, fn in (("hist", "fhist"), ("cdcl", "fcdcl"), ("hflux", "fheatflux"), ("temp", "ftemp")):
@@ -91,8 +98,11 @@ def main(names=None):
             # ftrpoittr.dat: time, dt, iterations, residual*dtconst per solve (:221-222)
             rows["bicg_itr"] = text_rows(os.path.join(root, "output/ftr/poiss_itr.dat"))
             rows["bicg_solve"] = text_rows(os.path.join(root, "output/ftr/ftrpoittr.dat"))
-        else:
+        elif not ftft:
             rows["mgres"] = mg_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat"))
+        if spec.settings.ich != 0:
+            # fcmfr.dat: time, qflux, ubulk, pmiavg, cmfravg(1:3) (misc_hist.f90:136)
+            rows["cmfr"] = text_rows(os.path.join(root, "output/ftr/fcmfr.dat"))
         extra = {}
         if spec.settings.trace:
             for q in "uvwp":
@@ -102,7 +112,7 @@ def main(names=None):
             for q in ("u", "v", "w", "p", "t"):
                 extra["init_" + q] = f0[q]
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
-                            u=fl["u"], v=fl["v"], w=fl["w"], p=fl["p"], t=fl["t"],
+                            u=fl["u"], v=fl["v"], w=fl["w"], p=fl["p"], t=fl.get("t", np.zeros(0)),
                             time=fl["time"], dt=fl["dt"], ihist=fl["ihist"], **rows, **extra,
                             meta=json.dumps(dict(c, input_sha256=input_hash(root))))
         print(name, "time", fl["time"], "dvmax", rows["hist"][-1][3])

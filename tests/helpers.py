@@ -18,6 +18,16 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
                        cflfac=0.5)
         return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
+    if mode in ("chan_cpg", "chan_cfr"):
+        # a plane channel: x and z periodic, no-slip walls at y = -1, 1, no
+        # body in the domain (the cylinder of the case builder lies outside
+        # x in [1, 4]); SGS with zero constants, no heat; ICH 2 (constant
+        # pressure gradient -UDRV_I) or 1 (constant flow rate, PI control)
+        ich = 2 if mode == "chan_cpg" else 1
+        s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2, re=180.0,
+                       ihtrans=0, ich=ich, udrv_i=1.0, bc_ybtm=0, bc_ytop=0)
+        return C.CaseSpec(*dims, xlim=(1.0, 4.0), ylim=(-1.0, 1.0), zlen=1.0, noise=1e-2, settings=s,
+                          layout="channel")
     if mode == "nosgs":
         s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2)
     elif mode == "les":
@@ -72,10 +82,15 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
 
 
-def fld_regions(n):
+def fld_regions(n, heat=True):
+    """The parts of the padded fields a fld file holds (misc_hist.f90:236-247);
+    without heat transfer there is no T."""
     n1, n2, n3 = n
-    return {"u": (slice(1, n1 + 1), slice(0, n2 + 1), slice(0, n3 + 1)),
-            "v": (slice(0, n1 + 1), slice(1, n2 + 1), slice(0, n3 + 1)),
-            "w": (slice(0, n1 + 1), slice(0, n2 + 1), slice(1, n3 + 1)),
-            "p": (slice(1, n1), slice(1, n2), slice(1, n3)),
-            "t": (slice(1, n1), slice(1, n2), slice(1, n3))}
+    out = {"u": (slice(1, n1 + 1), slice(0, n2 + 1), slice(0, n3 + 1)),
+           "v": (slice(0, n1 + 1), slice(1, n2 + 1), slice(0, n3 + 1)),
+           "w": (slice(0, n1 + 1), slice(0, n2 + 1), slice(1, n3 + 1)),
+           "p": (slice(1, n1), slice(1, n2), slice(1, n3)),
+           "t": (slice(1, n1), slice(1, n2), slice(1, n3))}
+    if not heat:
+        del out["t"]
+    return out

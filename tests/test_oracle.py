@@ -21,10 +21,15 @@ def test_oracle_matches_reference_golden(name, oracle_lib):
     # the oracle restates the reference operation by operation; only libm
     # calls and the FFT's radix split differ, so a few steps stay within
     # 1e-11 (1e-10 over 100 steps, where the bar for 100 steps is 1e-6)
-    bar = 1e-11 if meta["nend"] <= 10 else 1e-10
-    for k, sl in fld_regions(n).items():
+    # In the x-periodic channels p holds only the small fluctuations (the
+    # mean gradient is pmi), so rounding of the two FFTs weighs more: 1e-10.
+    bar = 1e-11 if meta["nend"] <= 10 and meta.get("poisson") != "ftft" else 1e-10
+    for k, sl in fld_regions(n, z["t"].size > 0).items():
         err = rel(z[k][sl], f[k][sl])
         assert err <= bar, (name, k, err)
+    if "cmfr" in z.files:
+        # fcmfr.dat of the last step: time, qflux, ubulk, pmiavg, cmfravg(1:3)
+        np.testing.assert_allclose(o.array("cmfr"), z["cmfr"][-1], rtol=1e-6, atol=1e-12)
 
 
 @pytest.mark.parametrize("name", [n for n in golden_names() if n.startswith("mgrd_")])
@@ -59,7 +64,7 @@ def test_makefld_matches_reference_initial_field():
     view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])))
     f = view["fields"]
     n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
-    for k, sl in fld_regions(n).items():
+    for k, sl in fld_regions(n, z["t"].size > 0).items():
         assert np.array_equal(z["init_" + k][sl], f[k][sl]), k
 
 
