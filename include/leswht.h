@@ -59,7 +59,9 @@ typedef struct lw_settings {
    * and settings.input does not name it: 0 = poiss_z_ft (z-DFT, then a 2-D
    * multigrid per wavenumber), 1 = poiss_mgrd (BiCGSTAB preconditioned by
    * one 3-D multigrid V-cycle; lw_create_from_dir takes it from the
-   * environment variable LW_POISSON = "mgrd") */
+   * environment variable LW_POISSON = "mgrd"), 2 = poiss_ftft (x and z
+   * DFTs and a y tridiagonal solve; the one for x-periodic grids, which
+   * lw_create_from_dir picks for them) */
   int64_t poisson;
 } lw_settings;
 
@@ -116,6 +118,8 @@ typedef struct lw_history {
   double cd[3], cd_avg[3];
   double hflux[3], hflux_avg[3];
   double s_sgs, s_rhsnlhs, s_poisson, s_energy;
+  /* the channel's fcmfr.dat row (misc_hist.f90:112-136), ICH != 0 */
+  double qflux, ubulk, pmiavg, cmfravg[3];
 } lw_history;
 
 typedef struct lw_handle lw_handle;

@@ -991,6 +991,9 @@ def parse_settings(text: str, boundary: str, prdic: str) -> Settings:
     b.read(1); s.ich, s.iconjg = (int(x) for x in b.read(2))
     p = ListDirected(prdic).read(4)
     s.xprdic, s.yprdic, s.zprdic, s.iintp = (int(x) for x in p)
+    # an x-periodic grid takes poiss_ftft, the reference Makefile's default
+    if s.xprdic == 1:
+        s.poisson = 2
     return s
 
 
@@ -1179,7 +1182,7 @@ def case_view(case: Case) -> dict:
     fl.update(u=u, v=v, w=w, p=p)
     if s.ihtrans:
         fl["t"] = f["t"]
-    st = dataclasses.replace(s, xprdic=xp, yprdic=yp, zprdic=zp, iintp=1)
+    st = dataclasses.replace(s, xprdic=xp, yprdic=yp, zprdic=zp, iintp=1, poisson=2 if xp == 1 else s.poisson)
     if s.iread == 0:
         fields = makefld(st, g, BODIES[case.spec.body], case.spec.body_args)
     else:

@@ -29,6 +29,8 @@ struct Geo {
   // SGS test-filter weights cf?1(n,-1:1) stored [3*i + d + 1]
   const double *cfx1, *cfy1, *cfz1;
   int ib, jb, kb;  // i_bgpx, j_bgpy, k_bgpz
+  int xprdic;      // 1: x periodic (the channel)
+  int ich;         // channel driving: 0 none, 1 constant flow rate, 2 constant pressure gradient
   int adi_wave;    // sweeps with whole lines in registers: bit 1 y, bit 2 x, bit 4 z
 };
 

@@ -193,8 +193,9 @@ void read_settings(Case &c, const std::string &dir) {
   s.xprdic = b[0]; s.yprdic = b[1]; s.zprdic = b[2]; s.iintp = b[3];
   // the Poisson solver is the reference's link-time choice (Makefile:23-34),
   // named by no input file: LW_POISSON = mgrd selects poiss_mgrd
+  // and an x-periodic grid takes poiss_ftft, the Makefile's default
   const char *ps = getenv("LW_POISSON");
-  s.poisson = (ps && strcmp(ps, "mgrd") == 0) ? 1 : 0;
+  s.poisson = s.xprdic == 1 ? 2 : (ps && strcmp(ps, "mgrd") == 0) ? 1 : 0;
 }
 
 template <class T>
@@ -332,7 +333,27 @@ void read_field(Case &c, const std::string &dir) {
     unpack(recs[9], c.t, 1, c.n1m, 1, c.n2m, 1, c.n3m);
   }
   if (s.ipzero == 1) std::fill(c.p.begin(), c.p.end(), 0.0);
-  // periodic ghosts (misc_init.f90:653-783); only z is in scope
+  // periodic ghosts (misc_init.f90:653-783), x before z; y-periodic grids
+  // are out of scope
+  if (s.xprdic == 1) {
+    for (int64_t k = 0; k <= c.n3; ++k)
+      for (int64_t j = 0; j <= c.n2; ++j) {
+        c.u[P(0, j, k)] = c.u[P(c.n1m, j, k)];
+        c.u[P(c.n1, j, k)] = c.u[P(1, j, k)];
+        if (j >= 1) {
+          c.v[P(0, j, k)] = c.v[P(c.n1m, j, k)];
+          c.v[P(c.n1, j, k)] = c.v[P(1, j, k)];
+        }
+        if (k >= 1) {
+          c.w[P(0, j, k)] = c.w[P(c.n1m, j, k)];
+          c.w[P(c.n1, j, k)] = c.w[P(1, j, k)];
+        }
+        if (s.ihtrans == 1) {
+          c.t[P(0, j, k)] = c.t[P(c.n1m, j, k)];
+          c.t[P(c.n1, j, k)] = c.t[P(1, j, k)];
+        }
+      }
+  }
   if (s.zprdic == 1) {
     for (int64_t j = 0; j <= c.n2; ++j)
       for (int64_t i = 0; i <= c.n1; ++i) {

@@ -95,8 +95,11 @@ __global__ void k_rhs_u(Geo g, Sub sb, Fld f, int les) {
     rk3x = rk3x + 1.0 * (rhsu2 + rhsu3);
   }
   const double ro = f.rk3o[0][c];
+  // the channel's mean pressure gradient pmi(0) (meanpg) when ICH != 0
+  const double pmi0 = g.ich != 0 ? f.chan[0] : 0.0;
   f.rhs[0][c] = sb.dt * (sb.gam * rk3x + sb.ro * ro + 2. * sb.alpha * alx -
-                         2. * sb.alpha * (p[c] - p[c - dx]) * g.c2cxi[i] - 2. * sb.alpha * 0.0 * 0.0);
+                         2. * sb.alpha * (p[c] - p[c - dx]) * g.c2cxi[i] -
+                         2. * sb.alpha * pmi0 * (double)(g.ich != 0 ? 1 : 0));
   f.rk3oo[0][c] = ro;
   f.rk3o[0][c] = rk3x;
 }
@@ -437,8 +440,23 @@ static void launch_obc_flux(hipStream_t s, const Geo &g, const Fld &f) {
   LW_LAUNCH("k_obc_flux", b, 0, k_obc_flux, dim3(kObc), dim3(kObcT), 0, s, g, f);
 }
 
+// constant flow rate (ICH = 1): no outlet, its velocities are zero
+// (slv_mmtm.f90:980-984)
+__global__ void k_obc_zero(Geo g, Fld f) {
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= (long)g.NY * g.NZ) return;
+  f.uout[q] = 0.0;
+  f.vout[q] = 0.0;
+  f.wout[q] = 0.0;
+}
+
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &, int les) {
   const double pl = (double)g.n2m * g.n3m;
+  if (g.ich == 1) {
+    const long n = (long)g.NY * g.NZ;
+    LW_LAUNCH("k_obc_zero", n * 24.0, 0, k_obc_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, f);
+    return;
+  }
   launch_obc_flux(s, g, f);
   LW_LAUNCH("k_obc_out_mom", pl * 8.0 * 10, 0, k_obc_out_mom, dim3(kObc), dim3(kObcT), 0, s, g, sb, f);
   LW_LAUNCH("k_obc_apply_mom", pl * 8.0 * 20, 0, k_obc_apply_mom, dim3(kObc), dim3(kObcT), 0, s, g, sb, f, les);
@@ -682,6 +700,78 @@ __device__ inline void adi_z(const Geo &g, const Sub &sb, const Fld &f, double c
       if (kb - u < 1) break;
       r[q0 + (long)(kb - u) * st] = fnn * sv[u] + rv[u];
     }
+  }
+}
+
+// periodic x line (trdiag1p, slv_mmtm.f90:1501-1570: the recurrences of
+// trdiag3p along x) of the x-periodic channel; field += solution. One thread
+// per (j, k) line in the reference's order, q and s in the scratch arrays,
+// qe in place of the right-hand side. x is the contiguous direction, so the
+// threads' accesses are a row apart: this path is not tuned.
+template <int C>
+__device__ inline void adi_xp(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+  const int jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
+  const int nj = g.n2m - jlo + 1;
+  const int line = blockIdx.x * 64 + threadIdx.x;
+  if (line >= nj * (g.n3m - klo + 1)) return;
+  const int j = jlo + line % nj, k = klo + line / nj;
+  const int n = g.n1m;
+  const LineCf l = line_cf<0, C>(g, f, sb, cre, cre2);
+  const long q0 = P(g, 0, j, k);
+  double *r = f.rhs[C], *fld = C == 0 ? f.u : C == 1 ? f.v : f.w;
+  double *sq = f.sa[C % 3], *ss = f.sb[C % 3];
+  const double fn = l.bd * r[q0 + n];
+  double qp, sp, fp, an = 0, bn = 0, cn = 0;
+  {
+    const long q = q0 + 1;
+    const double a = l.am[1] * (1. + l.cc * l.vis[q + l.lo]);
+    const double c = l.cm[1] * (1. + l.cc * l.vis[q + l.lo + 1]);
+    const double b = l.bd - a - c;
+    const double binv = 1. / b;
+    qp = -c * binv;
+    sp = -a * binv;
+    fp = (l.bd * r[q]) * binv;
+    sq[q] = qp;
+    ss[q] = sp;
+    r[q] = fp;
+  }
+  for (int i = 2; i <= n; ++i) {
+    const long q = q0 + i;
+    const double a = l.am[i] * (1. + l.cc * l.vis[q + l.lo]);
+    const double c = l.cm[i] * (1. + l.cc * l.vis[q + l.lo + 1]);
+    const double b = l.bd - a - c;
+    const double pn = 1. / (b + a * qp);
+    qp = -c * pn;
+    sp = -a * sp * pn;
+    fp = (l.bd * r[q] - a * fp) * pn;
+    sq[q] = qp;
+    ss[q] = sp;
+    r[q] = fp;
+    an = a;
+    bn = b;
+    cn = c;
+  }
+  double snext = 1.0, qenext = 0.0;
+  for (int i = n - 1; i >= 1; --i) {
+    const long q = q0 + i;
+    const double sk = ss[q] + sq[q] * snext;
+    const double qek = r[q] + sq[q] * qenext;
+    ss[q] = sk;
+    r[q] = qek;
+    snext = sk;
+    qenext = qek;
+  }
+  const double fnn = (fn - cn * r[q0 + 1] - an * r[q0 + n - 1]) / (cn * ss[q0 + 1] + an * ss[q0 + n - 1] + bn);
+  fld[q0 + n] = fnn + fld[q0 + n];
+  for (int i = n - 1; i >= 1; --i) fld[q0 + i] = (fnn * ss[q0 + i] + r[q0 + i]) + fld[q0 + i];
+}
+
+template <int C0>
+__global__ __launch_bounds__(64) void k_adi_xp(Geo g, Sub sb, Fld f, double cre, double cre2) {
+  switch (blockIdx.y) {
+    case 0: adi_xp<0>(g, sb, f, cre, cre2); break;
+    case 1: adi_xp<1>(g, sb, f, cre, cre2); break;
+    default: adi_xp<2>(g, sb, f, cre, cre2); break;
   }
 }
 
@@ -1220,7 +1310,10 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   else
     LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, k_adi_y<C0>, grid3(g.n1m, g.n3m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
   const int xlines = g.n2m * g.n3m;
-  if (g.adi_wave & 2)
+  if (g.xprdic && C0 != 3)
+    LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_xp<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64),
+              0, s, g, sb, f, cre, cre2);
+  else if (g.adi_wave & 2)
     LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, pick_adi_xw<C0>(g.n1m),
               dim3((unsigned)((xlines + kXWL - 1) / kXWL), (unsigned)nc), dim3(64, kXWL), 0, s, g, sb, f, cre, cre2);
   else
@@ -1268,9 +1361,44 @@ __global__ void k_zghost_uvw(Geo g, Fld f) {
   f.w[hi] = f.w[b];
 }
 
+// x-periodic ghost planes (prdic_adj_uvw, slv_mmtm.f90:1755-1786; ucalc,
+// slv_auxi.f90:388-412): u over j 0..n2, v over j 1..n2 (k 0..n3), w over
+// k 1..n3 (j 0..n2)
+__global__ void k_xghost_uvw(Geo g, Fld f) {
+  const int j = blockIdx.x * 64 + threadIdx.x;  // 0..n2
+  const int k = blockIdx.y;                     // 0..n3
+  if (j > g.n2) return;
+  const long lo = P(g, 0, j, k), hi = P(g, g.n1, j, k), a = P(g, g.n1m, j, k), b = P(g, 1, j, k);
+  f.u[lo] = f.u[a];
+  f.u[hi] = f.u[b];
+  if (j >= 1) {
+    f.v[lo] = f.v[a];
+    f.v[hi] = f.v[b];
+  }
+  if (k >= 1) {
+    f.w[lo] = f.w[a];
+    f.w[hi] = f.w[b];
+  }
+}
+
+static void launch_xghost_uvw(hipStream_t s, const Geo &g, const Fld &f) {
+  LW_LAUNCH("k_xghost_uvw", (double)(g.n2 + 1) * (g.n3 + 1) * 8.0 * 12, 0, k_xghost_uvw,
+            dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);
+}
+
+// pcalc's x-periodic ghosts of p (slv_auxi.f90:480-489)
+__global__ void k_xghost_p(Geo g, Fld f) {
+  const int j = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int k = 1 + blockIdx.y;
+  if (j > g.n2m) return;
+  f.p[P(g, 0, j, k)] = f.p[P(g, g.n1m, j, k)];
+  f.p[P(g, g.n1, j, k)] = f.p[P(g, 1, j, k)];
+}
+
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f) {
   LW_LAUNCH("k_retrv_mom", (double)(g.n2 + 1) * (g.n3 + 1) * 8.0 * 9, 0, k_retrv_mom,
             dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);
+  if (g.xprdic) launch_xghost_uvw(s, g, f);
   LW_LAUNCH("k_zghost_uvw", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 12, 0, k_zghost_uvw,
             dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
 }
@@ -1399,8 +1527,12 @@ void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f)
             f);
   LW_LAUNCH("k_zghost_uvw", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 12, 0, k_zghost_uvw,
             dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+  if (g.xprdic) launch_xghost_uvw(s, g, f);
   LW_LAUNCH("k_zghost_p", (double)g.n1m * g.n2m * 8.0 * 4, 0, k_zghost_p, dim3((g.n1m + 63) / 64, g.n2m), dim3(64),
             0, s, g, f);
+  if (g.xprdic)
+    LW_LAUNCH("k_xghost_p", (double)g.n2m * g.n3m * 8.0 * 4, 0, k_xghost_p, dim3((g.n2m + 63) / 64, g.n3m), dim3(64),
+              0, s, g, f);
 }
 
 // ---------------------------------------------------------------------------

@@ -738,6 +738,7 @@ __global__ void k_zghost_scalar(Geo g, double *a) {
 struct EdgeCtx {
   int ib, jb, kb;
   int ybwall, ytwall;
+  int xprdic;
 };
 
 __device__ inline double edge_main(const Geo &g, const double *nu, int i, int j, int k, int c) {
@@ -766,8 +767,11 @@ __device__ double edge_234(const Geo &g, const double *nu, const EdgeCtx &e, int
   else inmain = k >= 1 && k <= n3m && j >= e.jb && j <= n2m && i >= e.ib && i <= n1m;
   if (inmain) v = edge_main(g, nu, i, j, k, c);
   const double *fx = g.f2fx, *fy = g.f2fy, *fz = g.f2fz;
-  // x boundaries (inflow/outflow, never walls)
-  if ((i == 1 || i == n1) && c >= 1 && j >= e.jb && j <= n2m && k >= e.kb && k <= n3m) {
+  // x boundaries: the periodic copy of line 1 into n1 (sgs.f90:1032-1036),
+  // or inflow/outflow (never walls)
+  if (e.xprdic) {
+    if (i == n1 && j >= e.jb && j <= n2m && k >= e.kb && k <= n3m) v = edge_main(g, nu, 1, j, k, c);
+  } else if ((i == 1 || i == n1) && c >= 1 && j >= e.jb && j <= n2m && k >= e.kb && k <= n3m) {
     const int jm = g.jmv[j], km = g.kmv[k];
     const int is = i == 1 ? 2 : n1m;
     if (c == 1) v = g.c2czi[k] * 0.5 * (fz[k] * N(is, j, km) + fz[km] * N(is, j, k));
@@ -804,10 +808,22 @@ __global__ void k_edges(Geo g, const double *nu, double *o1, double *o2, double 
   }
 }
 
+// x-periodic ghosts of a cell array (nutinterpol, sgs.f90:960-967)
+__global__ void k_xghost_scalar(Geo g, double *a) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (j > g.n2) return;
+  a[P(g, 0, j, k)] = a[P(g, g.n1m, j, k)];
+  a[P(g, g.n1, j, k)] = a[P(g, 1, j, k)];
+}
+
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1, int alp,
                        int scale_slot, const lw_settings &st) {
   LW_LAUNCH("k_scale_zero", pts(g) * 17.0, 0, k_scale_zero, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, f, nu,
             scale_slot);
+  if (g.xprdic)
+    LW_LAUNCH("k_xghost_scalar", (double)(g.n2 + 1) * (g.n3 + 1) * 32.0, 0, k_xghost_scalar,
+              dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, nu);
   LW_LAUNCH("k_zghost_scalar", (double)(g.n1 + 1) * (g.n2 + 1) * 32.0, 0, k_zghost_scalar,
             dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, nu);
   EdgeCtx e;
@@ -816,6 +832,7 @@ void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, do
   e.kb = alp ? 1 : g.kb;
   e.ybwall = st.bc_ybtm == 0;
   e.ytwall = st.bc_ytop == 0;
+  e.xprdic = g.xprdic;
   // nu in, three edge viscosities out
   LW_LAUNCH("k_edges", (double)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1) * 32.0, 0, k_edges,
             grid3(g.n1 + 1, g.n2 + 1, g.n3 + 1), BLK3, 0, s, g, nu, nu1[0], nu1[1], nu1[2], e);

@@ -45,6 +45,10 @@ struct Fld {
   double *obc;            // outlet-plane reduction partials (8 * 64)
   double *dpart;          // per-step diagnostic partials (D_NSUM * 512)
   unsigned long long *redmax;  // [0] cfl, [1] dvmax, [2] qmmax
+  // channel (ICH != 0): scalars (see lw_kern_chan.hip), block partials, and
+  // 1 per interior u point that funcbody counts as fluid
+  double *chan, *chanpart;
+  uint8_t *umask;
 };
 
 struct Body {
@@ -132,6 +136,22 @@ void launch_mg_zero(hipStream_t s, const Geo &g, const MG &mg);
 // toplevel (poiss_z_ft.f90:302-380), id 0 or 1, ending with the per-wavenumber
 // convergence test against `test`
 void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, double oldv, double test, bool check);
+// the x-periodic channel (lw_kern_chan.hip): meanpg, the substep sum of
+// pmi, the fcmfr.dat quantities, and poiss_ftft
+void launch_meanpg(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st, double dtconst,
+                   int ctrl);
+void launch_pmiavg(hipStream_t s, const Fld &f, double alpha, int reset);
+void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const lw_settings &st, double *hist, double time,
+                      double dt);
+struct Ftft {
+  int n3mh;
+  const double *twx_r, *twx_i;  // e^{-2 pi i q / n1m}, q = 0..n1m-1
+  const double *twz_r, *twz_i;  // e^{-2 pi i q / n3m}
+  const double *ak1, *ak3;      // modified wavenumbers (1-based, poisinit of poiss_ftft.f90:164-204)
+  double2 *a, *b;               // spectra [kz][j][kx], n3mh * n2m * n1m
+  double *gam;                  // the line solve's gam, same shape
+};
+void launch_poisson_ftft(hipStream_t s, const Geo &g, const Fld &f, const Ftft &t);
 bool mg_folds_check(const MG &mg);
 // kv = 1 of the two-stream solve in one launch (k_mg_kv1): whether the
 // tiles and levels of `mg` (kv0 = 1, nkv = 1) allow it, and the launch of up

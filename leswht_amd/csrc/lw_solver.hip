@@ -218,6 +218,8 @@ struct lw_handle {
   double hs_rfin[4] = {0};
   // poiss_mgrd: the solver's state and the log of the last step
   M3 m3{};
+  // poiss_ftft: tables and spectra
+  Ftft ftft{};
   std::vector<int64_t> bicg_sub, bicg_icg, bicg_loc, solve_icg;
   std::vector<double> bicg_rm, solve_rm;
   // tracer (misc_hist.f90:184-209): cells whose centre values are traced
@@ -320,6 +322,7 @@ static std::vector<int> vi(const int64_t *src, int64_t lo, int64_t hi) {
 // At a level's first and last line the reference multiplies a quotient by
 // 0 (iwest, ieast, kback, kforw) whose divisor holds a centre of another
 // level or one not yet set; that coefficient is 0 here.
+static void setup_ftft(lw_handle *h, const lw_grid *gr);
 static void setup_mgrd(lw_handle *h, const lw_grid *gr) {
   const lw_settings &st = h->st;
   const Geo &g = h->g;
@@ -542,6 +545,8 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   g.fixiu = h->upload(fixiu); g.fixju = h->upload(fixju); g.fixku = h->upload(fixku);
   // lhsinit (slv_mmtm.f90:180-249)
   g.ib = st.xprdic == 1 ? 1 : 2;
+  g.xprdic = st.xprdic == 1 ? 1 : 0;
+  g.ich = (int)st.ich;
   g.jb = st.yprdic == 1 ? 1 : 2;
   g.kb = st.zprdic == 1 ? 1 : 2;
   // 32 zero entries past the end: the chunked line sweeps read up to two
@@ -690,6 +695,20 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     for (int j = 0; j <= n2; ++j)
       if (funcbody(h->body, h->xmp[0], h->ymp[j], h->zmp[k]) <= 1.e-10) mask[PIh(0, j, k)] |= 128;
   f.mask = h->upload(mask);
+  if (st.ich != 0) {
+    // meanpg's and writehistory's fluid test at the u points (x(i), ymp(j),
+    // zmp(k)) of the interior, i fastest
+    std::vector<uint8_t> um((size_t)n1m * n2m * n3m, 0);
+    for (int k = 1; k <= n3m; ++k)
+      for (int j = 1; j <= n2m; ++j)
+        for (int i = 1; i <= n1m; ++i)
+          um[(size_t)(i - 1) + (size_t)n1m * ((j - 1) + (size_t)n2m * (k - 1))] =
+              funcbody(h->body, h->x[i], h->ymp[j], h->zmp[k]) >= 1.e-10 ? 1 : 0;
+    f.umask = h->upload(um);
+    f.chan = h->dalloc<double>(32);
+    f.chanpart = h->dalloc<double>(4 * 64);
+    HIPCHK(hipMemsetAsync(f.chan, 0, 32 * sizeof(double), h->s));
+  }
   // fluid tests of the boundary planes used by draglift's wall terms and by
   // heatflux (slv_auxi.f90:590-640, 733-829): funcbody >= 1e-10
   {
@@ -731,6 +750,11 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
 
   if (st.poisson == 1) {
     setup_mgrd(h, gr);
+    HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
+    return;
+  }
+  if (st.poisson == 2) {
+    setup_ftft(h, gr);
     HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
     return;
   }
@@ -911,14 +935,67 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
   mg.twi = h->upload(twi);
 }
 
+// poisinit of poiss_ftft.f90 (164-204): the modified wavenumbers by the
+// reference's expressions, the twiddle tables e^{-2 pi i q/N} of both
+// transforms, and the spectra of the solve
+static void setup_ftft(lw_handle *h, const lw_grid *gr) {
+  Ftft &t = h->ftft;
+  const int n1m = (int)h->g.n1m, n2m = (int)h->g.n2m, n3m = (int)h->g.n3m;
+  const int n3mh = n3m / 2 + 1, n1mh = n1m / 2 + 1;
+  t.n3mh = n3mh;
+  const double pi = 2.0 * asin(1.0);
+  std::vector<double> ak3(n3mh + 2, 0.0), ak1(n1m + 2, 0.0);
+  const double sdzis = gr->f2fzi[1], sdxis = gr->f2fxi[1];
+  for (int kk = 1; kk <= n3mh; ++kk) {
+    const double ai = kk == 1 ? 0.0 : (double)(kk - 1) * 2.0 * pi;
+    ak3[kk] = 2.0 * (1.0 - cos(ai / (double)n3m)) * sdzis * sdzis;
+  }
+  for (int kk = 1; kk <= n1mh; ++kk) {
+    const double ai = kk == 1 ? 0.0 : (double)(kk - 1) * 2.0 * pi;
+    ak1[kk] = 2.0 * (1.0 - cos(ai / (double)n1m)) * sdxis * sdxis;
+  }
+  for (int kk = n1m; kk >= n1mh + 1; --kk) ak1[kk] = ak1[n1m + 2 - kk];
+  auto table = [](int n, std::vector<double> &re, std::vector<double> &im) {
+    re.resize(n);
+    im.resize(n);
+    const double pi2 = 8.0 * atan(1.0);
+    for (int q = 0; q < n; ++q) {
+      const double ang = -pi2 * (double)q / (double)n;
+      re[q] = cos(ang);
+      im[q] = sin(ang);
+    }
+  };
+  std::vector<double> xr, xi, zr, zi;
+  table(n1m, xr, xi);
+  table(n3m, zr, zi);
+  t.twx_r = h->upload(xr);
+  t.twx_i = h->upload(xi);
+  t.twz_r = h->upload(zr);
+  t.twz_i = h->upload(zi);
+  t.ak1 = h->upload(ak1);
+  t.ak3 = h->upload(ak3);
+  const size_t ns = (size_t)n3mh * n2m * n1m;
+  t.a = h->dalloc<double2>(ns);
+  t.b = h->dalloc<double2>(ns);
+  t.gam = h->dalloc<double>(ns);
+}
+
 static void check_scope(const lw_settings *s, const lw_grid *g) {
   auto bad = [](const char *m) {
     g_err = m;
     throw LW_ERR_ARG;
   };
-  if (s->xprdic || s->yprdic) bad("x- or y-periodic grids are not supported (poiss_z_ft path, inflow/outflow in x)");
+  if (s->yprdic) bad("y-periodic grids are not supported");
+  // the x-periodic channel takes poiss_ftft (the reference Makefile's
+  // choice for it); inflow/outflow grids the other two solvers
+  if (s->xprdic && s->poisson != 2) bad("an x-periodic grid needs the poiss_ftft solver (poisson = 2)");
+  if (!s->xprdic && s->poisson == 2) bad("poiss_ftft needs an x-periodic grid");
+  if (s->xprdic && s->ihtrans == 1) bad("heat transfer on an x-periodic grid is not supported");
+  if (s->xprdic && s->iles == 1 && s->idvmon == 1)
+    bad("the dynamic SGS coefficients on an x-periodic grid are not supported");
+  if (s->ich < 0 || s->ich > 2) bad("ICH must be 0, 1 or 2");
+  if (s->ich != 0 && !s->xprdic) bad("ICH != 0 (channel driving) needs an x-periodic grid");
   if (s->zprdic != 1) bad("the z direction must be periodic (the step's z sweeps and poiss_z_ft)");
-  if (s->ich != 0) bad("ICH != 0 (channel driving) is not supported");
   if (s->iconjg) bad("ICONJG = 1 (conjugate heat transfer) is not supported");
   if (s->imovingon) bad("IMOVING = 1 is not supported");
   if (s->idecomp) bad("IDECOMP = 1 is not supported");
@@ -931,7 +1008,7 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
   if (g->n1m % 2 != 0) bad("n1m must be even");
   if (g->n3m < 2) bad("n3m must be at least 2");
   if (s->nbli < 1) bad("NBLI must be >= 1");
-  if (s->poisson != 0 && s->poisson != 1) bad("poisson must be 0 (poiss_z_ft) or 1 (poiss_mgrd)");
+  if (s->poisson < 0 || s->poisson > 2) bad("poisson must be 0 (poiss_z_ft), 1 (poiss_mgrd) or 2 (poiss_ftft)");
   if (s->poisson == 1) {
     // poiss_mgrd coarsens z as well; with IOLDV = 1 its V-cycle starts from
     // an uninitialised local array (poiss_mgrd.f90:236, 239)
@@ -942,7 +1019,7 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
     // neighbouring lines in one zebra half; the reference's result then
     // depends on the order its loop visits them
     if (nz0 > 1 && nz0 % 2 == 1) bad("poiss_mgrd: n3m / 2**NLEV must be 1 or even");
-  } else {
+  } else if (s->poisson == 0) {
     // kv = 1, the singular mean mode, needs the multigrid's gauge shift
     if (s->imgsor < 1) bad("IMGSOR must be at least 1");
     if (s->imgsor < g->n3m / 2 + 1 && (g->n2m + 63) / 64 > 8)
@@ -1132,6 +1209,9 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
     // poiss_mgrd: gi, resid and the pivots over the stacked levels, and the
     // seven BiCGSTAB vectors
     b += (2 * n1m + 1) * (n2m + 2) * (2 * n3m + 1) * 8 + 2 * (2 * n1m) * n2m * (2 * n3m) * 8 + 7 * np * 8;
+  } else if (s->poisson == 2) {
+    // poiss_ftft: two complex spectra and the solve's gam
+    b += (n3m / 2 + 1) * n2m * n1m * (2 * 16 + 8) + ni;
   } else {
     const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
     const int64_t n3mh = n3m / 2 + 1;
@@ -1167,6 +1247,15 @@ int lw_set_fields(lw_handle *h, const double *u, const double *v, const double *
       h->time = 0.;
     }
     h->ntime = 0;
+    // solver.f90:54: the channel's first mean pressure gradient, from the
+    // loaded field (ntime 0: no controller term); the controller's integral
+    // and the fcmfr.dat sums start from zero
+    if (h->g.ich != 0) {
+      HIPCHK(hipMemsetAsync(h->f.chan, 0, 32 * sizeof(double), h->s));
+      launch_meanpg(h->s, h->g, h->f, h->ib[0], h->st, 0.0, 0);
+      check_launches("meanpg");
+      HIPCHK(hipStreamSynchronize(h->s));
+    }
     return LW_OK;
   } catch (int code) {
     return code;
@@ -1194,6 +1283,19 @@ int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n) {
   const int64_t np = (int64_t)h->g.NX * h->g.NY * h->g.NZ;
   const double *src = nullptr;
   std::string s(name);
+  if (s == "pmi") {
+    // the channel's mean pressure gradient pmi(0:4) (meanpg)
+    if (h->g.ich == 0) return fail(LW_ERR_ARG, "pmi exists with ICH != 0 only");
+    if (!out) return 5;
+    if (n < 5) return fail(LW_ERR_ARG, "output buffer too small");
+    try {
+      HIPCHK(hipMemcpyAsync(out, h->f.chan, 5 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+      HIPCHK(hipStreamSynchronize(h->s));
+    } catch (int code) {
+      return code;
+    }
+    return 5;
+  }
   if (s == "nusgs") src = h->f.nu;
   else if (s == "alsgs") src = h->f.al;
   else if (s == "divgsum") src = h->f.div;
@@ -1303,6 +1405,16 @@ static void substepinit(lw_handle *h, int ms) {
   sb.acoefi = 1. / sb.acoef;
   sb.re = h->st.re;
   sb.pr = h->st.pr;
+  // pmiavg = pmiavg + pmi(0)*2*alpha (slv_auxi.f90:118), restarted by
+  // stepinit (:13)
+  if (h->g.ich != 0) launch_pmiavg(h->s, h->f, sb.alpha, ms == 1);
+}
+
+// after pcalc (solver.f90:136): the channel's mean pressure gradient for the
+// next substep, with the controller term inside a step (ntime > 0, dt > 0)
+static void after_projection(lw_handle *h) {
+  if (h->g.ich != 0)
+    launch_meanpg(h->s, h->g, h->f, h->ib[0], h->st, h->sub.dtconst, (int)(h->ntime > 0 && h->dt > 0.0));
 }
 
 static void phase_sgs(lw_handle *h) {
@@ -1571,6 +1683,11 @@ static void phase_poisson(lw_handle *h) {
     phase_poisson_mgrd(h);
     return;
   }
+  if (h->st.poisson == 2) {
+    // poiss_ftft is direct: no cycles, no residual rows
+    launch_poisson_ftft(h->s, h->g, h->f, h->ftft);
+    return;
+  }
   const Geo &g = h->g;
   MG &mg = h->mg;
   launch_fft_forward(h->s, g, mg, h->f);
@@ -1694,6 +1811,8 @@ static void end_of_step(lw_handle *h) {
   launch_step_diag(h->s, h->g, h->f, h->ib, h->dplanes, act, st.re);
   launch_hist_update(h->s, h->f, h->hist, act, st.ibmon == 1, h->heat, hstep, h->time, h->dt, st.avg_tst,
                      h->g.xl * h->g.yl * h->g.zl);
+  // writehistory's fcmfr.dat quantities (misc_hist.f90:112-136)
+  if (h->g.ich != 0 && hstep) launch_chan_hist(h->s, h->g, h->f, st, h->hist, h->time, h->dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1795,9 +1914,9 @@ static void field_avg(lw_handle *h) {
 static void read_stats(lw_handle *h, lw_step_stats *stats) {
   HIPCHK(hipMemcpyAsync(h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipMemcpyAsync(h->h_red, h->f.red, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-  HIPCHK(hipMemcpyAsync(h->h_red + 16, h->hist, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(h->h_red + 16, h->hist, 24 * sizeof(double), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
-  memcpy(h->h_hist, h->h_red + 16, 16 * sizeof(double));
+  memcpy(h->h_hist, h->h_red + 16, 24 * sizeof(double));
   for (int q = 0; q < 4; ++q) {
     float a = 0.f, b = 0.f, c = 0.f;
     if (h->tev_valid) {
@@ -1854,7 +1973,7 @@ extern "C" int lw_autotune(lw_handle *h, int64_t reps, lw_tune_result *out) {
       out->tile1[q] = mg.tile1[q];
     }
     // separate launches, or the 3-D solver: no tiles to tune
-    if (!mg.fused || h->st.poisson == 1) return LW_OK;
+    if (!mg.fused || h->st.poisson != 0) return LW_OK;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     const double dt0 = h->dt;
@@ -2063,6 +2182,7 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
         check_launches("divgs/poisson");
         HIPCHK(hipEventRecord(ev[5], h->s));
         launch_projection(h->s, h->g, h->sub, h->f);
+        after_projection(h);
         if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
         check_launches("ucalc/pcalc");
         HIPCHK(hipEventRecord(ev[6], h->s));
@@ -2104,6 +2224,7 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
     else if (phase == 4) phase_poisson(h);
     else if (phase == 5) {
       launch_projection(h->s, h->g, h->sub, h->f);
+      after_projection(h);
       if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
     } else if (phase == 6) phase_energy(h);
     else if (phase == 7) {
@@ -2144,6 +2265,10 @@ extern "C" int lw_get_history(const lw_handle *h, lw_history *out) {
   out->s_rhsnlhs = h->s_phase[1];
   out->s_poisson = h->s_phase[2];
   out->s_energy = h->s_phase[3];
+  out->qflux = h->h_hist[16];
+  out->ubulk = h->h_hist[17];
+  out->pmiavg = h->h_hist[18];
+  for (int l = 0; l < 3; ++l) out->cmfravg[l] = h->h_hist[19 + l];
   return LW_OK;
 }
 

@@ -36,6 +36,7 @@ module leswht_c
     real(c_double) :: cd(3), cd_avg(3)
     real(c_double) :: hflux(3), hflux_avg(3)
     real(c_double) :: s_sgs, s_rhsnlhs, s_poisson, s_energy
+    real(c_double) :: qflux, ubulk, pmiavg, cmfravg(3)
   end type lw_history
 
   ! body level set (lw_body): kind 0 = circular cylinder of radius p(1)

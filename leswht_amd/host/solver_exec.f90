@@ -8,7 +8,8 @@
 ! the grid file, the initial field; misc_init.f90:19-786) and writes the same
 ! outputs: ../output/field/fld%07d at every NPRINT steps and at the end
 ! (solver.f90:171,185; misc_hist.f90:212-252) and the history files
-! ../output/ftr/{fhist,ftemp,fcdcl,fheatflux,ftime}.dat, the tracer files
+! ../output/ftr/{fhist,ftemp,fcdcl,fheatflux,ftime}.dat, fcmfr.dat of a
+! driven channel (ICH /= 0), the tracer files
 ! f{u,v,w,p}trace.dat and mgftresiduemax.dat with the reference's records and
 ! formats (misc_hist.f90:2-209, 255-284; slv_auxi.f90:573-840;
 ! poiss_z_ft.f90:49,293-297). With IREAD = 0 it makes the initial field
@@ -82,7 +83,7 @@ program solver_exec
     allocate (bsub(maxit), bicg(maxit), bloc(3 * maxit), brm(maxit))
     open (77, file='../output/ftr/poiss_itr.dat')
     open (78, file='../output/ftr/ftrpoittr.dat')
-  else
+  else if (st%poisson == 0) then
     allocate (rkind(64), rkv(64), rkk(64), rrm(64))
     open (77, file='../output/ftr/mgftresiduemax.dat')
   end if
@@ -106,7 +107,7 @@ program solver_exec
       do q = 1, min(nres, 3_c_int64_t)
         write (78, 103) t_begin, hs%dt, sicg(q), srm(q)
       end do
-    else
+    else if (st%poisson == 0) then
       ! mg2d's rows, then gsor2d's per sweep (IMGSOR < n3m/2+1)
       nres = lw_get_residual_rows(h, rkind, rkv, rkk, rrm, 0_c_int64_t)
       if (nres > size(rkv)) then
@@ -139,6 +140,8 @@ program solver_exec
         hs%cd_avg(1), hs%cd_avg(2), hs%cd_avg(3)
       if (heat) write (2002, 140) hs%time, hs%hflux(1), hs%hflux(2), hs%hflux(3), &
         hs%hflux_avg(1), hs%hflux_avg(2), hs%hflux_avg(3)
+      if (st%ich /= 0) write (2010, 140) hs%time, hs%qflux, hs%ubulk, hs%pmiavg, &
+        hs%cmfravg(1), hs%cmfravg(2), hs%cmfravg(3)
     end if
     if (ntrace > 0) then
       if (mod(hs%ntime, ntr) == 0) then
@@ -202,6 +205,7 @@ contains
       open (2007, file='../output/ftr/fwtrace.dat', position=trim(pos))
       open (2008, file='../output/ftr/fptrace.dat', position=trim(pos))
     end if
+    if (st%ich /= 0) open (2010, file='../output/ftr/fcmfr.dat', position=trim(pos))
   end subroutine open_histories
 
   subroutine close_histories()
@@ -210,8 +214,9 @@ contains
     close (2002)
     close (2012)
     close (2011)
-    close (77)
+    if (st%poisson /= 2) close (77)
     if (st%poisson == 1) close (78)
+    if (st%ich /= 0) close (2010)
     if (ntrace /= 0) then
       close (2005)
       close (2006)

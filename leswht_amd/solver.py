@@ -78,7 +78,8 @@ class lw_history(C.Structure):
         [(n, _D) for n in ("time", "dt", "cflmax", "dvmax", "qmmax")] + \
         [("tbulk", _D * 2), ("tbulk_avg", _D * 2), ("cd", _D * 3), ("cd_avg", _D * 3),
          ("hflux", _D * 3), ("hflux_avg", _D * 3)] + \
-        [(n, _D) for n in ("s_sgs", "s_rhsnlhs", "s_poisson", "s_energy")]
+        [(n, _D) for n in ("s_sgs", "s_rhsnlhs", "s_poisson", "s_energy")] + \
+        [(n, _D) for n in ("qflux", "ubulk", "pmiavg")] + [("cmfravg", _D * 3)]
 
     def as_dict(self):
         out = {}
@@ -465,7 +466,9 @@ class Solver:
             _check(int(n))
         out = np.zeros(n)
         _check(0 if L.lw_get_array(self._h, name.encode(), _pd(out), n) == n else -1)
-        return out.reshape(self.shape(), order="F")
+        if n == int(np.prod(self.shape())):
+            return out.reshape(self.shape(), order="F")
+        return out  # "pmi": the channel's pmi(0:4)
 
     # -- time-averaged statistics (field_avg) --------------------------------
     AVG_NAMES = ("uavg", "vavg", "wavg") + tuple("uiujavg_%d" % q for q in range(1, 7)) + \

@@ -2530,7 +2530,8 @@ void ora_step(ora *o, ora_diag *d) {
   }
   o->time = o->time + o->dt;
   diagnostics(o);
-  if (o->s.ich != 0) channel_history(o);
+  /* writehistory runs on steps with mod(ntime, npin) = 0 (solver.f90:166) */
+  if (o->s.ich != 0 && (o->s.npin <= 0 || o->ntime % o->s.npin == 0)) channel_history(o);
   if (o->s.iavg == 1) field_avg(o);
   if (d) {
     d->dt = o->dt; d->time = o->time; d->cflmax = o->cflmax; d->dvmax = o->dvmax;

@@ -66,6 +66,38 @@ def test_gpu_phases_match_oracle(mode, oracle_lib):
             assert rel(fo[k], fg[k]) <= 1e-11, ("substep", ms, k)
 
 
+@pytest.mark.parametrize("mode", ["chan_cfr", "chan_cpg"])
+def test_gpu_channel_phases_match_oracle(mode, oracle_lib):
+    """The x-periodic channel phase by phase: the momentum step with the
+    periodic x sweep and pmi(0), poiss_ftft's phi (direct DFT sums here,
+    FFTs in the oracle: 1e-10), the projection with the x ghosts, and
+    meanpg's pmi(0:4) after it, with the flow-rate controller's integral."""
+    import oracle
+    view = _view((32, 32, 8), mode, 1)
+    print("channel phases", mode, flush=True)
+    o = oracle.Oracle(view)
+    g = Solver.from_case(view)
+    L, h = oracle_lib, o.h
+    assert np.allclose(o.array("pmi"), g.array("pmi"), rtol=1e-12, atol=1e-15)
+    L.ora_stepinit(h); g.phase(0)
+    for ms in (1, 2, 3):
+        L.ora_substep_begin(h, ms); g.phase(10 + ms)
+        L.ora_phase_sgs(h); g.phase(1)
+        L.ora_phase_rhsnlhs(h); g.phase(2)
+        fo, fg = o.fields(), g.fields()
+        for k in "uvw":
+            assert rel(fo[k], fg[k]) <= 1e-12, ("rhsnlhs", ms, k)
+        L.ora_phase_divgs(h); g.phase(3)
+        assert rel(o.array("divgsum"), g.array("divgsum")) <= 1e-12
+        L.ora_phase_poisson(h); g.phase(4)
+        assert rel(o.array("phi"), g.array("phi")) <= 1e-10, ("phi", ms)
+        L.ora_phase_ucalc_pcalc(h); g.phase(5)
+        fo, fg = o.fields(), g.fields()
+        for k in "uvwp":
+            assert rel(fo[k], fg[k]) <= 1e-10, ("substep", ms, k)
+        np.testing.assert_allclose(g.array("pmi"), o.array("pmi"), rtol=1e-9, atol=1e-14, err_msg=str(ms))
+
+
 @pytest.mark.parametrize("dims,mode,nsteps,nlev", [((64, 32, 16), "les", 3, 2), ((48, 32, 16), "les_cfl", 4, 3),
                                                    ((96, 64, 32), "les", 2, 3), ((96, 64, 16), "les", 2, 4)])
 def test_gpu_steps_match_oracle(dims, mode, nsteps, nlev):
@@ -211,10 +243,13 @@ def test_gpu_rejects_out_of_scope_configuration():
 def _hist_rows(sol, nend):
     """Per step: the history rows, the mgftresiduemax.dat rows and, every NTR
     steps, the tracer rows, as the reference writes them."""
-    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": [], "bicg_itr": [], "bicg_solve": []}
+    rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": [], "bicg_itr": [], "bicg_solve": [],
+            "cmfr": []}
     ntrace, ntr, _ = sol.trace_info()
     trows = {c: [] for c in "uvwp"}
-    mgrd = sol.settings()["poisson"] == 1
+    sts = sol.settings()
+    mgrd = sts["poisson"] == 1
+    heat = sts["ihtrans"] == 1
     for _ in range(nend):
         sol.step(1)
         h = sol.history()
@@ -232,8 +267,12 @@ def _hist_rows(sol, nend):
             continue
         rows["hist"].append([h["time"], h["dt"], h["cflmax"], h["dvmax"], h["qmmax"]])
         rows["cdcl"].append([h["time"]] + h["cd"] + h["cd_avg"])
-        rows["hflux"].append([h["time"]] + h["hflux"] + h["hflux_avg"])
-        rows["temp"].append([h["time"]] + h["tbulk"] + h["tbulk_avg"])
+        # heatflux and the ftemp.dat row are written with IHTRANS = 1 only
+        if heat:
+            rows["hflux"].append([h["time"]] + h["hflux"] + h["hflux_avg"])
+            rows["temp"].append([h["time"]] + h["tbulk"] + h["tbulk_avg"])
+        if sts["ich"] != 0:
+            rows["cmfr"].append([h["time"], h["qflux"], h["ubulk"], h["pmiavg"]] + h["cmfravg"])
     out = {k: np.array(v) for k, v in rows.items()}
     if ntrace:
         out.update({"trace_" + c: np.array(v) for c, v in trows.items()})
@@ -292,6 +331,9 @@ def _check_rows(name, key, ref, got):
     each value must agree to that print, relative to the row's largest value
     for the near-zero components. dvmax sits at the multigrid tolerance and
     is compared only as a bound."""
+    if ref.size == 0:
+        assert np.asarray(got).size == 0, (name, key, "rows the reference does not write")
+        return
     assert ref.shape == got.shape, (name, key, ref.shape, got.shape)
     for r, g in zip(ref, got):
         assert abs(r[0] - g[0]) <= 6e-6, (name, key, "time", r[0], g[0])
@@ -320,6 +362,8 @@ def test_gpu_history_matches_reference_golden(name):
         _check_bicg(name, z, rows)
     if "trace_u" in z.files:
         _check_trace(name, z, rows)
+    if "cmfr" in z.files:
+        _check_rows(name, "cmfr", z["cmfr"], rows["cmfr"])
 
 
 def _mg_file_rows(path):
@@ -331,7 +375,8 @@ def _mg_file_rows(path):
     return np.array(rows).reshape(-1, 3)
 
 
-@pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4", "mgrd_cyl32_les_3", "cyl32_sor_3"])
+@pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4", "mgrd_cyl32_les_3", "cyl32_sor_3",
+                                  "chan32_cfr_4"])
 def test_gpu_fortran_host_matches_reference(name):
     """The Fortran solver_exec (ISO_C_BINDING host over the C ABI) run on the
     reference's case directory writes the reference's fld file, history rows,
@@ -379,6 +424,10 @@ def test_gpu_fortran_host_matches_reference(name):
                                            open(os.path.join(root, "output/ftr/f%strace.dat" % c)) if ln.strip()])
                    for c in "uvwp"}
             _check_trace(name, z, got)
+        if "cmfr" in z.files:
+            got = np.array([[float(x) for x in ln.split()] for ln in
+                            open(os.path.join(root, "output/ftr/fcmfr.dat")) if ln.strip()])
+            _check_rows(name, "cmfr", z["cmfr"], got)
         if "init_u" in z.files:
             f0 = C.read_fld(os.path.join(root, "output/field/fld0000000"))
             for k, sl in fld_regions(n, z["t"].size > 0).items():
