@@ -991,9 +991,14 @@ def parse_settings(text: str, boundary: str, prdic: str) -> Settings:
     b.read(1); s.ich, s.iconjg = (int(x) for x in b.read(2))
     p = ListDirected(prdic).read(4)
     s.xprdic, s.yprdic, s.zprdic, s.iintp = (int(x) for x in p)
-    # an x-periodic grid takes poiss_ftft, the reference Makefile's default
+    # the Poisson solver, as lw_create_from_dir picks it: an x-periodic grid
+    # takes poiss_ftft (the reference Makefile's default) or, with
+    # LW_POISSON = x_ft, poiss_x_ft; LW_POISSON = mgrd names poiss_mgrd
+    env = os.environ.get("LW_POISSON", "")
     if s.xprdic == 1:
-        s.poisson = 2
+        s.poisson = 3 if env == "x_ft" else 2
+    elif env == "mgrd":
+        s.poisson = 1
     return s
 
 
@@ -1182,7 +1187,9 @@ def case_view(case: Case) -> dict:
     fl.update(u=u, v=v, w=w, p=p)
     if s.ihtrans:
         fl["t"] = f["t"]
-    st = dataclasses.replace(s, xprdic=xp, yprdic=yp, zprdic=zp, iintp=1, poisson=2 if xp == 1 else s.poisson)
+    # an x-periodic grid takes poiss_ftft unless poiss_x_ft (3) is named
+    st = dataclasses.replace(s, xprdic=xp, yprdic=yp, zprdic=zp, iintp=1,
+                             poisson=(s.poisson if s.poisson == 3 else 2) if xp == 1 else s.poisson)
     if s.iread == 0:
         fields = makefld(st, g, BODIES[case.spec.body], case.spec.body_args)
     else:

@@ -193,9 +193,11 @@ void read_settings(Case &c, const std::string &dir) {
   s.xprdic = b[0]; s.yprdic = b[1]; s.zprdic = b[2]; s.iintp = b[3];
   // the Poisson solver is the reference's link-time choice (Makefile:23-34),
   // named by no input file: LW_POISSON = mgrd selects poiss_mgrd
-  // and an x-periodic grid takes poiss_ftft, the Makefile's default
+  // and an x-periodic grid takes poiss_ftft, the Makefile's default, or
+  // poiss_x_ft with LW_POISSON = x_ft
   const char *ps = getenv("LW_POISSON");
-  s.poisson = s.xprdic == 1 ? 2 : (ps && strcmp(ps, "mgrd") == 0) ? 1 : 0;
+  if (s.xprdic == 1) s.poisson = (ps && strcmp(ps, "x_ft") == 0) ? 3 : 2;
+  else s.poisson = (ps && strcmp(ps, "mgrd") == 0) ? 1 : 0;
 }
 
 template <class T>

@@ -94,6 +94,8 @@ def lib():
         L.ora_bicg_log.argtypes = [C.c_void_p, _PI, _PI, _PD, _PI, _I]
         L.ora_bicg_log.restype = _I
         L.ora_bicg_solves.argtypes = [C.c_void_p, _PI, _PD]
+        L.ora_xft_rows.argtypes = [C.c_void_p, _PI, _PI, _PI, _PD, _I]
+        L.ora_xft_rows.restype = _I
         L.ora_bicg_solves.restype = _I
         _lib = L
     return _lib
@@ -186,11 +188,15 @@ class Oracle:
         L.ora_set_fields(self.h, *[_pd(a) for a in arrs], _pd(tt) if tt is not None else None,
                          cd["ihist"], cd["m"], cd["time"], cd["dt"])
         L.ora_init(self.h)
-        # an x-periodic case takes poiss_ftft, the reference Makefile's default
+        # the solver the settings name (poisson = 1 mgrd, 2 ftft, 3 x_ft); an
+        # x-periodic case otherwise takes poiss_ftft, the reference Makefile's
+        # default
+        if poisson == "z_ft":
+            poisson = {1: "mgrd", 2: "ftft", 3: "x_ft"}.get(int(getattr(cd["settings"], "poisson", 0)), "z_ft")
         if poisson == "z_ft" and cd["settings"].xprdic == 1:
             poisson = "ftft"
         if poisson != "z_ft":
-            kind = {"mgrd": 1, "ftft": 2}.get(poisson, -1)
+            kind = {"mgrd": 1, "ftft": 2, "x_ft": 3}.get(poisson, -1)
             if kind < 0 or L.ora_set_poisson(self.h, kind) != 0:
                 raise ValueError("oracle: Poisson path %r not available for this case" % poisson)
 
@@ -208,6 +214,16 @@ class Oracle:
         si, sr = np.zeros(3, np.int64), np.zeros(3)
         ns = L.ora_bicg_solves(self.h, _pi(si), _pd(sr))
         return it, [(int(si[q]), float(sr[q])) for q in range(ns)]
+
+    def xft_rows(self):
+        """poiss_x_ft's mgftresiduemax.dat rows of the last step: (kind,
+        wavenumber, cycle counter, residual * dtconst * n1mh / n1m), kind 0
+        for mg2d, 1 for gsor2d at its limit."""
+        cap = 4096
+        k, iv, ii = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        rg = np.zeros(cap)
+        n = lib().ora_xft_rows(self.h, _pi(k), _pi(iv), _pi(ii), _pd(rg), cap)
+        return [(int(k[q]), int(iv[q]), int(ii[q]), float(rg[q])) for q in range(min(n, cap))]
 
     def step(self, n: int = 1):
         d = Diag()
@@ -248,6 +264,7 @@ class Oracle:
 
 REF_EXEC_MGRD = os.path.join(HERE, "_ref", "solver_exec_mgrd")
 REF_EXEC_FTFT = os.path.join(HERE, "_ref", "solver_exec_ftft")
+REF_EXEC_XFT = os.path.join(HERE, "_ref", "solver_exec_xft")
 
 
 def run_reference(root: str, threads: int = 1, timeout: int = 600, exe: str = REF_EXEC) -> str:

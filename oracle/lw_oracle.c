@@ -97,6 +97,8 @@ struct ora {
   double pmi[5], pmiavg, err_integral, cmfr[7], cmfravg_int[3], cmfravg_dur;
   /* poiss_ftft (poisinit, poiss_ftft.f90:164-204): ak1(n1m), ak3(n3mh) */
   double *ak1f, *ak3f;
+  /* poiss_x_ft: its tables and residual rows */
+  struct xft *xf;
 };
 
 /* ---- indexing ------------------------------------------------------- */
@@ -3107,6 +3109,457 @@ static void poisson_ftft(ora *o) {
   free(cc); free(re); free(im); free(zz); free(a); free(b); free(c); free(r);
 }
 
+/* ===================================================================== */
+/* poiss_x_ft.f90: a DFT in x, then per x wavenumber a zebra multigrid in  */
+/* the (z, y) plane with tridiagonal line solves in y (mg2d), or zebra     */
+/* line-SOR above IMGSOR (gsor2d). z is periodic: the reference sets the   */
+/* z neighbour tables kpm/kmm only for ZPRDIC = 1 (coefmg, 749-789).       */
+/* ===================================================================== */
+struct xft {
+  int64_t n1mh, n3md;
+  int64_t kkmg[21][4];
+  double *ab, *af, *as, *an, *coi1, *coi2, *cor1, *cor2, *ai3;
+  int64_t *kpm, *kmm;
+  double *ac, *gam, *bet; /* (n3md, n2m, n1mh) */
+  /* mgftresiduemax.dat rows of the last step: kind 0 mg2d (iv <= 2),
+   * 1 gsor2d at its iteration limit; iv, ii, residual * dtconst * n1mh/n1m */
+  int64_t nrow, cap, *rkind, *riv, *rii;
+  double *rrg;
+};
+
+#define KK(l, c) x->kkmg[l][c]
+#define XI(k, j, i) (((k)-1) + x->n3md * (((j)-1) + o->n2m * (int64_t)((i)-1)))
+
+/* poisinit (18-52), pmat (53-97), mwavenumber (99-114), coefmg (640-792) */
+static void xft_init(ora *o) {
+  struct xft *x = (struct xft *)xcalloc(1, sizeof(struct xft));
+  o->xf = x;
+  int64_t n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n3 = o->n3, nlev = o->s.nlev;
+  x->n1mh = n1m / 2 + 1;
+  x->n3md = 2 * n3m;
+  int64_t n1mh = x->n1mh, n3md = x->n3md;
+  x->ab = (double *)xcalloc(n3md + 2, 8); x->af = (double *)xcalloc(n3md + 2, 8);
+  x->as = (double *)xcalloc(n2m + 2, 8); x->an = (double *)xcalloc(n2m + 2, 8);
+  x->coi1 = (double *)xcalloc(n3md + 2, 8); x->coi2 = (double *)xcalloc(n3md + 2, 8);
+  x->cor1 = (double *)xcalloc(n3md + 2, 8); x->cor2 = (double *)xcalloc(n3md + 2, 8);
+  x->ai3 = (double *)xcalloc(n1mh + 2, 8);
+  x->kpm = (int64_t *)xcalloc(n3md + 2, 8); x->kmm = (int64_t *)xcalloc(n3md + 2, 8);
+  size_t na = (size_t)n3md * n2m * n1mh;
+  x->ac = (double *)xcalloc(na, 8); x->gam = (double *)xcalloc(na, 8); x->bet = (double *)xcalloc(na, 8);
+  /* pmat: the top level */
+  for (int64_t k = 1; k <= n3m; ++k) {
+    int64_t kp = o->kpv[k];
+    x->ab[k] = (1.0 - (double)o->fixkl[k]) * o->c2czi[k] * o->f2fzi[k];
+    x->af[k] = (1.0 - (double)o->fixku[k]) * o->c2czi[kp] * o->f2fzi[k];
+  }
+  for (int64_t j = 1; j <= n2m; ++j) {
+    int64_t jp = o->jpv[j];
+    x->as[j] = (1.0 - (double)o->fixjl[j]) * o->c2cyi[j] * o->f2fyi[j];
+    x->an[j] = (1.0 - (double)o->fixju[j]) * o->c2cyi[jp] * o->f2fyi[j];
+  }
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; ++k) x->ac[XI(k, j, 1)] = -1.0 * (x->ab[k] + x->af[k] + x->as[j] + x->an[j]);
+  double pi = acos(-1.0);
+  for (int64_t i = 1; i <= n1mh; ++i)
+    x->ai3[i] = 2.0 * (1.0 - cos(2.0 * pi * (double)(i - 1) / (double)n1m)) * o->f2fxi[1] * o->f2fxi[1];
+  for (int64_t i = 2; i <= n1mh; ++i)
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = 1; k <= n3m; ++k) x->ac[XI(k, j, i)] = x->ac[XI(k, j, 1)] - x->ai3[i];
+  /* coefmg: the coarse levels stacked after the top level in k */
+  int64_t minrow = n3m / ((int64_t)1 << nlev);
+  double *zmpm = (double *)xcalloc(n3md + 2, 8), *kbz = (double *)xcalloc(n3md + 2, 8),
+         *kfz = (double *)xcalloc(n3md + 2, 8);
+  KK(nlev, 1) = 1; KK(nlev, 2) = n3m; KK(nlev, 3) = n3m;
+  for (int64_t k = 1; k <= n3m; ++k) zmpm[k] = o->zmp[k];
+  for (int64_t ilev = nlev - 1; ilev >= 0; --ilev) {
+    KK(ilev, 1) = KK(ilev + 1, 1) + KK(ilev + 1, 3);
+    KK(ilev, 3) = minrow * ((int64_t)1 << ilev);
+    KK(ilev, 2) = KK(ilev, 1) + KK(ilev, 3) - 1;
+    int64_t kbg = KK(ilev, 1), kend = KK(ilev, 2), ksp = (int64_t)1 << (nlev - ilev), kc = 0;
+    for (int64_t k = kbg; k <= kend; ++k) {
+      kc += 1;
+      zmpm[k] = 0.5 * (o->z[kc * ksp + 1] + o->z[(kc - 1) * ksp + 1]);
+      kbz[k] = (double)(1 - kbg / k);
+      kfz[k] = (double)(1 - k / kend);
+    }
+    kc = 0;
+    for (int64_t k = kbg; k <= kend; ++k) {
+      kc += 1;
+      double dzc = o->z[kc * ksp + 1] - o->z[(kc - 1) * ksp + 1];
+      x->ab[k] = kbz[k] / ((zmpm[k] - zmpm[k - 1]) * dzc);
+      x->af[k] = kfz[k] / ((zmpm[k + 1] - zmpm[k]) * dzc);
+    }
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = kbg; k <= kend; ++k) x->ac[XI(k, j, 1)] = -1.0 * (x->ab[k] + x->af[k] + x->as[j] + x->an[j]);
+    for (int64_t i = 2; i <= n1mh; ++i)
+      for (int64_t j = 1; j <= n2m; ++j)
+        for (int64_t k = kbg; k <= kend; ++k) x->ac[XI(k, j, i)] = x->ac[XI(k, j, 1)] - x->ai3[i];
+  }
+  for (int64_t ilev = nlev; ilev >= 1; --ilev) {
+    int64_t kbgh = KK(ilev, 1);
+    for (int64_t k = KK(ilev - 1, 1); k <= KK(ilev - 1, 2); ++k) {
+      x->cor1[k] = (zmpm[kbgh + 1] - zmpm[k]) / (zmpm[kbgh + 1] - zmpm[kbgh]);
+      x->cor2[k] = 1.0 - x->cor1[k];
+      kbgh += 2;
+    }
+  }
+  for (int64_t ilev = 0; ilev <= nlev - 1; ++ilev) {
+    int64_t kbgh = KK(ilev + 1, 1) + 1;
+    for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2) - 1; ++k) {
+      x->coi1[k] = (zmpm[k + 1] - zmpm[kbgh]) / (zmpm[k + 1] - zmpm[k]);
+      x->coi2[k] = 1.0 - x->coi1[k];
+      kbgh += 2;
+    }
+    x->coi1[KK(ilev, 2)] = 1.0;
+  }
+  /* z periodicity (749-789): wrapped neighbours, the end cells' couplings
+   * across the period, their diagonals, the last cell's interpolation */
+  for (int64_t ilev = nlev; ilev >= 0; --ilev) {
+    int64_t kbg = KK(ilev, 1), kend = KK(ilev, 2);
+    for (int64_t k = kbg; k <= kend; ++k) { x->kpm[k] = k + 1; x->kmm[k] = k - 1; }
+    x->kpm[kend] = kbg;
+    x->kmm[kbg] = kend;
+  }
+  for (int64_t ilev = nlev - 1; ilev >= 0; --ilev) {
+    int64_t kbg = KK(ilev, 1), kend = KK(ilev, 2), ksp = (int64_t)1 << (nlev - ilev);
+    double vdz = zmpm[kbg] - o->z[1] + o->z[n3] - zmpm[kend];
+    double sdz_kbg = o->z[1 + ksp] - o->z[1], sdz_kend = o->z[n3] - o->z[n3 - ksp];
+    x->ab[kbg] = 1.0 / (vdz * sdz_kbg);
+    x->af[kend] = 1.0 / (vdz * sdz_kend);
+  }
+  for (int64_t ilev = nlev - 1; ilev >= 0; --ilev) {
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k)
+        x->ac[XI(k, j, 1)] = -1.0 * (x->ab[k] + x->af[k] + x->as[j] + x->an[j]);
+    for (int64_t i = 2; i <= n1mh; ++i)
+      for (int64_t j = 1; j <= n2m; ++j)
+        for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k) x->ac[XI(k, j, i)] = x->ac[XI(k, j, 1)] - x->ai3[i];
+  }
+  for (int64_t ilev = 0; ilev <= nlev - 1; ++ilev) {
+    int64_t kbg = KK(ilev, 1), kend = KK(ilev, 2);
+    double vdz = zmpm[kbg] - o->z[1] + o->z[n3] - zmpm[kend];
+    x->coi2[kend] = (zmpm[KK(ilev + 1, 2)] - zmpm[kend]) / vdz;
+    x->coi1[kend] = 1.0 - x->coi2[kend];
+  }
+  /* the LU pivots of the y lines (poisinit, 33-46) */
+  for (int64_t ilev = 0; ilev <= nlev; ++ilev)
+    for (int64_t i = 1; i <= n1mh; ++i) {
+      for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k) x->bet[XI(k, 1, i)] = 1.0 / x->ac[XI(k, 1, i)];
+      for (int64_t j = 2; j <= n2m; ++j)
+        for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k) {
+          x->gam[XI(k, j, i)] = x->an[j - 1] * x->bet[XI(k, j - 1, i)];
+          x->bet[XI(k, j, i)] = 1.0 / (x->ac[XI(k, j, i)] - x->as[j] * x->gam[XI(k, j, i)]);
+        }
+    }
+  free(zmpm); free(kbz); free(kfz);
+}
+
+typedef struct {
+  ora *o;
+  struct xft *x;
+  int64_t iv;
+  cplx *pc;   /* (0:n3, 0:n2) */
+  cplx *rhs;  /* (n3, n2) */
+  cplx *resd; /* (n3md, n2m) */
+  cplx *gg;   /* (0:n3md, 0:n2) */
+} xctx;
+
+#define XPC(k, j) c->pc[(k) + (o->n3 + 1) * (j)]
+#define XRHS(k, j) c->rhs[((k)-1) + o->n3 * ((j)-1)]
+#define XRES(k, j) c->resd[((k)-1) + x->n3md * ((j)-1)]
+#define XGG(k, j) c->gg[(k) + (x->n3md + 1) * (j)]
+#define XBET(k, j) x->bet[XI(k, j, c->iv)]
+#define XGAM(k, j) x->gam[XI(k, j, c->iv)]
+#define XAC(k, j) x->ac[XI(k, j, c->iv)]
+
+/* trdiag1m (372-394): the y lines k = l1..l2 step 2 of gg, in place */
+static void xft_trdiag(xctx *c, int64_t l1, int64_t l2) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t n2m = o->n2m;
+  for (int64_t k = l1; k <= l2; k += 2) XGG(k, 1) = c_rmul(XBET(k, 1), XGG(k, 1));
+  for (int64_t j = 2; j <= n2m; ++j)
+    for (int64_t k = l1; k <= l2; k += 2)
+      XGG(k, j) = c_rmul(XBET(k, j), c_sub(XGG(k, j), c_rmul(x->as[j], XGG(k, j - 1))));
+  for (int64_t j = n2m - 1; j >= 1; --j)
+    for (int64_t k = l1; k <= l2; k += 2) XGG(k, j) = c_sub(XGG(k, j), c_rmul(XGAM(k, j + 1), XGG(k, j + 1)));
+}
+
+/* the five-point residual of level cells against r, in gg (godown 421-426,
+ * toplevel 349-351) */
+static cplx xft_res(xctx *c, cplx r, int64_t k, int64_t j, const cplx *v, int64_t ld) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+#define V(kk, jj) v[(kk) + ld * (jj)]
+  r = c_sub(r, c_rmul(x->ab[k], V(x->kmm[k], j)));
+  r = c_sub(r, c_rmul(x->af[k], V(x->kpm[k], j)));
+  r = c_sub(r, c_rmul(x->as[j], V(k, j - 1)));
+  r = c_sub(r, c_rmul(x->an[j], V(k, j + 1)));
+  r = c_sub(r, c_rmul(XAC(k, j), V(k, j)));
+#undef V
+  return r;
+}
+
+/* toplevel (300-370), zebra version */
+static void xft_toplevel(xctx *c, int id, double *sumres, double oldv, double test) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t n2m = o->n2m, n3m = o->n3m, nlev = o->s.nlev, ldg = x->n3md + 1;
+  if (id == 1) {
+    int64_t kc = 0;
+    for (int64_t k = KK(nlev - 1, 1); k <= KK(nlev - 1, 2); ++k) {
+      kc += 2;
+      for (int64_t j = 1; j <= n2m; ++j)
+        XPC(kc, j) = c_add(c_add(XPC(kc, j), c_rmul(x->coi1[k], XGG(k, j))), c_rmul(x->coi2[k], XGG(x->kpm[k], j)));
+    }
+  }
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; k += 2)
+      XGG(k, j) = c_sub(XRHS(k, j), c_rmul(oldv, c_add(c_rmul(x->ab[k], XPC(x->kmm[k], j)),
+                                                        c_rmul(x->af[k], XPC(x->kpm[k], j)))));
+  xft_trdiag(c, 1, n3m);
+  if (c->iv == 1) {
+    cplx tt = XGG(1, n2m);
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = 1; k <= n3m; k += 2) XGG(k, j) = c_sub(XGG(k, j), tt);
+  }
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 2; k <= n3m; k += 2)
+      XGG(k, j) = c_sub(XRHS(k, j), c_add(c_rmul(x->ab[k], XGG(x->kmm[k], j)), c_rmul(x->af[k], XGG(x->kpm[k], j))));
+  xft_trdiag(c, 2, n3m);
+  double sr = 0.0;
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; ++k) {
+      cplx r = xft_res(c, XRHS(k, j), k, j, c->gg, ldg);
+      XRES(k, j) = r;
+      double ar = c_abs(r);
+      sr = sr > ar ? sr : ar;
+      XPC(k, j) = XGG(k, j);
+    }
+  *sumres = sr;
+  if (sr < test || id == 2) return;
+  int64_t kc = -1;
+  for (int64_t k = KK(nlev - 1, 1); k <= KK(nlev - 1, 2); ++k) {
+    kc += 2;
+    for (int64_t j = 1; j <= n2m; ++j)
+      XRES(k, j) = c_add(c_rmul(x->cor1[k], XRES(kc, j)), c_rmul(x->cor2[k], XRES(kc + 1, j)));
+  }
+}
+
+/* goup (397-412) */
+static void xft_goup(xctx *c, int64_t ilev) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t kbgh = KK(ilev + 1, 1) - 1;
+  for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k) {
+    kbgh += 2;
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      XGG(kbgh, j) = c_add(c_add(XGG(kbgh, j), c_rmul(x->coi1[k], XGG(k, j))), c_rmul(x->coi2[k], XGG(x->kpm[k], j)));
+  }
+}
+
+/* godown (415-437) */
+static void xft_godown(xctx *c, int64_t ilev) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t ldg = x->n3md + 1;
+  for (int64_t j = 1; j <= o->n2m; ++j)
+    for (int64_t k = KK(ilev, 1); k <= KK(ilev, 2); ++k) XRES(k, j) = xft_res(c, XRES(k, j), k, j, c->gg, ldg);
+  int64_t kbg = KK(ilev, 1) - 2;
+  for (int64_t k = KK(ilev - 1, 1); k <= KK(ilev - 1, 2); ++k) {
+    kbg += 2;
+    for (int64_t j = 1; j <= o->n2m; ++j)
+      XRES(k, j) = c_add(c_rmul(x->cor1[k], XRES(kbg, j)), c_rmul(x->cor2[k], XRES(kbg + 1, j)));
+  }
+}
+
+/* relax (440-482), zebra version: a half with oldv, the other half, then
+ * iiter-1 more pairs of halves */
+static void xft_relax(xctx *c, int64_t ilev, double oldv, int64_t iiter) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t kb = KK(ilev, 1), ke = KK(ilev, 2), n2m = o->n2m;
+  for (int64_t pass = 0; pass < iiter; ++pass) {
+    double w = pass == 0 ? oldv : 1.0;
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = kb; k <= ke; k += 2)
+        XGG(k, j) = c_sub(XRES(k, j), c_rmul(w, c_add(c_rmul(x->ab[k], XGG(x->kmm[k], j)),
+                                                      c_rmul(x->af[k], XGG(x->kpm[k], j)))));
+    xft_trdiag(c, kb, ke);
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = kb + 1; k <= ke; k += 2)
+        XGG(k, j) = c_sub(c_sub(XRES(k, j), c_rmul(x->ab[k], XGG(x->kmm[k], j))), c_rmul(x->af[k], XGG(x->kpm[k], j)));
+    xft_trdiag(c, kb + 1, ke);
+  }
+}
+
+static void xft_row(struct xft *x, int64_t kind, int64_t iv, int64_t ii, double rg) {
+  if (x->nrow == x->cap) {
+    x->cap = x->cap ? 2 * x->cap : 64;
+    x->rkind = (int64_t *)realloc(x->rkind, x->cap * 8); x->riv = (int64_t *)realloc(x->riv, x->cap * 8);
+    x->rii = (int64_t *)realloc(x->rii, x->cap * 8); x->rrg = (double *)realloc(x->rrg, x->cap * 8);
+  }
+  x->rkind[x->nrow] = kind; x->riv[x->nrow] = iv; x->rii[x->nrow] = ii; x->rrg[x->nrow] = rg;
+  x->nrow += 1;
+}
+
+/* mg2d (235-297) with oldv = 0 as poisson calls it; returns the V-cycles
+ * (0 when the first toplevel passed, MGITR at the limit; the row takes the
+ * reference's loop counter, MGITR+1 there) */
+static int64_t xft_mg2d(xctx *c, double test) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t nlev = o->s.nlev, ii = 0;
+  double sumres;
+  xft_toplevel(c, 0, &sumres, 0.0, test);
+  if (!(sumres < test)) {
+    for (ii = 1; ii <= o->s.mgitr; ++ii) {
+      for (int64_t ilev = nlev - 1; ilev >= 1; --ilev) {
+        xft_relax(c, ilev, 0.0, 1);
+        xft_godown(c, ilev);
+      }
+      xft_relax(c, 0, 0.0, o->s.nbli);
+      for (int64_t ilev = 0; ilev <= nlev - 2; ++ilev) {
+        xft_goup(c, ilev);
+        xft_relax(c, ilev + 1, 1.0, 1);
+      }
+      xft_toplevel(c, 1, &sumres, 1.0, test);
+      if (sumres < test) break;
+    }
+    if (ii > o->s.mgitr) fprintf(stderr, "oracle: ITERATION LIMIT EXCEEDED (x_ft iv=%lld)\n", (long long)c->iv);
+  }
+  if (c->iv <= 2) xft_row(x, 0, c->iv, ii, sumres * o->dtconst * (double)x->n1mh / (double)o->n1m);
+  return ii > o->s.mgitr ? o->s.mgitr : ii;
+}
+
+/* resid3 (618-638) on the top level of u = pc */
+static double xft_resid3(xctx *c) {
+  ora *o = c->o;
+  double errmax = 0.0;
+  for (int64_t j = 1; j <= o->n2m; ++j)
+    for (int64_t k = 1; k <= o->n3m; ++k) {
+      double ar = c_abs(xft_res(c, XRHS(k, j), k, j, c->pc, o->n3 + 1));
+      errmax = errmax > ar ? errmax : ar;
+    }
+  return errmax;
+}
+
+/* one zebra line over-relaxation of u = pc (the halves of gsor2d,
+ * 498-552 with wgt = oldv, 560-606 with wgt = 1); between the halves the
+ * mean mode's odd lines are shifted by u(1, n2m) (520-527, 576-583) */
+static void xft_sor_sweep(xctx *c, double wgt) {
+  ora *o = c->o;
+  struct xft *x = c->x;
+  int64_t n2m = o->n2m, n3m = o->n3m;
+  double ww = o->s.wwsor, ww2 = 1.0 - ww, cw = wgt * ww2;
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; k += 2)
+      XGG(k, j) = c_sub(XRHS(k, j), c_rmul(wgt, c_add(c_rmul(x->ab[k], XPC(x->kmm[k], j)),
+                                                      c_rmul(x->af[k], XPC(x->kpm[k], j)))));
+  xft_trdiag(c, 1, n3m);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; k += 2) XPC(k, j) = c_add(c_rmul(ww, XGG(k, j)), c_rmul(cw, XPC(k, j)));
+  if (c->iv == 1) {
+    cplx tt = XPC(1, n2m);
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = 1; k <= n3m; k += 2) XPC(k, j) = c_sub(XPC(k, j), tt);
+  }
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 2; k <= n3m; k += 2)
+      XGG(k, j) = c_sub(XRHS(k, j), c_add(c_rmul(x->ab[k], XPC(x->kmm[k], j)), c_rmul(x->af[k], XPC(x->kpm[k], j))));
+  xft_trdiag(c, 2, n3m);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 2; k <= n3m; k += 2) XPC(k, j) = c_add(c_rmul(ww, XGG(k, j)), c_rmul(cw, XPC(k, j)));
+}
+
+/* gsor2d (485-615) with oldv = 0; a row at the iteration limit only */
+static int64_t xft_gsor2d(xctx *c, double test) {
+  ora *o = c->o;
+  xft_sor_sweep(c, 0.0);
+  double e = xft_resid3(c);
+  if (e < test) return 0;
+  int64_t ii;
+  for (ii = 1; ii <= o->s.mgitr; ++ii) {
+    xft_sor_sweep(c, 1.0);
+    e = xft_resid3(c);
+    if (e < test) return ii;
+  }
+  fprintf(stderr, "oracle: ITERATION LIMIT EXCEEDED (x_ft line-SOR iv=%lld)\n", (long long)c->iv);
+  xft_row(c->x, 1, c->iv, ii, e * o->dtconst * (double)c->x->n1mh / (double)o->n1m);
+  return o->s.mgitr;
+}
+
+/* poisson (115-232) */
+static void poisson_xft(ora *o) {
+  struct xft *x = o->xf;
+  int64_t n1m = o->n1m, n2m = o->n2m, n3m = o->n3m, n2 = o->n2, n3 = o->n3, n1mh = x->n1mh;
+  cplx *ccap = (cplx *)xcalloc((size_t)n3 * n2 * n1mh, sizeof(cplx));
+#define XCC(k, j, i) ccap[((k)-1) + n3 * (((j)-1) + n2 * (int64_t)((i)-1))]
+  double *re = (double *)xcalloc(n1m, 8), *im = (double *)xcalloc(n1m, 8);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; ++k) {
+      for (int64_t i = 1; i <= n1m; ++i) { re[i - 1] = F(o->divgsum, i, j, k); im[i - 1] = 0.; }
+      ora_fft_forward((int)n1m, re, im);
+      for (int64_t i = 1; i <= n1mh; ++i) XCC(k, j, i) = (cplx){re[i - 1], im[i - 1]};
+    }
+  double test = o->test1 / (double)n1mh * (double)n1m * 0.9;
+  xctx c;
+  c.o = o;
+  c.x = x;
+  size_t npc = (size_t)(n3 + 1) * (n2 + 1), nres = (size_t)x->n3md * n2m, ngg = (size_t)(x->n3md + 1) * (n2 + 1);
+  c.pc = (cplx *)xcalloc(npc, sizeof(cplx));
+  c.resd = (cplx *)xcalloc(nres, sizeof(cplx));
+  c.gg = (cplx *)xcalloc(ngg, sizeof(cplx));
+  if (o->msub == 1) x->nrow = 0; /* the rows of the step's three solves */
+  o->mg_cycles_max = 0;
+  for (int64_t iii = 1; iii <= n1mh; ++iii) {
+    memset(c.pc, 0, npc * sizeof(cplx));
+    memset(c.resd, 0, nres * sizeof(cplx));
+    memset(c.gg, 0, ngg * sizeof(cplx));
+    c.iv = iii;
+    c.rhs = &XCC(1, 1, iii);
+    int64_t cyc = iii <= o->s.imgsor ? xft_mg2d(&c, test) : xft_gsor2d(&c, test);
+    if (cyc > o->mg_cycles_max) o->mg_cycles_max = cyc;
+    o->mg_cycles_sum += cyc;
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = 1; k <= n3m; ++k) XCC(k, j, iii) = c.pc[k + (n3 + 1) * j];
+  }
+  free(c.pc); free(c.resd); free(c.gg);
+  memset(o->phi, 0, NX1 * NY1 * NZ1 * 8);
+  for (int64_t j = 1; j <= n2m; ++j)
+    for (int64_t k = 1; k <= n3m; ++k) {
+      for (int64_t i = 1; i <= n1mh; ++i) { re[i - 1] = XCC(k, j, i).re; im[i - 1] = XCC(k, j, i).im; }
+      for (int64_t i = n1mh + 1; i <= n1m; ++i) {
+        cplx q = XCC(k, j, n1m + 2 - i);
+        re[i - 1] = q.re; im[i - 1] = -q.im;
+      }
+      ora_fft_inverse((int)n1m, re, im);
+      for (int64_t i = 1; i <= n1m; ++i) F(o->phi, i, j, k) = re[i - 1];
+    }
+#undef XCC
+  /* ICH = 1: the mean phi at the upper wall row of k = 1 to zero (215-229) */
+  if (o->s.ich == 1) {
+    double phiref = 0.0;
+    for (int64_t i = 1; i <= n1m; ++i) phiref = phiref + F(o->phi, i, n2m, 1) * o->f2fx[i];
+    phiref = phiref / o->xl;
+    for (int64_t k = 1; k <= n3m; ++k)
+      for (int64_t j = 1; j <= n2m; ++j)
+        for (int64_t i = 1; i <= n1m; ++i) F(o->phi, i, j, k) = F(o->phi, i, j, k) - phiref;
+  }
+  free(re); free(im); free(ccap);
+}
+
+int64_t ora_xft_rows(const ora *o, int64_t *kind, int64_t *iv, int64_t *ii, double *rg, int64_t cap) {
+  const struct xft *x = o->xf;
+  if (!x) return 0;
+  for (int64_t q = 0; q < x->nrow && q < cap; ++q) {
+    kind[q] = x->rkind[q]; iv[q] = x->riv[q]; ii[q] = x->rii[q]; rg[q] = x->rrg[q];
+  }
+  return x->nrow;
+}
+#undef KK
+#undef XI
+
 static void poisson_any(ora *o) {
   if (o->poiss_kind == 1) {
     o->mg_cycles_max = 0;
@@ -3114,16 +3567,26 @@ static void poisson_any(ora *o) {
   } else if (o->poiss_kind == 2) {
     o->mg_cycles_max = 0;
     poisson_ftft(o);
+  } else if (o->poiss_kind == 3) {
+    poisson_xft(o);
   } else {
     poisson(o);
   }
 }
 
 int ora_set_poisson(ora *o, int64_t kind) {
-  if (kind < 0 || kind > 2) return -1;
-  /* poiss_ftft needs x periodicity; the other two an inflow/outflow x */
-  if ((kind == 2) != (o->s.xprdic == 1)) return -1;
+  if (kind < 0 || kind > 3) return -1;
+  /* poiss_ftft and poiss_x_ft need x periodicity; the other two an
+   * inflow/outflow x */
+  if ((kind >= 2) != (o->s.xprdic == 1)) return -1;
   if (kind == 2 && !o->ak1f) ftft_init(o);
+  if (kind == 3) {
+    /* the z multigrid: periodic z (the only one coefmg makes neighbour
+     * tables for), n3m = minrow * 2**NLEV */
+    const int64_t nl = o->s.nlev;
+    if (o->s.zprdic != 1 || nl < 1 || (o->n3m % (1LL << nl)) != 0) return -1;
+    if (!o->xf) xft_init(o);
+  }
   if (kind == 1) {
     const int64_t nl = o->s.nlev;
     if (o->s.ioldv != 0 || o->s.xprdic != 0 || nl < 0 || (o->n1m % (1LL << nl)) != 0 ||

@@ -95,8 +95,15 @@ void ora_stepinit(ora *o);
 /* building blocks exposed for unit tests */
 /* Poisson path: 0 = poiss_z_ft (default), 1 = poiss_mgrd (BiCGSTAB with a
  * 3-D multigrid preconditioner; IOLDV = 0, z periodic or not, x not
- * periodic). Returns 0, or -1 when the case is outside that scope. */
+ * periodic), 2 = poiss_ftft and 3 = poiss_x_ft (x periodic; x_ft also z
+ * periodic with n3m divisible by 2**NLEV). Returns 0, or -1 when the case
+ * is outside that scope. */
 int ora_set_poisson(ora *o, int64_t kind);
+/* poiss_x_ft's mgftresiduemax.dat rows of the last step: kind 0 for mg2d
+ * (wavenumbers 1 and 2, poiss_x_ft.f90:293), 1 for gsor2d at its iteration
+ * limit (:606); the wavenumber, the cycle counter and the residual times
+ * dtconst * n1mh / n1m. Returns the number of rows. */
+int64_t ora_xft_rows(const ora *o, int64_t *kind, int64_t *iv, int64_t *ii, double *rg, int64_t cap);
 /* the BiCGSTAB log of the last step: per iteration (substep, iteration,
  * max |residual| * dtconst, its cell i, j, k as poiss_itr.dat holds them);
  * per solve (the final iteration number, MGITR+1 if not converged, and

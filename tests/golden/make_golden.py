@@ -47,6 +47,11 @@ CASES = {
     # constant flow rate, with their fcmfr.dat rows
     "chan32_cpg_3": dict(dims=(32, 32, 8), mode="chan_cpg", nend=3, poisson="ftft"),
     "chan32_cfr_4": dict(dims=(32, 32, 8), mode="chan_cfr", nend=4, poisson="ftft"),
+    # the same channels with poiss_x_ft.f90 (oracle/_ref/solver_exec_xft): the
+    # z multigrid per x wavenumber with its ' mg' rows, and line-SOR above
+    # wavenumber 3 up to its iteration limit with its 'SOR' rows
+    "xft_chan32_cfr_4": dict(dims=(32, 32, 8), mode="chan_cfr_xft", nend=4, poisson="x_ft"),
+    "xft_chan32_sor_3": dict(dims=(32, 32, 8), mode="chan_cpg_xsor", nend=3, poisson="x_ft"),
     # the 3-D MG-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90) linked
     # in place of poiss_z_ft.f90 (oracle/_ref/solver_exec_mgrd)
     "mgrd_cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3, poisson="mgrd"),
@@ -78,6 +83,20 @@ def mg_rows(path):
     return np.array(rows, dtype=np.float64).reshape(-1, 3)
 
 
+def xft_rows(path):
+    """poiss_x_ft's mgftresiduemax.dat: ' mg, time = t IV= v II= n RG= r'
+    (mg2d, poiss_x_ft.f90:293) and 'SOR IV= v II= n RG= r' (gsor2d at its
+    limit, :606), as (kind 0/1, time or 0, iv, ii, rg)."""
+    rows = []
+    for ln in open(path):
+        t = ln.replace("=", " ").split()
+        if len(t) == 9 and t[0] == "mg,":
+            rows.append((0.0, float(t[2]), float(t[4]), float(t[6]), float(t[8].replace("D", "E"))))
+        elif len(t) == 7 and t[0] == "SOR":
+            rows.append((1.0, 0.0, float(t[2]), float(t[4]), float(t[6].replace("D", "E"))))
+    return np.array(rows, dtype=np.float64).reshape(-1, 5)
+
+
 def main(names=None):
     for name, c in CASES.items():
         if names and name not in names:
@@ -87,7 +106,9 @@ def main(names=None):
         C.write_case(C.build_case(spec), root)
         mgrd = c.get("poisson") == "mgrd"
         ftft = c.get("poisson") == "ftft"
-        exe = oracle.REF_EXEC_MGRD if mgrd else oracle.REF_EXEC_FTFT if ftft else oracle.REF_EXEC
+        xft = c.get("poisson") == "x_ft"
+        exe = (oracle.REF_EXEC_MGRD if mgrd else oracle.REF_EXEC_FTFT if ftft else
+               oracle.REF_EXEC_XFT if xft else oracle.REF_EXEC)
         oracle.run_reference(root, threads=1, exe=exe)
         fl = C.read_fld(os.path.join(root, "output/field/fld%07d" % c["nend"]))
         rows = {}
@@ -98,6 +119,8 @@ def main(names=None):
             # ftrpoittr.dat: time, dt, iterations, residual*dtconst per solve (:221-222)
             rows["bicg_itr"] = text_rows(os.path.join(root, "output/ftr/poiss_itr.dat"))
             rows["bicg_solve"] = text_rows(os.path.join(root, "output/ftr/ftrpoittr.dat"))
+        elif xft:
+            rows["xres"] = xft_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat"))
         elif not ftft:
             rows["mgres"] = mg_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat"))
         if spec.settings.ich != 0:

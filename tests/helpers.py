@@ -18,14 +18,21 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
                        cflfac=0.5)
         return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
-    if mode in ("chan_cpg", "chan_cfr"):
+    if mode in ("chan_cpg", "chan_cfr", "chan_cfr_xft", "chan_cpg_xsor"):
         # a plane channel: x and z periodic, no-slip walls at y = -1, 1, no
         # body in the domain (the cylinder of the case builder lies outside
         # x in [1, 4]); SGS with zero constants, no heat; ICH 2 (constant
-        # pressure gradient -UDRV_I) or 1 (constant flow rate, PI control)
-        ich = 2 if mode == "chan_cpg" else 1
+        # pressure gradient -UDRV_I) or 1 (constant flow rate, PI control).
+        # The _xft and _xsor forms take poiss_x_ft (x DFT, z multigrid), the
+        # latter with line-SOR above x wavenumber 3 and an iteration limit
+        # the SOR wavenumbers reach
+        ich = 2 if mode.startswith("chan_cpg") else 1
         s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2, re=180.0,
                        ihtrans=0, ich=ich, udrv_i=1.0, bc_ybtm=0, bc_ytop=0)
+        if mode == "chan_cfr_xft":
+            s.poisson = 3
+        elif mode == "chan_cpg_xsor":
+            s.poisson, s.imgsor, s.wwsor, s.mgitr = 3, 3, 1.5, 25
         return C.CaseSpec(*dims, xlim=(1.0, 4.0), ylim=(-1.0, 1.0), zlen=1.0, noise=1e-2, settings=s,
                           layout="channel")
     if mode == "nosgs":
@@ -62,6 +69,8 @@ def golden_view(meta):
     view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))))
     if meta.get("poisson") == "mgrd":
         view["settings"].poisson = 1
+    elif meta.get("poisson") == "x_ft":
+        view["settings"].poisson = 3
     return view
 
 

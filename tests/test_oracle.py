@@ -23,13 +23,36 @@ def test_oracle_matches_reference_golden(name, oracle_lib):
     # 1e-11 (1e-10 over 100 steps, where the bar for 100 steps is 1e-6)
     # In the x-periodic channels p holds only the small fluctuations (the
     # mean gradient is pmi), so rounding of the two FFTs weighs more: 1e-10.
-    bar = 1e-11 if meta["nend"] <= 10 and meta.get("poisson") != "ftft" else 1e-10
+    bar = 1e-11 if meta["nend"] <= 10 and meta.get("poisson") not in ("ftft", "x_ft") else 1e-10
     for k, sl in fld_regions(n, z["t"].size > 0).items():
         err = rel(z[k][sl], f[k][sl])
         assert err <= bar, (name, k, err)
     if "cmfr" in z.files:
         # fcmfr.dat of the last step: time, qflux, ubulk, pmiavg, cmfravg(1:3)
         np.testing.assert_allclose(o.array("cmfr"), z["cmfr"][-1], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names() if n.startswith("xft_")])
+def test_oracle_x_ft_rows_match_reference(name, oracle_lib):
+    """poiss_x_ft's mgftresiduemax.dat over every step: mg2d's rows for x
+    wavenumbers 1 and 2 (cycle counts exact, the residual to 1e-3: it is the
+    small difference of A phi and the right-hand side) and gsor2d's rows where
+    a wavenumber reached the iteration limit (MGITR+1 sweeps)."""
+    import oracle
+    z, meta = golden(name)
+    view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"], meta.get("nlev"))))
+    o = oracle.Oracle(view, poisson="x_ft")
+    rows = []
+    for _ in range(meta["nend"]):
+        o.step(1)
+        rows += o.xft_rows()
+    ref = z["xres"]
+    got = np.array([[r[0], r[1], r[2], r[3]] for r in rows])
+    assert got.shape == (len(ref), 4), (got.shape, ref.shape)
+    assert np.array_equal(got[:, :3], ref[:, [0, 2, 3]])
+    np.testing.assert_allclose(got[:, 3], ref[:, 4], rtol=1e-3, atol=1e-14)
+    # a mg2d row carries the time of the step it belongs to
+    assert np.all(ref[ref[:, 0] == 0, 1] >= 0.0)
 
 
 @pytest.mark.parametrize("name", [n for n in golden_names() if n.startswith("mgrd_")])
