@@ -61,7 +61,9 @@ typedef struct lw_settings {
    * one 3-D multigrid V-cycle; lw_create_from_dir takes it from the
    * environment variable LW_POISSON = "mgrd"), 2 = poiss_ftft (x and z
    * DFTs and a y tridiagonal solve; the one for x-periodic grids, which
-   * lw_create_from_dir picks for them) */
+   * lw_create_from_dir picks for them), 3 = poiss_x_ft (an x DFT and a
+   * periodic-z multigrid per x wavenumber; LW_POISSON = "x_ft" for an
+   * x-periodic case directory) */
   int64_t poisson;
 } lw_settings;
 
@@ -213,8 +215,14 @@ int64_t lw_get_mg_residuals(const lw_handle *h, int64_t *kv, int64_t *kk, double
  * and then, for every kv above IMGSOR in turn, gsor2d's row of each sweep of
  * its loop (kind 1: KK the sweep, RG its max-norm residual times dtconst
  * times n3m/2+1; 'KV=',i3,3x,'KK=',i3,3x,'RG=',es24.16;
- * poiss_z_ft.f90:293-297, 584). Returns the row count; copies at most cap
- * rows into the arrays that are not NULL. */
+ * poiss_z_ft.f90:293-297, 584). With poisson = 3 (poiss_x_ft) the rows are
+ * mg2d's for x wavenumbers 1 and 2 (kind 2: KV the wavenumber, KK its
+ * V-cycles, MGITR+1 at the limit, and the max norm times dtconst times
+ * (n1m/2+1)/n1m; ' mg, time = ',f13.5,' IV=',i5,' II=',i5,' RG=',es16.8 with
+ * the time the step began) and gsor2d's of each wavenumber above IMGSOR
+ * that reached its limit (kind 3; 'SOR','  IV=',i5,'  II=',i5,'  RG=',
+ * es23.15; poiss_x_ft.f90:293, 606). Returns the row count; copies at most
+ * cap rows into the arrays that are not NULL. */
 int64_t lw_get_residual_rows(const lw_handle *h, int64_t *kind, int64_t *kv, int64_t *kk, double *val, int64_t cap);
 /* with poisson = 1: the rows poiss_mgrd.f90 writes during the last step,
  * per BiCGSTAB iteration to poiss_itr.dat (:185-186: the substep, the

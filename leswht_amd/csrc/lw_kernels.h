@@ -152,6 +152,26 @@ struct Ftft {
   double *gam;                  // the line solve's gam, same shape
 };
 void launch_poisson_ftft(hipStream_t s, const Geo &g, const Fld &f, const Ftft &t);
+// poiss_x_ft (lw_kern_xft.hip): x DFT, per x wavenumber a periodic-z zebra
+// multigrid (mg2d) or line-SOR (gsor2d), inverse DFT
+struct Xft {
+  int n1mh, n3md, nlev, nbli;
+  int kb[20], ke[20];  // the levels' first and last k (kkmg(l, 1:2))
+  const double *ab, *af, *as, *an, *coi1, *coi2, *cor1, *cor2;  // 1-based (pmat, coefmg)
+  const int *kpm, *kmm;
+  const double *ac, *bet, *gam;  // [iv][j][k], k fastest over n3md
+  const double *twr, *twi;       // e^{-2 pi i q / n1m}
+  double2 *rhs, *pc, *gg, *resd, *tt;
+  int *done, *ii, *nact;
+  unsigned long long *sres;  // max-norm gathering slot per wavenumber
+  double *sfin, *misc;       // the last max norm per wavenumber; misc[0] phiref
+};
+void launch_xft_forward(hipStream_t s, const Geo &g, const Fld &f, const Xft &x);
+void launch_xft_inverse(hipStream_t s, const Geo &g, const Fld &f, const Xft &x, int ich);
+void launch_xft_toplevel(hipStream_t s, const Geo &g, const Xft &x, int iv0, int nv, int id, double test, int cyc);
+void launch_xft_vcycle(hipStream_t s, const Geo &g, const Xft &x, int iv0, int nv);
+void launch_xft_sor(hipStream_t s, const Geo &g, const Xft &x, int iv0, int nv, double wgt, double ww, double test,
+                    int cyc);
 bool mg_folds_check(const MG &mg);
 // kv = 1 of the two-stream solve in one launch (k_mg_kv1): whether the
 // tiles and levels of `mg` (kv0 = 1, nkv = 1) allow it, and the launch of up

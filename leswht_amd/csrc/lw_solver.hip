@@ -220,6 +220,8 @@ struct lw_handle {
   M3 m3{};
   // poiss_ftft: tables and spectra
   Ftft ftft{};
+  // poiss_x_ft: tables, level planes and per-wavenumber state
+  Xft xft{};
   std::vector<int64_t> bicg_sub, bicg_icg, bicg_loc, solve_icg;
   std::vector<double> bicg_rm, solve_rm;
   // tracer (misc_hist.f90:184-209): cells whose centre values are traced
@@ -323,6 +325,7 @@ static std::vector<int> vi(const int64_t *src, int64_t lo, int64_t hi) {
 // 0 (iwest, ieast, kback, kforw) whose divisor holds a centre of another
 // level or one not yet set; that coefficient is 0 here.
 static void setup_ftft(lw_handle *h, const lw_grid *gr);
+static void setup_xft(lw_handle *h, const lw_grid *gr);
 static void setup_mgrd(lw_handle *h, const lw_grid *gr) {
   const lw_settings &st = h->st;
   const Geo &g = h->g;
@@ -758,6 +761,11 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
     return;
   }
+  if (st.poisson == 3) {
+    setup_xft(h, gr);
+    HIPCHK(lw::prepare_adi_kernels(g, &g.adi_wave));
+    return;
+  }
   // ---- Poisson (poisinit, poiss_z_ft.f90:18-113 and coefmg 643-798) ----
   MG &mg = h->mg;
   const int nlev = (int)st.nlev;
@@ -980,6 +988,158 @@ static void setup_ftft(lw_handle *h, const lw_grid *gr) {
   t.gam = h->dalloc<double>(ns);
 }
 
+// poisinit of poiss_x_ft.f90 (18-52) with pmat (53-97), mwavenumber
+// (99-114) and coefmg (640-792), by the reference's expressions: the z
+// levels stacked after the top level, their couplings with the periodic
+// wrap of the end cells, the transfer weights, and the LU pivots of the y
+// lines per (k, j, x wavenumber)
+static void setup_xft(lw_handle *h, const lw_grid *gr) {
+  Xft &x = h->xft;
+  const lw_settings &st = h->st;
+  const int n1m = (int)gr->n1m, n2m = (int)gr->n2m, n3m = (int)gr->n3m, n3 = (int)gr->n3, nlev = (int)st.nlev;
+  const int n1mh = n1m / 2 + 1, n3md = 2 * n3m;
+  x.n1mh = n1mh;
+  x.n3md = n3md;
+  x.nlev = nlev;
+  x.nbli = (int)st.nbli;
+  auto z = vd(gr->z, 1, n3), zmp = vd(gr->zmp, 0, n3);
+  auto c2czi = vd(gr->c2czi, 0, n3), f2fzi = vd(gr->f2fzi, 0, n3), c2cyi = vd(gr->c2cyi, 0, gr->n2),
+       f2fyi = vd(gr->f2fyi, 0, gr->n2), f2fxi = vd(gr->f2fxi, 0, gr->n1);
+  auto kpv = vi(gr->kpv, 1, n3m), jpv = vi(gr->jpv, 1, n2m);
+  auto fixkl = vi(gr->fixkl, 1, n3m), fixku = vi(gr->fixku, 1, n3m), fixjl = vi(gr->fixjl, 1, n2m),
+       fixju = vi(gr->fixju, 1, n2m);
+  std::vector<double> ab(n3md + 2, 0.0), af(n3md + 2, 0.0), as(n2m + 2, 0.0), an(n2m + 2, 0.0);
+  std::vector<double> coi1(n3md + 2, 0.0), coi2(n3md + 2, 0.0), cor1(n3md + 2, 0.0), cor2(n3md + 2, 0.0);
+  std::vector<double> ai3(n1mh + 1, 0.0);
+  std::vector<int> kpm(n3md + 2, 0), kmm(n3md + 2, 0);
+  const size_t na = (size_t)n3md * n2m * n1mh;
+  std::vector<double> ac(na, 0.0), bet(na, 0.0), gam(na, 0.0);
+  auto L = [&](int k, int j, int i) { return (size_t)(k - 1) + (size_t)n3md * ((size_t)(j - 1) + (size_t)n2m * (i - 1)); };
+  for (int k = 1; k <= n3m; ++k) {
+    const int kp = kpv[k];
+    ab[k] = (1.0 - (double)fixkl[k]) * c2czi[k] * f2fzi[k];
+    af[k] = (1.0 - (double)fixku[k]) * c2czi[kp] * f2fzi[k];
+  }
+  for (int j = 1; j <= n2m; ++j) {
+    const int jp = jpv[j];
+    as[j] = (1.0 - (double)fixjl[j]) * c2cyi[j] * f2fyi[j];
+    an[j] = (1.0 - (double)fixju[j]) * c2cyi[jp] * f2fyi[j];
+  }
+  const double pi = acos(-1.0);
+  for (int i = 1; i <= n1mh; ++i)
+    ai3[i] = 2.0 * (1.0 - cos(2.0 * pi * (double)(i - 1) / (double)n1m)) * f2fxi[1] * f2fxi[1];
+  auto diag = [&](int kb, int ke) {
+    for (int j = 1; j <= n2m; ++j)
+      for (int k = kb; k <= ke; ++k) ac[L(k, j, 1)] = -1.0 * (ab[k] + af[k] + as[j] + an[j]);
+    for (int i = 2; i <= n1mh; ++i)
+      for (int j = 1; j <= n2m; ++j)
+        for (int k = kb; k <= ke; ++k) ac[L(k, j, i)] = ac[L(k, j, 1)] - ai3[i];
+  };
+  diag(1, n3m);
+  int kk[20][4] = {{0}};
+  const int minrow = n3m >> nlev;
+  std::vector<double> zmpm(n3md + 2, 0.0), kbz(n3md + 2, 0.0), kfz(n3md + 2, 0.0);
+  kk[nlev][1] = 1; kk[nlev][2] = n3m; kk[nlev][3] = n3m;
+  for (int k = 1; k <= n3m; ++k) zmpm[k] = zmp[k];
+  for (int l = nlev - 1; l >= 0; --l) {
+    kk[l][1] = kk[l + 1][1] + kk[l + 1][3];
+    kk[l][3] = minrow << l;
+    kk[l][2] = kk[l][1] + kk[l][3] - 1;
+    const int kbg = kk[l][1], kend = kk[l][2], ksp = 1 << (nlev - l);
+    int kc = 0;
+    for (int k = kbg; k <= kend; ++k) {
+      kc += 1;
+      zmpm[k] = 0.5 * (z[kc * ksp + 1] + z[(kc - 1) * ksp + 1]);
+      kbz[k] = (double)(1 - kbg / k);
+      kfz[k] = (double)(1 - k / kend);
+    }
+    kc = 0;
+    for (int k = kbg; k <= kend; ++k) {
+      kc += 1;
+      const double dzc = z[kc * ksp + 1] - z[(kc - 1) * ksp + 1];
+      ab[k] = kbz[k] / ((zmpm[k] - zmpm[k - 1]) * dzc);
+      af[k] = kfz[k] / ((zmpm[k + 1] - zmpm[k]) * dzc);
+    }
+    diag(kbg, kend);
+  }
+  for (int l = nlev; l >= 1; --l) {
+    int kbgh = kk[l][1];
+    for (int k = kk[l - 1][1]; k <= kk[l - 1][2]; ++k) {
+      cor1[k] = (zmpm[kbgh + 1] - zmpm[k]) / (zmpm[kbgh + 1] - zmpm[kbgh]);
+      cor2[k] = 1.0 - cor1[k];
+      kbgh += 2;
+    }
+  }
+  for (int l = 0; l <= nlev - 1; ++l) {
+    int kbgh = kk[l + 1][1] + 1;
+    for (int k = kk[l][1]; k <= kk[l][2] - 1; ++k) {
+      coi1[k] = (zmpm[k + 1] - zmpm[kbgh]) / (zmpm[k + 1] - zmpm[k]);
+      coi2[k] = 1.0 - coi1[k];
+      kbgh += 2;
+    }
+    coi1[kk[l][2]] = 1.0;
+  }
+  // z periodicity (749-789)
+  for (int l = nlev; l >= 0; --l) {
+    const int kbg = kk[l][1], kend = kk[l][2];
+    for (int k = kbg; k <= kend; ++k) {
+      kpm[k] = k + 1;
+      kmm[k] = k - 1;
+    }
+    kpm[kend] = kbg;
+    kmm[kbg] = kend;
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int kbg = kk[l][1], kend = kk[l][2], ksp = 1 << (nlev - l);
+    const double vdz = zmpm[kbg] - z[1] + z[n3] - zmpm[kend];
+    ab[kbg] = 1.0 / (vdz * (z[1 + ksp] - z[1]));
+    af[kend] = 1.0 / (vdz * (z[n3] - z[n3 - ksp]));
+  }
+  for (int l = nlev - 1; l >= 0; --l) diag(kk[l][1], kk[l][2]);
+  for (int l = 0; l <= nlev - 1; ++l) {
+    const int kbg = kk[l][1], kend = kk[l][2];
+    const double vdz = zmpm[kbg] - z[1] + z[n3] - zmpm[kend];
+    coi2[kend] = (zmpm[kk[l + 1][2]] - zmpm[kend]) / vdz;
+    coi1[kend] = 1.0 - coi2[kend];
+  }
+  for (int l = 0; l <= nlev; ++l)
+    for (int i = 1; i <= n1mh; ++i) {
+      for (int k = kk[l][1]; k <= kk[l][2]; ++k) bet[L(k, 1, i)] = 1.0 / ac[L(k, 1, i)];
+      for (int j = 2; j <= n2m; ++j)
+        for (int k = kk[l][1]; k <= kk[l][2]; ++k) {
+          gam[L(k, j, i)] = an[j - 1] * bet[L(k, j - 1, i)];
+          bet[L(k, j, i)] = 1.0 / (ac[L(k, j, i)] - as[j] * gam[L(k, j, i)]);
+        }
+    }
+  for (int l = 0; l <= nlev; ++l) {
+    x.kb[l] = kk[l][1];
+    x.ke[l] = kk[l][2];
+  }
+  std::vector<double> twr(n1m), twi(n1m);
+  const double pi2 = 8.0 * atan(1.0);
+  for (int q = 0; q < n1m; ++q) {
+    const double ang = -pi2 * (double)q / (double)n1m;
+    twr[q] = cos(ang);
+    twi[q] = sin(ang);
+  }
+  x.ab = h->upload(ab); x.af = h->upload(af); x.as = h->upload(as); x.an = h->upload(an);
+  x.coi1 = h->upload(coi1); x.coi2 = h->upload(coi2); x.cor1 = h->upload(cor1); x.cor2 = h->upload(cor2);
+  x.kpm = h->upload(kpm); x.kmm = h->upload(kmm);
+  x.ac = h->upload(ac); x.bet = h->upload(bet); x.gam = h->upload(gam);
+  x.twr = h->upload(twr); x.twi = h->upload(twi);
+  x.rhs = h->dalloc<double2>((size_t)n1mh * n2m * n3m);
+  x.pc = h->dalloc<double2>((size_t)n1mh * (n2m + 2) * n3m);
+  x.gg = h->dalloc<double2>((size_t)n1mh * (n2m + 2) * (n3md + 1));
+  x.resd = h->dalloc<double2>((size_t)n1mh * n2m * n3md);
+  x.tt = h->dalloc<double2>((size_t)n1mh);
+  x.done = h->dalloc<int>((size_t)n1mh);
+  x.ii = h->dalloc<int>((size_t)n1mh);
+  x.nact = h->dalloc<int>(2);
+  x.sres = h->dalloc<unsigned long long>((size_t)n1mh);
+  x.sfin = h->dalloc<double>((size_t)n1mh);
+  x.misc = h->dalloc<double>(2);
+}
+
 static void check_scope(const lw_settings *s, const lw_grid *g) {
   auto bad = [](const char *m) {
     g_err = m;
@@ -988,8 +1148,9 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
   if (s->yprdic) bad("y-periodic grids are not supported");
   // the x-periodic channel takes poiss_ftft (the reference Makefile's
   // choice for it); inflow/outflow grids the other two solvers
-  if (s->xprdic && s->poisson != 2) bad("an x-periodic grid needs the poiss_ftft solver (poisson = 2)");
-  if (!s->xprdic && s->poisson == 2) bad("poiss_ftft needs an x-periodic grid");
+  if (s->xprdic && s->poisson != 2 && s->poisson != 3)
+    bad("an x-periodic grid needs poiss_ftft (poisson = 2) or poiss_x_ft (poisson = 3)");
+  if (!s->xprdic && (s->poisson == 2 || s->poisson == 3)) bad("poiss_ftft and poiss_x_ft need an x-periodic grid");
   if (s->xprdic && s->ihtrans == 1) bad("heat transfer on an x-periodic grid is not supported");
   if (s->xprdic && s->iles == 1 && s->idvmon == 1)
     bad("the dynamic SGS coefficients on an x-periodic grid are not supported");
@@ -1004,11 +1165,21 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
   if (s->iles == 1 && s->ihtrans == 1 && s->itemdl != 0) bad("ITEMDL must be 0");
   if (s->iles == 1 && (s->filter < 1 || s->filter > 3)) bad("FILTER must be 1, 2 or 3");
   if (s->nlev < 1 || s->nlev > 19) bad("NLEV must be in 1..19");
-  if (g->n1m % (1 << s->nlev) != 0) bad("n1m must be divisible by 2**NLEV");
+  if (s->poisson <= 1 && g->n1m % (1 << s->nlev) != 0) bad("n1m must be divisible by 2**NLEV");
   if (g->n1m % 2 != 0) bad("n1m must be even");
   if (g->n3m < 2) bad("n3m must be at least 2");
   if (s->nbli < 1) bad("NBLI must be >= 1");
-  if (s->poisson < 0 || s->poisson > 2) bad("poisson must be 0 (poiss_z_ft), 1 (poiss_mgrd) or 2 (poiss_ftft)");
+  if (s->poisson < 0 || s->poisson > 3)
+    bad("poisson must be 0 (poiss_z_ft), 1 (poiss_mgrd), 2 (poiss_ftft) or 3 (poiss_x_ft)");
+  if (s->poisson == 3) {
+    // the z multigrid coarsens n3m into n3m / 2**NLEV lines; a level with
+    // an odd number of lines above 1 would put two coupled lines in one
+    // zebra half, whose result then depends on the order they are solved in
+    if (g->n3m % (1 << s->nlev) != 0) bad("poiss_x_ft: n3m must be divisible by 2**NLEV");
+    const int64_t nz0 = g->n3m >> s->nlev;
+    if (nz0 > 1 && nz0 % 2 == 1) bad("poiss_x_ft: n3m / 2**NLEV must be 1 or even");
+    if (s->nlev > 19) bad("poiss_x_ft: NLEV must be at most 19");
+  }
   if (s->poisson == 1) {
     // poiss_mgrd coarsens z as well; with IOLDV = 1 its V-cycle starts from
     // an uninitialised local array (poiss_mgrd.f90:236, 239)
@@ -1212,6 +1383,11 @@ int64_t lw_estimate_device_bytes(const lw_settings *s, int64_t n1m, int64_t n2m,
   } else if (s->poisson == 2) {
     // poiss_ftft: two complex spectra and the solve's gam
     b += (n3m / 2 + 1) * n2m * n1m * (2 * 16 + 8) + ni;
+  } else if (s->poisson == 3) {
+    // poiss_x_ft: rhs, pc, gg and resd per x wavenumber, ac, bet and gam
+    const int64_t n1mh = n1m / 2 + 1;
+    b += n1mh * (n2m + 2) * (n3m + (2 * n3m + 1)) * 16 + n1mh * n2m * (n3m + 2 * n3m) * 16 +
+         3 * n1mh * n2m * 2 * n3m * 8 + ni;
   } else {
     const int64_t ldj = (n2m + 2 + 7) / 8 * 8, plane = (2 * n1m + 1) * ldj, pcplane = (n1m + 2) * ldj;
     const int64_t n3mh = n3m / 2 + 1;
@@ -1678,9 +1854,87 @@ static void phase_poisson_mgrd(lw_handle *h) {
   h->mgres_n = 0;
 }
 
+// poisson of poiss_x_ft.f90 (115-232): the wavenumbers up to IMGSOR by
+// mg2d, the rest by gsor2d, each running its own cycle count; the kernels
+// skip wavenumbers whose flag is set, so the host checks the flags only
+// every few cycles. Then the rows mg2d (wavenumbers 1 and 2) and gsor2d (at
+// its limit) append to mgftresiduemax.dat, and the inverse DFT.
+static void phase_poisson_xft(lw_handle *h) {
+  const Geo &g = h->g;
+  const Xft &x = h->xft;
+  const lw_settings &st = h->st;
+  const int n1mh = x.n1mh;
+  launch_xft_forward(h->s, g, h->f, x);
+  HIPCHK(hipMemsetAsync(x.pc, 0, sizeof(double2) * (size_t)n1mh * (g.n2m + 2) * g.n3m, h->s));
+  HIPCHK(hipMemsetAsync(x.gg, 0, sizeof(double2) * (size_t)n1mh * (g.n2m + 2) * (x.n3md + 1), h->s));
+  HIPCHK(hipMemsetAsync(x.resd, 0, sizeof(double2) * (size_t)n1mh * g.n2m * x.n3md, h->s));
+  HIPCHK(hipMemsetAsync(x.done, 0, sizeof(int) * n1mh, h->s));
+  HIPCHK(hipMemsetAsync(x.ii, 0, sizeof(int) * n1mh, h->s));
+  HIPCHK(hipMemsetAsync(x.nact, 0, sizeof(int) * 2, h->s));
+  HIPCHK(hipMemsetAsync(x.sres, 0, sizeof(unsigned long long) * n1mh, h->s));
+  const double test1 = st.resid1 * h->sub.dtconsti;
+  const double test = test1 / (double)n1mh * (double)g.n1m * 0.9;
+  const int nmg = (int)std::max<int64_t>(0, std::min<int64_t>(st.imgsor, n1mh)), nsor = n1mh - nmg;
+  const int mgitr = (int)st.mgitr;
+  if (nmg > 0) launch_xft_toplevel(h->s, g, x, 1, nmg, 0, test, 0);
+  if (nsor > 0) launch_xft_sor(h->s, g, x, nmg + 1, nsor, 0.0, st.wwsor, test, 0);
+  int *nact = h->h_flags;
+  auto poll = [&]() {
+    HIPCHK(hipMemcpyAsync(nact, x.nact, 2 * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  };
+  poll();
+  int amg = nmg > 0 ? nact[0] : 0, asor = nsor > 0 ? nact[1] : 0;
+  for (int cyc = 1; cyc <= mgitr && (amg > 0 || asor > 0); ++cyc) {
+    if (amg > 0) {
+      launch_xft_vcycle(h->s, g, x, 1, nmg);
+      launch_xft_toplevel(h->s, g, x, 1, nmg, 1, test, cyc);
+    }
+    if (asor > 0) launch_xft_sor(h->s, g, x, nmg + 1, nsor, 1.0, st.wwsor, test, cyc);
+    check_launches("poiss_x_ft");
+    if (cyc % 4 == 0 || cyc == mgitr) {
+      poll();
+      amg = amg > 0 ? nact[0] : 0;
+      asor = asor > 0 ? nact[1] : 0;
+    }
+  }
+  std::vector<int> done(n1mh), ii(n1mh);
+  std::vector<double> sfin(n1mh);
+  HIPCHK(hipMemcpyAsync(done.data(), x.done, sizeof(int) * n1mh, hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(ii.data(), x.ii, sizeof(int) * n1mh, hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipMemcpyAsync(sfin.data(), x.sfin, sizeof(double) * n1mh, hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  if (h->sub.msub == 1) {
+    h->rrow_kind.clear(); h->rrow_kv.clear(); h->rrow_kk.clear(); h->rrow_val.clear();
+  }
+  bool limit = false;
+  int cmax = 0;
+  for (int q = 0; q < n1mh; ++q) {
+    const int iv = q + 1;
+    const int64_t kk = done[q] ? ii[q] : (int64_t)mgitr + 1;
+    const double rg = sfin[q] * h->sub.dtconst * (double)n1mh / (double)g.n1m;
+    if (!done[q]) limit = true;
+    if (iv <= nmg && iv <= 2) {
+      h->rrow_kind.push_back(2); h->rrow_kv.push_back(iv); h->rrow_kk.push_back(kk); h->rrow_val.push_back(rg);
+    } else if (iv > nmg && !done[q]) {
+      h->rrow_kind.push_back(3); h->rrow_kv.push_back(iv); h->rrow_kk.push_back(kk); h->rrow_val.push_back(rg);
+    }
+    const int c = done[q] ? ii[q] : mgitr;
+    cmax = c > cmax ? c : cmax;
+    h->cyc_sum += c;
+  }
+  if (limit) fprintf(stderr, "leswht: poiss_x_ft: ITERATION LIMIT EXCEEDED.\n");
+  if (cmax > h->cyc_max) h->cyc_max = cmax;
+  launch_xft_inverse(h->s, g, h->f, x, (int)g.ich);
+}
+
 static void phase_poisson(lw_handle *h) {
   if (h->st.poisson == 1) {
     phase_poisson_mgrd(h);
+    return;
+  }
+  if (h->st.poisson == 3) {
+    phase_poisson_xft(h);
     return;
   }
   if (h->st.poisson == 2) {

@@ -83,7 +83,7 @@ program solver_exec
     allocate (bsub(maxit), bicg(maxit), bloc(3 * maxit), brm(maxit))
     open (77, file='../output/ftr/poiss_itr.dat')
     open (78, file='../output/ftr/ftrpoittr.dat')
-  else if (st%poisson == 0) then
+  else if (st%poisson == 0 .or. st%poisson == 3) then
     allocate (rkind(64), rkv(64), rkk(64), rrm(64))
     open (77, file='../output/ftr/mgftresiduemax.dat')
   end if
@@ -107,8 +107,9 @@ program solver_exec
       do q = 1, min(nres, 3_c_int64_t)
         write (78, 103) t_begin, hs%dt, sicg(q), srm(q)
       end do
-    else if (st%poisson == 0) then
-      ! mg2d's rows, then gsor2d's per sweep (IMGSOR < n3m/2+1)
+    else if (st%poisson == 0 .or. st%poisson == 3) then
+      ! mg2d's rows, then gsor2d's (poiss_z_ft: per sweep above IMGSOR;
+      ! poiss_x_ft: at the iteration limit)
       nres = lw_get_residual_rows(h, rkind, rkv, rkk, rrm, 0_c_int64_t)
       if (nres > size(rkv)) then
         deallocate (rkind, rkv, rkk, rrm)
@@ -118,13 +119,19 @@ program solver_exec
       do q = 1, min(nres, int(size(rkv), c_int64_t))
         if (rkind(q) == 0) then
           write (77, 999) rkv(q), rkk(q), rrm(q)
-        else
+        else if (rkind(q) == 1) then
           write (77, 998) rkv(q), rkk(q), rrm(q)
+        else if (rkind(q) == 2) then
+          write (77, 997) ' mg, time = ', t_begin, rkv(q), rkk(q), rrm(q)
+        else
+          write (77, 996) 'SOR', rkv(q), rkk(q), rrm(q)
         end if
       end do
     end if
 999 format('KV=', i4, 3x, 'KK=', i4, 3x, 'RM=', es24.16)
 998 format('KV=', i3, 3x, 'KK=', i3, 3x, 'RG=', es24.16)
+997 format(a13, f13.5, ' IV=', i5, ' II=', i5, ' RG=', es16.8)
+996 format(a5, '  IV=', i5, '  II=', i5, '  RG=', es23.15)
 102 format(i4, es15.7, 3i4)
 103 format(2f13.5, i5, es15.7)
 

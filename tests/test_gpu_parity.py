@@ -66,11 +66,11 @@ def test_gpu_phases_match_oracle(mode, oracle_lib):
             assert rel(fo[k], fg[k]) <= 1e-11, ("substep", ms, k)
 
 
-@pytest.mark.parametrize("mode", ["chan_cfr", "chan_cpg"])
+@pytest.mark.parametrize("mode", ["chan_cfr", "chan_cpg", "chan_cfr_xft", "chan_cpg_xsor"])
 def test_gpu_channel_phases_match_oracle(mode, oracle_lib):
     """The x-periodic channel phase by phase: the momentum step with the
-    periodic x sweep and pmi(0), poiss_ftft's phi (direct DFT sums here,
-    FFTs in the oracle: 1e-10), the projection with the x ghosts, and
+    periodic x sweep and pmi(0), phi of poiss_ftft or poiss_x_ft (direct
+    DFT sums here, FFTs in the oracle: 1e-10), the projection with the x ghosts, and
     meanpg's pmi(0:4) after it, with the flow-rate controller's integral."""
     import oracle
     view = _view((32, 32, 8), mode, 1)
@@ -244,17 +244,21 @@ def _hist_rows(sol, nend):
     """Per step: the history rows, the mgftresiduemax.dat rows and, every NTR
     steps, the tracer rows, as the reference writes them."""
     rows = {"hist": [], "cdcl": [], "hflux": [], "temp": [], "mgres": [], "bicg_itr": [], "bicg_solve": [],
-            "cmfr": []}
+            "cmfr": [], "xres": []}
     ntrace, ntr, _ = sol.trace_info()
     trows = {c: [] for c in "uvwp"}
     sts = sol.settings()
     mgrd = sts["poisson"] == 1
     heat = sts["ihtrans"] == 1
     for _ in range(nend):
+        t0 = sol.history()["time"]
         sol.step(1)
         h = sol.history()
-        # mg2d's RM rows and gsor2d's RG rows, in the file's order
-        rows["mgres"] += [[kv, kk, v] for _, kv, kk, v in sol.residual_rows()]
+        # mg2d's RM rows and gsor2d's RG rows, in the file's order; for
+        # poiss_x_ft its ' mg' rows (with the time the step began) and 'SOR' rows
+        rr = sol.residual_rows()
+        rows["mgres"] += [[kv, kk, v] for kind, kv, kk, v in rr if kind < 2]
+        rows["xres"] += [[kind - 2, t0 if kind == 2 else 0.0, kv, kk, v] for kind, kv, kk, v in rr if kind >= 2]
         if mgrd:
             it, sv = sol.bicg_log()
             rows["bicg_itr"] += [[r[1], r[2], r[3], r[4], r[5]] for r in it]
@@ -292,6 +296,17 @@ def _check_mgres(name, ref, got):
     assert ref.shape == got.shape, (name, ref.shape, got.shape)
     assert np.array_equal(ref[:, :2], got[:, :2]), (name, "kv/kk")
     np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=1e-3, atol=1e-12, err_msg=name)
+
+
+def _check_xres(name, ref, got):
+    """poiss_x_ft's rows (kind 0 ' mg', 1 'SOR'; time, IV, II, RG): kinds,
+    wavenumbers and cycle counts exact, the time to its f13.5 print, RG to
+    1e-3 as in _check_mgres."""
+    got = np.asarray(got, dtype=np.float64).reshape(-1, 5)
+    assert ref.shape == got.shape, (name, ref.shape, got.shape)
+    assert np.array_equal(ref[:, [0, 2, 3]], got[:, [0, 2, 3]]), (name, "kind/iv/ii")
+    assert np.all(np.abs(ref[:, 1] - got[:, 1]) <= 6e-6), (name, "time")
+    np.testing.assert_allclose(got[:, 4], ref[:, 4], rtol=1e-3, atol=1e-14, err_msg=name)
 
 
 def _check_bicg(name, z, got):
@@ -364,6 +379,20 @@ def test_gpu_history_matches_reference_golden(name):
         _check_trace(name, z, rows)
     if "cmfr" in z.files:
         _check_rows(name, "cmfr", z["cmfr"], rows["cmfr"])
+    if "xres" in z.files:
+        _check_xres(name, z["xres"], rows["xres"])
+
+
+def _xft_file_rows(path):
+    """poiss_x_ft's mgftresiduemax.dat lines as (kind, time, iv, ii, rg)."""
+    rows = []
+    for ln in open(path):
+        t = ln.replace("=", " ").split()
+        if len(t) == 9 and t[0] == "mg,":
+            rows.append((0.0, float(t[2]), float(t[4]), float(t[6]), float(t[8])))
+        elif len(t) == 7 and t[0] == "SOR":
+            rows.append((1.0, 0.0, float(t[2]), float(t[4]), float(t[6])))
+    return np.array(rows).reshape(-1, 5)
 
 
 def _mg_file_rows(path):
@@ -376,7 +405,7 @@ def _mg_file_rows(path):
 
 
 @pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4", "mgrd_cyl32_les_3", "cyl32_sor_3",
-                                  "chan32_cfr_4"])
+                                  "chan32_cfr_4", "xft_chan32_sor_3"])
 def test_gpu_fortran_host_matches_reference(name):
     """The Fortran solver_exec (ISO_C_BINDING host over the C ABI) run on the
     reference's case directory writes the reference's fld file, history rows,
@@ -393,8 +422,8 @@ def test_gpu_fortran_host_matches_reference(name):
         print("fortran host", name, flush=True)
         env = dict(os.environ)
         env.pop("LW_POISSON", None)
-        if meta.get("poisson") == "mgrd":
-            env["LW_POISSON"] = "mgrd"  # the reference's link-time choice of poiss_mgrd
+        if meta.get("poisson") in ("mgrd", "x_ft"):
+            env["LW_POISSON"] = meta["poisson"]  # the reference's link-time choice of its Poisson solver
         res = subprocess.run([EXE], cwd=os.path.join(root, "02_solver"), capture_output=True, text=True,
                              timeout=120, env=env)
         assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
@@ -413,6 +442,8 @@ def test_gpu_fortran_host_matches_reference(name):
         assert len(tim) == meta["nend"] and all(len(r) == 5 for r in tim)
         if "mgres" in z.files:
             _check_mgres(name, z["mgres"], _mg_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat")))
+        if "xres" in z.files:
+            _check_xres(name, z["xres"], _xft_file_rows(os.path.join(root, "output/ftr/mgftresiduemax.dat")))
         if "bicg_solve" in z.files:
             txt = lambda f: np.array([[float(x) for x in ln.split()] for ln in  # noqa: E731
                                       open(os.path.join(root, "output/ftr", f)) if ln.strip()])
