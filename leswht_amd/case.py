@@ -1063,7 +1063,8 @@ def load_case_dir(root: str) -> dict:
 
 def makefld(s: Settings, g: dict, body: Callable, body_args: Tuple = ()) -> dict:
     """The initial field of IREAD = 0 (makefld, misc_init.f90:443-607): u =
-    UDRV_I where the level set at the u point is >= 1e-10 (ICH = 0), v = w =
+    the ICH's profile (UDRV_I; 1.5 UDRV_I (1 - y^2); 18 (1 - y^2)) where the
+    level set at the u point is >= 1e-10, v = w =
     p = 0, u = 0 on the rows next to wall boundaries; T = TSOL_I, TFLU_I at
     fluid cell centres of rows 1..n2m, the inlet ghost column by the level
     set, y (and z) ghost rows by the thermal BCs. The u loop reads x(0), which
@@ -1074,7 +1075,14 @@ def makefld(s: Settings, g: dict, body: Callable, body_args: Tuple = ()) -> dict
     x0 = np.concatenate([[0.0], g["x"]])  # Fortran x(0..n1)
     xu, yc, zc = np.meshgrid(x0, g["ymp"], g["zmp"], indexing="ij")
     phi = body(xu, yc, zc, *body_args)
-    u = np.where(phi >= 1.e-10, s.udrv_i, 0.0)
+    # the profile by ICH (misc_init.f90:471-481)
+    if s.ich == 1:
+        prof = 1.5 * s.udrv_i * (1.0 - yc * yc)
+    elif s.ich == 2:
+        prof = (1.5 * (1.0 - yc * yc)) * 12.0
+    else:
+        prof = np.full_like(yc, float(s.udrv_i))
+    u = np.where(phi >= 1.e-10, prof, 0.0)
     if s.bc_ybtm == 0:
         u[1:, 0:2, :] = 0.0
     if s.bc_ytop == 0:

@@ -378,7 +378,7 @@ void read_field(Case &c, const std::string &dir) {
 }
 
 // makefld (misc_init.f90:443-607), the initial field when IREAD = 0: u =
-// UDRV_I at fluid u points (the ICH = 0 profile), v = w = p = 0, no-slip
+// the ICH's profile at fluid u points, v = w = p = 0, no-slip
 // rows at wall boundaries; T = TSOL_I, then TFLU_I at fluid cell centres of
 // rows 1..n2m, the inlet ghost by the level set and the y (and z) ghosts by
 // the thermal BCs. The u loop starts at i = 0 and reads x(0), which the
@@ -395,10 +395,17 @@ void make_field(Case &c, const lw_body *body) {
   c.m = 0;
   c.time = 0.0;
   c.dt = s.dt_size;
+  // the profile by ICH (misc_init.f90:471-481): uniform UDRV_I, the
+  // laminar channel 1.5 UDRV_I (1 - y^2), or that of bulk velocity 12
   for (int64_t k = 0; k <= c.n3; ++k)
-    for (int64_t j = 0; j <= c.n2; ++j)
+    for (int64_t j = 0; j <= c.n2; ++j) {
+      const double y = c.ymp[j];
+      const double prof = s.ich == 1 ? 1.5 * s.udrv_i * (1.0 - y * y)
+                          : s.ich == 2 ? (1.5 * (1.0 - y * y)) * 12.0
+                                       : s.udrv_i;
       for (int64_t i = 0; i <= c.n1; ++i)
-        c.u[P(i, j, k)] = fb(c.x[i], c.ymp[j], c.zmp[k]) >= 1.e-10 ? s.udrv_i : 0.0;
+        c.u[P(i, j, k)] = fb(c.x[i], c.ymp[j], c.zmp[k]) >= 1.e-10 ? prof : 0.0;
+    }
   for (int64_t k = 0; k <= c.n3; ++k)
     for (int64_t i = 1; i <= c.n1; ++i) {
       if (s.bc_ybtm == 0) c.u[P(i, 0, k)] = c.u[P(i, 1, k)] = 0.0;

@@ -1,7 +1,8 @@
 // gfx950 kernels of the x-periodic channel (ICH != 0, XPRDIC = 1):
 // the mean pressure gradient of meanpg (slv_mmtm.f90:2-177) with its
 // constant-flow-rate controller, the fcmfr.dat quantities of writehistory
-// (misc_hist.f90:91-137), and the x- and z-FFT Poisson solver of
+// (misc_hist.f90:91-137), the start-up perturbation of addperturb
+// (misc_init.f90:789-882), and the x- and z-FFT Poisson solver of
 // poiss_ftft.f90 (the reference Makefile's default for x-periodic grids).
 //
 // Channel scalars f.chan (doubles):
@@ -174,6 +175,73 @@ void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const lw_settin
                       double dt) {
   launch_chan_sums(s, g, f, st);
   LW_LAUNCH("k_chan_hist", kChB * 16.0, 0, k_chan_hist, dim3(1), dim3(kChT), 0, s, g, f, hist, time, dt, st.avg_tst);
+}
+
+// ---------------------------------------------------------------------------
+// addperturb (misc_init.f90:789-882): u += ptbu(j, k) and w += ptbw(i, j) in
+// the band 0.5 < ymp < 0.99 of the fluid points (funcbody > 1e-10), each
+// less its mean over the fluid section; the profiles come from the host,
+// computed by the reference's expressions. Fixed-order block partials.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kChT) void k_ptb_add(Geo g, Fld f, Ptb t) {
+  __shared__ double sh[kChT / 64];
+  const int n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
+  const long ncell = (long)n1m * n2m * n3m;
+  const long str = (long)kChB * kChT;
+  const uint8_t *mask = W ? t.mw : t.mu;
+  double *a = W ? f.w : f.u;
+  double area = 0.0, rate = 0.0;
+  for (long q = (long)blockIdx.x * kChT + threadIdx.x; q < ncell; q += str) {
+    if (!mask[q]) continue;
+    const int i = 1 + (int)(q % n1m), j = 1 + (int)((q / n1m) % n2m), k = 1 + (int)(q / ((long)n1m * n2m));
+    area = area + (W ? g.f2fx[i] * g.f2fy[j] : g.f2fy[j] * g.f2fz[k]);
+    if (t.band[j]) {
+      const double p = W ? t.pw[(i - 1) + (long)n1m * (j - 1)] : t.pu[(j - 1) + (long)n2m * (k - 1)];
+      const long c = P(g, i, j, k);
+      a[c] = a[c] + p;
+      rate = rate + (W ? p * g.f2fx[i] * g.f2fy[j] : p * g.f2fy[j] * g.f2fz[k]);
+    }
+  }
+  const double s0 = bsum(area, sh), s1 = bsum(rate, sh);
+  if (threadIdx.x == 0) {
+    t.part[blockIdx.x] = s0;
+    t.part[kChB + blockIdx.x] = s1;
+  }
+}
+
+// adj = rate / area when the section holds fluid: part[2 kChB], and the flag
+__global__ __launch_bounds__(kChT) void k_ptb_adj(Ptb t) {
+  __shared__ double sh[kChT / 64];
+  const double area = bsum(threadIdx.x < kChB ? t.part[threadIdx.x] : 0.0, sh);
+  const double rate = bsum(threadIdx.x < kChB ? t.part[kChB + threadIdx.x] : 0.0, sh);
+  if (threadIdx.x == 0) {
+    t.part[2 * kChB] = area > 0.0 ? rate / area : 0.0;
+    t.part[2 * kChB + 1] = area > 0.0 ? 1.0 : 0.0;
+  }
+}
+
+template <int W>
+__global__ void k_ptb_sub(Geo g, Fld f, Ptb t) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  const int n1m = g.n1m, n2m = g.n2m;
+  if (q >= (long)n1m * n2m * g.n3m) return;
+  if (!(W ? t.mw : t.mu)[q] || t.part[2 * kChB + 1] == 0.0) return;
+  const int i = 1 + (int)(q % n1m), j = 1 + (int)((q / n1m) % n2m), k = 1 + (int)(q / ((long)n1m * n2m));
+  double *a = W ? f.w : f.u;
+  const long c = P(g, i, j, k);
+  a[c] = a[c] - t.part[2 * kChB];
+}
+
+void launch_addperturb(hipStream_t s, const Geo &g, const Fld &f, const Ptb &t) {
+  const long n = (long)g.n1m * g.n2m * g.n3m;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  LW_LAUNCH("k_ptb_add", (double)n * 17.0, 0, k_ptb_add<0>, dim3(kChB), dim3(kChT), 0, s, g, f, t);
+  LW_LAUNCH("k_ptb_adj", kChB * 16.0, 0, k_ptb_adj, dim3(1), dim3(kChT), 0, s, t);
+  LW_LAUNCH("k_ptb_sub", (double)n * 17.0, 0, k_ptb_sub<0>, dim3(nb), dim3(256), 0, s, g, f, t);
+  LW_LAUNCH("k_ptb_add", (double)n * 17.0, 0, k_ptb_add<1>, dim3(kChB), dim3(kChT), 0, s, g, f, t);
+  LW_LAUNCH("k_ptb_adj", kChB * 16.0, 0, k_ptb_adj, dim3(1), dim3(kChT), 0, s, t);
+  LW_LAUNCH("k_ptb_sub", (double)n * 17.0, 0, k_ptb_sub<1>, dim3(nb), dim3(256), 0, s, g, f, t);
 }
 
 // ---------------------------------------------------------------------------

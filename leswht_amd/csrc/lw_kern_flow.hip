@@ -1395,6 +1395,12 @@ __global__ void k_xghost_p(Geo g, Fld f) {
   f.p[P(g, g.n1, j, k)] = f.p[P(g, 1, j, k)];
 }
 
+void launch_prdic_uvw(hipStream_t s, const Geo &g, const Fld &f) {
+  if (g.xprdic) launch_xghost_uvw(s, g, f);
+  LW_LAUNCH("k_zghost_uvw", (double)(g.n1 + 1) * (g.n2 + 1) * 8.0 * 12, 0, k_zghost_uvw,
+            dim3((g.n1 + 64) / 64, g.n2 + 1), dim3(64), 0, s, g, f);
+}
+
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f) {
   LW_LAUNCH("k_retrv_mom", (double)(g.n2 + 1) * (g.n3 + 1) * 8.0 * 9, 0, k_retrv_mom,
             dim3((g.n2 + 64) / 64, g.n3 + 1), dim3(64), 0, s, g, f);

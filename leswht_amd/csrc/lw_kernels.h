@@ -143,6 +143,18 @@ void launch_meanpg(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu,
 void launch_pmiavg(hipStream_t s, const Fld &f, double alpha, int reset);
 void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const lw_settings &st, double *hist, double time,
                       double dt);
+// addperturb's tables (lw_kern_chan.hip): fluid masks of the interior u
+// and w points (funcbody > 1e-10, i fastest), the profiles ptbu(j, k) and
+// ptbw(i, j), the band 0.5 < ymp(j) < 0.99, and partial sums (2 * 64 + 2)
+struct Ptb {
+  const uint8_t *mu, *mw, *band;
+  const double *pu, *pw;
+  double *part;
+};
+void launch_addperturb(hipStream_t s, const Geo &g, const Fld &f, const Ptb &t);
+// prdic_adj_uvw(0) (slv_mmtm.f90:1746-1857): the x (x-periodic grids), then
+// z periodic ghost planes of u, v, w
+void launch_prdic_uvw(hipStream_t s, const Geo &g, const Fld &f);
 struct Ftft {
   int n3mh;
   const double *twx_r, *twx_i;  // e^{-2 pi i q / n1m}, q = 0..n1m-1

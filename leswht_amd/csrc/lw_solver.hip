@@ -222,6 +222,10 @@ struct lw_handle {
   Ftft ftft{};
   // poiss_x_ft: tables, level planes and per-wavenumber state
   Xft xft{};
+  // addperturb (IREAD = 0, EPS_PTR != 0): tables; pending before the first
+  // step when PTB_TST <= 0; applied once
+  Ptb ptb{};
+  int ptb_armed = 0, ptb_pending = 0, ptb_applied = 0;
   std::vector<int64_t> bicg_sub, bicg_icg, bicg_loc, solve_icg;
   std::vector<double> bicg_rm, solve_rm;
   // tracer (misc_hist.f90:184-209): cells whose centre values are traced
@@ -698,6 +702,40 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
     for (int j = 0; j <= n2; ++j)
       if (funcbody(h->body, h->xmp[0], h->ymp[j], h->zmp[k]) <= 1.e-10) mask[PIh(0, j, k)] |= 128;
   f.mask = h->upload(mask);
+  if (st.iread == 0 && st.eps_ptr != 0.0) {
+    // addperturb's profiles by the reference's expressions
+    // (misc_init.f90:805-866) and its fluid tests (funcbody > 1e-10)
+    h->ptb_armed = 1;
+    const double ret = 180.0, pi = acos(-1.0), yr = y[n2 - 2];
+    std::vector<uint8_t> mu((size_t)n1m * n2m * n3m, 0), mw((size_t)n1m * n2m * n3m, 0), band(n2 + 1, 0);
+    std::vector<double> pu((size_t)n2m * n3m, 0.0), pw((size_t)n1m * n2m, 0.0);
+    for (int j = 1; j <= n2m; ++j) band[j] = h->ymp[j] > .5 && h->ymp[j] < .99;
+    for (int k = 1; k <= n3m; ++k)
+      for (int j = 1; j <= n2m; ++j) {
+        const double d = fabs(yr - h->ymp[j]);
+        pu[(size_t)(j - 1) + (size_t)n2m * (k - 1)] =
+            st.eps_ptr * st.udrv_i * cos(h->zmp[k] / gr->zl * pi) * d * ret * exp(-.01 * pow(d * ret, 2.0) + .5);
+      }
+    for (int j = 1; j <= n2m; ++j)
+      for (int i = 1; i <= n1m; ++i) {
+        const double d = fabs(yr - h->ymp[j]);
+        pw[(size_t)(i - 1) + (size_t)n1m * (j - 1)] =
+            st.eps_ptr * sin(h->xmp[i] / gr->xl * pi) * d * ret * exp(-.01 * pow(d * ret, 2.0));
+      }
+    for (int k = 1; k <= n3m; ++k)
+      for (int j = 1; j <= n2m; ++j)
+        for (int i = 1; i <= n1m; ++i) {
+          const size_t q = (size_t)(i - 1) + (size_t)n1m * ((j - 1) + (size_t)n2m * (k - 1));
+          mu[q] = funcbody(h->body, h->x[i], h->ymp[j], h->zmp[k]) > 1.e-10;
+          mw[q] = funcbody(h->body, h->xmp[i], h->ymp[j], z[k]) > 1.e-10;
+        }
+    h->ptb.mu = h->upload(mu);
+    h->ptb.mw = h->upload(mw);
+    h->ptb.band = h->upload(band);
+    h->ptb.pu = h->upload(pu);
+    h->ptb.pw = h->upload(pw);
+    h->ptb.part = h->dalloc<double>(2 * 64 + 2);
+  }
   if (st.ich != 0) {
     // meanpg's and writehistory's fluid test at the u points (x(i), ymp(j),
     // zmp(k)) of the interior, i fastest
@@ -1423,6 +1461,10 @@ int lw_set_fields(lw_handle *h, const double *u, const double *v, const double *
       h->time = 0.;
     }
     h->ntime = 0;
+    // addperturb before the first step when PTB_TST <= 0 (solver.f90:76-80):
+    // after the initial field was taken (meanpg below) and could be written
+    h->ptb_applied = 0;
+    h->ptb_pending = h->ptb_armed && h->st.ptb_tst <= 0.0;
     // solver.f90:54: the channel's first mean pressure gradient, from the
     // loaded field (ntime 0: no controller term); the controller's integral
     // and the fcmfr.dat sums start from zero
@@ -1584,6 +1626,15 @@ static void substepinit(lw_handle *h, int ms) {
   // pmiavg = pmiavg + pmi(0)*2*alpha (slv_auxi.f90:118), restarted by
   // stepinit (:13)
   if (h->g.ich != 0) launch_pmiavg(h->s, h->f, sb.alpha, ms == 1);
+}
+
+// addperturb with prdic_adj_uvw(0) (solver.f90:76-80, 153-160)
+static void apply_perturb(lw_handle *h) {
+  launch_addperturb(h->s, h->g, h->f, h->ptb);
+  launch_prdic_uvw(h->s, h->g, h->f);
+  check_launches("addperturb");
+  h->ptb_applied = 1;
+  h->ptb_pending = 0;
 }
 
 // after pcalc (solver.f90:136): the channel's mean pressure gradient for the
@@ -2044,7 +2095,11 @@ static void phase_energy(lw_handle *h) {
 // temperatures of writehistory
 static void end_of_step(lw_handle *h) {
   const lw_settings &st = h->st;
+  const double time_prev = h->time;
   h->time = h->time + h->dt;
+  // addperturb once, in the step that reaches PTB_TST (solver.f90:150-160),
+  // before the step's diagnostics
+  if (h->ptb_armed && !h->ptb_applied && time_prev < st.ptb_tst && h->time >= st.ptb_tst) apply_perturb(h);
   launch_diag(h->s, h->g, h->f, (int)(st.masson == 1 && st.ibmon == 1));
   const int hstep = st.npin > 0 ? (h->ntime % st.npin) == 0 : 1;
   if (hstep) h->ihist += 1;
@@ -2417,6 +2472,7 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
     for (int64_t n = 0; n < nsteps; ++n) {
       h->cyc_max = 0;
       h->cyc_sum = 0;
+      if (h->ptb_pending) apply_perturb(h);
       stepinit(h);
       check_launches("stepinit");
       for (int ms = 1; ms <= 3; ++ms) {
@@ -2470,6 +2526,7 @@ extern "C" int lw_phase(lw_handle *h, int phase) {
     if (phase == 0) {
       h->cyc_max = 0;
       h->cyc_sum = 0;
+      if (h->ptb_pending) apply_perturb(h);
       stepinit(h);
     } else if (phase >= 11 && phase <= 13) substepinit(h, phase - 10);
     else if (phase == 1) phase_sgs(h);

@@ -99,6 +99,9 @@ struct ora {
   double *ak1f, *ak3f;
   /* poiss_x_ft: its tables and residual rows */
   struct xft *xf;
+  /* addperturb: 1 once applied (solver.f90:66,155); pending at the start
+   * of the run (PTB_TST <= 0, solver.f90:76-80) */
+  int ptb_applied, ptb_pending;
 };
 
 /* ---- indexing ------------------------------------------------------- */
@@ -2057,6 +2060,78 @@ static void channel_history(ora *o) {
   for (int q = 0; q < 3; ++q) o->cmfr[4 + q] = avg[q];
 }
 
+/* addperturb (misc_init.f90:789-882): a wall-normal-shaped streamwise
+ * (u, cos in z) and spanwise (w, sin in x) perturbation in the band
+ * 0.5 < ymp < 0.99, each less its mean over the fluid section, then the
+ * periodic ghosts of prdic_adj_uvw(0) (slv_mmtm.f90:1746-1857) */
+static void addperturb(ora *o) {
+  const double ret = 180.0, pi = acos(-1.0);
+  int64_t n1 = o->n1, n2 = o->n2, n3 = o->n3, n1m = o->n1m, n2m = o->n2m, n3m = o->n3m;
+  const double yr = o->y[n2 - 2];
+  double area = 0.0, rate = 0.0;
+  for (int64_t i = 1; i <= n1m; ++i)
+    for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t k = 1; k <= n3m; ++k)
+        if (funcbody(o, o->x[i], o->ymp[j], o->zmp[k]) > 1.e-10) {
+          area = area + o->f2fy[j] * o->f2fz[k];
+          if (o->ymp[j] > .5 && o->ymp[j] < .99) {
+            const double d = fabs(yr - o->ymp[j]);
+            const double ptb = o->s.eps_ptr * o->s.udrv_i * cos(o->zmp[k] / o->zl * pi) * d * ret *
+                               exp(-.01 * pow(d * ret, 2.0) + .5);
+            F(o->u, i, j, k) = F(o->u, i, j, k) + ptb;
+            rate = rate + ptb * o->f2fy[j] * o->f2fz[k];
+          }
+        }
+  if (area > 0.0) {
+    const double adj = rate / area;
+    for (int64_t i = 1; i <= n1m; ++i)
+      for (int64_t j = 1; j <= n2m; ++j)
+        for (int64_t k = 1; k <= n3m; ++k)
+          if (funcbody(o, o->x[i], o->ymp[j], o->zmp[k]) > 1.e-10) F(o->u, i, j, k) = F(o->u, i, j, k) - adj;
+  }
+  area = 0.0;
+  rate = 0.0;
+  for (int64_t k = 1; k <= n3m; ++k)
+    for (int64_t i = 1; i <= n1m; ++i)
+      for (int64_t j = 1; j <= n2m; ++j)
+        if (funcbody(o, o->xmp[i], o->ymp[j], o->z[k]) > 1.e-10) {
+          area = area + o->f2fx[i] * o->f2fy[j];
+          if (o->ymp[j] > .5 && o->ymp[j] < .99) {
+            const double d = fabs(yr - o->ymp[j]);
+            const double ptb = o->s.eps_ptr * sin(o->xmp[i] / o->xl * pi) * d * ret * exp(-.01 * pow(d * ret, 2.0));
+            F(o->w, i, j, k) = F(o->w, i, j, k) + ptb;
+            rate = rate + ptb * o->f2fx[i] * o->f2fy[j];
+          }
+        }
+  if (area > 0.0) {
+    const double adj = rate / area;
+    for (int64_t k = 1; k <= n3m; ++k)
+      for (int64_t i = 1; i <= n1m; ++i)
+        for (int64_t j = 1; j <= n2m; ++j)
+          if (funcbody(o, o->xmp[i], o->ymp[j], o->z[k]) > 1.e-10) F(o->w, i, j, k) = F(o->w, i, j, k) - adj;
+  }
+  if (o->s.xprdic == 1) xghost_uvw(o);
+  if (o->s.zprdic == 1) {
+    for (int64_t j = 0; j <= n2; ++j)
+      for (int64_t i = 1; i <= n1; ++i) {
+        F(o->u, i, j, 0) = F(o->u, i, j, n3m);
+        F(o->u, i, j, n3) = F(o->u, i, j, 1);
+      }
+    for (int64_t j = 1; j <= n2; ++j)
+      for (int64_t i = 0; i <= n1; ++i) {
+        F(o->v, i, j, 0) = F(o->v, i, j, n3m);
+        F(o->v, i, j, n3) = F(o->v, i, j, 1);
+      }
+    for (int64_t j = 0; j <= n2; ++j)
+      for (int64_t i = 0; i <= n1; ++i) {
+        F(o->w, i, j, 0) = F(o->w, i, j, n3m);
+        F(o->w, i, j, n3) = F(o->w, i, j, 1);
+      }
+  }
+  o->ptb_applied = 1;
+  o->ptb_pending = 0;
+}
+
 /* slv_auxi.f90:519-570 lagforce */
 static void lagforce(ora *o) {
   if (o->msub == 1) { o->dudta = 0.; o->dvdta = 0.; o->dwdta = 0.; }
@@ -2289,6 +2364,9 @@ static void rhsnlhs_t(ora *o) {
 /* slv_auxi.f90:2-123 stepinit / cfl / substepinit                        */
 /* ===================================================================== */
 void ora_stepinit(ora *o) {
+  /* the run-start perturbation (PTB_TST <= 0), after the initial field was
+   * written and meanpg took it (solver.f90:50-54, 76-80) */
+  if (o->ptb_pending) addperturb(o);
   o->ntime = o->ntime + 1;
   o->m = o->ntime;
   if (o->s.ich != 0) o->pmiavg = 0.;
@@ -2334,6 +2412,8 @@ void ora_init(ora *o) {
   /* solver.f90:54: the channel's first mean pressure gradient, from the
    * loaded field, before dttimeinit (ntime is still 0: no controller term) */
   if (o->s.ich != 0) meanpg(o);
+  o->ptb_applied = 0;
+  o->ptb_pending = o->s.iread == 0 && o->s.eps_ptr != 0.0 && o->s.ptb_tst <= 0.0;
   dttimeinit(o);
   rk3coefinit(o);
   lhsinit(o);
@@ -2530,7 +2610,12 @@ void ora_step(ora *o, ora_diag *d) {
     if (o->s.ibmon == 1) lagforce(o);
     if (o->s.ihtrans == 1) rhsnlhs_t(o);
   }
+  const double time_prev = o->time;
   o->time = o->time + o->dt;
+  /* addperturb once, in the step that reaches PTB_TST (solver.f90:150-160) */
+  if (o->s.iread == 0 && o->s.eps_ptr != 0.0 && !o->ptb_applied && time_prev < o->s.ptb_tst &&
+      o->time >= o->s.ptb_tst)
+    addperturb(o);
   diagnostics(o);
   /* writehistory runs on steps with mod(ntime, npin) = 0 (solver.f90:166) */
   if (o->s.ich != 0 && (o->s.npin <= 0 || o->ntime % o->s.npin == 0)) channel_history(o);

@@ -52,6 +52,10 @@ CASES = {
     # wavenumber 3 up to its iteration limit with its 'SOR' rows
     "xft_chan32_cfr_4": dict(dims=(32, 32, 8), mode="chan_cfr_xft", nend=4, poisson="x_ft"),
     "xft_chan32_sor_3": dict(dims=(32, 32, 8), mode="chan_cpg_xsor", nend=3, poisson="x_ft"),
+    # IREAD = 0 channels: makefld's ICH profiles and addperturb, in the step
+    # that reaches PTB_TST and before the first step (PTB_TST = 0)
+    "chan32_ptb_4": dict(dims=(32, 32, 8), mode="chan_cfr_ptb", nend=4, poisson="ftft"),
+    "chan32_ptb0_3": dict(dims=(32, 32, 8), mode="chan_cpg_ptb0", nend=3, poisson="ftft"),
     # the 3-D MG-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90) linked
     # in place of poiss_z_ft.f90 (oracle/_ref/solver_exec_mgrd)
     "mgrd_cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3, poisson="mgrd"),
@@ -133,7 +137,7 @@ def main(names=None):
         if spec.settings.iread == 0:
             f0 = C.read_fld(os.path.join(root, "output/field/fld0000000"))
             for q in ("u", "v", "w", "p", "t"):
-                extra["init_" + q] = f0[q]
+                extra["init_" + q] = f0.get(q, np.zeros(0))
         np.savez_compressed(os.path.join(HERE, name + ".npz"),
                             u=fl["u"], v=fl["v"], w=fl["w"], p=fl["p"], t=fl.get("t", np.zeros(0)),
                             time=fl["time"], dt=fl["dt"], ihist=fl["ihist"], **rows, **extra,

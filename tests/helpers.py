@@ -18,7 +18,7 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
                        cflfac=0.5)
         return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
-    if mode in ("chan_cpg", "chan_cfr", "chan_cfr_xft", "chan_cpg_xsor"):
+    if mode in ("chan_cpg", "chan_cfr", "chan_cfr_xft", "chan_cpg_xsor", "chan_cfr_ptb", "chan_cpg_ptb0"):
         # a plane channel: x and z periodic, no-slip walls at y = -1, 1, no
         # body in the domain (the cylinder of the case builder lies outside
         # x in [1, 4]); SGS with zero constants, no heat; ICH 2 (constant
@@ -33,6 +33,13 @@ def spec_for(dims, mode, nend=1, nlev=None):
             s.poisson = 3
         elif mode == "chan_cpg_xsor":
             s.poisson, s.imgsor, s.wwsor, s.mgitr = 3, 3, 1.5, 25
+        elif mode == "chan_cfr_ptb":
+            # makefld's laminar profile, perturbed (addperturb) in the step
+            # that reaches PTB_TST = 0.01
+            s.iread, s.eps_ptr, s.ptb_tst = 0, 0.1, 0.01
+        elif mode == "chan_cpg_ptb0":
+            # makefld's bulk-12 profile, perturbed before the first step
+            s.iread, s.eps_ptr, s.ptb_tst = 0, 0.05, 0.0
         return C.CaseSpec(*dims, xlim=(1.0, 4.0), ylim=(-1.0, 1.0), zlen=1.0, noise=1e-2, settings=s,
                           layout="channel")
     if mode == "nosgs":

@@ -405,7 +405,7 @@ def _mg_file_rows(path):
 
 
 @pytest.mark.parametrize("name", ["cyl32_les_3", "cyl32_iread0_trace_4", "mgrd_cyl32_les_3", "cyl32_sor_3",
-                                  "chan32_cfr_4", "xft_chan32_sor_3"])
+                                  "chan32_cfr_4", "xft_chan32_sor_3", "chan32_ptb_4"])
 def test_gpu_fortran_host_matches_reference(name):
     """The Fortran solver_exec (ISO_C_BINDING host over the C ABI) run on the
     reference's case directory writes the reference's fld file, history rows,

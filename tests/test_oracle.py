@@ -80,15 +80,33 @@ def test_oracle_bicgstab_logs_match_reference(name, oracle_lib):
     assert [tuple(r[3:]) for r in it] == [tuple(int(v) for v in r[2:]) for r in last]
 
 
-def test_makefld_matches_reference_initial_field():
+@pytest.mark.parametrize("name", ["cyl32_iread0_trace_4", "chan32_ptb_4", "chan32_ptb0_3"])
+def test_makefld_matches_reference_initial_field(name):
     """IREAD = 0: the case builder's makefld gives the field the reference
-    wrote as fld0000000 (misc_init.f90:443-607, solver.f90:50-53) bit for bit."""
-    z, meta = golden("cyl32_iread0_trace_4")
+    wrote as fld0000000 (misc_init.f90:443-607, solver.f90:50-53) bit for bit:
+    the uniform inflow, and the channel profiles of ICH = 1 and 2 (written
+    before addperturb changes the field)."""
+    z, meta = golden(name)
     view = C.case_view(C.build_case(spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])))
     f = view["fields"]
     n = (view["grid"]["n1"], view["grid"]["n2"], view["grid"]["n3"])
     for k, sl in fld_regions(n, z["t"].size > 0).items():
         assert np.array_equal(z["init_" + k][sl], f[k][sl]), k
+
+
+def test_oracle_addperturb_is_what_moves_the_channel(oracle_lib):
+    """The PTB_TST golden pins addperturb: without the perturbation (EPS_PTR
+    = 0) the oracle's w after 4 steps stays zero while the reference's is
+    not, and with it the oracle matches the reference (the golden test)."""
+    import dataclasses
+    import oracle
+    z, meta = golden("chan32_ptb_4")
+    spec = spec_for(tuple(meta["dims"]), meta["mode"], meta["nend"])
+    assert np.max(np.abs(z["w"])) > 1e-3
+    spec0 = dataclasses.replace(spec, settings=dataclasses.replace(spec.settings, eps_ptr=0.0))
+    o = oracle.Oracle(C.case_view(C.build_case(spec0)))
+    o.step(meta["nend"])
+    assert np.max(np.abs(o.fields()["w"])) < 1e-12
 
 
 def test_case_inputs_unchanged_since_fixtures():
