@@ -44,15 +44,15 @@ __device__ inline double bsum(double v, double *sh) {
 }  // namespace
 
 // the fluid volume and the u flux over the interior u points that funcbody
-// counts as fluid (at x(i), ymp(j), zmp(k)), and the wall-shear terms of
-// meanpg at the y walls: partials [c * kChB + block], c = 0 flowvol, 1 qvol,
-// 2 bottom wall, 3 top wall
-__global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, double re, int ybw, int ytw) {
+// counts as fluid (at x(i), ymp(j), zmp(k)), the wall-shear terms of meanpg
+// at the y walls and the volume-weighted IB forcing of u: partials
+// [c * kChB + block], c = 0 flowvol, 1 qvol, 2 bottom wall, 3 top wall, 4 IB
+__global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, IBComp ibu, double re, int ybw, int ytw) {
   __shared__ double sh[kChT / 64];
   const int n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2;
   const long ncell = (long)n1m * n2m * n3m;
   const long str = (long)kChB * kChT, q0 = (long)blockIdx.x * kChT + threadIdx.x;
-  double fv = 0.0, qv = 0.0, wb = 0.0, wt = 0.0;
+  double fv = 0.0, qv = 0.0, wb = 0.0, wt = 0.0, fb = 0.0;
   for (long q = q0; q < ncell; q += str) {
     const int i = 1 + (int)(q % n1m), j = 1 + (int)((q / n1m) % n2m), k = 1 + (int)(q / ((long)n1m * n2m));
     if (f.umask[q]) {
@@ -81,12 +81,15 @@ __global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, double re, int
       }
     }
   }
-  const double s0 = bsum(fv, sh), s1 = bsum(qv, sh), s2 = bsum(wb, sh), s3 = bsum(wt, sh);
+  for (long n = q0; n < (long)ibu.nbody; n += str)
+    fb = fb + ibu.fcv[n] * g.c2cx[ibu.ifc[n]] * g.f2fy[ibu.jfc[n]] * g.f2fz[ibu.kfc[n]];
+  const double s0 = bsum(fv, sh), s1 = bsum(qv, sh), s2 = bsum(wb, sh), s3 = bsum(wt, sh), s4 = bsum(fb, sh);
   if (threadIdx.x == 0) {
     f.chanpart[0 * kChB + blockIdx.x] = s0;
     f.chanpart[1 * kChB + blockIdx.x] = s1;
     f.chanpart[2 * kChB + blockIdx.x] = s2;
     f.chanpart[3 * kChB + blockIdx.x] = s3;
+    f.chanpart[4 * kChB + blockIdx.x] = s4;
   }
 }
 
@@ -94,11 +97,10 @@ __global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, double re, int
 // pmi(0) their sum plus the IB forcing of u, all over the domain volume;
 // ICH = 1: the PI controller on the bulk velocity (when ntime > 0 and
 // dt > 0), ICH = 2: pmi(0) = -UDRV_I
-__global__ __launch_bounds__(kChT) void k_meanpg(Geo g, Fld f, IBComp ibu, double udrv, double dtconst, int ctrl,
-                                               int ybw, int ytw) {
+__global__ __launch_bounds__(kChT) void k_meanpg(Geo g, Fld f, double udrv, double dtconst, int ctrl, int ybw, int ytw) {
   __shared__ double sh[kChT / 64];
-  double t[4];
-  for (int c = 0; c < 4; ++c) t[c] = bsum(threadIdx.x < kChB ? f.chanpart[c * kChB + threadIdx.x] : 0.0, sh);
+  double t[5];
+  for (int c = 0; c < 5; ++c) t[c] = bsum(threadIdx.x < kChB ? f.chanpart[c * kChB + threadIdx.x] : 0.0, sh);
   if (threadIdx.x != 0) return;
   const double flowvol = t[0], qvol = t[1];
   const double ubulk = flowvol > 0.0 ? qvol / flowvol : 0.0;
@@ -106,10 +108,7 @@ __global__ __launch_bounds__(kChT) void k_meanpg(Geo g, Fld f, IBComp ibu, doubl
   if (ybw) pmi[1] = pmi[1] + t[2];
   if (ytw) pmi[2] = pmi[2] + t[3];
   pmi[0] = pmi[1] + pmi[2] + pmi[3] + pmi[4];
-  double tmp = 0.0;
-  for (int n = 0; n < ibu.nbody; ++n)
-    tmp = tmp + ibu.fcv[n] * g.c2cx[ibu.ifc[n]] * g.f2fy[ibu.jfc[n]] * g.f2fz[ibu.kfc[n]];
-  pmi[0] = pmi[0] + tmp;
+  pmi[0] = pmi[0] + t[4];
   const double volume = g.xl * g.yl * g.zl;
   for (int q = 0; q < 5; ++q) pmi[q] = pmi[q] / volume;
   if (g.ich == 1) {
@@ -155,15 +154,15 @@ __global__ __launch_bounds__(kChT) void k_chan_hist(Geo g, Fld f, double *hist, 
   for (int q = 0; q < 3; ++q) hist[19 + q] = avg[q];
 }
 
-static void launch_chan_sums(hipStream_t s, const Geo &g, const Fld &f, const lw_settings &st) {
+static void launch_chan_sums(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st) {
   LW_LAUNCH("k_chan_sums", (double)g.n1m * g.n2m * g.n3m * 9.0, 0, k_chan_sums, dim3(kChB), dim3(kChT), 0, s, g, f,
-            st.re, (int)(st.bc_ybtm == 0), (int)(st.bc_ytop == 0));
+            ibu, st.re, (int)(st.bc_ybtm == 0), (int)(st.bc_ytop == 0));
 }
 
 void launch_meanpg(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st, double dtconst,
                    int ctrl) {
-  launch_chan_sums(s, g, f, st);
-  LW_LAUNCH("k_meanpg", kChB * 32.0, 0, k_meanpg, dim3(1), dim3(kChT), 0, s, g, f, ibu, st.udrv_i, dtconst, ctrl,
+  launch_chan_sums(s, g, f, ibu, st);
+  LW_LAUNCH("k_meanpg", kChB * 40.0, 0, k_meanpg, dim3(1), dim3(kChT), 0, s, g, f, st.udrv_i, dtconst, ctrl,
             (int)(st.bc_ybtm == 0), (int)(st.bc_ytop == 0));
 }
 
@@ -171,9 +170,9 @@ void launch_pmiavg(hipStream_t s, const Fld &f, double alpha, int reset) {
   LW_LAUNCH("k_pmiavg", 24.0, 0, k_pmiavg, dim3(1), dim3(1), 0, s, f, alpha, reset);
 }
 
-void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const lw_settings &st, double *hist, double time,
-                      double dt) {
-  launch_chan_sums(s, g, f, st);
+void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st,
+                      double *hist, double time, double dt) {
+  launch_chan_sums(s, g, f, ibu, st);
   LW_LAUNCH("k_chan_hist", kChB * 16.0, 0, k_chan_hist, dim3(1), dim3(kChT), 0, s, g, f, hist, time, dt, st.avg_tst);
 }
 

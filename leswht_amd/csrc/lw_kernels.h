@@ -141,8 +141,8 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
 void launch_meanpg(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st, double dtconst,
                    int ctrl);
 void launch_pmiavg(hipStream_t s, const Fld &f, double alpha, int reset);
-void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const lw_settings &st, double *hist, double time,
-                      double dt);
+void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const IBComp &ibu, const lw_settings &st,
+                      double *hist, double time, double dt);
 // addperturb's tables (lw_kern_chan.hip): fluid masks of the interior u
 // and w points (funcbody > 1e-10, i fastest), the profiles ptbu(j, k) and
 // ptbw(i, j), the band 0.5 < ymp(j) < 0.99, and partial sums (2 * 64 + 2)

@@ -747,7 +747,7 @@ static void setup(lw_handle *h, const lw_grid *gr, const lw_tables *tb) {
               funcbody(h->body, h->x[i], h->ymp[j], h->zmp[k]) >= 1.e-10 ? 1 : 0;
     f.umask = h->upload(um);
     f.chan = h->dalloc<double>(32);
-    f.chanpart = h->dalloc<double>(4 * 64);
+    f.chanpart = h->dalloc<double>(5 * 64);
     HIPCHK(hipMemsetAsync(f.chan, 0, 32 * sizeof(double), h->s));
   }
   // fluid tests of the boundary planes used by draglift's wall terms and by
@@ -2121,7 +2121,7 @@ static void end_of_step(lw_handle *h) {
   launch_hist_update(h->s, h->f, h->hist, act, st.ibmon == 1, h->heat, hstep, h->time, h->dt, st.avg_tst,
                      h->g.xl * h->g.yl * h->g.zl);
   // writehistory's fcmfr.dat quantities (misc_hist.f90:112-136)
-  if (h->g.ich != 0 && hstep) launch_chan_hist(h->s, h->g, h->f, st, h->hist, h->time, h->dt);
+  if (h->g.ich != 0 && hstep) launch_chan_hist(h->s, h->g, h->f, h->ib[0], st, h->hist, h->time, h->dt);
 }
 
 // ---------------------------------------------------------------------------

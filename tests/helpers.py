@@ -18,6 +18,14 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
                        cflfac=0.5)
         return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
+    if mode == "chan_body":
+        # a constant-flow-rate channel with the cylinder (radius 0.5) across
+        # its middle: the IB forcing enters meanpg's pmi(0) and its tables
+        # wrap through the x-periodic boundary
+        s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2, re=180.0,
+                       ihtrans=0, ich=1, udrv_i=1.0, bc_ybtm=0, bc_ytop=0)
+        return C.CaseSpec(*dims, xlim=(-1.5, 1.5), ylim=(-1.0, 1.0), zlen=1.0, noise=1e-2, settings=s,
+                          layout="channel")
     if mode in ("chan_cpg", "chan_cfr", "chan_cfr_xft", "chan_cpg_xsor", "chan_cfr_ptb", "chan_cpg_ptb0"):
         # a plane channel: x and z periodic, no-slip walls at y = -1, 1, no
         # body in the domain (the cylinder of the case builder lies outside
