@@ -258,10 +258,11 @@ __global__ __launch_bounds__(256) void k_filter(Geo g, const double *__restrict_
   const int n3m = g.n3m;
   const int k0 = 1 + blockIdx.z * kFiltK, k1 = k0 + kFiltK - 1 < n3m ? k0 + kFiltK - 1 : n3m;
   if (i > g.n1m || j > g.n2m) return;
-  const bool xi = fx && i >= 2 && i <= g.n1m - 1;
+  // x periodic: the end cells wrap (sgs.f90:687-700); otherwise they stay
+  const bool xi = fx && (g.xprdic ? true : (i >= 2 && i <= g.n1m - 1));
   const int ic = xi ? i : 2;  // a weight row that exists
   const double c0 = g.cfx1[3 * ic], c1 = g.cfx1[3 * ic + 1], c2 = g.cfx1[3 * ic + 2];
-  const long dm = xi ? -1 : 0, dp = xi ? 1 : 0;
+  const long dm = !xi ? 0 : i == 1 ? g.n1m - 1 : -1, dp = !xi ? 0 : i == g.n1m ? 1 - g.n1m : 1;
   const long q0 = I3(g, i, j, 1), st = (long)g.n1m * g.n2m;
   auto X = [&](int k) {
     const double *a = in + q0 + (long)(k - 1) * st;
@@ -333,9 +334,12 @@ __device__ inline bool sgs_tile(const Geo &g, long t, int fx, SgsTile &T) {
   const int tk = (int)(rest / g.n2m);
   T.lane = threadIdx.x & 63;
   T.i = ti * kSgsOwn + T.lane;
-  T.ic = T.i < 1 ? 1 : (T.i > g.n1m ? g.n1m : T.i);
+  // the x neighbours of the row's end cells: clamped (their filter is off),
+  // or on an x-periodic grid the cells across the period
+  if (g.xprdic) T.ic = T.i < 1 ? g.n1m : (T.i > g.n1m ? 1 : T.i);
+  else T.ic = T.i < 1 ? 1 : (T.i > g.n1m ? g.n1m : T.i);
   T.own = T.lane >= 1 && T.lane <= kSgsOwn && T.i <= g.n1m;
-  T.xi = fx && T.i >= 2 && T.i <= g.n1m - 1;
+  T.xi = fx && (g.xprdic ? (T.i >= 1 && T.i <= g.n1m) : (T.i >= 2 && T.i <= g.n1m - 1));
   const int ir = T.xi ? T.i : 2;
   T.cx0 = g.cfx1[3 * ir];
   T.cx1 = g.cfx1[3 * ir + 1];

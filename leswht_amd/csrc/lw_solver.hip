@@ -1190,8 +1190,6 @@ static void check_scope(const lw_settings *s, const lw_grid *g) {
     bad("an x-periodic grid needs poiss_ftft (poisson = 2) or poiss_x_ft (poisson = 3)");
   if (!s->xprdic && (s->poisson == 2 || s->poisson == 3)) bad("poiss_ftft and poiss_x_ft need an x-periodic grid");
   if (s->xprdic && s->ihtrans == 1) bad("heat transfer on an x-periodic grid is not supported");
-  if (s->xprdic && s->iles == 1 && s->idvmon == 1)
-    bad("the dynamic SGS coefficients on an x-periodic grid are not supported");
   if (s->ich < 0 || s->ich > 2) bad("ICH must be 0, 1 or 2");
   if (s->ich != 0 && !s->xprdic) bad("ICH != 0 (channel driving) needs an x-periodic grid");
   if (s->zprdic != 1) bad("the z direction must be periodic (the step's z sweeps and poiss_z_ft)");

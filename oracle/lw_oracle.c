@@ -163,7 +163,7 @@ static void *xcalloc(size_t n, size_t sz) {
 
 ora *ora_create(const ora_settings *s, const ora_grid *g, const ora_tables *t) {
   /* x periodicity is restated for the channel: no heat, no dynamic SGS */
-  const int chan_ok = s->xprdic == 1 && s->ihtrans == 0 && (s->iles == 0 || s->idvmon == 0);
+  const int chan_ok = s->xprdic == 1 && s->ihtrans == 0;
   if ((s->xprdic && !chan_ok) || s->yprdic || (s->ich != 0 && !s->xprdic) || s->ich < 0 || s->ich > 2 ||
       s->iconjg || s->imovingon || s->idecomp || s->gr != 0.0) {
     fprintf(stderr, "oracle: configuration outside the restated scope\n");
@@ -502,8 +502,8 @@ static void poisinit(ora *o) {
 #define CF(a, i, d) (a)[3 * (i) + (d) + 1]
 
 static void sgsfilterinit(ora *o) {
-  /* x: non-periodic branch (sgs.f90:74-84): only cfx1 is used by the
-   * test filter (cfx2 never is) */
+  /* x: both branches (sgs.f90:27-36, 74-84) give cfx1 by this formula;
+   * only cfx1 is used by the test filter (cfx2 never is) */
   if (o->n1m != 1)
     for (int64_t i = 1; i <= o->n1m; ++i) {
       double h1 = o->c2cx[i], h2 = o->c2cx[i + 1];
@@ -540,11 +540,19 @@ static void test_filter(ora *o, double *a0, double *tmp, double *src) {
   if (xf) {
     memcpy(src, tmp, nint * 8);
     for (int64_t k = 1; k <= n3m; ++k)
-      for (int64_t j = 1; j <= n2m; ++j)
+      for (int64_t j = 1; j <= n2m; ++j) {
         for (int64_t i = 2; i <= n1m - 1; ++i)
           G(tmp, i, j, k) = CF(o->cfx1, i, -1) * G(src, i - 1, j, k) +
                             CF(o->cfx1, i, 0) * G(src, i, j, k) +
                             CF(o->cfx1, i, 1) * G(src, i + 1, j, k);
+        /* x periodic: the end cells wrap (sgs.f90:687-700) */
+        if (o->s.xprdic == 1) {
+          G(tmp, 1, j, k) = CF(o->cfx1, 1, -1) * G(src, n1m, j, k) + CF(o->cfx1, 1, 0) * G(src, 1, j, k) +
+                            CF(o->cfx1, 1, 1) * G(src, 2, j, k);
+          G(tmp, n1m, j, k) = CF(o->cfx1, n1m, -1) * G(src, n1m - 1, j, k) +
+                              CF(o->cfx1, n1m, 0) * G(src, n1m, j, k) + CF(o->cfx1, n1m, 1) * G(src, 1, j, k);
+        }
+      }
   }
   if (zf) {
     memcpy(src, tmp, nint * 8);

@@ -58,6 +58,8 @@ CASES = {
     "chan32_ptb0_3": dict(dims=(32, 32, 8), mode="chan_cpg_ptb0", nend=3, poisson="ftft"),
     # the cylinder across a constant-flow-rate channel: IB forcing in meanpg
     "chan32_body_3": dict(dims=(32, 32, 8), mode="chan_body", nend=3, poisson="ftft"),
+    # channel LES with the dynamic Vreman model (x- and z-periodic test filter)
+    "chan32_les_3": dict(dims=(32, 32, 8), mode="chan_les", nend=3, poisson="ftft"),
     # the 3-D MG-preconditioned BiCGSTAB Poisson solver (poiss_mgrd.f90) linked
     # in place of poiss_z_ft.f90 (oracle/_ref/solver_exec_mgrd)
     "mgrd_cyl32_les_3": dict(dims=(32, 32, 8), mode="les", nend=3, poisson="mgrd"),

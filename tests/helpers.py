@@ -18,6 +18,14 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 5, re=3900.0, idtopt=1,
                        cflfac=0.5)
         return C.CaseSpec(*dims, dmin=0.01, noise=1e-2, settings=s)
+    if mode == "chan_les":
+        # the reference's own kind of case: a constant-flow-rate channel LES
+        # with the dynamic Vreman model, test filter in x and z (both
+        # periodic)
+        s = C.Settings(nend=nend, iles=1, idvmon=1, filter=3, resid1=1e-8, nlev=nlev or 2, re=5000.0, ihtrans=0,
+                       ich=1, udrv_i=1.0, bc_ybtm=0, bc_ytop=0)
+        return C.CaseSpec(*dims, xlim=(1.0, 4.0), ylim=(-1.0, 1.0), zlen=1.0, noise=2e-1, settings=s,
+                          layout="channel")
     if mode == "chan_body":
         # a constant-flow-rate channel with the cylinder (radius 0.5) across
         # its middle: the IB forcing enters meanpg's pmi(0) and its tables
