@@ -240,6 +240,93 @@ def serve_bench(args) -> dict:
     }
 
 
+def batch_bench(args) -> dict:
+    """The batched form (config 4): independent cases drawn as the served
+    stream's small cases (cells 64..256 x 64..192 x 8..32, Re log-uniform in
+    40..400, Pr 0.71 or 7, circle, square or ellipse at a random angle, CFL
+    0.5), resident in HBM together. Timed three ways over the same steps of
+    the same cases: one case at a time (the plain form), every case on its
+    own streams from its own host thread (the concurrent form of the request
+    loop), and one batch (lw_batch: each kernel launched once for all the
+    cases that issue it)."""
+    import threading
+
+    from leswht_amd import serve
+    from leswht_amd.solver import Batch
+    cases = serve.draw_cases(args.batch, seed=args.seed, large_share=0.0)
+    t0 = time.time()
+    views = [C.case_view(C.build_case(c.spec)) for c in cases]
+    t_prep = time.time() - t0
+    print("built %d cases in %.1f s" % (len(cases), t_prep), file=sys.stderr, flush=True)
+    sols = [Solver.from_case(v) for v in views]
+    if args.mixed:
+        for sv in sols:
+            sv.set_option("mg_fp32", 1)
+    cells = [c.cells for c in cases]
+    k, w = args.steps, args.warmup
+    # plain form: one case at a time
+    t_plain = 0.0
+    for sv in sols:
+        sv.step(w)
+        t0 = time.perf_counter()
+        sv.step(k)
+        t_plain += time.perf_counter() - t0
+    print("one at a time: %.3f s" % t_plain, file=sys.stderr, flush=True)
+
+    # concurrent form: every case on its own streams from its own thread
+    def run(sv, n):
+        sv.step(n)
+
+    def concurrent(n):
+        th = [threading.Thread(target=run, args=(sv, n)) for sv in sols]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0
+
+    concurrent(w)
+    t_conc = concurrent(k)
+    print("concurrent streams: %.3f s" % t_conc, file=sys.stderr, flush=True)
+    # batched form
+    b = Batch(sols)
+    b.step(w)
+    i0 = b.info()
+    t0 = time.perf_counter()
+    st = b.step(k)
+    t_bat = time.perf_counter() - t0
+    i1 = b.info()
+    b.close()
+    print("batched: %.3f s" % t_bat, file=sys.stderr, flush=True)
+    upd = float(sum(cells)) * k
+    info = {q: i1[q] - i0[q] for q in i1}
+    for sv in sols:
+        sv.close()
+    return {
+        "metric": "aggregate cell-updates/s of a batch of independent cases (1 GPU)",
+        "value": upd / t_bat,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": w,
+        "higher_is_better": True,
+        "dtype": "f64+f32 multigrid levels" if args.mixed else "f64",
+        "data": "synthetic (seeded cases: grids, bodies, IB tables and initial fields generated)",
+        "config": {"workload": "%d independent heated-body cases drawn as config 4 (cells 64..256 x 64..192 x "
+                               "8..32, Re 40..400, Pr 0.71/7, circle/square/ellipse, CFL 0.5)" % len(cases),
+                   "cases": len(cases), "cells_total": int(sum(cells)), "seed": args.seed, "prep_s": round(t_prep, 1)},
+        "one_at_a_time": upd / t_plain,
+        "concurrent_streams": upd / t_conc,
+        "ratio_vs_one_at_a_time": t_plain / t_bat,
+        "ratio_vs_concurrent_streams": t_conc / t_bat,
+        "launch_sharing": {"batched_launches": info["launches"], "case_launches": info["case_launches"],
+                           "rendezvous": info["rendezvous"],
+                           "cases_per_launch": info["case_launches"] / max(info["launches"], 1)},
+        "mg_cycles_sum_last_step": int(sum(x["mg_cycles_sum"] for x in st)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +359,8 @@ def main():
                          "level and the convergence test stay FP64); not the default, and not the metric's form")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the multigrid tile sizes with the library's autotuner before the warmup")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="run the batched form on this many independent small cases instead of the time step")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -291,6 +380,9 @@ def main():
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.hw_queues, 1), 32))
     if args.serve:
         print(json.dumps(serve_bench(args)))
+        return
+    if args.batch:
+        print(json.dumps(batch_bench(args)))
         return
     if args.copy_probe_mib:
         import torch

@@ -355,6 +355,26 @@ int lw_server_stats(lw_server *s, lw_server_info *info);
 /* runs every queued request to the end, then stops the workers */
 int lw_server_destroy(lw_server *s);
 
+/* Batched stepping (no counterpart in the reference, which runs one case
+ * per solver_exec process; it serves the independent cases of a request
+ * stream). The handles of a batch step together: each kernel of the step is
+ * launched once for every case that issues it, its workgroups divided among
+ * the cases, so many small cases fill the GPU that one of them leaves
+ * mostly idle. Each case runs exactly the arithmetic of lw_step and gives
+ * the same bits as stepped alone. The handles must live on the current
+ * device, use the z-DFT (default) or poiss_ftft Poisson solver, have
+ * IAVG = 0 and no profile running; a handle belongs to one batch at a time
+ * and cannot be destroyed while it does. lw_batch_step runs nsteps steps of
+ * every case; stats (NULL or n entries) get each case's last step (phase
+ * times are not measured: 0). lw_batch_info: info[0] rendezvous (host
+ * synchronisation points), [1] batched launches, [2] per-case launches they
+ * carried, [3] per-case read-backs. */
+typedef struct lw_batch lw_batch;
+int lw_batch_create(lw_handle *const *handles, int64_t n, lw_batch **out);
+int lw_batch_step(lw_batch *b, int64_t nsteps, lw_step_stats *stats);
+int lw_batch_info(const lw_batch *b, int64_t info[4]);
+int lw_batch_destroy(lw_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

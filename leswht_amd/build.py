@@ -19,7 +19,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libleswht.so")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["lw_solver.hip", "lw_kern_flow.hip", "lw_kern_sgs.hip", "lw_kern_pois.hip", "lw_kern_stats.hip",
+HIP_SOURCES = ["lw_solver.hip", "lw_batch.hip", "lw_kern_flow.hip", "lw_kern_sgs.hip", "lw_kern_pois.hip", "lw_kern_stats.hip",
                "lw_static.hip", "lw_kern_mgrd.hip", "lw_kern_chan.hip",
                "lw_kern_xft.hip"]
 CXX_SOURCES = ["lw_io.cpp", "lw_serve.cpp", "lw_ibm.cpp"]

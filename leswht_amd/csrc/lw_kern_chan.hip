@@ -47,7 +47,7 @@ __device__ inline double bsum(double v, double *sh) {
 // counts as fluid (at x(i), ymp(j), zmp(k)), the wall-shear terms of meanpg
 // at the y walls and the volume-weighted IB forcing of u: partials
 // [c * kChB + block], c = 0 flowvol, 1 qvol, 2 bottom wall, 3 top wall, 4 IB
-__global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, IBComp ibu, double re, int ybw, int ytw) {
+LW_KERNEL void k_chan_sums(LW_KP(kChT, 0), const Geo &g, const Fld &f, const IBComp &ibu, double re, int ybw, int ytw) {
   __shared__ double sh[kChT / 64];
   const int n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2;
   const long ncell = (long)n1m * n2m * n3m;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kChT) void k_chan_sums(Geo g, Fld f, IBComp ibu, do
 // pmi(0) their sum plus the IB forcing of u, all over the domain volume;
 // ICH = 1: the PI controller on the bulk velocity (when ntime > 0 and
 // dt > 0), ICH = 2: pmi(0) = -UDRV_I
-__global__ __launch_bounds__(kChT) void k_meanpg(Geo g, Fld f, double udrv, double dtconst, int ctrl, int ybw, int ytw) {
+LW_KERNEL void k_meanpg(LW_KP(kChT, 0), const Geo &g, const Fld &f, double udrv, double dtconst, int ctrl, int ybw, int ytw) {
   __shared__ double sh[kChT / 64];
   double t[5];
   for (int c = 0; c < 5; ++c) t[c] = bsum(threadIdx.x < kChB ? f.chanpart[c * kChB + threadIdx.x] : 0.0, sh);
@@ -125,14 +125,14 @@ __global__ __launch_bounds__(kChT) void k_meanpg(Geo g, Fld f, double udrv, doub
 
 // substepinit's running sum pmiavg = pmiavg + pmi(0)*2*alpha
 // (slv_auxi.f90:118), restarted by stepinit (:13)
-__global__ void k_pmiavg(Fld f, double alpha, int reset) {
+LW_KERNEL void k_pmiavg(LW_KP(1024, 0), const Fld &f, double alpha, int reset) {
   const double a = reset ? 0.0 : f.chan[5];
   f.chan[5] = a + f.chan[0] * 2. * alpha;
 }
 
 // writehistory's fcmfr.dat row (misc_hist.f90:112-136): qflux, ubulk and,
 // from AVG_TST on, the running averages; into hist[16..21]
-__global__ __launch_bounds__(kChT) void k_chan_hist(Geo g, Fld f, double *hist, double time, double dt,
+LW_KERNEL void k_chan_hist(LW_KP(kChT, 0), const Geo &g, const Fld &f, double *hist, double time, double dt,
                                                   double avg_tst) {
   __shared__ double sh[kChT / 64];
   const double flowvol = bsum(threadIdx.x < kChB ? f.chanpart[threadIdx.x] : 0.0, sh);
@@ -183,7 +183,7 @@ void launch_chan_hist(hipStream_t s, const Geo &g, const Fld &f, const IBComp &i
 // computed by the reference's expressions. Fixed-order block partials.
 // ---------------------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(kChT) void k_ptb_add(Geo g, Fld f, Ptb t) {
+LW_KERNEL void k_ptb_add(LW_KP(kChT, 0), const Geo &g, const Fld &f, const Ptb &t) {
   __shared__ double sh[kChT / 64];
   const int n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
   const long ncell = (long)n1m * n2m * n3m;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kChT) void k_ptb_add(Geo g, Fld f, Ptb t) {
 }
 
 // adj = rate / area when the section holds fluid: part[2 kChB], and the flag
-__global__ __launch_bounds__(kChT) void k_ptb_adj(Ptb t) {
+LW_KERNEL void k_ptb_adj(LW_KP(kChT, 0), const Ptb &t) {
   __shared__ double sh[kChT / 64];
   const double area = bsum(threadIdx.x < kChB ? t.part[threadIdx.x] : 0.0, sh);
   const double rate = bsum(threadIdx.x < kChB ? t.part[kChB + threadIdx.x] : 0.0, sh);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kChT) void k_ptb_adj(Ptb t) {
 }
 
 template <int W>
-__global__ void k_ptb_sub(Geo g, Fld f, Ptb t) {
+LW_KERNEL void k_ptb_sub(LW_KP(1024, 0), const Geo &g, const Fld &f, const Ptb &t) {
   const long q = (long)blockIdx.x * 256 + threadIdx.x;
   const int n1m = g.n1m, n2m = g.n2m;
   if (q >= (long)n1m * n2m * g.n3m) return;
@@ -261,7 +261,7 @@ __device__ inline long fidx(const Geo &g, int kz, int j, int kx) {
 }
 
 // z: X(kz) = sum_k d(k) w^{(kz-1)(k-1)}, kz = 1..n3mh, per (i, j)
-__global__ void k_ftft_zf(Geo g, Fld f, Ftft t) {
+LW_KERNEL void k_ftft_zf(LW_KP(1024, 0), const Geo &g, const Fld &f, const Ftft &t) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int kz = 1 + blockIdx.z;
@@ -278,7 +278,7 @@ __global__ void k_ftft_zf(Geo g, Fld f, Ftft t) {
 
 // x: Y(kx) = sum_i X(i) w^{(kx-1)(i-1)} (sg = +1), or the inverse with the
 // conjugate twiddles over n1m (sg = -1)
-__global__ void k_ftft_x(Geo g, Ftft t, const double2 *in, double2 *out, int sg) {
+LW_KERNEL void k_ftft_x(LW_KP(1024, 0), const Geo &g, const Ftft &t, const double2 *in, double2 *out, int sg) {
   const int kx = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int kz = 1 + blockIdx.z;
@@ -303,7 +303,7 @@ __global__ void k_ftft_x(Geo g, Ftft t, const double2 *in, double2 *out, int sg)
 // poiss_ftft.f90:75-121, 207-244): a = ajm, b = ajc, c = ajp as the
 // reference forms them; for (kz, kx) = (1, 1) row n2m is replaced by
 // phi = 0 before the solve and the solution shifted after it
-__global__ void k_ftft_solve(Geo g, Ftft t) {
+LW_KERNEL void k_ftft_solve(LW_KP(1024, 0), const Geo &g, const Ftft &t) {
   const int kx = 1 + blockIdx.x * 64 + threadIdx.x;
   const int kz = 1 + blockIdx.y;
   if (kx > g.n1m) return;
@@ -366,7 +366,7 @@ __global__ void k_ftft_solve(Geo g, Ftft t) {
 }
 
 // inverse z of the Hermitian-extended spectrum, real part into phi (i, j, k)
-__global__ void k_ftft_zi(Geo g, Fld f, Ftft t) {
+LW_KERNEL void k_ftft_zi(LW_KP(1024, 0), const Geo &g, const Fld &f, const Ftft &t) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int k = 1 + blockIdx.z;

@@ -45,7 +45,7 @@ __device__ inline void atomic_max_nonneg(unsigned long long *a, double v) {
 // momentum right-hand side, slv_mmtm.f90:315-567, with the LES cross terms of
 // rhssgs (sgs.f90:1136-1200) computed in place of the rhs1 round trip
 // ---------------------------------------------------------------------------
-__global__ void k_rhs_u(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_rhs_u(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   const int i = g.ib + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -104,7 +104,7 @@ __global__ void k_rhs_u(Geo g, Sub sb, Fld f, int les) {
   f.rk3o[0][c] = rk3x;
 }
 
-__global__ void k_rhs_v(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_rhs_v(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = g.jb + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -160,7 +160,7 @@ __global__ void k_rhs_v(Geo g, Sub sb, Fld f, int les) {
   f.rk3o[1][c] = rk3y;
 }
 
-__global__ void k_rhs_w(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_rhs_w(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = g.kb + blockIdx.z;
@@ -237,7 +237,7 @@ __device__ inline void reindex_dev(int n, int nm, int &ip, int &ipp) {
   else if (ipp == -1) { ip = nm; ipp = nm - 1; }
 }
 
-__global__ void k_ibm_gather(Geo g, IBComp ib, const double *fld, const double *rhs) {
+LW_KERNEL void k_ibm_gather(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, const double *rhs) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ib.nintp) return;
   const int ii = ib.ifc[n], jj = ib.jfc[n], kk = ib.kfc[n];
@@ -262,7 +262,7 @@ __global__ void k_ibm_gather(Geo g, IBComp ib, const double *fld, const double *
   ib.targ[n] = targ;
 }
 
-__global__ void k_ibm_scatter(Geo g, IBComp ib, const double *fld, double *rhs, double dti, int has_dudtr) {
+LW_KERNEL void k_ibm_scatter(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, double *rhs, double dti, int has_dudtr) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ib.nbody) return;
   const long q = P(g, ib.ifc[n], ib.jfc[n], ib.kfc[n]);
@@ -313,7 +313,7 @@ __device__ inline double obc_total(const double *part, double *sh) {
 
 // boundary fluxes: qin = inlet + (bottom - top) + (front - back), and the
 // outlet-plane mean of u
-__global__ __launch_bounds__(kObcT) void k_obc_flux(Geo g, Fld f) {
+LW_KERNEL void k_obc_flux(LW_KP(kObcT, 0), const Geo &g, const Fld &f) {
   __shared__ double sh[kObcT / 64 + 1];
   const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m, n2 = g.n2, n3 = g.n3;
   const int str = kObc * kObcT, q0 = blockIdx.x * kObcT + threadIdx.x;
@@ -357,7 +357,7 @@ __device__ inline ObcIn obc_in(const Geo &g, const Fld &f, double *sh) {
 }
 
 // outlet velocities before the mass correction, and the outflow partials
-__global__ __launch_bounds__(kObcT) void k_obc_out_mom(Geo g, Sub sb, Fld f) {
+LW_KERNEL void k_obc_out_mom(LW_KP(kObcT, 0), const Geo &g, const Sub &sb, const Fld &f) {
   __shared__ double sh[kObcT / 64 + 1];
   const ObcIn in = obc_in(g, f, sh);
   const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kObcT) void k_obc_out_mom(Geo g, Sub sb, Fld f) {
 }
 
 // mass correction qin/qout, the outlet deltas, and rhsincorpbc (ICH = 0)
-__global__ __launch_bounds__(kObcT) void k_obc_apply_mom(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_obc_apply_mom(LW_KP(kObcT, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   __shared__ double sh[kObcT / 64 + 1];
   const ObcIn in = obc_in(g, f, sh);
   const double qout = obc_total(f.obc + 4 * kObc, sh);
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kObcT) void k_obc_apply_mom(Geo g, Sub sb, Fld f, i
 }
 
 // z periodic ghosts of the outlet deltas
-__global__ void k_obc_ghost(Geo g, double *d0, double *d1, double *d2) {
+LW_KERNEL void k_obc_ghost(LW_KP(1024, 0), const Geo &g, double *d0, double *d1, double *d2) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   if (j > g.n2) return;
   double *d[3] = {d0, d1, d2};
@@ -442,7 +442,7 @@ static void launch_obc_flux(hipStream_t s, const Geo &g, const Fld &f) {
 
 // constant flow rate (ICH = 1): no outlet, its velocities are zero
 // (slv_mmtm.f90:980-984)
-__global__ void k_obc_zero(Geo g, Fld f) {
+LW_KERNEL void k_obc_zero(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const long q = blockIdx.x * 256L + threadIdx.x;
   if (q >= (long)g.NY * g.NZ) return;
   f.uout[q] = 0.0;
@@ -600,7 +600,7 @@ __device__ inline double line_forward(const LineCf &l, double *r, double *gm, lo
 
 // periodic z line (trdiag3p, slv_mmtm.f90:1572-1641); result into rhs
 template <int C>
-__device__ inline void adi_z(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+__device__ inline void adi_z(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1;
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int j = jlo + blockIdx.y * 4 + threadIdx.y;
@@ -709,7 +709,7 @@ __device__ inline void adi_z(const Geo &g, const Sub &sb, const Fld &f, double c
 // qe in place of the right-hand side. x is the contiguous direction, so the
 // threads' accesses are a row apart: this path is not tuned.
 template <int C>
-__device__ inline void adi_xp(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+__device__ inline void adi_xp(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   const int jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int nj = g.n2m - jlo + 1;
   const int line = blockIdx.x * 64 + threadIdx.x;
@@ -767,17 +767,17 @@ __device__ inline void adi_xp(const Geo &g, const Sub &sb, const Fld &f, double 
 }
 
 template <int C0>
-__global__ __launch_bounds__(64) void k_adi_xp(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_xp(LW_KP(64, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   switch (blockIdx.y) {
-    case 0: adi_xp<0>(g, sb, f, cre, cre2); break;
-    case 1: adi_xp<1>(g, sb, f, cre, cre2); break;
-    default: adi_xp<2>(g, sb, f, cre, cre2); break;
+    case 0: adi_xp<0>(LW_KBA, g, sb, f, cre, cre2); break;
+    case 1: adi_xp<1>(LW_KBA, g, sb, f, cre, cre2); break;
+    default: adi_xp<2>(LW_KBA, g, sb, f, cre, cre2); break;
   }
 }
 
 // non-periodic y line (trdiag2, slv_mmtm.f90:1415-1455); result into rhs
 template <int C>
-__device__ inline void adi_y(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
+__device__ inline void adi_y(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int i = ilo + blockIdx.x * 64 + threadIdx.x;
   const int k = klo + blockIdx.y * 4 + threadIdx.y;
@@ -832,7 +832,7 @@ constexpr int kXch = 31;        // points per chunk
 constexpr int kXld = kXch + 2;  // LDS row stride (odd, conflict-free columns)
 
 template <int C>
-__device__ inline void adi_x(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *V, double *R,
+__device__ inline void adi_x(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *V, double *R,
                              double *G, double *AM, double *CM) {
   const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int lane = threadIdx.x;
@@ -974,7 +974,7 @@ constexpr int kYL = 16;
 __host__ __device__ inline int adi_y_ld(int n2) { return n2 + 1; }
 
 template <int R, bool FULL, int C>
-__device__ inline void adi_y_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+__device__ inline void adi_y_wave(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
                                   double *tr) {
   const LineCf l = line_cf<1, C>(g, f, sb, cre, cre2);
   const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
@@ -1019,36 +1019,36 @@ __device__ inline void adi_y_wave(const Geo &g, const Sub &sb, const Fld &f, dou
 }
 
 template <int R, int C>
-__device__ inline void adi_y_wave_c(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+__device__ inline void adi_y_wave_c(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
                                     double *tr) {
   const int jlo = C == 1 ? g.jb : 1;
-  if (g.n2m - jlo + 1 == 32 * R) adi_y_wave<R, true, C>(g, sb, f, cre, cre2, tv, tr);
-  else adi_y_wave<R, false, C>(g, sb, f, cre, cre2, tv, tr);
+  if (g.n2m - jlo + 1 == 32 * R) adi_y_wave<R, true, C>(LW_KBA, g, sb, f, cre, cre2, tv, tr);
+  else adi_y_wave<R, false, C>(LW_KBA, g, sb, f, cre, cre2, tv, tr);
 }
 
 template <int R, int C0>
-__global__ __launch_bounds__(32 * kYL) void k_adi_yw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_yw(LW_KP(32 * kYL, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   extern __shared__ double yl[];
   double *tv = yl, *tr = yl + kYL * adi_y_ld(g.n2);
   if constexpr (C0 == 3) {
-    adi_y_wave_c<R, 3>(g, sb, f, cre, cre2, tv, tr);
+    adi_y_wave_c<R, 3>(LW_KBA, g, sb, f, cre, cre2, tv, tr);
   } else {
     switch (blockIdx.z) {
-      case 0: adi_y_wave_c<R, 0>(g, sb, f, cre, cre2, tv, tr); break;
-      case 1: adi_y_wave_c<R, 1>(g, sb, f, cre, cre2, tv, tr); break;
-      default: adi_y_wave_c<R, 2>(g, sb, f, cre, cre2, tv, tr); break;
+      case 0: adi_y_wave_c<R, 0>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
+      case 1: adi_y_wave_c<R, 1>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
+      default: adi_y_wave_c<R, 2>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
     }
   }
 }
 
-typedef void (*AdiWaveKernel)(Geo, Sub, Fld, double, double);
+typedef ::lw::KFn<Geo, Sub, Fld, double, double> AdiWaveKernel;
 template <int C0>
 static AdiWaveKernel pick_adi_yw(int n2m) {  // rows per lane on lines of 32 lanes
-  if (n2m <= 32) return k_adi_yw<1, C0>;
-  if (n2m <= 64) return k_adi_yw<2, C0>;
-  if (n2m <= 128) return k_adi_yw<4, C0>;
-  if (n2m <= 256) return k_adi_yw<8, C0>;
-  return nullptr;
+  if (n2m <= 32) return LW_KFN((k_adi_yw<1, C0>));
+  if (n2m <= 64) return LW_KFN((k_adi_yw<2, C0>));
+  if (n2m <= 128) return LW_KFN((k_adi_yw<4, C0>));
+  if (n2m <= 256) return LW_KFN((k_adi_yw<8, C0>));
+  return {};
 }
 static inline unsigned adi_yw_lds(const Geo &g) { return (unsigned)(2 * kYL * adi_y_ld(g.n2) * sizeof(double)); }
 // LDS limit set on the instance pick_adi_yw chooses for n2m: the size its
@@ -1067,7 +1067,7 @@ static inline unsigned adi_yw_lds_bucket(int n2m) {
 constexpr int kZL = 32;
 
 template <int R, int C>
-__device__ inline void adi_z_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
+__device__ inline void adi_z_wave(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *tv,
                                   double *tr) {
   constexpr int N = 32 * R, LD = N + 1;
   const LineCf l = line_cf<2, C>(g, f, sb, cre, cre2);
@@ -1110,26 +1110,26 @@ __device__ inline void adi_z_wave(const Geo &g, const Sub &sb, const Fld &f, dou
 }
 
 template <int R, int C0>
-__global__ __launch_bounds__(32 * kZL) void k_adi_zw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_zw(LW_KP(32 * kZL, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   __shared__ double zl[2 * kZL * (32 * R + 1)];
   double *tv = zl, *tr = zl + kZL * (32 * R + 1);
   if constexpr (C0 == 3) {
-    adi_z_wave<R, 3>(g, sb, f, cre, cre2, tv, tr);
+    adi_z_wave<R, 3>(LW_KBA, g, sb, f, cre, cre2, tv, tr);
   } else {
     switch (blockIdx.z) {
-      case 0: adi_z_wave<R, 0>(g, sb, f, cre, cre2, tv, tr); break;
-      case 1: adi_z_wave<R, 1>(g, sb, f, cre, cre2, tv, tr); break;
-      default: adi_z_wave<R, 2>(g, sb, f, cre, cre2, tv, tr); break;
+      case 0: adi_z_wave<R, 0>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
+      case 1: adi_z_wave<R, 1>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
+      default: adi_z_wave<R, 2>(LW_KBA, g, sb, f, cre, cre2, tv, tr); break;
     }
   }
 }
 
 template <int C0>
 static AdiWaveKernel pick_adi_zw(int n3m) {  // N = 32*R planes exactly
-  if (n3m == 32) return k_adi_zw<1, C0>;
-  if (n3m == 64) return k_adi_zw<2, C0>;
-  if (n3m == 128) return k_adi_zw<4, C0>;
-  return nullptr;
+  if (n3m == 32) return LW_KFN((k_adi_zw<1, C0>));
+  if (n3m == 64) return LW_KFN((k_adi_zw<2, C0>));
+  if (n3m == 128) return LW_KFN((k_adi_zw<4, C0>));
+  return {};
 }
 
 // x sweep with one wavefront per line (tri_line, lines of 64 lanes, R rows
@@ -1149,7 +1149,7 @@ __device__ inline int xw_p(int s) {
 }
 
 template <int R, bool FULL, int C>
-__device__ inline void adi_x_wave(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
+__device__ inline void adi_x_wave(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
   const int ilo = C == 0 ? g.ib : 1, jlo = C == 1 ? g.jb : 1, klo = C == 2 ? g.kb : 1;
   const int lane = threadIdx.x, w = threadIdx.y;
   const int nj = g.n2m - jlo + 1, nlines = nj * (g.n3m - klo + 1);
@@ -1191,35 +1191,35 @@ __device__ inline void adi_x_wave(const Geo &g, const Sub &sb, const Fld &f, dou
 }
 
 template <int R, int C>
-__device__ inline void adi_x_wave_c(const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
+__device__ inline void adi_x_wave_c(LW_KB, const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2, double *lds) {
   const int ilo = C == 0 ? g.ib : 1;
-  if (g.n1m - ilo + 1 == 64 * R) adi_x_wave<R, true, C>(g, sb, f, cre, cre2, lds);
-  else adi_x_wave<R, false, C>(g, sb, f, cre, cre2, lds);
+  if (g.n1m - ilo + 1 == 64 * R) adi_x_wave<R, true, C>(LW_KBA, g, sb, f, cre, cre2, lds);
+  else adi_x_wave<R, false, C>(LW_KBA, g, sb, f, cre, cre2, lds);
 }
 
 template <int R, int C0>
-__global__ __launch_bounds__(64 * kXWL) void k_adi_xw(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_xw(LW_KP(64 * kXWL, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   __shared__ double xl[kXWL * 2 * xw_ld<R>()];
   if constexpr (C0 == 3) {
-    adi_x_wave_c<R, 3>(g, sb, f, cre, cre2, xl);
+    adi_x_wave_c<R, 3>(LW_KBA, g, sb, f, cre, cre2, xl);
   } else {
     switch (blockIdx.y) {
-      case 0: adi_x_wave_c<R, 0>(g, sb, f, cre, cre2, xl); break;
-      case 1: adi_x_wave_c<R, 1>(g, sb, f, cre, cre2, xl); break;
-      default: adi_x_wave_c<R, 2>(g, sb, f, cre, cre2, xl); break;
+      case 0: adi_x_wave_c<R, 0>(LW_KBA, g, sb, f, cre, cre2, xl); break;
+      case 1: adi_x_wave_c<R, 1>(LW_KBA, g, sb, f, cre, cre2, xl); break;
+      default: adi_x_wave_c<R, 2>(LW_KBA, g, sb, f, cre, cre2, xl); break;
     }
   }
 }
 
 template <int C0>
 static AdiWaveKernel pick_adi_xw(int n1m) {  // rows per lane on lines of 64 lanes
-  if (n1m <= 64) return k_adi_xw<1, C0>;
-  if (n1m <= 128) return k_adi_xw<2, C0>;
-  if (n1m <= 192) return k_adi_xw<3, C0>;
-  if (n1m <= 256) return k_adi_xw<4, C0>;
-  if (n1m <= 384) return k_adi_xw<6, C0>;
-  if (n1m <= 512) return k_adi_xw<8, C0>;
-  return nullptr;
+  if (n1m <= 64) return LW_KFN((k_adi_xw<1, C0>));
+  if (n1m <= 128) return LW_KFN((k_adi_xw<2, C0>));
+  if (n1m <= 192) return LW_KFN((k_adi_xw<3, C0>));
+  if (n1m <= 256) return LW_KFN((k_adi_xw<4, C0>));
+  if (n1m <= 384) return LW_KFN((k_adi_xw<6, C0>));
+  if (n1m <= 512) return LW_KFN((k_adi_xw<8, C0>));
+  return {};
 }
 
 // once per handle: which sweeps run with whole lines in registers (bit 1:
@@ -1235,10 +1235,9 @@ hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
   if ((mask & 4) && pick_adi_zw<0>(g.n3m)) *use_wave |= 4;
   if (!(mask & 1) || !pick_adi_yw<0>(g.n2m) || adi_yw_lds_bucket(g.n2m) > 160 * 1024) return hipSuccess;
   const int lds = (int)adi_yw_lds_bucket(g.n2m);
-  hipError_t e = hipFuncSetAttribute((const void *)pick_adi_yw<0>(g.n2m),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipError_t e = ::lw::set_max_lds(pick_adi_yw<0>(g.n2m), lds);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)pick_adi_yw<3>(g.n2m), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    e = ::lw::set_max_lds(pick_adi_yw<3>(g.n2m), lds);
   if (e == hipSuccess) *use_wave |= 1;
   return e;
 }
@@ -1249,41 +1248,41 @@ hipError_t prepare_adi_kernels(const Geo &g, int *use_wave) {
 // blockIdx.z (z, y sweeps) or blockIdx.y (x sweep) is the component offset
 // from C0; C0 = 3 is the temperature alone.
 template <int C0>
-__global__ __launch_bounds__(256) void k_adi_z(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_z(LW_KP(256, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   if constexpr (C0 == 3) {
-    adi_z<3>(g, sb, f, cre, cre2);
+    adi_z<3>(LW_KBA, g, sb, f, cre, cre2);
   } else {
     switch (blockIdx.z) {
-      case 0: adi_z<0>(g, sb, f, cre, cre2); break;
-      case 1: adi_z<1>(g, sb, f, cre, cre2); break;
-      default: adi_z<2>(g, sb, f, cre, cre2); break;
+      case 0: adi_z<0>(LW_KBA, g, sb, f, cre, cre2); break;
+      case 1: adi_z<1>(LW_KBA, g, sb, f, cre, cre2); break;
+      default: adi_z<2>(LW_KBA, g, sb, f, cre, cre2); break;
     }
   }
 }
 
 template <int C0>
-__global__ __launch_bounds__(256) void k_adi_y(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_y(LW_KP(256, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   if constexpr (C0 == 3) {
-    adi_y<3>(g, sb, f, cre, cre2);
+    adi_y<3>(LW_KBA, g, sb, f, cre, cre2);
   } else {
     switch (blockIdx.z) {
-      case 0: adi_y<0>(g, sb, f, cre, cre2); break;
-      case 1: adi_y<1>(g, sb, f, cre, cre2); break;
-      default: adi_y<2>(g, sb, f, cre, cre2); break;
+      case 0: adi_y<0>(LW_KBA, g, sb, f, cre, cre2); break;
+      case 1: adi_y<1>(LW_KBA, g, sb, f, cre, cre2); break;
+      default: adi_y<2>(LW_KBA, g, sb, f, cre, cre2); break;
     }
   }
 }
 
 template <int C0>
-__global__ __launch_bounds__(64) void k_adi_x(Geo g, Sub sb, Fld f, double cre, double cre2) {
+LW_KERNEL void k_adi_x(LW_KP(64, 0), const Geo &g, const Sub &sb, const Fld &f, double cre, double cre2) {
   __shared__ double V[64 * kXld], R[64 * kXld], G[64 * kXld], AM[kXch], CM[kXch];
   if constexpr (C0 == 3) {
-    adi_x<3>(g, sb, f, cre, cre2, V, R, G, AM, CM);
+    adi_x<3>(LW_KBA, g, sb, f, cre, cre2, V, R, G, AM, CM);
   } else {
     switch (blockIdx.y) {
-      case 0: adi_x<0>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
-      case 1: adi_x<1>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
-      default: adi_x<2>(g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+      case 0: adi_x<0>(LW_KBA, g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+      case 1: adi_x<1>(LW_KBA, g, sb, f, cre, cre2, V, R, G, AM, CM); break;
+      default: adi_x<2>(LW_KBA, g, sb, f, cre, cre2, V, R, G, AM, CM); break;
     }
   }
 }
@@ -1298,13 +1297,13 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
   // the grids cover the widest component; z and y sweeps: viscosity and rhs
   // in, rhs out; x sweep: also the field in and out (per component)
   if (g.adi_wave & 4)
-    LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, pick_adi_zw<C0>(g.n3m),
+    LW_LAUNCH_K(nz, pts(g) * 24.0 * nc, 0, pick_adi_zw<C0>(g.n3m),
               dim3((unsigned)((g.n1m + kZL - 1) / kZL), (unsigned)g.n2m, (unsigned)nc), dim3(64, kZL / 2), 0, s, g, sb,
               f, cre, cre2);
   else if (g.n3m != 1)
     LW_LAUNCH(nz, pts(g) * 24.0 * nc, 0, k_adi_z<C0>, grid3(g.n1m, g.n2m, nc), BLK3, 0, s, g, sb, f, cre, cre2);
   if (g.adi_wave & 1)
-    LW_LAUNCH(ny, pts(g) * 24.0 * nc, 0, pick_adi_yw<C0>(g.n2m),
+    LW_LAUNCH_K(ny, pts(g) * 24.0 * nc, 0, pick_adi_yw<C0>(g.n2m),
               dim3((unsigned)((g.n1m + kYL - 1) / kYL), (unsigned)g.n3m, (unsigned)nc), dim3(64, kYL / 2), adi_yw_lds(g), s,
               g, sb, f, cre, cre2);
   else
@@ -1314,7 +1313,7 @@ static void adi3(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, doubl
     LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_xp<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64),
               0, s, g, sb, f, cre, cre2);
   else if (g.adi_wave & 2)
-    LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, pick_adi_xw<C0>(g.n1m),
+    LW_LAUNCH_K(nx, pts(g) * 32.0 * nc, 0, pick_adi_xw<C0>(g.n1m),
               dim3((unsigned)((xlines + kXWL - 1) / kXWL), (unsigned)nc), dim3(64, kXWL), 0, s, g, sb, f, cre, cre2);
   else
     LW_LAUNCH(nx, pts(g) * 32.0 * nc, 0, k_adi_x<C0>, dim3((unsigned)((xlines + 63) / 64), (unsigned)nc), dim3(64),
@@ -1332,7 +1331,7 @@ void launch_adi_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int 
 }
 
 // retrv_uvw + prdic_adj_uvw(0) for periodic z (slv_mmtm.f90:1710-1857)
-__global__ void k_retrv_mom(Geo g, Fld f) {
+LW_KERNEL void k_retrv_mom(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int j = blockIdx.x * 64 + threadIdx.x;  // 0..n2
   const int k = blockIdx.y;                     // 0..n3
   if (j > g.n2) return;
@@ -1344,7 +1343,7 @@ __global__ void k_retrv_mom(Geo g, Fld f) {
   }
 }
 
-__global__ void k_zghost_uvw(Geo g, Fld f) {
+LW_KERNEL void k_zghost_uvw(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y;
   if (i > g.n1) return;
@@ -1364,7 +1363,7 @@ __global__ void k_zghost_uvw(Geo g, Fld f) {
 // x-periodic ghost planes (prdic_adj_uvw, slv_mmtm.f90:1755-1786; ucalc,
 // slv_auxi.f90:388-412): u over j 0..n2, v over j 1..n2 (k 0..n3), w over
 // k 1..n3 (j 0..n2)
-__global__ void k_xghost_uvw(Geo g, Fld f) {
+LW_KERNEL void k_xghost_uvw(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int j = blockIdx.x * 64 + threadIdx.x;  // 0..n2
   const int k = blockIdx.y;                     // 0..n3
   if (j > g.n2) return;
@@ -1387,7 +1386,7 @@ static void launch_xghost_uvw(hipStream_t s, const Geo &g, const Fld &f) {
 }
 
 // pcalc's x-periodic ghosts of p (slv_auxi.f90:480-489)
-__global__ void k_xghost_p(Geo g, Fld f) {
+LW_KERNEL void k_xghost_p(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int j = 1 + blockIdx.x * 64 + threadIdx.x;
   const int k = 1 + blockIdx.y;
   if (j > g.n2m) return;
@@ -1414,7 +1413,7 @@ void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f) {
 // reference's loop order (u own, u from i+1, v own, v from j+1, w own, w from
 // k+1), which reproduces its sequential accumulation without atomics.
 // ---------------------------------------------------------------------------
-__global__ void k_divgs(Geo g, Sub sb, Fld f, int masson) {
+LW_KERNEL void k_divgs(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int masson) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -1479,7 +1478,7 @@ void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int 
 // mean of phi is reduced first (block partials into f.obc[5 * kObc..], then
 // one fixed-order sum)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kObcT) void k_phiref(Geo g, Fld f) {
+LW_KERNEL void k_phiref(LW_KP(kObcT, 0), const Geo &g, const Fld &f) {
   __shared__ double sh[kObcT / 64 + 1];
   double a = 0.0;
   for (int q = blockIdx.x * kObcT + threadIdx.x; q < g.n2m * g.n3m; q += kObc * kObcT) {
@@ -1490,13 +1489,13 @@ __global__ __launch_bounds__(kObcT) void k_phiref(Geo g, Fld f) {
   if (threadIdx.x == 0) f.obc[5 * kObc + blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kObcT) void k_phiref_sum(Geo g, Fld f) {
+LW_KERNEL void k_phiref_sum(LW_KP(kObcT, 0), const Geo &g, const Fld &f) {
   __shared__ double sh[kObcT / 64 + 1];
   const double s = obc_total(f.obc + 5 * kObc, sh);
   if (threadIdx.x == 0) f.red[0] = s / (g.yl * g.zl);
 }
 
-__global__ void k_ucalc_pcalc(Geo g, Sub sb, Fld f) {
+LW_KERNEL void k_ucalc_pcalc(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -1517,7 +1516,7 @@ __global__ void k_ucalc_pcalc(Geo g, Sub sb, Fld f) {
   f.p[c] = pn - f.red[0];
 }
 
-__global__ void k_zghost_p(Geo g, Fld f) {
+LW_KERNEL void k_zghost_p(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   if (i > g.n1m) return;
@@ -1546,7 +1545,7 @@ void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f)
 // ---------------------------------------------------------------------------
 // energy right-hand side, slv_engy.f90:29-198 (ICONJG = 0, IDECOMP = 0)
 // ---------------------------------------------------------------------------
-__global__ void k_rhs_t(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_rhs_t(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -1608,7 +1607,7 @@ void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int 
 // convbc_t (slv_engy.f90:309-413) + rhsincorpbc_t (416-457), ICONJG = 0,
 // with the plane reductions done as in the momentum outlet
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kObcT) void k_obc_out_t(Geo g, Sub sb, Fld f) {
+LW_KERNEL void k_obc_out_t(LW_KP(kObcT, 0), const Geo &g, const Sub &sb, const Fld &f) {
   __shared__ double sh[kObcT / 64 + 1];
   const ObcIn in = obc_in(g, f, sh);
   const int n1 = g.n1, n1m = g.n1m, n2m = g.n2m, n3m = g.n3m;
@@ -1632,7 +1631,7 @@ __global__ __launch_bounds__(kObcT) void k_obc_out_t(Geo g, Sub sb, Fld f) {
   (void)n3m;
 }
 
-__global__ __launch_bounds__(kObcT) void k_obc_apply_t(Geo g, Sub sb, Fld f, int les) {
+LW_KERNEL void k_obc_apply_t(LW_KP(kObcT, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
   __shared__ double sh[kObcT / 64 + 1];
   const ObcIn in = obc_in(g, f, sh);
   const double qout = obc_total(f.obc + 4 * kObc, sh);
@@ -1666,14 +1665,14 @@ void launch_convbc_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, c
 }
 
 // retrv_t, prdic_adj_t (periodic z) and wallbc_t (slv_engy.f90:612-812)
-__global__ void k_retrv_t(Geo g, Fld f) {
+LW_KERNEL void k_retrv_t(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int j = 1 + blockIdx.x * 64 + threadIdx.x;
   const int k = 1 + blockIdx.y;
   if (j > g.n2m) return;
   f.t[P(g, g.n1, j, k)] = f.tout[j + g.NY * k];
 }
 
-__global__ void k_zghost_t(Geo g, Fld f) {
+LW_KERNEL void k_zghost_t(LW_KP(1024, 0), const Geo &g, const Fld &f) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y;
   if (i > g.n1) return;
@@ -1681,7 +1680,7 @@ __global__ void k_zghost_t(Geo g, Fld f) {
   f.t[P(g, i, j, g.n3)] = f.t[P(g, i, j, 1)];
 }
 
-__global__ void k_inlet_t(Geo g, Fld f, double tsol, double tflu) {
+LW_KERNEL void k_inlet_t(LW_KP(1024, 0), const Geo &g, const Fld &f, double tsol, double tflu) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y;
   if (j > g.n2) return;
@@ -1689,7 +1688,7 @@ __global__ void k_inlet_t(Geo g, Fld f, double tsol, double tflu) {
   f.t[c] = (f.mask[c] & 128) ? tsol : tflu;
 }
 
-__global__ void k_ywall_t(Geo g, Fld f, int bb, int bt, double vb, double vt) {
+LW_KERNEL void k_ywall_t(LW_KP(1024, 0), const Geo &g, const Fld &f, int bb, int bt, double vb, double vt) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y;
   if (i > g.n1) return;
@@ -1725,7 +1724,7 @@ __device__ inline void block_max_to(double v, unsigned long long *dst, double *s
 }
 
 // each thread walks k for one (i, j); one workgroup covers 64 x 4 columns
-__global__ __launch_bounds__(256) void k_cfl(Geo g, Fld f, double dt) {
+LW_KERNEL void k_cfl(LW_KP(256, 0), const Geo &g, const Fld &f, double dt) {
   __shared__ double sh[4];
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
@@ -1746,7 +1745,7 @@ void launch_cfl(hipStream_t s, const Geo &g, const Fld &f, double dt) {
   LW_LAUNCH("k_cfl", pts(g) * 24.0, 0, k_cfl, grid3(g.n1m, g.n2m, 1), BLK3, 0, s, g, f, dt);
 }
 
-__global__ __launch_bounds__(256) void k_diag(Geo g, Fld f, int masson) {
+LW_KERNEL void k_diag(LW_KP(256, 0), const Geo &g, const Fld &f, int masson) {
   __shared__ double sh[4];
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
@@ -1774,7 +1773,7 @@ void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson) {
 
 // lagforce (slv_auxi.f90:519-570): volume-weighted sums of the IB forcing of
 // u, v, w over the forcing points; block partials, then one fixed-order sum
-__global__ __launch_bounds__(256) void k_lagforce(Geo g, Sub sb, Fld f, IBComp u, IBComp v, IBComp w) {
+LW_KERNEL void k_lagforce(LW_KP(256, 0), const Geo &g, const Sub &sb, const Fld &f, const IBComp &u, const IBComp &v, const IBComp &w) {
   __shared__ double sh[4];
   const IBComp *cs[3] = {&u, &v, &w};
   for (int c = 0; c < 3; ++c) {
@@ -1795,7 +1794,7 @@ __global__ __launch_bounds__(256) void k_lagforce(Geo g, Sub sb, Fld f, IBComp u
   }
 }
 
-__global__ __launch_bounds__(256) void k_lagforce_sum(Sub sb, Fld f) {
+LW_KERNEL void k_lagforce_sum(LW_KP(256, 0), const Sub &sb, const Fld &f) {
   __shared__ double sh[4];
   for (int c = 0; c < 3; ++c) {
     double a = 0.0;
@@ -1908,7 +1907,7 @@ __device__ inline long diag_size(const Geo &g, const IBComp *ib, int s) {
 // per sum; the other sums use the first kDiagB).
 constexpr int kDiagP = 4 * kDiagB;
 
-__global__ __launch_bounds__(256) void k_step_diag(Geo g, Fld f, IBComp u, IBComp v, IBComp w, DiagPlanes dp,
+LW_KERNEL void k_step_diag(LW_KP(256, 0), const Geo &g, const Fld &f, const IBComp &u, const IBComp &v, const IBComp &w, const DiagPlanes &dp,
                                                    unsigned active, double re) {
   __shared__ double sh[4];
   const int s = blockIdx.y;
@@ -1941,7 +1940,7 @@ __global__ __launch_bounds__(256) void k_step_diag(Geo g, Fld f, IBComp u, IBCom
   if (threadIdx.x == 0) f.dpart[s * kDiagP + blockIdx.x] = r;
 }
 
-__global__ __launch_bounds__(256) void k_step_diag_sum(Fld f, unsigned active) {
+LW_KERNEL void k_step_diag_sum(LW_KP(256, 0), const Fld &f, unsigned active) {
   __shared__ double sh[4];
   for (int s = 0; s < D_NSUM; ++s) {
     if (!((active >> s) & 1u)) continue;
@@ -1973,7 +1972,7 @@ void launch_step_diag(hipStream_t s, const Geo &g, const Fld &f, IBComp *ib, con
 // flux, [9..11] its average, [12..13] bulk T fluid/solid, [14..15] their
 // averages, [16..19] cd integral and duration, [20..23] heat-flux integral and
 // duration, [24..26] bulk-T integral and duration.
-__global__ void k_hist_update(Fld f, double *hist, unsigned active, int ibmon, int heat, int hstep, double time,
+LW_KERNEL void k_hist_update(LW_KP(1024, 0), const Fld &f, double *hist, unsigned active, int ibmon, int heat, int hstep, double time,
                               double dt, double avg_tst, double vol) {
   if (threadIdx.x != 0) return;
   const double *S = f.red + kDiagRed;

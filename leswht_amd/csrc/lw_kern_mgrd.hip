@@ -57,7 +57,7 @@ __device__ inline double acc3(const M3 &m, int i, int j, int k) {
 // or their own old values where a level has one z line. out (the last
 // toplevel of a V-cycle): the lines' final values also go to out, the
 // BiCGSTAB vector in the fields' padded layout.
-__global__ __launch_bounds__(64) void k3_half(Geo g, M3 m, int l, int half, double wgt, const double *rhs,
+LW_KERNEL void k3_half(LW_KP(64, 0), const Geo &g, const M3 &m, int l, int half, double wgt, const double *rhs,
                                               double *out) {
   const int k = m.kkmg[l][0] + blockIdx.y;
   const int is = m.iimg[l][0] + (half == 0 ? m.kpm[k] % 2 : k % 2);
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(64) void k3_half(Geo g, M3 m, int l, int half, doub
 
 // the residual of level l (toplevel 349-360, rescal 407-432): resid =
 // rhs - A gi (finest level, rhs given) or resid - A gi
-__global__ __launch_bounds__(kT) void k3_resid(Geo g, M3 m, int l, const double *rhs) {
+LW_KERNEL void k3_resid(LW_KP(kT, 0), const Geo &g, const M3 &m, int l, const double *rhs) {
   const int ni = m.iimg[l][2];
   const int q = blockIdx.x * kT + threadIdx.x;
   if (q >= ni) return;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kT) void k3_resid(Geo g, M3 m, int l, const double 
 
 // godown (435-458), and the restriction at the end of toplevel (364-375):
 // level l from level l + 1
-__global__ __launch_bounds__(kT) void k3_godown(Geo g, M3 m, int l) {
+LW_KERNEL void k3_godown(LW_KP(kT, 0), const Geo &g, const M3 &m, int l) {
   const int ni = m.iimg[l][2];
   const int q = blockIdx.x * kT + threadIdx.x;
   if (q >= ni) return;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kT) void k3_godown(Geo g, M3 m, int l) {
 
 // goup (461-486): the lines i = ibg + mod(k, 2) + 2q of level l add the
 // bilinear (x, z) interpolation of level l - 1
-__global__ __launch_bounds__(kT) void k3_goup(Geo g, M3 m, int l) {
+LW_KERNEL void k3_goup(LW_KP(kT, 0), const Geo &g, const M3 &m, int l) {
   const int k = m.kkmg[l][0] + blockIdx.z, j = 1 + blockIdx.y;
   const int i = m.iimg[l][0] + k % 2 + 2 * (int)(blockIdx.x * kT + threadIdx.x);
   if (i > m.iimg[l][1]) return;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kT) void k3_goup(Geo g, M3 m, int l) {
 
 // start of a solve (IOLDV = 0): phi = 0 (the whole array, solver.f90:117),
 // res = divgsum, and the scalars rho_ol = alp = w = 1
-__global__ __launch_bounds__(kT) void k3_cg_init(Geo g, M3 m, Fld f, double test1) {
+LW_KERNEL void k3_cg_init(LW_KP(kT, 0), const Geo &g, const M3 &m, const Fld &f, double test1) {
   const long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)kT + threadIdx.x; q < n; q += (long)gridDim.x * kT) f.phi[q] = 0.0;
   INTERIOR_LOOP(g, m.res[c] = f.div[c];)
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kT) void k3_cg_init(Geo g, M3 m, Fld f, double test
 
 // block partial sums of up to two products over the interior
 template <int WHICH>
-__global__ __launch_bounds__(kT) void k3_dot(Geo g, M3 m, Fld f) {
+LW_KERNEL void k3_dot(LW_KP(kT, 0), const Geo &g, const M3 &m, const Fld &f) {
   __shared__ double sh[kT / 64];
   double a = 0.0, b = 0.0;
   if constexpr (WHICH == 0) {  // rho = sum divgsum * res (84-91)
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kT) void k3_dot(Geo g, M3 m, Fld f) {
 // one workgroup: the fixed-order sum of the partials and the scalar update
 // that follows it: 0 rho and bet (93), 1 rv and alp (129), 2 uppp, bttt, w (166)
 template <int WHICH>
-__global__ __launch_bounds__(kT) void k3_scal(M3 m) {
+LW_KERNEL void k3_scal(LW_KP(kT, 0), const M3 &m) {
   __shared__ double sh[kT / 64];
   double a = 0.0, b = 0.0;
   for (int q = threadIdx.x; q < kRedBlocks; q += kT) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kT) void k3_scal(M3 m) {
 }
 
 // p = res + bet (p - w v) (96-103); the first iteration reads p = v = 0
-__global__ __launch_bounds__(kT) void k3_cg_p(Geo g, M3 m, int first) {
+LW_KERNEL void k3_cg_p(LW_KP(kT, 0), const Geo &g, const M3 &m, int first) {
   const double bet = m.sc[S_BET], w = m.sc[S_W];
   if (first) {
     INTERIOR_LOOP(g, m.p[c] = m.res[c] + bet * (0.0 - w * 0.0);)
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kT) void k3_cg_p(Geo g, M3 m, int first) {
 
 // v (or t) = A y (or A z) on the finest level (107-118, 143-154); y's
 // ghost cells stay 0 as the reference's do
-__global__ __launch_bounds__(kT) void k3_apply(Geo g, M3 m, const double *y, double *v) {
+LW_KERNEL void k3_apply(LW_KP(kT, 0), const Geo &g, const M3 &m, const double *y, double *v) {
   INTERIOR_LOOP(g, {
     const double acc = acc3(m, i, j, k);
     v[c] = m.aw[i] * y[P(g, m.imm[i], j, k)] + m.ae[i] * y[P(g, m.ipm[i], j, k)] + m.as[j] * y[c - g.NX] +
@@ -236,14 +236,14 @@ __global__ __launch_bounds__(kT) void k3_apply(Geo g, M3 m, const double *y, dou
 }
 
 // s = res - alp v (131-138)
-__global__ __launch_bounds__(kT) void k3_cg_s(Geo g, M3 m) {
+LW_KERNEL void k3_cg_s(LW_KP(kT, 0), const Geo &g, const M3 &m) {
   const double alp = m.sc[S_ALP];
   INTERIOR_LOOP(g, m.s[c] = m.res[c] - alp * m.v[c];)
 }
 
 // phi += alp y + w z; res = s - w t with its max norm and the last cell in
 // k, j, i order that holds it (168-193): block (max, cell) pairs
-__global__ __launch_bounds__(kT) void k3_cg_update(Geo g, M3 m, Fld f) {
+LW_KERNEL void k3_cg_update(LW_KP(kT, 0), const Geo &g, const M3 &m, const Fld &f) {
   __shared__ double shm[kT];
   __shared__ long shi[kT];
   const double alp = m.sc[S_ALP], w = m.sc[S_W];
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kT) void k3_cg_update(Geo g, M3 m, Fld f) {
 }
 
 // the iteration's max norm and its cell from the block pairs; rho_ol = rho
-__global__ __launch_bounds__(kT) void k3_cg_resm(M3 m) {
+LW_KERNEL void k3_cg_resm(LW_KP(kT, 0), const M3 &m) {
   __shared__ double shm[kT];
   __shared__ double shi[kT];
   double mx = -1.0, at = -1.0;
@@ -314,8 +314,8 @@ __global__ __launch_bounds__(kT) void k3_cg_resm(M3 m) {
 
 // phi -= phi(1, n2m, 1) over the interior (210-218): the reference value is
 // taken by one thread first, since the subtraction overwrites it
-__global__ void k3_phiref_take(Geo g, M3 m, Fld f) { m.sc[S_REF] = f.phi[P(g, 1, g.n2m, 1)]; }
-__global__ __launch_bounds__(kT) void k3_phiref(Geo g, M3 m, Fld f) {
+LW_KERNEL void k3_phiref_take(LW_KP(1024, 0), const Geo &g, const M3 &m, const Fld &f) { m.sc[S_REF] = f.phi[P(g, 1, g.n2m, 1)]; }
+LW_KERNEL void k3_phiref(LW_KP(kT, 0), const Geo &g, const M3 &m, const Fld &f) {
   const double ref = m.sc[S_REF];
   INTERIOR_LOOP(g, f.phi[c] = f.phi[c] - ref;)
 }

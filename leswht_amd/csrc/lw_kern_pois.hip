@@ -139,7 +139,7 @@ __host__ __device__ constexpr int bitrev(int k, int M) {
 // LDS so that it is written along j (contiguous in the multigrid planes),
 // kFftKb wavenumbers at a time.
 template <int N>
-__global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div) {
+LW_KERNEL void k_fft_fwd(LW_KP(256, 0), const Geo &g, const MG &mg, const double *div) {
   constexpr int M = N / 2, NH = N / 2 + 1;
   __shared__ double2 S[kFftKb * kFftT * kFftLd];
   const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_fft_fwd(Geo g, MG mg, const double *div
 // O_k = (X_k - conj X_{M-k}) conj(w^k)/2, give Z_k = E_k + i O_k, whose
 // inverse FFT of length M (scaled 1/M) holds x_{2m} + i x_{2m+1}.
 template <int N>
-__global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
+LW_KERNEL void k_fft_inv(LW_KP(256, 0), const Geo &g, const MG &mg, double *phi) {
   constexpr int M = N / 2, NH = N / 2 + 1;
   __shared__ double2 S[kFftKb * kFftT * kFftLd];
   const int t = threadIdx.x, ti = t & (kFftT - 1), tj = t >> 4;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_fft_inv(Geo g, MG mg, double *phi) {
 }
 
 // runtime-length fallback (any n3m): re-reads the column
-__global__ void k_fft_fwd_any(Geo g, MG mg, const double *div, int N) {
+LW_KERNEL void k_fft_fwd_any(LW_KP(1024, 0), const Geo &g, const MG &mg, const double *div, int N) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   if (i > g.n1m) return;
@@ -273,7 +273,7 @@ __global__ void k_fft_fwd_any(Geo g, MG mg, const double *div, int N) {
   }
 }
 
-__global__ void k_fft_inv_any(Geo g, MG mg, double *phi, int N) {
+LW_KERNEL void k_fft_inv_any(LW_KP(1024, 0), const Geo &g, const MG &mg, double *phi, int N) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   if (i > g.n1m) return;
@@ -300,34 +300,34 @@ static double fft_bytes(const Geo &g, const MG &mg) {
 void launch_fft_forward(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
   const dim3 gt((g.n1m + kFftT - 1) / kFftT, (g.n2m + kFftT - 1) / kFftT), bt(256);
-  void (*k)(Geo, MG, const double *) = nullptr;
+  ::lw::KFn<Geo, MG, const double *> k;
   switch (g.n3m) {
-    case 2: k = k_fft_fwd<2>; break;
-    case 4: k = k_fft_fwd<4>; break;
-    case 8: k = k_fft_fwd<8>; break;
-    case 16: k = k_fft_fwd<16>; break;
-    case 32: k = k_fft_fwd<32>; break;
-    case 64: k = k_fft_fwd<64>; break;
+    case 2: k = LW_KFN(k_fft_fwd<2>); break;
+    case 4: k = LW_KFN(k_fft_fwd<4>); break;
+    case 8: k = LW_KFN(k_fft_fwd<8>); break;
+    case 16: k = LW_KFN(k_fft_fwd<16>); break;
+    case 32: k = LW_KFN(k_fft_fwd<32>); break;
+    case 64: k = LW_KFN(k_fft_fwd<64>); break;
     default: break;
   }
-  if (k) LW_LAUNCH("k_fft_fwd", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.div);
+  if (k) LW_LAUNCH_K("k_fft_fwd", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.div);
   else LW_LAUNCH("k_fft_fwd_any", fft_bytes(g, mg), 0, k_fft_fwd_any, gr, bl, 0, s, g, mg, f.div, g.n3m);
 }
 
 void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f) {
   const dim3 gr((g.n1m + 63) / 64, g.n2m), bl(64);
   const dim3 gt((g.n1m + kFftT - 1) / kFftT, (g.n2m + kFftT - 1) / kFftT), bt(256);
-  void (*k)(Geo, MG, double *) = nullptr;
+  ::lw::KFn<Geo, MG, double *> k;
   switch (g.n3m) {
-    case 2: k = k_fft_inv<2>; break;
-    case 4: k = k_fft_inv<4>; break;
-    case 8: k = k_fft_inv<8>; break;
-    case 16: k = k_fft_inv<16>; break;
-    case 32: k = k_fft_inv<32>; break;
-    case 64: k = k_fft_inv<64>; break;
+    case 2: k = LW_KFN(k_fft_inv<2>); break;
+    case 4: k = LW_KFN(k_fft_inv<4>); break;
+    case 8: k = LW_KFN(k_fft_inv<8>); break;
+    case 16: k = LW_KFN(k_fft_inv<16>); break;
+    case 32: k = LW_KFN(k_fft_inv<32>); break;
+    case 64: k = LW_KFN(k_fft_inv<64>); break;
     default: break;
   }
-  if (k) LW_LAUNCH("k_fft_inv", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.phi);
+  if (k) LW_LAUNCH_K("k_fft_inv", fft_bytes(g, mg), 0, k, gt, bt, 0, s, g, mg, f.phi);
   else LW_LAUNCH("k_fft_inv_any", fft_bytes(g, mg), 0, k_fft_inv_any, gr, bl, 0, s, g, mg, f.phi, g.n3m);
 }
 
@@ -342,7 +342,7 @@ void launch_fft_inverse(hipStream_t s, const Geo &g, const MG &mg, const Fld &f)
 // residual, restricts it, applies the kv = 1 gauge shift and, in its last
 // workgroup, runs the convergence test of every wavenumber.
 // ---------------------------------------------------------------------------
-__global__ void k_mg_reset(MG mg, long n) {
+LW_KERNEL void k_mg_reset(LW_KP(1024, 0), const MG &mg, long n) {
   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (q < n) mg.pc[q] = make_double2(0.0, 0.0);
   if (q < mg.n3mh) {
@@ -557,7 +557,7 @@ __device__ inline void line_solve(const Geo &g, const MG &mg, int kv, int ibase,
 
 // one wavefront per line, four lines per workgroup, all wavenumbers
 template <int R, int MODE, bool FULL>
-__global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nlines, double oldv, int clev) {
+LW_KERNEL void k_line(LW_KP(256, 0), const Geo &g, const MG &mg, int ibase, int nlines, double oldv, int clev) {
   const int t = blockIdx.x * 4 + threadIdx.y;
   const int kv = mg.kv0 + blockIdx.y;
   if (mg.done[kv - 1]) return;
@@ -565,16 +565,16 @@ __global__ __launch_bounds__(256) void k_line(Geo g, MG mg, int ibase, int nline
   line_solve<R, MODE, FULL>(g, mg, kv, ibase, t, oldv, clev, threadIdx.x);
 }
 
-typedef void (*LineKernel)(Geo, MG, int, int, double, int);
+typedef ::lw::KFn<Geo, MG, int, int, double, int> LineKernel;
 
 template <int MODE, bool F>
 static LineKernel pick_line_rf(int R) {
   switch (R) {
-    case 1: return k_line<1, MODE, F>;
-    case 2: return k_line<2, MODE, F>;
-    case 4: return k_line<4, MODE, F>;
-    case 8: return k_line<8, MODE, F>;
-    default: return k_line<16, MODE, F>;
+    case 1: return LW_KFN((k_line<1, MODE, F>));
+    case 2: return LW_KFN((k_line<2, MODE, F>));
+    case 4: return LW_KFN((k_line<4, MODE, F>));
+    case 8: return LW_KFN((k_line<8, MODE, F>));
+    default: return LW_KFN((k_line<16, MODE, F>));
   }
 }
 
@@ -613,7 +613,7 @@ static void launch_lines(hipStream_t s, const Geo &g, const MG &mg, int mode, in
   // lines), bet 8, solution 16; the prolonging modes also read one coarse row
   // per line (16)
   const double per_row = mode >= 4 ? 72.0 : 56.0;
-  LW_LAUNCH(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.nkv),
+  LW_LAUNCH_K(names[mode], 0, per_row * g.n2m * nlines, k, dim3((unsigned)((nlines + 3) / 4), (unsigned)mg.nkv),
             dim3(64, 4), 0, s, g, mg, ibase, nlines, oldv, clev);
 }
 
@@ -654,7 +654,7 @@ __device__ inline void lev_rr(const Geo &g, const MG &mg, int kv, int ilev, int 
   }
 }
 
-__global__ __launch_bounds__(256) void k_lev_rr(Geo g, MG mg, int ilev) {
+LW_KERNEL void k_lev_rr(LW_KP(256, 0), const Geo &g, const MG &mg, int ilev) {
   const int kv = mg.kv0 + blockIdx.y;
   if (mg.done[kv - 1]) return;
   lev_rr(g, mg, kv, ilev, blockIdx.x, 1 + threadIdx.x, 256);
@@ -678,7 +678,7 @@ __device__ inline void wg_sweep(const Geo &g, const MG &mg, int kv, int ibase, i
 }
 
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * kCoarseWaves) void k_mg_coarse(Geo g, MG mg, int lc) {
+LW_KERNEL void k_mg_coarse(LW_KP(64 * kCoarseWaves, 0), const Geo &g, const MG &mg, int lc) {
   const int kv = mg.kv0 + blockIdx.x;
   if (mg.done[kv - 1]) return;
   if (threadIdx.x == 0 && threadIdx.y == 0) mg.sumres[kv - 1] = 0ull;  // for the next toplevel's max norm
@@ -978,19 +978,19 @@ __device__ __forceinline__ void mg_coarse_lds_body(const Geo &g, const MG &mg, i
 }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_coarse_lds(Geo g, MG mg) {
+LW_KERNEL void k_mg_coarse_lds(LW_KP(64 * kCoarseLdsWaves, 0), const Geo &g, const MG &mg) {
   mg_coarse_lds_body<R, FULL, S>(g, mg, blockIdx.x);
 }
 
-typedef void (*CoarseLdsKernel)(Geo, MG);
+typedef ::lw::KFn<Geo, MG> CoarseLdsKernel;
 
 template <bool F, typename S>
 static CoarseLdsKernel pick_coarse_lds_rf(int R) {
   switch (R) {
-    case 1: return k_mg_coarse_lds<1, F, S>;
-    case 2: return k_mg_coarse_lds<2, F, S>;
-    case 4: return k_mg_coarse_lds<4, F, S>;
-    default: return nullptr;
+    case 1: return LW_KFN((k_mg_coarse_lds<1, F, S>));
+    case 2: return LW_KFN((k_mg_coarse_lds<2, F, S>));
+    case 4: return LW_KFN((k_mg_coarse_lds<4, F, S>));
+    default: return {};
   }
 }
 
@@ -1011,16 +1011,16 @@ static long coarse_lds_bytes(const Geo &g, const MG &mg) {
   return bytes <= kCoarseLdsBytes ? bytes : 0;
 }
 
-typedef void (*CoarseKernel)(Geo, MG, int);
+typedef ::lw::KFn<Geo, MG, int> CoarseKernel;
 
 template <bool F>
 static CoarseKernel pick_coarse_rf(int R) {
   switch (R) {
-    case 1: return k_mg_coarse<1, F>;
-    case 2: return k_mg_coarse<2, F>;
-    case 4: return k_mg_coarse<4, F>;
-    case 8: return k_mg_coarse<8, F>;
-    default: return k_mg_coarse<16, F>;
+    case 1: return LW_KFN((k_mg_coarse<1, F>));
+    case 2: return LW_KFN((k_mg_coarse<2, F>));
+    case 4: return LW_KFN((k_mg_coarse<4, F>));
+    case 8: return LW_KFN((k_mg_coarse<8, F>));
+    default: return LW_KFN((k_mg_coarse<16, F>));
   }
 }
 
@@ -1040,11 +1040,11 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
   const long lds = mg.coarse_lds ? coarse_lds_bytes(g, mg) : 0;
   if (lds > 0) {
     const int n0 = mg.iimg[0][2] - mg.iimg[0][1] + 1;
-    LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g), mg.f32), dim3((unsigned)mg.nkv), dim3(64, n0),
+    LW_LAUNCH_K("k_mg_coarse", 0, per_kv, pick_coarse_lds(lanes_r(g.n2m), full_rows(g), mg.f32), dim3((unsigned)mg.nkv), dim3(64, n0),
               (unsigned)lds, s, g, mg);
     return;
   }
-  LW_LAUNCH("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.nkv), dim3(64, kCoarseWaves),
+  LW_LAUNCH_K("k_mg_coarse", 0, per_kv, pick_coarse(lanes_r(g.n2m), full_rows(g)), dim3((unsigned)mg.nkv), dim3(64, kCoarseWaves),
             0, s, g, mg, mg.lc);
 }
 
@@ -1054,7 +1054,7 @@ static void launch_coarse(hipStream_t s, const Geo &g, const MG &mg) {
 // (the shifted odd rows reach pc; gg's odd rows are overwritten by the next
 // toplevel's first half before anything reads them). Residual into resd,
 // pc = solution, coarse row t, per-wavenumber max norm.
-__global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
+LW_KERNEL void k_top_rr(LW_KP(256, 0), const Geo &g, const MG &mg) {
   __shared__ double sh[4];
   const int kv = mg.kv0 + blockIdx.y;
   if (!mg.done[kv - 1]) {
@@ -1124,7 +1124,7 @@ __global__ __launch_bounds__(256) void k_top_rr(Geo g, MG mg) {
 // counts a completed V-cycle; a wavenumber stops when its max norm is below
 // `test`, or when it has used MGITR cycles. Clears the norms for the next
 // toplevel.
-__global__ void k_mg_check(MG mg, double test, int id, int flip) {
+LW_KERNEL void k_mg_check(LW_KP(1024, 0), const MG &mg, double test, int id, int flip) {
   for (int q = mg.kv0 - 1 + threadIdx.x; q < mg.kv0 - 1 + mg.nkv; q += blockDim.x) {
     if (!mg.done[q]) {
       const int c = mg.cyc[q] + id;
@@ -1343,7 +1343,7 @@ __device__ __forceinline__ void mg_down_body(const Geo &g, const MG &mg, int ile
 }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) __attribute__((amdgpu_waves_per_eu(4))) void k_mg_down(Geo g, MG mg, int ilev, int T, double test,
+LW_KERNEL void k_mg_down(LW_KP(64 * (kTileMax / 2 + 2), 4), const Geo &g, const MG &mg, int ilev, int T, double test,
                                                                       int cyc) {
   mg_down_body<R, FULL, S>(g, mg, ilev, T, test, cyc, blockIdx.x);
 }
@@ -1432,7 +1432,7 @@ __device__ __forceinline__ void mg_up_body(const Geo &g, const MG &mg, int ilev,
 }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 1)) void k_mg_up(Geo g, MG mg, int ilev, int T) {
+LW_KERNEL void k_mg_up(LW_KP(64 * (kTileMax / 2 + 1), 0), const Geo &g, const MG &mg, int ilev, int T) {
   mg_up_body<R, FULL, S>(g, mg, ilev, T, blockIdx.x);
 }
 
@@ -1614,7 +1614,7 @@ __device__ __forceinline__ void mg_top_body(const Geo &g, const MG &mg, int T, d
 }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg, int T, double oldv, int id) {
+LW_KERNEL void k_mg_top(LW_KP(64 * (kTileMax / 2 + 3), 0), const Geo &g, const MG &mg, int T, double oldv, int id) {
   mg_top_body<R, FULL, S>(g, mg, T, oldv, id, blockIdx.x);
 }
 
@@ -1632,7 +1632,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 3)) void k_mg_top(Geo g, MG mg
 // rhs - wgt*(aw*u(i-1) + ae*u(i+1)) against rhs - (aw*u(i-1) + ae*u(i+1)).
 // ---------------------------------------------------------------------------
 template <int R, bool FULL>
-__global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_sor(Geo g, MG mg, int T, double wgt, double ww) {
+LW_KERNEL void k_sor(LW_KP(64 * (kTileMax / 2 + 2), 0), const Geo &g, const MG &mg, int T, double wgt, double ww) {
   extern __shared__ double2 lds[];
   __shared__ double wmax[kWaveMax];
   const int ntile = g.n1m / T;
@@ -1765,7 +1765,7 @@ __global__ __launch_bounds__(64 * (kTileMax / 2 + 2)) void k_sor(Geo g, MG mg, i
 // the test after a sweep of gsor2d (poiss_z_ft.f90:551-552, 585-587): as
 // k_mg_check, and for the sweeps of the loop (id = 1) the max norm of sweep
 // c of wavenumber q is kept at log[(c-1)*ld + q] for the residual rows
-__global__ void k_sor_check(MG mg, double test, int id, double *log, int ld) {
+LW_KERNEL void k_sor_check(LW_KP(1024, 0), const MG &mg, double test, int id, double *log, int ld) {
   for (int q = mg.kv0 - 1 + threadIdx.x; q < mg.kv0 - 1 + mg.nkv; q += blockDim.x) {
     if (!mg.done[q]) {
       const int c = mg.cyc[q] + id;
@@ -1833,7 +1833,7 @@ __device__ inline int nrow_of(const MG &mg, int l) { return mg.iimg[l][2] - mg.i
 __device__ inline int xcd_blocks(int ntile) { return kXcds * ((ntile + kXcds - 1) / kXcds); }
 
 template <int R, bool FULL, typename S>
-__global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_kv1(Geo g, MG mg, double test, int ncyc, long long tlim) {
+LW_KERNEL void k_mg_kv1(LW_KP(64 * kCoarseLdsWaves, 0), const Geo &g, const MG &mg, double test, int ncyc, long long tlim) {
   const int nb = gridDim.x;
   unsigned target = 0;
   bool alive = true, conv = false;
@@ -1888,22 +1888,22 @@ __global__ __launch_bounds__(64 * kCoarseLdsWaves) void k_mg_kv1(Geo g, MG mg, d
   }
 }
 
-__global__ void k_mg_kv1_reset(MG mg) {
+LW_KERNEL void k_mg_kv1_reset(LW_KP(1024, 0), const MG &mg) {
   *mg.kbar = 0u;
   *mg.kerr = 0;
 }
 
-typedef void (*FusedLevelKernel)(Geo, MG, int, int);
-typedef void (*FusedDownKernel)(Geo, MG, int, int, double, int);
+typedef ::lw::KFn<Geo, MG, int, int> FusedLevelKernel;
+typedef ::lw::KFn<Geo, MG, int, int, double, int> FusedDownKernel;
 
 template <bool F, typename S>
 static FusedDownKernel pick_down_rf(int R) {
   switch (R) {
-    case 1: return k_mg_down<1, F, S>;
-    case 2: return k_mg_down<2, F, S>;
-    case 4: return k_mg_down<4, F, S>;
-    case 8: return k_mg_down<8, F, S>;
-    default: return k_mg_down<16, F, S>;
+    case 1: return LW_KFN((k_mg_down<1, F, S>));
+    case 2: return LW_KFN((k_mg_down<2, F, S>));
+    case 4: return LW_KFN((k_mg_down<4, F, S>));
+    case 8: return LW_KFN((k_mg_down<8, F, S>));
+    default: return LW_KFN((k_mg_down<16, F, S>));
   }
 }
 
@@ -1914,11 +1914,11 @@ static FusedDownKernel pick_down(int R, bool full, bool f32 = false) {
 template <bool F, typename S>
 static FusedLevelKernel pick_up_rf(int R) {
   switch (R) {
-    case 1: return k_mg_up<1, F, S>;
-    case 2: return k_mg_up<2, F, S>;
-    case 4: return k_mg_up<4, F, S>;
-    case 8: return k_mg_up<8, F, S>;
-    default: return k_mg_up<16, F, S>;
+    case 1: return LW_KFN((k_mg_up<1, F, S>));
+    case 2: return LW_KFN((k_mg_up<2, F, S>));
+    case 4: return LW_KFN((k_mg_up<4, F, S>));
+    case 8: return LW_KFN((k_mg_up<8, F, S>));
+    default: return LW_KFN((k_mg_up<16, F, S>));
   }
 }
 
@@ -1927,15 +1927,15 @@ static FusedLevelKernel pick_up(int R, bool full, bool f32 = false) {
   return full ? pick_up_rf<true, double2>(R) : pick_up_rf<false, double2>(R);
 }
 
-typedef void (*FusedTopKernel)(Geo, MG, int, double, int);
+typedef ::lw::KFn<Geo, MG, int, double, int> FusedTopKernel;
 template <bool F, typename S>
 static FusedTopKernel pick_top_rf(int R) {
   switch (R) {
-    case 1: return k_mg_top<1, F, S>;
-    case 2: return k_mg_top<2, F, S>;
-    case 4: return k_mg_top<4, F, S>;
-    case 8: return k_mg_top<8, F, S>;
-    default: return k_mg_top<16, F, S>;
+    case 1: return LW_KFN((k_mg_top<1, F, S>));
+    case 2: return LW_KFN((k_mg_top<2, F, S>));
+    case 4: return LW_KFN((k_mg_top<4, F, S>));
+    case 8: return LW_KFN((k_mg_top<8, F, S>));
+    default: return LW_KFN((k_mg_top<16, F, S>));
   }
 }
 static FusedTopKernel pick_top(int R, bool full, bool f32 = false) {
@@ -1957,7 +1957,7 @@ static void launch_down(hipStream_t s, const Geo &g, const MG &mg, int ilev, dou
   // unique bytes per point of the level: reads resd 16, bet 8; writes the
   // second-half solution 16 on half the lines (8), the residual 16 and the
   // restricted row 16 per pair (8)
-  LW_LAUNCH("k_mg_down", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_down(R, full_rows(g), mg.f32),
+  LW_LAUNCH_K("k_mg_down", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_down(R, full_rows(g), mg.f32),
             xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 2), fused_lds(R, T, false), s, g, mg, ilev, T, test, cyc);
 }
 
@@ -1966,7 +1966,7 @@ static void launch_up(hipStream_t s, const Geo &g, const MG &mg, int ilev) {
   // unique bytes per point of the level: reads residual 16, bet 8, the
   // down-sweep row on half the lines (8) and the coarse row per pair (8);
   // writes the solution 16
-  LW_LAUNCH("k_mg_up", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_up(R, full_rows(g), mg.f32), xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 1),
+  LW_LAUNCH_K("k_mg_up", 0, (mg.f32 ? 32.0 : 56.0) * g.n2m * n, pick_up(R, full_rows(g), mg.f32), xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 1),
             fused_lds(R, T, true), s, g, mg, ilev, T);
 }
 
@@ -1977,7 +1977,7 @@ static void launch_top(hipStream_t s, const Geo &g, const MG &mg, int id, double
   // 16 and the restricted row per pair (8)
   // (the coarse rows take 4 and 4 in the mixed-precision form)
   const double per_pt = (id == 1 ? 64.0 : 56.0) - (mg.f32 ? (id == 1 ? 8.0 : 4.0) : 0.0);
-  LW_LAUNCH(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g), mg.f32),
+  LW_LAUNCH_K(id == 1 ? "k_mg_top_up" : "k_mg_top", 0, per_pt * g.n2m * n, pick_top(R, full_rows(g), mg.f32),
             xcd_grid(n / T, mg.nkv), dim3(64, T / 2 + 3), fused_lds(R, T, false), s, g, mg, T, oldv, id);
 }
 
@@ -2001,13 +2001,13 @@ void launch_mg_toplevel(hipStream_t s, const Geo &g, const MG &mg, int id, doubl
 // kernel (which clears the max norms for the next toplevel)
 bool mg_folds_check(const MG &mg) { return mg.fused && mg.lc >= 0 && mg.nlev - 1 > mg.lc; }
 
-typedef void (*Kv1Kernel)(Geo, MG, double, int, long long);
+typedef ::lw::KFn<Geo, MG, double, int, long long> Kv1Kernel;
 template <bool F, typename S>
 static Kv1Kernel pick_kv1_rf(int R) {
   switch (R) {
-    case 1: return k_mg_kv1<1, F, S>;
-    case 2: return k_mg_kv1<2, F, S>;
-    default: return k_mg_kv1<4, F, S>;  // the LDS coarse levels need R <= 4
+    case 1: return LW_KFN((k_mg_kv1<1, F, S>));
+    case 2: return LW_KFN((k_mg_kv1<2, F, S>));
+    default: return LW_KFN((k_mg_kv1<4, F, S>));  // the LDS coarse levels need R <= 4
   }
 }
 static Kv1Kernel pick_kv1(int R, bool full, bool f32) {
@@ -2031,15 +2031,15 @@ bool mg_kv1_supported(const Geo &g, const MG &a) {
   return nw >= need && nw <= kCoarseLdsWaves;
 }
 
-typedef void (*SorKernel)(Geo, MG, int, double, double);
+typedef ::lw::KFn<Geo, MG, int, double, double> SorKernel;
 template <bool F>
 static SorKernel pick_sor_rf(int R) {
   switch (R) {
-    case 1: return k_sor<1, F>;
-    case 2: return k_sor<2, F>;
-    case 4: return k_sor<4, F>;
-    case 8: return k_sor<8, F>;
-    default: return k_sor<16, F>;
+    case 1: return LW_KFN((k_sor<1, F>));
+    case 2: return LW_KFN((k_sor<2, F>));
+    case 4: return LW_KFN((k_sor<4, F>));
+    case 8: return LW_KFN((k_sor<8, F>));
+    default: return LW_KFN((k_sor<16, F>));
   }
 }
 static SorKernel pick_sor(int R, bool full) { return full ? pick_sor_rf<true>(R) : pick_sor_rf<false>(R); }
@@ -2049,7 +2049,7 @@ void launch_sor_sweep(hipStream_t s, const Geo &g, const MG &m, double wgt, doub
   const int n = g.n1m, T = fused_tile(n, m.tile[2]), R = lanes_r(g.n2m);
   // per top point: rhs 16, bet 8, the old u on every line 16 (and its
   // neighbours on the first half, from L2) read; the new u 16 written
-  LW_LAUNCH("k_sor", 0, 56.0 * g.n2m * n, pick_sor(R, full_rows(g)), xcd_grid(n / T, m.nkv), dim3(64, T / 2 + 2),
+  LW_LAUNCH_K("k_sor", 0, 56.0 * g.n2m * n, pick_sor(R, full_rows(g)), xcd_grid(n / T, m.nkv), dim3(64, T / 2 + 2),
             fused_lds(R, T, false), s, g, m, T, wgt, ww);
   LW_LAUNCH("k_sor_check", 0, 0, k_sor_check, dim3(1), dim3(128), 0, s, m, test, id, log, ld);
 }
@@ -2067,7 +2067,7 @@ void launch_mg_kv1(hipStream_t s, const Geo &g, const MG &a, double test, int nc
   }
   const unsigned grid = (unsigned)(kXcds * ((tiles + kXcds - 1) / kXcds));
   LW_LAUNCH("k_mg_kv1_reset", 0, 0, k_mg_kv1_reset, dim3(1), dim3(1), 0, s, a);
-  LW_LAUNCH("k_mg_kv1", 0, 0, pick_kv1(R, full_rows(g), a.f32 != 0), dim3(grid), dim3(64, kv1_waves(a)), lds, s, g, a,
+  LW_LAUNCH_K("k_mg_kv1", 0, 0, pick_kv1(R, full_rows(g), a.f32 != 0), dim3(grid), dim3(64, kv1_waves(a)), lds, s, g, a,
             test, ncyc, tlim);
 }
 
@@ -2135,7 +2135,7 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
   if (R > 8) mg.fused = 0;
   if (R <= 8) {
     // the line-SOR sweeps take the top kernel's tiles
-    hipError_t e = hipFuncSetAttribute((const void *)pick_sor(R, full_rows(g)), hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = ::lw::set_max_lds(pick_sor(R, full_rows(g)),
                                        (int)fused_lds(R, kTileMax, false));
     if (e != hipSuccess) return e;
   }
@@ -2143,28 +2143,26 @@ hipError_t prepare_mg_kernels(const Geo &g, MG &mg) {
     const int big = (int)fused_lds(R, kTileMax, false);
     hipError_t e = hipSuccess;
     for (int f32 = 0; f32 <= 1 && e == hipSuccess; ++f32) {
-      e = hipFuncSetAttribute((const void *)pick_down(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+      e = ::lw::set_max_lds(pick_down(R, full_rows(g), f32),
                               big);
       if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)pick_up(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+        e = ::lw::set_max_lds(pick_up(R, full_rows(g), f32),
                                 big);
       if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)pick_top(R, full_rows(g), f32), hipFuncAttributeMaxDynamicSharedMemorySize,
+        e = ::lw::set_max_lds(pick_top(R, full_rows(g), f32),
                                 big);
     }
     if (e != hipSuccess) return e;
   }
   if (!mg.coarse_lds || coarse_lds_bytes(g, mg) == 0) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), false),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+  hipError_t e = ::lw::set_max_lds(pick_coarse_lds(R, full_rows(g), false), kCoarseLdsBytes);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)pick_coarse_lds(R, full_rows(g), true),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kCoarseLdsBytes);
+    e = ::lw::set_max_lds(pick_coarse_lds(R, full_rows(g), true), kCoarseLdsBytes);
   if (R > 4) return e;
   // k_mg_kv1 also holds a few static LDS words (the top stage's norms, the
   // barrier's flag): the dynamic part may take the rest
   for (int f32 = 0; f32 < 2 && e == hipSuccess; ++f32) {
-    const void *fn = (const void *)pick_kv1(R, full_rows(g), f32 != 0);
+    const void *fn = pick_kv1(R, full_rows(g), f32 != 0).one_;
     hipFuncAttributes at{};
     e = hipFuncGetAttributes(&at, fn);
     if (e == hipSuccess)

@@ -133,7 +133,7 @@ __device__ void temgrad(const Geo &g, const Fld &f, int i, int j, int k, double 
 // Vreman, dynamic global coefficient (sgs_dvmg)
 // buffers: a[0..8] alpha_ij, a[9..14] M_ij, a[15..17] u_i, a[18..23] L_ij
 // ---------------------------------------------------------------------------
-__global__ void k_dvmg1(Geo g, Fld f, SgsBuf b) {
+LW_KERNEL void k_dvmg1(LW_KP(1024, 0), const Geo &g, const Fld &f, const SgsBuf &b) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -149,7 +149,7 @@ __global__ void k_dvmg1(Geo g, Fld f, SgsBuf b) {
   for (int l = 0; l < 6; ++l) b.a[9 + l][q] = r * str[l];
 }
 
-__global__ void k_dvmg2(Geo g, Fld f, SgsBuf b) {
+LW_KERNEL void k_dvmg2(LW_KP(1024, 0), const Geo &g, const Fld &f, const SgsBuf &b) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -201,7 +201,7 @@ __device__ inline double bsum(double v, double *sh) {
   return r;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_dvmg3(Geo g, Fld f, SgsBuf b, const int64_t *nwall) {
+LW_KERNEL void k_dvmg3(LW_KP(kRedThreads, 0), const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
   __shared__ double sh[kRedThreads / 64];
   const long n = (long)g.n1m * g.n2m * g.n3m;
   double lm = 0.0, mm = 0.0;
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dvmg3(Geo g, Fld f, SgsBuf b, c
 }
 
 // final sum of the block partials and the clipped coefficient into red[slot]
-__global__ __launch_bounds__(kRedThreads) void k_coef(Fld f, int nblk, double scale, int slot) {
+LW_KERNEL void k_coef(LW_KP(kRedThreads, 0), const Fld &f, int nblk, double scale, int slot) {
   __shared__ double sh[kRedThreads / 64];
   double a = 0.0, c = 0.0;
   for (int q = threadIdx.x; q < nblk; q += kRedThreads) {
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kRedThreads) void k_coef(Fld f, int nblk, double sc
 // loads of later planes can be issued ahead of the arithmetic.
 constexpr int kFiltK = 16;
 
-__global__ __launch_bounds__(256) void k_filter(Geo g, const double *__restrict__ in, double *__restrict__ out,
+LW_KERNEL void k_filter(LW_KP(256, 0), const Geo &g, const double *__restrict__ in, double *__restrict__ out,
                                                 int fx, int fz) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
@@ -459,7 +459,7 @@ __device__ inline void edm_germano(const Geo &g, int i, int j, int k, const doub
 // alpha). Workgroups of 4 wavefronts walk the tiles; each workgroup's two
 // sums go to f.partial[2 b], f.partial[2 b + 1].
 template <int MODEL, int NQ>
-__global__ __launch_bounds__(256) void k_sgs_dyn(Geo g, Fld f, const int64_t *nwall, int fx, int fz) {
+LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_t *nwall, int fx, int fz) {
   __shared__ double sh[4];
   double lm = 0.0, mm = 0.0;
   const int w = threadIdx.x >> 6;
@@ -571,7 +571,7 @@ void launch_sgs_dvmg_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBu
   LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -0.5, 4);
 }
 
-__global__ void k_cvm(Geo g, Fld f, double csgsts) {
+LW_KERNEL void k_cvm(LW_KP(1024, 0), const Geo &g, const Fld &f, double csgsts) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -591,7 +591,7 @@ void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts) {
 // buffers: a[0..5] S_ij, a[6..8] dT/dx_k, a[9..11] M_k, a[15..17] u_i,
 // a[18..20] L_k, a[0..2] reused for T after the first filter pass
 // ---------------------------------------------------------------------------
-__global__ void k_edm1(Geo g, Fld f, SgsBuf b) {
+LW_KERNEL void k_edm1(LW_KP(1024, 0), const Geo &g, const Fld &f, const SgsBuf &b) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -613,7 +613,7 @@ __global__ void k_edm1(Geo g, Fld f, SgsBuf b) {
   for (int l = 0; l < 3; ++l) b.a[6 + l][q] = alp[l];
 }
 
-__global__ void k_edm2(Geo g, Fld f, SgsBuf b) {
+LW_KERNEL void k_edm2(LW_KP(1024, 0), const Geo &g, const Fld &f, const SgsBuf &b) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -645,7 +645,7 @@ __global__ void k_edm2(Geo g, Fld f, SgsBuf b) {
   b.a[20][q] = u3 * tc;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_edm3(Geo g, Fld f, SgsBuf b, const int64_t *nwall) {
+LW_KERNEL void k_edm3(LW_KP(kRedThreads, 0), const Geo &g, const Fld &f, const SgsBuf &b, const int64_t *nwall) {
   __shared__ double sh[kRedThreads / 64];
   const long n = (long)g.n1m * g.n2m * g.n3m;
   double lm = 0.0, mm = 0.0;
@@ -687,7 +687,7 @@ void launch_sgs_edm_coef(hipStream_t s, const Geo &g, const Fld &f, const SgsBuf
   LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, -1.0, 5);
 }
 
-__global__ void k_cedm(Geo g, Fld f, double csgshf) {
+LW_KERNEL void k_cedm(LW_KP(1024, 0), const Geo &g, const Fld &f, double csgshf) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -708,7 +708,7 @@ void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf) {
 // scale by the dynamic coefficient (slot >= 0) and zero in the body and at
 // the listed points (nutzero/alpzero); then z-periodic ghosts
 // ---------------------------------------------------------------------------
-__global__ void k_scale_zero(Geo g, Fld f, double *nu, int slot) {
+LW_KERNEL void k_scale_zero(LW_KP(1024, 0), const Geo &g, const Fld &f, double *nu, int slot) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -722,13 +722,13 @@ __global__ void k_scale_zero(Geo g, Fld f, double *nu, int slot) {
 
 // listed points that fall on ghost indices (the 0-based values read as
 // 1-based can be 0) are zeroed here; interior ones carry M_NUTZERO
-__global__ void k_zero_list(Geo g, double *nu, const int *lst, int n) {
+LW_KERNEL void k_zero_list(LW_KP(1024, 0), const Geo &g, double *nu, const int *lst, int n) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   nu[P(g, lst[3 * q], lst[3 * q + 1], lst[3 * q + 2])] = 0.;
 }
 
-__global__ void k_zghost_scalar(Geo g, double *a) {
+LW_KERNEL void k_zghost_scalar(LW_KP(1024, 0), const Geo &g, double *a) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y;
   if (i > g.n1) return;
@@ -796,7 +796,7 @@ __device__ double edge_234(const Geo &g, const double *nu, const EdgeCtx &e, int
   return v;
 }
 
-__global__ void k_edges(Geo g, const double *nu, double *o1, double *o2, double *o3, EdgeCtx e) {
+LW_KERNEL void k_edges(LW_KP(1024, 0), const Geo &g, const double *nu, double *o1, double *o2, double *o3, const EdgeCtx &e) {
   const int i = blockIdx.x * 64 + threadIdx.x;  // 0..n1
   const int j = blockIdx.y * 4 + threadIdx.y;   // 0..n2
   const int k = blockIdx.z;                     // 0..n3
@@ -813,7 +813,7 @@ __global__ void k_edges(Geo g, const double *nu, double *o1, double *o2, double 
 }
 
 // x-periodic ghosts of a cell array (nutinterpol, sgs.f90:960-967)
-__global__ void k_xghost_scalar(Geo g, double *a) {
+LW_KERNEL void k_xghost_scalar(LW_KP(1024, 0), const Geo &g, double *a) {
   const int j = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y;
   if (j > g.n2) return;

@@ -16,7 +16,7 @@ __device__ inline long I3s(const Geo &g, int i, int j, int k) {
 // misc_hist.f90:339-364 (centre values, products) and 383-466 (gradients with
 // literal i+-1, j+-1, k+-1 neighbours and the fix* one-sided switches,
 // vorticity, strain), in the reference's operation order
-__global__ __launch_bounds__(256) void k_field_avg(Geo g, Fld f, AvgBuf b, double dt, int heat) {
+LW_KERNEL void k_field_avg(LW_KP(256, 0), const Geo &g, const Fld &f, const AvgBuf &b, double dt, int heat) {
   const int i = blockIdx.x * 256 + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > g.n1m) return;
   const double *u = f.u, *v = f.v, *w = f.w;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_field_avg(Geo g, Fld f, AvgBuf b, doubl
 
 // tracer (misc_hist.f90:184-209): the cell-centre velocities and the
 // pressure of each traced cell (i, j, k) of n: out[c n + q] for c = u, v, w, p
-__global__ void k_trace(Geo g, Fld f, const int *pts, int n, double *out) {
+LW_KERNEL void k_trace(LW_KP(1024, 0), const Geo &g, const Fld &f, const int *pts, int n, double *out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int i = pts[3 * q], j = pts[3 * q + 1], k = pts[3 * q + 2];

@@ -44,7 +44,7 @@ __device__ inline XIdx xidx(const Geo &g, const Xft &x) { return XIdx{g.n2m, g.n
 }  // namespace
 
 // forward DFT in x of the divergence: rhs(iv) = sum_i d(i) e^{-2 pi i (iv-1)(i-1)/n1m}
-__global__ void k_xft_fwd(Geo g, Fld f, Xft x) {
+LW_KERNEL void k_xft_fwd(LW_KP(1024, 0), const Geo &g, const Fld &f, const Xft &x) {
   const int k = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int iv = 1 + blockIdx.z;
@@ -60,7 +60,7 @@ __global__ void k_xft_fwd(Geo g, Fld f, Xft x) {
 }
 
 // inverse DFT of the Hermitian-extended spectrum, real part over n1m into phi
-__global__ void k_xft_inv(Geo g, Fld f, Xft x) {
+LW_KERNEL void k_xft_inv(LW_KP(1024, 0), const Geo &g, const Fld &f, const Xft &x) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int k = 1 + blockIdx.z;
@@ -85,13 +85,13 @@ __global__ void k_xft_inv(Geo g, Fld f, Xft x) {
 
 // ICH = 1 (poiss_x_ft.f90:215-229): phiref = sum_i phi(i, n2m, 1) f2fx(i) / xl
 // in the reference's order, subtracted from the interior
-__global__ void k_xft_phiref(Geo g, Fld f, Xft x) {
+LW_KERNEL void k_xft_phiref(LW_KP(1024, 0), const Geo &g, const Fld &f, const Xft &x) {
   double s = 0.0;
   for (int i = 1; i <= g.n1m; ++i) s = s + f.phi[P(g, i, g.n2m, 1)] * g.f2fx[i];
   x.misc[0] = s / g.xl;
 }
 
-__global__ void k_xft_phisub(Geo g, Fld f, Xft x) {
+LW_KERNEL void k_xft_phisub(LW_KP(1024, 0), const Geo &g, const Fld &f, const Xft &x) {
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int k = 1 + blockIdx.z;
@@ -109,7 +109,7 @@ __global__ void k_xft_phisub(Geo g, Fld f, Xft x) {
 // then trdiag1m (372-394) in place in gg. SOR: u = ww gg + cw u on pc.
 // With gauge, line 1 stores its value at j = n2m (the mean mode's shift).
 template <int TOP, int PCN, int SOR>
-__global__ __launch_bounds__(64) void k_xft_line(Geo g, Xft x, int iv0, int kb, int ke, int par, double w,
+LW_KERNEL void k_xft_line(LW_KP(64, 0), const Geo &g, const Xft &x, int iv0, int kb, int ke, int par, double w,
                                                  int formB, double ww, double cw, int gauge) {
   const int k = kb + par + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
   const int iv = iv0 + blockIdx.y;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64) void k_xft_line(Geo g, Xft x, int iv0, int kb, 
 // the mean mode (wavenumber 1): the odd lines less the stored shift
 // (toplevel 326-335 on gg, gsor2d 520-527 on pc)
 template <int SOR>
-__global__ void k_xft_gauge(Geo g, Xft x) {
+LW_KERNEL void k_xft_gauge(LW_KP(1024, 0), const Geo &g, const Xft &x) {
   const int k = 1 + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
   const int j = 1 + blockIdx.y;
   if (k > g.n3m) return;
@@ -166,7 +166,7 @@ __global__ void k_xft_gauge(Geo g, Xft x) {
 // the top level's residual (toplevel 344-354; resid3 618-638 when SOR on
 // pc): its max norm per wavenumber and, for the multigrid, resd and pc = gg
 template <int SOR>
-__global__ void k_xft_res(Geo g, Xft x, int iv0) {
+LW_KERNEL void k_xft_res(LW_KP(1024, 0), const Geo &g, const Xft &x, int iv0) {
   const int k = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int iv = iv0 + blockIdx.z;
@@ -190,7 +190,7 @@ __global__ void k_xft_res(Geo g, Xft x, int iv0) {
 }
 
 // toplevel's restriction to level nlev-1 (359-367)
-__global__ void k_xft_restrict_top(Geo g, Xft x, int iv0) {
+LW_KERNEL void k_xft_restrict_top(LW_KP(1024, 0), const Geo &g, const Xft &x, int iv0) {
   const int lv = x.nlev - 1;
   const int k = x.kb[lv] + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
@@ -204,7 +204,7 @@ __global__ void k_xft_restrict_top(Geo g, Xft x, int iv0) {
 }
 
 // toplevel's interpolation from level nlev-1 to the even top lines (305-313)
-__global__ void k_xft_interp_top(Geo g, Xft x, int iv0) {
+LW_KERNEL void k_xft_interp_top(LW_KP(1024, 0), const Geo &g, const Xft &x, int iv0) {
   const int lv = x.nlev - 1;
   const int k = x.kb[lv] + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
@@ -219,7 +219,7 @@ __global__ void k_xft_interp_top(Geo g, Xft x, int iv0) {
 
 // godown (415-437): the residual of level l's two cells under each cell of
 // level l-1, then their restriction
-__global__ void k_xft_godown(Geo g, Xft x, int iv0, int l) {
+LW_KERNEL void k_xft_godown(LW_KP(1024, 0), const Geo &g, const Xft &x, int iv0, int l) {
   const int k = x.kb[l - 1] + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int iv = iv0 + blockIdx.z;
@@ -243,7 +243,7 @@ __global__ void k_xft_godown(Geo g, Xft x, int iv0, int l) {
 }
 
 // goup (397-412): level l's correction added to the even cells of level l+1
-__global__ void k_xft_goup(Geo g, Xft x, int iv0, int l) {
+LW_KERNEL void k_xft_goup(LW_KP(1024, 0), const Geo &g, const Xft &x, int iv0, int l) {
   const int k = x.kb[l] + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int iv = iv0 + blockIdx.z;
@@ -259,7 +259,7 @@ __global__ void k_xft_goup(Geo g, Xft x, int iv0, int l) {
 // iv0..iv0+nv-1: the max norm against `test`, the cycle count of those that
 // pass, the norm kept, the gathering slot cleared; nact[slot] = the number
 // still iterating
-__global__ __launch_bounds__(256) void k_xft_check(Xft x, int iv0, int nv, int cyc, double test, int slot) {
+LW_KERNEL void k_xft_check(LW_KP(256, 0), const Xft &x, int iv0, int nv, int cyc, double test, int slot) {
   __shared__ int sh[4];
   int act = 0;
   for (int q = threadIdx.x; q < nv; q += 256) {

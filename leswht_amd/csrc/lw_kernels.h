@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_ext.h>
 
+#include "lw_launch.h"
+
 #include "lw_internal.h"
 
 namespace lw {
@@ -19,14 +21,25 @@ extern thread_local Prof *g_prof;
 bool prof_take(const char *name, hipEvent_t *a, hipEvent_t *b);
 void prof_record(const char *name, hipEvent_t a, hipEvent_t b, double bytes, double bytes_kv);
 
-#define LW_LAUNCH(NAME, BYTES, BYTES_KV, KERNEL, GRID, BLOCK, SHM, STREAM, ...)                      \
+// KERNEL names a kernel body (lw_launch.h); LW_LAUNCH_K takes a KFn value
+// chosen at run time. While the calling thread records (batched stepping),
+// the launch is appended to its recorder instead.
+#define LW_LAUNCH(NAME, BYTES, BYTES_KV, KERNEL, GRID, BLOCK, SHM, STREAM, ...) \
+  LW_LAUNCH_K(NAME, BYTES, BYTES_KV, LW_KFN(KERNEL), GRID, BLOCK, SHM, STREAM, __VA_ARGS__)
+
+#define LW_LAUNCH_K(NAME, BYTES, BYTES_KV, KFN, GRID, BLOCK, SHM, STREAM, ...)                      \
   do {                                                                                               \
+    const auto lw_k_ = (KFN);                                                                        \
+    if (::lw::g_rec != nullptr) {                                                                    \
+      ::lw::rec_launch(lw_k_, NAME, GRID, BLOCK, SHM, __VA_ARGS__);                                  \
+      break;                                                                                         \
+    }                                                                                                \
     hipEvent_t lw_a_ = nullptr, lw_b_ = nullptr;                                                     \
     if (::lw::g_prof != nullptr && ::lw::prof_take(NAME, &lw_a_, &lw_b_)) {                          \
-      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, lw_a_, lw_b_, 0u, __VA_ARGS__);        \
+      hipExtLaunchKernelGGL(lw_k_.one(), GRID, BLOCK, SHM, STREAM, lw_a_, lw_b_, 0u, __VA_ARGS__);   \
       ::lw::prof_record(NAME, lw_a_, lw_b_, (double)(BYTES), (double)(BYTES_KV));                    \
     } else {                                                                                         \
-      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                             \
+      hipLaunchKernelGGL(lw_k_.one(), GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                        \
     }                                                                                                \
   } while (0)
 
@@ -207,3 +220,8 @@ void launch_mgrd_iteration(hipStream_t s, const Geo &g, const M3 &m, const Fld &
 void launch_mgrd_finish(hipStream_t s, const Geo &g, const M3 &m, const Fld &f);
 
 }  // namespace lw
+
+// Kernel bodies take their workgroup index and grid size as parameters
+// (lw_launch.h): in the kernel sources these names refer to them.
+#define blockIdx lw_bid_
+#define gridDim lw_gdim_

@@ -50,11 +50,27 @@ static int fail(int code, const std::string &msg) {
 // instead of leaving wrong fields. hipErrorNotReady is what the host's
 // polling of a busy stream leaves there, not a failure.
 static void check_launches(const char *where) {
+  if (g_rec) return;  // a recording thread launches nothing itself
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess && e != hipErrorNotReady) {
     g_err = std::string("HIP error: ") + hipGetErrorString(e) + " in " + where;
     throw LW_ERR_DEVICE;
   }
+}
+
+// Stream operations of the step. While the calling thread records for a
+// batch (g_rec set), they join its recording, and a host synchronisation is
+// the batch's rendezvous: the recorded work of every case runs, and the
+// read-backs have arrived when it returns.
+struct lw_handle;
+static void st_zero(lw_handle *h, void *p, size_t bytes);
+static void st_read(lw_handle *h, void *dst, const void *src, size_t bytes);
+static void st_sync(lw_handle *h);
+static void st_event(lw_handle *h, hipEvent_t e);
+// a batch's gate (lw_batch.hip): the cases of a batch wait for each other
+// here, so that they record the same phase of the step together
+static inline void st_gate() {
+  if (g_rec) batch_gate();
 }
 
 // ---------------------------------------------------------------------------
@@ -238,6 +254,7 @@ struct lw_handle {
   double s_phase[4] = {0, 0, 0, 0};
   bool tev_valid = false;  // tev hold a whole step recorded by lw_step
   Prof *prof = nullptr;
+  struct lw_batch *batch = nullptr;        // the batch the handle steps in, if any
   double *h_red = nullptr;                 // pinned
   // lw_create_static: every device array comes from the device's static
   // area (arena, a device address) instead of hipMalloc, and the host copies
@@ -1361,6 +1378,7 @@ int64_t lw_static_area_bytes(void) { return (int64_t)static_area_bytes(); }
 
 int lw_destroy(lw_handle *h) {
   if (!h) return LW_OK;
+  if (h->batch) return fail(LW_ERR_ARG, "lw_destroy: the handle belongs to a batch (lw_batch_destroy first)");
   if (h->s) hipStreamSynchronize(h->s);
   if (h->s2) hipStreamSynchronize(h->s2);
   for (void *p : h->allocs) hipFree(p);
@@ -1562,6 +1580,22 @@ int64_t lw_get_array(lw_handle *h, const char *name, double *out, int64_t n) {
 // V-cycle is a defect correction in FP64 with an FP32 coarse-grid
 // correction. Switching clears the planes: the other form's bit patterns are
 // not values of this one.
+static void st_zero(lw_handle *h, void *p, size_t bytes) {
+  if (g_rec) rec_zero(p, bytes);
+  else HIPCHK(hipMemsetAsync(p, 0, bytes, h->s));
+}
+static void st_read(lw_handle *h, void *dst, const void *src, size_t bytes) {
+  if (g_rec) rec_read(dst, src, bytes);
+  else HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->s));
+}
+static void st_sync(lw_handle *h) {
+  if (g_rec) batch_sync();
+  else HIPCHK(hipStreamSynchronize(h->s));
+}
+static void st_event(lw_handle *h, hipEvent_t e) {
+  if (!g_rec) HIPCHK(hipEventRecord(e, h->s));
+}
+
 static void set_mg_fp32(lw_handle *h, bool on) {
   MG &mg = h->mg;
   if (on && (h->st.poisson != 0 || !mg_mixed_supported(h->g, mg))) {
@@ -1584,14 +1618,14 @@ static void stepinit(lw_handle *h) {
   h->m = h->ntime;
   for (int c = 0; c < h->ncomp; ++c)
     if (h->ib[c].nbody > 0)
-      HIPCHK(hipMemsetAsync(h->ib[c].fcvavg, 0, sizeof(double) * h->ib[c].nbody, h->s));
+      st_zero(h, h->ib[c].fcvavg, sizeof(double) * h->ib[c].nbody);
   if (st.idtopt == 0) h->dt = st.dt_size;
-  HIPCHK(hipMemsetAsync(h->f.redmax, 0, 8 * sizeof(unsigned long long), h->s));
+  st_zero(h, h->f.redmax, 8 * sizeof(unsigned long long));
   launch_cfl(h->s, h->g, h->f, h->dt);
   if (st.idtopt != 0) {
     // the new dt depends on the CFL number: one host round trip per step
-    HIPCHK(hipMemcpyAsync(h->h_sumres, h->f.redmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
+    st_read(h, h->h_sumres, h->f.redmax, sizeof(unsigned long long));
+    st_sync(h);
     double c;
     memcpy(&c, &h->h_sumres[0], 8);
     h->cflmax = c;
@@ -1651,7 +1685,7 @@ static void phase_sgs(lw_handle *h) {
   const bool zero_first = st.iread == 0 && h->ntime == 1;
   // sgscalc (sgs.f90:288-332)
   int slot = -1;
-  if (zero_first) HIPCHK(hipMemsetAsync(f.nu, 0, np * sizeof(double), h->s));
+  if (zero_first) st_zero(h, f.nu, np * sizeof(double));
   else if (st.idvmon == 1) {
     if (h->sgs_fused) {
       launch_sgs_dvmg_fused(h->s, g, f, h->nwall, (int)st.filter);
@@ -1667,7 +1701,7 @@ static void phase_sgs(lw_handle *h) {
   if (!h->heat) return;
   // sgscalc_t (sgs.f90:1203-1247)
   slot = -1;
-  if (zero_first) HIPCHK(hipMemsetAsync(f.al, 0, np * sizeof(double), h->s));
+  if (zero_first) st_zero(h, f.al, np * sizeof(double));
   else if (st.idvmon == 1) {
     if (h->sgs_fused) {
       launch_sgs_edm_fused(h->s, g, f, h->nwall, (int)st.filter);
@@ -1833,9 +1867,9 @@ static void poisson_sor(lw_handle *h, double test) {
       launch_sor_sweep(h->s, g, m, 1.0, ww, test, 1, h->sorlog, ld);
     }
     launched += n;
-    HIPCHK(hipMemcpyAsync(h->h_flags, m.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipMemcpyAsync(h->h_rfin, m.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
+    st_read(h, h->h_flags, m.done, 256 * sizeof(int));
+    st_read(h, h->h_rfin, m.rfin, 3 * sizeof(double));
+    st_sync(h);
     bool all = true;
     for (int q = m.kv0 - 1; q < m.kv0 - 1 + m.nkv; ++q) all = all && h->h_flags[q] != 0;
     if (all || launched >= itr) break;
@@ -1846,9 +1880,8 @@ static void poisson_sor(lw_handle *h, double test) {
   h->sor_hint = kmax;
   if (kmax > 0) {
     h->h_sorlog.resize((size_t)kmax * ld);
-    HIPCHK(hipMemcpyAsync(h->h_sorlog.data(), h->sorlog, h->h_sorlog.size() * sizeof(double), hipMemcpyDeviceToHost,
-                          h->s));
-    HIPCHK(hipStreamSynchronize(h->s));
+    st_read(h, h->h_sorlog.data(), h->sorlog, h->h_sorlog.size() * sizeof(double));
+    st_sync(h);
   }
   // RG = errmax * dtconst * float(n3mh) (poiss_z_ft.f90:584)
   for (int q = m.kv0 - 1; q < m.kv0 - 1 + m.nkv; ++q)
@@ -2006,7 +2039,7 @@ static void phase_poisson(lw_handle *h) {
   const bool fold = mg_folds_check(mg);
   const int nmg = mg.nkv;  // kv = 1..nmg by multigrid, the rest by line-SOR
   if (g_prof) g_prof->nkv = nmg;
-  if (h->mg_split && mg.fused && nmg >= 2 && mg.mgitr > 0) {
+  if (h->mg_split && !g_rec && mg.fused && nmg >= 2 && mg.mgitr > 0) {
     poisson_split(h, test, fold);
   } else {
     launch_mg_toplevel(h->s, g, mg, 0, 0.0, test, !fold);
@@ -2021,17 +2054,17 @@ static void phase_poisson(lw_handle *h) {
         launch_mg_cycle(h->s, g, mg, test, cyc, !fold || c == batch - 1);
       }
       launched += batch;
-      HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
-      HIPCHK(hipMemcpyAsync(h->h_rfin, mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-      HIPCHK(hipStreamSynchronize(h->s));
+      st_read(h, h->h_flags, mg.done, 256 * sizeof(int));
+      st_read(h, h->h_rfin, mg.rfin, 3 * sizeof(double));
+      st_sync(h);
       all_done = true;
       for (int q = 0; q < nmg; ++q) all_done = all_done && h->h_flags[q] != 0;
       batch = 2;
     }
     if (launched == 0) {  // every wavenumber passed on the first toplevel
-      HIPCHK(hipMemcpyAsync(h->h_flags, mg.done, 256 * sizeof(int), hipMemcpyDeviceToHost, h->s));
-      HIPCHK(hipMemcpyAsync(h->h_rfin, mg.rfin, 3 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-      HIPCHK(hipStreamSynchronize(h->s));
+      st_read(h, h->h_flags, mg.done, 256 * sizeof(int));
+      st_read(h, h->h_rfin, mg.rfin, 3 * sizeof(double));
+      st_sync(h);
     }
   }
   if (h->sub.msub == 1) {
@@ -2067,8 +2100,9 @@ static void phase_poisson(lw_handle *h) {
     h->mgres_rm[n] = h->h_rfin[q] * h->sub.dtconst;
   }
   if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
+  st_gate();
   if (cmax > h->cyc_max) h->cyc_max = cmax;
-  if (h->mg_split && mg.fused && nmg >= 2) {
+  if (h->mg_split && !g_rec && mg.fused && nmg >= 2) {
     h->mg_hint1 = h->h_flags[128];
     h->mg_hint = crest;
   } else {
@@ -2219,10 +2253,10 @@ static void field_avg(lw_handle *h) {
 }
 
 static void read_stats(lw_handle *h, lw_step_stats *stats) {
-  HIPCHK(hipMemcpyAsync(h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->s));
-  HIPCHK(hipMemcpyAsync(h->h_red, h->f.red, 16 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-  HIPCHK(hipMemcpyAsync(h->h_red + 16, h->hist, 24 * sizeof(double), hipMemcpyDeviceToHost, h->s));
-  HIPCHK(hipStreamSynchronize(h->s));
+  st_read(h, h->h_sumres + 8, h->f.redmax, 3 * sizeof(unsigned long long));
+  st_read(h, h->h_red, h->f.red, 16 * sizeof(double));
+  st_read(h, h->h_red + 16, h->hist, 24 * sizeof(double));
+  st_sync(h);
   memcpy(h->h_hist, h->h_red + 16, 24 * sizeof(double));
   for (int q = 0; q < 4; ++q) {
     float a = 0.f, b = 0.f, c = 0.f;
@@ -2475,29 +2509,30 @@ extern "C" int lw_step(lw_handle *h, int64_t nsteps, lw_step_stats *stats) {
       check_launches("stepinit");
       for (int ms = 1; ms <= 3; ++ms) {
         substepinit(h, ms);
+        st_gate();
         hipEvent_t *ev = h->tev[ms - 1];
-        HIPCHK(hipEventRecord(ev[0], h->s));
+        st_event(h, ev[0]);
         phase_sgs(h);
         check_launches("sgscalc");
-        HIPCHK(hipEventRecord(ev[1], h->s));
-        HIPCHK(hipEventRecord(ev[2], h->s));
+        st_event(h, ev[1]);
+        st_event(h, ev[2]);
         phase_rhsnlhs(h);
         check_launches("rhsnlhs");
-        HIPCHK(hipEventRecord(ev[3], h->s));
-        HIPCHK(hipEventRecord(ev[4], h->s));
+        st_event(h, ev[3]);
+        st_event(h, ev[4]);
         launch_divgs(h->s, h->g, h->sub, h->f, (int)(h->st.masson == 1 && h->st.imovingon == 0));
         phase_poisson(h);
         check_launches("divgs/poisson");
-        HIPCHK(hipEventRecord(ev[5], h->s));
+        st_event(h, ev[5]);
         launch_projection(h->s, h->g, h->sub, h->f);
         after_projection(h);
         if (h->st.ibmon == 1) launch_lagforce(h->s, h->g, h->sub, h->f, h->ib);
         check_launches("ucalc/pcalc");
-        HIPCHK(hipEventRecord(ev[6], h->s));
+        st_event(h, ev[6]);
         phase_energy(h);
         check_launches("rhsnlhs_t");
-        HIPCHK(hipEventRecord(ev[7], h->s));
-        if (ms == 3) h->tev_valid = true;
+        st_event(h, ev[7]);
+        if (ms == 3) h->tev_valid = !g_rec;
       }
       end_of_step(h);
       if (h->st.iavg == 1) field_avg(h);
@@ -2757,4 +2792,88 @@ extern "C" int lw_write_field(lw_handle *h, const char *path) {
   const int64_t ich[2] = {s.ich, s.iconjg};
   return lw_write_fld(path, n, s.re, s.pr, s.gr, h->ihist, h->m, h->time, h->dt, prdic, bcs, ich, u.data(), v.data(),
                       w.data(), p.data(), h->heat ? t.data() : nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// batched stepping (lw_batch.hip): each case's lw_step runs on a thread of the
+// batch that records its launches; the batch launches every kernel once for
+// all the cases that issue it
+// ---------------------------------------------------------------------------
+struct lw_batch {
+  std::vector<lw_handle *> hs;
+  std::vector<lw_step_stats> stats;
+  std::vector<std::string> errs;
+  Batch *core = nullptr;
+};
+
+static int batch_member(void *ctx, int i, int64_t nsteps) {
+  lw_batch *b = (lw_batch *)ctx;
+  const int rc = lw_step(b->hs[i], nsteps, &b->stats[i]);
+  if (rc != LW_OK) b->errs[i] = g_err;
+  return rc;
+}
+
+extern "C" int lw_batch_create(lw_handle *const *hs, int64_t n, lw_batch **out) {
+  if (!out) return fail(LW_ERR_ARG, "lw_batch_create: null output");
+  *out = nullptr;
+  if (!hs || n < 1 || n > 4096) return fail(LW_ERR_ARG, "lw_batch_create: 1..4096 handles required");
+  for (int64_t i = 0; i < n; ++i) {
+    const lw_handle *h = hs[i];
+    const std::string c = "lw_batch_create: case " + std::to_string(i) + ": ";
+    if (!h) return fail(LW_ERR_ARG, c + "null handle");
+    for (int64_t k = 0; k < i; ++k)
+      if (hs[k] == h) return fail(LW_ERR_ARG, c + "the same handle twice");
+    if (h->batch) return fail(LW_ERR_ARG, c + "the handle already belongs to a batch");
+    if (h->st.poisson == 1 || h->st.poisson == 3)
+      return fail(LW_ERR_ARG, c + "batched stepping takes the z-DFT or poiss_ftft Poisson solvers");
+    if (h->st.iavg == 1) return fail(LW_ERR_ARG, c + "batched stepping takes IAVG = 0");
+    if (h->prof) return fail(LW_ERR_ARG, c + "a profile is running on the handle");
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(LW_ERR_DEVICE, "lw_batch_create: no device");
+  lw_batch *b = new lw_batch;
+  b->hs.assign(hs, hs + n);
+  b->stats.assign((size_t)n, lw_step_stats{});
+  b->errs.assign((size_t)n, std::string());
+  b->core = batch_new(dev, (int)n, batch_member, b);
+  if (!b->core) {
+    delete b;
+    return fail(LW_ERR_DEVICE, "lw_batch_create: the batch's stream could not be created");
+  }
+  for (lw_handle *h : b->hs) h->batch = b;
+  *out = b;
+  return LW_OK;
+}
+
+extern "C" int lw_batch_step(lw_batch *b, int64_t nsteps, lw_step_stats *stats) {
+  if (!b || nsteps < 0) return fail(LW_ERR_ARG, "lw_batch_step: bad argument");
+  // work a case enqueued on its own streams comes first
+  for (lw_handle *h : b->hs) {
+    if (h->s && hipStreamSynchronize(h->s) != hipSuccess) return fail(LW_ERR_DEVICE, "lw_batch_step: stream error");
+    if (h->s2 && hipStreamSynchronize(h->s2) != hipSuccess) return fail(LW_ERR_DEVICE, "lw_batch_step: stream error");
+  }
+  std::vector<int> rc(b->hs.size(), 0);
+  const int r = batch_run(b->core, nsteps, rc.data());
+  if (stats)
+    for (size_t i = 0; i < b->hs.size(); ++i) stats[i] = b->stats[i];
+  if (r != LW_OK) {
+    for (size_t i = 0; i < rc.size(); ++i)
+      if (rc[i] != LW_OK) return fail(rc[i], "lw_batch_step: case " + std::to_string(i) + ": " + b->errs[i]);
+    return fail(r, "lw_batch_step: the batch failed");
+  }
+  return LW_OK;
+}
+
+extern "C" int lw_batch_info(const lw_batch *b, int64_t info[4]) {
+  if (!b || !info) return fail(LW_ERR_ARG, "lw_batch_info: null argument");
+  batch_counts(b->core, info);
+  return LW_OK;
+}
+
+extern "C" int lw_batch_destroy(lw_batch *b) {
+  if (!b) return LW_OK;
+  batch_free(b->core);
+  for (lw_handle *h : b->hs) h->batch = nullptr;
+  delete b;
+  return LW_OK;
 }

@@ -125,7 +125,8 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
            "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
            "lw_create_static", "lw_static_area_bytes", "lw_set_trace", "lw_trace_info", "lw_get_trace",
-           "lw_get_mg_residuals", "lw_get_residual_rows", "lw_get_bicg_log", "lw_get_bicg_solves")
+           "lw_get_mg_residuals", "lw_get_residual_rows", "lw_get_bicg_log", "lw_get_bicg_solves",
+           "lw_batch_create", "lw_batch_step", "lw_batch_info", "lw_batch_destroy")
 
 _lib = None
 
@@ -195,6 +196,10 @@ def load_library(path: str = LIB_PATH):
     L.lw_server_wait.argtypes = [H, _I, C.POINTER(lw_case_result)]
     L.lw_server_stats.argtypes = [H, C.POINTER(lw_server_info)]
     L.lw_server_destroy.argtypes = [H]
+    L.lw_batch_create.argtypes = [C.POINTER(H), _I, C.POINTER(H)]
+    L.lw_batch_step.argtypes = [H, _I, C.POINTER(lw_step_stats)]
+    L.lw_batch_info.argtypes = [H, _PI]
+    L.lw_batch_destroy.argtypes = [H]
     _lib = L
     return L
 
@@ -301,6 +306,42 @@ class IBTablesNative:
         if self._h:
             load_library().lw_ib_free(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Solvers stepped together (lw_batch_*): every kernel of the step is
+    launched once for all the cases that issue it. Each case gives the same
+    results as stepped alone. The solvers stay usable for fields and
+    histories between steps; destroy the batch (``close``) before the
+    solvers."""
+
+    def __init__(self, solvers):
+        L = load_library()
+        self.solvers = list(solvers)
+        hs = (C.c_void_p * len(self.solvers))(*[sv._h.value for sv in self.solvers])
+        self._b = C.c_void_p()
+        _check(L.lw_batch_create(hs, len(self.solvers), C.byref(self._b)))
+
+    def step(self, nsteps: int = 1) -> list:
+        st = (lw_step_stats * len(self.solvers))()
+        _check(load_library().lw_batch_step(self._b, int(nsteps), st))
+        return [x.as_dict() for x in st]
+
+    def info(self) -> dict:
+        a = np.zeros(4, dtype=np.int64)
+        _check(load_library().lw_batch_info(self._b, _pi(a)))
+        return dict(rendezvous=int(a[0]), launches=int(a[1]), case_launches=int(a[2]), reads=int(a[3]))
+
+    def close(self):
+        if self._b:
+            load_library().lw_batch_destroy(self._b)
+            self._b = None
 
     def __del__(self):
         try:
@@ -612,7 +653,7 @@ class Solver:
 
     def close(self):
         if self._h:
-            load_library().lw_destroy(self._h)
+            _check(load_library().lw_destroy(self._h))
             self._h = None
 
     def __del__(self):
