@@ -161,14 +161,18 @@ def serve_leg(args) -> dict:
         t0 = time.time()
         dirs = serve.prepare(cases, root)
         t_prep = time.time() - t0
-        out = serve.run_stream(cases, dirs, int(args.budget_gb * (1 << 30)), args.slots, honour_arrivals=True,
-                               write_fields=False)
+        budget = int(args.budget_gb * (1 << 30))
+        out = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=True, write_fields=False)
+        bat = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=True, write_fields=False,
+                               batch_max_cells=args.batch_cells)
     finally:
         shutil.rmtree(root, ignore_errors=True)
     return {"cases": len(cases), "large_cases": sum(c.large for c in cases), "rate_per_s": args.rate,
             "step_budgets": "U{200..2000} x %g" % args.serve_leg_scale, "slots": args.slots,
             "p50_s": out["p50"], "p99_s": out["p99"], "cell_updates_per_s": out["throughput"],
-            "failed": len(out["failed"]), "prep_s": round(t_prep, 1)}
+            "failed": len(out["failed"]) + len(bat["failed"]), "prep_s": round(t_prep, 1),
+            "continuous_batching": {"batch_max_cells": args.batch_cells, "p50_s": bat["p50"], "p99_s": bat["p99"],
+                                    "cell_updates_per_s": bat["throughput"]}}
 
 
 def serve_bench(args) -> dict:
@@ -198,6 +202,14 @@ def serve_bench(args) -> dict:
               file=sys.stderr, flush=True)
         single = serve.run_stream(cases, dirs, budget, 1, honour_arrivals=False, write_fields=False)
         print("all at once, 1 slot: %.3e cell-updates/s" % single["throughput"], file=sys.stderr, flush=True)
+        # continuous batching of the cases of at most batch_cells cells
+        bstream = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=True, write_fields=False,
+                                   batch_max_cells=args.batch_cells)
+        print("stream, batched: p50 %.3f s p99 %.3f s, %.3e cell-updates/s" % (
+            bstream["p50"], bstream["p99"], bstream["throughput"]), file=sys.stderr, flush=True)
+        bclosed = serve.run_stream(cases, dirs, budget, args.slots, honour_arrivals=False, write_fields=False,
+                                   batch_max_cells=args.batch_cells)
+        print("all at once, batched: %.3e cell-updates/s" % bclosed["throughput"], file=sys.stderr, flush=True)
         mixed = None
         if args.mixed:
             # the served mixed-precision form: the same batch through all
@@ -232,10 +244,15 @@ def serve_bench(args) -> dict:
                    "peak_reserved_gb": round(stream["info"]["peak_reserved_bytes"] / (1 << 30), 2)},
         "closed_batch": {"slots": args.slots, "value": closed["throughput"], "one_slot": single["throughput"],
                          "ratio": closed["throughput"] / single["throughput"]},
+        "continuous_batching": {"batch_max_cells": args.batch_cells, "stream_p50_s": bstream["p50"],
+                                "stream_p99_s": bstream["p99"], "stream_value": bstream["throughput"],
+                                "closed_value": bclosed["throughput"],
+                                "closed_ratio_vs_one_slot": bclosed["throughput"] / single["throughput"]},
         "closed_batch_mixed": None if mixed is None else {
             "slots": args.slots, "value": mixed["throughput"],
             "ratio_vs_one_slot_fp64": mixed["throughput"] / single["throughput"]},
-        "failed": len(stream["failed"]) + len(closed["failed"]) + len(single["failed"]) +
+        "failed": len(stream["failed"]) + len(closed["failed"]) + len(single["failed"]) + len(bstream["failed"]) +
+                  len(bclosed["failed"]) +
                   (len(mixed["failed"]) if mixed else 0),
     }
 
@@ -259,9 +276,15 @@ def batch_bench(args) -> dict:
     t_prep = time.time() - t0
     print("built %d cases in %.1f s" % (len(cases), t_prep), file=sys.stderr, flush=True)
     sols = [Solver.from_case(v) for v in views]
+    n_mixed = 0
     if args.mixed:
+        # the FP32 multigrid levels wherever a case's levels allow them
         for sv in sols:
-            sv.set_option("mg_fp32", 1)
+            try:
+                sv.set_option("mg_fp32", 1)
+                n_mixed += 1
+            except Exception:
+                pass
     cells = [c.cells for c in cases]
     k, w = args.steps, args.warmup
     # plain form: one case at a time
@@ -324,6 +347,8 @@ def batch_bench(args) -> dict:
                            "rendezvous": info["rendezvous"],
                            "cases_per_launch": info["case_launches"] / max(info["launches"], 1)},
         "mg_cycles_sum_last_step": int(sum(x["mg_cycles_sum"] for x in st)),
+        "cases_with_fp32_levels": n_mixed,
+        "host_ms_per_step": {"flush_prepare": info["prepare_us"] / 1e3 / k, "device_wait": info["wait_us"] / 1e3 / k},
     }
 
 
@@ -361,6 +386,10 @@ def main():
                     help="pick the multigrid tile sizes with the library's autotuner before the warmup")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batched form on this many independent small cases instead of the time step")
+    ap.add_argument("--batch-leg", type=int, default=32,
+                    help="cases of the short batched-form comparison reported beside the step (0: none)")
+    ap.add_argument("--batch-cells", type=int, default=2000000,
+                    help="served forms: cases of at most this many cells join the continuous batch")
     ap.add_argument("--serve", type=int, default=0,
                     help="run the served form on a stream of this many cases instead of the time step")
     ap.add_argument("--rate", type=float, default=10.0, help="served form: arrivals per second")
@@ -526,6 +555,7 @@ def main():
         "poisson": poisson,
         "later_window": later,
         "serve": None,
+        "batch": None,
         "cpu_baseline": None,
     }
     if args.serve_leg > 0:
@@ -533,6 +563,17 @@ def main():
             out["serve"] = serve_leg(args)
         except Exception as e:  # reported beside the step, never required
             out["serve"] = {"error": str(e)[:300]}
+    if args.batch_leg > 0:
+        try:
+            leg = argparse.Namespace(**vars(args))
+            leg.batch, leg.steps, leg.warmup, leg.mixed = args.batch_leg, 5, 2, False
+            bb = batch_bench(leg)
+            out["batch"] = {q: bb[q] for q in ("value", "unit", "steps", "one_at_a_time", "concurrent_streams",
+                                               "ratio_vs_one_at_a_time", "ratio_vs_concurrent_streams",
+                                               "launch_sharing")}
+            out["batch"]["workload"] = bb["config"]["workload"]
+        except Exception as e:  # reported beside the step, never required
+            out["batch"] = {"error": str(e)[:300]}
     if not args.no_cpu_baseline:
         k = min(args.cpu_steps, args.steps)
         gpu_same = {"s": sum(step_s[:k]), "mg_rows": mg_rows}

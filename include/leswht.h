@@ -345,6 +345,15 @@ typedef struct lw_server_info {
 } lw_server_info;
 typedef struct lw_server lw_server;
 int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out);
+/* The same server with continuous batching: an admitted case of at most
+ * batch_max_cells cells is created by a worker and then joins the server's
+ * batch (lw_batch_*), which steps all its cases together, one step at a
+ * time; cases join and leave it between those steps, as they arrive and
+ * finish, so a slot is held only while its case is created. Larger cases
+ * run on their own streams as in lw_server_create. batch_max_cells = 0 is
+ * lw_server_create. Every case gives the same results as alone. */
+int lw_server_create_batched(int64_t mem_budget_bytes, int64_t slots, int64_t batch_max_cells,
+                             lw_server **out);
 int lw_server_submit(lw_server *s, const char *solver_dir, const lw_body *body, int64_t nsteps,
                      const char *fld_out, int64_t *ticket);
 /* current state of a request (non-blocking) */
@@ -368,11 +377,13 @@ int lw_server_destroy(lw_server *s);
  * every case; stats (NULL or n entries) get each case's last step (phase
  * times are not measured: 0). lw_batch_info: info[0] rendezvous (host
  * synchronisation points), [1] batched launches, [2] per-case launches they
- * carried, [3] per-case read-backs. */
+ * carried, [3] per-case read-backs, [4] host microseconds spent merging the
+ * recordings and filling the staging buffers, [5] host microseconds waiting
+ * for the device. */
 typedef struct lw_batch lw_batch;
 int lw_batch_create(lw_handle *const *handles, int64_t n, lw_batch **out);
 int lw_batch_step(lw_batch *b, int64_t nsteps, lw_step_stats *stats);
-int lw_batch_info(const lw_batch *b, int64_t info[4]);
+int lw_batch_info(const lw_batch *b, int64_t info[6]);
 int lw_batch_destroy(lw_batch *b);
 
 #ifdef __cplusplus

@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -107,6 +108,7 @@ struct Batch {
   size_t rcap = 0;
   // counters: flushes, batched launches, recorded launches, read-backs
   int64_t n_flush = 0, n_launch = 0, n_ops = 0, n_read = 0;
+  int64_t us_prepare = 0, us_wait = 0;
 };
 
 thread_local Batch *g_batch = nullptr;
@@ -151,7 +153,12 @@ void grow(Batch *b, size_t need, char **h, char **d, size_t *cap) {
 
 // merges and issues every recorder's operations; called with mu held while
 // every active member waits at its rendezvous
+int64_t usec_since(std::chrono::steady_clock::time_point t0) {
+  return (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void flush(Batch *b) {
+  const auto t_start = std::chrono::steady_clock::now();
   const int nr = (int)b->rec.size();
   std::vector<size_t> pos(nr, 0);
   std::vector<Group> plan;
@@ -283,6 +290,8 @@ void flush(Batch *b) {
     launches.push_back(Launch{gp.key.kind == 1 ? kcopy.bat_ : gp.key.bat, blk, block, shm,
                               (const BItem *)(D + o), (int)gp.ops.size()});
   }
+  b->us_prepare += usec_since(t_start);
+  const auto t_issue = std::chrono::steady_clock::now();
   hipError_t e = hipMemcpyAsync(D, H, total, hipMemcpyHostToDevice, b->s);
   for (const Launch &l : launches) {
     if (e != hipSuccess) break;
@@ -293,6 +302,7 @@ void flush(Batch *b) {
   }
   if (e == hipSuccess && roff > 0) e = hipMemcpyAsync(b->hrb, b->drb, roff, hipMemcpyDeviceToHost, b->s);
   if (e == hipSuccess) e = hipStreamSynchronize(b->s);
+  b->us_wait += usec_since(t_issue);
   if (e == hipSuccess) e = hipGetLastError();
   if (e != hipSuccess) throw std::string("HIP error in a batched launch: ") + hipGetErrorString(e);
   for (size_t q = 0; q < scatter.size(); ++q) {
@@ -432,11 +442,13 @@ int batch_run(Batch *b, int64_t nsteps, int *rc) {
 
 const std::string &batch_error(const Batch *b) { return b->msg; }
 
-void batch_counts(const Batch *b, int64_t out[4]) {
+void batch_counts(const Batch *b, int64_t out[6]) {
   out[0] = b->n_flush;
   out[1] = b->n_launch;
   out[2] = b->n_ops;
   out[3] = b->n_read;
+  out[4] = b->us_prepare;
+  out[5] = b->us_wait;
 }
 
 void batch_free(Batch *b) {

@@ -271,7 +271,8 @@ void batch_gate();
 struct Batch;
 Batch *batch_new(int dev, int n, int (*run)(void *, int, int64_t), void *ctx);
 int batch_run(Batch *b, int64_t nsteps, int *rc);
-void batch_counts(const Batch *b, int64_t out[4]);
+void batch_counts(const Batch *b, int64_t out[6]);
+
 void batch_free(Batch *b);
 
 }  // namespace lw

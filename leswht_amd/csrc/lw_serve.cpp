@@ -14,9 +14,18 @@
 // Admission is first-come first-served: the oldest queued request is admitted
 // next, and it waits (blocking those behind it) until enough of the budget
 // is free, so a large case is never starved by a stream of small ones.
+//
+// Continuous batching (lw_server_create_batched): a small case, once its
+// worker has created it, joins the server's batch instead of being stepped
+// by the worker. One batch thread steps every joined case together
+// (lw_batch_step: each kernel launched once for all of them), one step at a
+// time; between those steps it takes in the cases that joined and lets go
+// of those that reached their step count (their fields are written, their
+// memory freed). The worker is free again as soon as its case has joined.
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -38,6 +47,7 @@ struct Request {
   int64_t nsteps = 0;
   lw_case_result res{};
   std::string err;
+  lw_handle *h = nullptr;  // a batched case's handle while it is in the batch
 };
 
 double seconds_since(std::chrono::steady_clock::time_point t0) {
@@ -48,6 +58,9 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
 
 struct lw_server {
   int64_t budget = 0, nslots = 1;
+  int64_t batch_max_cells = 0;
+  std::vector<Request *> joining;  // batched cases created and waiting to join
+  std::thread batcher;
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -90,6 +103,96 @@ struct lw_server {
     return o;
   }
 
+  // with mu held: a case's end (done or failed), its budget released
+  void finish_locked(Request *r, int rc, const std::string &err) {
+    r->err = err;
+    r->res.t_done = seconds_since(t0);
+    r->res.status = rc == LW_OK ? LW_CASE_DONE : rc;
+    reserved -= r->res.reserved_bytes;
+    running -= 1;
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+
+  std::condition_variable cv_batch;
+
+  // The batch thread: steps the joined cases together, takes in new ones
+  // and lets go of finished ones between calls of lw_batch_step.
+  void batch_loop() {
+    std::vector<Request *> act;
+    lw_batch *b = nullptr;
+    bool changed = false;
+    std::vector<lw_step_stats> st;
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_batch.wait(lk, [&] { return !joining.empty() || !act.empty() || (stopping && queue.empty() && running == 0); });
+      if (act.empty() && joining.empty()) break;
+      if (!joining.empty()) {
+        act.insert(act.end(), joining.begin(), joining.end());
+        joining.clear();
+        changed = true;
+      }
+      lk.unlock();
+      // one step per call: lw_batch_step's statistics are those of the
+      // call's last step, and cases join and leave between calls
+      int64_t k = 1;
+      for (Request *r : act) k = std::min(k, r->nsteps - r->res.steps_done);
+      int rc = LW_OK;
+      std::string err;
+      if (changed) {
+        if (b) lw_batch_destroy(b);
+        b = nullptr;
+        std::vector<lw_handle *> hs;
+        for (Request *r : act) hs.push_back(r->h);
+        rc = lw_batch_create(hs.data(), (int64_t)hs.size(), &b);
+        changed = false;
+      }
+      if (rc == LW_OK && k > 0) {
+        st.assign(act.size(), lw_step_stats{});
+        rc = lw_batch_step(b, k, st.data());
+      }
+      if (rc != LW_OK) err = lw_last_error();
+      // results of the step (published under mu below; this thread is the
+      // only writer of a batched case's result); finished or failed cases
+      // leave the batch
+      std::vector<Request *> keep, out;
+      for (size_t i = 0; i < act.size(); ++i) {
+        Request *r = act[i];
+        const int64_t done = r->res.steps_done + (rc == LW_OK ? k : 0);
+        (rc != LW_OK || done >= r->nsteps ? out : keep).push_back(r);
+      }
+      if (!out.empty()) {
+        if (b) lw_batch_destroy(b);
+        b = nullptr;
+        changed = true;
+      }
+      std::vector<std::pair<int, std::string>> ends;
+      for (Request *r : out) {
+        int rr = rc;
+        std::string e = err;
+        if (rr == LW_OK && !r->fld_out.empty()) {
+          rr = lw_write_field(r->h, r->fld_out.c_str());
+          if (rr != LW_OK) e = lw_last_error();
+        }
+        lw_destroy(r->h);
+        r->h = nullptr;
+        ends.emplace_back(rr, e);
+      }
+      lk.lock();
+      if (rc == LW_OK && k > 0)
+        for (size_t i = 0; i < act.size(); ++i) {
+          Request *r = act[i];
+          r->res.steps_done += k;
+          r->res.mg_cycles_sum += st[i].mg_cycles_sum;
+          r->res.time = st[i].time;
+          r->res.dt = st[i].dt;
+        }
+      for (size_t q = 0; q < out.size(); ++q) finish_locked(out[q], ends[q].first, ends[q].second);
+      act.swap(keep);
+    }
+    if (b) lw_batch_destroy(b);
+  }
+
   void worker() {
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
@@ -107,6 +210,23 @@ struct lw_server {
       r->res.status = LW_CASE_RUNNING;
       r->res.t_admit = seconds_since(t0);
       lk.unlock();
+      if (batch_max_cells > 0 && r->res.cells <= batch_max_cells) {
+        // a batched case: create it and hand it to the batch thread
+        lw_body body = r->body;
+        lw_handle *h = nullptr;
+        const int rc = lw_create_from_dir(r->dir.c_str(), &body, &h);
+        const std::string err = rc == LW_OK ? std::string() : std::string(lw_last_error());
+        lk.lock();
+        if (rc == LW_OK) {
+          r->h = h;
+          r->res.device_bytes = lw_device_bytes(h);
+          joining.push_back(r);
+          cv_batch.notify_all();
+        } else {
+          finish_locked(r, rc, err);
+        }
+        continue;
+      }
       Outcome o = run_case(*r);
       lk.lock();
       r->res.steps_done = o.steps_done;
@@ -114,31 +234,32 @@ struct lw_server {
       r->res.device_bytes = o.device_bytes;
       r->res.time = o.time;
       r->res.dt = o.dt;
-      r->err = o.err;
-      r->res.t_done = seconds_since(t0);
-      r->res.status = o.rc == LW_OK ? LW_CASE_DONE : o.rc;
-      reserved -= r->res.reserved_bytes;
-      running -= 1;
-      cv_done.notify_all();
-      cv_work.notify_all();
+      finish_locked(r, o.rc, o.err);
     }
   }
 };
 
 extern "C" {
 
-int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out) {
-  if (!out || mem_budget_bytes <= 0 || slots < 1 || slots > 64) {
-    lw_set_last_error("lw_server_create: budget must be > 0 and slots in 1..64");
+int lw_server_create_batched(int64_t mem_budget_bytes, int64_t slots, int64_t batch_max_cells,
+                             lw_server **out) {
+  if (!out || mem_budget_bytes <= 0 || slots < 1 || slots > 64 || batch_max_cells < 0) {
+    lw_set_last_error("lw_server_create: budget must be > 0, slots in 1..64 and batch_max_cells >= 0");
     return LW_ERR_ARG;
   }
   auto *s = new lw_server();
   s->budget = mem_budget_bytes;
   s->nslots = slots;
+  s->batch_max_cells = batch_max_cells;
   s->t0 = std::chrono::steady_clock::now();
   for (int64_t q = 0; q < slots; ++q) s->workers.emplace_back([s] { s->worker(); });
+  if (batch_max_cells > 0) s->batcher = std::thread([s] { s->batch_loop(); });
   *out = s;
   return LW_OK;
+}
+
+int lw_server_create(int64_t mem_budget_bytes, int64_t slots, lw_server **out) {
+  return lw_server_create_batched(mem_budget_bytes, slots, 0, out);
 }
 
 int lw_server_submit(lw_server *s, const char *solver_dir, const lw_body *body, int64_t nsteps,
@@ -229,6 +350,11 @@ int lw_server_destroy(lw_server *s) {
     s->cv_work.notify_all();
   }
   for (auto &t : s->workers) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv_batch.notify_all();
+  }
+  if (s->batcher.joinable()) s->batcher.join();
   delete s;
   return LW_OK;
 }

@@ -2100,7 +2100,6 @@ static void phase_poisson(lw_handle *h) {
     h->mgres_rm[n] = h->h_rfin[q] * h->sub.dtconst;
   }
   if (g_prof) g_prof->resolve_mg(h->h_flags + 128, mg.n3mh);
-  st_gate();
   if (cmax > h->cyc_max) h->cyc_max = cmax;
   if (h->mg_split && !g_rec && mg.fused && nmg >= 2) {
     h->mg_hint1 = h->h_flags[128];
@@ -2108,6 +2107,7 @@ static void phase_poisson(lw_handle *h) {
   } else {
     h->mg_hint = cmax;
   }
+  st_gate();
   launch_fft_inverse(h->s, g, mg, h->f);
 }
 
@@ -2864,7 +2864,7 @@ extern "C" int lw_batch_step(lw_batch *b, int64_t nsteps, lw_step_stats *stats) 
   return LW_OK;
 }
 
-extern "C" int lw_batch_info(const lw_batch *b, int64_t info[4]) {
+extern "C" int lw_batch_info(const lw_batch *b, int64_t info[6]) {
   if (!b || !info) return fail(LW_ERR_ARG, "lw_batch_info: null argument");
   batch_counts(b->core, info);
   return LW_OK;

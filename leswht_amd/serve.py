@@ -103,11 +103,12 @@ def fld_path(solver_dir: str) -> str:
 
 
 def run_stream(cases: List[ServedCase], dirs: List[str], budget_bytes: int, slots: int,
-               honour_arrivals: bool = True, write_fields: bool = True) -> dict:
+               honour_arrivals: bool = True, write_fields: bool = True, batch_max_cells: int = 0) -> dict:
     """Submits every case at its arrival time (or all at once), waits for all
     and returns per-case results and the server's counters. Time-to-solution
-    is t_done - t_submit of each case."""
-    srv = Server(budget_bytes, slots)
+    is t_done - t_submit of each case. ``batch_max_cells`` > 0 batches the
+    cases of at most that many cells (continuous batching)."""
+    srv = Server(budget_bytes, slots, batch_max_cells)
     tickets = []
     t0 = time.perf_counter()
     try:

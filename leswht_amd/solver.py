@@ -123,7 +123,7 @@ EXPORTS = ("lw_create", "lw_create_from_dir", "lw_destroy", "lw_set_fields", "lw
            "lw_profile_get", "lw_get_history", "lw_get_settings", "lw_last_error",
            "lw_estimate_device_bytes", "lw_probe_dir", "lw_body_eval", "lw_set_avg_dir", "lw_avg_finalize", "lw_set_option", "lw_autotune",
            "lw_ib_build", "lw_ib_tables", "lw_ib_geomfac_raw", "lw_ib_write", "lw_ib_free", "lw_server_create", "lw_server_submit",
-           "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy",
+           "lw_server_poll", "lw_server_wait", "lw_server_stats", "lw_server_destroy", "lw_server_create_batched",
            "lw_create_static", "lw_static_area_bytes", "lw_set_trace", "lw_trace_info", "lw_get_trace",
            "lw_get_mg_residuals", "lw_get_residual_rows", "lw_get_bicg_log", "lw_get_bicg_solves",
            "lw_batch_create", "lw_batch_step", "lw_batch_info", "lw_batch_destroy")
@@ -191,6 +191,7 @@ def load_library(path: str = LIB_PATH):
     L.lw_body_eval.argtypes = [C.POINTER(lw_body), _D, _D, _D]
     L.lw_body_eval.restype = _D
     L.lw_server_create.argtypes = [_I, _I, C.POINTER(H)]
+    L.lw_server_create_batched.argtypes = [_I, _I, _I, C.POINTER(H)]
     L.lw_server_submit.argtypes = [H, C.c_char_p, C.POINTER(lw_body), _I, C.c_char_p, _PI]
     L.lw_server_poll.argtypes = [H, _I, C.POINTER(lw_case_result)]
     L.lw_server_wait.argtypes = [H, _I, C.POINTER(lw_case_result)]
@@ -334,9 +335,10 @@ class Batch:
         return [x.as_dict() for x in st]
 
     def info(self) -> dict:
-        a = np.zeros(4, dtype=np.int64)
+        a = np.zeros(6, dtype=np.int64)
         _check(load_library().lw_batch_info(self._b, _pi(a)))
-        return dict(rendezvous=int(a[0]), launches=int(a[1]), case_launches=int(a[2]), reads=int(a[3]))
+        return dict(rendezvous=int(a[0]), launches=int(a[1]), case_launches=int(a[2]), reads=int(a[3]),
+                    prepare_us=int(a[4]), wait_us=int(a[5]))
 
     def close(self):
         if self._b:
@@ -356,10 +358,12 @@ class Server:
     by free slot and memory budget, run concurrently on their own streams,
     and written to ``fld_out`` when their steps are done."""
 
-    def __init__(self, budget_bytes: int, slots: int):
+    def __init__(self, budget_bytes: int, slots: int, batch_max_cells: int = 0):
+        """``batch_max_cells`` > 0: continuous batching of the cases of at
+        most that many cells (lw_server_create_batched)."""
         L = load_library()
         self._s = C.c_void_p()
-        _check(L.lw_server_create(int(budget_bytes), int(slots), C.byref(self._s)))
+        _check(L.lw_server_create_batched(int(budget_bytes), int(slots), int(batch_max_cells), C.byref(self._s)))
 
     def submit(self, solver_dir: str, nsteps: int, fld_out: str = None, body=None) -> int:
         t = _I()

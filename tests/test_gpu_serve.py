@@ -55,6 +55,23 @@ def test_gpu_served_cases_equal_alone_and_in_the_mix():
             fa, fm = _fields(da), _fields(dm)
             for k in ("u", "v", "w", "p", "t"):
                 assert np.array_equal(fa[k], fm[k]), (c.name, k)
+        # continuous batching: the cases of at most 70000 cells join the
+        # server's batch as they arrive (staggered) and leave it as they
+        # finish; the largest runs on its own streams. Same bits as alone.
+        import dataclasses
+        staggered = [dataclasses.replace(c, arrival=0.05 * q) for q, c in enumerate(MIX)]
+        bat_dirs = serve.prepare(staggered, os.path.join(root, "batched"))
+        print("batched mix of", len(MIX), flush=True)
+        bout = serve.run_stream(staggered, bat_dirs, budget_bytes=64 << 30, slots=2, honour_arrivals=True,
+                                batch_max_cells=70000)
+        assert not bout["failed"], bout["failed"]
+        for c, a, m, da, dm in zip(MIX, alone, bout["results"], alone_dirs, bat_dirs):
+            assert a["steps_done"] == m["steps_done"] == c.nsteps, c.name
+            assert a["time"] == m["time"] and a["dt"] == m["dt"], c.name
+            assert a["mg_cycles_sum"] == m["mg_cycles_sum"], c.name
+            fa, fm = _fields(da), _fields(dm)
+            for k in ("u", "v", "w", "p", "t"):
+                assert np.array_equal(fa[k], fm[k]), (c.name, k)
         # the served path is the single-handle path: same bits as a Solver
         # created from the same directory and stepped directly
         c, d = MIX[1], alone_dirs[1]
