@@ -54,29 +54,32 @@ def traffic_from_profile(kernel: str, config: str):
     return None, None
 
 
-def c2_spec(nend: int = 2) -> C.CaseSpec:
+def c2_spec(nend: int = 2, layout: str = "tanh") -> C.CaseSpec:
     s = C.Settings(re=3900.0, pr=0.71, iles=1, insmdl=1, itemdl=0, idvmon=1, filter=3, idtopt=1,
                    cflfac=0.5, dt_size=0.005, resid1=1.0e-6, nlev=5, nbli=3, mgitr=500, imgsor=170,
                    nend=nend)
-    return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s)
+    return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s, layout=layout)
 
 
-def c1_spec(nend: int = 2) -> C.CaseSpec:
+def c1_spec(nend: int = 2, layout: str = "tanh") -> C.CaseSpec:
     """The plain FP64 case: Re 100, 192x128x16, fixed dt 0.005, no SGS model."""
     s = C.Settings(re=100.0, pr=0.71, iles=0, idtopt=0, dt_size=0.005, resid1=1.0e-6, nlev=5, nbli=3,
                    mgitr=500, nend=nend)
-    return C.CaseSpec(192, 128, 16, dmin=0.01, noise=1.0e-3, settings=s)
+    return C.CaseSpec(192, 128, 16, dmin=0.01, noise=1.0e-3, settings=s, layout=layout)
 
 
 WORKLOAD_TAGS = {"c2": "c2_384x256x64", "c1": "c1_192x128x16"}
 
 WORKLOADS = {
     "c2": "heated circular cylinder LES Re=3900 Pr=0.71, 384x256x64 on [-15,30]x[-15,15]x[0,pi], dynamic "
-          "Vreman + dynamic eddy diffusivity, box test filter x/z, CFL 0.5, NLEV 5; x and y lines clustered "
-          "by a two-sided sinh at the body centre (spacing 0.0155 x, 0.021 y at the surface)",
+          "Vreman + dynamic eddy diffusivity, box test filter x/z, CFL 0.5, NLEV 5; x and y lines stretched by "
+          "hyperbolic-tangent segments, spacing 0.01D at the cylinder surface",
     "c1": "heated circular cylinder Re=100 Pr=0.71, 192x128x16 on [-15,30]x[-15,15]x[0,pi], no SGS model, "
-          "dt 0.005, NLEV 5; sinh-clustered lines (spacing 0.030 x, 0.046 y at the surface)",
+          "dt 0.005, NLEV 5; x and y lines stretched by hyperbolic-tangent segments, spacing 0.01D at the "
+          "cylinder surface",
 }
+GRID_SINH = ("; the earlier layout instead: x and y lines clustered by a two-sided sinh at the body centre "
+             "(spacing 0.01 there, 0.0155-0.046 at the surface)")
 
 
 SERIAL_REF_FILES = "slv_mmtm.f90, slv_engy.f90 and slv_auxi.f90"
@@ -379,6 +382,9 @@ def main():
     ap.add_argument("--config", choices=("c2", "c1"), default="c2",
                     help="c2: the 384x256x64 LES the metric is quoted on; c1: the plain 192x128x16 case")
     ap.add_argument("--nlev", type=int, default=0, help="multigrid levels (0: the workload's own)")
+    ap.add_argument("--grid", choices=("tanh", "sinh"), default="tanh",
+                    help="tanh: the configurations' lines (0.01D at the body surface); sinh: the earlier "
+                         "layout, clustered at the body centre")
     ap.add_argument("--mixed", action="store_true",
                     help="store the multigrid level planes below the top level in FP32 (arithmetic, the top "
                          "level and the convergence test stay FP64); not the default, and not the metric's form")
@@ -428,7 +434,7 @@ def main():
               "(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
 
-    spec = c1_spec() if args.config == "c1" else c2_spec()
+    spec = c1_spec(layout=args.grid) if args.config == "c1" else c2_spec(layout=args.grid)
     if args.nlev:
         spec = dataclasses.replace(spec, settings=dataclasses.replace(spec.settings, nlev=args.nlev))
     t0 = time.time()
@@ -546,7 +552,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32 multigrid levels" if args.mixed else "f64",
         "data": "synthetic (seeded initial field, generated grid and IB tables)",
-        "config": {"workload": WORKLOADS[args.config],
+        "config": {"workload": WORKLOADS[args.config] + (GRID_SINH if args.grid == "sinh" else ""),
                    "cells": n, "nlev": spec.settings.nlev, "resid1": spec.settings.resid1,
                    "mg_cycles_max_last_step": st["mg_cycles_max"], "dt_last": st["dt"],
                    "case_build_s": round(t_build, 2), "mg_tiles": tune,

@@ -117,6 +117,72 @@ def gridlines_sinh(ncell: int, lo: float, hi: float, dmin: float) -> np.ndarray:
     return xs
 
 
+def _bisect(f, lo, hi, iters=200):
+    """Root of a function decreasing on [lo, hi] (f(lo) > 0 > f(hi))."""
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def _tanh_expand(n: int, length: float, d1: float) -> np.ndarray:
+    """n + 1 points from 0 to ``length`` with the hyperbolic-tangent
+    expansion of the pre-processor's 'H' segments (lib_gridfunc.py:64-69 with
+    Factor1 = 1): x(p) = L (1 - tanh(g (1 - p)) / tanh g), p = i / n, fine at
+    0; g is chosen so that the first spacing is d1."""
+    p = np.arange(n + 1, dtype=np.float64) / n
+    if length / n <= d1:
+        return length * p
+    first = lambda g: length * (1.0 - math.tanh(g * (1.0 - 1.0 / n)) / math.tanh(g)) - d1
+    g = _bisect(first, 1e-6, 30.0)
+    x = length * (1.0 - np.tanh(g * (1.0 - p)) / math.tanh(g))
+    x[0], x[-1] = 0.0, length
+    return x
+
+
+def _tanh_sym(n: int, length: float, d1: float) -> np.ndarray:
+    """n + 1 points from 0 to ``length``, fine at both ends: the 'H' segment
+    with Factor1 = 0.5 (lib_gridfunc.py:64-69), x(p) = L/2 (1 - tanh(g (1/2 -
+    p)) / tanh(g/2)); g chosen so that the end spacings are d1."""
+    p = np.arange(n + 1, dtype=np.float64) / n
+    if length / n <= d1:
+        return length * p
+    first = lambda g: 0.5 * length * (1.0 - math.tanh(g * (0.5 - 1.0 / n)) / math.tanh(0.5 * g)) - d1
+    g = _bisect(first, 1e-6, 60.0)
+    x = 0.5 * length * (1.0 - np.tanh(g * (0.5 - p)) / math.tanh(0.5 * g))
+    x[0], x[-1] = 0.0, length
+    return x
+
+
+def gridlines_tanh(ncell: int, lo: float, hi: float, dmin: float, half: float = 0.5,
+                   ncore: int = 0) -> np.ndarray:
+    """Lines clustered at the body surface x = -half and x = +half with
+    spacing ``dmin`` there: hyperbolic-tangent segments as the pre-processor
+    writes them (grid.input 'H'), an expansion from +half to hi, its mirror
+    image from -half down to lo, and a symmetric segment across the body
+    (coarsest at its centre). ``ncore`` cells across the body (default
+    ncell / 6, even); the rest split between the two sides in proportion to
+    the square root of their lengths."""
+    if not (lo < -half < half < hi):
+        raise ValueError("tanh clustering needs lo < -half < half < hi")
+    nc = ncore or max(2, 2 * int(round(ncell / 12.0)))
+    rest = ncell - nc
+    wl, wr = math.sqrt(-half - lo), math.sqrt(hi - half)
+    nl = int(round(rest * wl / (wl + wr)))
+    nr = rest - nl
+    if min(nl, nr, nc) < 2:
+        raise ValueError("too few cells for the tanh layout")
+    right = half + _tanh_expand(nr, hi - half, dmin)
+    left = -half - _tanh_expand(nl, -half - lo, dmin)[::-1]
+    core = -half + _tanh_sym(nc, 2.0 * half, dmin)
+    x = np.concatenate([left[:-1], core[:-1], right])
+    x[0], x[nl], x[nl + nc], x[-1] = lo, -half, half, hi
+    return x
+
+
 def _fmt15(values: np.ndarray) -> np.ndarray:
     """Round-trip through the ``%23.15f`` text that grid.dat holds."""
     return np.array([float("%23.15f" % v) for v in values], dtype=np.float64)
@@ -756,7 +822,9 @@ class CaseSpec:
     seed: int = 20261203
     settings: Settings = dataclasses.field(default_factory=Settings)
     # "body": sinh-clustered x and y lines around the body, inflow/outflow in
-    # x; "channel": uniform lines, x and z periodic, the y ends the settings'
+    # x; "tanh": x and y lines clustered at the body surface (|x|, |y| = 0.5)
+    # with spacing dmin there, hyperbolic-tangent segments (gridlines_tanh);
+    # "channel": uniform lines, x and z periodic, the y ends the settings'
     # boundaries (walls with BC_Y = 0), the body (if it reaches the domain)
     # immersed as usual
     layout: str = "body"
@@ -779,6 +847,9 @@ def build_case(spec: CaseSpec) -> Case:
     if channel:
         xl = gridlines_uniform(spec.n1m, spec.xlim[0], spec.xlim[1])
         yl = gridlines_uniform(spec.n2m, spec.ylim[0], spec.ylim[1])
+    elif spec.layout == "tanh":
+        xl = gridlines_tanh(spec.n1m, spec.xlim[0], spec.xlim[1], spec.dmin)
+        yl = gridlines_tanh(spec.n2m, spec.ylim[0], spec.ylim[1], spec.dmin)
     else:
         xl = gridlines_sinh(spec.n1m, spec.xlim[0], spec.xlim[1], spec.dmin)
         yl = gridlines_sinh(spec.n2m, spec.ylim[0], spec.ylim[1], spec.dmin)

@@ -80,11 +80,12 @@ def spec_for(dims, mode, nend=1, nlev=None):
 
 def c2_spec(nend=1):
     """The 384x256x64 heated-cylinder LES: Re 3900, dynamic Vreman and eddy
-    diffusivity, box filter in x and z, CFL 0.5, NLEV 5, noise 1e-2."""
+    diffusivity, box filter in x and z, CFL 0.5, NLEV 5, noise 1e-2; x and y
+    lines clustered at the cylinder surface with spacing 0.01 there."""
     s = C.Settings(re=3900.0, pr=0.71, iles=1, insmdl=1, itemdl=0, idvmon=1, filter=3, idtopt=1,
                    cflfac=0.5, dt_size=0.005, resid1=1.0e-6, nlev=5, nbli=3, mgitr=500, imgsor=170,
                    nend=nend)
-    return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s)
+    return C.CaseSpec(384, 256, 64, dmin=0.01, noise=1.0e-2, settings=s, layout="tanh")
 
 
 def golden_view(meta):

@@ -4,6 +4,7 @@ import os
 import tempfile
 
 import numpy as np
+import pytest
 
 from leswht_amd import case as C
 from tests.helpers import spec_for
@@ -125,3 +126,29 @@ def test_fld_writer_reader_round_trip():
     # record markers are 4-byte little endian lengths
     raw = open(path, "rb").read()
     assert int.from_bytes(raw[:4], "little") == 48
+
+
+@pytest.mark.parametrize("n,lo,hi", [(384, -15.0, 30.0), (256, -15.0, 15.0), (192, -15.0, 30.0), (128, -15.0, 15.0)])
+def test_tanh_lines_are_clustered_at_the_surface(n, lo, hi):
+    """The configurations' stretching: spacing dmin on both sides of the
+    body surface x = -0.5 and x = 0.5, lines exactly at the ends and at the
+    surface, strictly increasing, neighbouring spacings within 40 % (a
+    plausible slip, such as a segment's points in the wrong order or a
+    spacing matched on the wrong side, breaks one of these)."""
+    x = C.gridlines_tanh(n, lo, hi, 0.01)
+    assert len(x) == n + 1 and x[0] == lo and x[-1] == hi
+    d = np.diff(x)
+    assert (d > 0).all()
+    for a in (-0.5, 0.5):
+        i = int(np.flatnonzero(x == a)[0])
+        assert abs(d[i - 1] - 0.01) < 1e-12 and abs(d[i] - 0.01) < 1e-12
+    assert (d[1:] / d[:-1]).max() < 1.4 and (d[:-1] / d[1:]).max() < 1.4
+    # the expansion segment follows the pre-processor's formula: with its
+    # own g it is reproduced point for point
+    i5 = int(np.flatnonzero(x == 0.5)[0])
+    seg = x[i5:] - 0.5
+    m, L = len(seg) - 1, hi - 0.5
+    g = C._bisect(lambda g: L * (1.0 - np.tanh(g * (1.0 - 1.0 / m)) / np.tanh(g)) - 0.01, 1e-6, 30.0)
+    p = np.arange(m + 1) / m
+    ref = L * (1.0 - np.tanh(g * (1.0 - p)) / np.tanh(g))
+    assert np.abs(seg[1:-1] - ref[1:-1]).max() < 1e-12
