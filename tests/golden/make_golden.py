@@ -36,6 +36,9 @@ CASES = {
     "cyl32_const_3": dict(dims=(32, 32, 8), mode="les_const", nend=3),
     # the configurations' domain and near-body spacing, five multigrid levels
     "cyl192_dom_nlev5_3": dict(dims=(192, 64, 16), mode="dom_les", nend=3, nlev=5),
+    # the divergence floor after projection: RESID1 = 1e-11 (its fhist rows
+    # hold the reference's own max |div u - qmass| per step)
+    "cyl64_div_2": dict(dims=(64, 64, 16), mode="les_div", nend=2, nlev=3),
     # 100 steps of the dynamic LES
     "cyl32_les_100": dict(dims=(32, 32, 8), mode="les", nend=100),
     # makefld (IREAD = 0) and the tracer every 2 steps

@@ -62,6 +62,10 @@ def spec_for(dims, mode, nend=1, nlev=None):
         s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.0, csgshf=0.0, resid1=1e-8, nlev=nlev or 2)
     elif mode == "les":
         s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-8, nlev=nlev or 2, re=3900.0)
+    elif mode == "les_div":
+        # the divergence floor: the dynamic LES with the Poisson tolerance
+        # RESID1 = 1e-11
+        s = C.Settings(nend=nend, iles=1, idvmon=1, resid1=1e-11, nlev=nlev or 3, re=3900.0)
     elif mode == "les_const":
         s = C.Settings(nend=nend, iles=1, idvmon=0, csgsts=0.025, csgshf=0.003, resid1=1e-8, nlev=nlev or 2,
                        re=3900.0)

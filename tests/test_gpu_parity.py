@@ -483,19 +483,21 @@ def test_gpu_100_steps_match_oracle(oracle_lib):
         assert err <= 1e-6, (k, err)
 
 
-def test_gpu_post_projection_divergence(oracle_lib):
+def test_gpu_post_projection_divergence():
     """With RESID1 = 1e-11 the projected field's max |div u - qmass| reaches
-    the floor the reference algorithm itself reaches on this grid (the oracle
-    at the same settings, about 1.6e-11), within a factor 2."""
-    import dataclasses
-    import oracle
-    sp = spec_for((64, 64, 16), "les", 1, 3)
-    sp = dataclasses.replace(sp, settings=dataclasses.replace(sp.settings, resid1=1e-11))
-    view = C.case_view(C.build_case(sp))
+    the floor the reference solver itself reaches on this case: its own
+    fhist.dat rows (golden cyl64_div_2, 64x64x16 LES, NLEV 3: 1.6e-11 and
+    2.4e-11 after steps 1 and 2), within a factor 2, every step. The north
+    star's 1e-12 lies below what the reference reaches at any RESID1 here
+    (at 1e-13 kv = 1 runs into MGITR)."""
+    z, meta = golden("cyl64_div_2")
+    view = golden_view(meta)
     print("divergence", flush=True)
-    do = oracle.Oracle(view).step(1)
-    dg = Solver.from_case(view).step(1)
-    assert dg["dvmax"] <= 2.0 * do.dvmax + 1e-14, (dg["dvmax"], do.dvmax)
+    g = Solver.from_case(view)
+    for n in range(meta["nend"]):
+        dg = g.step(1)
+        ref = float(z["hist"][n][3])
+        assert dg["dvmax"] <= 2.0 * ref, (n + 1, dg["dvmax"], ref)
     assert dg["dvmax"] <= 5e-11
 
 
