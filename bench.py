@@ -386,8 +386,9 @@ def main():
                     help="tanh: the configurations' lines (0.01D at the body surface); sinh: the earlier "
                          "layout, clustered at the body centre")
     ap.add_argument("--mixed", action="store_true",
-                    help="store the multigrid level planes below the top level in FP32 (arithmetic, the top "
-                         "level and the convergence test stay FP64); not the default, and not the metric's form")
+                    help="the mixed-precision form: the multigrid level planes below the top level stored in FP32 "
+                         "(arithmetic, the top level and the convergence test stay FP64) and the dynamic SGS "
+                         "models' test filter in FP32; not the default, and not the metric's form")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the multigrid tile sizes with the library's autotuner before the warmup")
     ap.add_argument("--batch", type=int, default=0,
@@ -444,6 +445,7 @@ def main():
     sol = Solver.from_case(view)
     if args.mixed:
         sol.set_option("mg_fp32", 1)
+        sol.set_option("sgs_fp32", 1)
     tune = sol.autotune(reps=2) if args.autotune else None
     # untimed warmup; its last step times every kernel once to find the one
     # with the most device time
@@ -550,7 +552,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32 multigrid levels" if args.mixed else "f64",
+        "dtype": "f64+f32 multigrid levels and test filter" if args.mixed else "f64",
         "data": "synthetic (seeded initial field, generated grid and IB tables)",
         "config": {"workload": WORKLOADS[args.config] + (GRID_SINH if args.grid == "sinh" else ""),
                    "cells": n, "nlev": spec.settings.nlev, "resid1": spec.settings.resid1,

@@ -350,40 +350,47 @@ __device__ inline bool sgs_tile(const Geo &g, long t, int fx, SgsTile &T) {
 }
 
 // the x filter, in place, of NQ quantities of the lanes' cells (every lane
-// of the wavefront calls it)
-template <int NQ>
-__device__ inline void sgs_xfilter(const SgsTile &T, double (&q)[NQ]) {
+// of the wavefront calls it); FT is the filter's arithmetic (double, or float
+// in the mixed-precision form)
+template <int NQ, typename FT>
+__device__ inline void sgs_xfilter(const SgsTile &T, FT (&q)[NQ]) {
+  const FT c0 = (FT)T.cx0, c1 = (FT)T.cx1, c2 = (FT)T.cx2;
 #pragma unroll
   for (int l = 0; l < NQ; ++l) {
-    const double am = __shfl_up(q[l], 1, 64), ap = __shfl_down(q[l], 1, 64);
-    q[l] = T.xi ? T.cx0 * am + T.cx1 * q[l] + T.cx2 * ap : q[l];
+    const FT am = __shfl_up(q[l], 1, 64), ap = __shfl_down(q[l], 1, 64);
+    q[l] = T.xi ? c0 * am + c1 * q[l] + c2 * ap : q[l];
   }
 }
 
 // Vreman quantities of a cell: alpha_ij (9), nu S_ij (6), u_i (3), u_i u_j
 // (6) (sgs.f90:434-470, 538-566); returns nu
-__device__ inline double dvmg_cell(const Geo &g, const Fld &f, int i, int j, int k, double (&q)[24]) {
+template <typename FT>
+__device__ inline double dvmg_cell(const Geo &g, const Fld &f, int i, int j, int k, FT (&q)[24]) {
   double alp[9], str[6];
   velgrad(g, f, i, j, k, alp, str);
   const double r = vreman(alp, g.f2fx[i] * g.f2fx[i], g.f2fy[j] * g.f2fy[j], g.f2fz[k] * g.f2fz[k]);
 #pragma unroll
-  for (int l = 0; l < 9; ++l) q[l] = alp[l];
+  for (int l = 0; l < 9; ++l) q[l] = (FT)alp[l];
 #pragma unroll
-  for (int l = 0; l < 6; ++l) q[9 + l] = r * str[l];
+  for (int l = 0; l < 6; ++l) q[9 + l] = (FT)(r * str[l]);
   const long c = P(g, i, j, k);
   const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
   const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
   const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
-  q[15] = u1; q[16] = u2; q[17] = u3;
-  q[18] = u1 * u1; q[19] = u1 * u2; q[20] = u1 * u3;
-  q[21] = u2 * u2; q[22] = u2 * u3; q[23] = u3 * u3;
+  q[15] = (FT)u1; q[16] = (FT)u2; q[17] = (FT)u3;
+  q[18] = (FT)(u1 * u1); q[19] = (FT)(u1 * u2); q[20] = (FT)(u1 * u3);
+  q[21] = (FT)(u2 * u2); q[22] = (FT)(u2 * u3); q[23] = (FT)(u3 * u3);
   return r;
 }
 
 // the Germano products of a cell from its filtered quantities (k_dvmg2's M,
 // k_dvmg3's L and sums)
-__device__ inline void dvmg_germano(const Geo &g, int i, int j, int k, const double (&F)[24], const int64_t *nwall,
+template <typename FT>
+__device__ inline void dvmg_germano(const Geo &g, int i, int j, int k, const FT (&Ff)[24], const int64_t *nwall,
                                     double &lm, double &mm) {
+  double F[24];
+#pragma unroll
+  for (int l = 0; l < 24; ++l) F[l] = (double)Ff[l];
   double alp[9], str[6];
 #pragma unroll
   for (int l = 0; l < 9; ++l) alp[l] = F[l];
@@ -411,7 +418,8 @@ __device__ inline void dvmg_germano(const Geo &g, int i, int j, int k, const dou
 // eddy-diffusivity quantities of a cell: S_ij (6), dT/dx_i (3), alpha
 // dT/dx_i (3), T, u_i (3), u_i T (3) (sgs.f90:1325-1345, 1395-1430);
 // returns alpha
-__device__ inline double edm_cell(const Geo &g, const Fld &f, int i, int j, int k, double (&q)[19]) {
+template <typename FT>
+__device__ inline double edm_cell(const Geo &g, const Fld &f, int i, int j, int k, FT (&q)[19]) {
   double d[9], s[6], alp[3];
   velgrad(g, f, i, j, k, d, s);
   temgrad(g, f, i, j, k, alp);
@@ -420,25 +428,29 @@ __device__ inline double edm_cell(const Geo &g, const Fld &f, int i, int j, int 
   sss = fmax(sss, 1.0e-16);
   const double al = (del * del) * sqrt(2. * sss);
 #pragma unroll
-  for (int l = 0; l < 6; ++l) q[l] = s[l];
+  for (int l = 0; l < 6; ++l) q[l] = (FT)s[l];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
-    q[6 + l] = alp[l];
-    q[9 + l] = al * alp[l];
+    q[6 + l] = (FT)alp[l];
+    q[9 + l] = (FT)(al * alp[l]);
   }
   const long c = P(g, i, j, k);
   const double u1 = 0.5 * (f.u[c] + f.u[c + 1]);
   const double u2 = 0.5 * (f.v[c] + f.v[c + g.NX]);
   const double u3 = 0.5 * (f.w[c] + f.w[c + g.NX * g.NY]);
   const double tc = f.t[c];
-  q[12] = tc;
-  q[13] = u1; q[14] = u2; q[15] = u3;
-  q[16] = u1 * tc; q[17] = u2 * tc; q[18] = u3 * tc;
+  q[12] = (FT)tc;
+  q[13] = (FT)u1; q[14] = (FT)u2; q[15] = (FT)u3;
+  q[16] = (FT)(u1 * tc); q[17] = (FT)(u2 * tc); q[18] = (FT)(u3 * tc);
   return al;
 }
 
-__device__ inline void edm_germano(const Geo &g, int i, int j, int k, const double (&F)[19], const int64_t *nwall,
+template <typename FT>
+__device__ inline void edm_germano(const Geo &g, int i, int j, int k, const FT (&Ff)[19], const int64_t *nwall,
                                    double &lm, double &mm) {
+  double F[19];
+#pragma unroll
+  for (int l = 0; l < 19; ++l) F[l] = (double)Ff[l];
   const double d1 = fwidth(g.f2fx[i - 1], g.f2fx[i], g.f2fx[i + 1], g.fixiu[i], g.fixil[i]);
   const double d2 = fwidth(g.f2fy[j - 1], g.f2fy[j], g.f2fy[j + 1], g.fixju[j], g.fixjl[j]);
   const double d3 = fwidth(g.f2fz[k - 1], g.f2fz[k], g.f2fz[k + 1], g.fixku[k], g.fixkl[k]);
@@ -458,8 +470,14 @@ __device__ inline void edm_germano(const Geo &g, int i, int j, int k, const doub
 // MODEL 0: dynamic Vreman (writes nu), 1: dynamic eddy diffusivity (writes
 // alpha). Workgroups of 4 wavefronts walk the tiles; each workgroup's two
 // sums go to f.partial[2 b], f.partial[2 b + 1].
-template <int MODEL, int NQ>
-LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_t *nwall, int fx, int fz) {
+// FT: the test filter's arithmetic, double (the reference's) or float (the
+// mixed-precision form: the filtered quantities feed only the Germano sums,
+// which stay FP64, and the model coefficient they give). The FP32 Vreman
+// instance asks for two waves per SIMD (its float arrays leave room; 29
+// spilled registers, 0.82 against 1.25 ms at 384x256x64); the eddy-
+// diffusivity one keeps one (two spilled 122 registers and ran 1.7 ms).
+template <int MODEL, int NQ, typename FT>
+LW_KERNEL void k_sgs_dyn(LW_KP(256, (sizeof(FT) == 4 && MODEL == 0 ? 2 : 0)), const Geo &g, const Fld &f, const int64_t *nwall, int fx, int fz) {
   __shared__ double sh[4];
   double lm = 0.0, mm = 0.0;
   const int w = threadIdx.x >> 6;
@@ -468,10 +486,10 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_
     const int n3m = g.n3m;
     // z sums: A of the plane awaiting its last term, B of the plane awaiting
     // its middle one
-    double A[NQ], B[NQ];
-    double x[NQ];
+    FT A[NQ], B[NQ];
+    FT x[NQ];
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) A[l] = B[l] = 0.0;
+    for (int l = 0; l < NQ; ++l) A[l] = B[l] = (FT)0.0;
     const int kb = fz ? T.k0 - 1 : T.k0, ke = fz ? T.k1 + 1 : T.k1;
     for (int kk = kb; kk <= ke; ++kk) {
       const int k = kk < 1 ? n3m : (kk > n3m ? 1 : kk);
@@ -490,8 +508,9 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_
       // the plane below (kk - 1) is complete with this plane's term
       if (kk >= T.k0 + 1) {
         const int km = kk - 1;
+        const FT cz = (FT)g.cfz1[3 * km + 2];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) A[l] = A[l] + g.cfz1[3 * km + 2] * x[l];
+        for (int l = 0; l < NQ; ++l) A[l] = A[l] + cz * x[l];
         if (T.own) {
           if constexpr (MODEL == 0) dvmg_germano(g, T.i, T.j, km, A, nwall, lm, mm);
           else edm_germano(g, T.i, T.j, km, A, nwall, lm, mm);
@@ -499,14 +518,16 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_
       }
       // this plane's middle term; it then awaits its last
       if (kk >= T.k0 && kk <= T.k1) {
+        const FT cz = (FT)g.cfz1[3 * k + 1];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) A[l] = B[l] + g.cfz1[3 * k + 1] * x[l];
+        for (int l = 0; l < NQ; ++l) A[l] = B[l] + cz * x[l];
       }
       // the first term of the plane above
       if (kk + 1 <= T.k1) {
         const int kp = kk + 1;
+        const FT cz = (FT)g.cfz1[3 * kp];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) B[l] = g.cfz1[3 * kp] * x[l];
+        for (int l = 0; l < NQ; ++l) B[l] = cz * x[l];
       }
     }
   }
@@ -525,34 +546,39 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, 0), const Geo &g, const Fld &f, const int64_
   if (threadIdx.x == 0) f.partial[2 * blockIdx.x + 1] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// read at every handle's creation, so that cases of one process may differ
 bool sgs_fused_enabled() {
-  static const bool on = [] {
-    const char *e = getenv("LW_SGS_FUSED");
-    return !(e && atoi(e) == 0);
-  }();
-  return on;
+  const char *e = getenv("LW_SGS_FUSED");
+  return !(e && atoi(e) == 0);
 }
 
 static void launch_sgs_dyn(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int model,
-                           double scale, int slot) {
+                           double scale, int slot, int fp32) {
   const int fx = (filter == 1 || filter == 3) ? 1 : 0;
   const int fz = (filter == 2 || filter == 3) ? 1 : 0;
   const double cells = pts(g);
-  if (model == 0)
-    // u v w read (with their x and z neighbour planes), nu and nwall: about 6 arrays
-    LW_LAUNCH("k_sgs_dvmg", cells * 8.0 * 6, 0, (k_sgs_dyn<0, 24>), dim3(kRedBlocks), dim3(256), 0, s, g, f, nwall,
-              fx, fz);
+  // u v w read (with their x and z neighbour planes), nu and nwall: about 6
+  // arrays (7 with T)
+  if (model == 0 && !fp32)
+    LW_LAUNCH("k_sgs_dvmg", cells * 8.0 * 6, 0, (k_sgs_dyn<0, 24, double>), dim3(kRedBlocks), dim3(256), 0, s, g, f,
+              nwall, fx, fz);
+  else if (model == 0)
+    LW_LAUNCH("k_sgs_dvmg", cells * 8.0 * 6, 0, (k_sgs_dyn<0, 24, float>), dim3(kRedBlocks), dim3(256), 0, s, g, f,
+              nwall, fx, fz);
+  else if (!fp32)
+    LW_LAUNCH("k_sgs_edm", cells * 8.0 * 7, 0, (k_sgs_dyn<1, 19, double>), dim3(kRedBlocks), dim3(256), 0, s, g, f,
+              nwall, fx, fz);
   else
-    LW_LAUNCH("k_sgs_edm", cells * 8.0 * 7, 0, (k_sgs_dyn<1, 19>), dim3(kRedBlocks), dim3(256), 0, s, g, f, nwall,
-              fx, fz);
+    LW_LAUNCH("k_sgs_edm", cells * 8.0 * 7, 0, (k_sgs_dyn<1, 19, float>), dim3(kRedBlocks), dim3(256), 0, s, g, f,
+              nwall, fx, fz);
   LW_LAUNCH("k_coef", kRedBlocks * 16.0, 0, k_coef, dim3(1), dim3(kRedThreads), 0, s, f, kRedBlocks, scale, slot);
 }
 
-void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter) {
-  launch_sgs_dyn(s, g, f, nwall, filter, 0, -0.5, 4);
+void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int fp32) {
+  launch_sgs_dyn(s, g, f, nwall, filter, 0, -0.5, 4, fp32);
 }
-void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter) {
-  launch_sgs_dyn(s, g, f, nwall, filter, 1, -1.0, 5);
+void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int fp32) {
+  launch_sgs_dyn(s, g, f, nwall, filter, 1, -1.0, 5, fp32);
 }
 
 // red slots: 4 = csgsts, 5 = csgshf

@@ -121,8 +121,9 @@ void launch_sgs_cvm(hipStream_t s, const Geo &g, const Fld &f, double csgsts);
 // coefficient into red[4] (Vreman) or red[5] (eddy diffusivity).
 // LW_SGS_FUSED=0 selects the separate kernels and filter launches instead.
 bool sgs_fused_enabled();
-void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter);
-void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter);
+// fp32: the test filter's arithmetic in FP32 (the mixed-precision form)
+void launch_sgs_dvmg_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int fp32);
+void launch_sgs_edm_fused(hipStream_t s, const Geo &g, const Fld &f, const int64_t *nwall, int filter, int fp32);
 void launch_sgs_edm(hipStream_t s, const Geo &g, const Fld &f, SgsBuf &b, int filter);
 void launch_sgs_cedm(hipStream_t s, const Geo &g, const Fld &f, double csgshf);
 void launch_sgs_finish(hipStream_t s, const Geo &g, const Fld &f, double *nu, double **nu1,

@@ -213,6 +213,7 @@ struct lw_handle {
   long long kv1_tlim = 0;
   int mg_split = 1;                        // kv = 1 on its own stream (LW_MG_SPLIT=0: one stream)
   int sgs_fused = 1;                       // dynamic SGS in one pass (LW_SGS_FUSED=0: separate kernels)
+  int sgs_fp32 = 0;                        // the fused models' test filter in FP32 (option "sgs_fp32")
   DiagPlanes dplanes{};
   double *hist = nullptr;  // device: the step's history values and their running averages
   double h_hist[32] = {0};
@@ -1356,6 +1357,9 @@ static int create_handle(const lw_settings *s, const lw_grid *gr, const lw_table
     // allows it (the option "mg_fp32" insists and reports when it does not)
     if (const char *e = getenv("LW_MG_FP32"))
       if (atoi(e) != 0 && h->st.poisson == 0 && mg_mixed_supported(h->g, h->mg)) set_mg_fp32(h, true);
+    // LW_SGS_FP32=1: the FP32 test filter wherever the dynamic models run fused
+    if (const char *e = getenv("LW_SGS_FP32"))
+      if (atoi(e) != 0 && h->sgs_fused) h->sgs_fp32 = 1;
     HIPCHK(hipStreamSynchronize(h->s));
     *out = h;
     return LW_OK;
@@ -1688,7 +1692,7 @@ static void phase_sgs(lw_handle *h) {
   if (zero_first) st_zero(h, f.nu, np * sizeof(double));
   else if (st.idvmon == 1) {
     if (h->sgs_fused) {
-      launch_sgs_dvmg_fused(h->s, g, f, h->nwall, (int)st.filter);
+      launch_sgs_dvmg_fused(h->s, g, f, h->nwall, (int)st.filter, h->sgs_fp32);
     } else {
       launch_sgs_dvmg(h->s, g, f, h->sb, (int)st.filter);
       launch_sgs_dvmg_coef(h->s, g, f, h->sb, h->nwall);
@@ -1704,7 +1708,7 @@ static void phase_sgs(lw_handle *h) {
   if (zero_first) st_zero(h, f.al, np * sizeof(double));
   else if (st.idvmon == 1) {
     if (h->sgs_fused) {
-      launch_sgs_edm_fused(h->s, g, f, h->nwall, (int)st.filter);
+      launch_sgs_edm_fused(h->s, g, f, h->nwall, (int)st.filter, h->sgs_fp32);
     } else {
       launch_sgs_edm(h->s, g, f, h->sb, (int)st.filter);
       launch_sgs_edm_coef(h->s, g, f, h->sb, h->nwall);
@@ -2412,6 +2416,12 @@ extern "C" int lw_set_option(lw_handle *h, const char *name, int64_t value) {
     } catch (int code) {
       return code;
     }
+    return LW_OK;
+  }
+  if (n == "sgs_fp32") {
+    if (value != 0 && !h->sgs_fused)
+      return fail(LW_ERR_ARG, "sgs_fp32: needs the one-pass dynamic SGS kernels (LW_SGS_FUSED=0 keeps the separate ones)");
+    h->sgs_fp32 = value != 0;
     return LW_OK;
   }
   return fail(LW_ERR_ARG, "unknown option " + n);
