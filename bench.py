@@ -290,14 +290,16 @@ def batch_bench(args) -> dict:
                 pass
     cells = [c.cells for c in cases]
     k, w = args.steps, args.warmup
-    # plain form: one case at a time
-    t_plain = 0.0
-    for sv in sols:
-        sv.step(w)
-        t0 = time.perf_counter()
-        sv.step(k)
-        t_plain += time.perf_counter() - t0
-    print("one at a time: %.3f s" % t_plain, file=sys.stderr, flush=True)
+    t_plain = t_conc = None
+    if not args.batch_only:
+        # plain form: one case at a time
+        t_plain = 0.0
+        for sv in sols:
+            sv.step(w)
+            t0 = time.perf_counter()
+            sv.step(k)
+            t_plain += time.perf_counter() - t0
+        print("one at a time: %.3f s" % t_plain, file=sys.stderr, flush=True)
 
     # concurrent form: every case on its own streams from its own thread
     def run(sv, n):
@@ -312,9 +314,10 @@ def batch_bench(args) -> dict:
             t.join()
         return time.perf_counter() - t0
 
-    concurrent(w)
-    t_conc = concurrent(k)
-    print("concurrent streams: %.3f s" % t_conc, file=sys.stderr, flush=True)
+    if not args.batch_only:
+        concurrent(w)
+        t_conc = concurrent(k)
+        print("concurrent streams: %.3f s" % t_conc, file=sys.stderr, flush=True)
     # batched form
     b = Batch(sols)
     b.step(w)
@@ -342,10 +345,10 @@ def batch_bench(args) -> dict:
         "config": {"workload": "%d independent heated-body cases drawn as config 4 (cells 64..256 x 64..192 x "
                                "8..32, Re 40..400, Pr 0.71/7, circle/square/ellipse, CFL 0.5)" % len(cases),
                    "cases": len(cases), "cells_total": int(sum(cells)), "seed": args.seed, "prep_s": round(t_prep, 1)},
-        "one_at_a_time": upd / t_plain,
-        "concurrent_streams": upd / t_conc,
-        "ratio_vs_one_at_a_time": t_plain / t_bat,
-        "ratio_vs_concurrent_streams": t_conc / t_bat,
+        "one_at_a_time": upd / t_plain if t_plain else None,
+        "concurrent_streams": upd / t_conc if t_conc else None,
+        "ratio_vs_one_at_a_time": t_plain / t_bat if t_plain else None,
+        "ratio_vs_concurrent_streams": t_conc / t_bat if t_conc else None,
         "launch_sharing": {"batched_launches": info["launches"], "case_launches": info["case_launches"],
                            "rendezvous": info["rendezvous"],
                            "cases_per_launch": info["case_launches"] / max(info["launches"], 1)},
@@ -393,6 +396,9 @@ def main():
                     help="pick the multigrid tile sizes with the library's autotuner before the warmup")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batched form on this many independent small cases instead of the time step")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched form: time the batch alone, without the one-at-a-time and concurrent forms "
+                         "(for profiling it)")
     ap.add_argument("--batch-leg", type=int, default=32,
                     help="cases of the short batched-form comparison reported beside the step (0: none)")
     ap.add_argument("--batch-cells", type=int, default=2000000,
@@ -574,7 +580,7 @@ def main():
     if args.batch_leg > 0:
         try:
             leg = argparse.Namespace(**vars(args))
-            leg.batch, leg.steps, leg.warmup, leg.mixed = args.batch_leg, 5, 2, False
+            leg.batch, leg.steps, leg.warmup, leg.mixed, leg.batch_only = args.batch_leg, 5, 2, False, False
             bb = batch_bench(leg)
             out["batch"] = {q: bb[q] for q in ("value", "unit", "steps", "one_at_a_time", "concurrent_streams",
                                                "ratio_vs_one_at_a_time", "ratio_vs_concurrent_streams",

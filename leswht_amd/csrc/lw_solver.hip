@@ -2063,7 +2063,12 @@ static void phase_poisson(lw_handle *h) {
       st_sync(h);
       all_done = true;
       for (int q = 0; q < nmg; ++q) all_done = all_done && h->h_flags[q] != 0;
-      batch = 2;
+      // further batches of 2 cycles; in a batch, where every read of the
+      // flags is a rendezvous of all its cases, batches that grow with the
+      // cycles done (half of them), so a long solve needs a few rendezvous
+      // instead of one per 2 cycles (surplus cycles of a converged
+      // wavenumber leave it untouched)
+      batch = g_rec ? std::max(2, launched / 2) : 2;
     }
     if (launched == 0) {  // every wavenumber passed on the first toplevel
       st_read(h, h->h_flags, mg.done, 256 * sizeof(int));
