@@ -373,7 +373,9 @@ int lw_server_destroy(lw_server *s);
  * the same bits as stepped alone. The handles must live on the current
  * device, use the z-DFT (default) or poiss_ftft Poisson solver, have
  * IAVG = 0 and no profile running; a handle belongs to one batch at a time
- * and cannot be destroyed while it does. lw_batch_step runs nsteps steps of
+ * and cannot be destroyed while it does. A batch is driven from one thread
+ * at a time. After a device error every later lw_batch_step of the batch
+ * returns LW_ERR_DEVICE with the same message. lw_batch_step runs nsteps steps of
  * every case; stats (NULL or n entries) get each case's last step (phase
  * times are not measured: 0). lw_batch_info: info[0] rendezvous (host
  * synchronisation points), [1] batched launches, [2] per-case launches they

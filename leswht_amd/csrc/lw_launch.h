@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <string>
 #include <tuple>
 #include <type_traits>
 #include <utility>
@@ -272,6 +273,8 @@ struct Batch;
 Batch *batch_new(int dev, int n, int (*run)(void *, int, int64_t), void *ctx);
 int batch_run(Batch *b, int64_t nsteps, int *rc);
 void batch_counts(const Batch *b, int64_t out[6]);
+// the message of the device error that ended the batch (empty if none)
+const std::string &batch_error(const Batch *b);
 
 void batch_free(Batch *b);
 

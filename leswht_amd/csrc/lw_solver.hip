@@ -2026,6 +2026,7 @@ static void phase_poisson(lw_handle *h) {
   if (h->st.poisson == 2) {
     // poiss_ftft is direct: no cycles, no residual rows
     launch_poisson_ftft(h->s, h->g, h->f, h->ftft);
+    st_gate();  // where the z-DFT path's cases wait after their solve
     return;
   }
   const Geo &g = h->g;
@@ -2874,7 +2875,7 @@ extern "C" int lw_batch_step(lw_batch *b, int64_t nsteps, lw_step_stats *stats) 
   if (r != LW_OK) {
     for (size_t i = 0; i < rc.size(); ++i)
       if (rc[i] != LW_OK) return fail(rc[i], "lw_batch_step: case " + std::to_string(i) + ": " + b->errs[i]);
-    return fail(r, "lw_batch_step: the batch failed");
+    return fail(r, "lw_batch_step: the batch failed earlier: " + batch_error(b->core));
   }
   return LW_OK;
 }
