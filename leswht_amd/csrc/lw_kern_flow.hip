@@ -45,11 +45,14 @@ __device__ inline void atomic_max_nonneg(unsigned long long *a, double v) {
 // momentum right-hand side, slv_mmtm.f90:315-567, with the LES cross terms of
 // rhssgs (sgs.f90:1136-1200) computed in place of the rhs1 round trip
 // ---------------------------------------------------------------------------
-LW_KERNEL void k_rhs_u(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
-  const int i = g.ib + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.n1m || j > g.n2m) return;
+// the right-hand side of a component at a cell and its new explicit term
+// (the next substep's rk3o); the old rk3o stays where it is, and the launch
+// functions swap rk3o and rk3oo after writing the new term into rk3oo
+struct RhsOut {
+  double rhs, rk3;
+};
+
+__device__ inline RhsOut rhs_u_at(const Geo &g, const Sub &sb, const Fld &f, int les, int i, int j, int k) {
   const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
   const long c = P(g, i, j, k);
   const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
@@ -97,18 +100,13 @@ LW_KERNEL void k_rhs_u(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f
   const double ro = f.rk3o[0][c];
   // the channel's mean pressure gradient pmi(0) (meanpg) when ICH != 0
   const double pmi0 = g.ich != 0 ? f.chan[0] : 0.0;
-  f.rhs[0][c] = sb.dt * (sb.gam * rk3x + sb.ro * ro + 2. * sb.alpha * alx -
-                         2. * sb.alpha * (p[c] - p[c - dx]) * g.c2cxi[i] -
-                         2. * sb.alpha * pmi0 * (double)(g.ich != 0 ? 1 : 0));
-  f.rk3oo[0][c] = ro;
-  f.rk3o[0][c] = rk3x;
+  return {sb.dt * (sb.gam * rk3x + sb.ro * ro + 2. * sb.alpha * alx -
+                   2. * sb.alpha * (p[c] - p[c - dx]) * g.c2cxi[i] -
+                   2. * sb.alpha * pmi0 * (double)(g.ich != 0 ? 1 : 0)),
+          rk3x};
 }
 
-LW_KERNEL void k_rhs_v(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = g.jb + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.n1m || j > g.n2m) return;
+__device__ inline RhsOut rhs_v_at(const Geo &g, const Sub &sb, const Fld &f, int les, int i, int j, int k) {
   const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
   const long c = P(g, i, j, k);
   const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
@@ -154,17 +152,11 @@ LW_KERNEL void k_rhs_v(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f
     rk3y = rk3y + 1.0 * (rhsv1 + rhsv3);
   }
   const double ro = f.rk3o[1][c];
-  f.rhs[1][c] = sb.dt * (sb.gam * rk3y + sb.ro * ro + 2. * sb.alpha * aly -
-                         2. * sb.alpha * (p[c] - p[c - dy]) * g.c2cyi[j]);
-  f.rk3oo[1][c] = ro;
-  f.rk3o[1][c] = rk3y;
+  return {sb.dt * (sb.gam * rk3y + sb.ro * ro + 2. * sb.alpha * aly - 2. * sb.alpha * (p[c] - p[c - dy]) * g.c2cyi[j]),
+          rk3y};
 }
 
-LW_KERNEL void k_rhs_w(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = g.kb + blockIdx.z;
-  if (i > g.n1m || j > g.n2m) return;
+__device__ inline RhsOut rhs_w_at(const Geo &g, const Sub &sb, const Fld &f, int les, int i, int j, int k) {
   const double *u = f.u, *v = f.v, *w = f.w, *p = f.p;
   const long c = P(g, i, j, k);
   const long dx = 1, dy = g.NX, dz = g.NX * g.NY;
@@ -210,19 +202,96 @@ LW_KERNEL void k_rhs_w(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f
     rk3z = rk3z + 1.0 * (rhsw1 + rhsw2);
   }
   const double ro = f.rk3o[2][c];
-  f.rhs[2][c] = sb.dt * (sb.gam * rk3z + sb.ro * ro + 2. * sb.alpha * alz -
-                         2. * sb.alpha * (p[c] - p[c - dz]) * g.c2czi[k]);
-  f.rk3oo[2][c] = ro;
-  f.rk3o[2][c] = rk3z;
+  return {sb.dt * (sb.gam * rk3z + sb.ro * ro + 2. * sb.alpha * alz - 2. * sb.alpha * (p[c] - p[c - dz]) * g.c2czi[k]),
+          rk3z};
 }
 
-void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
-  // reads u v w p rk3o (+ three viscosities with LES), writes rhs rk3o rk3oo
-  const double b = pts(g) * 8.0 * (les ? 11 : 8);
-  LW_LAUNCH("k_rhs_u", b, 0, k_rhs_u, grid3(g.n1m - g.ib + 1, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
-  LW_LAUNCH("k_rhs_v", b, 0, k_rhs_v, grid3(g.n1m, g.n2m - g.jb + 1, g.n3m), BLK3, 0, s, g, sb, f, les);
-  if (g.n3m != 1)
-    LW_LAUNCH("k_rhs_w", b, 0, k_rhs_w, grid3(g.n1m, g.n2m, g.n3m - g.kb + 1), BLK3, 0, s, g, sb, f, les);
+// one component per launch (LW_RHS_FUSED=0)
+LW_KERNEL void k_rhs_u(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
+  const int i = g.ib + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const RhsOut o = rhs_u_at(g, sb, f, les, i, j, k);
+  const long c = P(g, i, j, k);
+  f.rhs[0][c] = o.rhs;
+  f.rk3oo[0][c] = o.rk3;
+}
+
+LW_KERNEL void k_rhs_v(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = g.jb + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const RhsOut o = rhs_v_at(g, sb, f, les, i, j, k);
+  const long c = P(g, i, j, k);
+  f.rhs[1][c] = o.rhs;
+  f.rk3oo[1][c] = o.rk3;
+}
+
+LW_KERNEL void k_rhs_w(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = g.kb + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const RhsOut o = rhs_w_at(g, sb, f, les, i, j, k);
+  const long c = P(g, i, j, k);
+  f.rhs[2][c] = o.rhs;
+  f.rk3oo[2][c] = o.rk3;
+}
+
+// the three components in one pass over the cells: every load of u, v, w,
+// p and the viscosities serves all three. Each component is evaluated at
+// every interior cell, also at i = 1 < ib, j = 1 < jb and k = 1 < kb, where
+// its stencil stays inside the arrays (ghost layer 0 and metric index 0, as
+// on a periodic grid), and stored only where its own kernel would store it.
+// 150 VGPRs, three waves per SIMD: at 384x256x64 LES 0.36 ms per launch
+// against 0.46 for the three launches (limiting it to 128 registers spills
+// and takes 0.54 ms).
+LW_KERNEL void k_rhs_uvw(LW_KP(256, 0), const Geo &g, const Sub &sb, const Fld &f, int les) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.n1m || j > g.n2m) return;
+  const RhsOut ou = rhs_u_at(g, sb, f, les, i, j, k);
+  const RhsOut ov = rhs_v_at(g, sb, f, les, i, j, k);
+  const RhsOut ow = rhs_w_at(g, sb, f, les, i, j, k);
+  const long c = P(g, i, j, k);
+  if (i >= g.ib) {
+    f.rhs[0][c] = ou.rhs;
+    f.rk3oo[0][c] = ou.rk3;
+  }
+  if (j >= g.jb) {
+    f.rhs[1][c] = ov.rhs;
+    f.rk3oo[1][c] = ov.rk3;
+  }
+  if (k >= g.kb) {
+    f.rhs[2][c] = ow.rhs;
+    f.rk3oo[2][c] = ow.rk3;
+  }
+}
+
+static bool rhs_fused_enabled() {
+  const char *e = getenv("LW_RHS_FUSED");
+  return !(e && atoi(e) == 0);
+}
+
+void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, Fld &f, int les) {
+  if (g.n3m != 1 && rhs_fused_enabled()) {
+    // reads u v w p, three rk3o (+ four viscosities with LES), writes three
+    // rhs and three new terms
+    LW_LAUNCH("k_rhs_uvw", pts(g) * 8.0 * (les ? 17 : 13), 0, k_rhs_uvw, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g,
+              sb, f, les);
+  } else {
+    // reads u v w p rk3o (+ three viscosities with LES), writes rhs and the
+    // new term
+    const double b = pts(g) * 8.0 * (les ? 10 : 7);
+    LW_LAUNCH("k_rhs_u", b, 0, k_rhs_u, grid3(g.n1m - g.ib + 1, g.n2m, g.n3m), BLK3, 0, s, g, sb, f, les);
+    LW_LAUNCH("k_rhs_v", b, 0, k_rhs_v, grid3(g.n1m, g.n2m - g.jb + 1, g.n3m), BLK3, 0, s, g, sb, f, les);
+    if (g.n3m != 1)
+      LW_LAUNCH("k_rhs_w", b, 0, k_rhs_w, grid3(g.n1m, g.n2m, g.n3m - g.kb + 1), BLK3, 0, s, g, sb, f, les);
+  }
+  for (int c = 0; c < 3; ++c) std::swap(f.rk3o[c], f.rk3oo[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1592,14 +1661,16 @@ LW_KERNEL void k_rhs_t(LW_KP(1024, 0), const Geo &g, const Sub &sb, const Fld &f
   }
   const double ro = f.rk3o[3][c];
   f.rhs[3][c] = sb.dt * (sb.gam * rk3t + sb.ro * ro + 2. * sb.alpha * alt);
-  f.rk3oo[3][c] = ro;
-  f.rk3o[3][c] = rk3t;
+  // the new term into the spare buffer; the host swaps rk3o and rk3oo
+  f.rk3oo[3][c] = rk3t;
 }
 
-void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les) {
-  // reads t u v w rk3o (+ four diffusivities with LES), writes rhs rk3o rk3oo
-  LW_LAUNCH("k_rhs_t", pts(g) * 8.0 * (les ? 12 : 8), 0, k_rhs_t, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f,
+void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, Fld &f, int les) {
+  // reads t u v w rk3o (+ three diffusivities with LES), writes rhs and the
+  // new term
+  LW_LAUNCH("k_rhs_t", pts(g) * 8.0 * (les ? 10 : 7), 0, k_rhs_t, grid3(g.n1m, g.n2m, g.n3m), BLK3, 0, s, g, sb, f,
             les);
+  std::swap(f.rk3o[3], f.rk3oo[3]);
 }
 
 

@@ -74,7 +74,7 @@ constexpr int kRedBlocks = 512;
 constexpr int kRedThreads = 256;
 
 // ---- momentum / continuity / projection / energy (lw_kern_flow.hip) ----
-void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, Fld &f, int les);
 void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp);
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b, int les);
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
@@ -82,7 +82,7 @@ hipError_t prepare_adi_kernels(const Geo &g, int *use_wave);
 void launch_retrv_mom(hipStream_t s, const Geo &g, const Fld &f);
 void launch_divgs(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int masson);
 void launch_projection(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f);
-void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
+void launch_rhs_t(hipStream_t s, const Geo &g, const Sub &sb, Fld &f, int les);
 void launch_convbc_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b,
                      int les, const lw_settings &st);
 void launch_adi_t(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
