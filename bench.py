@@ -42,15 +42,17 @@ def traffic_from_profile(kernel: str, config: str):
     """HBM bytes per launch of ``kernel`` measured by the FETCH_SIZE and
     WRITE_SIZE counters (rocprofv3 --pmc, scaled by the same counters on a
     device copy of known size), from the committed profile summary of this
-    workload, whose line ``traffic <kernel>: <bytes> bytes per launch`` the
-    summary's own commands produce. None when no summary holds it."""
-    path = os.path.join(PROFILES, "%s_%s.md" % (kernel, WORKLOAD_TAGS[config]))
-    if not os.path.exists(path):
-        return None, None
-    for ln in open(path):
-        t = ln.strip().split()
-        if len(t) >= 3 and t[0] == "traffic" and t[1] == kernel + ":":
-            return int(t[2]), os.path.relpath(path, HERE)
+    workload (a file ``*_<workload>.md``), whose line ``traffic <kernel>:
+    <bytes> bytes per launch`` the summary's own commands produce. None when
+    no summary holds it."""
+    tag = "_%s.md" % WORKLOAD_TAGS[config]
+    names = sorted(n for n in os.listdir(PROFILES) if n.endswith(tag)) if os.path.isdir(PROFILES) else []
+    for name in names:
+        path = os.path.join(PROFILES, name)
+        for ln in open(path):
+            t = ln.strip().split()
+            if len(t) >= 3 and t[0] == "traffic" and t[1] == kernel + ":":
+                return int(t[2]), os.path.relpath(path, HERE)
     return None, None
 
 
