@@ -306,9 +306,7 @@ __device__ inline void reindex_dev(int n, int nm, int &ip, int &ipp) {
   else if (ipp == -1) { ip = nm; ipp = nm - 1; }
 }
 
-LW_KERNEL void k_ibm_gather(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, const double *rhs) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ib.nintp) return;
+__device__ inline void ibm_gather_at(const Geo &g, const IBComp &ib, const double *fld, const double *rhs, int n) {
   const int ii = ib.ifc[n], jj = ib.jfc[n], kk = ib.kfc[n];
   const int *ix = ib.indx + 3 * n;
   int ip = ii + ix[0], jp = jj + ix[1], kp = kk + ix[2];
@@ -331,9 +329,8 @@ LW_KERNEL void k_ibm_gather(LW_KP(1024, 0), const Geo &g, const IBComp &ib, cons
   ib.targ[n] = targ;
 }
 
-LW_KERNEL void k_ibm_scatter(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, double *rhs, double dti, int has_dudtr) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ib.nbody) return;
+__device__ inline void ibm_scatter_at(const Geo &g, const IBComp &ib, const double *fld, double *rhs, double dti,
+                                      int has_dudtr, int n) {
   const long q = P(g, ib.ifc[n], ib.jfc[n], ib.kfc[n]);
   const double targ = n < ib.nintp ? ib.targ[n] : 0.0;
   const double fv = fld[q];
@@ -342,6 +339,53 @@ LW_KERNEL void k_ibm_scatter(LW_KP(1024, 0), const Geo &g, const IBComp &ib, con
   if (has_dudtr) ib.dudtr[n] = (targ - fv) * dti;
   rhs[q] = targ - fv;
   ib.fcvavg[n] = ib.fcvavg[n] + fc;
+}
+
+LW_KERNEL void k_ibm_gather(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, const double *rhs) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < ib.nintp) ibm_gather_at(g, ib, fld, rhs, n);
+}
+
+LW_KERNEL void k_ibm_scatter(LW_KP(1024, 0), const Geo &g, const IBComp &ib, const double *fld, double *rhs, double dti,
+                             int has_dudtr) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < ib.nbody) ibm_scatter_at(g, ib, fld, rhs, dti, has_dudtr, n);
+}
+
+// the three momentum components in one launch per phase (blockIdx.y is the
+// component): a component's gather reads only its own field and rhs, and its
+// scatter writes only its own rhs, so gathering all three before scattering
+// any gives the per-component order's results
+LW_KERNEL void k_ibm_gather3(LW_KP(256, 0), const Geo &g, const IBComp &i0, const IBComp &i1, const IBComp &i2,
+                             const Fld &f) {
+  const int c = blockIdx.y, n = blockIdx.x * blockDim.x + threadIdx.x;
+  const IBComp &ib = c == 0 ? i0 : (c == 1 ? i1 : i2);
+  if (n < ib.nintp) ibm_gather_at(g, ib, c == 0 ? f.u : (c == 1 ? f.v : f.w), f.rhs[c], n);
+}
+
+LW_KERNEL void k_ibm_scatter3(LW_KP(256, 0), const Geo &g, const IBComp &i0, const IBComp &i1, const IBComp &i2,
+                              const Fld &f, double dti) {
+  const int c = blockIdx.y, n = blockIdx.x * blockDim.x + threadIdx.x;
+  const IBComp &ib = c == 0 ? i0 : (c == 1 ? i1 : i2);
+  if (n < ib.nbody) ibm_scatter_at(g, ib, c == 0 ? f.u : (c == 1 ? f.v : f.w), f.rhs[c], dti, 1, n);
+}
+
+void launch_ibm_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib) {
+  const double dti = 1. / sb.dt;
+  int ni = 0, nb = 0;
+  double bi = 0, bb = 0;
+  for (int c = 0; c < 3; ++c) {
+    ni = std::max(ni, ib[c].nintp);
+    nb = std::max(nb, ib[c].nbody);
+    bi += ib[c].nintp * (27.0 * 16 + 3 * 4 + 8 + 8);
+    bb += ib[c].nbody * (3 * 4 + 8 + 8 + 8 + 8);
+  }
+  if (ni > 0)
+    LW_LAUNCH("k_ibm_gather", bi, 0, k_ibm_gather3, dim3((ni + 255) / 256, 3), dim3(256), 0, s, g, ib[0], ib[1], ib[2],
+              f);
+  if (nb > 0)
+    LW_LAUNCH("k_ibm_scatter", bb, 0, k_ibm_scatter3, dim3((nb + 255) / 256, 3), dim3(256), 0, s, g, ib[0], ib[1], ib[2],
+              f, dti);
 }
 
 void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp) {

@@ -76,6 +76,7 @@ constexpr int kRedThreads = 256;
 // ---- momentum / continuity / projection / energy (lw_kern_flow.hip) ----
 void launch_rhs_mom(hipStream_t s, const Geo &g, const Sub &sb, Fld &f, int les);
 void launch_ibm(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib, int comp);
+void launch_ibm_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib);
 void launch_convbc_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, const Body &b, int les);
 void launch_adi_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, int les);
 hipError_t prepare_adi_kernels(const Geo &g, int *use_wave);

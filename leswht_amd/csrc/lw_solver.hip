@@ -1724,8 +1724,15 @@ static void phase_rhsnlhs(lw_handle *h) {
   const Geo &g = h->g;
   Body b{};
   launch_rhs_mom(h->s, g, h->sub, h->f, h->les);
-  if (h->st.ibmon != 0)
-    for (int c = 0; c < 3; ++c) launch_ibm(h->s, g, h->sub, h->f, h->ib, c);
+  if (h->st.ibmon != 0) {
+    // one launch per phase for the three components (LW_IBM_FUSED=0: one
+    // per component)
+    const char *e = getenv("LW_IBM_FUSED");
+    if (e && atoi(e) == 0)
+      for (int c = 0; c < 3; ++c) launch_ibm(h->s, g, h->sub, h->f, h->ib, c);
+    else
+      launch_ibm_mom(h->s, g, h->sub, h->f, h->ib);
+  }
   launch_convbc_mom(h->s, g, h->sub, h->f, b, h->les);
   launch_adi_mom(h->s, g, h->sub, h->f, h->les);
   launch_retrv_mom(h->s, g, h->f);
