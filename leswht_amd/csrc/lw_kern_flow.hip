@@ -358,16 +358,24 @@ LW_KERNEL void k_ibm_scatter(LW_KP(1024, 0), const Geo &g, const IBComp &ib, con
 // any gives the per-component order's results
 LW_KERNEL void k_ibm_gather3(LW_KP(256, 0), const Geo &g, const IBComp &i0, const IBComp &i1, const IBComp &i2,
                              const Fld &f) {
-  const int c = blockIdx.y, n = blockIdx.x * blockDim.x + threadIdx.x;
-  const IBComp &ib = c == 0 ? i0 : (c == 1 ? i1 : i2);
-  if (n < ib.nintp) ibm_gather_at(g, ib, c == 0 ? f.u : (c == 1 ? f.v : f.w), f.rhs[c], n);
+  // a switch with constant component indices: selecting among the
+  // arguments at run time would copy them to scratch memory
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  switch (blockIdx.y) {
+    case 0: if (n < i0.nintp) ibm_gather_at(g, i0, f.u, f.rhs[0], n); break;
+    case 1: if (n < i1.nintp) ibm_gather_at(g, i1, f.v, f.rhs[1], n); break;
+    default: if (n < i2.nintp) ibm_gather_at(g, i2, f.w, f.rhs[2], n); break;
+  }
 }
 
 LW_KERNEL void k_ibm_scatter3(LW_KP(256, 0), const Geo &g, const IBComp &i0, const IBComp &i1, const IBComp &i2,
                               const Fld &f, double dti) {
-  const int c = blockIdx.y, n = blockIdx.x * blockDim.x + threadIdx.x;
-  const IBComp &ib = c == 0 ? i0 : (c == 1 ? i1 : i2);
-  if (n < ib.nbody) ibm_scatter_at(g, ib, c == 0 ? f.u : (c == 1 ? f.v : f.w), f.rhs[c], dti, 1, n);
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  switch (blockIdx.y) {
+    case 0: if (n < i0.nbody) ibm_scatter_at(g, i0, f.u, f.rhs[0], dti, 1, n); break;
+    case 1: if (n < i1.nbody) ibm_scatter_at(g, i1, f.v, f.rhs[1], dti, 1, n); break;
+    default: if (n < i2.nbody) ibm_scatter_at(g, i2, f.w, f.rhs[2], dti, 1, n); break;
+  }
 }
 
 void launch_ibm_mom(hipStream_t s, const Geo &g, const Sub &sb, const Fld &f, IBComp *ib) {
@@ -1891,6 +1899,8 @@ void launch_diag(hipStream_t s, const Geo &g, const Fld &f, int masson) {
 LW_KERNEL void k_lagforce(LW_KP(256, 0), const Geo &g, const Sub &sb, const Fld &f, const IBComp &u, const IBComp &v, const IBComp &w) {
   __shared__ double sh[4];
   const IBComp *cs[3] = {&u, &v, &w};
+  // unrolled: constant component indices keep the arguments out of scratch
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const IBComp &ib = *cs[c];
     double acc = 0.0;
