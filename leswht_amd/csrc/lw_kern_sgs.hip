@@ -472,13 +472,21 @@ __device__ inline void edm_germano(const Geo &g, int i, int j, int k, const FT (
 // sums go to f.partial[2 b], f.partial[2 b + 1].
 // FT: the test filter's arithmetic, double (the reference's) or float (the
 // mixed-precision form: the filtered quantities feed only the Germano sums,
-// which stay FP64, and the model coefficient they give). The FP32 Vreman
-// instance asks for two waves per SIMD (its float arrays leave room; 29
-// spilled registers, 0.82 against 1.25 ms at 384x256x64); the eddy-
-// diffusivity one keeps one (two spilled 122 registers and ran 1.7 ms).
+// which stay FP64, and the model coefficient they give). The Vreman
+// instances ask for two waves per SIMD (FP64: 39 spilled registers, FP32:
+// 4); the eddy-diffusivity ones keep one (two spilled 151 registers in FP64,
+// and in FP32 an earlier form spilled 122 and ran 1.7 ms).
 template <int MODEL, int NQ, typename FT>
-LW_KERNEL void k_sgs_dyn(LW_KP(256, (sizeof(FT) == 4 && MODEL == 0 ? 2 : 0)), const Geo &g, const Fld &f, const int64_t *nwall, int fx, int fz) {
+LW_KERNEL void k_sgs_dyn(LW_KP(256, MODEL == 0 ? 2 : 0), const Geo &g, const Fld &f, const int64_t *nwall, int fx, int fz) {
   __shared__ double sh[4];
+  // the first term of the plane above for each quantity: for the Vreman
+  // model in LDS, one slot per lane (each lane reads back only what it
+  // wrote), which leaves the registers for two waves per SIMD (FP64 1.12
+  // against 1.19 ms, FP32 0.65 against 0.82 at 384x256x64); the eddy-
+  // diffusivity model keeps it in registers (in LDS it ran 1.06 against
+  // 1.03 ms at one wave per SIMD either way)
+  constexpr bool kLdsB = MODEL == 0;
+  __shared__ FT Bs[4][kLdsB ? NQ : 1][64];
   double lm = 0.0, mm = 0.0;
   const int w = threadIdx.x >> 6;
   SgsTile T;
@@ -486,10 +494,12 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, (sizeof(FT) == 4 && MODEL == 0 ? 2 : 0)), co
     const int n3m = g.n3m;
     // z sums: A of the plane awaiting its last term, B of the plane awaiting
     // its middle one
-    FT A[NQ], B[NQ];
+    FT A[NQ], B[kLdsB ? 1 : NQ];
     FT x[NQ];
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) A[l] = B[l] = (FT)0.0;
+    for (int l = 0; l < NQ; ++l) A[l] = (FT)0.0;
+    // B is written at kk = k0 - 1 before its first read at kk = k0 (fz), and
+    // not used without the z filter
     const int kb = fz ? T.k0 - 1 : T.k0, ke = fz ? T.k1 + 1 : T.k1;
     for (int kk = kb; kk <= ke; ++kk) {
       const int k = kk < 1 ? n3m : (kk > n3m ? 1 : kk);
@@ -520,14 +530,20 @@ LW_KERNEL void k_sgs_dyn(LW_KP(256, (sizeof(FT) == 4 && MODEL == 0 ? 2 : 0)), co
       if (kk >= T.k0 && kk <= T.k1) {
         const FT cz = (FT)g.cfz1[3 * k + 1];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) A[l] = B[l] + cz * x[l];
+        for (int l = 0; l < NQ; ++l) {
+          if constexpr (kLdsB) A[l] = Bs[w][l][T.lane] + cz * x[l];
+          else A[l] = B[l] + cz * x[l];
+        }
       }
       // the first term of the plane above
       if (kk + 1 <= T.k1) {
         const int kp = kk + 1;
         const FT cz = (FT)g.cfz1[3 * kp];
 #pragma unroll
-        for (int l = 0; l < NQ; ++l) B[l] = cz * x[l];
+        for (int l = 0; l < NQ; ++l) {
+          if constexpr (kLdsB) Bs[w][l][T.lane] = cz * x[l];
+          else B[l] = cz * x[l];
+        }
       }
     }
   }
